@@ -1,0 +1,84 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * A plain-C, CPU, FP64 restatement of the ChIMES serial evaluation path of
+ * rk-lindsey/chimes_calculator (chimesFF + serial_chimes_interface).  It is the
+ * checker for the HIP product and the CPU baseline timed by bench.py.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it;
+ * the product library never links or calls it.
+ *
+ * Every function cites the reference file:line it follows.  Parity pins for
+ * each function are listed at the top of chimes_oracle.c.
+ */
+#ifndef CHIMES_ORACLE_H
+#define CHIMES_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oc_params oc_params;
+
+/* Cluster lists in canonical key form.  A member of a cluster is the pair
+ * (parent atom index in the (replicated) system, image shift sx, sy, sz), the
+ * shift being the ghost-layer offset in cell vectors of the neighbour frame.
+ *   pairs   : 5 ints per entry  (i, a, sx, sy, sz)
+ *   triplets: 9 ints per entry  (i, a_j, s_j[3], a_k, s_k[3])
+ *   quads   : 13 ints per entry (i, a_j, s_j[3], a_k, s_k[3], a_l, s_l[3])
+ * Entries are in the order the reference's loops generate them. */
+typedef struct {
+    int n_atoms;     /* real atoms after replication */
+    int n_layers;    /* ghost layers of the neighbour frame */
+    int n_rep;       /* replication count (0 = none) */
+    long n2, n3, n4;
+    int *pairs, *trips, *quads;
+} oc_lists;
+
+/* Parse a ChIMES parameter file (chimesFF.cpp:354-1570) and build the
+ * triplet/quadruplet pair maps (chimesFF.cpp:2958-3080).  NULL on error. */
+oc_params *oc_load(const char *path);
+void oc_free(oc_params *P);
+
+int    oc_order(const oc_params *P, int body);          /* body = 2, 3, 4 */
+double oc_max_cutoff(const oc_params *P, int body);     /* chimesFF.cpp:2903-2943 */
+int    oc_ntypes(const oc_params *P);
+const char *oc_type_name(const oc_params *P, int t);
+
+/* Full system evaluation (serial_chimes_interface.cpp:960-1124).
+ * energy, fx/fy/fz are accumulated onto (like the reference); stress[9] is
+ * overwritten.  If owner_stride > 1 only clusters whose first (owner) atom i
+ * satisfies i % owner_stride == 0 are evaluated (a work sample for timing).
+ * Returns 0 on success, nonzero on an error the reference would exit() on. */
+int oc_calculate(const oc_params *P, int natom, const double *x, const double *y,
+                 const double *z, const char *const *types, const double ca[3],
+                 const double cb[3], const double cc[3], int small,
+                 double *energy, double *fx, double *fy, double *fz,
+                 double stress[9], oc_lists *lists, int owner_stride);
+
+void oc_lists_free(oc_lists *L);
+
+/* Single-cluster evaluators (chimesFF.cpp:1657-1775, 1944-2176, 2509-2872).
+ * Type arguments are type indices.  force / stress are packed as in the
+ * reference (force[natoms*3], stress[6]) and are accumulated onto. */
+void oc_compute_2b(const oc_params *P, double dx, const double dr[3], const int t[2],
+                   double force[6], double stress[6], double *energy);
+void oc_compute_3b(const oc_params *P, const double dx[3], const double dr[9],
+                   const int t[3], double force[9], double stress[6], double *energy);
+void oc_compute_4b(const oc_params *P, const double dx[6], const double dr[18],
+                   const int t[4], double force[12], double stress[6], double *energy);
+
+/* Chebyshev polynomials of the Morse-transformed distance and the cutoff
+ * function, exposed for unit tests (chimesFF.h:347-386, 542-628,
+ * chimesFF.cpp:1572-1613). */
+void oc_cheby(double dx, double lambda, double rin, double rout, int order,
+              double *Tn, double *Tnd);
+void oc_fcut(const oc_params *P, double dx, double rout, double *fcut, double *fcutd);
+
+/* Penalty / out-of-range event counters of the last oc_calculate call. */
+long oc_last_penalty_count(void);
+long oc_last_out_of_range_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

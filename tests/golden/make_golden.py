@@ -1,0 +1,169 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Runs only where the reference checkout and oracle/_ref/libchimes_ref.so exist
+(make -C oracle).  Outputs:
+  ref_inputs/*.gz       input data the reference's own tests hold (parameter
+                        files, .xyz configurations, expected_output .dat)
+  ref_cases.json        per reference test case: E, F, stress at full
+                        precision, sizes and SHA-256 of the sorted cluster
+                        lists, as computed by the reference itself
+  synth_cases.json      the same for the seeded synthetic workloads (C0 with
+                        full cluster lists, C1, and the first configurations
+                        of the C2 batch)
+  cluster_vectors.json  single-cluster compute_2B/3B/4B input/output vectors
+"""
+import gzip
+import hashlib
+import json
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, ROOT)
+
+from oracle_bindings import Reference, read_xyz, ref_available, sort_keys  # noqa: E402
+from chimes_calculator_amd import workloads as W  # noqa: E402
+
+REFT = "/root/reference/serial_interface/tests"
+
+CASES = [
+    ("published_params.liqC.2b.cubic.txt", "liqC.2.5gcc_6000K.OUTCAR_#000.xyz", 0),
+    ("published_params.liqC.2+3b.cubic.txt", "liqC.2.5gcc_6000K.OUTCAR_#000.xyz", 0),
+    ("published_params.liqC.2+3b.cubic.txt", "liqC.2.5gcc_6000K.OUTCAR_#001.xyz", 0),
+    ("published_params.liqCO.2+3b.cubic.txt", "CO.2.5gcc_6500K.OUTCAR_#000.xyz", 1),
+    ("published_params.liqCO.2+3b.cubic.txt", "CO.2.5gcc_6500K.OUTCAR_#000.xyz", 0),
+    ("validated_params.liqCO.2+3b.cubic.txt", "CO.2.5gcc_6500K.OUTCAR_#000.relabel.xyz", 1),
+    ("published_params.liqCO.2+3b.cubic.txt", "CO.2.5gcc_6500K.OUTCAR_#000.scramble.xyz", 1),
+    ("published_params.liqCO.2+3b.cubic.txt", "CO.2.5gcc_6500K.OUTCAR_#000.translate.xyz", 1),
+    ("published_params.HN3.2+3+4b.Tersoff.special.offsets.txt", "HN3.2gcc_3000K.OUTCAR_#000.xyz", 0),
+    ("validated_params.TiO2.2+3b.Tersoff.txt", "TiO2.unitcell_arbrot_#000.xyz", 0),
+    ("validated_params.TiO2.2+3b.Tersoff.txt", "TiO2.unitcell_arbrot_#001.xyz", 0),
+    ("test_params.CHON.txt", "CHON.testfile_#000.xyz", 0),
+    ("published_params.liqCO.2+3b.cubic.txt", "diam.2_#000.xyz", 1),
+    ("published_params.liqCO.2+3b.cubic.txt", "diam.8_#000.xyz", 1),
+    ("published_params.liqCO.2+3b.cubic.txt", "diam.16_#000.xyz", 1),
+    ("published_params.liqCO.2+3b.cubic.txt", "diam.64_#000.xyz", 1),
+    ("published_params.CO2400K.2+3+4b.Tersoff.special.offsets.txt", "CO.9GPa_2400K.OUTCAR_#000.xyz", 1),
+    ("test_params.h2o_2bcheby.txt", "H2O.1.50gcc_2000K_#000.xyz", 0),
+    ("test_params.HN3.penalty.txt", "HN3.2.04gcc_20000K_#000.xyz", 0),
+]
+
+
+def sha(keys):
+    return hashlib.sha256(np.ascontiguousarray(sort_keys(keys), dtype="<i4").tobytes()).hexdigest()
+
+
+def record(res, full_lists=False):
+    d = {
+        "energy": repr(float(res.energy)),
+        "forces": [[repr(float(v)) for v in row] for row in res.forces],
+        "stress": [repr(float(v)) for v in res.stress],
+        "n_pairs": int(len(res.pairs)), "n_trips": int(len(res.trips)), "n_quads": int(len(res.quads)),
+        "sha_pairs": sha(res.pairs), "sha_trips": sha(res.trips), "sha_quads": sha(res.quads),
+        "meta": [int(v) for v in res.meta],
+    }
+    if full_lists:
+        d["pairs"] = sort_keys(res.pairs).tolist()
+        d["trips"] = sort_keys(res.trips).tolist()
+        d["quads"] = sort_keys(res.quads).tolist()
+    return d
+
+
+def gz_copy(src, dst):
+    with open(src, "rb") as f, gzip.open(dst, "wb", compresslevel=9) as g:
+        shutil.copyfileobj(f, g)
+
+
+def main():
+    if not ref_available():
+        sys.exit("oracle/_ref/libchimes_ref.so is not built; run make -C oracle here")
+    which = sys.argv[1:] or ["ref", "synth", "vectors"]
+    inp = os.path.join(HERE, "ref_inputs")
+    os.makedirs(inp, exist_ok=True)
+    if "ref" in which:
+        out = {}
+        for pf, xyz, small in CASES:
+            for src in (os.path.join(REFT, "force_fields", pf), os.path.join(REFT, "configurations", xyz),
+                        os.path.join(REFT, "expected_output", pf + "." + xyz + ".dat")):
+                dst = os.path.join(inp, os.path.basename(src) + ".gz")
+                if os.path.exists(src) and not os.path.exists(dst):
+                    gz_copy(src, dst)
+            pos, types, cell = read_xyz(os.path.join(REFT, "configurations", xyz))
+            res = Reference(os.path.join(REFT, "force_fields", pf), small).calculate(pos, types, cell, lists=True)
+            out["%s|%s|%d" % (pf, xyz, small)] = record(res)
+            print("ref case", pf, xyz, small, res.energy, flush=True)
+        with open(os.path.join(HERE, "ref_cases.json"), "w") as f:
+            json.dump(out, f)
+    if "synth" in which:
+        tmp = os.path.join(ROOT, "_work", "golden_models")
+        os.makedirs(tmp, exist_ok=True)
+        out = {}
+        m, cfgs = W.workload("C0")
+        pf = W.MODELS[m].write(os.path.join(tmp, m + ".txt"))
+        for small in (0, 1):
+            res = Reference(pf, small).calculate(*cfgs[0], lists=True)
+            out["C0|%d" % small] = record(res, full_lists=True)
+        m, cfgs = W.workload("C2")
+        pf = W.MODELS[m].write(os.path.join(tmp, m + ".txt"))
+        ref = Reference(pf, 0)
+        for k in range(24):
+            res = ref.calculate(*cfgs[k], lists=True)
+            out["C2|%d" % k] = record(res)
+        print("synth C0/C2 done", flush=True)
+        if "C1" in which or len(sys.argv) == 1:
+            m, cfgs = W.workload("C1")
+            pf = W.MODELS[m].write(os.path.join(tmp, m + ".txt"))
+            res = Reference(pf, 0).calculate(*cfgs[0], lists=True)
+            out["C1|0"] = record(res)
+            print("synth C1 done", res.energy, flush=True)
+        else:
+            old = os.path.join(HERE, "synth_cases.json")
+            if os.path.exists(old):
+                prev = json.load(open(old))
+                if "C1|0" in prev:
+                    out["C1|0"] = prev["C1|0"]
+        with open(os.path.join(HERE, "synth_cases.json"), "w") as f:
+            json.dump(out, f)
+    if "vectors" in which:
+        import ctypes as C
+        rng = np.random.default_rng(1234)
+        out = {}
+        for pf, nbodies in (("published_params.liqCO.2+3b.cubic.txt", (2, 3)),
+                            ("published_params.HN3.2+3+4b.Tersoff.special.offsets.txt", (2, 3, 4)),
+                            ("test_params.HN3.penalty.txt", (2,))):
+            ref = Reference(os.path.join(REFT, "force_fields", pf), 0)
+            nt = 2
+            for nb in nbodies:
+                npair = nb * (nb - 1) // 2
+                rows = []
+                for _ in range(48):
+                    pts = rng.normal(size=(nb, 3))
+                    pts *= rng.uniform(0.5, 3.5) / max(1e-9, np.linalg.norm(pts[1] - pts[0]))
+                    idx = [(a, b) for a in range(nb) for b in range(a + 1, nb)]
+                    dr = np.concatenate([pts[b] - pts[a] for a, b in idx])
+                    dx = np.array([np.linalg.norm(pts[b] - pts[a]) for a, b in idx])
+                    t = rng.integers(0, nt, nb).astype(np.int32)
+                    f = np.zeros(3 * nb)
+                    s = np.zeros(6)
+                    e = C.c_double(0.0)
+                    fn = getattr(ref.L, "ref_compute_%db" % nb)
+                    if nb == 2:
+                        fn(ref.h, float(dx[0]), dr, t, f, s, C.byref(e))
+                    else:
+                        fn(ref.h, dx, dr, t, f, s, C.byref(e))
+                    rows.append({"dx": dx.tolist(), "dr": dr.tolist(), "types": t.tolist(),
+                                 "energy": repr(e.value), "force": [repr(v) for v in f],
+                                 "stress": [repr(v) for v in s]})
+                out["%s|%d" % (pf, nb)] = rows
+        with open(os.path.join(HERE, "cluster_vectors.json"), "w") as f:
+            json.dump(out, f)
+        print("vectors done", flush=True)
+
+
+if __name__ == "__main__":
+    main()

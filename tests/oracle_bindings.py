@@ -1,0 +1,221 @@
+"""ctypes bindings for the oracle (test infrastructure only).
+
+Loads oracle/liboracle.so (the C restatement of the reference path) and, when
+it was built in this checkout, oracle/_ref/libchimes_ref.so (the reference's
+own classes compiled from its sources).  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg import this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libchimes_ref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class _Lists(C.Structure):
+    _fields_ = [("n_atoms", C.c_int), ("n_layers", C.c_int), ("n_rep", C.c_int),
+                ("n2", C.c_long), ("n3", C.c_long), ("n4", C.c_long),
+                ("pairs", C.POINTER(C.c_int)), ("trips", C.POINTER(C.c_int)),
+                ("quads", C.POINTER(C.c_int))]
+
+
+def _cstrs(types):
+    arr = (C.c_char_p * len(types))()
+    arr[:] = [t.encode() for t in types]
+    return arr
+
+
+def _keys(ptr, n, width):
+    if n == 0:
+        return np.zeros((0, width), np.int32)
+    return np.ctypeslib.as_array(ptr, shape=(n * width,)).reshape(n, width).copy()
+
+
+_oracle = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = C.CDLL(ORACLE_SO)
+        L.oc_load.restype = C.c_void_p
+        L.oc_load.argtypes = [C.c_char_p]
+        L.oc_free.argtypes = [C.c_void_p]
+        L.oc_order.argtypes = [C.c_void_p, C.c_int]
+        L.oc_max_cutoff.restype = C.c_double
+        L.oc_max_cutoff.argtypes = [C.c_void_p, C.c_int]
+        L.oc_ntypes.argtypes = [C.c_void_p]
+        L.oc_type_name.restype = C.c_char_p
+        L.oc_type_name.argtypes = [C.c_void_p, C.c_int]
+        L.oc_calculate.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p, _dp, _dp, _dp,
+                                   C.c_int, C.POINTER(C.c_double), _dp, _dp, _dp, _dp,
+                                   C.POINTER(_Lists), C.c_int]
+        L.oc_lists_free.argtypes = [C.POINTER(_Lists)]
+        L.oc_compute_2b.argtypes = [C.c_void_p, C.c_double, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        L.oc_compute_3b.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        L.oc_compute_4b.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        L.oc_cheby.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.oc_fcut.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)]
+        L.oc_last_penalty_count.restype = C.c_long
+        L.oc_last_out_of_range_count.restype = C.c_long
+        _oracle = L
+    return _oracle
+
+
+class Result:
+    def __init__(self, energy, forces, stress, pairs=None, trips=None, quads=None, meta=None):
+        self.energy = energy
+        self.forces = forces
+        self.stress = stress
+        self.pairs, self.trips, self.quads = pairs, trips, quads
+        self.meta = meta
+
+
+class Oracle:
+    """The C restatement of the reference's calculate() path."""
+
+    def __init__(self, param_file):
+        self.L = oracle_lib()
+        self.h = self.L.oc_load(param_file.encode())
+        if not self.h:
+            raise ValueError("oracle could not parse %s" % param_file)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oc_free(self.h)
+            self.h = None
+
+    def order(self, body):
+        return self.L.oc_order(self.h, body)
+
+    def max_cutoff(self, body):
+        return self.L.oc_max_cutoff(self.h, body)
+
+    def type_names(self):
+        return [self.L.oc_type_name(self.h, t).decode() for t in range(self.L.oc_ntypes(self.h))]
+
+    def calculate(self, pos, types, cell, small=False, lists=False, owner_stride=1):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        n = pos.shape[0]
+        x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+        cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+        ca, cb, cc = (np.ascontiguousarray(cell[k]) for k in range(3))
+        fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+        st = np.zeros(9)
+        e = C.c_double(0.0)
+        L = _Lists()
+        ts = _cstrs(types)
+        rc = self.L.oc_calculate(self.h, n, x, y, z, C.cast(ts, C.c_void_p), ca, cb, cc,
+                                 int(bool(small)), C.byref(e), fx, fy, fz, st,
+                                 C.byref(L) if lists else None, int(owner_stride))
+        if rc != 0:
+            raise RuntimeError("oracle calculate failed with code %d" % rc)
+        res = Result(e.value, np.stack([fx, fy, fz], 1), st)
+        if lists:
+            res.pairs = _keys(L.pairs, L.n2, 5)
+            res.trips = _keys(L.trips, L.n3, 9)
+            res.quads = _keys(L.quads, L.n4, 13)
+            res.meta = (L.n_atoms, L.n_layers, L.n_rep)
+            self.L.oc_lists_free(C.byref(L))
+        return res
+
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_SO)
+        L.ref_open.restype = C.c_void_p
+        L.ref_open.argtypes = [C.c_char_p, C.c_int]
+        L.ref_close.argtypes = [C.c_void_p]
+        pp = C.POINTER(C.POINTER(C.c_int))
+        L.ref_calculate.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_void_p, _dp, _dp, _dp,
+                                    C.POINTER(C.c_double), _dp, _dp, _dp, _dp,
+                                    C.POINTER(C.c_long), pp, C.POINTER(C.c_long), pp,
+                                    C.POINTER(C.c_long), pp, _ip]
+        L.ref_free.argtypes = [C.c_void_p]
+        L.ref_compute_2b.argtypes = [C.c_void_p, C.c_double, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        L.ref_compute_3b.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        L.ref_compute_4b.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        _ref = L
+    return _ref
+
+
+class Reference:
+    """The reference's own classes (compiled from its sources) behind a C shim."""
+
+    def __init__(self, param_file, small=False):
+        self.L = ref_lib()
+        self.h = self.L.ref_open(param_file.encode(), int(bool(small)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_close(self.h)
+            self.h = None
+
+    def calculate(self, pos, types, cell, lists=False):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        n = pos.shape[0]
+        x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+        cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+        ca, cb, cc = (np.ascontiguousarray(cell[k]) for k in range(3))
+        fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+        st = np.zeros(9)
+        e = C.c_double(0.0)
+        ts = _cstrs(types)
+        n2, n3, n4 = C.c_long(0), C.c_long(0), C.c_long(0)
+        p2, p3, p4 = C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        meta = np.zeros(3, np.int32)
+        if lists:
+            self.L.ref_calculate(self.h, n, x, y, z, C.cast(ts, C.c_void_p), ca, cb, cc, C.byref(e),
+                                 fx, fy, fz, st, C.byref(n2), C.byref(p2), C.byref(n3),
+                                 C.byref(p3), C.byref(n4), C.byref(p4), meta)
+        else:
+            self.L.ref_calculate(self.h, n, x, y, z, C.cast(ts, C.c_void_p), ca, cb, cc, C.byref(e),
+                                 fx, fy, fz, st, None, None, None, None, None, None, meta)
+        res = Result(e.value, np.stack([fx, fy, fz], 1), st, meta=tuple(meta))
+        if lists:
+            res.pairs = _keys(p2, n2.value, 5)
+            res.trips = _keys(p3, n3.value, 9)
+            res.quads = _keys(p4, n4.value, 13)
+            for p in (p2, p3, p4):
+                self.L.ref_free(C.cast(p, C.c_void_p))
+        return res
+
+
+def read_xyz(path):
+    """The reference driver's input format: natoms, 9 cell numbers, then
+    'type x y z' lines (serial_interface/examples/cpp/main.cpp:84-110)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    n = int(lines[0].split()[0])
+    cell = np.array([float(v) for v in lines[1].split()[:9]]).reshape(3, 3)
+    types, pos = [], []
+    for ln in lines[2:2 + n]:
+        t = ln.split()
+        types.append(t[0])
+        pos.append([float(t[1]), float(t[2]), float(t[3])])
+    return np.array(pos), types, cell
+
+
+def sort_keys(keys):
+    """Canonical (lexicographic) order of a cluster key array."""
+    if len(keys) == 0:
+        return keys
+    order = np.lexsort(keys.T[::-1])
+    return keys[order]
