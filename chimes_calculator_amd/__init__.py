@@ -1,0 +1,224 @@
+"""MI355X-native ChIMES evaluator: Python host interface.
+
+Loads the in-tree HIP library (chimes_calculator_amd/lib/libchimescalc_dl.so)
+and exposes
+  * ChimesSerial -- the reference's Python wrapper interface
+    (serial_interface/api/chimescalc_serial_py.py:24-101): init_chimes(),
+    calculate_chimes() returning (fx, fy, fz, stress, energy);
+  * GpuModel / eval_batch -- batched configurations, FP64 or FP32-mixed;
+  * cluster evaluation helpers for the per-cluster API.
+There is no CPU fallback: without the built library or a GPU every entry
+point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libchimescalc_dl.so")
+
+FP64, MIXED = 0, 1
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+# Every symbol the C headers under include/ declare.
+EXPORTS = {
+    "chimescalc_serial_C.h": ["chimes_open_instance", "chimes_close_instance", "set_chimes_serial",
+                              "set_chimes_serial_instance", "init_chimes_serial",
+                              "init_chimes_serial_instance", "calculate_chimes", "calculate_chimes_instance"],
+    "chimescalc_C.h": ["get_chimes_max_2b_cutoff", "get_chimes_max_3b_cutoff", "get_chimes_max_4b_cutoff",
+                       "get_chimes_2b_order", "get_chimes_3b_order", "get_chimes_4b_order", "set_chimes",
+                       "init_chimes", "chimes_read_params", "chimes_build_pair_int_trip_map",
+                       "chimes_build_pair_int_quad_map", "chimes_compute_2b_props", "chimes_compute_3b_props",
+                       "chimes_compute_4b_props", "chimes_compute_2b_props_fromf90",
+                       "chimes_compute_3b_props_fromf90", "chimes_compute_4b_props_fromf90"],
+    "chimes_gpu.h": ["chimes_gpu_model_load", "chimes_gpu_model_free", "chimes_gpu_last_error",
+                     "chimes_gpu_model_ntypes", "chimes_gpu_model_type_index", "chimes_gpu_model_order",
+                     "chimes_gpu_model_max_cutoff", "chimes_gpu_eval_batch", "chimes_gpu_eval_batch_device",
+                     "chimes_gpu_last_timing", "chimes_gpu_compute_clusters", "chimes_gpu_cluster_lists",
+                     "chimes_gpu_free"],
+}
+
+
+class ChimesError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ChimesError("libchimescalc_dl.so is not built; run __graft_entry__.build() "
+                              "or make -C chimes_calculator_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        L.chimes_open_instance.restype = C.c_void_p
+        L.chimes_close_instance.argtypes = [C.c_void_p]
+        L.set_chimes_serial.argtypes = [C.c_int]
+        L.set_chimes_serial_instance.argtypes = [C.c_void_p, C.c_int]
+        L.init_chimes_serial.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
+        L.init_chimes_serial_instance.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        cargs = [C.c_int, _dp, _dp, _dp, C.c_void_p, _dp, _dp, _dp, C.POINTER(C.c_double), _dp, _dp, _dp, _dp]
+        L.calculate_chimes.argtypes = cargs
+        L.calculate_chimes_instance.argtypes = [C.c_void_p] + cargs
+        L.chimes_gpu_model_load.restype = C.c_void_p
+        L.chimes_gpu_model_load.argtypes = [C.c_char_p, C.c_int]
+        L.chimes_gpu_model_free.argtypes = [C.c_void_p]
+        L.chimes_gpu_last_error.restype = C.c_char_p
+        L.chimes_gpu_model_ntypes.argtypes = [C.c_void_p]
+        L.chimes_gpu_model_type_index.argtypes = [C.c_void_p, C.c_char_p]
+        L.chimes_gpu_model_order.argtypes = [C.c_void_p, C.c_int]
+        L.chimes_gpu_model_max_cutoff.restype = C.c_double
+        L.chimes_gpu_model_max_cutoff.argtypes = [C.c_void_p, C.c_int]
+        L.chimes_gpu_eval_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _dp, _ip, _dp, C.c_int,
+                                            C.c_int, _dp, _dp, _dp]
+        L.chimes_gpu_eval_batch_device.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, C.c_void_p,
+                                                   C.c_void_p, _dp, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                                   C.c_void_p]
+        L.chimes_gpu_last_timing.argtypes = [C.c_void_p, _dp]
+        L.chimes_gpu_compute_clusters.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp]
+        pp = C.POINTER(C.POINTER(C.c_int))
+        L.chimes_gpu_cluster_lists.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), pp,
+                                               C.POINTER(C.c_long), pp, C.POINTER(C.c_long), pp]
+        L.chimes_gpu_free.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _err():
+    return lib().chimes_gpu_last_error().decode()
+
+
+class GpuModel:
+    """A ChIMES parameter set resident on one GPU (chimes_gpu.h)."""
+
+    def __init__(self, param_file, device=0):
+        self.L = lib()
+        self.h = self.L.chimes_gpu_model_load(os.fsencode(param_file), device)
+        if not self.h:
+            raise ChimesError(_err())
+        self.param_file = param_file
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.chimes_gpu_model_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def ntypes(self):
+        return self.L.chimes_gpu_model_ntypes(self.h)
+
+    def type_index(self, name):
+        return self.L.chimes_gpu_model_type_index(self.h, name.encode())
+
+    def types_to_index(self, types):
+        idx = np.array([self.type_index(t) for t in types], dtype=np.int32)
+        if (idx < 0).any():
+            raise ChimesError("unknown atom type(s): %s" % sorted({t for t, k in zip(types, idx) if k < 0}))
+        return idx
+
+    def order(self, body):
+        return self.L.chimes_gpu_model_order(self.h, body)
+
+    def max_cutoff(self, body):
+        return self.L.chimes_gpu_model_max_cutoff(self.h, body)
+
+    def timing(self):
+        t = np.zeros(4)
+        self.L.chimes_gpu_last_timing(self.h, t)
+        return t
+
+    def compute_clusters(self, nb, dist, dr, types):
+        """Batched explicit clusters; returns (energy[n], force[n, nb, 3], stress[n, 6])."""
+        dist = np.ascontiguousarray(dist, dtype=np.float64)
+        n = dist.shape[0]
+        dr = np.ascontiguousarray(dr, dtype=np.float64).reshape(-1)
+        types = np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
+        E, F, S = np.zeros(n), np.zeros(n * nb * 3), np.zeros(n * 6)
+        if self.L.chimes_gpu_compute_clusters(self.h, nb, n, dist.reshape(-1), dr, types, E, F, S):
+            raise ChimesError(_err())
+        return E, F.reshape(n, nb, 3), S.reshape(n, 6)
+
+    def cluster_lists(self, conf=0):
+        """Sorted canonical cluster keys of configuration `conf` of the last evaluation."""
+        n2, n3, n4 = C.c_long(), C.c_long(), C.c_long()
+        p2, p3, p4 = (C.POINTER(C.c_int)() for _ in range(3))
+        if self.L.chimes_gpu_cluster_lists(self.h, conf, C.byref(n2), C.byref(p2), C.byref(n3), C.byref(p3),
+                                           C.byref(n4), C.byref(p4)):
+            raise ChimesError(_err())
+        out = []
+        for p, n, w in ((p2, n2.value, 5), (p3, n3.value, 9), (p4, n4.value, 13)):
+            a = np.ctypeslib.as_array(p, shape=(max(n, 1) * w,))[: n * w].reshape(n, w).copy()
+            self.L.chimes_gpu_free(C.cast(p, C.c_void_p))
+            out.append(a)
+        return tuple(out)
+
+
+def _flatten(configs, model_of, models):
+    natoms = np.array([len(c[0]) for c in configs], dtype=np.int32)
+    pos = np.ascontiguousarray(np.concatenate([np.asarray(c[0], dtype=np.float64).reshape(-1, 3)
+                                               for c in configs]).reshape(-1))
+    types = np.concatenate([models[m].types_to_index(c[1]) for c, m in zip(configs, model_of)]).astype(np.int32)
+    cells = np.ascontiguousarray(np.concatenate([np.asarray(c[2], dtype=np.float64).reshape(9) for c in configs]))
+    return natoms, pos, types, cells
+
+
+def eval_batch(models, configs, model_of=None, small=False, precision=FP64):
+    """Evaluates configurations [(pos[n,3], types[n], cell[3,3]), ...].
+
+    Returns (energy[nconf], [forces[n_c, 3]], stress[nconf, 9])."""
+    if isinstance(models, GpuModel):
+        models = [models]
+    nconf = len(configs)
+    model_of = np.zeros(nconf, np.int32) if model_of is None else np.asarray(model_of, dtype=np.int32)
+    natoms, pos, types, cells = _flatten(configs, model_of, models)
+    arr = (C.c_void_p * len(models))(*[m.h for m in models])
+    E, F, S = np.zeros(nconf), np.zeros(pos.shape[0]), np.zeros(9 * nconf)
+    rc = lib().chimes_gpu_eval_batch(arr, len(models), nconf, model_of, natoms, pos, types, cells, int(small),
+                                     int(precision), E, F, S)
+    if rc:
+        raise ChimesError(_err())
+    F = F.reshape(-1, 3)
+    offs = np.concatenate([[0], np.cumsum(natoms)])
+    return E, [F[offs[k]:offs[k + 1]] for k in range(nconf)], S.reshape(nconf, 9)
+
+
+class ChimesSerial:
+    """System-level interface with the reference Python wrapper's calls
+    (serial_interface/api/chimescalc_serial_py.py:24-101) over the C ABI of
+    chimescalc_serial_C.h, one handle per object."""
+
+    def __init__(self, small=False):
+        self.L = lib()
+        self.h = self.L.chimes_open_instance()
+        self.L.set_chimes_serial_instance(self.h, int(bool(small)))
+
+    def init_chimes(self, param_file, rank=0):
+        self.L.init_chimes_serial_instance(self.h, os.fsencode(param_file), int(rank))
+
+    def calculate_chimes(self, natom, xc, yc, zc, atom_types, ca, cb, cc, energy=0.0, fx=None, fy=None, fz=None,
+                         stress=None):
+        """Returns (fx, fy, fz, stress, energy) like the reference wrapper."""
+        f = [np.ascontiguousarray(v if v is not None else np.zeros(natom), dtype=np.float64)
+             for v in (fx, fy, fz)]
+        st = np.ascontiguousarray(stress if stress is not None else np.zeros(9), dtype=np.float64)
+        e = C.c_double(energy)
+        ts = (C.c_char_p * natom)(*[t.encode() for t in atom_types])
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (xc, yc, zc, ca, cb, cc)]
+        self.L.calculate_chimes_instance(self.h, natom, arrs[0], arrs[1], arrs[2], C.cast(ts, C.c_void_p),
+                                         arrs[3], arrs[4], arrs[5], C.byref(e), f[0], f[1], f[2], st)
+        return f[0], f[1], f[2], st, e.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.chimes_close_instance(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

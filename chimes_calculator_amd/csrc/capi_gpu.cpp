@@ -1,0 +1,200 @@
+// chimes_gpu.h: batched, mixed-precision and device-buffer entry points.
+#include "chimes_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "host_model.h"
+
+struct chimes_gpu_model {
+    std::unique_ptr<chx::DeviceModel> dm;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &m)
+{
+    g_err = m;
+    return 1;
+}
+
+// Checks that all models live on one engine; returns it.
+chx::EngineHandle *common_engine(chimes_gpu_model *const *models, int nmodels, std::vector<chx_model *> &ms)
+{
+    if (nmodels <= 0 || !models) return nullptr;
+    chx::EngineHandle *eh = models[0]->dm->eh;
+    ms.resize(nmodels);
+    for (int k = 0; k < nmodels; k++) {
+        if (!models[k] || models[k]->dm->eh != eh) return nullptr;
+        ms[k] = models[k]->dm->m;
+    }
+    return eh;
+}
+
+void sort_rows(int *a, long n, int w)
+{
+    std::vector<std::vector<int>> rows(n);
+    for (long i = 0; i < n; i++) rows[i].assign(a + i * w, a + (i + 1) * w);
+    std::sort(rows.begin(), rows.end());
+    for (long i = 0; i < n; i++) std::memcpy(a + i * w, rows[i].data(), sizeof(int) * w);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *chimes_gpu_last_error(void) { return g_err.c_str(); }
+
+chimes_gpu_model *chimes_gpu_model_load(const char *param_file, int device)
+{
+    chx::ParamSet P;
+    std::string err;
+    if (!chx::load_params(param_file, P, err)) { fail(err); return nullptr; }
+    std::unique_ptr<chx::DeviceModel> dm = chx::make_device_model(P, device, err);
+    if (!dm) { fail(err); return nullptr; }
+    chimes_gpu_model *m = new chimes_gpu_model;
+    m->dm = std::move(dm);
+    return m;
+}
+
+void chimes_gpu_model_free(chimes_gpu_model *m) { delete m; }
+
+int chimes_gpu_model_ntypes(const chimes_gpu_model *m) { return m->dm->P.natmtyps; }
+
+int chimes_gpu_model_type_index(const chimes_gpu_model *m, const char *name)
+{
+    return chx::type_index(m->dm->P, name);
+}
+
+int chimes_gpu_model_order(const chimes_gpu_model *m, int body)
+{
+    return (body >= 2 && body <= 4) ? m->dm->P.order[body - 2] : 0;
+}
+
+double chimes_gpu_model_max_cutoff(const chimes_gpu_model *m, int body)
+{
+    return (body >= 2 && body <= 4) ? m->dm->P.max_cutoff(body) : 0.0;
+}
+
+int chimes_gpu_eval_batch(chimes_gpu_model *const *models, int nmodels, int nconf, const int *model_of,
+                          const int *natoms, const double *pos, const int *types, const double *cells, int small,
+                          int precision, double *energy, double *forces, double *stress)
+{
+    std::vector<chx_model *> ms;
+    chx::EngineHandle *eh = common_engine(models, nmodels, ms);
+    if (!eh) return fail("models must be loaded on the same device");
+    for (int c = 0; c < nconf; c++)
+        if (model_of[c] < 0 || model_of[c] >= nmodels) return fail("model index out of range");
+    long a = 0;
+    for (int c = 0; c < nconf; c++) {
+        const int nt = models[model_of[c]]->dm->P.natmtyps;
+        for (int k = 0; k < natoms[c]; k++, a++)
+            if (types[a] < 0 || types[a] >= nt) return fail("atom type index out of range");
+    }
+    chx_batch b{};
+    b.nconf = nconf;
+    b.natoms = natoms;
+    b.pos = pos;
+    b.types = types;
+    b.cells = cells;
+    b.model = model_of;
+    b.small = small;
+    b.precision = precision;
+    std::vector<double> V(6 * (size_t)nconf), vol(nconf);
+    std::vector<int> nrep(nconf);
+    chx_out o{};
+    o.energy = energy;
+    o.forces = forces;
+    o.virial = V.data();
+    o.volume = vol.data();
+    o.n_rep = nrep.data();
+    {
+        std::lock_guard<std::mutex> lk(eh->mu);
+        if (chx_evaluate(eh->eng, ms.data(), nmodels, &b, &o)) return fail(chx_last_error(eh->eng));
+    }
+    for (int c = 0; c < nconf; c++) {
+        energy[c] /= pow(nrep[c] + 1.0, 3.0);
+        const double *v = &V[6 * c];
+        double *s = stress + 9 * c;
+        s[0] = v[0]; s[1] = v[1]; s[2] = v[2];
+        s[3] = v[1]; s[4] = v[3]; s[5] = v[4];
+        s[6] = v[2]; s[7] = v[4]; s[8] = v[5];
+        for (int k = 0; k < 9; k++) s[k] /= vol[c];
+    }
+    return 0;
+}
+
+int chimes_gpu_eval_batch_device(chimes_gpu_model *const *models, int nmodels, int nconf, const int *model_of,
+                                 const int *natoms, const double *pos_dev, const int *types_dev,
+                                 const double *cells, int small, int precision, double *energy_dev,
+                                 double *forces_dev, double *virial_dev)
+{
+    std::vector<chx_model *> ms;
+    chx::EngineHandle *eh = common_engine(models, nmodels, ms);
+    if (!eh) return fail("models must be loaded on the same device");
+    chx_batch b{};
+    b.nconf = nconf;
+    b.natoms = natoms;
+    b.pos = pos_dev;
+    b.types = types_dev;
+    b.cells = cells;
+    b.model = model_of;
+    b.small = small;
+    b.pos_dev = 1;
+    b.precision = precision;
+    chx_out o{};
+    o.energy = energy_dev;
+    o.forces = forces_dev;
+    o.virial = virial_dev;
+    o.out_dev = 1;
+    std::lock_guard<std::mutex> lk(eh->mu);
+    if (chx_evaluate(eh->eng, ms.data(), nmodels, &b, &o)) return fail(chx_last_error(eh->eng));
+    return 0;
+}
+
+void chimes_gpu_last_timing(const chimes_gpu_model *m, double ms[4])
+{
+    chx_last_timing(m->dm->eh->eng, ms);
+}
+
+int chimes_gpu_compute_clusters(chimes_gpu_model *m, int nb, int n, const double *dist, const double *dr,
+                                const int *types, double *energy, double *force, double *stress)
+{
+    if (nb < 2 || nb > 4) return fail("cluster size must be 2, 3 or 4");
+    const int P = nb * (nb - 1) / 2;
+    const int nt = m->dm->P.natmtyps;
+    for (long k = 0; k < (long)n * nb; k++)
+        if (types[k] < 0 || types[k] >= nt) return fail("atom type index out of range");
+    std::vector<double> last((size_t)n * P, 0.0);
+    std::vector<int> flags(n);
+    chx::EngineHandle *eh = m->dm->eh;
+    std::lock_guard<std::mutex> lk(eh->mu);
+    if (chx_eval_clusters(eh->eng, m->dm->m, nb, n, dist, dr, types, energy, force, stress, last.data(),
+                          flags.data()))
+        return fail(chx_last_error(eh->eng));
+    return 0;
+}
+
+int chimes_gpu_cluster_lists(chimes_gpu_model *m, int conf, long *n2, int **pairs, long *n3, int **trips,
+                             long *n4, int **quads)
+{
+    chx::EngineHandle *eh = m->dm->eh;
+    std::lock_guard<std::mutex> lk(eh->mu);
+    if (chx_export_lists(eh->eng, conf, n2, pairs, n3, trips, n4, quads)) return fail(chx_last_error(eh->eng));
+    sort_rows(*pairs, *n2, 5);
+    sort_rows(*trips, *n3, 9);
+    sort_rows(*quads, *n4, 13);
+    return 0;
+}
+
+void chimes_gpu_free(void *p) { chx_free(p); }
+
+}  // extern "C"

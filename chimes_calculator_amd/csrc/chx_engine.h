@@ -1,0 +1,102 @@
+// Internal C-ABI layer between the C++ host classes and the HIP engine.
+// Plain pointers and sizes only; every array argument is host memory unless
+// its name ends in _dev.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chx_engine chx_engine;
+typedef struct chx_model chx_model;
+
+// A model description built by the host from a parsed parameter file.
+typedef struct {
+    int nt;
+    int order[3];
+    int npairs;
+    int tersoff;
+    double fcut_var, pen_dist, pen_scale;
+    const double *eoff;          // [nt]
+    const int *pairmap;          // [nt*nt]
+    const double *pair_slots;    // [npairs*5]: rin rout lam xavg xdiff
+    const double *c2;            // [npairs*(order2+1)] dense by Chebyshev degree
+    double rc[3];                // max outer cutoffs per body order
+    int ntrip, D3;
+    const int *tripmap;          // [nt^3]
+    const int *trip_perm;        // [nt^3*3]
+    const double *trip_slots;    // [ntrip*3*5]
+    const double *C3;            // [ntrip*D3^3]
+    const int *last3;            // [ntrip*3] powers of each type's last listed term
+    const double *lastc3;        // [ntrip]
+    int nquad, D4;
+    const int *quadmap;          // [nt^4]
+    const int *quad_perm;        // [nt^4*4]
+    const double *quad_slots;    // [nquad*6*5]
+    const double *C4;            // [nquad*D4^6]
+    const int *last4;            // [nquad*6]
+    const double *lastc4;        // [nquad]
+} chx_model_desc;
+
+typedef enum { CHX_FP64 = 0, CHX_MIXED = 1 } chx_precision;
+
+// A batch of independent configurations.  Positions are xyz-interleaved
+// input coordinates; types are indices into the configuration's model.
+typedef struct {
+    int nconf;
+    const int *natoms;           // [nconf]
+    const double *pos;           // [sum natoms * 3]   (host, or device if pos_dev)
+    const int *types;            // [sum natoms]       (host, or device if pos_dev)
+    const double *cells;         // [nconf * 9]: a, b, c cell vectors
+    const int *model;            // [nconf] index into the models array
+    int small;                   // replicate small cells (serial interface flag)
+    int pos_dev;                 // 1: pos/types are device pointers
+    int precision;               // chx_precision
+} chx_batch;
+
+typedef struct {
+    double *energy;              // [nconf] raw sum (before the 1/(R+1)^3 factor)
+    double *forces;              // [sum natoms * 3]
+    double *virial;              // [nconf * 6]: xx xy xz yy yz zz (before /volume)
+    double *volume;              // [nconf]
+    int *n_rep;                  // [nconf]
+    long long *counts;           // [nconf * 4]: penalty events, out-of-range events, 0, 0
+    int *badness;                // [nconf] 0 good, 1 penalty region, 2 below inner cutoff
+    int out_dev;                 // 1: energy/forces/virial are device pointers (others stay host)
+} chx_out;
+
+chx_engine *chx_engine_create(int device);
+void chx_engine_destroy(chx_engine *e);
+const char *chx_last_error(chx_engine *e);
+
+chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d);
+void chx_model_destroy(chx_model *m);
+
+// Runs the whole pipeline on the engine's stream and waits for it.
+// Returns 0 on success, nonzero on error (message in chx_last_error).
+int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx_batch *b, chx_out *o);
+
+// Cluster lists of configuration cfg of the last evaluate call, in canonical
+// key form (see oracle/chimes_oracle.h): 5 / 9 / 13 ints per entry.  Arrays
+// are malloc'd; free with chx_free.
+int chx_export_lists(chx_engine *e, int cfg, long *n2, int **pairs, long *n3, int **trips, long *n4,
+                     int **quads);
+void chx_free(void *p);
+
+// Explicit clusters (the per-cluster API): nb = 2, 3 or 4 bodies, n clusters.
+// dist[n*P], dr[n*P*3], types[n*nb]; outputs energy[n], force[n*nb*3],
+// stress[n*6] (packed xx xy xz yy yz zz), last[n*P] (last-term scalars,
+// 2-body: penalty scalar), flags[n].
+int chx_eval_clusters(chx_engine *e, chx_model *m, int nb, int n, const double *dist, const double *dr,
+                      const int *types, double *energy, double *force, double *stress, double *last,
+                      int *flags);
+
+// Timing of the last evaluate call (HIP events on the engine's stream, ms):
+// [0] whole pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] enumeration.
+void chx_last_timing(chx_engine *e, double *ms4);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,43 @@
+// Kernel declarations shared by the HIP translation units of the engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "chx_types.h"
+
+namespace chx {
+
+// Per-model data the enumeration kernel needs.
+struct chx_enum_model {
+    int nt, has3, has4, ntrip, nquad, g3, g4, pad;
+    const int *tripmap;   // [nt^3] cluster type, -1 excluded
+    const int *quadmap;   // [nt^4]
+};
+
+// Per-model data the owner reduction needs.
+struct chx_red_model {
+    int g3, ntrip, g4, nquad;
+    const double *eoff;
+};
+
+__global__ void k_fill_i32(int *p, long n, int v);
+__global__ void k_fill_f64(double *p, long n);
+__global__ void k_prep(int natoms, int ncfg, const int *cfg_atom_off, const chx_cfg *cfgs, const double *pos,
+                       const int *types, chx_atom_s *as, chx_atom_n *an, int *cell_count);
+__global__ void k_cell_fill(int natoms, const chx_atom_n *an, const int *cell_start, int *cursor, int *cell_atoms);
+__global__ void k_segment_sort(int nseg, const int *start, int *vals);
+template <int MODE>
+__global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
+                      const int *cell_start, const int *cell_atoms, int *nbr_cnt, const int *nbr_off,
+                      chx_entry_s *es, chx_entry_n *en, int *e_owner);
+template <int MODE>
+__global__ void k_enum(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
+                       const int *nbr_off, chx_entry_s *es, chx_entry_n *en, const chx_enum_model *em, int *cnt3,
+                       int *cnt4, const int *off3, const int *off4, int2 *list3, int2 *list4, int *status);
+__global__ void k_rev_count(long nent, const chx_entry_s *es, int *rcnt);
+__global__ void k_rev_fill(long nent, const chx_entry_s *es, const int *rstart, int *cursor, int *rev);
+__global__ void k_scan_block(const int *in, int *out, long n, int *bsum);
+__global__ void k_scan_add(int *out, long n, const int *bscan);
+__global__ void k_scan_total(int *out, long n, const int *in);
+__global__ void k_gather_strided(const int *in, long stride, int count, int *out);
+
+}  // namespace chx

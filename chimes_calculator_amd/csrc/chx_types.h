@@ -1,0 +1,94 @@
+// Plain-old-data layouts shared by the host engine and the HIP kernels.
+#pragma once
+#include <stdint.h>
+
+#define CHX_MAX_TYPES 8
+#define CHX_MAX_NB 1023        // neighbours per owner atom (slot codes are 10 bits)
+#define CHX_ENUM_MAX_NB 256    // neighbours per owner for the LDS enumeration path
+#define CHX_MAX_CLUSTER_TYPES 64
+
+// Per-configuration frame data (one per configuration in a batch).
+struct chx_cfg {
+    double h[9], ih[9];        // evaluation frame: replicated cell and inverse
+    double h0[9], ih0[9];      // input cell and inverse (replication step)
+    double hn[9], ihn[9];      // neighbour frame (reoriented cell) and inverse
+    double vol;
+    double search;             // max(rc2, rc3, rc4)
+    double rc3, rc4;           // max 3B / 4B outer cutoffs (list predicates)
+    int natoms_in;             // input atoms
+    int n_rep;                 // replication count R (R+1)^3 copies
+    int natoms;                // replicated atoms
+    int n_layers;              // ghost layers of the neighbour frame
+    int m[3];                  // cell grid
+    int reach[3];              // cells searched on each side
+    int atom_off;              // first replicated atom (global index)
+    int atom_in_off;           // first input atom (global index)
+    int cell_off;              // first cell (global index)
+    int model;                 // parameter set index
+    int pad[2];
+};
+
+// Replicated atom, evaluation frame: fractional coordinates u = ih * x.
+struct chx_atom_s {
+    double u[3];
+    int cfg;
+    int type;
+};
+
+// Replicated atom, neighbour frame.
+struct chx_atom_n {
+    double u[3];
+    int cell;
+    int orig;                  // global input atom this replica maps to
+};
+
+// Neighbour entry of an owner atom: an image (parent atom, lattice shift).
+struct chx_entry_s {
+    double v[3];               // evaluation-frame fractional coordinates of the image
+    int parent;                // global replicated atom index
+    int type;
+};
+struct chx_entry_n {
+    double v[3];               // neighbour-frame fractional coordinates of the image
+    int shift;                 // packed (sx+128) | (sy+128) << 8 | (sz+128) << 16
+    int pad;
+};
+
+// Per-slot constants of one cluster type (or pair type).
+struct chx_slot {
+    double rin, rout, lam, xavg, xdiff;
+};
+
+// Device parameter tables of one model; all pointers are device pointers.
+struct chx_model_dev {
+    int nt;
+    int order[3];
+    int npairs;
+    int tersoff;
+    double fcut_var;
+    double pen_dist, pen_scale;
+    const double *eoff;        // [nt]
+    const int *pairmap;        // [nt*nt]
+    // 2-body, per pair type p: slots2[p], coefficient k of pair p at c2[p*o2 + k]
+    // multiplies T_{pow2[p*o2+k]+1}
+    const chx_slot *slots2;
+    const int *ncoef2;
+    const int *pow2;
+    const double *c2;
+    // 3-body
+    int ntrip, D3;
+    const int *tripmap;        // [nt^3] cluster type, -1 excluded
+    const int *trip_perm;      // [nt^3 * 3]
+    const chx_slot *slots3;    // [ntrip * 3]
+    const double *C3;          // [ntrip * D3^3] dense coefficients, parameter slot order
+    const int *last3;          // [ntrip * 3] powers of the last listed term
+    const double *lastc3;      // [ntrip] its coefficient
+    // 4-body
+    int nquad, D4;
+    const int *quadmap;        // [nt^4]
+    const int *quad_perm;      // [nt^4 * 4]
+    const chx_slot *slots4;    // [nquad * 6]
+    const double *C4;          // [nquad * D4^6]
+    const int *last4;          // [nquad * 6]
+    const double *lastc4;      // [nquad]
+};

@@ -1,0 +1,512 @@
+// Geometry kernels: atom preparation (replication, wrapping, frames), cell
+// lists, neighbour (image) lists, 3-/4-body cluster enumeration, reverse
+// neighbour lists and an integer scan.
+//
+// All arithmetic on positions reproduces the reference's operation order
+// (serial_chimes_interface.h:141-208, serial_chimes_interface.cpp:150-341,
+// 477-587) and this file is compiled with -ffp-contract=off, so cluster
+// membership decided here is bit-identical to the reference's.
+#include <hip/hip_runtime.h>
+
+#include "chx_kernels.h"
+
+namespace chx {
+
+__device__ __forceinline__ void mv(const double *m, double x, double y, double z, double *o)
+{
+    o[0] = m[0] * x + m[1] * y + m[2] * z;
+    o[1] = m[3] * x + m[4] * y + m[5] * z;
+    o[2] = m[6] * x + m[7] * y + m[8] * z;
+}
+
+// Length of h * (b - a), the reference's get_dist on fractional coordinates.
+__device__ __forceinline__ double frac_dist(const double *h, const double *a, const double *b)
+{
+    const double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+    double r[3];
+    mv(h, dx, dy, dz, r);
+    return sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+}
+
+__device__ __forceinline__ int find_cfg(const int *off, int n, int r)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= r) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_fill_i32(int *p, long n, int v)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Atom preparation: one thread per replicated atom.
+// ---------------------------------------------------------------------------
+__global__ void k_prep(int natoms, int ncfg, const int *__restrict__ cfg_atom_off, const chx_cfg *__restrict__ cfgs,
+                       const double *__restrict__ pos, const int *__restrict__ types, chx_atom_s *as,
+                       chx_atom_n *an, int *cell_count)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= natoms) return;
+    const int c = find_cfg(cfg_atom_off, ncfg, r);
+    const chx_cfg &C = cfgs[c];
+    const int l = r - C.atom_off;
+    const int rho = l / C.natoms_in;
+    const int a = l - rho * C.natoms_in;
+    const int ain = C.atom_in_off + a;
+    double x[3] = {pos[3 * (long)ain], pos[3 * (long)ain + 1], pos[3 * (long)ain + 2]};
+    if (rho > 0) {  // replica (i, j, k) of serial_chimes_interface.cpp:243-283
+        const int w = C.n_rep + 1;
+        const int ri = rho / (w * w), rj = (rho / w) % w, rk = rho % w;
+        double t[3];
+        mv(C.ih0, x[0], x[1], x[2], t);
+        t[0] += ri; t[1] += rj; t[2] += rk;
+        mv(C.h0, t[0], t[1], t[2], x);
+    }
+    double t[3];
+    mv(C.ih, x[0], x[1], x[2], t);  // wrap (serial_chimes_interface.cpp:301-320)
+    t[0] -= floor(t[0]); t[1] -= floor(t[1]); t[2] -= floor(t[2]);
+    double xs[3], us[3], xn[3], un[3];
+    mv(C.h, t[0], t[1], t[2], xs);
+    mv(C.ih, xs[0], xs[1], xs[2], us);
+    mv(C.hn, us[0], us[1], us[2], xn);  // reorient (serial_chimes_interface.cpp:355-412)
+    mv(C.ihn, xn[0], xn[1], xn[2], un);
+    int cc[3];
+    for (int d = 0; d < 3; d++) {
+        int v = (int)floor(us[d] * C.m[d]);
+        cc[d] = v < 0 ? 0 : (v >= C.m[d] ? C.m[d] - 1 : v);
+    }
+    const int cell = C.cell_off + cc[0] + C.m[0] * (cc[1] + C.m[1] * cc[2]);
+    chx_atom_s s;
+    s.u[0] = us[0]; s.u[1] = us[1]; s.u[2] = us[2];
+    s.cfg = c;
+    s.type = types[ain];
+    as[r] = s;
+    chx_atom_n n;
+    n.u[0] = un[0]; n.u[1] = un[1]; n.u[2] = un[2];
+    n.cell = cell;
+    n.orig = ain;
+    an[r] = n;
+    atomicAdd(&cell_count[cell], 1);
+}
+
+__global__ void k_cell_fill(int natoms, const chx_atom_n *__restrict__ an, const int *__restrict__ cell_start,
+                            int *cursor, int *cell_atoms)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= natoms) return;
+    const int c = an[r].cell;
+    const int p = atomicAdd(&cursor[c], 1);
+    cell_atoms[cell_start[c] + p] = r;
+}
+
+// Sort the atoms of each cell by index, and of each reverse-list segment by
+// entry index, so that every later traversal is in a fixed order.
+__global__ void k_segment_sort(int nseg, const int *__restrict__ start, int *vals)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const int b = start[s], e = start[s + 1];
+    for (int i = b + 1; i < e; i++) {
+        const int v = vals[i];
+        int j = i - 1;
+        while (j >= b && vals[j] > v) { vals[j + 1] = vals[j]; j--; }
+        vals[j + 1] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists: for real atom i every image g = (a, n) with i <= a,
+// g != i and |r_g - r_i| < search in the neighbour frame
+// (serial_chimes_interface.cpp:682-731).  MODE 0 counts, MODE 1 writes.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void k_nbr(int natoms, const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
+                      const chx_atom_n *__restrict__ an, const int *__restrict__ cell_start,
+                      const int *__restrict__ cell_atoms, int *nbr_cnt, const int *__restrict__ nbr_off,
+                      chx_entry_s *es, chx_entry_n *en, int *e_owner)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms) return;
+    const chx_atom_s ai = as[i];
+    const chx_atom_n ni = an[i];
+    const chx_cfg &C = cfgs[ai.cfg];
+    const int lc = ni.cell - C.cell_off;
+    const int ci[3] = {lc % C.m[0], (lc / C.m[0]) % C.m[1], lc / (C.m[0] * C.m[1])};
+    int cnt = 0;
+    long w = MODE ? nbr_off[i] : 0;
+    const double S = C.search;
+    for (int dz = -C.reach[2]; dz <= C.reach[2]; dz++)
+        for (int dy = -C.reach[1]; dy <= C.reach[1]; dy++)
+            for (int dx = -C.reach[0]; dx <= C.reach[0]; dx++) {
+                const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
+                int wc[3], sh[3];
+                bool ok = true;
+                for (int d = 0; d < 3; d++) {
+                    int q = u[d] % C.m[d];
+                    if (q < 0) q += C.m[d];
+                    wc[d] = q;
+                    sh[d] = (u[d] - q) / C.m[d];
+                    if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
+                }
+                if (!ok) continue;
+                const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
+                const int cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
+                for (int p = cell_start[cell]; p < cell_start[cell + 1]; p++) {
+                    const int a = cell_atoms[p];
+                    if (a < i || (a == i && zero)) continue;
+                    const chx_atom_n na = an[a];
+                    double vn[3];
+                    if (zero) {
+                        vn[0] = na.u[0]; vn[1] = na.u[1]; vn[2] = na.u[2];
+                    } else {
+                        double t[3] = {na.u[0], na.u[1], na.u[2]};
+                        t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
+                        double xg[3];
+                        mv(C.hn, t[0], t[1], t[2], xg);
+                        mv(C.ihn, xg[0], xg[1], xg[2], vn);
+                    }
+                    if (!(frac_dist(C.hn, ni.u, vn) < S)) continue;
+                    if (MODE == 0) { cnt++; continue; }
+                    const chx_atom_s sa = as[a];
+                    chx_entry_s e;
+                    if (zero) {
+                        e.v[0] = sa.u[0]; e.v[1] = sa.u[1]; e.v[2] = sa.u[2];
+                    } else {
+                        double t[3] = {sa.u[0], sa.u[1], sa.u[2]};
+                        t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
+                        double xg[3];
+                        mv(C.h, t[0], t[1], t[2], xg);
+                        mv(C.ih, xg[0], xg[1], xg[2], e.v);
+                    }
+                    e.parent = a;
+                    e.type = sa.type;
+                    chx_entry_n f;
+                    f.v[0] = vn[0]; f.v[1] = vn[1]; f.v[2] = vn[2];
+                    f.shift = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
+                    f.pad = 0;
+                    es[w] = e;
+                    en[w] = f;
+                    e_owner[w] = i;
+                    w++;
+                }
+            }
+    if (MODE == 0) nbr_cnt[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Cluster enumeration, one 64-lane workgroup per owner atom i.
+// MODE 0: sort N2(i) by (parent, shift) in place, count clusters per group.
+// MODE 1: write cluster records (owner, packed slot codes) per group.
+// Predicates (serial_chimes_interface.cpp:749-857, with max 4B cutoff <= max
+// 3B cutoff): j, k in N2(i), j != k, parent(j) <= parent(k), all three
+// distances < rc3; quadruplets add l in N2(i), l not in {j, k},
+// parent(j), parent(k) <= parent(l), all six distances < rc4.
+// ---------------------------------------------------------------------------
+#define EW (CHX_ENUM_MAX_NB / 32)
+
+struct EnumShared {
+    double vn[CHX_ENUM_MAX_NB][3];
+    int par[CHX_ENUM_MAX_NB];
+    int typ[CHX_ENUM_MAX_NB];
+    int gst[CHX_ENUM_MAX_NB];
+    unsigned A3[CHX_ENUM_MAX_NB][EW];
+    unsigned A4[CHX_ENUM_MAX_NB][EW];
+    unsigned I3[EW], I4[EW];
+    int cnt[2][CHX_MAX_CLUSTER_TYPES + 1];
+    int run[CHX_MAX_CLUSTER_TYPES + 1];
+    int rowcnt[64][CHX_MAX_CLUSTER_TYPES + 1];
+    long long keys[CHX_ENUM_MAX_NB];
+};
+
+template <int MODE>
+__global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restrict__ cfgs,
+                                             const chx_atom_s *__restrict__ as, const chx_atom_n *__restrict__ an,
+                                             const int *__restrict__ nbr_off, chx_entry_s *es, chx_entry_n *en,
+                                             const chx_enum_model *__restrict__ em, int *cnt3, int *cnt4,
+                                             const int *__restrict__ off3, const int *__restrict__ off4,
+                                             int2 *list3, int2 *list4, int *status)
+{
+    __shared__ EnumShared sh;
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int off = nbr_off[i];
+    const int nb = nbr_off[i + 1] - off;
+    if (MODE == 0 && lane == 0) atomicMax(&status[1], nb);
+    if (nb > CHX_ENUM_MAX_NB) {
+        if (lane == 0) atomicOr(&status[0], 1);
+        return;
+    }
+    const chx_atom_s ai = as[i];
+    const chx_atom_n ni = an[i];
+    const chx_cfg &C = cfgs[ai.cfg];
+    const chx_enum_model &M = em[C.model];
+    const int N = natoms;
+    if (MODE == 0) {
+        // rank sort of the entries by (parent, shift)
+        long long key[CHX_ENUM_MAX_NB / 64];
+        chx_entry_s ps[CHX_ENUM_MAX_NB / 64];
+        chx_entry_n pn[CHX_ENUM_MAX_NB / 64];
+        int rank[CHX_ENUM_MAX_NB / 64];
+        for (int j = lane, q = 0; q < CHX_ENUM_MAX_NB / 64; j += 64, q++) {
+            if (j < nb) {
+                ps[q] = es[off + j];
+                pn[q] = en[off + j];
+                key[q] = ((long long)ps[q].parent << 24) | pn[q].shift;
+                sh.gst[j] = 0;
+            }
+        }
+        for (int j = lane, q = 0; q < CHX_ENUM_MAX_NB / 64; j += 64, q++)
+            if (j < nb) sh.keys[j] = key[q];
+        __syncthreads();
+        for (int j = lane, q = 0; q < CHX_ENUM_MAX_NB / 64; j += 64, q++) {
+            if (j >= nb) continue;
+            int rk = 0;
+            for (int k = 0; k < nb; k++) rk += sh.keys[k] < key[q];
+            rank[q] = rk;
+        }
+        __syncthreads();
+        for (int j = lane, q = 0; q < CHX_ENUM_MAX_NB / 64; j += 64, q++) {
+            if (j >= nb) continue;
+            const int r = rank[q];
+            es[off + r] = ps[q];
+            en[off + r] = pn[q];
+            sh.vn[r][0] = pn[q].v[0]; sh.vn[r][1] = pn[q].v[1]; sh.vn[r][2] = pn[q].v[2];
+            sh.par[r] = ps[q].parent;
+            sh.typ[r] = ps[q].type;
+        }
+        __syncthreads();
+    }
+    const bool any3 = M.has3 && C.rc3 > 0.0, any4 = M.has4 && C.rc4 > 0.0;
+    if (!any3 && !any4) return;  // counts were zero-filled by the host
+    for (int j = lane; j < nb; j += 64) {
+        if (MODE == 1) {  // MODE 0 holds the sorted entries in LDS already
+            const chx_entry_n e = en[off + j];
+            sh.vn[j][0] = e.v[0]; sh.vn[j][1] = e.v[1]; sh.vn[j][2] = e.v[2];
+            const chx_entry_s f = es[off + j];
+            sh.par[j] = f.parent;
+            sh.typ[j] = f.type;
+        }
+        for (int w = 0; w < EW; w++) { sh.A3[j][w] = 0u; sh.A4[j][w] = 0u; }
+    }
+    for (int w = lane; w < EW; w += 64) { sh.I3[w] = 0u; sh.I4[w] = 0u; }
+    for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) { sh.cnt[0][g] = 0; sh.cnt[1][g] = 0; sh.run[g] = 0; }
+    __syncthreads();
+    const double rc3 = C.rc3, rc4 = C.rc4;
+    for (int j = lane; j < nb; j += 64) {
+        const double d = frac_dist(C.hn, ni.u, sh.vn[j]);
+        if (d < rc3) atomicOr(&sh.I3[j >> 5], 1u << (j & 31));
+        if (d < rc4) atomicOr(&sh.I4[j >> 5], 1u << (j & 31));
+        int g = j;
+        while (g > 0 && sh.par[g - 1] == sh.par[j]) g--;
+        sh.gst[j] = g;
+    }
+    const int npair = nb * (nb - 1) / 2;
+    for (int p = lane; p < npair; p += 64) {
+        // decode p -> (j < k)
+        int k = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
+        while (k * (k - 1) / 2 > p) k--;
+        while ((k + 1) * k / 2 <= p) k++;
+        const int j = p - k * (k - 1) / 2;
+        const double d = frac_dist(C.hn, sh.vn[j], sh.vn[k]);
+        if (d < rc3) { atomicOr(&sh.A3[j][k >> 5], 1u << (k & 31)); atomicOr(&sh.A3[k][j >> 5], 1u << (j & 31)); }
+        if (d < rc4) { atomicOr(&sh.A4[j][k >> 5], 1u << (k & 31)); atomicOr(&sh.A4[k][j >> 5], 1u << (j & 31)); }
+    }
+    __syncthreads();
+    const int W = (nb + 31) >> 5;
+    const int nt = M.nt;
+    const int ti = ai.type;
+    // body 3 then body 4
+    for (int body = 3; body <= 4; body++) {
+        if ((body == 3 && !any3) || (body == 4 && !any4)) continue;
+        const int ngl = body == 3 ? M.ntrip + 1 : M.nquad + 1;
+        const int gbase = body == 3 ? M.g3 : M.g4;
+        const int *cmap = body == 3 ? M.tripmap : M.quadmap;
+        int *cntv = body == 3 ? cnt3 : cnt4;
+        const int *offv = body == 3 ? off3 : off4;
+        int2 *list = body == 3 ? list3 : list4;
+        for (int round = 0; round * 64 < nb; round++) {
+            const int j = round * 64 + lane;
+            if (MODE == 1) {
+                for (int g = 0; g < ngl; g++) sh.rowcnt[lane][g] = 0;
+            }
+            // pass A: count per local group; pass B (MODE 1): write
+            for (int pass = 0; pass < (MODE == 1 ? 2 : 1); pass++) {
+                if (MODE == 1 && pass == 1) {
+                    __syncthreads();
+                    for (int g = lane; g < ngl; g += 64) {
+                        int acc = sh.run[g];
+                        for (int q = 0; q < 64; q++) {
+                            const int c = sh.rowcnt[q][g];
+                            sh.rowcnt[q][g] = acc;
+                            acc += c;
+                        }
+                        sh.run[g] = acc;
+                    }
+                    __syncthreads();
+                }
+                if (j >= nb) continue;
+                const int gj = sh.gst[j];
+                const int tj = sh.typ[j];
+                if (body == 3) {
+                    if (!((sh.I3[j >> 5] >> (j & 31)) & 1u)) continue;
+                    for (int w = 0; w < W; w++) {
+                        unsigned m = sh.A3[j][w] & sh.I3[w];
+                        const int lo = w * 32;
+                        if (gj > lo) m &= (gj - lo >= 32) ? 0u : (~0u << (gj - lo));
+                        if (j >= lo && j < lo + 32) m &= ~(1u << (j - lo));
+                        while (m) {
+                            const int b = __ffs(m) - 1;
+                            m &= m - 1;
+                            const int k = lo + b;
+                            const int tup = (ti * nt + tj) * nt + sh.typ[k];
+                            int g = cmap[tup];
+                            if (g < 0) g = ngl - 1;
+                            if (MODE == 0) atomicAdd(&sh.cnt[0][g], 1);
+                            else if (pass == 0) sh.rowcnt[lane][g]++;
+                            else {
+                                const long pos = offv[(long)(gbase + g) * N + i] + sh.rowcnt[lane][g]++;
+                                list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10));
+                            }
+                        }
+                    }
+                } else {
+                    if (!((sh.I4[j >> 5] >> (j & 31)) & 1u)) continue;
+                    for (int w = 0; w < W; w++) {
+                        unsigned mk = sh.A4[j][w] & sh.I4[w];
+                        const int lo = w * 32;
+                        if (gj > lo) mk &= (gj - lo >= 32) ? 0u : (~0u << (gj - lo));
+                        if (j >= lo && j < lo + 32) mk &= ~(1u << (j - lo));
+                        while (mk) {
+                            const int bk = __ffs(mk) - 1;
+                            mk &= mk - 1;
+                            const int k = lo + bk;
+                            const int gk = sh.gst[k];
+                            const int tk = sh.typ[k];
+                            for (int v = 0; v < W; v++) {
+                                unsigned ml = sh.A4[j][v] & sh.A4[k][v] & sh.I4[v];
+                                const int lo2 = v * 32;
+                                if (gk > lo2) ml &= (gk - lo2 >= 32) ? 0u : (~0u << (gk - lo2));
+                                if (j >= lo2 && j < lo2 + 32) ml &= ~(1u << (j - lo2));
+                                if (k >= lo2 && k < lo2 + 32) ml &= ~(1u << (k - lo2));
+                                while (ml) {
+                                    const int bl = __ffs(ml) - 1;
+                                    ml &= ml - 1;
+                                    const int l = lo2 + bl;
+                                    const int tup = ((ti * nt + tj) * nt + tk) * nt + sh.typ[l];
+                                    int g = cmap[tup];
+                                    if (g < 0) g = ngl - 1;
+                                    if (MODE == 0) atomicAdd(&sh.cnt[0][g], 1);
+                                    else if (pass == 0) sh.rowcnt[lane][g]++;
+                                    else {
+                                        const long pos = offv[(long)(gbase + g) * N + i] + sh.rowcnt[lane][g]++;
+                                        list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10) | ((l + 1) << 20));
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (MODE == 0) {
+            __syncthreads();
+            for (int g = lane; g < ngl; g += 64) {
+                cntv[(long)(gbase + g) * N + i] = sh.cnt[0][g];
+                sh.cnt[0][g] = 0;
+            }
+            __syncthreads();
+        } else {
+            for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) sh.run[g] = 0;
+            __syncthreads();
+        }
+    }
+}
+
+template __global__ void k_nbr<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
+                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *);
+template __global__ void k_nbr<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
+                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *);
+template __global__ void k_enum<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
+                                   chx_entry_s *, chx_entry_n *, const chx_enum_model *, int *, int *,
+                                   const int *, const int *, int2 *, int2 *, int *);
+template __global__ void k_enum<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
+                                   chx_entry_s *, chx_entry_n *, const chx_enum_model *, int *, int *,
+                                   const int *, const int *, int2 *, int2 *, int *);
+
+// Reverse neighbour lists: entries grouped by parent atom.
+__global__ void k_rev_count(long nent, const chx_entry_s *__restrict__ es, int *rcnt)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nent) atomicAdd(&rcnt[es[e].parent], 1);
+}
+
+__global__ void k_rev_fill(long nent, const chx_entry_s *__restrict__ es, const int *__restrict__ rstart, int *cursor,
+                           int *rev)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nent) return;
+    const int p = es[e].parent;
+    rev[rstart[p] + atomicAdd(&cursor[p], 1)] = (int)e;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of int32 (out[n] = total), 2048 elements per block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scan_block(const int *in, int *out, long n, int *bsum)
+{
+    __shared__ int s[256];
+    const long base = (long)blockIdx.x * 2048;
+    int v[8], t = 0;
+    for (int q = 0; q < 8; q++) {
+        const long idx = base + threadIdx.x * 8 + q;
+        v[q] = idx < n ? in[idx] : 0;
+        t += v[q];
+    }
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        int x = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+        __syncthreads();
+        s[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - t;
+    for (int q = 0; q < 8; q++) {
+        const long idx = base + threadIdx.x * 8 + q;
+        if (idx < n) out[idx] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
+}
+
+__global__ void k_scan_add(int *out, long n, const int *bscan)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) out[idx] += bscan[idx / 2048];
+}
+
+__global__ void k_scan_total(int *out, long n, const int *in)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = (n > 0 ? out[n - 1] + in[n - 1] : 0);
+}
+
+__global__ void k_gather_strided(const int *in, long stride, int count, int *out)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g <= count) out[g] = in[(long)g * stride];
+}
+
+}  // namespace chx

@@ -1,0 +1,76 @@
+/*
+ * MI355X extensions of the ChIMES evaluator: batched configurations, batched
+ * explicit clusters, FP32-mixed precision and device-resident buffers.
+ *
+ * These entry points have no counterpart in the reference; they sit beside
+ * its C ABIs (chimescalc_serial_C.h, chimescalc_C.h) for callers that
+ * evaluate many systems at once.  Every function returns 0 on success and a
+ * nonzero code on error (message in chimes_gpu_last_error()); none exits.
+ */
+#ifndef CHIMES_GPU_H
+#define CHIMES_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chimes_gpu_model chimes_gpu_model;
+
+enum { CHIMES_GPU_FP64 = 0, CHIMES_GPU_MIXED = 1 };
+
+/* Parses a ChIMES parameter file and uploads its tables to GPU `device`. */
+chimes_gpu_model *chimes_gpu_model_load(const char *param_file, int device);
+void chimes_gpu_model_free(chimes_gpu_model *m);
+const char *chimes_gpu_last_error(void);
+
+int chimes_gpu_model_ntypes(const chimes_gpu_model *m);
+/* Index of an element name in the model's atom types, or -1. */
+int chimes_gpu_model_type_index(const chimes_gpu_model *m, const char *name);
+int chimes_gpu_model_order(const chimes_gpu_model *m, int body);          /* body = 2, 3, 4 */
+double chimes_gpu_model_max_cutoff(const chimes_gpu_model *m, int body);  /* body = 2, 3, 4 */
+
+/* Evaluates nconf independent periodic configurations in one pipeline.
+ *   models[nmodels], model_of[nconf]  model of each configuration
+ *   natoms[nconf]                      atoms per configuration
+ *   pos[3*sum natoms]                  x y z interleaved, configurations back to back
+ *   types[sum natoms]                  atom type indices of each configuration's model
+ *   cells[9*nconf]                     cell vectors a, b, c
+ *   small                              1: replicate cells shorter than the 2B cutoff
+ *   precision                          CHIMES_GPU_FP64 or CHIMES_GPU_MIXED
+ * Outputs (overwritten): energy[nconf] (kcal/mol, 1B..4B, divided by (R+1)^3
+ * when replicated), forces[3*sum natoms] (kcal/mol/A), stress[9*nconf]
+ * (virial / volume, kcal/mol/A^3, xx xy xz yx yy yz zx zy zz). */
+int chimes_gpu_eval_batch(chimes_gpu_model *const *models, int nmodels, int nconf, const int *model_of,
+                          const int *natoms, const double *pos, const int *types, const double *cells, int small,
+                          int precision, double *energy, double *forces, double *stress);
+
+/* Same, with pos/types already in device memory and energy/forces/virial
+ * written to device memory (virial: 6 per configuration, xx xy xz yy yz zz,
+ * not divided by volume; energy not divided by (R+1)^3). */
+int chimes_gpu_eval_batch_device(chimes_gpu_model *const *models, int nmodels, int nconf, const int *model_of,
+                                 const int *natoms, const double *pos_dev, const int *types_dev,
+                                 const double *cells, int small, int precision, double *energy_dev,
+                                 double *forces_dev, double *virial_dev);
+
+/* Milliseconds of the last evaluation on the model's device, from HIP events:
+ * [0] pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] lists. */
+void chimes_gpu_last_timing(const chimes_gpu_model *m, double ms[4]);
+
+/* Batched explicit clusters: n clusters of nb = 2, 3 or 4 atoms given by the
+ * nb(nb-1)/2 pair distances dist[n*P], pair vectors dr[n*P*3] (x_b - x_a,
+ * pair order 01 02 03 12 13 23) and type indices types[n*nb].  Overwrites
+ * energy[n], force[n*nb*3], stress[n*6] (xx xy xz yy yz zz, -sum f dr dr). */
+int chimes_gpu_compute_clusters(chimes_gpu_model *m, int nb, int n, const double *dist, const double *dr,
+                                const int *types, double *energy, double *force, double *stress);
+
+/* Cluster lists of configuration `conf` of the last evaluation, as sorted
+ * canonical keys: pairs (i, a, sx, sy, sz), triplets (i, a_j, s_j, a_k, s_k),
+ * quadruplets (i, a_j, s_j, a_k, s_k, a_l, s_l); free with chimes_gpu_free. */
+int chimes_gpu_cluster_lists(chimes_gpu_model *m, int conf, long *n2, int **pairs, long *n3, int **trips,
+                             long *n4, int **quads);
+void chimes_gpu_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,108 @@
+"""GPU parity: the HIP path against the reference's own outputs.
+
+Golden values (tests/golden/*.json) were produced by the compiled reference
+(tests/golden/make_golden.py).  FP64 energy, forces and stress must agree to
+1e-10 relative (forces and stress relative to the largest component of the
+quantity); cluster lists must be identical (sorted canonical keys, compared
+in full for C0 and by SHA-256 for the larger systems).
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import assert_close, golden_arrays
+from oracle_bindings import read_xyz, sort_keys
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def sha(keys):
+    return hashlib.sha256(np.ascontiguousarray(sort_keys(keys), dtype="<i4").tobytes()).hexdigest()
+
+
+def check_lists(model, rec, conf=0):
+    p2, p3, p4 = model.cluster_lists(conf)
+    assert len(p2) == rec["n_pairs"] and len(p3) == rec["n_trips"] and len(p4) == rec["n_quads"]
+    assert sha(p2) == rec["sha_pairs"], "pair list differs"
+    assert sha(p3) == rec["sha_trips"], "triplet list differs"
+    assert sha(p4) == rec["sha_quads"], "quadruplet list differs"
+
+
+def check_efs(tag, E, F, S, rec):
+    e0, f0, s0 = golden_arrays(rec)
+    assert abs(E - e0) <= RTOL * abs(e0), "%s: energy %.17g vs %.17g" % (tag, E, e0)
+    assert_close(tag + " forces", F, f0, RTOL)
+    assert_close(tag + " stress", S, s0, RTOL)
+
+
+@pytest.fixture(scope="module")
+def cc():
+    import chimes_calculator_amd as cc
+    return cc
+
+
+def test_reference_cases_c_abi(cc, ref_dir, ref_cases):
+    """Every reference fixture through the system C ABI (calculate_chimes)."""
+    for key, rec in ref_cases.items():
+        pf, xyz, small = key.split("|")
+        pos, types, cell = read_xyz(os.path.join(ref_dir, xyz))
+        print("case", key, flush=True)
+        s = cc.ChimesSerial(small=int(small))
+        s.init_chimes(os.path.join(ref_dir, pf))
+        fx, fy, fz, st, e = s.calculate_chimes(len(pos), pos[:, 0], pos[:, 1], pos[:, 2], types, cell[0], cell[1],
+                                               cell[2])
+        s.close()
+        check_efs(key, e, np.stack([fx, fy, fz], 1), st, rec)
+
+
+def test_reference_cases_lists(cc, ref_dir, ref_cases):
+    """Cluster lists of every reference fixture equal the reference's."""
+    for key, rec in ref_cases.items():
+        pf, xyz, small = key.split("|")
+        pos, types, cell = read_xyz(os.path.join(ref_dir, xyz))
+        print("lists", key, flush=True)
+        m = cc.GpuModel(os.path.join(ref_dir, pf))
+        E, F, S = cc.eval_batch(m, [(pos, types, cell)], small=int(small))
+        check_efs(key, E[0], F[0], S[0], rec)
+        check_lists(m, rec)
+
+
+def test_c0_full_lists(cc, model_dir, synth_cases):
+    from chimes_calculator_amd import workloads as W
+    mname, cfgs = W.workload("C0")
+    m = cc.GpuModel(os.path.join(model_dir, mname + ".txt"))
+    for small in (0, 1):
+        rec = synth_cases["C0|%d" % small]
+        E, F, S = cc.eval_batch(m, cfgs, small=small)
+        check_efs("C0", E[0], F[0], S[0], rec)
+        p2, p3, p4 = m.cluster_lists(0)
+        assert np.array_equal(p2, np.array(rec["pairs"], dtype=np.int32).reshape(-1, 5))
+        assert np.array_equal(p3, np.array(rec["trips"], dtype=np.int32).reshape(-1, 9))
+        assert len(p4) == 0
+
+
+def test_c1_water(cc, model_dir, synth_cases):
+    from chimes_calculator_amd import workloads as W
+    mname, cfgs = W.workload("C1")
+    m = cc.GpuModel(os.path.join(model_dir, mname + ".txt"))
+    rec = synth_cases["C1|0"]
+    E, F, S = cc.eval_batch(m, cfgs)
+    check_efs("C1", E[0], F[0], S[0], rec)
+    check_lists(m, rec)
+
+
+def test_c2_batch_subset(cc, model_dir, synth_cases):
+    """The first 24 configurations of the C2 batch, evaluated as one batch."""
+    from chimes_calculator_amd import workloads as W
+    mname, cfgs = W.workload("C2")
+    m = cc.GpuModel(os.path.join(model_dir, mname + ".txt"))
+    sub = cfgs[:24]
+    E, F, S = cc.eval_batch(m, sub)
+    for k in range(24):
+        rec = synth_cases["C2|%d" % k]
+        check_efs("C2[%d]" % k, E[k], F[k], S[k], rec)
+        check_lists(m, rec, conf=k)

@@ -1,0 +1,246 @@
+"""Pins the oracle (the C restatement, oracle/chimes_oracle.c) on the CPU.
+
+  * golden vectors of the compiled reference: E, F, stress and the cluster
+    lists of every reference fixture and of the synthetic C0/C1/C2 inputs;
+  * the reference's own expected_output files (6 decimals, 1e-3 abs);
+  * closed forms and brute force for the pieces the fixtures leave open.
+"""
+import hashlib
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import assert_close, golden_arrays
+from oracle_bindings import Oracle, read_xyz, sort_keys
+
+GPA = 6.9479  # serial_interface/examples/cpp/main.cpp:138
+
+
+def sha(keys):
+    return hashlib.sha256(np.ascontiguousarray(sort_keys(keys), dtype="<i4").tobytes()).hexdigest()
+
+
+def test_reference_cases(ref_dir, ref_cases):
+    for key, rec in ref_cases.items():
+        pf, xyz, small = key.split("|")
+        if "CO2400K" in pf:
+            continue  # 15 s of CPU; covered by test_reference_co2400k
+        pos, types, cell = read_xyz(os.path.join(ref_dir, xyz))
+        r = Oracle(os.path.join(ref_dir, pf)).calculate(pos, types, cell, small=int(small), lists=True)
+        e0, f0, s0 = golden_arrays(rec)
+        assert abs(r.energy - e0) <= 1e-12 * abs(e0), key
+        assert_close(key + " F", r.forces, f0, 1e-12)
+        assert_close(key + " S", r.stress, s0, 1e-12)
+        assert (len(r.pairs), len(r.trips), len(r.quads)) == (rec["n_pairs"], rec["n_trips"], rec["n_quads"])
+        assert sha(r.pairs) == rec["sha_pairs"] and sha(r.trips) == rec["sha_trips"]
+        assert sha(r.quads) == rec["sha_quads"]
+
+
+@pytest.mark.slow
+def test_reference_co2400k(ref_dir, ref_cases):
+    key = [k for k in ref_cases if "CO2400K" in k][0]
+    pf, xyz, small = key.split("|")
+    pos, types, cell = read_xyz(os.path.join(ref_dir, xyz))
+    r = Oracle(os.path.join(ref_dir, pf)).calculate(pos, types, cell, small=int(small))
+    e0, f0, s0 = golden_arrays(ref_cases[key])
+    assert abs(r.energy - e0) <= 1e-12 * abs(e0)
+    assert_close("F", r.forces, f0, 1e-12)
+
+
+def test_expected_output_files(ref_dir, ref_cases):
+    """The reference's regression files (energy, GPa stress, forces)."""
+    for key in ref_cases:
+        pf, xyz, small = key.split("|")
+        dat = os.path.join(ref_dir, pf + "." + xyz + ".dat")
+        if not os.path.exists(dat) or "CO2400K" in pf:
+            continue
+        if "liqCO" in pf and small == "0":
+            continue  # the reference's regression row for this input runs with small = 1
+        vals = np.array([float(v) for v in open(dat).read().split()])
+        pos, types, cell = read_xyz(os.path.join(ref_dir, xyz))
+        r = Oracle(os.path.join(ref_dir, pf)).calculate(pos, types, cell, small=int(small))
+        s = r.stress * GPA
+        ours = np.concatenate([[r.energy], [s[0], s[4], s[8], s[1], s[2], s[5]], r.forces.reshape(-1)])
+        # the files hold 6 decimals / 6 significant digits; compare to 1e-3 abs like compare.sh
+        assert len(vals) == len(ours), key
+        tol = 1e-3 + 1e-5 * np.abs(vals)  # 6 decimals / 6 significant digits in the file
+        assert (np.abs(ours - vals) <= tol).all(), key
+
+
+def test_synthetic_goldens(model_dir, synth_cases):
+    from chimes_calculator_amd import workloads as W
+    for wl in ("C0", "C2"):
+        mname, cfgs = W.workload(wl)
+        o = Oracle(os.path.join(model_dir, mname + ".txt"))
+        keys = [k for k in synth_cases if k.startswith(wl + "|")]
+        for k in keys:
+            idx = int(k.split("|")[1])
+            cfg = cfgs[0] if wl == "C0" else cfgs[idx]
+            small = idx if wl == "C0" else 0
+            r = o.calculate(*cfg, small=small, lists=True)
+            rec = synth_cases[k]
+            e0, f0, s0 = golden_arrays(rec)
+            assert abs(r.energy - e0) <= 1e-12 * abs(e0), k
+            assert_close(k + " F", r.forces, f0, 1e-12)
+            assert sha(r.pairs) == rec["sha_pairs"] and sha(r.trips) == rec["sha_trips"], k
+            assert sha(r.quads) == rec["sha_quads"], k
+
+
+# ---------------------------------------------------------------------------
+# closed forms
+# ---------------------------------------------------------------------------
+
+def _r_of_x(x, lam, rin, rout):
+    xmin, xmax = math.exp(-rin / lam), math.exp(-rout / lam)
+    xavg, xdiff = 0.5 * (xmax + xmin), -0.5 * (xmax - xmin)
+    return -lam * math.log(x * xdiff + xavg), xdiff
+
+
+def test_cheby_closed_forms():
+    from oracle_bindings import oracle_lib
+    L = oracle_lib()
+    lam, rin, rout, n = 1.4, 1.0, 4.0, 12
+    for theta in np.linspace(0.05, math.pi - 0.05, 17):
+        x = math.cos(theta)
+        r, xdiff = _r_of_x(x, lam, rin, rout)
+        T, dT = np.zeros(n + 1), np.zeros(n + 1)
+        L.oc_cheby(r, lam, rin, rout, n, T, dT)
+        k = np.arange(n + 1)
+        assert np.allclose(T, np.cos(k * theta), atol=1e-12)
+        # dT/dr = n U_{n-1}(x) dx/dr, U_{n-1}(cos t) = sin(n t)/sin(t)
+        dxdr = -math.exp(-r / lam) / lam / xdiff
+        want = k * np.sin(k * theta) / math.sin(theta) * dxdr
+        assert np.allclose(dT, want, rtol=1e-10, atol=1e-10)
+    # Morse endpoints: x(r_in) = +1, x(r_out) = -1
+    for r, xe in ((rin, 1.0), (rout, -1.0)):
+        T, dT = np.zeros(n + 1), np.zeros(n + 1)
+        L.oc_cheby(r + (1e-13 if xe > 0 else 0.0), lam, rin, rout, n, T, dT)
+        assert abs(T[1] - xe) < 1e-9
+
+
+def test_cheby_out_of_range_continuity():
+    from oracle_bindings import oracle_lib
+    L = oracle_lib()
+    lam, rin, rout, n = 1.25, 1.0, 3.15, 8
+    T0, d0 = np.zeros(n + 1), np.zeros(n + 1)
+    T1, d1 = np.zeros(n + 1), np.zeros(n + 1)
+    L.oc_cheby(rin, lam, rin, rout, n, T0, d0)
+    L.oc_cheby(rin - 1e-9, lam, rin, rout, n, T1, d1)
+    assert np.allclose(T0, T1, atol=1e-6) and np.allclose(d0, d1, rtol=1e-6, atol=1e-6)
+    # below r_in the derivative decays as exp((r - rin)/0.01)
+    T2, d2 = np.zeros(n + 1), np.zeros(n + 1)
+    L.oc_cheby(rin - 0.05, lam, rin, rout, n, T2, d2)
+    assert np.allclose(d2, d0 * math.exp(-5.0), rtol=1e-12, atol=1e-14)
+
+
+def test_fcut_closed_forms(ref_dir):
+    import ctypes as C
+    from oracle_bindings import oracle_lib
+    L = oracle_lib()
+    cubic = Oracle(os.path.join(ref_dir, "published_params.liqC.2+3b.cubic.txt"))
+    ters = Oracle(os.path.join(ref_dir, "published_params.HN3.2+3+4b.Tersoff.special.offsets.txt"))
+    f, fd = C.c_double(), C.c_double()
+    for r in np.linspace(0.5, 3.9, 9):
+        L.oc_fcut(cubic.h, r, 4.0, C.byref(f), C.byref(fd))
+        assert abs(f.value - (1 - r / 4.0) ** 3) < 1e-14
+        assert abs(fd.value + 3 * (1 - r / 4.0) ** 2 / 4.0) < 1e-14
+    rc, fv = 5.0, 0.5  # HN3 file: FCUT TYPE: TERSOFF 0.5
+    thr = rc - fv * rc
+    for r in np.linspace(0.5, 5.5, 23):
+        L.oc_fcut(ters.h, r, rc, C.byref(f), C.byref(fd))
+        if r < thr:
+            assert f.value == 1.0 and fd.value == 0.0
+        elif r > rc:
+            assert f.value == 0.0 and fd.value == 0.0
+        else:
+            pi = 3.14159265359
+            a = (r - thr) / (rc - thr) * pi + pi / 2
+            assert abs(f.value - (0.5 + 0.5 * math.sin(a))) < 1e-14
+            assert abs(fd.value - 0.5 * math.cos(a) * pi / (rc - thr)) < 1e-14
+
+
+def _random_cluster(rng, nb, scale):
+    pts = rng.normal(size=(nb, 3))
+    pts *= scale / np.linalg.norm(pts[1] - pts[0])
+    return pts
+
+
+def _cluster_eval(L, h, nb, pts, t):
+    import ctypes as C
+    idx = [(a, b) for a in range(nb) for b in range(a + 1, nb)]
+    dr = np.concatenate([pts[b] - pts[a] for a, b in idx])
+    dx = np.array([np.linalg.norm(pts[b] - pts[a]) for a, b in idx])
+    f, s, e = np.zeros(3 * nb), np.zeros(6), C.c_double(0.0)
+    fn = getattr(L, "oc_compute_%db" % nb)
+    if nb == 2:
+        fn(h, float(dx[0]), dr, t, f, s, C.byref(e))
+    else:
+        fn(h, dx, dr, t, f, s, C.byref(e))
+    return e.value, f.reshape(nb, 3), s
+
+
+def test_cluster_forces_finite_difference(ref_dir):
+    """F = -dE/dx and stress = sum_i x_i (x) F_i = -sum f dr (x) dr for 2/3/4-body clusters."""
+    from oracle_bindings import oracle_lib
+    L = oracle_lib()
+    o = Oracle(os.path.join(ref_dir, "published_params.HN3.2+3+4b.Tersoff.special.offsets.txt"))
+    rng = np.random.default_rng(7)
+    for nb in (2, 3, 4):
+        for _ in range(6):
+            pts = _random_cluster(rng, nb, rng.uniform(1.2, 2.4))
+            t = rng.integers(0, 2, nb).astype(np.int32)
+            e, f, s = _cluster_eval(L, o.h, nb, pts, t)
+            if e == 0.0:
+                continue
+            h = 1e-6
+            for a in range(nb):
+                for k in range(3):
+                    p1, p2 = pts.copy(), pts.copy()
+                    p1[a, k] += h
+                    p2[a, k] -= h
+                    fd = -(_cluster_eval(L, o.h, nb, p1, t)[0] - _cluster_eval(L, o.h, nb, p2, t)[0]) / (2 * h)
+                    assert abs(fd - f[a, k]) <= 1e-5 * max(1.0, np.abs(f).max()), (nb, a, k, fd, f[a, k])
+            # virial: sum_i x_i (x) f_i (translation invariant since sum f = 0)
+            assert np.abs(f.sum(0)).max() < 1e-9 * max(1.0, np.abs(f).max())
+            W = np.einsum("ai,aj->ij", pts, f)
+            packed = np.array([W[0, 0], W[0, 1], W[0, 2], W[1, 1], W[1, 2], W[2, 2]])
+            assert np.allclose(s, packed, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(s).max()))
+
+
+def test_cluster_energy_brute_force(model_dir):
+    """Long-double term sum of E = fcut_all * sum_k c_k prod T for 3-body
+    clusters of the generated C0 model, independent of the oracle's loop."""
+    import ctypes as C
+    from oracle_bindings import oracle_lib
+    L = oracle_lib()
+    path = os.path.join(model_dir, "carbon_2b3b.txt")
+    o = Oracle(path)
+    terms = []
+    lines = open(path).read().splitlines()
+    i = [k for k, l in enumerate(lines) if "TRIPLETTYPE PARAMS" in l][0]
+    K = int(lines[i + 2].split()[-1])
+    for l in lines[i + 5:i + 5 + K]:
+        t = l.split()
+        terms.append(((int(t[1]), int(t[2]), int(t[3])), float(t[6])))
+    rng = np.random.default_rng(3)
+    lam, rin, rc = 1.4, 1.0, 3.5
+    checked = 0
+    for _ in range(40):
+        pts = _random_cluster(rng, 3, rng.uniform(1.5, 2.5))
+        e, _, _ = _cluster_eval(L, o.h, 3, pts, np.zeros(3, np.int32))
+        d = [np.linalg.norm(pts[b] - pts[a]) for a, b in ((0, 1), (0, 2), (1, 2))]
+        if max(d) >= rc:
+            assert e == 0.0
+            continue
+        if min(d) < rin:
+            continue  # damped out-of-range branch, pinned by test_cheby_out_of_range_continuity
+        xmin, xmax = math.exp(-rin / lam), math.exp(-rc / lam)
+        xs = [(math.exp(-r / lam) - 0.5 * (xmax + xmin)) / (-0.5 * (xmax - xmin)) for r in d]
+        T = [np.polynomial.chebyshev.chebvander(np.longdouble(x), 8)[0] for x in xs]
+        fc = np.prod([(1 - np.longdouble(r) / rc) ** 3 for r in d])
+        want = fc * sum(np.longdouble(c) * T[0][p[0]] * T[1][p[1]] * T[2][p[2]] for p, c in terms)
+        assert abs(e - float(want)) <= 1e-12 * max(1e-12, abs(float(want)))
+        checked += 1
+    assert checked >= 5
