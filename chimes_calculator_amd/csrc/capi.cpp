@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "chimesFF.h"
+#include "errors.h"
 #include "chimescalc_C.h"
 #include "chimescalc_serial_C.h"
 #include "serial_chimes_interface.h"
@@ -32,8 +33,7 @@ void check_small(int small)
 {
     if (small != 0 && small != 1) {
         std::cout << "ERROR: Small must be set to 0 (false) or 1 (true)" << std::endl;
-        std::cout << "Received: " << small << std::endl;
-        exit(0);
+        chx::fatal_error("Received: " + std::to_string(small));
     }
 }
 
@@ -41,8 +41,7 @@ int type_of(chimesFF *ff, const char *name)
 {
     auto it = std::find(ff->atmtyps.begin(), ff->atmtyps.end(), std::string(name));
     if (it == ff->atmtyps.end()) {
-        std::cout << "ERROR: input atom type not in parameter file: " << name << std::endl;
-        exit(0);
+        chx::fatal_error(std::string("ERROR: input atom type not in parameter file: ") + name);
     }
     return (int)(it - ff->atmtyps.begin());
 }
@@ -75,8 +74,7 @@ void set_chimes_serial_instance(void *handle, int small)
 void init_chimes_serial(char *param_file, int *rank)
 {
     if (!g_serial) {
-        std::cout << "ERROR: set_chimes_serial must be called before init_chimes_serial" << std::endl;
-        exit(0);
+        chx::fatal_error("ERROR: set_chimes_serial must be called before init_chimes_serial");
     }
     g_serial->init_chimesFF(param_file, *rank);
 }
@@ -92,8 +90,7 @@ void calculate_chimes(int natom, double *xc, double *yc, double *zc, char *atom_
                       double stress[9])
 {
     if (!g_serial) {
-        std::cout << "ERROR: set_chimes_serial must be called before calculate_chimes" << std::endl;
-        exit(0);
+        chx::fatal_error("ERROR: set_chimes_serial must be called before calculate_chimes");
     }
     calculate_chimes_instance(g_serial, natom, xc, yc, zc, atom_types, ca, cb, cc, energy, fx, fy, fz, stress);
 }
@@ -130,8 +127,7 @@ void calculate_chimes_instance(void *handle, int natom, double *xc, double *yc, 
 static chimesFF *ff()
 {
     if (!g_ff) {
-        std::cout << "ERROR: set_chimes must be called first" << std::endl;
-        exit(0);
+        chx::fatal_error("ERROR: set_chimes must be called first");
     }
     return g_ff;
 }

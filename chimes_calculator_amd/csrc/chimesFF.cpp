@@ -13,6 +13,7 @@
 
 #include "chimesFF.h"
 #include "host_model.h"
+#include "errors.h"
 #include "serial_chimes_interface.h"
 
 struct chimesFF_impl {
@@ -21,11 +22,7 @@ struct chimesFF_impl {
 
 namespace {
 
-[[noreturn]] void fatal(const std::string &msg)
-{
-    std::cout << "chimesFF: ERROR: " << msg << std::endl;
-    exit(0);
-}
+[[noreturn]] void fatal(const std::string &msg) { chx::fatal_error("chimesFF: ERROR: " + msg); }
 
 bool quiet() { return getenv("CHIMES_QUIET") != nullptr; }
 
@@ -228,8 +225,8 @@ void serial_chimes_interface::calculate(vector<double> &x_in, vector<double> &y_
 {
     require_model("calculate");
     const int n = (int)x_in.size();
-    if ((int)y_in.size() != n) { std::cout << "ERROR: x and y coordinate vector lengths do not match!" << std::endl; exit(0); }
-    if ((int)z_in.size() != n) { std::cout << "ERROR: x and z coordinate vector lengths do not match!" << std::endl; exit(0); }
+    if ((int)y_in.size() != n) fatal("x and y coordinate vector lengths do not match!");
+    if ((int)z_in.size() != n) fatal("x and z coordinate vector lengths do not match!");
     chx::DeviceModel &dm = *impl_->dm;
     // the reference truncates the caller's type list to n (serial_chimes_interface.cpp:187)
     if ((int)atmtyps_in.size() > n) atmtyps_in.erase(atmtyps_in.begin() + n, atmtyps_in.end());
@@ -240,11 +237,9 @@ void serial_chimes_interface::calculate(vector<double> &x_in, vector<double> &y_
         pos[3 * a + 1] = y_in[a];
         pos[3 * a + 2] = z_in[a];
         typ[a] = chx::type_index(dm.P, a < (int)atmtyps_in.size() ? atmtyps_in[a] : std::string());
-        if (typ[a] < 0) {
-            std::cout << "ERROR: Couldn't assign an atom type index for (index/type) " << a << " "
-                      << (a < (int)atmtyps_in.size() ? atmtyps_in[a] : std::string()) << std::endl;
-            exit(0);
-        }
+        if (typ[a] < 0)
+            fatal("Couldn't assign an atom type index for (index/type) " + std::to_string(a) + " " +
+                  (a < (int)atmtyps_in.size() ? atmtyps_in[a] : std::string()));
     }
     double cells[9] = {cella_in[0], cella_in[1], cella_in[2], cellb_in[0], cellb_in[1],
                        cellb_in[2], cellc_in[0], cellc_in[1], cellc_in[2]};
@@ -273,10 +268,7 @@ void serial_chimes_interface::calculate(vector<double> &x_in, vector<double> &y_
     {
         std::lock_guard<std::mutex> lk(dm.eh->mu);
         chx_model *m = dm.m;
-        if (chx_evaluate(dm.eh->eng, &m, 1, &b, &o)) {
-            std::cout << "ERROR: " << chx_last_error(dm.eh->eng) << std::endl;
-            exit(0);
-        }
+        if (chx_evaluate(dm.eh->eng, &m, 1, &b, &o)) fatal(chx_last_error(dm.eh->eng));
     }
     if (bad > badness) badness = bad;
     if (counts[1] && !quiet())

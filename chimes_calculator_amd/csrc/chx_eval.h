@@ -36,8 +36,8 @@ __global__ void k_eval2(long nent, const int *e_owner, const chx_cfg *cfgs, cons
                         const chx_entry_s *es, const chx_model_dev *models, double *rec2E, double *rec2F,
                         int *cfg_stat);
 __global__ void k_reduce(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const int *nbr_off,
-                         const chx_entry_s *es, const chx_red_args ra, double *Fe, double *Fself, double *Eown,
-                         double *Sown);
+                         const chx_entry_s *es, const unsigned char *e_in3, const chx_red_args ra, double *Fe,
+                         double *Fself, double *Eown, double *Sown);
 __global__ void k_gather_rep(int natoms, const int *rstart, const int *rev, const double *Fe, const double *Fself,
                              double *Frep);
 __global__ void k_gather_in(int natoms_in, int ncfg, const int *cfg_in_off, const chx_cfg *cfgs,

@@ -29,10 +29,13 @@ template <int MODE>
 __global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
                       const int *cell_start, const int *cell_atoms, int *nbr_cnt, const int *nbr_off,
                       chx_entry_s *es, chx_entry_n *en, int *e_owner);
+__global__ void k_nbr_sort(int natoms, const int *nbr_off, const chx_entry_s *es, const chx_entry_n *en,
+                           chx_entry_s *es2, chx_entry_n *en2, int *status);
 template <int MODE>
 __global__ void k_enum(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
-                       const int *nbr_off, chx_entry_s *es, chx_entry_n *en, const chx_enum_model *em, int *cnt3,
-                       int *cnt4, const int *off3, const int *off4, int2 *list3, int2 *list4, int *status);
+                       const int *nbr_off, const chx_entry_s *es, const chx_entry_n *en, const chx_enum_model *em,
+                       int *cnt3, int *cnt4, const int *off3, const int *off4, int2 *list3, int2 *list4,
+                       unsigned char *e_in3, int *status);
 __global__ void k_rev_count(long nent, const chx_entry_s *es, int *rcnt);
 __global__ void k_rev_fill(long nent, const chx_entry_s *es, const int *rstart, int *cursor, int *rev);
 __global__ void k_scan_block(const int *in, int *out, long n, int *bsum);

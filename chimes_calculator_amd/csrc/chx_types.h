@@ -3,8 +3,9 @@
 #include <stdint.h>
 
 #define CHX_MAX_TYPES 8
-#define CHX_MAX_NB 1023        // neighbours per owner atom (slot codes are 10 bits)
-#define CHX_ENUM_MAX_NB 256    // neighbours per owner for the LDS enumeration path
+#define CHX_MAX_NB 512         // neighbour images per owner atom within the search cutoff
+#define CHX_MAX_N3 512         // images per owner within the 3-body cutoff
+#define CHX_MAX_N4 256         // images per owner within the 4-body cutoff
 #define CHX_MAX_CLUSTER_TYPES 64
 
 // Per-configuration frame data (one per configuration in a batch).

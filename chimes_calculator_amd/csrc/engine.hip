@@ -220,8 +220,9 @@ struct chx_engine {
     DevBuf<chx_atom_n> an;
     DevBuf<int> cell_cnt, cell_start, cell_cur, cell_atoms;
     DevBuf<int> nbr_cnt, nbr_off, e_owner;
-    DevBuf<chx_entry_s> es;
-    DevBuf<chx_entry_n> en;
+    DevBuf<chx_entry_s> es, es2;
+    DevBuf<chx_entry_n> en, en2;
+    DevBuf<unsigned char> e_in3;
     DevBuf<int> cnt3, off3, cnt4, off4, status, gtmp;
     DevBuf<int2> list3, list4;
     DevBuf<int> rcnt, rstart, rcur, rev;
@@ -601,19 +602,38 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
         hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 128)), dim3(128), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
                            e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
                            e->e_owner.p);
+        // canonical (parent, shift) order of every owner's images
+        e->es2.ensure(nent);
+        e->en2.ensure(nent);
+        e->e_in3.ensure(nent);
+        e->status.ensure(4);
+        e->zero_i(e->status.p, 4);
+        hipLaunchKernelGGL(k_nbr_sort, dim3((unsigned)N), dim3(64), 0, st, (int)N, e->nbr_off.p, e->es.p, e->en.p,
+                           e->es2.p, e->en2.p, e->status.p);
+        std::swap(e->es.p, e->es2.p);
+        std::swap(e->es.cap, e->es2.cap);
+        std::swap(e->en.p, e->en2.p);
+        std::swap(e->en.cap, e->en2.cap);
+        {
+            std::vector<int> s0(2);
+            HIPCHK(hipMemcpyAsync(s0.data(), e->status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (s0[0])
+                throw EngineError("an atom has " + std::to_string(s0[1]) + " neighbour images within the search "
+                                  "cutoff; at most " + std::to_string(CHX_MAX_NB) + " are supported");
+        }
         // ---------------- cluster enumeration ------------------------------
         const long n3c = (long)G3 * N + 1, n4c = (long)G4 * N + 1;
         e->cnt3.ensure(n3c);
         e->off3.ensure(n3c + 1);
         e->cnt4.ensure(n4c);
         e->off4.ensure(n4c + 1);
-        e->status.ensure(4);
         e->zero_i(e->cnt3.p, n3c);
         e->zero_i(e->cnt4.p, n4c);
-        e->zero_i(e->status.p, 4);
         hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
                            e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p, e->cnt4.p, (const int *)nullptr,
-                           (const int *)nullptr, (int2 *)nullptr, (int2 *)nullptr, e->status.p);
+                           (const int *)nullptr, (int2 *)nullptr, (int2 *)nullptr, (unsigned char *)nullptr,
+                           e->status.p);
         e->scan(e->cnt3.p, e->off3.p, (long)G3 * N);
         e->scan(e->cnt4.p, e->off4.p, (long)G4 * N);
         e->gtmp.ensure(G3 + G4 + 2);
@@ -623,7 +643,9 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
         HIPCHK(hipMemcpyAsync(gh.data(), e->gtmp.p, (G3 + G4 + 2) * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(sth.data(), e->status.p, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (sth[0]) throw EngineError("an atom has more than 256 neighbour images within the search cutoff");
+        if (sth[0])
+            throw EngineError("an atom has more than " + std::to_string(CHX_MAX_N3) + " images within the 3-body "
+                              "cutoff or more than " + std::to_string(CHX_MAX_N4) + " within the 4-body cutoff");
         e->goff3.assign(gh.begin(), gh.begin() + G3 + 1);
         e->goff4.assign(gh.begin() + G3 + 1, gh.end());
         const long n3 = e->goff3[G3], n4 = e->goff4[G4];
@@ -631,7 +653,7 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
         e->list4.ensure(n4);
         hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
                            e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p, e->cnt4.p, e->off3.p, e->off4.p,
-                           e->list3.p, e->list4.p, e->status.p);
+                           e->list3.p, e->list4.p, e->e_in3.p, e->status.p);
         // ---------------- reverse lists ------------------------------------
         e->rcnt.ensure(N + 1);
         e->rstart.ensure(N + 1);
@@ -716,7 +738,7 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
         ra.rec4F = e->rec4F.p;
         ra.rec4M = e->rec4M.p;
         hipLaunchKernelGGL(k_reduce, dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs.p, e->as.p, e->nbr_off.p,
-                           e->es.p, ra, e->Fe.p, e->Fself.p, e->Eown.p, e->Sown.p);
+                           e->es.p, e->e_in3.p, ra, e->Fe.p, e->Fself.p, e->Eown.p, e->Sown.p);
         hipLaunchKernelGGL(k_gather_rep, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, e->rstart.p, e->rev.p,
                            e->Fe.p, e->Fself.p, e->Frep.p);
         double *Fout = o->out_dev ? o->forces : e->Fout.p;

@@ -245,13 +245,16 @@ __device__ __forceinline__ void scan_chunk(const int *cidx, const double *cval, 
 __global__ __launch_bounds__(64) void k_reduce(int natoms, const chx_cfg *__restrict__ cfgs,
                                                 const chx_atom_s *__restrict__ as, const int *__restrict__ nbr_off,
                                                 const chx_entry_s *__restrict__ es,
-                                                const chx_red_args ra, double *Fe, double *Fself, double *Eown,
-                                                double *Sown)
+                                                const unsigned char *__restrict__ e_in3, const chx_red_args ra,
+                                                double *Fe, double *Fself, double *Eown, double *Sown)
 {
     __shared__ double cells[RTILE];
-    __shared__ double pos[CHX_ENUM_MAX_NB + 1][3];
+    __shared__ double pos[CHX_MAX_NB + 1][3];
+    __shared__ short s2m[CHX_MAX_NB + 1];     // slot -> many-body index (0 = owner), -1 if none
+    __shared__ short m2s[CHX_MAX_N3 + 1];     // many-body index -> slot
     __shared__ int cidx[64 * 6];
     __shared__ double cval[64 * 6];
+    __shared__ int nmb;
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int off = nbr_off[i];
@@ -267,31 +270,59 @@ __global__ __launch_bounds__(64) void k_reduce(int natoms, const chx_cfg *__rest
             pos[s][0] = g.v[0]; pos[s][1] = g.v[1]; pos[s][2] = g.v[2];
         }
     }
-    double F[5][3];
+    // many-body slot set: owner + images flagged by the enumeration, in slot order
+    if (lane == 0) { nmb = 1; s2m[0] = 0; m2s[0] = 0; }
+    __syncthreads();
+    for (int base = 1; base <= nb; base += 64) {
+        const int s = base + lane;
+        const bool in = s <= nb && e_in3[off + s - 1];
+        const unsigned long long b = __ballot(in);
+        const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const int m0 = nmb;
+        if (s <= nb) {
+            const int m = in ? m0 + __popcll(b & below) : -1;
+            s2m[s] = (short)m;
+            if (in) m2s[m] = (short)s;
+        }
+        __syncthreads();
+        if (lane == 0) nmb = m0 + __popcll(b);
+        __syncthreads();
+    }
+    const int nm = nmb;
+    // 2-body: direct, slot s gets -f dr, the owner +f dr
+    double F2[CHX_MAX_NB / 64][3];
 #pragma unroll
-    for (int q = 0; q < 5; q++) F[q][0] = F[q][1] = F[q][2] = 0.0;
+    for (int q = 0; q < CHX_MAX_NB / 64; q++) F2[q][0] = F2[q][1] = F2[q][2] = 0.0;
+    double F0[3] = {0.0, 0.0, 0.0};
     double Sv[6] = {0, 0, 0, 0, 0, 0};
     double Eacc = 0.0;
-    const int ncell = nb * (nb + 1) / 2;
-    for (int t0 = 0; t0 < ncell; t0 += RTILE) {
+#pragma unroll
+    for (int q = 0; q < CHX_MAX_NB / 64; q++) {
+        const int s = 1 + lane + 64 * q;
+        if (s > nb) continue;
+        const double f = ra.rec2F[off + s - 1];
+        Eacc += ra.rec2E[off + s - 1];
+        if (f == 0.0) continue;
+        double dr[3];
+        pair_vec(C.h, pos[0], pos[s], dr);
+        F0[0] += f * dr[0]; F0[1] += f * dr[1]; F0[2] += f * dr[2];
+        F2[q][0] = -f * dr[0]; F2[q][1] = -f * dr[1]; F2[q][2] = -f * dr[2];
+        Sv[0] -= f * (dr[0] * dr[0]);
+        Sv[1] -= f * (dr[0] * dr[1]);
+        Sv[2] -= f * (dr[0] * dr[2]);
+        Sv[3] -= f * (dr[1] * dr[1]);
+        Sv[4] -= f * (dr[1] * dr[2]);
+        Sv[5] -= f * (dr[2] * dr[2]);
+    }
+    // many-body: pair cells over the many-body slot set
+    double Fm[(CHX_MAX_N3 + 64) / 64][3];
+#pragma unroll
+    for (int q = 0; q < (CHX_MAX_N3 + 64) / 64; q++) Fm[q][0] = Fm[q][1] = Fm[q][2] = 0.0;
+    const int ncell = nm * (nm - 1) / 2;
+    const bool has_mb = (ra.off3 && M.ntrip) || (ra.off4 && M.nquad);
+    for (int t0 = 0; has_mb && t0 < ncell; t0 += RTILE) {
         for (int c = lane; c < RTILE; c += 64) cells[c] = 0.0;
         __syncthreads();
-        // 2-body: cell (0, s)
-        for (int cb = 0; cb < nb; cb += 64) {
-            const int s = cb + lane;
-            if (s < nb) {
-                cidx[lane] = tri(0, s + 1);
-                cval[lane] = ra.rec2F[off + s];
-                if (t0 == 0) Eacc += ra.rec2E[off + s];
-            } else {
-                cidx[lane] = -1;
-                cval[lane] = 0.0;
-            }
-            __syncthreads();
-            scan_chunk(cidx, cval, 64, t0, lane, cells);
-            __syncthreads();
-        }
-        // 3-body groups of the owner's model
         if (ra.off3) {
             for (int g = M.g3; g < M.g3 + M.ntrip; g++) {
                 const long b = ra.off3[g * N + i], e = ra.off3[g * N + i + 1];
@@ -299,7 +330,7 @@ __global__ __launch_bounds__(64) void k_reduce(int natoms, const chx_cfg *__rest
                     const long c = cb + lane;
                     if (c < e) {
                         const int m = ra.rec3M[c];
-                        const int s0 = m & 1023, s1 = (m >> 10) & 1023, s2 = (m >> 20) & 1023;
+                        const int s0 = s2m[m & 1023], s1 = s2m[(m >> 10) & 1023], s2 = s2m[(m >> 20) & 1023];
                         cidx[3 * lane] = tri(s0, s1);
                         cidx[3 * lane + 1] = tri(s0, s2);
                         cidx[3 * lane + 2] = tri(s1, s2);
@@ -323,8 +354,8 @@ __global__ __launch_bounds__(64) void k_reduce(int natoms, const chx_cfg *__rest
                     const long c = cb + lane;
                     if (c < e) {
                         const long long m = ra.rec4M[c];
-                        const int s[4] = {(int)(m & 1023), (int)((m >> 10) & 1023), (int)((m >> 20) & 1023),
-                                          (int)((m >> 30) & 1023)};
+                        const int s[4] = {s2m[m & 1023], s2m[(m >> 10) & 1023], s2m[(m >> 20) & 1023],
+                                          s2m[(m >> 30) & 1023]};
                         cidx[6 * lane] = tri(s[0], s[1]);
                         cidx[6 * lane + 1] = tri(s[0], s[2]);
                         cidx[6 * lane + 2] = tri(s[0], s[3]);
@@ -344,23 +375,24 @@ __global__ __launch_bounds__(64) void k_reduce(int natoms, const chx_cfg *__rest
                 }
             }
         }
-        // forces and virial from this tile of cells
+        // forces on the many-body slots and the virial from this tile
 #pragma unroll
-        for (int q = 0; q < 5; q++) {
-            const int s = lane + 64 * q;
-            if (s > nb) continue;
-            for (int t = 0; t <= nb; t++) {
-                if (t == s) continue;
-                const int c = tri(s, t) - t0;
+        for (int q = 0; q < (CHX_MAX_N3 + 64) / 64; q++) {
+            const int a = lane + 64 * q;
+            if (a >= nm) continue;
+            const int sa = m2s[a];
+            for (int b = 0; b < nm; b++) {
+                if (b == a) continue;
+                const int c = tri(a, b) - t0;
                 if (c < 0 || c >= RTILE) continue;
                 const double v = cells[c];
                 if (v == 0.0) continue;
                 double dr[3];
-                pair_vec(C.h, pos[s], pos[t], dr);
-                F[q][0] += v * dr[0];
-                F[q][1] += v * dr[1];
-                F[q][2] += v * dr[2];
-                if (s < t) {
+                pair_vec(C.h, pos[sa], pos[m2s[b]], dr);
+                Fm[q][0] += v * dr[0];
+                Fm[q][1] += v * dr[1];
+                Fm[q][2] += v * dr[2];
+                if (a < b) {
                     Sv[0] -= v * (dr[0] * dr[0]);
                     Sv[1] -= v * (dr[0] * dr[1]);
                     Sv[2] -= v * (dr[0] * dr[2]);
@@ -372,23 +404,40 @@ __global__ __launch_bounds__(64) void k_reduce(int natoms, const chx_cfg *__rest
         }
         __syncthreads();
     }
+    // many-body forces to LDS (reusing the cell tile), then per-slot totals
+    double *fm = cells;
+    for (int a = lane; a < 3 * nm && a < RTILE; a += 64) fm[a] = 0.0;
+    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-        const int s = lane + 64 * q;
-        if (s > nb) continue;
-        double *dst = s == 0 ? Fself + 3L * i : Fe + 3L * (off + s - 1);
-        dst[0] = F[q][0]; dst[1] = F[q][1]; dst[2] = F[q][2];
+    for (int q = 0; q < (CHX_MAX_N3 + 64) / 64; q++) {
+        const int a = lane + 64 * q;
+        if (a >= nm || !has_mb) continue;
+        fm[3 * a] = Fm[q][0]; fm[3 * a + 1] = Fm[q][1]; fm[3 * a + 2] = Fm[q][2];
     }
-    // fixed-pattern wave reductions of the energy and the virial
-    double red[7] = {Eacc, Sv[0], Sv[1], Sv[2], Sv[3], Sv[4], Sv[5]};
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CHX_MAX_NB / 64; q++) {
+        const int s = 1 + lane + 64 * q;
+        if (s > nb) continue;
+        const int m = s2m[s];
+        double f[3] = {F2[q][0], F2[q][1], F2[q][2]};
+        if (m > 0) { f[0] += fm[3 * m]; f[1] += fm[3 * m + 1]; f[2] += fm[3 * m + 2]; }
+        double *dst = Fe + 3L * (off + s - 1);
+        dst[0] = f[0]; dst[1] = f[1]; dst[2] = f[2];
+    }
+    // owner force, energy and virial: fixed-pattern wave reductions
+    double red[10] = {Eacc, Sv[0], Sv[1], Sv[2], Sv[3], Sv[4], Sv[5], F0[0], F0[1], F0[2]};
 #pragma unroll
     for (int w = 32; w >= 1; w >>= 1)
 #pragma unroll
-        for (int k = 0; k < 7; k++) red[k] += __shfl_xor(red[k], w, 64);
+        for (int k = 0; k < 10; k++) red[k] += __shfl_xor(red[k], w, 64);
     if (lane == 0) {
         Eown[i] = red[0] + M.eoff[ai.type];
 #pragma unroll
         for (int k = 0; k < 6; k++) Sown[6L * i + k] = red[1 + k];
+        Fself[3L * i] = red[7] + fm[0];
+        Fself[3L * i + 1] = red[8] + fm[1];
+        Fself[3L * i + 2] = red[9] + fm[2];
     }
 }
 

@@ -206,128 +206,161 @@ __global__ void k_nbr(int natoms, const chx_cfg *__restrict__ cfgs, const chx_at
 }
 
 // ---------------------------------------------------------------------------
-// Cluster enumeration, one 64-lane workgroup per owner atom i.
-// MODE 0: sort N2(i) by (parent, shift) in place, count clusters per group.
-// MODE 1: write cluster records (owner, packed slot codes) per group.
-// Predicates (serial_chimes_interface.cpp:749-857, with max 4B cutoff <= max
-// 3B cutoff): j, k in N2(i), j != k, parent(j) <= parent(k), all three
-// distances < rc3; quadruplets add l in N2(i), l not in {j, k},
-// parent(j), parent(k) <= parent(l), all six distances < rc4.
+// Sort each owner's images by (parent, shift) into the second entry buffer,
+// one 64-lane workgroup per owner.  A fixed order makes the cluster lists
+// canonical and every later traversal deterministic.
 // ---------------------------------------------------------------------------
-#define EW (CHX_ENUM_MAX_NB / 32)
+__global__ __launch_bounds__(64) void k_nbr_sort(int natoms, const int *__restrict__ nbr_off,
+                                                 const chx_entry_s *__restrict__ es,
+                                                 const chx_entry_n *__restrict__ en, chx_entry_s *es2,
+                                                 chx_entry_n *en2, int *status)
+{
+    __shared__ long long keys[CHX_MAX_NB];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int off = nbr_off[i];
+    const int nb = nbr_off[i + 1] - off;
+    if (lane == 0) atomicMax(&status[1], nb);
+    if (nb > CHX_MAX_NB) {
+        if (lane == 0) atomicOr(&status[0], 1);
+        return;
+    }
+    for (int j = lane; j < nb; j += 64) keys[j] = ((long long)es[off + j].parent << 24) | en[off + j].shift;
+    __syncthreads();
+    for (int j = lane; j < nb; j += 64) {
+        const long long k = keys[j];
+        int rk = 0;
+        for (int q = 0; q < nb; q++) rk += keys[q] < k;
+        es2[off + rk] = es[off + j];
+        en2[off + rk] = en[off + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cluster enumeration over the sorted images, one 64-lane workgroup per owner.
+// MODE 0 counts clusters per (model, cluster type) group; MODE 1 writes the
+// records (owner, packed slot codes) and flags the images within the 3-body
+// cutoff.  Predicates (serial_chimes_interface.cpp:749-857, with the max 4B
+// cutoff <= the max 3B cutoff): j, k in N2(i), j != k, parent(j) <=
+// parent(k), all three distances < rc3; quadruplets add l in N2(i), l not in
+// {j, k}, parent(j), parent(k) <= parent(l), all six distances < rc4.
+// Candidates are the images within rc3 (S3) of the owner; quadruplets use
+// adjacency bit masks over the images within rc4 (S4).
+// ---------------------------------------------------------------------------
+#define EW4 (CHX_MAX_N4 / 32)
 
 struct EnumShared {
-    double vn[CHX_ENUM_MAX_NB][3];
-    int par[CHX_ENUM_MAX_NB];
-    int typ[CHX_ENUM_MAX_NB];
-    int gst[CHX_ENUM_MAX_NB];
-    unsigned A3[CHX_ENUM_MAX_NB][EW];
-    unsigned A4[CHX_ENUM_MAX_NB][EW];
-    unsigned I3[EW], I4[EW];
-    int cnt[2][CHX_MAX_CLUSTER_TYPES + 1];
+    double vn3[CHX_MAX_N3][3];
+    int idx3[CHX_MAX_N3];
+    int par3[CHX_MAX_N3];
+    int typ3[CHX_MAX_N3];
+    int gst3[CHX_MAX_N3];
+    int idx4[CHX_MAX_N4];      // S4 member -> S3 index
+    int gst4[CHX_MAX_N4];
+    unsigned A4[CHX_MAX_N4][EW4];
+    int cnt[CHX_MAX_CLUSTER_TYPES + 1];
     int run[CHX_MAX_CLUSTER_TYPES + 1];
     int rowcnt[64][CHX_MAX_CLUSTER_TYPES + 1];
-    long long keys[CHX_ENUM_MAX_NB];
+    int n3, n4;
 };
 
 template <int MODE>
 __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restrict__ cfgs,
                                              const chx_atom_s *__restrict__ as, const chx_atom_n *__restrict__ an,
-                                             const int *__restrict__ nbr_off, chx_entry_s *es, chx_entry_n *en,
+                                             const int *__restrict__ nbr_off, const chx_entry_s *__restrict__ es,
+                                             const chx_entry_n *__restrict__ en,
                                              const chx_enum_model *__restrict__ em, int *cnt3, int *cnt4,
                                              const int *__restrict__ off3, const int *__restrict__ off4,
-                                             int2 *list3, int2 *list4, int *status)
+                                             int2 *list3, int2 *list4, unsigned char *e_in3, int *status)
 {
     __shared__ EnumShared sh;
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int off = nbr_off[i];
     const int nb = nbr_off[i + 1] - off;
-    if (MODE == 0 && lane == 0) atomicMax(&status[1], nb);
-    if (nb > CHX_ENUM_MAX_NB) {
-        if (lane == 0) atomicOr(&status[0], 1);
-        return;
-    }
     const chx_atom_s ai = as[i];
     const chx_atom_n ni = an[i];
     const chx_cfg &C = cfgs[ai.cfg];
     const chx_enum_model &M = em[C.model];
-    const int N = natoms;
-    if (MODE == 0) {
-        // rank sort of the entries by (parent, shift)
-        long long key[CHX_ENUM_MAX_NB / 64];
-        chx_entry_s ps[CHX_ENUM_MAX_NB / 64];
-        chx_entry_n pn[CHX_ENUM_MAX_NB / 64];
-        int rank[CHX_ENUM_MAX_NB / 64];
-        for (int j = lane, q = 0; q < CHX_ENUM_MAX_NB / 64; j += 64, q++) {
-            if (j < nb) {
-                ps[q] = es[off + j];
-                pn[q] = en[off + j];
-                key[q] = ((long long)ps[q].parent << 24) | pn[q].shift;
-                sh.gst[j] = 0;
+    const long N = natoms;
+    const double rc3 = C.rc3, rc4 = C.rc4;
+    const bool any3 = M.has3 && rc3 > 0.0, any4 = M.has4 && rc4 > 0.0;
+    const double rcS = rc3 > rc4 ? rc3 : rc4;  // many-body candidate radius
+    if (!any3 && !any4) {
+        if (MODE == 1)
+            for (int j = lane; j < nb; j += 64) e_in3[off + j] = 0;
+        return;
+    }
+    if (lane == 0) { sh.n3 = 0; sh.n4 = 0; }
+    __syncthreads();
+    // compact the images within rc3 / rc4, in sorted order
+    for (int base = 0; base < nb; base += 64) {
+        const int j = base + lane;
+        double d = 1e300;
+        chx_entry_n e;
+        if (j < nb) {
+            e = en[off + j];
+            d = frac_dist(C.hn, ni.u, e.v);
+        }
+        const bool in3 = j < nb && d < rcS, in4 = j < nb && d < rc4;
+        if (MODE == 1 && j < nb) e_in3[off + j] = in3 ? 1 : 0;
+        const unsigned long long b3 = __ballot(in3), b4 = __ballot(in4);
+        const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const int n3b = sh.n3, n4b = sh.n4;
+        if (in3) {
+            const int a = n3b + __popcll(b3 & below);
+            if (a < CHX_MAX_N3) {
+                sh.vn3[a][0] = e.v[0]; sh.vn3[a][1] = e.v[1]; sh.vn3[a][2] = e.v[2];
+                sh.idx3[a] = j;
+                const chx_entry_s f = es[off + j];
+                sh.par3[a] = f.parent;
+                sh.typ3[a] = f.type;
+                if (in4) {
+                    const int u = n4b + __popcll(b4 & below);
+                    if (u < CHX_MAX_N4) sh.idx4[u] = a;
+                }
             }
         }
-        for (int j = lane, q = 0; q < CHX_ENUM_MAX_NB / 64; j += 64, q++)
-            if (j < nb) sh.keys[j] = key[q];
         __syncthreads();
-        for (int j = lane, q = 0; q < CHX_ENUM_MAX_NB / 64; j += 64, q++) {
-            if (j >= nb) continue;
-            int rk = 0;
-            for (int k = 0; k < nb; k++) rk += sh.keys[k] < key[q];
-            rank[q] = rk;
-        }
-        __syncthreads();
-        for (int j = lane, q = 0; q < CHX_ENUM_MAX_NB / 64; j += 64, q++) {
-            if (j >= nb) continue;
-            const int r = rank[q];
-            es[off + r] = ps[q];
-            en[off + r] = pn[q];
-            sh.vn[r][0] = pn[q].v[0]; sh.vn[r][1] = pn[q].v[1]; sh.vn[r][2] = pn[q].v[2];
-            sh.par[r] = ps[q].parent;
-            sh.typ[r] = ps[q].type;
-        }
+        if (lane == 0) { sh.n3 = n3b + __popcll(b3); sh.n4 = n4b + __popcll(b4); }
         __syncthreads();
     }
-    const bool any3 = M.has3 && C.rc3 > 0.0, any4 = M.has4 && C.rc4 > 0.0;
-    if (!any3 && !any4) return;  // counts were zero-filled by the host
-    for (int j = lane; j < nb; j += 64) {
-        if (MODE == 1) {  // MODE 0 holds the sorted entries in LDS already
-            const chx_entry_n e = en[off + j];
-            sh.vn[j][0] = e.v[0]; sh.vn[j][1] = e.v[1]; sh.vn[j][2] = e.v[2];
-            const chx_entry_s f = es[off + j];
-            sh.par[j] = f.parent;
-            sh.typ[j] = f.type;
-        }
-        for (int w = 0; w < EW; w++) { sh.A3[j][w] = 0u; sh.A4[j][w] = 0u; }
+    const int n3 = sh.n3, n4 = sh.n4;
+    if (n3 > CHX_MAX_N3 || n4 > CHX_MAX_N4) {
+        if (lane == 0) atomicOr(&status[0], 2);
+        return;
     }
-    for (int w = lane; w < EW; w += 64) { sh.I3[w] = 0u; sh.I4[w] = 0u; }
-    for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) { sh.cnt[0][g] = 0; sh.cnt[1][g] = 0; sh.run[g] = 0; }
+    for (int a = lane; a < n3; a += 64) {
+        int g = a;
+        while (g > 0 && sh.par3[g - 1] == sh.par3[a]) g--;
+        sh.gst3[a] = g;
+    }
+    for (int u = lane; u < n4; u += 64) {
+        int g = u;
+        const int p = sh.par3[sh.idx4[u]];
+        while (g > 0 && sh.par3[sh.idx4[g - 1]] == p) g--;
+        sh.gst4[u] = g;
+        for (int w = 0; w < EW4; w++) sh.A4[u][w] = 0u;
+    }
+    for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) { sh.cnt[g] = 0; sh.run[g] = 0; }
     __syncthreads();
-    const double rc3 = C.rc3, rc4 = C.rc4;
-    for (int j = lane; j < nb; j += 64) {
-        const double d = frac_dist(C.hn, ni.u, sh.vn[j]);
-        if (d < rc3) atomicOr(&sh.I3[j >> 5], 1u << (j & 31));
-        if (d < rc4) atomicOr(&sh.I4[j >> 5], 1u << (j & 31));
-        int g = j;
-        while (g > 0 && sh.par[g - 1] == sh.par[j]) g--;
-        sh.gst[j] = g;
-    }
-    const int npair = nb * (nb - 1) / 2;
-    for (int p = lane; p < npair; p += 64) {
-        // decode p -> (j < k)
-        int k = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
-        while (k * (k - 1) / 2 > p) k--;
-        while ((k + 1) * k / 2 <= p) k++;
-        const int j = p - k * (k - 1) / 2;
-        const double d = frac_dist(C.hn, sh.vn[j], sh.vn[k]);
-        if (d < rc3) { atomicOr(&sh.A3[j][k >> 5], 1u << (k & 31)); atomicOr(&sh.A3[k][j >> 5], 1u << (j & 31)); }
-        if (d < rc4) { atomicOr(&sh.A4[j][k >> 5], 1u << (k & 31)); atomicOr(&sh.A4[k][j >> 5], 1u << (j & 31)); }
+    if (any4) {
+        const int npair = n4 * (n4 - 1) / 2;
+        for (int p = lane; p < npair; p += 64) {
+            int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
+            while (v * (v - 1) / 2 > p) v--;
+            while ((v + 1) * v / 2 <= p) v++;
+            const int u = p - v * (v - 1) / 2;
+            if (frac_dist(C.hn, sh.vn3[sh.idx4[u]], sh.vn3[sh.idx4[v]]) < rc4) {
+                atomicOr(&sh.A4[u][v >> 5], 1u << (v & 31));
+                atomicOr(&sh.A4[v][u >> 5], 1u << (u & 31));
+            }
+        }
     }
     __syncthreads();
-    const int W = (nb + 31) >> 5;
+    const int W4 = (n4 + 31) >> 5;
     const int nt = M.nt;
     const int ti = ai.type;
-    // body 3 then body 4
     for (int body = 3; body <= 4; body++) {
         if ((body == 3 && !any3) || (body == 4 && !any4)) continue;
         const int ngl = body == 3 ? M.ntrip + 1 : M.nquad + 1;
@@ -336,12 +369,11 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
         int *cntv = body == 3 ? cnt3 : cnt4;
         const int *offv = body == 3 ? off3 : off4;
         int2 *list = body == 3 ? list3 : list4;
-        for (int round = 0; round * 64 < nb; round++) {
-            const int j = round * 64 + lane;
-            if (MODE == 1) {
+        const int nrow = body == 3 ? n3 : n4;
+        for (int round = 0; round * 64 < nrow; round++) {
+            const int r = round * 64 + lane;
+            if (MODE == 1)
                 for (int g = 0; g < ngl; g++) sh.rowcnt[lane][g] = 0;
-            }
-            // pass A: count per local group; pass B (MODE 1): write
             for (int pass = 0; pass < (MODE == 1 ? 2 : 1); pass++) {
                 if (MODE == 1 && pass == 1) {
                     __syncthreads();
@@ -356,58 +388,56 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
                     }
                     __syncthreads();
                 }
-                if (j >= nb) continue;
-                const int gj = sh.gst[j];
-                const int tj = sh.typ[j];
+                if (r >= nrow) continue;
                 if (body == 3) {
-                    if (!((sh.I3[j >> 5] >> (j & 31)) & 1u)) continue;
-                    for (int w = 0; w < W; w++) {
-                        unsigned m = sh.A3[j][w] & sh.I3[w];
-                        const int lo = w * 32;
-                        if (gj > lo) m &= (gj - lo >= 32) ? 0u : (~0u << (gj - lo));
-                        if (j >= lo && j < lo + 32) m &= ~(1u << (j - lo));
-                        while (m) {
-                            const int b = __ffs(m) - 1;
-                            m &= m - 1;
-                            const int k = lo + b;
-                            const int tup = (ti * nt + tj) * nt + sh.typ[k];
-                            int g = cmap[tup];
-                            if (g < 0) g = ngl - 1;
-                            if (MODE == 0) atomicAdd(&sh.cnt[0][g], 1);
-                            else if (pass == 0) sh.rowcnt[lane][g]++;
-                            else {
-                                const long pos = offv[(long)(gbase + g) * N + i] + sh.rowcnt[lane][g]++;
-                                list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10));
-                            }
+                    const int a = r;
+                    const int j = sh.idx3[a];
+                    const int tj = sh.typ3[a];
+                    for (int b = sh.gst3[a]; b < n3; b++) {
+                        if (b == a) continue;
+                        if (!(frac_dist(C.hn, sh.vn3[a], sh.vn3[b]) < rc3)) continue;
+                        const int k = sh.idx3[b];
+                        int g = cmap[(ti * nt + tj) * nt + sh.typ3[b]];
+                        if (g < 0) g = ngl - 1;
+                        if (MODE == 0) atomicAdd(&sh.cnt[g], 1);
+                        else if (pass == 0) sh.rowcnt[lane][g]++;
+                        else {
+                            const long pos = offv[(long)(gbase + g) * N + i] + sh.rowcnt[lane][g]++;
+                            list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10));
                         }
                     }
                 } else {
-                    if (!((sh.I4[j >> 5] >> (j & 31)) & 1u)) continue;
-                    for (int w = 0; w < W; w++) {
-                        unsigned mk = sh.A4[j][w] & sh.I4[w];
+                    const int u = r;
+                    const int au = sh.idx4[u];
+                    const int j = sh.idx3[au];
+                    const int tj = sh.typ3[au];
+                    const int gu = sh.gst4[u];
+                    for (int w = 0; w < W4; w++) {
+                        unsigned mk = sh.A4[u][w];
                         const int lo = w * 32;
-                        if (gj > lo) mk &= (gj - lo >= 32) ? 0u : (~0u << (gj - lo));
-                        if (j >= lo && j < lo + 32) mk &= ~(1u << (j - lo));
+                        if (gu > lo) mk &= (gu - lo >= 32) ? 0u : (~0u << (gu - lo));
+                        if (u >= lo && u < lo + 32) mk &= ~(1u << (u - lo));
                         while (mk) {
-                            const int bk = __ffs(mk) - 1;
+                            const int v = lo + __ffs(mk) - 1;
                             mk &= mk - 1;
-                            const int k = lo + bk;
-                            const int gk = sh.gst[k];
-                            const int tk = sh.typ[k];
-                            for (int v = 0; v < W; v++) {
-                                unsigned ml = sh.A4[j][v] & sh.A4[k][v] & sh.I4[v];
-                                const int lo2 = v * 32;
-                                if (gk > lo2) ml &= (gk - lo2 >= 32) ? 0u : (~0u << (gk - lo2));
-                                if (j >= lo2 && j < lo2 + 32) ml &= ~(1u << (j - lo2));
-                                if (k >= lo2 && k < lo2 + 32) ml &= ~(1u << (k - lo2));
+                            const int av = sh.idx4[v];
+                            const int k = sh.idx3[av];
+                            const int tk = sh.typ3[av];
+                            const int gv = sh.gst4[v];
+                            for (int x = 0; x < W4; x++) {
+                                unsigned ml = sh.A4[u][x] & sh.A4[v][x];
+                                const int lo2 = x * 32;
+                                if (gv > lo2) ml &= (gv - lo2 >= 32) ? 0u : (~0u << (gv - lo2));
+                                if (u >= lo2 && u < lo2 + 32) ml &= ~(1u << (u - lo2));
+                                if (v >= lo2 && v < lo2 + 32) ml &= ~(1u << (v - lo2));
                                 while (ml) {
-                                    const int bl = __ffs(ml) - 1;
+                                    const int wl = lo2 + __ffs(ml) - 1;
                                     ml &= ml - 1;
-                                    const int l = lo2 + bl;
-                                    const int tup = ((ti * nt + tj) * nt + tk) * nt + sh.typ[l];
-                                    int g = cmap[tup];
+                                    const int aw = sh.idx4[wl];
+                                    const int l = sh.idx3[aw];
+                                    int g = cmap[((ti * nt + tj) * nt + tk) * nt + sh.typ3[aw]];
                                     if (g < 0) g = ngl - 1;
-                                    if (MODE == 0) atomicAdd(&sh.cnt[0][g], 1);
+                                    if (MODE == 0) atomicAdd(&sh.cnt[g], 1);
                                     else if (pass == 0) sh.rowcnt[lane][g]++;
                                     else {
                                         const long pos = offv[(long)(gbase + g) * N + i] + sh.rowcnt[lane][g]++;
@@ -421,17 +451,15 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
             }
             __syncthreads();
         }
-        if (MODE == 0) {
-            __syncthreads();
+        __syncthreads();
+        if (MODE == 0)
             for (int g = lane; g < ngl; g += 64) {
-                cntv[(long)(gbase + g) * N + i] = sh.cnt[0][g];
-                sh.cnt[0][g] = 0;
+                cntv[(long)(gbase + g) * N + i] = sh.cnt[g];
+                sh.cnt[g] = 0;
             }
-            __syncthreads();
-        } else {
+        else
             for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) sh.run[g] = 0;
-            __syncthreads();
-        }
+        __syncthreads();
     }
 }
 
@@ -440,11 +468,11 @@ template __global__ void k_nbr<0>(int, const chx_cfg *, const chx_atom_s *, cons
 template __global__ void k_nbr<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
                                   const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *);
 template __global__ void k_enum<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                   chx_entry_s *, chx_entry_n *, const chx_enum_model *, int *, int *,
-                                   const int *, const int *, int2 *, int2 *, int *);
+                                   const chx_entry_s *, const chx_entry_n *, const chx_enum_model *, int *, int *,
+                                   const int *, const int *, int2 *, int2 *, unsigned char *, int *);
 template __global__ void k_enum<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                   chx_entry_s *, chx_entry_n *, const chx_enum_model *, int *, int *,
-                                   const int *, const int *, int2 *, int2 *, int *);
+                                   const chx_entry_s *, const chx_entry_n *, const chx_enum_model *, int *, int *,
+                                   const int *, const int *, int2 *, int2 *, unsigned char *, int *);
 
 // Reverse neighbour lists: entries grouped by parent atom.
 __global__ void k_rev_count(long nent, const chx_entry_s *__restrict__ es, int *rcnt)

@@ -3,7 +3,8 @@
 Golden values (tests/golden/*.json) were produced by the compiled reference
 (tests/golden/make_golden.py).  FP64 energy, forces and stress must agree to
 1e-10 relative (forces and stress relative to the largest component of the
-quantity); cluster lists must be identical (sorted canonical keys, compared
+quantity, plus 1e-9 kcal/mol/A absolute for forces that vanish by symmetry);
+cluster lists must be identical (sorted canonical keys, compared
 in full for C0 and by SHA-256 for the larger systems).
 """
 import hashlib
@@ -18,6 +19,7 @@ from oracle_bindings import read_xyz, sort_keys
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
+FATOL = 1e-9  # kcal/mol/A: forces that cancel by symmetry (perfect crystals) are summation noise
 
 
 def sha(keys):
@@ -35,7 +37,7 @@ def check_lists(model, rec, conf=0):
 def check_efs(tag, E, F, S, rec):
     e0, f0, s0 = golden_arrays(rec)
     assert abs(E - e0) <= RTOL * abs(e0), "%s: energy %.17g vs %.17g" % (tag, E, e0)
-    assert_close(tag + " forces", F, f0, RTOL)
+    assert_close(tag + " forces", F, f0, RTOL, atol=FATOL)
     assert_close(tag + " stress", S, s0, RTOL)
 
 
