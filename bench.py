@@ -1,0 +1,195 @@
+"""Benchmark: atom-evaluations per second of the ChIMES evaluation path.
+
+Default workload: the plain FP64 single-system case, 1536 O/H atoms (512
+H2O-like molecules, L = 24.8 A), 2B+3B+4B at orders 12/8/4, rc 5.0/4.5/4.0 A.
+One step = one full evaluation (ghost images, cluster lists, 2/3/4-body
+energies, forces and stress) with inputs already resident in GPU memory.
+
+Prints one JSON line with the metric, a roofline entry for the dominant
+kernel (the 4-body evaluation, algorithmic FLOPs by the term model) and the
+CPU baseline (the oracle's C restatement on a sample of the same workload).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12   # MI355X dense FP64 (vector and matrix), FLOP/s
+PEAK_FP32 = 157.3e12  # MI355X FP32, FLOP/s
+
+
+def term_flops(nterms, order, npairs):
+    """Algorithmic FLOPs of one cluster (term model): one FMA per listed
+    coefficient for the energy and each pair gradient, plus per-pair setup."""
+    return 2 * nterms * (npairs + 1) + npairs * (6 * order + 20)
+
+
+def param_terms(path):
+    """TOTAL term counts of the 3-body and 4-body blocks of a parameter file."""
+    t3, t4 = [], []
+    cur = None
+    for line in open(path):
+        if "TRIPLETTYPE PARAMS" in line:
+            cur = t3
+        elif "QUADRUPLETYPE PARAMS" in line:
+            cur = t4
+        elif line.startswith("PAIRS:") and cur is not None:
+            cur.append(int(line.split()[-1]))
+            cur = None
+    return t3, t4
+
+
+class Hip:
+    """Minimal device-memory plumbing (hipMalloc / hipMemcpy) for resident inputs."""
+
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.L.hipFree.argtypes = [C.c_void_p]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert self.L.hipMalloc(C.byref(p), max(a.nbytes, 8)) == 0
+        assert self.L.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+        return p
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        assert self.L.hipMalloc(C.byref(p), max(nbytes, 8)) == 0
+        return p
+
+    def download(self, p, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        assert self.L.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0
+        return out
+
+    def sync(self):
+        assert self.L.hipDeviceSynchronize() == 0
+
+
+def cpu_baseline(param_file, cfg, stride=3):
+    """The oracle (C restatement, single thread) on 1/stride of the clusters
+    (owners i with i % stride == 0), scaled to a full evaluation."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bindings import Oracle
+    o = Oracle(param_file)
+    t = time.perf_counter()
+    o.calculate(*cfg, owner_stride=stride)
+    dt = time.perf_counter() - t
+    n = len(cfg[0])
+    return {"value": n / (dt * stride), "unit": "atom-evals/s", "cores": 1, "kind": "port",
+            "sample": "oracle calculate() on the same %d-atom system, clusters of every %d-th owner atom "
+                      "(1/%d of the work, %.1f s; lists built in full), scaled by %d" % (n, stride, stride, dt,
+                                                                                       stride)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every configuration of this workload fits one MI355X and the "
+              "multi-GPU decomposition is not part of this build (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import chimes_calculator_amd as cc
+    from chimes_calculator_amd import workloads as W
+
+    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
+    mname, cfgs = W.workload("C1")
+    pf = W.MODELS[mname].write(os.path.join(ROOT, "_work", "bench_%s.txt" % mname))
+    model = cc.GpuModel(pf)
+    pos, types, cell = cfgs[0]
+    n = len(pos)
+    hip = Hip()
+    d_pos = hip.upload(np.ascontiguousarray(pos, dtype=np.float64).reshape(-1))
+    d_typ = hip.upload(model.types_to_index(types))
+    d_E, d_F, d_V = hip.alloc(8), hip.alloc(8 * 3 * n), hip.alloc(8 * 6)
+    natoms = np.array([n], np.int32)
+    model_of = np.zeros(1, np.int32)
+    cells = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+    prec = cc.FP64 if args.precision == "fp64" else cc.MIXED
+    arr = (C.c_void_p * 1)(model.h)
+    L = cc.lib()
+
+    def step():
+        rc = L.chimes_gpu_eval_batch_device(arr, 1, 1, model_of, natoms, d_pos, d_typ, cells, 0, prec, d_E, d_F,
+                                            d_V)
+        if rc:
+            raise cc.ChimesError(cc._err())
+
+    for _ in range(args.warmup):
+        step()
+    hip.sync()
+    t4, tall = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        tm = model.timing()
+        t4.append(tm[1])
+        tall.append(tm[0])
+    hip.sync()
+    dt = time.perf_counter() - t0
+    counts = model.counts()
+
+    # error of the measured result against the reference's own FP64 values
+    E = hip.download(d_E, (1,), np.float64)[0]
+    F = hip.download(d_F, (n, 3), np.float64)
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "synth_cases.json")))["C1|0"]
+    e0 = float(gold["energy"])
+    f0 = np.array([[float(v) for v in r] for r in gold["forces"]])
+    ev_per_kcal = 0.0433641
+    err = {"energy_eV_per_atom": abs(E - e0) * ev_per_kcal / n,
+           "force_max_eV_per_A": float(np.abs(F - f0).max() * ev_per_kcal),
+           "energy_rel": abs(E - e0) / abs(e0)}
+
+    t3terms, t4terms = param_terms(pf)
+    order4 = model.order(4)
+    flops4 = counts[2] * term_flops(t4terms[0], order4, 6)
+    t4_s = np.mean(t4) * 1e-3
+    achieved = flops4 / t4_s
+    peak = PEAK_FP64 if prec == cc.FP64 else PEAK_FP32
+    out = {
+        "metric": "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency",
+        "value": n * args.steps / dt,
+        "unit": "atom-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == cc.FP64 else "f32-mixed",
+        "data": "synthetic: seeded O/H water-like box and seeded random-coefficient ChIMES model",
+        "config": {"workload": "O/H 1536 atoms (512 H2O-like, L=24.8 A), 2B+3B+4B orders 12/8/4, "
+                               "rc 5.0/4.5/4.0 A, single system, FP64",
+                   "natoms": n, "pairs": counts[0], "triplets": counts[1], "quadruplets": counts[2]},
+        "latency_ms": {"p50": float(np.percentile(tall, 50)), "p99": float(np.percentile(tall, 99))},
+        "error_vs_reference_fp64": err,
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "k_eval4 (4-body evaluation, %d cluster types)" % len(t4terms),
+                     "flops_per_step": flops4, "kernel_ms_per_step": t4_s * 1e3},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pf, cfgs[0])
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -38,7 +38,7 @@ EXPORTS = {
     "chimes_gpu.h": ["chimes_gpu_model_load", "chimes_gpu_model_free", "chimes_gpu_last_error",
                      "chimes_gpu_model_ntypes", "chimes_gpu_model_type_index", "chimes_gpu_model_order",
                      "chimes_gpu_model_max_cutoff", "chimes_gpu_eval_batch", "chimes_gpu_eval_batch_device",
-                     "chimes_gpu_last_timing", "chimes_gpu_compute_clusters", "chimes_gpu_cluster_lists",
+                     "chimes_gpu_last_timing", "chimes_gpu_last_counts", "chimes_gpu_compute_clusters", "chimes_gpu_cluster_lists",
                      "chimes_gpu_free"],
 }
 
@@ -79,6 +79,7 @@ def lib():
                                                    C.c_void_p, _dp, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                                    C.c_void_p]
         L.chimes_gpu_last_timing.argtypes = [C.c_void_p, _dp]
+        L.chimes_gpu_last_counts.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.chimes_gpu_compute_clusters.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp]
         pp = C.POINTER(C.POINTER(C.c_int))
         L.chimes_gpu_cluster_lists.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), pp,
@@ -133,6 +134,12 @@ class GpuModel:
         t = np.zeros(4)
         self.L.chimes_gpu_last_timing(self.h, t)
         return t
+
+    def counts(self):
+        """(pair images, triplets, quadruplets) of the last evaluation."""
+        c = np.zeros(3, np.int64)
+        self.L.chimes_gpu_last_counts(self.h, c)
+        return tuple(int(v) for v in c)
 
     def compute_clusters(self, nb, dist, dr, types):
         """Batched explicit clusters; returns (energy[n], force[n, nb, 3], stress[n, 6])."""

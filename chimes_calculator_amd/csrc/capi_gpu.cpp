@@ -165,6 +165,11 @@ void chimes_gpu_last_timing(const chimes_gpu_model *m, double ms[4])
     chx_last_timing(m->dm->eh->eng, ms);
 }
 
+void chimes_gpu_last_counts(const chimes_gpu_model *m, long long counts[3])
+{
+    chx_last_counts(m->dm->eh->eng, counts);
+}
+
 int chimes_gpu_compute_clusters(chimes_gpu_model *m, int nb, int n, const double *dist, const double *dr,
                                 const int *types, double *energy, double *force, double *stress)
 {

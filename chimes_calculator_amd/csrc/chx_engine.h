@@ -93,6 +93,10 @@ int chx_eval_clusters(chx_engine *e, chx_model *m, int nb, int n, const double *
                       const int *types, double *energy, double *force, double *stress, double *last,
                       int *flags);
 
+// Cluster counts of the last evaluate call: [0] pair entries, [1] triplets,
+// [2] quadruplets (all configurations, excluded types included).
+void chx_last_counts(chx_engine *e, long long *c3);
+
 // Timing of the last evaluate call (HIP events on the engine's stream, ms):
 // [0] whole pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] enumeration.
 void chx_last_timing(chx_engine *e, double *ms4);

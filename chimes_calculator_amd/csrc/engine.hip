@@ -297,6 +297,13 @@ const char *chx_last_error(chx_engine *e) { return e ? e->err.c_str() : "no engi
 
 void chx_free(void *p) { free(p); }
 
+void chx_last_counts(chx_engine *e, long long *c3)
+{
+    c3[0] = e->nent;
+    c3[1] = e->goff3.empty() ? 0 : e->goff3.back();
+    c3[2] = e->goff4.empty() ? 0 : e->goff4.back();
+}
+
 void chx_last_timing(chx_engine *e, double *ms4)
 {
     for (int k = 0; k < 4; k++) ms4[k] = e->timing[k];

@@ -56,6 +56,10 @@ int chimes_gpu_eval_batch_device(chimes_gpu_model *const *models, int nmodels, i
  * [0] pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] lists. */
 void chimes_gpu_last_timing(const chimes_gpu_model *m, double ms[4]);
 
+/* Cluster counts of the last evaluation on the model's device: [0] pair
+ * images, [1] triplets, [2] quadruplets, summed over configurations. */
+void chimes_gpu_last_counts(const chimes_gpu_model *m, long long counts[3]);
+
 /* Batched explicit clusters: n clusters of nb = 2, 3 or 4 atoms given by the
  * nb(nb-1)/2 pair distances dist[n*P], pair vectors dr[n*P*3] (x_b - x_a,
  * pair order 01 02 03 12 13 23) and type indices types[n*nb].  Overwrites
