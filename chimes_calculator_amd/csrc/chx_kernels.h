@@ -24,23 +24,20 @@ __global__ void k_fill_f64(double *p, long n);
 __global__ void k_prep(int natoms, int ncfg, const int *cfg_atom_off, const chx_cfg *cfgs, const double *pos,
                        const int *types, chx_atom_s *as, chx_atom_n *an, int *cell_count);
 __global__ void k_cell_fill(int natoms, const chx_atom_n *an, const int *cell_start, int *cursor, int *cell_atoms);
-__global__ void k_segment_sort(int nseg, const int *start, int *vals);
 template <int MODE>
 __global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
                       const int *cell_start, const int *cell_atoms, int *nbr_cnt, const int *nbr_off,
-                      chx_entry_s *es, chx_entry_n *en, int *e_owner);
-__global__ void k_nbr_sort(int natoms, const int *nbr_off, const chx_entry_s *es, const chx_entry_n *en,
-                           chx_entry_s *es2, chx_entry_n *en2, int *status);
+                      chx_entry_s *es, chx_entry_n *en, int *e_owner, int *status);
+__global__ void k_check_entries(const int *nbr_off, int natoms, long cap, int *status);
+__global__ void k_group_bases(const int *off, long N, int G, long cap, int *base, int *end, int *status, int word,
+                              int retry_bit);
 template <int MODE>
 __global__ void k_enum(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
                        const int *nbr_off, const chx_entry_s *es, const chx_entry_n *en, const chx_enum_model *em,
-                       int *cnt3, int *cnt4, const int *off3, const int *off4, int2 *list3, int2 *list4,
-                       unsigned char *e_in3, int *status);
-__global__ void k_rev_count(long nent, const chx_entry_s *es, int *rcnt);
-__global__ void k_rev_fill(long nent, const chx_entry_s *es, const int *rstart, int *cursor, int *rev);
+                       int *cnt3, int *cnt4, const int *off3, const int *off4, const int *gb3, const int *gb4,
+                       int2 *list3, int2 *list4, int *status, int ngl_max);
 __global__ void k_scan_block(const int *in, int *out, long n, int *bsum);
 __global__ void k_scan_add(int *out, long n, const int *bscan);
 __global__ void k_scan_total(int *out, long n, const int *in);
-__global__ void k_gather_strided(const int *in, long stride, int count, int *out);
 
 }  // namespace chx

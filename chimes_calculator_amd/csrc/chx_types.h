@@ -7,6 +7,30 @@
 #define CHX_MAX_N3 512         // images per owner within the 3-body cutoff
 #define CHX_MAX_N4 256         // images per owner within the 4-body cutoff
 #define CHX_MAX_CLUSTER_TYPES 64
+#define CHX_LIST_ALIGN 64      // 3-/4-body list segments of one cluster type start on this multiple
+#define CHX_MFMA_MAX_D 4       // 4-body dense dimension handled by the matrix-core kernel
+
+// Status words of one evaluation (device int array, read back at the end).
+enum {
+    CHX_ST_FLAGS = 0,          // error / retry bits below
+    CHX_ST_MAXNB = 1,          // largest image count of an owner
+    CHX_ST_NENT = 2,           // neighbour entries needed
+    CHX_ST_N3 = 3,             // triplet list slots needed (aligned)
+    CHX_ST_N4 = 4,             // quadruplet list slots needed (aligned)
+    CHX_ST_C3 = 5,             // triplets listed
+    CHX_ST_C4 = 6,             // quadruplets listed
+    CHX_ST_WORDS = 8
+};
+enum {
+    CHX_BAD_NB = 1,            // an owner has more than CHX_MAX_NB images
+    CHX_BAD_N34 = 2,           // an owner exceeds CHX_MAX_N3 / CHX_MAX_N4
+    CHX_RETRY_ENT = 4,         // entry buffer too small
+    CHX_RETRY_L3 = 8,          // triplet list buffer too small
+    CHX_RETRY_L4 = 16,         // quadruplet list buffer too small
+    CHX_BAD_RANGE = 32,        // a contribution outside the fixed-point range
+    CHX_RETRY_ANY = CHX_RETRY_ENT | CHX_RETRY_L3 | CHX_RETRY_L4,
+    CHX_STOP = CHX_BAD_NB | CHX_BAD_N34 | CHX_RETRY_ANY
+};
 
 // Per-configuration frame data (one per configuration in a batch).
 struct chx_cfg {
@@ -58,6 +82,20 @@ struct chx_entry_n {
 // Per-slot constants of one cluster type (or pair type).
 struct chx_slot {
     double rin, rout, lam, xavg, xdiff;
+};
+
+// One (model, cluster type) group of the 3- or 4-body list.  type < 0 marks
+// the group of excluded type tuples (listed, never evaluated).
+struct chx_group {
+    int model, type, nt, D;
+    int tersoff, pad;
+    double fcut_var;
+    const int *perm;           // canonical member order per type tuple
+    const double *C;           // dense coefficients, parameter pair order
+    const float *Cf;           // FP32 copy
+    const double *Cm;          // 4-body matrix image for the matrix-core kernel (D <= 4)
+    const float *Cmf;          // FP32 copy
+    chx_slot s[6];
 };
 
 // Device parameter tables of one model; all pointers are device pointers.

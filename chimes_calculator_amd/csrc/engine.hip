@@ -3,15 +3,17 @@
 //
 // Pipeline of one batch (all on one stream):
 //   k_prep        replicate, wrap, evaluation and neighbour frames, cell index
-//   cells         count, scan, fill, sort (fixed atom order per cell)
-//   k_nbr         image neighbour lists (count, scan, fill)
-//   k_enum        sort each owner's images, 3-/4-body lists grouped by
-//                 (model, cluster type) so evaluation waves share coefficients
-//   reverse lists entries grouped by parent atom
-//   k_eval2/3/4   per-cluster energies and pair force scalars
-//   k_reduce      owner-ordered pair-cell sums -> forces, virial, energy
-//   gathers       forces per replica and per input atom, energy and virial
-//                 per configuration
+//   cells         count, scan, fill
+//   k_nbr         image neighbour lists (count, scan, fill sorted by
+//                 (parent, shift) per owner)
+//   k_enum        3-/4-body lists in segments of one (model, cluster type),
+//                 aligned so evaluation tiles share coefficients
+//   k_eval2/3/4   cluster energies, forces and virials, accumulated exactly
+//                 in fixed point per atom and per owner
+//   k_fin_*       forces per input atom, energy and virial per configuration
+// Buffers are sized by capacity: a run that finds a capacity short sets a
+// status bit, skips the rest and is repeated with the size it reported, so a
+// run needs one host synchronisation (at the end).
 // Host frame set-up follows serial_chimes_interface.cpp:150-476 operation by
 // operation (this file is built with -ffp-contract=off).
 #include <hip/hip_runtime.h>
@@ -169,9 +171,32 @@ void lattice_lengths(const double *h, double *l)
 // ---------------------------------------------------------------------------
 struct TypeLaunch {
     chx_slot s[6];
-    const double *C = nullptr;   // device, FP64
+    const double *C = nullptr;   // device, FP64 dense coefficients
     const float *Cf = nullptr;   // device, FP32 copy for the mixed path
+    const double *Cm = nullptr;  // device, 4-body matrix image (D <= CHX_MFMA_MAX_D)
+    const float *Cmf = nullptr;
 };
+
+// Matrix image of one 4-body coefficient tensor for the matrix-core kernel:
+// Cm[k][m] with k = (a*D + b)*4 + c (pairs 01, 02, 03) and m = (d*D + e)*4 + f
+// (pairs 12, 13, 23), innermost indices padded to 4, m padded to a multiple
+// of 16, zero padding.
+std::vector<double> quad_matrix_image(const double *C, int D)
+{
+    const int KP = D * D * 4, MP = (D * D * 4 + 15) / 16 * 16;
+    std::vector<double> m((size_t)KP * MP, 0.0);
+    for (int a = 0; a < D; a++)
+        for (int b = 0; b < D; b++)
+            for (int c = 0; c < D; c++)
+                for (int d = 0; d < D; d++)
+                    for (int e = 0; e < D; e++)
+                        for (int f = 0; f < D; f++) {
+                            const size_t src = (((((size_t)a * D + b) * D + c) * D + d) * D + e) * D + f;
+                            const size_t k = (size_t)(a * D + b) * 4 + c, col = (size_t)(d * D + e) * 4 + f;
+                            m[k * MP + col] = C[src];
+                        }
+    return m;
+}
 
 }  // namespace
 
@@ -202,15 +227,16 @@ struct chx_engine {
     hipEvent_t ev[6];
     std::string err;
     double timing[4] = {0, 0, 0, 0};
+    long long counts[3] = {0, 0, 0};
     // batch state
     std::vector<chx_cfg> cfg_h;
     std::vector<int> cfg_atom_off_h, cfg_in_off_h;
     int N = 0, Nin = 0, ncfg = 0, ncell = 0;
-    long nent = 0;
     int G3 = 0, G4 = 0;
     std::vector<int> g3b, g4b;
-    std::vector<int> goff3, goff4;  // group boundaries (G+1)
     std::vector<chx_model *> models;
+    // buffer capacities (entries, list slots); grow on demand
+    long ent_cap = 0, l3_cap = 0, l4_cap = 0;
     // device buffers
     DevBuf<chx_cfg> cfgs;
     DevBuf<int> cfg_atom_off, cfg_in_off;
@@ -220,19 +246,16 @@ struct chx_engine {
     DevBuf<chx_atom_n> an;
     DevBuf<int> cell_cnt, cell_start, cell_cur, cell_atoms;
     DevBuf<int> nbr_cnt, nbr_off, e_owner;
-    DevBuf<chx_entry_s> es, es2;
-    DevBuf<chx_entry_n> en, en2;
-    DevBuf<unsigned char> e_in3;
-    DevBuf<int> cnt3, off3, cnt4, off4, status, gtmp;
+    DevBuf<chx_entry_s> es;
+    DevBuf<chx_entry_n> en;
+    DevBuf<int> cnt3, off3, cnt4, off4, status;
+    DevBuf<int> gb3, ge3, gb4, ge4;
+    DevBuf<chx_group> grp3, grp4;
     DevBuf<int2> list3, list4;
-    DevBuf<int> rcnt, rstart, rcur, rev;
-    DevBuf<double> rec2E, rec2F, rec3E, rec3F, rec4E, rec4F;
-    DevBuf<int> rec3M;
-    DevBuf<long long> rec4M;
-    DevBuf<double> Fe, Fself, Eown, Sown, Frep, Fout, Ecfg, Vcfg;
+    DevBuf<unsigned long long> Facc, Oacc;
+    DevBuf<double> Fout, Ecfg, Vcfg;
     DevBuf<int> cfg_stat;
     DevBuf<chx_enum_model> em;
-    DevBuf<chx_red_model> rm;
     DevBuf<chx_model_dev> mdev;
     std::vector<DevBuf<int>> scan_tmp;
     // per-cluster API buffers
@@ -258,13 +281,6 @@ struct chx_engine {
     void zero_i(int *p, long n, int v = 0)
     {
         if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long>(nblk(n, 256), 4096)), dim3(256), 0, st, p, n, v);
-    }
-    int read_int(const int *p)
-    {
-        int v;
-        HIPCHK(hipMemcpyAsync(&v, p, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        return v;
     }
 };
 
@@ -299,9 +315,7 @@ void chx_free(void *p) { free(p); }
 
 void chx_last_counts(chx_engine *e, long long *c3)
 {
-    c3[0] = e->nent;
-    c3[1] = e->goff3.empty() ? 0 : e->goff3.back();
-    c3[2] = e->goff4.empty() ? 0 : e->goff4.back();
+    for (int k = 0; k < 3; k++) c3[k] = e->counts[k];
 }
 
 void chx_last_timing(chx_engine *e, double *ms4)
@@ -388,6 +402,12 @@ chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
                 for (int q = 0; q < 6; q++) m->t4[t].s[q] = s4[6 * t + q];
                 m->t4[t].C = v.C4 + t * cs;
                 m->t4[t].Cf = Cf + t * cs;
+                if (d->D4 <= CHX_MFMA_MAX_D) {
+                    const std::vector<double> img = quad_matrix_image(d->C4 + t * cs, d->D4);
+                    const std::vector<float> imgf(img.begin(), img.end());
+                    m->t4[t].Cm = m->up(img.data(), img.size());
+                    m->t4[t].Cmf = m->up(imgf.data(), imgf.size());
+                }
             }
             m->quadmap_h.assign(d->quadmap, d->quadmap + nt4);
         }
@@ -400,6 +420,257 @@ chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
 
 void chx_model_destroy(chx_model *m) { delete m; }
 
+}  // extern "C"
+
+namespace {
+
+// Host frame set-up of every configuration (serial_chimes_interface.cpp:150-534).
+void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const chx_batch *b)
+{
+    const int ncfg = b->nconf;
+    e->cfg_h.assign(ncfg, chx_cfg{});
+    e->cfg_atom_off_h.assign(ncfg + 1, 0);
+    e->cfg_in_off_h.assign(ncfg + 1, 0);
+    long N = 0, Nin = 0, ncell = 0;
+    for (int c = 0; c < ncfg; c++) {
+        chx_cfg &C = e->cfg_h[c];
+        const int mi = b->model[c];
+        if (mi < 0 || mi >= nmodels) throw EngineError("model index out of range");
+        const chx_model *M = models[mi];
+        const double rc2 = M->rc[0], rc3 = M->rc[1], rc4 = M->rc[2];
+        const double *ca = b->cells + 9 * c, *cb = ca + 3, *cc = ca + 6;
+        const int nin = b->natoms[c];
+        if (nin <= 0) throw EngineError("configuration without atoms");
+        double la = mag3(ca[0], ca[1], ca[2]), lb = mag3(cb[0], cb[1], cb[2]), lc = mag3(cc[0], cc[1], cc[2]);
+        double mn = la;
+        if (lb < mn) mn = lb;
+        if (lc < mn) mn = lc;
+        C.n_rep = b->small ? (int)(ceil(rc2 / mn) - 1) : 0;
+        if (C.n_rep < 0) C.n_rep = 0;
+        set_hmat(ca, cb, cc, C.h0, C.ih0, 0);
+        set_hmat(ca, cb, cc, C.h, C.ih, C.n_rep);
+        double L[3];
+        lattice_lengths(C.h, L);
+        double A[3], B[3], Cv[3];
+        col(C.h, 0, A); col(C.h, 1, B); col(C.h, 2, Cv);
+        const double al = angle3(B, Cv), be = angle3(Cv, A), ga = angle3(A, B);
+        double vv = 1;
+        vv += 2 * cos(al) * cos(be) * cos(ga);
+        vv -= cos(al) * cos(al);
+        vv -= cos(be) * cos(be);
+        vv -= cos(ga) * cos(ga);
+        C.vol = L[0] * L[1] * L[2] * sqrt(vv);
+        // ghost layers (serial_chimes_interface.cpp:477-534) in both frames
+        double lm = L[0];
+        if (L[1] < lm) lm = L[1];
+        if (L[2] < lm) lm = L[2];
+        const int nl_s = (int)ceil(rc2 * 2.0 / lm + 1);
+        reorient(C.h, C.hn, C.ihn);
+        double Ln[3];
+        lattice_lengths(C.hn, Ln);
+        double lmn = Ln[0];
+        if (Ln[1] < lmn) lmn = Ln[1];
+        if (Ln[2] < lmn) lmn = Ln[2];
+        C.n_layers = (int)ceil(rc2 * 2.0 / lmn + 1);
+        if (C.n_layers != nl_s) throw EngineError("ghost layer counts of the two frames differ");
+        const int nl = C.n_layers;
+        for (int f = 0; f < 2; f++) {
+            const double *LL = f ? Ln : L;
+            const double ex = LL[0] * (2 * nl + 1), ey = LL[1] * (2 * nl + 1), ez = LL[2] * (2 * nl + 1);
+            if (rc2 > 0.5 * ex || rc2 > 0.5 * ey || rc2 > 0.5 * ez ||
+                (M->d.order[1] > 0 && (rc3 > 0.5 * ex || rc3 > 0.5 * ey || rc3 > 0.5 * ez)) ||
+                (M->d.order[2] > 0 && (rc4 > 0.5 * ex || rc4 > 0.5 * ey || rc4 > 0.5 * ez)))
+                throw EngineError("Maximum cutoff is greater than half at least one box length.");
+        }
+        {
+            const double mc[3] = {rc2, rc3, rc4};
+            for (int k = 0; k < 3; k++)
+                if (mc[k] > 2 * L[0] * (2 * nl + 1) || mc[k] > 2 * L[1] * (2 * nl + 1) ||
+                    mc[k] > 2 * L[2] * (2 * nl + 1))
+                    throw EngineError("Layered system is smaller than 2x the model maximum outer cutoff.");
+        }
+        C.search = rc2;
+        if (rc3 > C.search) C.search = rc3;
+        if (rc4 > C.search) C.search = rc4;
+        C.rc3 = M->d.order[1] > 0 ? rc3 : 0.0;
+        C.rc4 = M->d.order[2] > 0 ? rc4 : 0.0;
+        if (C.rc4 > C.rc3 && C.rc3 > 0.0)
+            throw EngineError("4-body cutoff above the 3-body cutoff is not supported");
+        // cell grid over the cell's fractional coordinates
+        double a[3], bb[3], cv[3];
+        col(C.hn, 0, a); col(C.hn, 1, bb); col(C.hn, 2, cv);
+        auto cross = [](const double *x, const double *y, double *z) {
+            z[0] = x[1] * y[2] - x[2] * y[1];
+            z[1] = x[2] * y[0] - x[0] * y[2];
+            z[2] = x[0] * y[1] - x[1] * y[0];
+        };
+        double bc[3], ca2[3], ab[3];
+        cross(bb, cv, bc); cross(cv, a, ca2); cross(a, bb, ab);
+        const double V = fabs(dot3(a, bc));
+        const double w[3] = {V / mag3(bc[0], bc[1], bc[2]), V / mag3(ca2[0], ca2[1], ca2[2]),
+                             V / mag3(ab[0], ab[1], ab[2])};
+        long cells = 1;
+        for (int k = 0; k < 3; k++) {
+            int m = (int)floor(w[k] / (C.search * (1.0 + 1e-6)));
+            if (m < 1) m = 1;
+            if (m > 256) m = 256;
+            const double delta = C.search / w[k] * (1.0 + 1e-8) + 1e-12;
+            int reach = (int)ceil(delta * m);
+            if (reach < 1) reach = 1;
+            C.m[k] = m;
+            C.reach[k] = reach;
+            cells *= m;
+        }
+        C.natoms_in = nin;
+        C.natoms = nin * (C.n_rep + 1) * (C.n_rep + 1) * (C.n_rep + 1);
+        C.atom_off = (int)N;
+        C.atom_in_off = (int)Nin;
+        C.cell_off = (int)ncell;
+        C.model = mi;
+        e->cfg_atom_off_h[c] = (int)N;
+        e->cfg_in_off_h[c] = (int)Nin;
+        N += C.natoms;
+        Nin += nin;
+        ncell += cells;
+    }
+    e->cfg_atom_off_h[ncfg] = (int)N;
+    e->cfg_in_off_h[ncfg] = (int)Nin;
+    if (N >= (1L << 28)) throw EngineError("batch too large");
+    e->N = (int)N;
+    e->Nin = (int)Nin;
+    e->ncell = (int)ncell;
+}
+
+// First capacity guesses from the density of each configuration; a run that
+// finds them too small reports the sizes it needs and is repeated.
+void guess_capacities(chx_engine *e, chx_model *const *models)
+{
+    const double pi = 3.141592653589793;
+    double ent = 0, l3 = 0, l4 = 0;
+    for (const chx_cfg &C : e->cfg_h) {
+        const chx_model *M = models[C.model];
+        const double rho = C.natoms / C.vol;
+        const double nb = 0.5 * rho * 4.0 / 3.0 * pi * C.search * C.search * C.search;
+        ent += C.natoms * (1.3 * nb + 8);
+        if (M->d.order[1] > 0) {
+            const double n3 = 0.5 * rho * 4.0 / 3.0 * pi * C.rc3 * C.rc3 * C.rc3;
+            l3 += C.natoms * (0.4 * n3 * n3 + 4);
+        }
+        if (M->d.order[2] > 0) {
+            const double n4 = 0.5 * rho * 4.0 / 3.0 * pi * C.rc4 * C.rc4 * C.rc4;
+            l4 += C.natoms * (0.08 * n4 * n4 * n4 + 4);
+        }
+    }
+    e->ent_cap = std::max<long>(e->ent_cap, (long)ent + 1024);
+    e->l3_cap = std::max<long>(e->l3_cap, (long)l3 + (long)CHX_LIST_ALIGN * (e->G3 + 1));
+    e->l4_cap = std::max<long>(e->l4_cap, (long)l4 + (long)CHX_LIST_ALIGN * (e->G4 + 1));
+    e->l3_cap = (e->l3_cap + 255) / 256 * 256;
+    e->l4_cap = (e->l4_cap + 255) / 256 * 256;
+}
+
+// One pass of the pipeline on the engine's stream (no host synchronisation).
+void run_pipeline(chx_engine *e, const double *pos_d, const int *typ_d, bool mixed, int ngl_max,
+                  const std::vector<int> &D3s, const std::vector<int> &D4s, double *Fout, double *Ecfg,
+                  double *Vcfg)
+{
+    hipStream_t st = e->st;
+    const long N = e->N, Nin = e->Nin, ncell = e->ncell;
+    const int ncfg = e->ncfg, G3 = e->G3, G4 = e->G4;
+    (void)hipEventRecord(e->ev[0], st);
+    e->zero_i(e->status.p, CHX_ST_WORDS);
+    e->zero_i(e->cell_cnt.p, ncell + 1);
+    e->zero_i(e->cell_cur.p, ncell + 1);
+    hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, ncfg, e->cfg_atom_off.p, e->cfgs.p,
+                       pos_d, typ_d, e->as.p, e->an.p, e->cell_cnt.p);
+    e->scan(e->cell_cnt.p, e->cell_start.p, ncell);
+    hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, e->an.p, e->cell_start.p,
+                       e->cell_cur.p, e->cell_atoms.p);
+    // neighbour images, sorted per owner
+    hipLaunchKernelGGL((k_nbr<0>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
+                       e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr, (chx_entry_s *)nullptr,
+                       (chx_entry_n *)nullptr, (int *)nullptr, e->status.p);
+    e->scan(e->nbr_cnt.p, e->nbr_off.p, N);
+    hipLaunchKernelGGL(k_check_entries, dim3(1), dim3(1), 0, st, e->nbr_off.p, (int)N, e->ent_cap, e->status.p);
+    hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
+                       e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
+                       e->e_owner.p, e->status.p);
+    // 3-/4-body lists in type-aligned segments
+    if (G3 + G4 > 0) {
+        const long n3c = (long)G3 * N + 1, n4c = (long)G4 * N + 1;
+        e->zero_i(e->cnt3.p, n3c);
+        e->zero_i(e->cnt4.p, n4c);
+        hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
+                           e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p, e->cnt4.p, (const int *)nullptr,
+                           (const int *)nullptr, (const int *)nullptr, (const int *)nullptr, (int2 *)nullptr,
+                           (int2 *)nullptr, e->status.p, ngl_max);
+        if (G3) {
+            e->scan(e->cnt3.p, e->off3.p, (long)G3 * N);
+            hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off3.p, N, G3, e->l3_cap, e->gb3.p,
+                               e->ge3.p, e->status.p, (int)CHX_ST_N3, (int)CHX_RETRY_L3);
+        }
+        if (G4) {
+            e->scan(e->cnt4.p, e->off4.p, (long)G4 * N);
+            hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off4.p, N, G4, e->l4_cap, e->gb4.p,
+                               e->ge4.p, e->status.p, (int)CHX_ST_N4, (int)CHX_RETRY_L4);
+        }
+        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), (size_t)64 * ngl_max * sizeof(int), st,
+                           (int)N, e->cfgs.p, e->as.p, e->an.p, e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p,
+                           e->cnt4.p, e->off3.p, e->off4.p, e->gb3.p, e->gb4.p, e->list3.p, e->list4.p,
+                           e->status.p, ngl_max);
+    }
+    (void)hipEventRecord(e->ev[1], st);
+    // evaluation with exact accumulation
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long>(nblk(12 * N, 256), 4096)), dim3(256), 0, st,
+                       (int *)e->Facc.p, 12 * N, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long>(nblk(28 * N, 256), 4096)), dim3(256), 0, st,
+                       (int *)e->Oacc.p, 28 * N, 0);
+    e->zero_i(e->cfg_stat.p, 4 * ncfg);
+    launch_eval2(e->ent_cap, st, e->e_owner.p, e->cfgs.p, e->as.p, e->es.p, e->mdev.p, e->Facc.p, e->Oacc.p,
+                 e->cfg_stat.p, e->status.p);
+    (void)hipEventRecord(e->ev[2], st);
+    chx_list_args la{};
+    la.cfgs = e->cfgs.p;
+    la.as = e->as.p;
+    la.nbr_off = e->nbr_off.p;
+    la.es = e->es.p;
+    la.Facc = e->Facc.p;
+    la.Oacc = e->Oacc.p;
+    la.cfg_stat = e->cfg_stat.p;
+    la.status = e->status.p;
+    if (G3) {
+        chx_list_args l3 = la;
+        l3.cap = e->l3_cap;
+        l3.G = G3;
+        l3.gbase = e->gb3.p;
+        l3.gend = e->ge3.p;
+        l3.groups = e->grp3.p;
+        l3.list = e->list3.p;
+        for (int D : D3s) launch_evalv(3, D, mixed, l3, st);
+    }
+    (void)hipEventRecord(e->ev[3], st);
+    if (G4) {
+        chx_list_args l4 = la;
+        l4.cap = e->l4_cap;
+        l4.G = G4;
+        l4.gbase = e->gb4.p;
+        l4.gend = e->ge4.p;
+        l4.groups = e->grp4.p;
+        l4.list = e->list4.p;
+        for (int D : D4s)
+            if (!launch_eval4m(D, mixed, l4, st)) launch_evalv(4, D, mixed, l4, st);
+    }
+    (void)hipEventRecord(e->ev[4], st);
+    hipLaunchKernelGGL(k_fin_forces, dim3(nblk(Nin, 256)), dim3(256), 0, st, (int)Nin, ncfg, e->cfg_in_off.p,
+                       e->cfgs.p, e->Facc.p, Fout);
+    hipLaunchKernelGGL(k_fin_cfg, dim3(ncfg), dim3(256), 0, st, e->cfgs.p, e->as.p, e->mdev.p, e->Oacc.p, Ecfg,
+                       Vcfg, e->status.p);
+    (void)hipEventRecord(e->ev[5], st);
+}
+
+}  // namespace
+
+extern "C" {
+
 int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx_batch *b, chx_out *o)
 {
     try {
@@ -409,125 +680,16 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
         if (ncfg <= 0) throw EngineError("empty batch");
         e->ncfg = ncfg;
         e->models.assign(models, models + nmodels);
-        // ---------------- host frame set-up per configuration --------------
-        e->cfg_h.assign(ncfg, chx_cfg{});
-        e->cfg_atom_off_h.assign(ncfg + 1, 0);
-        e->cfg_in_off_h.assign(ncfg + 1, 0);
-        long N = 0, Nin = 0, ncell = 0;
-        for (int c = 0; c < ncfg; c++) {
-            chx_cfg &C = e->cfg_h[c];
-            const int mi = b->model[c];
-            if (mi < 0 || mi >= nmodels) throw EngineError("model index out of range");
-            const chx_model *M = models[mi];
-            const double rc2 = M->rc[0], rc3 = M->rc[1], rc4 = M->rc[2];
-            const double *ca = b->cells + 9 * c, *cb = ca + 3, *cc = ca + 6;
-            const int nin = b->natoms[c];
-            if (nin <= 0) throw EngineError("configuration without atoms");
-            double la = mag3(ca[0], ca[1], ca[2]), lb = mag3(cb[0], cb[1], cb[2]), lc = mag3(cc[0], cc[1], cc[2]);
-            double mn = la;
-            if (lb < mn) mn = lb;
-            if (lc < mn) mn = lc;
-            C.n_rep = b->small ? (int)(ceil(rc2 / mn) - 1) : 0;
-            if (C.n_rep < 0) C.n_rep = 0;
-            set_hmat(ca, cb, cc, C.h0, C.ih0, 0);
-            set_hmat(ca, cb, cc, C.h, C.ih, C.n_rep);
-            double L[3];
-            lattice_lengths(C.h, L);
-            double A[3], B[3], Cv[3];
-            col(C.h, 0, A); col(C.h, 1, B); col(C.h, 2, Cv);
-            const double al = angle3(B, Cv), be = angle3(Cv, A), ga = angle3(A, B);
-            double vv = 1;
-            vv += 2 * cos(al) * cos(be) * cos(ga);
-            vv -= cos(al) * cos(al);
-            vv -= cos(be) * cos(be);
-            vv -= cos(ga) * cos(ga);
-            C.vol = L[0] * L[1] * L[2] * sqrt(vv);
-            // ghost layers (serial_chimes_interface.cpp:477-534) in both frames
-            double lm = L[0];
-            if (L[1] < lm) lm = L[1];
-            if (L[2] < lm) lm = L[2];
-            const int nl_s = (int)ceil(rc2 * 2.0 / lm + 1);
-            reorient(C.h, C.hn, C.ihn);
-            double Ln[3];
-            lattice_lengths(C.hn, Ln);
-            double lmn = Ln[0];
-            if (Ln[1] < lmn) lmn = Ln[1];
-            if (Ln[2] < lmn) lmn = Ln[2];
-            C.n_layers = (int)ceil(rc2 * 2.0 / lmn + 1);
-            if (C.n_layers != nl_s) throw EngineError("ghost layer counts of the two frames differ");
-            const int nl = C.n_layers;
-            for (int f = 0; f < 2; f++) {
-                const double *LL = f ? Ln : L;
-                const double ex = LL[0] * (2 * nl + 1), ey = LL[1] * (2 * nl + 1), ez = LL[2] * (2 * nl + 1);
-                if (rc2 > 0.5 * ex || rc2 > 0.5 * ey || rc2 > 0.5 * ez ||
-                    (M->d.order[1] > 0 && (rc3 > 0.5 * ex || rc3 > 0.5 * ey || rc3 > 0.5 * ez)) ||
-                    (M->d.order[2] > 0 && (rc4 > 0.5 * ex || rc4 > 0.5 * ey || rc4 > 0.5 * ez)))
-                    throw EngineError("Maximum cutoff is greater than half at least one box length.");
-            }
-            {
-                const double mc[3] = {rc2, rc3, rc4};
-                for (int k = 0; k < 3; k++)
-                    if (mc[k] > 2 * L[0] * (2 * nl + 1) || mc[k] > 2 * L[1] * (2 * nl + 1) ||
-                        mc[k] > 2 * L[2] * (2 * nl + 1))
-                        throw EngineError("Layered system is smaller than 2x the model maximum outer cutoff.");
-            }
-            C.search = rc2;
-            if (rc3 > C.search) C.search = rc3;
-            if (rc4 > C.search) C.search = rc4;
-            C.rc3 = M->d.order[1] > 0 ? rc3 : 0.0;
-            C.rc4 = M->d.order[2] > 0 ? rc4 : 0.0;
-            if (C.rc4 > C.rc3 && C.rc3 > 0.0)
-                throw EngineError("4-body cutoff above the 3-body cutoff is not supported");
-            // cell grid over the cell's fractional coordinates
-            double a[3], bb[3], cv[3];
-            col(C.hn, 0, a); col(C.hn, 1, bb); col(C.hn, 2, cv);
-            auto cross = [](const double *x, const double *y, double *z) {
-                z[0] = x[1] * y[2] - x[2] * y[1];
-                z[1] = x[2] * y[0] - x[0] * y[2];
-                z[2] = x[0] * y[1] - x[1] * y[0];
-            };
-            double bc[3], ca2[3], ab[3];
-            cross(bb, cv, bc); cross(cv, a, ca2); cross(a, bb, ab);
-            const double V = fabs(dot3(a, bc));
-            const double w[3] = {V / mag3(bc[0], bc[1], bc[2]), V / mag3(ca2[0], ca2[1], ca2[2]),
-                                 V / mag3(ab[0], ab[1], ab[2])};
-            long cells = 1;
-            for (int k = 0; k < 3; k++) {
-                int m = (int)floor(w[k] / (C.search * (1.0 + 1e-6)));
-                if (m < 1) m = 1;
-                if (m > 256) m = 256;
-                const double delta = C.search / w[k] * (1.0 + 1e-8) + 1e-12;
-                int reach = (int)ceil(delta * m);
-                if (reach < 1) reach = 1;
-                C.m[k] = m;
-                C.reach[k] = reach;
-                cells *= m;
-            }
-            C.natoms_in = nin;
-            C.natoms = nin * (C.n_rep + 1) * (C.n_rep + 1) * (C.n_rep + 1);
-            C.atom_off = (int)N;
-            C.atom_in_off = (int)Nin;
-            C.cell_off = (int)ncell;
-            C.model = mi;
-            e->cfg_atom_off_h[c] = (int)N;
-            e->cfg_in_off_h[c] = (int)Nin;
-            N += C.natoms;
-            Nin += nin;
-            ncell += cells;
-        }
-        e->cfg_atom_off_h[ncfg] = (int)N;
-        e->cfg_in_off_h[ncfg] = (int)Nin;
-        if (N >= (1L << 30)) throw EngineError("batch too large");
-        e->N = (int)N;
-        e->Nin = (int)Nin;
-        e->ncell = (int)ncell;
-        // group bases per model
+        setup_frames(e, models, nmodels, b);
+        const long N = e->N, Nin = e->Nin, ncell = e->ncell;
+        // ---------------- groups per model ---------------------------------
         e->g3b.assign(nmodels, 0);
         e->g4b.assign(nmodels, 0);
-        int G3 = 0, G4 = 0;
         std::vector<chx_enum_model> emh(nmodels);
-        std::vector<chx_red_model> rmh(nmodels);
         std::vector<chx_model_dev> mdh(nmodels);
+        std::vector<chx_group> g3h, g4h;
+        std::vector<int> D3s, D4s;
+        int ngl_max = 1;
         for (int mi = 0; mi < nmodels; mi++) {
             const chx_model *M = models[mi];
             chx_enum_model &q = emh[mi];
@@ -536,35 +698,64 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
             q.has4 = M->d.order[2] > 0 && M->d.nquad > 0;
             q.ntrip = M->d.ntrip;
             q.nquad = M->d.nquad;
-            q.g3 = G3;
-            q.g4 = G4;
+            q.g3 = (int)g3h.size();
+            q.g4 = (int)g4h.size();
             q.tripmap = M->dev.tripmap;
             q.quadmap = M->dev.quadmap;
-            e->g3b[mi] = G3;
-            e->g4b[mi] = G4;
-            if (q.has3) G3 += M->d.ntrip + 1;
-            if (q.has4) G4 += M->d.nquad + 1;
-            rmh[mi] = chx_red_model{q.g3, q.has3 ? M->d.ntrip : 0, q.g4, q.has4 ? M->d.nquad : 0, M->dev.eoff};
+            e->g3b[mi] = q.g3;
+            e->g4b[mi] = q.g4;
+            for (int body = 3; body <= 4; body++) {
+                const bool has = body == 3 ? q.has3 : q.has4;
+                if (!has) continue;
+                const int nty = body == 3 ? M->d.ntrip : M->d.nquad;
+                const int D = body == 3 ? M->d.D3 : M->d.D4;
+                std::vector<chx_group> &gv = body == 3 ? g3h : g4h;
+                std::vector<int> &Ds = body == 3 ? D3s : D4s;
+                if (std::find(Ds.begin(), Ds.end(), D) == Ds.end()) Ds.push_back(D);
+                ngl_max = std::max(ngl_max, nty + 1);
+                for (int t = 0; t <= nty; t++) {
+                    chx_group g{};
+                    g.model = mi;
+                    g.type = t < nty ? t : -1;
+                    g.nt = M->d.nt;
+                    g.D = D;
+                    g.tersoff = M->d.tersoff;
+                    g.fcut_var = M->d.fcut_var;
+                    g.perm = body == 3 ? M->dev.trip_perm : M->dev.quad_perm;
+                    if (t < nty) {
+                        const TypeLaunch &tl = body == 3 ? M->t3[t] : M->t4[t];
+                        g.C = tl.C;
+                        g.Cf = tl.Cf;
+                        g.Cm = tl.Cm;
+                        g.Cmf = tl.Cmf;
+                        for (int k = 0; k < (body == 3 ? 3 : 6); k++) g.s[k] = tl.s[k];
+                    }
+                    gv.push_back(g);
+                }
+            }
             mdh[mi] = M->dev;
         }
-        e->G3 = G3;
-        e->G4 = G4;
-        // ---------------- uploads ------------------------------------------
-        (void)hipEventRecord(e->ev[0], st);
+        e->G3 = (int)g3h.size();
+        e->G4 = (int)g4h.size();
+        const int G3 = e->G3, G4 = e->G4;
+        guess_capacities(e, models);
+        // ---------------- uploads and fixed-size buffers --------------------
         e->cfgs.ensure(ncfg);
         e->cfg_atom_off.ensure(ncfg + 1);
         e->cfg_in_off.ensure(ncfg + 1);
         e->em.ensure(nmodels);
-        e->rm.ensure(nmodels);
         e->mdev.ensure(nmodels);
+        e->grp3.ensure(G3 + 1);
+        e->grp4.ensure(G4 + 1);
         HIPCHK(hipMemcpyAsync(e->cfgs.p, e->cfg_h.data(), ncfg * sizeof(chx_cfg), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(e->cfg_atom_off.p, e->cfg_atom_off_h.data(), (ncfg + 1) * sizeof(int),
                               hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(e->cfg_in_off.p, e->cfg_in_off_h.data(), (ncfg + 1) * sizeof(int),
                               hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(e->em.p, emh.data(), nmodels * sizeof(chx_enum_model), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(e->rm.p, rmh.data(), nmodels * sizeof(chx_red_model), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(e->mdev.p, mdh.data(), nmodels * sizeof(chx_model_dev), hipMemcpyHostToDevice, st));
+        if (G3) HIPCHK(hipMemcpyAsync(e->grp3.p, g3h.data(), G3 * sizeof(chx_group), hipMemcpyHostToDevice, st));
+        if (G4) HIPCHK(hipMemcpyAsync(e->grp4.p, g4h.data(), G4 * sizeof(chx_group), hipMemcpyHostToDevice, st));
         const double *pos_d;
         const int *typ_d;
         if (b->pos_dev) {
@@ -578,184 +769,67 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
             pos_d = e->pos.p;
             typ_d = e->types.p;
         }
-        // ---------------- atoms and cells ----------------------------------
         e->as.ensure(N);
         e->an.ensure(N);
         e->cell_cnt.ensure(ncell + 1);
         e->cell_start.ensure(ncell + 1);
         e->cell_cur.ensure(ncell + 1);
         e->cell_atoms.ensure(N);
-        e->zero_i(e->cell_cnt.p, ncell + 1);
-        e->zero_i(e->cell_cur.p, ncell + 1);
-        hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, ncfg, e->cfg_atom_off.p, e->cfgs.p,
-                           pos_d, typ_d, e->as.p, e->an.p, e->cell_cnt.p);
-        e->scan(e->cell_cnt.p, e->cell_start.p, ncell);
-        hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, e->an.p, e->cell_start.p,
-                           e->cell_cur.p, e->cell_atoms.p);
-        hipLaunchKernelGGL(k_segment_sort, dim3(nblk(ncell, 256)), dim3(256), 0, st, (int)ncell, e->cell_start.p,
-                           e->cell_atoms.p);
-        // ---------------- neighbour lists ----------------------------------
         e->nbr_cnt.ensure(N + 1);
         e->nbr_off.ensure(N + 1);
-        hipLaunchKernelGGL((k_nbr<0>), dim3(nblk(N, 128)), dim3(128), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
-                           e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr,
-                           (chx_entry_s *)nullptr, (chx_entry_n *)nullptr, (int *)nullptr);
-        e->scan(e->nbr_cnt.p, e->nbr_off.p, N);
-        const long nent = e->read_int(e->nbr_off.p + N);
-        e->nent = nent;
-        e->es.ensure(nent);
-        e->en.ensure(nent);
-        e->e_owner.ensure(nent);
-        hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 128)), dim3(128), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
-                           e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
-                           e->e_owner.p);
-        // canonical (parent, shift) order of every owner's images
-        e->es2.ensure(nent);
-        e->en2.ensure(nent);
-        e->e_in3.ensure(nent);
-        e->status.ensure(4);
-        e->zero_i(e->status.p, 4);
-        hipLaunchKernelGGL(k_nbr_sort, dim3((unsigned)N), dim3(64), 0, st, (int)N, e->nbr_off.p, e->es.p, e->en.p,
-                           e->es2.p, e->en2.p, e->status.p);
-        std::swap(e->es.p, e->es2.p);
-        std::swap(e->es.cap, e->es2.cap);
-        std::swap(e->en.p, e->en2.p);
-        std::swap(e->en.cap, e->en2.cap);
-        {
-            std::vector<int> s0(2);
-            HIPCHK(hipMemcpyAsync(s0.data(), e->status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (s0[0])
-                throw EngineError("an atom has " + std::to_string(s0[1]) + " neighbour images within the search "
-                                  "cutoff; at most " + std::to_string(CHX_MAX_NB) + " are supported");
-        }
-        // ---------------- cluster enumeration ------------------------------
-        const long n3c = (long)G3 * N + 1, n4c = (long)G4 * N + 1;
-        e->cnt3.ensure(n3c);
-        e->off3.ensure(n3c + 1);
-        e->cnt4.ensure(n4c);
-        e->off4.ensure(n4c + 1);
-        e->zero_i(e->cnt3.p, n3c);
-        e->zero_i(e->cnt4.p, n4c);
-        hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
-                           e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p, e->cnt4.p, (const int *)nullptr,
-                           (const int *)nullptr, (int2 *)nullptr, (int2 *)nullptr, (unsigned char *)nullptr,
-                           e->status.p);
-        e->scan(e->cnt3.p, e->off3.p, (long)G3 * N);
-        e->scan(e->cnt4.p, e->off4.p, (long)G4 * N);
-        e->gtmp.ensure(G3 + G4 + 2);
-        hipLaunchKernelGGL(k_gather_strided, dim3(1), dim3(128), 0, st, e->off3.p, (long)N, G3, e->gtmp.p);
-        hipLaunchKernelGGL(k_gather_strided, dim3(1), dim3(128), 0, st, e->off4.p, (long)N, G4, e->gtmp.p + G3 + 1);
-        std::vector<int> gh(G3 + G4 + 2), sth(4);
-        HIPCHK(hipMemcpyAsync(gh.data(), e->gtmp.p, (G3 + G4 + 2) * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(sth.data(), e->status.p, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        if (sth[0])
-            throw EngineError("an atom has more than " + std::to_string(CHX_MAX_N3) + " images within the 3-body "
-                              "cutoff or more than " + std::to_string(CHX_MAX_N4) + " within the 4-body cutoff");
-        e->goff3.assign(gh.begin(), gh.begin() + G3 + 1);
-        e->goff4.assign(gh.begin() + G3 + 1, gh.end());
-        const long n3 = e->goff3[G3], n4 = e->goff4[G4];
-        e->list3.ensure(n3);
-        e->list4.ensure(n4);
-        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
-                           e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p, e->cnt4.p, e->off3.p, e->off4.p,
-                           e->list3.p, e->list4.p, e->e_in3.p, e->status.p);
-        // ---------------- reverse lists ------------------------------------
-        e->rcnt.ensure(N + 1);
-        e->rstart.ensure(N + 1);
-        e->rcur.ensure(N + 1);
-        e->rev.ensure(nent);
-        e->zero_i(e->rcnt.p, N + 1);
-        e->zero_i(e->rcur.p, N + 1);
-        hipLaunchKernelGGL(k_rev_count, dim3(nblk(nent, 256)), dim3(256), 0, st, nent, e->es.p, e->rcnt.p);
-        e->scan(e->rcnt.p, e->rstart.p, N);
-        hipLaunchKernelGGL(k_rev_fill, dim3(nblk(nent, 256)), dim3(256), 0, st, nent, e->es.p, e->rstart.p,
-                           e->rcur.p, e->rev.p);
-        hipLaunchKernelGGL(k_segment_sort, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, e->rstart.p, e->rev.p);
-        (void)hipEventRecord(e->ev[1], st);
-        // ---------------- evaluation ---------------------------------------
-        const bool mixed = b->precision == CHX_MIXED;
+        e->status.ensure(CHX_ST_WORDS);
+        e->cnt3.ensure((long)G3 * N + 1);
+        e->off3.ensure((long)G3 * N + 2);
+        e->cnt4.ensure((long)G4 * N + 1);
+        e->off4.ensure((long)G4 * N + 2);
+        e->gb3.ensure(G3 + 1);
+        e->ge3.ensure(G3 + 1);
+        e->gb4.ensure(G4 + 1);
+        e->ge4.ensure(G4 + 1);
+        e->Facc.ensure(6 * N);
+        e->Oacc.ensure(14 * N);
         e->cfg_stat.ensure(4 * ncfg);
-        e->zero_i(e->cfg_stat.p, 4 * ncfg);
-        e->rec2E.ensure(nent);
-        e->rec2F.ensure(nent);
-        hipLaunchKernelGGL(k_eval2, dim3(nblk(nent, 256)), dim3(256), 0, st, nent, e->e_owner.p, e->cfgs.p, e->as.p,
-                           e->es.p, e->mdev.p, e->rec2E.p, e->rec2F.p, e->cfg_stat.p);
-        e->rec3E.ensure(n3);
-        e->rec3F.ensure(3 * n3);
-        e->rec3M.ensure(n3);
-        e->rec4E.ensure(n4);
-        e->rec4F.ensure(6 * n4);
-        e->rec4M.ensure(n4);
-        (void)hipEventRecord(e->ev[2], st);
-        for (int mi = 0; mi < nmodels; mi++) {
-            const chx_model *M = models[mi];
-            if (!emh[mi].has3) continue;
-            for (int t = 0; t < M->d.ntrip; t++) {
-                const int g = e->g3b[mi] + t;
-                chx_type_args3 ta;
-                ta.nt = M->d.nt;
-                ta.tersoff = M->d.tersoff;
-                ta.fcut_var = M->d.fcut_var;
-                ta.perm = M->dev.trip_perm;
-                for (int q = 0; q < 3; q++) ta.s[q] = M->t3[t].s[q];
-                launch_eval3(M->d.D3, mixed, e->goff3[g], e->goff3[g + 1], st, e->list3.p, e->cfgs.p, e->as.p,
-                             e->nbr_off.p, e->es.p, ta, mixed ? (const void *)M->t3[t].Cf : (const void *)M->t3[t].C,
-                             e->rec3E.p, e->rec3F.p, e->rec3M.p, e->cfg_stat.p);
-            }
-        }
-        (void)hipEventRecord(e->ev[3], st);
-        for (int mi = 0; mi < nmodels; mi++) {
-            const chx_model *M = models[mi];
-            if (!emh[mi].has4) continue;
-            for (int t = 0; t < M->d.nquad; t++) {
-                const int g = e->g4b[mi] + t;
-                chx_type_args4 ta;
-                ta.nt = M->d.nt;
-                ta.tersoff = M->d.tersoff;
-                ta.fcut_var = M->d.fcut_var;
-                ta.perm = M->dev.quad_perm;
-                for (int q = 0; q < 6; q++) ta.s[q] = M->t4[t].s[q];
-                launch_eval4(M->d.D4, mixed, e->goff4[g], e->goff4[g + 1], st, e->list4.p, e->cfgs.p, e->as.p,
-                             e->nbr_off.p, e->es.p, ta, mixed ? (const void *)M->t4[t].Cf : (const void *)M->t4[t].C,
-                             e->rec4E.p, e->rec4F.p, e->rec4M.p, e->cfg_stat.p);
-            }
-        }
-        (void)hipEventRecord(e->ev[4], st);
-        // ---------------- reduction and gathers ----------------------------
-        e->Fe.ensure(3 * nent);
-        e->Fself.ensure(3 * N);
-        e->Eown.ensure(N);
-        e->Sown.ensure(6 * N);
-        e->Frep.ensure(3 * N);
         e->Fout.ensure(3 * Nin);
         e->Ecfg.ensure(ncfg);
         e->Vcfg.ensure(6 * ncfg);
-        chx_red_args ra;
-        ra.models = e->rm.p;
-        ra.rec2E = e->rec2E.p;
-        ra.rec2F = e->rec2F.p;
-        ra.off3 = G3 ? e->off3.p : nullptr;
-        ra.rec3E = e->rec3E.p;
-        ra.rec3F = e->rec3F.p;
-        ra.rec3M = e->rec3M.p;
-        ra.off4 = G4 ? e->off4.p : nullptr;
-        ra.rec4E = e->rec4E.p;
-        ra.rec4F = e->rec4F.p;
-        ra.rec4M = e->rec4M.p;
-        hipLaunchKernelGGL(k_reduce, dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs.p, e->as.p, e->nbr_off.p,
-                           e->es.p, e->e_in3.p, ra, e->Fe.p, e->Fself.p, e->Eown.p, e->Sown.p);
-        hipLaunchKernelGGL(k_gather_rep, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, e->rstart.p, e->rev.p,
-                           e->Fe.p, e->Fself.p, e->Frep.p);
         double *Fout = o->out_dev ? o->forces : e->Fout.p;
         double *Ecfg = o->out_dev ? o->energy : e->Ecfg.p;
         double *Vcfg = o->out_dev ? o->virial : e->Vcfg.p;
-        hipLaunchKernelGGL(k_gather_in, dim3(nblk(Nin, 256)), dim3(256), 0, st, (int)Nin, ncfg, e->cfg_in_off.p,
-                           e->cfgs.p, e->Frep.p, Fout);
-        hipLaunchKernelGGL(k_cfg_sum, dim3(ncfg), dim3(256), 0, st, e->cfgs.p, e->Eown.p, e->Sown.p, Ecfg, Vcfg);
-        (void)hipEventRecord(e->ev[5], st);
+        const bool mixed = b->precision == CHX_MIXED;
+        // ---------------- pipeline, repeated while a capacity is short ------
+        int st_h[CHX_ST_WORDS];
         std::vector<int> stat(4 * ncfg);
+        for (int attempt = 0;; attempt++) {
+            e->es.ensure(e->ent_cap);
+            e->en.ensure(e->ent_cap);
+            e->e_owner.ensure(e->ent_cap);
+            e->list3.ensure(e->l3_cap);
+            e->list4.ensure(e->l4_cap);
+            run_pipeline(e, pos_d, typ_d, mixed, ngl_max, D3s, D4s, Fout, Ecfg, Vcfg);
+            HIPCHK(hipMemcpyAsync(st_h, e->status.p, sizeof(st_h), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(hipGetLastError());
+            const int fl = st_h[CHX_ST_FLAGS];
+            if (fl & CHX_BAD_NB)
+                throw EngineError("an atom has " + std::to_string(st_h[CHX_ST_MAXNB]) + " neighbour images within "
+                                  "the search cutoff; at most " + std::to_string(CHX_MAX_NB) + " are supported");
+            if (fl & CHX_BAD_N34)
+                throw EngineError("an atom has more than " + std::to_string(CHX_MAX_N3) + " images within the "
+                                  "3-body cutoff or more than " + std::to_string(CHX_MAX_N4) + " within the 4-body "
+                                  "cutoff");
+            if (!(fl & CHX_RETRY_ANY)) {
+                if (fl & CHX_BAD_RANGE)
+                    throw EngineError("a cluster energy, force or virial contribution is outside the accumulator "
+                                      "range (|value| >= 2^42)");
+                break;
+            }
+            if (attempt >= 3) throw EngineError("buffer sizing did not converge");
+            if (fl & CHX_RETRY_ENT) e->ent_cap = std::max<long>(e->ent_cap, (long)(st_h[CHX_ST_NENT] * 1.2) + 1024);
+            if (fl & CHX_RETRY_L3) e->l3_cap = std::max<long>(e->l3_cap, (long)(st_h[CHX_ST_N3] * 1.2) + 4096);
+            if (fl & CHX_RETRY_L4) e->l4_cap = std::max<long>(e->l4_cap, (long)(st_h[CHX_ST_N4] * 1.2) + 4096);
+            e->l3_cap = (e->l3_cap + 255) / 256 * 256;
+            e->l4_cap = (e->l4_cap + 255) / 256 * 256;
+        }
         HIPCHK(hipMemcpyAsync(stat.data(), e->cfg_stat.p, 4 * ncfg * sizeof(int), hipMemcpyDeviceToHost, st));
         if (!o->out_dev) {
             HIPCHK(hipMemcpyAsync(o->forces, Fout, 3 * Nin * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -763,7 +837,9 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
             HIPCHK(hipMemcpyAsync(o->virial, Vcfg, 6 * ncfg * sizeof(double), hipMemcpyDeviceToHost, st));
         }
         HIPCHK(hipStreamSynchronize(st));
-        HIPCHK(hipGetLastError());
+        e->counts[0] = st_h[CHX_ST_NENT];
+        e->counts[1] = G3 ? st_h[CHX_ST_C3] : 0;
+        e->counts[2] = G4 ? st_h[CHX_ST_C4] : 0;
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[5]);
         e->timing[0] = ms;
@@ -835,15 +911,19 @@ int chx_export_lists(chx_engine *e, int cfg, long *n2, int **pairs, long *n3, in
                 const int gb = body == 3 ? e->g3b[C.model] : e->g4b[C.model];
                 const int ng = (body == 3 ? M->d.ntrip : M->d.nquad) + 1;
                 const int *offv = body == 3 ? e->off3.p : e->off4.p;
+                const int *gbase = body == 3 ? e->gb3.p : e->gb4.p;
                 const int2 *lst = body == 3 ? e->list3.p : e->list4.p;
                 for (int g = gb; g < gb + ng; g++) {
-                    int r[2];
-                    HIPCHK(hipMemcpy(&r[0], offv + (long)g * N + C.atom_off, sizeof(int), hipMemcpyDeviceToHost));
-                    HIPCHK(hipMemcpy(&r[1], offv + (long)g * N + C.atom_off + C.natoms, sizeof(int),
+                    int r[3], base;
+                    HIPCHK(hipMemcpy(&r[0], offv + (long)g * N, sizeof(int), hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(&r[1], offv + (long)g * N + C.atom_off, sizeof(int), hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(&r[2], offv + (long)g * N + C.atom_off + C.natoms, sizeof(int),
                                      hipMemcpyDeviceToHost));
-                    std::vector<int2> recs(r[1] - r[0]);
+                    HIPCHK(hipMemcpy(&base, gbase + g, sizeof(int), hipMemcpyDeviceToHost));
+                    std::vector<int2> recs(r[2] - r[1]);
                     if (!recs.empty())
-                        HIPCHK(hipMemcpy(recs.data(), lst + r[0], recs.size() * sizeof(int2), hipMemcpyDeviceToHost));
+                        HIPCHK(hipMemcpy(recs.data(), lst + base + (r[1] - r[0]), recs.size() * sizeof(int2),
+                                         hipMemcpyDeviceToHost));
                     for (const int2 &q : recs) {
                         const int il = q.x - C.atom_off;
                         out.push_back(il);

@@ -1,16 +1,33 @@
-// Evaluation kernels: 2-, 3- and 4-body cluster terms, the deterministic
-// owner-ordered reduction of pair force scalars into forces and virial, and
-// the explicit-cluster kernel behind the per-cluster API.
+// Evaluation kernels: 2-, 3- and 4-body cluster terms with exact fixed-point
+// accumulation of forces (per replicated atom) and of energy and virial (per
+// owner atom), the final per-atom / per-configuration totals, and the
+// explicit-cluster kernel behind the per-cluster API.
+//
+// 3-/4-body lists are laid out in segments of one (model, cluster type) group
+// starting on multiples of CHX_LIST_ALIGN, so every 64-cluster tile uses one
+// coefficient table.  The 4-body contraction for dense dimensions D <= 4 runs
+// on the matrix cores: with X = T_01 (x) T_02 (x) T_03 and Y = T_12 (x) T_13 (x)
+// T_23 per cluster, the energy form is S = X^T C Y for the coefficient matrix
+// C[(a,b,c)][(d,e,f)], and the gradients take X with one factor replaced by
+// its derivative (three more products) or Y likewise (elementwise).  One
+// wavefront evaluates 16 clusters per step as Z_v = C^T X_v (v = energy form
+// and the three X gradients) with 16x16x4 MFMA, then folds Z_v against Y.
 //
 // Compiled with -ffp-contract=off: the cluster geometry (dr = h (v_b - v_a))
 // reproduces the reference's get_dist bit for bit; the polynomial and
 // coefficient arithmetic uses explicit fma().
 #include <hip/hip_runtime.h>
 
+#include "chx_dev.h"
 #include "chx_eval.h"
 #include "eval_math.h"
 
+#include <algorithm>
+
 namespace chx {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void mv3(const double *m, const double *d, double *o)
 {
@@ -35,54 +52,87 @@ __device__ __forceinline__ void note_flags(int *cfg_stat, int cfg, int flags)
     if (flags & FLAG_OUT_OF_RANGE) atomicAdd(&cfg_stat[4 * cfg + 2], 1);
 }
 
-// ---------------------------------------------------------------------------
-// 2-body: one thread per neighbour entry.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_eval2(long nent, const int *__restrict__ e_owner,
-                                                const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
-                                                const chx_entry_s *__restrict__ es,
-                                                const chx_model_dev *__restrict__ models, double *rec2E,
-                                                double *rec2F, int *cfg_stat)
+// Group of list slot t: the last group whose aligned base is <= t.
+__device__ __forceinline__ int find_group(const int *__restrict__ gbase, int G, long t)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nent) return;
-    const int i = e_owner[e];
-    const chx_atom_s ai = as[i];
-    const chx_entry_s g = es[e];
-    const chx_cfg &C = cfgs[ai.cfg];
-    const chx_model_dev &M = models[C.model];
-    double dr[3];
-    const double r = pair_vec(C.h, ai.u, g.v, dr);
-    const int pt = M.pairmap[ai.type * M.nt + g.type];
-    double E, fs;
-    int flags = 0;
-    eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var, M.pen_dist,
-              M.pen_scale, E, fs, flags);
-    rec2E[e] = E;
-    rec2F[e] = fs;
-    note_flags(cfg_stat, ai.cfg, flags);
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (gbase[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
 }
 
-// Member positions and types of a cluster record.
+// ---------------------------------------------------------------------------
+// 2-body: one thread per neighbour entry.  F_owner += f dr, F_image -= f dr,
+// virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owner, const chx_cfg *__restrict__ cfgs,
+                                                const chx_atom_s *__restrict__ as,
+                                                const chx_entry_s *__restrict__ es,
+                                                const chx_model_dev *__restrict__ models, unsigned long long *Facc,
+                                                unsigned long long *Oacc, int *cfg_stat, int *status)
+{
+    if (block_stop(status, CHX_STOP)) return;
+    const long nent = status[CHX_ST_NENT];
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int owner = -1;
+    double q[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool bad = false;
+    if (e < nent) {
+        const int i = e_owner[e];
+        const chx_atom_s ai = as[i];
+        const chx_entry_s g = es[e];
+        const chx_cfg &C = cfgs[ai.cfg];
+        const chx_model_dev &M = models[C.model];
+        double dr[3];
+        const double r = pair_vec(C.h, ai.u, g.v, dr);
+        const int pt = M.pairmap[ai.type * M.nt + g.type];
+        double E, fs;
+        int flags = 0;
+        eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
+                  M.pen_dist, M.pen_scale, E, fs, flags);
+        if (fs != 0.0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                fx_atomic(Facc + 2L * (3L * i + k), fs * dr[k], bad);
+                fx_atomic(Facc + 2L * (3L * g.parent + k), -(fs * dr[k]), bad);
+            }
+        }
+        owner = i;
+        q[0] = E;
+        q[1] = -(fs * (dr[0] * dr[0]));
+        q[2] = -(fs * (dr[0] * dr[1]));
+        q[3] = -(fs * (dr[0] * dr[2]));
+        q[4] = -(fs * (dr[1] * dr[1]));
+        q[5] = -(fs * (dr[1] * dr[2]));
+        q[6] = -(fs * (dr[2] * dr[2]));
+        note_flags(cfg_stat, ai.cfg, flags);
+    }
+    fx_owner<7>(Oacc, owner, q, bad);
+    if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
+}
+
+// Member positions, types and replicated atom indices of a cluster record.
 template <int NB>
 __device__ __forceinline__ void load_members(const int2 rec, const chx_atom_s *__restrict__ as,
                                              const int *__restrict__ nbr_off, const chx_entry_s *__restrict__ es,
-                                             double (&p)[NB][3], int (&t)[NB], int (&code)[NB], int &cfg)
+                                             double (&p)[NB][3], int (&t)[NB], int (&par)[NB], int &cfg)
 {
     const int i = rec.x;
     const chx_atom_s ai = as[i];
     cfg = ai.cfg;
     p[0][0] = ai.u[0]; p[0][1] = ai.u[1]; p[0][2] = ai.u[2];
     t[0] = ai.type;
-    code[0] = 0;
+    par[0] = i;
     const int off = nbr_off[i] - 1;
 #pragma unroll
     for (int m = 1; m < NB; m++) {
         const int s = (rec.y >> (10 * (m - 1))) & 1023;
-        code[m] = s;
         const chx_entry_s g = es[off + s];
         p[m][0] = g.v[0]; p[m][1] = g.v[1]; p[m][2] = g.v[2];
         t[m] = g.type;
+        par[m] = g.parent;
     }
 }
 
@@ -90,7 +140,7 @@ __device__ __forceinline__ void load_members(const int2 rec, const chx_atom_s *_
 // runtime member perm[a] (the first matching permutation, chimesFF.h:480-526).
 template <int NB>
 __device__ __forceinline__ void canonical(const int *__restrict__ perm_tab, int tup, double (&p)[NB][3],
-                                          int (&code)[NB])
+                                          int (&par)[NB])
 {
     int pm[NB];
 #pragma unroll
@@ -99,12 +149,12 @@ __device__ __forceinline__ void canonical(const int *__restrict__ perm_tab, int 
     int c[NB];
 #pragma unroll
     for (int a = 0; a < NB; a++) {
-        c[a] = code[0];
+        c[a] = par[0];
         q[a][0] = p[0][0]; q[a][1] = p[0][1]; q[a][2] = p[0][2];
 #pragma unroll
         for (int b = 1; b < NB; b++) {
             const bool s = pm[a] == b;
-            c[a] = s ? code[b] : c[a];
+            c[a] = s ? par[b] : c[a];
             q[a][0] = s ? p[b][0] : q[a][0];
             q[a][1] = s ? p[b][1] : q[a][1];
             q[a][2] = s ? p[b][2] : q[a][2];
@@ -112,353 +162,414 @@ __device__ __forceinline__ void canonical(const int *__restrict__ perm_tab, int 
     }
 #pragma unroll
     for (int a = 0; a < NB; a++) {
-        code[a] = c[a];
+        par[a] = c[a];
         p[a][0] = q[a][0]; p[a][1] = q[a][1]; p[a][2] = q[a][2];
     }
 }
 
-// ---------------------------------------------------------------------------
-// 3-body: one thread per triplet of one cluster type (uniform coefficients).
-// ---------------------------------------------------------------------------
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_eval3(long begin, long end, const int2 *__restrict__ list,
-                                                const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
-                                                const int *__restrict__ nbr_off, const chx_entry_s *__restrict__ es,
-                                                chx_type_args3 ta, const R *__restrict__ Cd, double *recE,
-                                                double *recF, int *recM, int *cfg_stat)
+template <int NB>
+struct pairs_of;
+template <>
+struct pairs_of<3> {
+    static constexpr int P = 3;
+    static constexpr int a[3] = {0, 0, 1};
+    static constexpr int b[3] = {1, 2, 2};
+};
+template <>
+struct pairs_of<4> {
+    static constexpr int P = 6;
+    static constexpr int a[6] = {0, 0, 0, 1, 1, 2};
+    static constexpr int b[6] = {1, 2, 3, 2, 3, 3};
+};
+
+// Cluster geometry in canonical order: members, pair vectors and distances,
+// and whether every pair is inside its cutoff.
+template <int NB>
+__device__ __forceinline__ bool cluster_geometry(const chx_group &grp, const int2 rec,
+                                                 const chx_cfg *__restrict__ cfgs,
+                                                 const chx_atom_s *__restrict__ as, const int *__restrict__ nbr_off,
+                                                 const chx_entry_s *__restrict__ es, double (&p)[NB][3],
+                                                 int (&par)[NB], double (&r)[pairs_of<NB>::P], int &cfg)
 {
-    const long c = begin + (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= end) return;
-    double p[3][3];
-    int t[3], code[3], cfg;
-    load_members<3>(list[c], as, nbr_off, es, p, t, code, cfg);
-    const int tup = (t[0] * ta.nt + t[1]) * ta.nt + t[2];
-    canonical<3>(ta.perm, tup, p, code);
+    using PQ = pairs_of<NB>;
+    int t[NB];
+    load_members<NB>(rec, as, nbr_off, es, p, t, par, cfg);
+    int tup = 0;
+#pragma unroll
+    for (int a = 0; a < NB; a++) tup = tup * grp.nt + t[a];
+    canonical<NB>(grp.perm, tup, p, par);
     const chx_cfg &C = cfgs[cfg];
-    double r[3], dr[3];
-    r[0] = pair_vec(C.h, p[0], p[1], dr);
-    r[1] = pair_vec(C.h, p[0], p[2], dr);
-    r[2] = pair_vec(C.h, p[1], p[2], dr);
-    recM[c] = code[0] | (code[1] << 10) | (code[2] << 20);
     bool in = true;
 #pragma unroll
-    for (int q = 0; q < 3; q++) in = in && (r[q] < ta.s[q].rout);
-    if (!in) {
-        recE[c] = 0.0;
-        recF[3 * c] = 0.0; recF[3 * c + 1] = 0.0; recF[3 * c + 2] = 0.0;
-        return;
-    }
-    R T[3][D], dT[3][D];
-    double f[3], fd[3];
-    int flags = 0;
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        cheby<D, R>(r[q], ta.s[q], T[q], dT[q], flags);
-        fcut(r[q], ta.s[q].rout, ta.tersoff, ta.fcut_var, f[q], fd[q]);
-    }
-    double S, G[3], E, fs[3];
-    contract3<D, R, double>(Cd, T, dT, S, G);
-    cluster_scalars<3, double>(r, f, fd, S, G, E, fs);
-    recE[c] = E;
-    recF[3 * c] = fs[0]; recF[3 * c + 1] = fs[1]; recF[3 * c + 2] = fs[2];
-    note_flags(cfg_stat, cfg, flags);
-}
-
-// ---------------------------------------------------------------------------
-// 4-body: one thread per quadruplet of one cluster type.
-// ---------------------------------------------------------------------------
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_eval4(long begin, long end, const int2 *__restrict__ list,
-                                                const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
-                                                const int *__restrict__ nbr_off, const chx_entry_s *__restrict__ es,
-                                                chx_type_args4 ta, const R *__restrict__ Cd, double *recE,
-                                                double *recF, long long *recM, int *cfg_stat)
-{
-    const long c = begin + (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= end) return;
-    double p[4][3];
-    int t[4], code[4], cfg;
-    load_members<4>(list[c], as, nbr_off, es, p, t, code, cfg);
-    const int tup = ((t[0] * ta.nt + t[1]) * ta.nt + t[2]) * ta.nt + t[3];
-    canonical<4>(ta.perm, tup, p, code);
-    const chx_cfg &C = cfgs[cfg];
-    double r[6], dr[3];
-    r[0] = pair_vec(C.h, p[0], p[1], dr);
-    r[1] = pair_vec(C.h, p[0], p[2], dr);
-    r[2] = pair_vec(C.h, p[0], p[3], dr);
-    r[3] = pair_vec(C.h, p[1], p[2], dr);
-    r[4] = pair_vec(C.h, p[1], p[3], dr);
-    r[5] = pair_vec(C.h, p[2], p[3], dr);
-    recM[c] = (long long)code[0] | ((long long)code[1] << 10) | ((long long)code[2] << 20) |
-              ((long long)code[3] << 30);
-    bool in = true;
-#pragma unroll
-    for (int q = 0; q < 6; q++) in = in && (r[q] < ta.s[q].rout);
-    if (!in) {
-        recE[c] = 0.0;
-#pragma unroll
-        for (int q = 0; q < 6; q++) recF[6 * c + q] = 0.0;
-        return;
-    }
-    R T[6][D], dT[6][D];
-    double f[6], fd[6];
-    int flags = 0;
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-        cheby<D, R>(r[q], ta.s[q], T[q], dT[q], flags);
-        fcut(r[q], ta.s[q].rout, ta.tersoff, ta.fcut_var, f[q], fd[q]);
-    }
-    double S, G[6], E, fs[6];
-    contract4<D, R, double>(Cd, T, dT, S, G);
-    cluster_scalars<6, double>(r, f, fd, S, G, E, fs);
-    recE[c] = E;
-#pragma unroll
-    for (int q = 0; q < 6; q++) recF[6 * c + q] = fs[q];
-    note_flags(cfg_stat, cfg, flags);
-}
-
-// ---------------------------------------------------------------------------
-// Owner reduction: one 64-lane workgroup per owner atom.  The pair force
-// scalars of every cluster the owner enumerated are summed into cells (s, t)
-// of its local slot pairs (slot 0 = owner, slot s = neighbour s-1) in list
-// order, each cell by one fixed lane; forces F_s += cell * h (v_t - v_s) and
-// the virial -cell * dr (x) dr follow per slot and per cell.  Everything is
-// summed in a fixed order, so results do not depend on scheduling.
-// ---------------------------------------------------------------------------
-#define RTILE 4096
-
-__device__ __forceinline__ int tri(int s, int t)
-{
-    const int lo = s < t ? s : t, hi = s < t ? t : s;
-    return hi * (hi - 1) / 2 + lo;
-}
-
-__device__ __forceinline__ void scan_chunk(const int *cidx, const double *cval, int np, int t0, int lane,
-                                           double *cells)
-{
-    for (int p = 0; p < np; p++) {
-        const int c = cidx[p] - t0;
-        if (c >= 0 && c < RTILE && (c & 63) == lane) cells[c] += cval[p];
-    }
-}
-
-__global__ __launch_bounds__(64) void k_reduce(int natoms, const chx_cfg *__restrict__ cfgs,
-                                                const chx_atom_s *__restrict__ as, const int *__restrict__ nbr_off,
-                                                const chx_entry_s *__restrict__ es,
-                                                const unsigned char *__restrict__ e_in3, const chx_red_args ra,
-                                                double *Fe, double *Fself, double *Eown, double *Sown)
-{
-    __shared__ double cells[RTILE];
-    __shared__ double pos[CHX_MAX_NB + 1][3];
-    __shared__ short s2m[CHX_MAX_NB + 1];     // slot -> many-body index (0 = owner), -1 if none
-    __shared__ short m2s[CHX_MAX_N3 + 1];     // many-body index -> slot
-    __shared__ int cidx[64 * 6];
-    __shared__ double cval[64 * 6];
-    __shared__ int nmb;
-    const int i = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int off = nbr_off[i];
-    const int nb = nbr_off[i + 1] - off;
-    const chx_atom_s ai = as[i];
-    const chx_cfg &C = cfgs[ai.cfg];
-    const chx_red_model &M = ra.models[C.model];
-    const long N = natoms;
-    for (int s = lane; s <= nb; s += 64) {
-        if (s == 0) { pos[0][0] = ai.u[0]; pos[0][1] = ai.u[1]; pos[0][2] = ai.u[2]; }
-        else {
-            const chx_entry_s g = es[off + s - 1];
-            pos[s][0] = g.v[0]; pos[s][1] = g.v[1]; pos[s][2] = g.v[2];
-        }
-    }
-    // many-body slot set: owner + images flagged by the enumeration, in slot order
-    if (lane == 0) { nmb = 1; s2m[0] = 0; m2s[0] = 0; }
-    __syncthreads();
-    for (int base = 1; base <= nb; base += 64) {
-        const int s = base + lane;
-        const bool in = s <= nb && e_in3[off + s - 1];
-        const unsigned long long b = __ballot(in);
-        const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-        const int m0 = nmb;
-        if (s <= nb) {
-            const int m = in ? m0 + __popcll(b & below) : -1;
-            s2m[s] = (short)m;
-            if (in) m2s[m] = (short)s;
-        }
-        __syncthreads();
-        if (lane == 0) nmb = m0 + __popcll(b);
-        __syncthreads();
-    }
-    const int nm = nmb;
-    // 2-body: direct, slot s gets -f dr, the owner +f dr
-    double F2[CHX_MAX_NB / 64][3];
-#pragma unroll
-    for (int q = 0; q < CHX_MAX_NB / 64; q++) F2[q][0] = F2[q][1] = F2[q][2] = 0.0;
-    double F0[3] = {0.0, 0.0, 0.0};
-    double Sv[6] = {0, 0, 0, 0, 0, 0};
-    double Eacc = 0.0;
-#pragma unroll
-    for (int q = 0; q < CHX_MAX_NB / 64; q++) {
-        const int s = 1 + lane + 64 * q;
-        if (s > nb) continue;
-        const double f = ra.rec2F[off + s - 1];
-        Eacc += ra.rec2E[off + s - 1];
-        if (f == 0.0) continue;
+    for (int q = 0; q < PQ::P; q++) {
         double dr[3];
-        pair_vec(C.h, pos[0], pos[s], dr);
-        F0[0] += f * dr[0]; F0[1] += f * dr[1]; F0[2] += f * dr[2];
-        F2[q][0] = -f * dr[0]; F2[q][1] = -f * dr[1]; F2[q][2] = -f * dr[2];
-        Sv[0] -= f * (dr[0] * dr[0]);
-        Sv[1] -= f * (dr[0] * dr[1]);
-        Sv[2] -= f * (dr[0] * dr[2]);
-        Sv[3] -= f * (dr[1] * dr[1]);
-        Sv[4] -= f * (dr[1] * dr[2]);
-        Sv[5] -= f * (dr[2] * dr[2]);
+        r[q] = pair_vec(C.h, p[PQ::a[q]], p[PQ::b[q]], dr);
+        in = in && (r[q] < grp.s[q].rout);
     }
-    // many-body: pair cells over the many-body slot set
-    double Fm[(CHX_MAX_N3 + 64) / 64][3];
-#pragma unroll
-    for (int q = 0; q < (CHX_MAX_N3 + 64) / 64; q++) Fm[q][0] = Fm[q][1] = Fm[q][2] = 0.0;
-    const int ncell = nm * (nm - 1) / 2;
-    const bool has_mb = (ra.off3 && M.ntrip) || (ra.off4 && M.nquad);
-    for (int t0 = 0; has_mb && t0 < ncell; t0 += RTILE) {
-        for (int c = lane; c < RTILE; c += 64) cells[c] = 0.0;
-        __syncthreads();
-        if (ra.off3) {
-            for (int g = M.g3; g < M.g3 + M.ntrip; g++) {
-                const long b = ra.off3[g * N + i], e = ra.off3[g * N + i + 1];
-                for (long cb = b; cb < e; cb += 64) {
-                    const long c = cb + lane;
-                    if (c < e) {
-                        const int m = ra.rec3M[c];
-                        const int s0 = s2m[m & 1023], s1 = s2m[(m >> 10) & 1023], s2 = s2m[(m >> 20) & 1023];
-                        cidx[3 * lane] = tri(s0, s1);
-                        cidx[3 * lane + 1] = tri(s0, s2);
-                        cidx[3 * lane + 2] = tri(s1, s2);
-                        cval[3 * lane] = ra.rec3F[3 * c];
-                        cval[3 * lane + 1] = ra.rec3F[3 * c + 1];
-                        cval[3 * lane + 2] = ra.rec3F[3 * c + 2];
-                        if (t0 == 0) Eacc += ra.rec3E[c];
-                    } else {
-                        cidx[3 * lane] = cidx[3 * lane + 1] = cidx[3 * lane + 2] = -1;
-                    }
-                    __syncthreads();
-                    scan_chunk(cidx, cval, 3 * 64, t0, lane, cells);
-                    __syncthreads();
-                }
-            }
-        }
-        if (ra.off4) {
-            for (int g = M.g4; g < M.g4 + M.nquad; g++) {
-                const long b = ra.off4[g * N + i], e = ra.off4[g * N + i + 1];
-                for (long cb = b; cb < e; cb += 64) {
-                    const long c = cb + lane;
-                    if (c < e) {
-                        const long long m = ra.rec4M[c];
-                        const int s[4] = {s2m[m & 1023], s2m[(m >> 10) & 1023], s2m[(m >> 20) & 1023],
-                                          s2m[(m >> 30) & 1023]};
-                        cidx[6 * lane] = tri(s[0], s[1]);
-                        cidx[6 * lane + 1] = tri(s[0], s[2]);
-                        cidx[6 * lane + 2] = tri(s[0], s[3]);
-                        cidx[6 * lane + 3] = tri(s[1], s[2]);
-                        cidx[6 * lane + 4] = tri(s[1], s[3]);
-                        cidx[6 * lane + 5] = tri(s[2], s[3]);
-#pragma unroll
-                        for (int q = 0; q < 6; q++) cval[6 * lane + q] = ra.rec4F[6 * c + q];
-                        if (t0 == 0) Eacc += ra.rec4E[c];
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < 6; q++) cidx[6 * lane + q] = -1;
-                    }
-                    __syncthreads();
-                    scan_chunk(cidx, cval, 6 * 64, t0, lane, cells);
-                    __syncthreads();
-                }
-            }
-        }
-        // forces on the many-body slots and the virial from this tile
-#pragma unroll
-        for (int q = 0; q < (CHX_MAX_N3 + 64) / 64; q++) {
-            const int a = lane + 64 * q;
-            if (a >= nm) continue;
-            const int sa = m2s[a];
-            for (int b = 0; b < nm; b++) {
-                if (b == a) continue;
-                const int c = tri(a, b) - t0;
-                if (c < 0 || c >= RTILE) continue;
-                const double v = cells[c];
-                if (v == 0.0) continue;
-                double dr[3];
-                pair_vec(C.h, pos[sa], pos[m2s[b]], dr);
-                Fm[q][0] += v * dr[0];
-                Fm[q][1] += v * dr[1];
-                Fm[q][2] += v * dr[2];
-                if (a < b) {
-                    Sv[0] -= v * (dr[0] * dr[0]);
-                    Sv[1] -= v * (dr[0] * dr[1]);
-                    Sv[2] -= v * (dr[0] * dr[2]);
-                    Sv[3] -= v * (dr[1] * dr[1]);
-                    Sv[4] -= v * (dr[1] * dr[2]);
-                    Sv[5] -= v * (dr[2] * dr[2]);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // many-body forces to LDS (reusing the cell tile), then per-slot totals
-    double *fm = cells;
-    for (int a = lane; a < 3 * nm && a < RTILE; a += 64) fm[a] = 0.0;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < (CHX_MAX_N3 + 64) / 64; q++) {
-        const int a = lane + 64 * q;
-        if (a >= nm || !has_mb) continue;
-        fm[3 * a] = Fm[q][0]; fm[3 * a + 1] = Fm[q][1]; fm[3 * a + 2] = Fm[q][2];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < CHX_MAX_NB / 64; q++) {
-        const int s = 1 + lane + 64 * q;
-        if (s > nb) continue;
-        const int m = s2m[s];
-        double f[3] = {F2[q][0], F2[q][1], F2[q][2]};
-        if (m > 0) { f[0] += fm[3 * m]; f[1] += fm[3 * m + 1]; f[2] += fm[3 * m + 2]; }
-        double *dst = Fe + 3L * (off + s - 1);
-        dst[0] = f[0]; dst[1] = f[1]; dst[2] = f[2];
-    }
-    // owner force, energy and virial: fixed-pattern wave reductions
-    double red[10] = {Eacc, Sv[0], Sv[1], Sv[2], Sv[3], Sv[4], Sv[5], F0[0], F0[1], F0[2]};
-#pragma unroll
-    for (int w = 32; w >= 1; w >>= 1)
-#pragma unroll
-        for (int k = 0; k < 10; k++) red[k] += __shfl_xor(red[k], w, 64);
-    if (lane == 0) {
-        Eown[i] = red[0] + M.eoff[ai.type];
-#pragma unroll
-        for (int k = 0; k < 6; k++) Sown[6L * i + k] = red[1 + k];
-        Fself[3L * i] = red[7] + fm[0];
-        Fself[3L * i + 1] = red[8] + fm[1];
-        Fself[3L * i + 2] = red[9] + fm[2];
-    }
+    return in;
 }
 
-// Force on replicated atom r: its own owner force plus every entry whose
-// parent is r, in entry order.
-__global__ void k_gather_rep(int natoms, const int *__restrict__ rstart, const int *__restrict__ rev,
-                             const double *__restrict__ Fe, const double *__restrict__ Fself, double *Frep)
+// Forces of one cluster from its pair force scalars: F_a += fs dr, F_b -= fs
+// dr for pair (a, b); virial -= fs dr dr.  Member m's force goes to its
+// replicated atom, the virial to the q7 owner accumulator slots 1..6.
+template <int NB>
+__device__ __forceinline__ void cluster_forces(const double *h, const double (&p)[NB][3], const int (&par)[NB],
+                                               const double (&fs)[pairs_of<NB>::P], unsigned long long *Facc,
+                                               double (&q7)[7], bool &bad)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= natoms) return;
-    double f[3] = {Fself[3L * r], Fself[3L * r + 1], Fself[3L * r + 2]};
-    for (int p = rstart[r]; p < rstart[r + 1]; p++) {
-        const long e = rev[p];
-        f[0] += Fe[3 * e]; f[1] += Fe[3 * e + 1]; f[2] += Fe[3 * e + 2];
+    using PQ = pairs_of<NB>;
+    double F[NB][3];
+#pragma unroll
+    for (int m = 0; m < NB; m++) F[m][0] = F[m][1] = F[m][2] = 0.0;
+#pragma unroll
+    for (int q = 0; q < PQ::P; q++) {
+        double dr[3];
+        pair_vec(h, p[PQ::a[q]], p[PQ::b[q]], dr);
+        const double f = fs[q];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            F[PQ::a[q]][k] += f * dr[k];
+            F[PQ::b[q]][k] -= f * dr[k];
+        }
+        q7[1] -= f * (dr[0] * dr[0]);
+        q7[2] -= f * (dr[0] * dr[1]);
+        q7[3] -= f * (dr[0] * dr[2]);
+        q7[4] -= f * (dr[1] * dr[1]);
+        q7[5] -= f * (dr[1] * dr[2]);
+        q7[6] -= f * (dr[2] * dr[2]);
     }
-    Frep[3L * r] = f[0]; Frep[3L * r + 1] = f[1]; Frep[3L * r + 2] = f[2];
+#pragma unroll
+    for (int m = 0; m < NB; m++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) fx_atomic(Facc + 2L * (3L * par[m] + k), F[m][k], bad);
 }
 
-// Force on input atom a: sum over its replicas in replica order.
-__global__ void k_gather_in(int natoms_in, int ncfg, const int *__restrict__ cfg_in_off,
-                            const chx_cfg *__restrict__ cfgs, const double *__restrict__ Frep, double *Fout)
+// ---------------------------------------------------------------------------
+// 3- and 4-body on the vector units: one thread per cluster, one wavefront per
+// 64-cluster tile of one group (any supported D; the 4-body matrix-core kernel
+// below takes D <= 4).
+// ---------------------------------------------------------------------------
+template <int NB, int D, typename R>
+__global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
+                                                const chx_group *__restrict__ groups, int total_word,
+                                                const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
+                                                const chx_atom_s *__restrict__ as, const int *__restrict__ nbr_off,
+                                                const chx_entry_s *__restrict__ es, unsigned long long *Facc,
+                                                unsigned long long *Oacc, int *cfg_stat, int *status)
+{
+    using PQ = pairs_of<NB>;
+    constexpr int P = PQ::P;
+    if (block_stop(status, CHX_STOP)) return;
+    const int lane = threadIdx.x & 63;
+    const long t0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    if (t0 >= status[total_word]) return;
+    const int g = find_group(gbase, G, t0);
+    const chx_group &grp = groups[g];
+    if (grp.type < 0 || grp.D != D) return;
+    if (NB == 4 && D <= CHX_MFMA_MAX_D) return;  // matrix-core kernel
+    const long c = t0 + lane;
+    int owner = -1;
+    double q7[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool bad = false;
+    if (c < gend[g]) {
+        const int2 rec = list[c];
+        double p[NB][3], r[P];
+        int par[NB], cfg;
+        if (cluster_geometry<NB>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg)) {
+            R T[P][D], dT[P][D];
+            double f[P], fd[P];
+            int flags = 0;
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                cheby<D, R>(r[q], grp.s[q], T[q], dT[q], flags);
+                fcut(r[q], grp.s[q].rout, grp.tersoff, grp.fcut_var, f[q], fd[q]);
+            }
+            const R *Cd = sizeof(R) == 8 ? (const R *)grp.C : (const R *)grp.Cf;
+            double S, Gq[P], E, fs[P];
+            if constexpr (NB == 3) contract3<D, R, double>(Cd, T, dT, S, Gq);
+            else contract4<D, R, double>(Cd, T, dT, S, Gq);
+            cluster_scalars<P, double>(r, f, fd, S, Gq, E, fs);
+            q7[0] = E;
+            cluster_forces<NB>(cfgs[cfg].h, p, par, fs, Facc, q7, bad);
+            owner = rec.x;
+            note_flags(cfg_stat, cfg, flags);
+        }
+    }
+    fx_owner<7>(Oacc, owner, q7, bad);
+    if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
+}
+
+// ---------------------------------------------------------------------------
+// 4-body on the matrix cores (D <= 4).  Workgroup = 4 wavefronts and one
+// 64-cluster tile of one group (16 clusters per wavefront); the group's
+// coefficient matrix is staged in LDS.  Per wavefront: (1) the four lanes of
+// a cluster compute its geometry, lane sub the polynomials of pairs sub and
+// sub + 4, staged in LDS; (2) MFMA contraction; (3) lane sub accumulates the
+// force on member sub and the virial of pairs sub and sub + 4.
+//
+// Matrix image (built on the host): Cm[k][m], k = (a*D + b)*4 + c over pairs
+// 01, 02, 03 and m = (d*D + e)*4 + f over pairs 12, 13, 23, the innermost
+// index padded to 4 and m to a multiple of 16; padding is zero.
+//
+// Lane l of a wavefront: cluster j = l % 16 of the wavefront's 16, sub = l / 16.
+// MFMA 16x16x4 operands: A[m][k] = Cm[4s + sub][16 rb + j] (LDS), B[k][n] =
+// X_v[cluster j][4s + sub], i.e. c = sub and (a, b) = (s / D, s % D).  Output
+// row m of lane l, register v: FP64 m = sub + 4v (f = sub, (d, e) from
+// 4 rb + v); FP32 m = 4 sub + v (f = v, (d, e) from 4 rb + sub).
+// ---------------------------------------------------------------------------
+template <typename R>
+struct mfma_acc;
+template <>
+struct mfma_acc<double> {
+    typedef dbl4 type;
+    static constexpr bool f64 = true;
+    __device__ __forceinline__ static dbl4 op(double a, double b, dbl4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct mfma_acc<float> {
+    typedef flt4 type;
+    static constexpr bool f64 = false;
+    __device__ __forceinline__ static flt4 op(float a, float b, flt4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <int D>
+struct mfma_shape {
+    static constexpr int KP = D * D * 4;                // rows of Cm (k)
+    static constexpr int MP = (D * D * 4 + 15) / 16 * 16;  // columns of Cm (m)
+    static constexpr int KS = D * D;                    // k steps of 4
+    static constexpr int RB = MP / 16;                  // 16-row output blocks
+};
+
+template <int D, typename R>
+__global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict__ gbase,
+                                                    const int *__restrict__ gend,
+                                                    const chx_group *__restrict__ groups,
+                                                    const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
+                                                    const chx_atom_s *__restrict__ as,
+                                                    const int *__restrict__ nbr_off,
+                                                    const chx_entry_s *__restrict__ es, unsigned long long *Facc,
+                                                    unsigned long long *Oacc, int *cfg_stat, int *status)
+{
+    using SH = mfma_shape<D>;
+    using MA = mfma_acc<R>;
+    typedef typename MA::type acc_t;
+    constexpr int CLD = SH::MP + 16;  // LDS row of the matrix image (padding spreads the four k rows over banks)
+    constexpr int TS = 6 * D + 1;     // per-cluster stride of the staged polynomials (odd: conflict-free)
+    __shared__ R Cl[SH::KP * CLD];
+    __shared__ R Ts[4][16 * TS], dTs[4][16 * TS];
+    __shared__ double Gm[4][16][6][3];  // r, fcut, fcut' per pair
+    __shared__ double Pm[4][16][4][3];  // member positions (evaluation-frame fractional)
+    __shared__ int Pa[4][16][4];        // member replicated atoms
+    if (block_stop(status, CHX_STOP)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = lane & 15, sub = lane >> 4;
+    const long nbt = status[CHX_ST_N4] / 64;
+    {
+        const long bt = blockIdx.x;
+        if (bt >= nbt) return;
+        const long t0 = bt * 64;
+        const int g = find_group(gbase, G, t0);
+        const chx_group &grp = groups[g];
+        if (grp.type < 0 || grp.D != D) return;
+        {
+            const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
+            for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 256) Cl[(x / SH::MP) * CLD + x % SH::MP] = src[x];
+        }
+        // ---- stage: geometry (all four lanes of a cluster), polynomials of pairs sub and sub + 4
+        const long c = t0 + wv * 16 + j;
+        bool act = c < gend[g];
+        int2 rec = make_int2(-1, 0);
+        int cfg = 0, flags = 0;
+        {
+            double p[4][3], r[6];
+            int par[4];
+            if (act) {
+                rec = list[c];
+                act = cluster_geometry<4>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg);
+            }
+            double pm[3] = {p[0][0], p[0][1], p[0][2]};
+            int am = par[0];
+#pragma unroll
+            for (int m = 1; m < 4; m++)
+                if (sub == m) { pm[0] = p[m][0]; pm[1] = p[m][1]; pm[2] = p[m][2]; am = par[m]; }
+            Pm[wv][j][sub][0] = pm[0];
+            Pm[wv][j][sub][1] = pm[1];
+            Pm[wv][j][sub][2] = pm[2];
+            Pa[wv][j][sub] = am;
+            for (int q = sub; q < 6; q += 4) {
+                double rq = r[0];
+#pragma unroll
+                for (int x = 1; x < 6; x++) rq = q == x ? r[x] : rq;
+                R Tq[D], dTq[D];
+                double fq = 0.0, fdq = 0.0;
+                if (act) {
+                    const chx_slot sl = grp.s[q];
+                    cheby<D, R>(rq, sl, Tq, dTq, flags);
+                    fcut(rq, sl.rout, grp.tersoff, grp.fcut_var, fq, fdq);
+                } else {
+#pragma unroll
+                    for (int x = 0; x < D; x++) { Tq[x] = (R)0; dTq[x] = (R)0; }
+                }
+#pragma unroll
+                for (int x = 0; x < D; x++) {
+                    Ts[wv][j * TS + q * D + x] = Tq[x];
+                    dTs[wv][j * TS + q * D + x] = dTq[x];
+                }
+                Gm[wv][j][q][0] = rq;
+                Gm[wv][j][q][1] = fq;
+                Gm[wv][j][q][2] = fdq;
+            }
+        }
+        __syncthreads();
+        // ---- contraction on the matrix cores
+        const R *Tj = &Ts[wv][j * TS], *dTj = &dTs[wv][j * TS];
+        R T0[D], dT0[D], T1[D], dT1[D], T3[D], dT3[D], T4[D], dT4[D], T5[D], dT5[D];
+#pragma unroll
+        for (int x = 0; x < D; x++) {
+            T0[x] = Tj[0 * D + x]; dT0[x] = dTj[0 * D + x];
+            T1[x] = Tj[1 * D + x]; dT1[x] = dTj[1 * D + x];
+            T3[x] = Tj[3 * D + x]; dT3[x] = dTj[3 * D + x];
+            T4[x] = Tj[4 * D + x]; dT4[x] = dTj[4 * D + x];
+            T5[x] = Tj[5 * D + x]; dT5[x] = dTj[5 * D + x];
+        }
+        // lane-selected factors: c = sub (pair 03), and f = sub (pair 23) in the FP64 layout
+        const R t2 = sub < D ? Tj[2 * D + sub] : (R)0, dt2 = sub < D ? dTj[2 * D + sub] : (R)0;
+        const R t5 = sub < D ? Tj[5 * D + sub] : (R)0, dt5 = sub < D ? dTj[5 * D + sub] : (R)0;
+        R S = 0, Gs[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int rb = 0; rb < SH::RB; rb++) {
+            acc_t z0 = {0, 0, 0, 0}, z1 = {0, 0, 0, 0}, z2 = {0, 0, 0, 0}, z3 = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < SH::KS; s++) {
+                const int a = s / D, b = s % D;
+                const R A = Cl[(4 * s + sub) * CLD + 16 * rb + j];
+                const R x01 = T0[a] * T1[b];
+                z0 = MA::op(A, x01 * t2, z0);
+                z1 = MA::op(A, (dT0[a] * T1[b]) * t2, z1);
+                z2 = MA::op(A, (T0[a] * dT1[b]) * t2, z2);
+                z3 = MA::op(A, x01 * dt2, z3);
+            }
+            if (MA::f64) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int de = 4 * rb + v;
+                    if (de >= D * D) continue;
+                    const int d = de / D, e = de % D;
+                    const R t34 = T3[d] * T4[e];
+                    const R y = t34 * t5;
+                    S += z0[v] * y;
+                    Gs[0] += z1[v] * y;
+                    Gs[1] += z2[v] * y;
+                    Gs[2] += z3[v] * y;
+                    Gs[3] += z0[v] * ((dT3[d] * T4[e]) * t5);
+                    Gs[4] += z0[v] * ((T3[d] * dT4[e]) * t5);
+                    Gs[5] += z0[v] * (t34 * dt5);
+                }
+            } else {
+                R q34 = 0, q3 = 0, q4 = 0;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int de = 4 * rb + u;
+                    if (de >= D * D) continue;
+                    const int d = de / D, e = de % D;
+                    if (sub == u) {
+                        q34 = T3[d] * T4[e];
+                        q3 = dT3[d] * T4[e];
+                        q4 = T3[d] * dT4[e];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    if (v >= D) continue;
+                    const R y = q34 * T5[v];
+                    S += z0[v] * y;
+                    Gs[0] += z1[v] * y;
+                    Gs[1] += z2[v] * y;
+                    Gs[2] += z3[v] * y;
+                    Gs[3] += z0[v] * (q3 * T5[v]);
+                    Gs[4] += z0[v] * (q4 * T5[v]);
+                    Gs[5] += z0[v] * (q34 * dT5[v]);
+                }
+            }
+            // keep the operand loads of the next row block out of this one
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // sum over the four lanes of each cluster
+#pragma unroll
+        for (int w = 16; w <= 32; w <<= 1) {
+            S += __shfl_xor(S, w, 64);
+#pragma unroll
+            for (int q = 0; q < 6; q++) Gs[q] += __shfl_xor(Gs[q], w, 64);
+            flags |= __shfl_xor(flags, w, 64);
+        }
+        // ---- forces (lane sub: member sub, virial of pairs sub and sub + 4), energy, virial
+        double q7[7] = {0, 0, 0, 0, 0, 0, 0};
+        bool bad = false;
+        int owner = -1;
+        if (act) {
+            double r[6], f[6], fd[6], E, fs[6], Gd[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                r[q] = Gm[wv][j][q][0];
+                f[q] = Gm[wv][j][q][1];
+                fd[q] = Gm[wv][j][q][2];
+                Gd[q] = (double)Gs[q];
+            }
+            cluster_scalars<6, double>(r, f, fd, (double)S, Gd, E, fs);
+            const double *h = cfgs[cfg].h;
+            double F[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const int pa = pairs_of<4>::a[q], pb = pairs_of<4>::b[q];
+                const bool mine_f = pa == sub || pb == sub;
+                const bool mine_v = (q & 3) == sub;
+                if (!mine_f && !mine_v) continue;
+                double dr[3];
+                pair_vec(h, Pm[wv][j][pa], Pm[wv][j][pb], dr);
+                const double fq = fs[q];
+                if (mine_f) {
+                    const double sg = pa == sub ? 1.0 : -1.0;
+                    F[0] += sg * (fq * dr[0]);
+                    F[1] += sg * (fq * dr[1]);
+                    F[2] += sg * (fq * dr[2]);
+                }
+                if (mine_v) {
+                    q7[1] -= fq * (dr[0] * dr[0]);
+                    q7[2] -= fq * (dr[0] * dr[1]);
+                    q7[3] -= fq * (dr[0] * dr[2]);
+                    q7[4] -= fq * (dr[1] * dr[1]);
+                    q7[5] -= fq * (dr[1] * dr[2]);
+                    q7[6] -= fq * (dr[2] * dr[2]);
+                }
+            }
+            const int pm = Pa[wv][j][sub];
+#pragma unroll
+            for (int k = 0; k < 3; k++) fx_atomic(Facc + 2L * (3L * pm + k), F[k], bad);
+            if (sub == 0) {
+                q7[0] = E;
+                note_flags(cfg_stat, cfg, flags);
+            }
+            owner = rec.x;
+        }
+        fx_owner<7>(Oacc, owner, q7, bad);
+        if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Totals.  Force on input atom a: the exact sum over its replicas' force
+// accumulators.  Energy and virial of a configuration: the exact sum over its
+// owners plus the 1-body offsets (compute_1B, chimesFF.cpp:1636-1641).
+// ---------------------------------------------------------------------------
+__global__ void k_fin_forces(int natoms_in, int ncfg, const int *__restrict__ cfg_in_off,
+                             const chx_cfg *__restrict__ cfgs, const unsigned long long *__restrict__ Facc,
+                             double *Fout)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natoms_in) return;
@@ -470,40 +581,60 @@ __global__ void k_gather_in(int natoms_in, int ncfg, const int *__restrict__ cfg
     const chx_cfg &C = cfgs[lo];
     const int la = a - C.atom_in_off;
     const int ncopy = (C.n_rep + 1) * (C.n_rep + 1) * (C.n_rep + 1);
-    double f[3] = {0.0, 0.0, 0.0};
+    fx_sum f[3];
     for (int rho = 0; rho < ncopy; rho++) {
         const long r = C.atom_off + (long)rho * C.natoms_in + la;
-        f[0] += Frep[3 * r]; f[1] += Frep[3 * r + 1]; f[2] += Frep[3 * r + 2];
+#pragma unroll
+        for (int k = 0; k < 3; k++) f[k].add(Facc + 2L * (3L * r + k));
     }
-    Fout[3L * a] = f[0]; Fout[3L * a + 1] = f[1]; Fout[3L * a + 2] = f[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) Fout[3L * a + k] = f[k].value();
 }
 
-// Energy and virial of each configuration: fixed-order block reduction.
-__global__ __launch_bounds__(256) void k_cfg_sum(const chx_cfg *__restrict__ cfgs, const double *__restrict__ Eown,
-                                                  const double *__restrict__ Sown, double *Ecfg, double *Vcfg)
+__global__ __launch_bounds__(256) void k_fin_cfg(const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
+                                                  const chx_model_dev *__restrict__ models,
+                                                  const unsigned long long *__restrict__ Oacc, double *Ecfg,
+                                                  double *Vcfg, int *status)
 {
-    __shared__ double sm[7][256];
+    __shared__ long long sh_hi[7][256];
+    __shared__ unsigned long long sh_lo[7][256];
     const chx_cfg &C = cfgs[blockIdx.x];
-    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const chx_model_dev &M = models[C.model];
+    fx_sum acc[7];
+    bool bad = false;
     for (int r = C.atom_off + threadIdx.x; r < C.atom_off + C.natoms; r += 256) {
-        acc[0] += Eown[r];
 #pragma unroll
-        for (int k = 0; k < 6; k++) acc[1 + k] += Sown[6L * r + k];
+        for (int k = 0; k < 7; k++) acc[k].add(Oacc + 2L * (7L * r + k));
+        acc[0].add_value(M.eoff[as[r].type], bad);
     }
 #pragma unroll
-    for (int k = 0; k < 7; k++) sm[k][threadIdx.x] = acc[k];
+    for (int k = 0; k < 7; k++) {
+        sh_hi[k][threadIdx.x] = acc[k].hi + (long long)(acc[k].lo >> 40);
+        sh_lo[k][threadIdx.x] = acc[k].lo & 0xffffffffffull;
+    }
     __syncthreads();
     for (int w = 128; w >= 1; w >>= 1) {
         if ((int)threadIdx.x < w)
 #pragma unroll
-            for (int k = 0; k < 7; k++) sm[k][threadIdx.x] += sm[k][threadIdx.x + w];
+            for (int k = 0; k < 7; k++) {
+                sh_hi[k][threadIdx.x] += sh_hi[k][threadIdx.x + w];
+                sh_lo[k][threadIdx.x] += sh_lo[k][threadIdx.x + w];
+            }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        Ecfg[blockIdx.x] = sm[0][0];
+        fx_sum t;
+        t.hi = sh_hi[0][0];
+        t.lo = sh_lo[0][0];
+        Ecfg[blockIdx.x] = t.value();
 #pragma unroll
-        for (int k = 0; k < 6; k++) Vcfg[6L * blockIdx.x + k] = sm[1 + k][0];
+        for (int k = 0; k < 6; k++) {
+            t.hi = sh_hi[1 + k][0];
+            t.lo = sh_lo[1 + k][0];
+            Vcfg[6L * blockIdx.x + k] = t.value();
+        }
     }
+    if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
 }
 
 // ---------------------------------------------------------------------------
@@ -625,50 +756,69 @@ __global__ void k_clusters(int n, const double *__restrict__ dist, const double 
 // ---------------------------------------------------------------------------
 #define CHX_D3_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13)
 #define CHX_D4_CASES(X) X(2) X(3) X(4) X(5) X(6)
+#define CHX_D4M_CASES(X) X(2) X(3) X(4)
 
 bool supported_D3(int D) { return D >= 2 && D <= 13; }
 bool supported_D4(int D) { return D >= 2 && D <= 6; }
 
-void launch_eval3(int D, bool mixed, long begin, long end, hipStream_t st, const int2 *list, const chx_cfg *cfgs,
-                  const chx_atom_s *as, const int *nbr_off, const chx_entry_s *es, const chx_type_args3 &ta,
-                  const void *Cd, double *recE, double *recF, int *recM, int *cfg_stat)
+void launch_eval2(long cap, hipStream_t st, const int *e_owner, const chx_cfg *cfgs, const chx_atom_s *as,
+                  const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
+                  unsigned long long *Oacc, int *cfg_stat, int *status)
 {
-    const long n = end - begin;
-    if (n <= 0) return;
-    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
-#define X(d)                                                                                                    \
-    if (D == d) {                                                                                               \
-        if (mixed)                                                                                              \
-            hipLaunchKernelGGL((k_eval3<d, float>), grid, block, 0, st, begin, end, list, cfgs, as, nbr_off, es, \
-                               ta, (const float *)Cd, recE, recF, recM, cfg_stat);                              \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_eval3<d, double>), grid, block, 0, st, begin, end, list, cfgs, as, nbr_off,    \
-                               es, ta, (const double *)Cd, recE, recF, recM, cfg_stat);                         \
-        return;                                                                                                 \
+    if (cap <= 0) return;
+    hipLaunchKernelGGL(k_eval2, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, e_owner, cfgs, as, es, models,
+                       Facc, Oacc, cfg_stat, status);
+}
+
+void launch_evalv(int nb, int D, bool mixed, const chx_list_args &la, hipStream_t st)
+{
+    if (la.cap <= 0 || la.G <= 0) return;
+    const dim3 grid((unsigned)((la.cap + 255) / 256)), block(256);
+    const int word = nb == 3 ? CHX_ST_N3 : CHX_ST_N4;
+#define X(d)                                                                                                   \
+    if (D == d) {                                                                                              \
+        if (mixed)                                                                                             \
+            hipLaunchKernelGGL((k_evalv<NB_, d, float>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups, \
+                               word, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,  \
+                               la.status);                                                                     \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_evalv<NB_, d, double>), grid, block, 0, st, la.G, la.gbase, la.gend,          \
+                               la.groups, word, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,  \
+                               la.cfg_stat, la.status);                                                        \
+        return;                                                                                                \
     }
-    CHX_D3_CASES(X)
+    if (nb == 3) {
+#define NB_ 3
+        CHX_D3_CASES(X)
+#undef NB_
+    } else {
+#define NB_ 4
+        CHX_D4_CASES(X)
+#undef NB_
+    }
 #undef X
 }
 
-void launch_eval4(int D, bool mixed, long begin, long end, hipStream_t st, const int2 *list, const chx_cfg *cfgs,
-                  const chx_atom_s *as, const int *nbr_off, const chx_entry_s *es, const chx_type_args4 &ta,
-                  const void *Cd, double *recE, double *recF, long long *recM, int *cfg_stat)
+bool launch_eval4m(int D, bool mixed, const chx_list_args &la, hipStream_t st)
 {
-    const long n = end - begin;
-    if (n <= 0) return;
-    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (D > CHX_MFMA_MAX_D) return false;
+    if (la.cap <= 0 || la.G <= 0) return true;
+    const dim3 grid((unsigned)std::max<long>(1, la.cap / 64)), block(256);
 #define X(d)                                                                                                    \
     if (D == d) {                                                                                               \
         if (mixed)                                                                                              \
-            hipLaunchKernelGGL((k_eval4<d, float>), grid, block, 0, st, begin, end, list, cfgs, as, nbr_off, es, \
-                               ta, (const float *)Cd, recE, recF, recM, cfg_stat);                              \
+            hipLaunchKernelGGL((k_eval4m<d, float>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups,     \
+                               la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,        \
+                               la.status);                                                                      \
         else                                                                                                    \
-            hipLaunchKernelGGL((k_eval4<d, double>), grid, block, 0, st, begin, end, list, cfgs, as, nbr_off,    \
-                               es, ta, (const double *)Cd, recE, recF, recM, cfg_stat);                         \
-        return;                                                                                                 \
+            hipLaunchKernelGGL((k_eval4m<d, double>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups,    \
+                               la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,        \
+                               la.status);                                                                      \
+        return true;                                                                                            \
     }
-    CHX_D4_CASES(X)
+    CHX_D4M_CASES(X)
 #undef X
+    return false;
 }
 
 void launch_clusters(int nb, int D, int n, hipStream_t st, const double *dist, const double *drv, const int *types,
@@ -697,4 +847,3 @@ void launch_clusters(int nb, int D, int n, hipStream_t st, const double *dist, c
 }
 
 }  // namespace chx
-

@@ -1,6 +1,6 @@
 // Geometry kernels: atom preparation (replication, wrapping, frames), cell
-// lists, neighbour (image) lists, 3-/4-body cluster enumeration, reverse
-// neighbour lists and an integer scan.
+// lists, sorted neighbour (image) lists, 3-/4-body cluster enumeration into
+// type-aligned list segments, and an integer scan.
 //
 // All arithmetic on positions reproduces the reference's operation order
 // (serial_chimes_interface.h:141-208, serial_chimes_interface.cpp:150-341,
@@ -8,6 +8,7 @@
 // membership decided here is bit-identical to the reference's.
 #include <hip/hip_runtime.h>
 
+#include "chx_dev.h"
 #include "chx_kernels.h"
 
 namespace chx {
@@ -111,129 +112,159 @@ __global__ void k_cell_fill(int natoms, const chx_atom_n *__restrict__ an, const
     cell_atoms[cell_start[c] + p] = r;
 }
 
-// Sort the atoms of each cell by index, and of each reverse-list segment by
-// entry index, so that every later traversal is in a fixed order.
-__global__ void k_segment_sort(int nseg, const int *__restrict__ start, int *vals)
+// Image of atom a under lattice shift sh in the neighbour frame, and whether
+// it lies within the search radius of the owner at ui
+// (serial_chimes_interface.cpp:534-587, 700-726).
+__device__ __forceinline__ bool nbr_image(const chx_cfg &C, const double *ui, const double *ua, const int *sh,
+                                          bool zero, double *vn)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
-    const int b = start[s], e = start[s + 1];
-    for (int i = b + 1; i < e; i++) {
-        const int v = vals[i];
-        int j = i - 1;
-        while (j >= b && vals[j] > v) { vals[j + 1] = vals[j]; j--; }
-        vals[j + 1] = v;
+    if (zero) {
+        vn[0] = ua[0]; vn[1] = ua[1]; vn[2] = ua[2];
+    } else {
+        double t[3] = {ua[0], ua[1], ua[2]};
+        t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
+        double xg[3];
+        mv(C.hn, t[0], t[1], t[2], xg);
+        mv(C.ihn, xg[0], xg[1], xg[2], vn);
     }
+    return frac_dist(C.hn, ui, vn) < C.search;
 }
 
 // ---------------------------------------------------------------------------
 // Neighbour lists: for real atom i every image g = (a, n) with i <= a,
 // g != i and |r_g - r_i| < search in the neighbour frame
-// (serial_chimes_interface.cpp:682-731).  MODE 0 counts, MODE 1 writes.
+// (serial_chimes_interface.cpp:682-731).  One wavefront per owner, four
+// owners per workgroup; lanes stride over the atoms of each searched cell.
+// MODE 0 counts.  MODE 1 collects the (parent, shift) keys in LDS, ranks
+// them and writes each owner's images sorted by key, so every list and every
+// later traversal is in one fixed order.
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ void k_nbr(int natoms, const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
-                      const chx_atom_n *__restrict__ an, const int *__restrict__ cell_start,
-                      const int *__restrict__ cell_atoms, int *nbr_cnt, const int *__restrict__ nbr_off,
-                      chx_entry_s *es, chx_entry_n *en, int *e_owner)
+__global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restrict__ cfgs,
+                                              const chx_atom_s *__restrict__ as, const chx_atom_n *__restrict__ an,
+                                              const int *__restrict__ cell_start, const int *__restrict__ cell_atoms,
+                                              int *nbr_cnt, const int *__restrict__ nbr_off, chx_entry_s *es,
+                                              chx_entry_n *en, int *e_owner, int *status)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
-    const chx_atom_s ai = as[i];
+    __shared__ long long keys[4][MODE ? CHX_MAX_NB : 1];
+    if (MODE == 1 && block_stop(status, CHX_STOP)) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wv;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int n = 0;
+    if (i < natoms) {
+        const chx_atom_n ni = an[i];
+        const chx_cfg &C = cfgs[as[i].cfg];
+        const int lc = ni.cell - C.cell_off;
+        const int ci[3] = {lc % C.m[0], (lc / C.m[0]) % C.m[1], lc / (C.m[0] * C.m[1])};
+        for (int dz = -C.reach[2]; dz <= C.reach[2]; dz++)
+            for (int dy = -C.reach[1]; dy <= C.reach[1]; dy++)
+                for (int dx = -C.reach[0]; dx <= C.reach[0]; dx++) {
+                    const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
+                    int wc[3], sh[3];
+                    bool ok = true;
+                    for (int d = 0; d < 3; d++) {
+                        int q = u[d] % C.m[d];
+                        if (q < 0) q += C.m[d];
+                        wc[d] = q;
+                        sh[d] = (u[d] - q) / C.m[d];
+                        if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
+                    }
+                    if (!ok) continue;
+                    const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
+                    const int cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
+                    const int cs = cell_start[cell], ce = cell_start[cell + 1];
+                    const int spack = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
+                    for (int p0 = cs; p0 < ce; p0 += 64) {
+                        const int p = p0 + lane;
+                        bool in = false;
+                        int a = 0;
+                        if (p < ce) {
+                            a = cell_atoms[p];
+                            if (!(a < i || (a == i && zero))) {
+                                double vn[3];
+                                in = nbr_image(C, ni.u, an[a].u, sh, zero, vn);
+                            }
+                        }
+                        const unsigned long long b = __ballot(in);
+                        if (MODE == 1 && in) {
+                            const int idx = n + __popcll(b & below);
+                            if (idx < CHX_MAX_NB) keys[wv][idx] = ((long long)a << 24) | spack;
+                        }
+                        n += __popcll(b);
+                    }
+                }
+        if (MODE == 0 && lane == 0) {
+            nbr_cnt[i] = n;
+            atomicMax(&status[CHX_ST_MAXNB], n);
+            if (n > CHX_MAX_NB) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_NB);
+        }
+    }
+    if (MODE == 0) return;
+    __syncthreads();
+    if (i >= natoms || n > CHX_MAX_NB) return;
     const chx_atom_n ni = an[i];
-    const chx_cfg &C = cfgs[ai.cfg];
-    const int lc = ni.cell - C.cell_off;
-    const int ci[3] = {lc % C.m[0], (lc / C.m[0]) % C.m[1], lc / (C.m[0] * C.m[1])};
-    int cnt = 0;
-    long w = MODE ? nbr_off[i] : 0;
-    const double S = C.search;
-    for (int dz = -C.reach[2]; dz <= C.reach[2]; dz++)
-        for (int dy = -C.reach[1]; dy <= C.reach[1]; dy++)
-            for (int dx = -C.reach[0]; dx <= C.reach[0]; dx++) {
-                const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
-                int wc[3], sh[3];
-                bool ok = true;
-                for (int d = 0; d < 3; d++) {
-                    int q = u[d] % C.m[d];
-                    if (q < 0) q += C.m[d];
-                    wc[d] = q;
-                    sh[d] = (u[d] - q) / C.m[d];
-                    if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
-                }
-                if (!ok) continue;
-                const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
-                const int cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
-                for (int p = cell_start[cell]; p < cell_start[cell + 1]; p++) {
-                    const int a = cell_atoms[p];
-                    if (a < i || (a == i && zero)) continue;
-                    const chx_atom_n na = an[a];
-                    double vn[3];
-                    if (zero) {
-                        vn[0] = na.u[0]; vn[1] = na.u[1]; vn[2] = na.u[2];
-                    } else {
-                        double t[3] = {na.u[0], na.u[1], na.u[2]};
-                        t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
-                        double xg[3];
-                        mv(C.hn, t[0], t[1], t[2], xg);
-                        mv(C.ihn, xg[0], xg[1], xg[2], vn);
-                    }
-                    if (!(frac_dist(C.hn, ni.u, vn) < S)) continue;
-                    if (MODE == 0) { cnt++; continue; }
-                    const chx_atom_s sa = as[a];
-                    chx_entry_s e;
-                    if (zero) {
-                        e.v[0] = sa.u[0]; e.v[1] = sa.u[1]; e.v[2] = sa.u[2];
-                    } else {
-                        double t[3] = {sa.u[0], sa.u[1], sa.u[2]};
-                        t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
-                        double xg[3];
-                        mv(C.h, t[0], t[1], t[2], xg);
-                        mv(C.ih, xg[0], xg[1], xg[2], e.v);
-                    }
-                    e.parent = a;
-                    e.type = sa.type;
-                    chx_entry_n f;
-                    f.v[0] = vn[0]; f.v[1] = vn[1]; f.v[2] = vn[2];
-                    f.shift = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
-                    f.pad = 0;
-                    es[w] = e;
-                    en[w] = f;
-                    e_owner[w] = i;
-                    w++;
-                }
-            }
-    if (MODE == 0) nbr_cnt[i] = cnt;
+    const chx_cfg &C = cfgs[as[i].cfg];
+    const long off = nbr_off[i];
+    for (int j = lane; j < n; j += 64) {
+        const long long k = keys[wv][j];
+        int rk = 0;
+        for (int q = 0; q < n; q++) rk += keys[wv][q] < k;
+        const int a = (int)(k >> 24);
+        const int spack = (int)(k & 0xffffff);
+        const int sh[3] = {(spack & 255) - 128, ((spack >> 8) & 255) - 128, ((spack >> 16) & 255) - 128};
+        const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
+        chx_entry_n f;
+        nbr_image(C, ni.u, an[a].u, sh, zero, f.v);
+        f.shift = spack;
+        f.pad = 0;
+        const chx_atom_s sa = as[a];
+        chx_entry_s e;
+        if (zero) {
+            e.v[0] = sa.u[0]; e.v[1] = sa.u[1]; e.v[2] = sa.u[2];
+        } else {
+            double t[3] = {sa.u[0], sa.u[1], sa.u[2]};
+            t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
+            double xg[3];
+            mv(C.h, t[0], t[1], t[2], xg);
+            mv(C.ih, xg[0], xg[1], xg[2], e.v);
+        }
+        e.parent = a;
+        e.type = sa.type;
+        es[off + rk] = e;
+        en[off + rk] = f;
+        e_owner[off + rk] = i;
+    }
 }
 
-// ---------------------------------------------------------------------------
-// Sort each owner's images by (parent, shift) into the second entry buffer,
-// one 64-lane workgroup per owner.  A fixed order makes the cluster lists
-// canonical and every later traversal deterministic.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_nbr_sort(int natoms, const int *__restrict__ nbr_off,
-                                                 const chx_entry_s *__restrict__ es,
-                                                 const chx_entry_n *__restrict__ en, chx_entry_s *es2,
-                                                 chx_entry_n *en2, int *status)
+// Entry total against the entry buffer capacity.
+__global__ void k_check_entries(const int *__restrict__ nbr_off, int natoms, long cap, int *status)
 {
-    __shared__ long long keys[CHX_MAX_NB];
-    const int i = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int off = nbr_off[i];
-    const int nb = nbr_off[i + 1] - off;
-    if (lane == 0) atomicMax(&status[1], nb);
-    if (nb > CHX_MAX_NB) {
-        if (lane == 0) atomicOr(&status[0], 1);
-        return;
+    if (threadIdx.x || blockIdx.x) return;
+    const int tot = nbr_off[natoms];
+    status[CHX_ST_NENT] = tot;
+    if (tot > cap) atomicOr(&status[CHX_ST_FLAGS], CHX_RETRY_ENT);
+}
+
+// Aligned segment of each (model, cluster type) group in a 3-/4-body list:
+// base[g] = sum over h < g of the group sizes rounded up to CHX_LIST_ALIGN,
+// end[g] = base[g] + size of g.  off is the exclusive scan of the per-(group,
+// owner) counts laid out [G][N] (off[G*N] = total).
+__global__ void k_group_bases(const int *__restrict__ off, long N, int G, long cap, int *base, int *end,
+                              int *status, int word, int retry_bit)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    status[word + (CHX_ST_C3 - CHX_ST_N3)] = off[(long)G * N];
+    long b = 0;
+    for (int g = 0; g < G; g++) {
+        const long sz = (long)off[(long)(g + 1) * N] - off[(long)g * N];
+        base[g] = (int)b;
+        end[g] = (int)(b + sz);
+        b += (sz + CHX_LIST_ALIGN - 1) / CHX_LIST_ALIGN * CHX_LIST_ALIGN;
     }
-    for (int j = lane; j < nb; j += 64) keys[j] = ((long long)es[off + j].parent << 24) | en[off + j].shift;
-    __syncthreads();
-    for (int j = lane; j < nb; j += 64) {
-        const long long k = keys[j];
-        int rk = 0;
-        for (int q = 0; q < nb; q++) rk += keys[q] < k;
-        es2[off + rk] = es[off + j];
-        en2[off + rk] = en[off + j];
-    }
+    base[G] = (int)b;
+    status[word] = (int)b;
+    if (b > cap || b >= (1L << 31) - CHX_LIST_ALIGN) atomicOr(&status[CHX_ST_FLAGS], retry_bit);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,7 +291,6 @@ struct EnumShared {
     unsigned A4[CHX_MAX_N4][EW4];
     int cnt[CHX_MAX_CLUSTER_TYPES + 1];
     int run[CHX_MAX_CLUSTER_TYPES + 1];
-    int rowcnt[64][CHX_MAX_CLUSTER_TYPES + 1];
     int n3, n4;
 };
 
@@ -271,9 +301,12 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
                                              const chx_entry_n *__restrict__ en,
                                              const chx_enum_model *__restrict__ em, int *cnt3, int *cnt4,
                                              const int *__restrict__ off3, const int *__restrict__ off4,
-                                             int2 *list3, int2 *list4, unsigned char *e_in3, int *status)
+                                             const int *__restrict__ gb3, const int *__restrict__ gb4,
+                                             int2 *list3, int2 *list4, int *status, int ngl_max)
 {
     __shared__ EnumShared sh;
+    extern __shared__ int rowcnt_dyn[];  // [64][ngl_max] per-row counts of the ordered write
+    if (block_stop(status, CHX_STOP)) return;
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int off = nbr_off[i];
@@ -286,11 +319,7 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
     const double rc3 = C.rc3, rc4 = C.rc4;
     const bool any3 = M.has3 && rc3 > 0.0, any4 = M.has4 && rc4 > 0.0;
     const double rcS = rc3 > rc4 ? rc3 : rc4;  // many-body candidate radius
-    if (!any3 && !any4) {
-        if (MODE == 1)
-            for (int j = lane; j < nb; j += 64) e_in3[off + j] = 0;
-        return;
-    }
+    if (!any3 && !any4) return;
     if (lane == 0) { sh.n3 = 0; sh.n4 = 0; }
     __syncthreads();
     // compact the images within rc3 / rc4, in sorted order
@@ -303,7 +332,6 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
             d = frac_dist(C.hn, ni.u, e.v);
         }
         const bool in3 = j < nb && d < rcS, in4 = j < nb && d < rc4;
-        if (MODE == 1 && j < nb) e_in3[off + j] = in3 ? 1 : 0;
         const unsigned long long b3 = __ballot(in3), b4 = __ballot(in4);
         const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
         const int n3b = sh.n3, n4b = sh.n4;
@@ -369,19 +397,21 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
         int *cntv = body == 3 ? cnt3 : cnt4;
         const int *offv = body == 3 ? off3 : off4;
         int2 *list = body == 3 ? list3 : list4;
+        const int *gbv = body == 3 ? gb3 : gb4;
         const int nrow = body == 3 ? n3 : n4;
         for (int round = 0; round * 64 < nrow; round++) {
             const int r = round * 64 + lane;
+            int *rowcnt = rowcnt_dyn + lane * ngl_max;
             if (MODE == 1)
-                for (int g = 0; g < ngl; g++) sh.rowcnt[lane][g] = 0;
+                for (int g = 0; g < ngl; g++) rowcnt[g] = 0;
             for (int pass = 0; pass < (MODE == 1 ? 2 : 1); pass++) {
                 if (MODE == 1 && pass == 1) {
                     __syncthreads();
                     for (int g = lane; g < ngl; g += 64) {
                         int acc = sh.run[g];
                         for (int q = 0; q < 64; q++) {
-                            const int c = sh.rowcnt[q][g];
-                            sh.rowcnt[q][g] = acc;
+                            const int c = rowcnt_dyn[q * ngl_max + g];
+                            rowcnt_dyn[q * ngl_max + g] = acc;
                             acc += c;
                         }
                         sh.run[g] = acc;
@@ -400,9 +430,10 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
                         int g = cmap[(ti * nt + tj) * nt + sh.typ3[b]];
                         if (g < 0) g = ngl - 1;
                         if (MODE == 0) atomicAdd(&sh.cnt[g], 1);
-                        else if (pass == 0) sh.rowcnt[lane][g]++;
+                        else if (pass == 0) rowcnt[g]++;
                         else {
-                            const long pos = offv[(long)(gbase + g) * N + i] + sh.rowcnt[lane][g]++;
+                            const long gg = gbase + g;
+                            const long pos = gbv[gg] + (offv[gg * N + i] - offv[gg * N]) + rowcnt[g]++;
                             list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10));
                         }
                     }
@@ -438,9 +469,11 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
                                     int g = cmap[((ti * nt + tj) * nt + tk) * nt + sh.typ3[aw]];
                                     if (g < 0) g = ngl - 1;
                                     if (MODE == 0) atomicAdd(&sh.cnt[g], 1);
-                                    else if (pass == 0) sh.rowcnt[lane][g]++;
+                                    else if (pass == 0) rowcnt[g]++;
                                     else {
-                                        const long pos = offv[(long)(gbase + g) * N + i] + sh.rowcnt[lane][g]++;
+                                        const long gg = gbase + g;
+                                        const long pos =
+                                            gbv[gg] + (offv[gg * N + i] - offv[gg * N]) + rowcnt[g]++;
                                         list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10) | ((l + 1) << 20));
                                     }
                                 }
@@ -464,31 +497,15 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
 }
 
 template __global__ void k_nbr<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *);
+                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, int *);
 template __global__ void k_nbr<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *);
+                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, int *);
 template __global__ void k_enum<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
                                    const chx_entry_s *, const chx_entry_n *, const chx_enum_model *, int *, int *,
-                                   const int *, const int *, int2 *, int2 *, unsigned char *, int *);
+                                   const int *, const int *, const int *, const int *, int2 *, int2 *, int *, int);
 template __global__ void k_enum<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
                                    const chx_entry_s *, const chx_entry_n *, const chx_enum_model *, int *, int *,
-                                   const int *, const int *, int2 *, int2 *, unsigned char *, int *);
-
-// Reverse neighbour lists: entries grouped by parent atom.
-__global__ void k_rev_count(long nent, const chx_entry_s *__restrict__ es, int *rcnt)
-{
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < nent) atomicAdd(&rcnt[es[e].parent], 1);
-}
-
-__global__ void k_rev_fill(long nent, const chx_entry_s *__restrict__ es, const int *__restrict__ rstart, int *cursor,
-                           int *rev)
-{
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nent) return;
-    const int p = es[e].parent;
-    rev[rstart[p] + atomicAdd(&cursor[p], 1)] = (int)e;
-}
+                                   const int *, const int *, const int *, const int *, int2 *, int2 *, int *, int);
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of int32 (out[n] = total), 2048 elements per block.
@@ -529,12 +546,6 @@ __global__ void k_scan_add(int *out, long n, const int *bscan)
 __global__ void k_scan_total(int *out, long n, const int *in)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = (n > 0 ? out[n - 1] + in[n - 1] : 0);
-}
-
-__global__ void k_gather_strided(const int *in, long stride, int count, int *out)
-{
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g <= count) out[g] = in[(long)g * stride];
 }
 
 }  // namespace chx
