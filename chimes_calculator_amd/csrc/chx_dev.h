@@ -94,6 +94,105 @@ __device__ __forceinline__ void fx_owner(unsigned long long *acc, int owner, con
     }
 }
 
+// Adds a 3-vector per lane into acc[key] (acc[(3*key + k)*2 + {0,1}]) for
+// groups of W consecutive lanes (W = 16 or 64) whose keys come in runs: the
+// integer parts are summed over each run of equal keys (a segmented scan,
+// exact), and only the last lane of a run issues the atomics.  key < 0
+// contributes nothing.  Must be reached by all lanes of the wavefront.
+template <int W>
+__device__ __forceinline__ void fx_runs3(unsigned long long *acc, int key, const double (&v)[3], bool &bad)
+{
+    long long hi[3], lo[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        hi[k] = 0;
+        lo[k] = 0;
+        if (key >= 0) fx_split(v[k], hi[k], lo[k], bad);
+    }
+    const int l = threadIdx.x & (W - 1);
+    const int kprev = __shfl_up(key, 1, W);
+    const int knext = __shfl_down(key, 1, W);
+    int seg = (l == 0 || kprev != key) ? l : 0;  // first lane of this lane's run
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+        const int t = __shfl_up(seg, d, W);
+        if (l >= d && t > seg) seg = t;
+    }
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+        long long th[3], tl[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            th[k] = __shfl_up(hi[k], d, W);
+            tl[k] = __shfl_up(lo[k], d, W);
+        }
+        if (l - d >= seg)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                hi[k] += th[k];
+                lo[k] += tl[k];
+            }
+    }
+    if (key >= 0 && (l == W - 1 || knext != key)) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            unsigned long long *a = acc + 2L * (3L * key + k);
+            if (hi[k]) atomicAdd(a, (unsigned long long)hi[k]);
+            if (lo[k]) atomicAdd(a + 1, (unsigned long long)lo[k]);
+        }
+    }
+}
+
+// Per-workgroup LDS table that merges the force contributions of a tile by
+// atom before they reach global memory: open addressing on the atom index,
+// exact integer LDS atomics, then one global atomic pair per component and
+// distinct atom.  CAP must exceed the number of contributions per flush.
+template <int CAP>
+struct fx_table {
+    int key[CAP];
+    unsigned long long val[CAP][6];  // (hi, lo) per component
+
+    // all threads of the block; a barrier must follow before use
+    __device__ __forceinline__ void clear()
+    {
+        for (int s = threadIdx.x; s < CAP; s += blockDim.x) {
+            key[s] = -1;
+#pragma unroll
+            for (int k = 0; k < 6; k++) val[s][k] = 0ull;
+        }
+    }
+    __device__ __forceinline__ void add(int atom, const double (&v)[3], bool &bad)
+    {
+        if (atom < 0) return;
+        unsigned h = ((unsigned)atom * 2654435761u) >> 8;
+        int s = (int)(h & (CAP - 1));
+        for (int probe = 0; probe < CAP; probe++) {
+            const int prev = atomicCAS(&key[s], -1, atom);
+            if (prev == -1 || prev == atom) break;
+            s = (s + 1) & (CAP - 1);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (v[k] == 0.0) continue;
+            long long hi, lo;
+            fx_split(v[k], hi, lo, bad);
+            if (hi) atomicAdd(&val[s][2 * k], (unsigned long long)hi);
+            if (lo) atomicAdd(&val[s][2 * k + 1], (unsigned long long)lo);
+        }
+    }
+    // all threads of the block, after a barrier
+    __device__ __forceinline__ void flush(unsigned long long *acc)
+    {
+        for (int s = threadIdx.x; s < CAP; s += blockDim.x) {
+            const int a = key[s];
+            if (a < 0) continue;
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+                if (val[s][k]) atomicAdd(acc + 6L * a + k, val[s][k]);
+        }
+    }
+};
+
 // Exact running total of normalised accumulators: lo parts are reduced below
 // 2^40 before summing, so up to 2^23 accumulators add without overflow.
 struct fx_sum {

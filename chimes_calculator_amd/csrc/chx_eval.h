@@ -34,6 +34,8 @@ void launch_eval2(long cap, hipStream_t st, const int *e_owner, const chx_cfg *c
                   unsigned long long *Oacc, int *cfg_stat, int *status);
 // Vector-unit kernel for nb = 3 or 4 bodies (4-body only for D > CHX_MFMA_MAX_D).
 void launch_evalv(int nb, int D, bool mixed, const chx_list_args &la, hipStream_t st);
+// Matrix-core 3-body kernel; false if D is not handled by it.
+bool launch_eval3m(int D, bool mixed, const chx_list_args &la, hipStream_t st);
 // Matrix-core 4-body kernel; false if D is not handled by it.
 bool launch_eval4m(int D, bool mixed, const chx_list_args &la, hipStream_t st);
 void launch_clusters(int nb, int D, int n, hipStream_t st, const double *dist, const double *drv, const int *types,

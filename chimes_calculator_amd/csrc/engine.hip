@@ -173,7 +173,7 @@ struct TypeLaunch {
     chx_slot s[6];
     const double *C = nullptr;   // device, FP64 dense coefficients
     const float *Cf = nullptr;   // device, FP32 copy for the mixed path
-    const double *Cm = nullptr;  // device, 4-body matrix image (D <= CHX_MFMA_MAX_D)
+    const double *Cm = nullptr;  // device, matrix image for the matrix-core kernels
     const float *Cmf = nullptr;
 };
 
@@ -195,6 +195,18 @@ std::vector<double> quad_matrix_image(const double *C, int D)
                             const size_t k = (size_t)(a * D + b) * 4 + c, col = (size_t)(d * D + e) * 4 + f;
                             m[k * MP + col] = C[src];
                         }
+    return m;
+}
+
+// Matrix image of one 3-body coefficient tensor: Cm3[a][16 b + c], a padded to
+// a multiple of 4, c to 16, zero padding.
+std::vector<double> trip_matrix_image(const double *C, int D)
+{
+    const int KP = (D + 3) / 4 * 4, MP = 16 * D;
+    std::vector<double> m((size_t)KP * MP, 0.0);
+    for (int a = 0; a < D; a++)
+        for (int b = 0; b < D; b++)
+            for (int c = 0; c < D; c++) m[(size_t)a * MP + 16 * b + c] = C[((size_t)a * D + b) * D + c];
     return m;
 }
 
@@ -377,6 +389,12 @@ chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
                 for (int q = 0; q < 3; q++) m->t3[t].s[q] = s3[3 * t + q];
                 m->t3[t].C = v.C3 + t * cs;
                 m->t3[t].Cf = Cf + t * cs;
+                if (d->D3 <= CHX_MFMA3_MAX_D) {
+                    const std::vector<double> img = trip_matrix_image(d->C3 + t * cs, d->D3);
+                    const std::vector<float> imgf(img.begin(), img.end());
+                    m->t3[t].Cm = m->up(img.data(), img.size());
+                    m->t3[t].Cmf = m->up(imgf.data(), imgf.size());
+                }
             }
             m->tripmap_h.assign(d->tripmap, d->tripmap + nt3);
         }
@@ -613,7 +631,7 @@ void run_pipeline(chx_engine *e, const double *pos_d, const int *typ_d, bool mix
             hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off4.p, N, G4, e->l4_cap, e->gb4.p,
                                e->ge4.p, e->status.p, (int)CHX_ST_N4, (int)CHX_RETRY_L4);
         }
-        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), (size_t)64 * ngl_max * sizeof(int), st,
+        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st,
                            (int)N, e->cfgs.p, e->as.p, e->an.p, e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p,
                            e->cnt4.p, e->off3.p, e->off4.p, e->gb3.p, e->gb4.p, e->list3.p, e->list4.p,
                            e->status.p, ngl_max);
@@ -645,7 +663,8 @@ void run_pipeline(chx_engine *e, const double *pos_d, const int *typ_d, bool mix
         l3.gend = e->ge3.p;
         l3.groups = e->grp3.p;
         l3.list = e->list3.p;
-        for (int D : D3s) launch_evalv(3, D, mixed, l3, st);
+        for (int D : D3s)
+            if (!launch_eval3m(D, mixed, l3, st)) launch_evalv(3, D, mixed, l3, st);
     }
     (void)hipEventRecord(e->ev[3], st);
     if (G4) {

@@ -76,8 +76,8 @@ __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owne
     if (block_stop(status, CHX_STOP)) return;
     const long nent = status[CHX_ST_NENT];
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int owner = -1;
-    double q[7] = {0, 0, 0, 0, 0, 0, 0};
+    int owner = -1, k1 = -1;
+    double q[7] = {0, 0, 0, 0, 0, 0, 0}, F0[3] = {0, 0, 0}, F1[3] = {0, 0, 0};
     bool bad = false;
     if (e < nent) {
         const int i = e_owner[e];
@@ -92,13 +92,9 @@ __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owne
         int flags = 0;
         eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
                   M.pen_dist, M.pen_scale, E, fs, flags);
-        if (fs != 0.0) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                fx_atomic(Facc + 2L * (3L * i + k), fs * dr[k], bad);
-                fx_atomic(Facc + 2L * (3L * g.parent + k), -(fs * dr[k]), bad);
-            }
-        }
+        F0[0] = fs * dr[0]; F0[1] = fs * dr[1]; F0[2] = fs * dr[2];
+        F1[0] = -F0[0]; F1[1] = -F0[1]; F1[2] = -F0[2];
+        k1 = g.parent;
         owner = i;
         q[0] = E;
         q[1] = -(fs * (dr[0] * dr[0]));
@@ -109,6 +105,8 @@ __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owne
         q[6] = -(fs * (dr[2] * dr[2]));
         note_flags(cfg_stat, ai.cfg, flags);
     }
+    fx_runs3<64>(Facc, owner, F0, bad);
+    fx_runs3<64>(Facc, k1, F1, bad);
     fx_owner<7>(Oacc, owner, q, bad);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
 }
@@ -140,9 +138,8 @@ __device__ __forceinline__ void load_members(const int2 rec, const chx_atom_s *_
 // runtime member perm[a] (the first matching permutation, chimesFF.h:480-526).
 template <int NB>
 __device__ __forceinline__ void canonical(const int *__restrict__ perm_tab, int tup, double (&p)[NB][3],
-                                          int (&par)[NB])
+                                          int (&par)[NB], int (&pm)[NB])
 {
-    int pm[NB];
 #pragma unroll
     for (int a = 0; a < NB; a++) pm[a] = perm_tab[tup * NB + a];
     double q[NB][3];
@@ -183,13 +180,15 @@ struct pairs_of<4> {
 };
 
 // Cluster geometry in canonical order: members, pair vectors and distances,
-// and whether every pair is inside its cutoff.
+// and whether every pair is inside its cutoff; canonical member a is runtime
+// member pm[a] (runtime member 0 is the owner).
 template <int NB>
 __device__ __forceinline__ bool cluster_geometry(const chx_group &grp, const int2 rec,
                                                  const chx_cfg *__restrict__ cfgs,
                                                  const chx_atom_s *__restrict__ as, const int *__restrict__ nbr_off,
                                                  const chx_entry_s *__restrict__ es, double (&p)[NB][3],
-                                                 int (&par)[NB], double (&r)[pairs_of<NB>::P], int &cfg)
+                                                 int (&par)[NB], double (&r)[pairs_of<NB>::P], int &cfg,
+                                                 int (&pm)[NB])
 {
     using PQ = pairs_of<NB>;
     int t[NB];
@@ -197,7 +196,7 @@ __device__ __forceinline__ bool cluster_geometry(const chx_group &grp, const int
     int tup = 0;
 #pragma unroll
     for (int a = 0; a < NB; a++) tup = tup * grp.nt + t[a];
-    canonical<NB>(grp.perm, tup, p, par);
+    canonical<NB>(grp.perm, tup, p, par, pm);
     const chx_cfg &C = cfgs[cfg];
     bool in = true;
 #pragma unroll
@@ -210,15 +209,14 @@ __device__ __forceinline__ bool cluster_geometry(const chx_group &grp, const int
 }
 
 // Forces of one cluster from its pair force scalars: F_a += fs dr, F_b -= fs
-// dr for pair (a, b); virial -= fs dr dr.  Member m's force goes to its
-// replicated atom, the virial to the q7 owner accumulator slots 1..6.
+// dr for pair (a, b) of canonical members; virial -= fs dr dr into the q7
+// owner accumulator slots 1..6.
 template <int NB>
-__device__ __forceinline__ void cluster_forces(const double *h, const double (&p)[NB][3], const int (&par)[NB],
-                                               const double (&fs)[pairs_of<NB>::P], unsigned long long *Facc,
-                                               double (&q7)[7], bool &bad)
+__device__ __forceinline__ void cluster_forces(const double *h, const double (&p)[NB][3],
+                                               const double (&fs)[pairs_of<NB>::P], double (&F)[NB][3],
+                                               double (&q7)[7])
 {
     using PQ = pairs_of<NB>;
-    double F[NB][3];
 #pragma unroll
     for (int m = 0; m < NB; m++) F[m][0] = F[m][1] = F[m][2] = 0.0;
 #pragma unroll
@@ -238,10 +236,6 @@ __device__ __forceinline__ void cluster_forces(const double *h, const double (&p
         q7[5] -= f * (dr[1] * dr[2]);
         q7[6] -= f * (dr[2] * dr[2]);
     }
-#pragma unroll
-    for (int m = 0; m < NB; m++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) fx_atomic(Facc + 2L * (3L * par[m] + k), F[m][k], bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -261,21 +255,29 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
     constexpr int P = PQ::P;
     if (block_stop(status, CHX_STOP)) return;
     const int lane = threadIdx.x & 63;
-    const long t0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    // wave index made explicitly uniform, so the group, its slots and its
+    // coefficients are read through the scalar unit
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const long t0 = ((long)blockIdx.x * 4 + wave) * 64;
     if (t0 >= status[total_word]) return;
     const int g = find_group(gbase, G, t0);
     const chx_group &grp = groups[g];
     if (grp.type < 0 || grp.D != D) return;
-    if (NB == 4 && D <= CHX_MFMA_MAX_D) return;  // matrix-core kernel
+    if (NB == 4 && D <= CHX_MFMA_MAX_D) return;  // matrix-core kernels
+    if (NB == 3 && D <= CHX_MFMA3_MAX_D) return;
     const long c = t0 + lane;
     int owner = -1;
     double q7[7] = {0, 0, 0, 0, 0, 0, 0};
+    double Fr[NB][3];   // force on each runtime member
+    int key[NB];        // replicated atom of each runtime member, -1 none
+#pragma unroll
+    for (int m = 0; m < NB; m++) { key[m] = -1; Fr[m][0] = Fr[m][1] = Fr[m][2] = 0.0; }
     bool bad = false;
     if (c < gend[g]) {
         const int2 rec = list[c];
         double p[NB][3], r[P];
-        int par[NB], cfg;
-        if (cluster_geometry<NB>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg)) {
+        int par[NB], cfg, pm[NB];
+        if (cluster_geometry<NB>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, pm)) {
             R T[P][D], dT[P][D];
             double f[P], fd[P];
             int flags = 0;
@@ -290,11 +292,22 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
             else contract4<D, R, double>(Cd, T, dT, S, Gq);
             cluster_scalars<P, double>(r, f, fd, S, Gq, E, fs);
             q7[0] = E;
-            cluster_forces<NB>(cfgs[cfg].h, p, par, fs, Facc, q7, bad);
+            double F[NB][3];
+            cluster_forces<NB>(cfgs[cfg].h, p, fs, F, q7);
+#pragma unroll
+            for (int a = 0; a < NB; a++)
+#pragma unroll
+                for (int m = 0; m < NB; m++)
+                    if (pm[a] == m) {
+                        key[m] = par[a];
+                        Fr[m][0] = F[a][0]; Fr[m][1] = F[a][1]; Fr[m][2] = F[a][2];
+                    }
             owner = rec.x;
             note_flags(cfg_stat, cfg, flags);
         }
     }
+#pragma unroll
+    for (int m = 0; m < NB; m++) fx_runs3<64>(Facc, key[m], Fr[m], bad);
     fx_owner<7>(Oacc, owner, q7, bad);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
 }
@@ -338,6 +351,19 @@ struct mfma_acc<float> {
     }
 };
 
+// LDS layout of a KP x MP matrix image read as A operands (16 consecutive
+// columns by each group of 16 lanes, four consecutive rows per wavefront):
+// rows whose stride is a multiple of 64 elements get their 16-column blocks
+// XOR-swizzled by (row & 3); other strides are padded to 16 or 48 mod 64, so
+// the four rows of one read land in different banks.
+template <int KP, int MP>
+struct lds_mat {
+    static constexpr bool swz = MP % 64 == 0;
+    static constexpr int LD = swz ? MP : ((MP % 64 == 16 || MP % 64 == 48) ? MP : MP + (64 + 16 - MP % 64) % 64);
+    static constexpr int size = KP * LD;
+    __device__ __forceinline__ static int at(int k, int m) { return swz ? k * MP + (m ^ ((k & 3) << 4)) : k * LD + m; }
+};
+
 template <int D>
 struct mfma_shape {
     static constexpr int KP = D * D * 4;                // rows of Cm (k)
@@ -359,13 +385,19 @@ __global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict_
     using SH = mfma_shape<D>;
     using MA = mfma_acc<R>;
     typedef typename MA::type acc_t;
-    constexpr int CLD = SH::MP + 16;  // LDS row of the matrix image (padding spreads the four k rows over banks)
+    using LM = lds_mat<SH::KP, SH::MP>;
     constexpr int TS = 6 * D + 1;     // per-cluster stride of the staged polynomials (odd: conflict-free)
-    __shared__ R Cl[SH::KP * CLD];
-    __shared__ R Ts[4][16 * TS], dTs[4][16 * TS];
+    __shared__ R Cl[LM::size];
+    // staged polynomials; after the contraction the same memory holds the force table
+    union stage_t {
+        struct { R T[4][16 * TS], dT[4][16 * TS]; } p;
+        fx_table<256> tab;
+    };
+    __shared__ stage_t st;
+    R (&Ts)[4][16 * TS] = st.p.T;
+    R (&dTs)[4][16 * TS] = st.p.dT;
     __shared__ double Gm[4][16][6][3];  // r, fcut, fcut' per pair
     __shared__ double Pm[4][16][4][3];  // member positions (evaluation-frame fractional)
-    __shared__ int Pa[4][16][4];        // member replicated atoms
     if (block_stop(status, CHX_STOP)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int j = lane & 15, sub = lane >> 4;
@@ -379,29 +411,32 @@ __global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict_
         if (grp.type < 0 || grp.D != D) return;
         {
             const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
-            for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 256) Cl[(x / SH::MP) * CLD + x % SH::MP] = src[x];
+            for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 256) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
         }
         // ---- stage: geometry (all four lanes of a cluster), polynomials of pairs sub and sub + 4
         const long c = t0 + wv * 16 + j;
         bool act = c < gend[g];
         int2 rec = make_int2(-1, 0);
         int cfg = 0, flags = 0;
+        int a_s = 0, key_s = -1;  // canonical index and replicated atom of runtime member sub
         {
             double p[4][3], r[6];
-            int par[4];
+            int par[4], perm[4];
             if (act) {
                 rec = list[c];
-                act = cluster_geometry<4>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg);
+                act = cluster_geometry<4>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, perm);
             }
             double pm[3] = {p[0][0], p[0][1], p[0][2]};
-            int am = par[0];
 #pragma unroll
             for (int m = 1; m < 4; m++)
-                if (sub == m) { pm[0] = p[m][0]; pm[1] = p[m][1]; pm[2] = p[m][2]; am = par[m]; }
+                if (sub == m) { pm[0] = p[m][0]; pm[1] = p[m][1]; pm[2] = p[m][2]; }
             Pm[wv][j][sub][0] = pm[0];
             Pm[wv][j][sub][1] = pm[1];
             Pm[wv][j][sub][2] = pm[2];
-            Pa[wv][j][sub] = am;
+            if (act)
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+                    if (perm[a] == sub) { a_s = a; key_s = par[a]; }
             for (int q = sub; q < 6; q += 4) {
                 double rq = r[0];
 #pragma unroll
@@ -448,7 +483,7 @@ __global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict_
 #pragma unroll
             for (int s = 0; s < SH::KS; s++) {
                 const int a = s / D, b = s % D;
-                const R A = Cl[(4 * s + sub) * CLD + 16 * rb + j];
+                const R A = Cl[LM::at(4 * s + sub, 16 * rb + j)];
                 const R x01 = T0[a] * T1[b];
                 z0 = MA::op(A, x01 * t2, z0);
                 z1 = MA::op(A, (dT0[a] * T1[b]) * t2, z1);
@@ -508,8 +543,12 @@ __global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict_
             for (int q = 0; q < 6; q++) Gs[q] += __shfl_xor(Gs[q], w, 64);
             flags |= __shfl_xor(flags, w, 64);
         }
-        // ---- forces (lane sub: member sub, virial of pairs sub and sub + 4), energy, virial
+        // ---- forces (lane sub: runtime member sub, virial of pairs sub and sub + 4), energy, virial
+        __syncthreads();
+        st.tab.clear();
+        __syncthreads();
         double q7[7] = {0, 0, 0, 0, 0, 0, 0};
+        double F[3] = {0.0, 0.0, 0.0};
         bool bad = false;
         int owner = -1;
         if (act) {
@@ -523,18 +562,17 @@ __global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict_
             }
             cluster_scalars<6, double>(r, f, fd, (double)S, Gd, E, fs);
             const double *h = cfgs[cfg].h;
-            double F[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 const int pa = pairs_of<4>::a[q], pb = pairs_of<4>::b[q];
-                const bool mine_f = pa == sub || pb == sub;
+                const bool mine_f = pa == a_s || pb == a_s;
                 const bool mine_v = (q & 3) == sub;
                 if (!mine_f && !mine_v) continue;
                 double dr[3];
                 pair_vec(h, Pm[wv][j][pa], Pm[wv][j][pb], dr);
                 const double fq = fs[q];
                 if (mine_f) {
-                    const double sg = pa == sub ? 1.0 : -1.0;
+                    const double sg = pa == a_s ? 1.0 : -1.0;
                     F[0] += sg * (fq * dr[0]);
                     F[1] += sg * (fq * dr[1]);
                     F[2] += sg * (fq * dr[2]);
@@ -548,18 +586,216 @@ __global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict_
                     q7[6] -= fq * (dr[2] * dr[2]);
                 }
             }
-            const int pm = Pa[wv][j][sub];
-#pragma unroll
-            for (int k = 0; k < 3; k++) fx_atomic(Facc + 2L * (3L * pm + k), F[k], bad);
             if (sub == 0) {
                 q7[0] = E;
                 note_flags(cfg_stat, cfg, flags);
             }
             owner = rec.x;
         }
+        st.tab.add(act ? key_s : -1, F, bad);
         fx_owner<7>(Oacc, owner, q7, bad);
+        __syncthreads();
+        st.tab.flush(Facc);
         if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
     }
+}
+
+// ---------------------------------------------------------------------------
+// 3-body on the matrix cores (D <= 16): Z_v[(b,c)][cluster] = sum_a
+// Cm3[a][(b,c)] X_v[cluster][a] with X_0 = T_01 and X_1 = dT_01, folded
+// against T_02[b] T_12[c] and its derivatives.  Matrix image Cm3[a][16 b + c]
+// (a padded to a multiple of 4, c to 16, zero padding).  Same tiling as the
+// 4-body kernel: lane (j = l % 16, sub = l / 16); output row m = 16 b + c with
+// c = sub + 4v (FP64) or 4 sub + v (FP32).
+// ---------------------------------------------------------------------------
+template <int D>
+struct mfma3_shape {
+    static constexpr int KP = (D + 3) / 4 * 4;  // rows of Cm3 (a)
+    static constexpr int MP = 16 * D;           // columns of Cm3 (b, c)
+    static constexpr int KS = KP / 4;
+    static constexpr int RB = D;
+};
+
+template <int D, typename R>
+__global__ __launch_bounds__(256, 2) void k_eval3m(int G, const int *__restrict__ gbase,
+                                                    const int *__restrict__ gend,
+                                                    const chx_group *__restrict__ groups,
+                                                    const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
+                                                    const chx_atom_s *__restrict__ as,
+                                                    const int *__restrict__ nbr_off,
+                                                    const chx_entry_s *__restrict__ es, unsigned long long *Facc,
+                                                    unsigned long long *Oacc, int *cfg_stat, int *status)
+{
+    using SH = mfma3_shape<D>;
+    using MA = mfma_acc<R>;
+    typedef typename MA::type acc_t;
+    using LM = lds_mat<SH::KP, SH::MP>;
+    // staged polynomials per cluster: pair 01 [KP], pair 02 [D], pair 12 [16]; odd stride
+    constexpr int O1 = SH::KP, O2 = SH::KP + D, TS = (SH::KP + D + 16) | 1;
+    __shared__ R Cl[LM::size];
+    // staged polynomials; after the contraction the same memory holds the force table
+    union stage_t {
+        struct { R T[4][16 * TS], dT[4][16 * TS]; } p;
+        fx_table<256> tab;
+    };
+    __shared__ stage_t st;
+    R (&Ts)[4][16 * TS] = st.p.T;
+    R (&dTs)[4][16 * TS] = st.p.dT;
+    __shared__ double Gm[4][16][3][3];  // r, fcut, fcut' per pair
+    __shared__ double Pm[4][16][3][3];  // member positions
+    if (block_stop(status, CHX_STOP)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = lane & 15, sub = lane >> 4;
+    const long nbt = status[CHX_ST_N3] / 64;
+    const long bt = blockIdx.x;
+    if (bt >= nbt) return;
+    const long t0 = bt * 64;
+    const int g = find_group(gbase, G, t0);
+    const chx_group &grp = groups[g];
+    if (grp.type < 0 || grp.D != D) return;
+    {
+        const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
+        for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 256) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
+    }
+    // ---- stage: geometry (all lanes of a cluster), lane sub < 3 the polynomials of pair sub
+    const long c = t0 + wv * 16 + j;
+    bool act = c < gend[g];
+    int2 rec = make_int2(-1, 0);
+    int cfg = 0, flags = 0;
+    int a_s = 0, key_s = -1;  // canonical index and replicated atom of runtime member sub
+    {
+        double p[3][3], r[3];
+        int par[3], perm[3];
+        if (act) {
+            rec = list[c];
+            act = cluster_geometry<3>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, perm);
+        }
+        if (act)
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                if (perm[a] == sub) { a_s = a; key_s = par[a]; }
+        if (sub < 3) {
+            double pm[3] = {p[0][0], p[0][1], p[0][2]};
+            double rq = r[0];
+#pragma unroll
+            for (int m = 1; m < 3; m++)
+                if (sub == m) { pm[0] = p[m][0]; pm[1] = p[m][1]; pm[2] = p[m][2]; rq = r[m]; }
+            Pm[wv][j][sub][0] = pm[0];
+            Pm[wv][j][sub][1] = pm[1];
+            Pm[wv][j][sub][2] = pm[2];
+            R Tq[D], dTq[D];
+            double fq = 0.0, fdq = 0.0;
+            if (act) {
+                const chx_slot sl = grp.s[sub];
+                cheby<D, R>(rq, sl, Tq, dTq, flags);
+                fcut(rq, sl.rout, grp.tersoff, grp.fcut_var, fq, fdq);
+            } else {
+#pragma unroll
+                for (int x = 0; x < D; x++) { Tq[x] = (R)0; dTq[x] = (R)0; }
+            }
+            const int o = sub == 0 ? 0 : (sub == 1 ? O1 : O2);
+            const int w = sub == 0 ? SH::KP : (sub == 1 ? D : 16);
+#pragma unroll
+            for (int x = 0; x < 16; x++)
+                if (x < w) {
+                    Ts[wv][j * TS + o + x] = x < D ? Tq[x < D ? x : 0] : (R)0;
+                    dTs[wv][j * TS + o + x] = x < D ? dTq[x < D ? x : 0] : (R)0;
+                }
+            Gm[wv][j][sub][0] = rq;
+            Gm[wv][j][sub][1] = fq;
+            Gm[wv][j][sub][2] = fdq;
+        }
+    }
+    __syncthreads();
+    // ---- contraction on the matrix cores
+    const R *Tj = &Ts[wv][j * TS], *dTj = &dTs[wv][j * TS];
+    R t2s[4], dt2s[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int cc = MA::f64 ? sub + 4 * v : 4 * sub + v;
+        t2s[v] = Tj[O2 + cc];
+        dt2s[v] = dTj[O2 + cc];
+    }
+    R S = 0, Gs[3] = {0, 0, 0};
+#pragma unroll
+    for (int rb = 0; rb < SH::RB; rb++) {
+        acc_t z0 = {0, 0, 0, 0}, z1 = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < SH::KS; s++) {
+            const R A = Cl[LM::at(4 * s + sub, 16 * rb + j)];
+            z0 = MA::op(A, Tj[4 * s + sub], z0);
+            z1 = MA::op(A, dTj[4 * s + sub], z1);
+        }
+        const R t1 = Tj[O1 + rb], dt1 = dTj[O1 + rb];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const R y = t1 * t2s[v];
+            S += z0[v] * y;
+            Gs[0] += z1[v] * y;
+            Gs[1] += z0[v] * (dt1 * t2s[v]);
+            Gs[2] += z0[v] * (t1 * dt2s[v]);
+        }
+    }
+#pragma unroll
+    for (int w = 16; w <= 32; w <<= 1) {
+        S += __shfl_xor(S, w, 64);
+#pragma unroll
+        for (int q = 0; q < 3; q++) Gs[q] += __shfl_xor(Gs[q], w, 64);
+        flags |= __shfl_xor(flags, w, 64);
+    }
+    // ---- forces (lane sub < 3: runtime member sub and the virial of pair sub), energy, virial
+    __syncthreads();
+    st.tab.clear();
+    __syncthreads();
+    double q7[7] = {0, 0, 0, 0, 0, 0, 0};
+    double F[3] = {0.0, 0.0, 0.0};
+    bool bad = false;
+    int owner = -1;
+    if (act && sub < 3) {
+        double r[3], f[3], fd[3], E, fs[3], Gd[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            r[q] = Gm[wv][j][q][0];
+            f[q] = Gm[wv][j][q][1];
+            fd[q] = Gm[wv][j][q][2];
+            Gd[q] = (double)Gs[q];
+        }
+        cluster_scalars<3, double>(r, f, fd, (double)S, Gd, E, fs);
+        const double *h = cfgs[cfg].h;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int pa = pairs_of<3>::a[q], pb = pairs_of<3>::b[q];
+            const bool mine_f = pa == a_s || pb == a_s;
+            const bool mine_v = q == sub;
+            double dr[3];
+            pair_vec(h, Pm[wv][j][pa], Pm[wv][j][pb], dr);
+            const double fq = fs[q];
+            if (mine_f) {
+                const double sg = pa == a_s ? 1.0 : -1.0;
+                F[0] += sg * (fq * dr[0]);
+                F[1] += sg * (fq * dr[1]);
+                F[2] += sg * (fq * dr[2]);
+            }
+            if (mine_v) {
+                q7[1] -= fq * (dr[0] * dr[0]);
+                q7[2] -= fq * (dr[0] * dr[1]);
+                q7[3] -= fq * (dr[0] * dr[2]);
+                q7[4] -= fq * (dr[1] * dr[1]);
+                q7[5] -= fq * (dr[1] * dr[2]);
+                q7[6] -= fq * (dr[2] * dr[2]);
+            }
+        }
+        if (sub == 0) {
+            q7[0] = E;
+            note_flags(cfg_stat, cfg, flags);
+        }
+        owner = rec.x;
+    }
+    st.tab.add((act && sub < 3) ? key_s : -1, F, bad);
+    fx_owner<7>(Oacc, owner, q7, bad);
+    __syncthreads();
+    st.tab.flush(Facc);
+    if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
 }
 
 // ---------------------------------------------------------------------------
@@ -797,6 +1033,28 @@ void launch_evalv(int nb, int D, bool mixed, const chx_list_args &la, hipStream_
 #undef NB_
     }
 #undef X
+}
+
+bool launch_eval3m(int D, bool mixed, const chx_list_args &la, hipStream_t st)
+{
+    if (D > CHX_MFMA3_MAX_D) return false;
+    if (la.cap <= 0 || la.G <= 0) return true;
+    const dim3 grid((unsigned)std::max<long>(1, la.cap / 64)), block(256);
+#define X(d)                                                                                                    \
+    if (D == d) {                                                                                               \
+        if (mixed)                                                                                              \
+            hipLaunchKernelGGL((k_eval3m<d, float>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups,     \
+                               la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,        \
+                               la.status);                                                                      \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_eval3m<d, double>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups,    \
+                               la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,        \
+                               la.status);                                                                      \
+        return true;                                                                                            \
+    }
+    CHX_D3_CASES(X)
+#undef X
+    return false;
 }
 
 bool launch_eval4m(int D, bool mixed, const chx_list_args &la, hipStream_t st)

@@ -270,8 +270,9 @@ __global__ void k_group_bases(const int *__restrict__ off, long N, int G, long c
 // ---------------------------------------------------------------------------
 // Cluster enumeration over the sorted images, one 64-lane workgroup per owner.
 // MODE 0 counts clusters per (model, cluster type) group; MODE 1 writes the
-// records (owner, packed slot codes) and flags the images within the 3-body
-// cutoff.  Predicates (serial_chimes_interface.cpp:749-857, with the max 4B
+// records (owner, packed slot codes) into the owner's part of each group
+// segment (in any order: every later sum is exact, and exported lists are
+// sorted).  Predicates (serial_chimes_interface.cpp:749-857, with the max 4B
 // cutoff <= the max 3B cutoff): j, k in N2(i), j != k, parent(j) <=
 // parent(k), all three distances < rc3; quadruplets add l in N2(i), l not in
 // {j, k}, parent(j), parent(k) <= parent(l), all six distances < rc4.
@@ -290,7 +291,6 @@ struct EnumShared {
     int gst4[CHX_MAX_N4];
     unsigned A4[CHX_MAX_N4][EW4];
     int cnt[CHX_MAX_CLUSTER_TYPES + 1];
-    int run[CHX_MAX_CLUSTER_TYPES + 1];
     int n3, n4;
 };
 
@@ -305,7 +305,6 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
                                              int2 *list3, int2 *list4, int *status, int ngl_max)
 {
     __shared__ EnumShared sh;
-    extern __shared__ int rowcnt_dyn[];  // [64][ngl_max] per-row counts of the ordered write
     if (block_stop(status, CHX_STOP)) return;
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
@@ -370,7 +369,7 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
         sh.gst4[u] = g;
         for (int w = 0; w < EW4; w++) sh.A4[u][w] = 0u;
     }
-    for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) { sh.cnt[g] = 0; sh.run[g] = 0; }
+    for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) sh.cnt[g] = 0;
     __syncthreads();
     if (any4) {
         const int npair = n4 * (n4 - 1) / 2;
@@ -401,23 +400,7 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
         const int nrow = body == 3 ? n3 : n4;
         for (int round = 0; round * 64 < nrow; round++) {
             const int r = round * 64 + lane;
-            int *rowcnt = rowcnt_dyn + lane * ngl_max;
-            if (MODE == 1)
-                for (int g = 0; g < ngl; g++) rowcnt[g] = 0;
-            for (int pass = 0; pass < (MODE == 1 ? 2 : 1); pass++) {
-                if (MODE == 1 && pass == 1) {
-                    __syncthreads();
-                    for (int g = lane; g < ngl; g += 64) {
-                        int acc = sh.run[g];
-                        for (int q = 0; q < 64; q++) {
-                            const int c = rowcnt_dyn[q * ngl_max + g];
-                            rowcnt_dyn[q * ngl_max + g] = acc;
-                            acc += c;
-                        }
-                        sh.run[g] = acc;
-                    }
-                    __syncthreads();
-                }
+            {
                 if (r >= nrow) continue;
                 if (body == 3) {
                     const int a = r;
@@ -429,11 +412,10 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
                         const int k = sh.idx3[b];
                         int g = cmap[(ti * nt + tj) * nt + sh.typ3[b]];
                         if (g < 0) g = ngl - 1;
-                        if (MODE == 0) atomicAdd(&sh.cnt[g], 1);
-                        else if (pass == 0) rowcnt[g]++;
-                        else {
+                        const int slot = atomicAdd(&sh.cnt[g], 1);
+                        if (MODE == 1) {
                             const long gg = gbase + g;
-                            const long pos = gbv[gg] + (offv[gg * N + i] - offv[gg * N]) + rowcnt[g]++;
+                            const long pos = gbv[gg] + (offv[gg * N + i] - offv[gg * N]) + slot;
                             list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10));
                         }
                     }
@@ -468,12 +450,10 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
                                     const int l = sh.idx3[aw];
                                     int g = cmap[((ti * nt + tj) * nt + tk) * nt + sh.typ3[aw]];
                                     if (g < 0) g = ngl - 1;
-                                    if (MODE == 0) atomicAdd(&sh.cnt[g], 1);
-                                    else if (pass == 0) rowcnt[g]++;
-                                    else {
+                                    const int slot = atomicAdd(&sh.cnt[g], 1);
+                                    if (MODE == 1) {
                                         const long gg = gbase + g;
-                                        const long pos =
-                                            gbv[gg] + (offv[gg * N + i] - offv[gg * N]) + rowcnt[g]++;
+                                        const long pos = gbv[gg] + (offv[gg * N + i] - offv[gg * N]) + slot;
                                         list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10) | ((l + 1) << 20));
                                     }
                                 }
@@ -485,13 +465,10 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
             __syncthreads();
         }
         __syncthreads();
-        if (MODE == 0)
-            for (int g = lane; g < ngl; g += 64) {
-                cntv[(long)(gbase + g) * N + i] = sh.cnt[g];
-                sh.cnt[g] = 0;
-            }
-        else
-            for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) sh.run[g] = 0;
+        for (int g = lane; g < ngl; g += 64) {
+            if (MODE == 0) cntv[(long)(gbase + g) * N + i] = sh.cnt[g];
+            sh.cnt[g] = 0;
+        }
         __syncthreads();
     }
 }
