@@ -1,13 +1,22 @@
 """Benchmark: atom-evaluations per second of the ChIMES evaluation path.
 
-Default workload: the plain FP64 single-system case, 1536 O/H atoms (512
-H2O-like molecules, L = 24.8 A), 2B+3B+4B at orders 12/8/4, rc 5.0/4.5/4.0 A.
-One step = one full evaluation (ghost images, cluster lists, 2/3/4-body
-energies, forces and stress) with inputs already resident in GPU memory.
+Default workload (--workload C1): the plain FP64 single-system case, 1536
+O/H atoms (512 H2O-like molecules, L = 24.8 A), 2B+3B+4B at orders 12/8/4,
+rc 5.0/4.5/4.0 A.  One step = one full evaluation (ghost images, cluster
+lists, 2/3/4-body energies, forces and stress) with inputs already resident
+in GPU memory.  Prints one JSON line with the metric, a roofline entry for
+the dominant kernel (the 4-body evaluation, algorithmic FLOPs by the term
+model) and the CPU baseline (the oracle's C restatement on a sample of the
+same workload).
 
-Prints one JSON line with the metric, a roofline entry for the dominant
-kernel (the 4-body evaluation, algorithmic FLOPs by the term model) and the
-CPU baseline (the oracle's C restatement on a sample of the same workload).
+Other forms (one JSON line each):
+  --workload C2  4096 uneven C/O configurations: the plain FP64 path (one
+                 drop-in calculate() per configuration) against batch plans
+                 (one captured-graph replay per batch), FP64 and FP32-mixed.
+  --workload C3  32768-atom carbon, orders 16/10/4: FP64 and FP32-mixed, with
+                 the error of the mixed result per quantity.
+  --workload C4  Poisson request stream served with continuous batching:
+                 p50/p99 latency and throughput, FP64 and FP32-mixed.
 """
 import argparse
 import ctypes as C
@@ -98,6 +107,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4"], default="C1")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
@@ -105,7 +115,16 @@ def main():
         print("bench.py measures one GPU: every configuration of this workload fits one MI355X and the "
               "multi-GPU decomposition is not part of this build (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "C2":
+        return bench_c2(args)
+    if args.workload == "C3":
+        return bench_c3(args)
+    if args.workload == "C4":
+        return bench_c4(args)
+    bench_c1(args)
 
+
+def bench_c1(args):
     import chimes_calculator_amd as cc
     from chimes_calculator_amd import workloads as W
 
@@ -189,6 +208,120 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pf, cfgs[0])
     print(json.dumps(out))
+
+
+def atoms_of(cfgs):
+    return sum(len(c[0]) for c in cfgs)
+
+
+def bench_c2(args):
+    """4096 uneven configurations: plain FP64 drop-in path per configuration
+    against batch plans replayed from one captured graph."""
+    import chimes_calculator_amd as cc
+    from chimes_calculator_amd import workloads as W
+
+    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
+    mname, cfgs = W.workload("C2")
+    pf = W.MODELS[mname].write(os.path.join(ROOT, "_work", "bench_%s.txt" % mname))
+    model = cc.GpuModel(pf)
+    natoms = atoms_of(cfgs)
+    out = {"metric": "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency",
+           "unit": "atom-evals/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic: seeded C/O configurations and seeded ChIMES model",
+           "config": {"workload": W.WORKLOADS["C2"], "nconf": len(cfgs), "natoms": natoms}}
+    # plain FP64 GPU path: the drop-in calculate() once per configuration (host buffers)
+    s = cc.ChimesSerial(small=False)
+    s.init_chimes(pf)
+    for p, t, c in cfgs[:8]:
+        s.calculate_chimes(len(p), p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), t, c[0], c[1], c[2])
+    t0 = time.perf_counter()
+    for p, t, c in cfgs:
+        s.calculate_chimes(len(p), p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), t, c[0], c[1], c[2])
+    plain = natoms / (time.perf_counter() - t0)
+    s.close()
+    out["plain_fp64_per_config"] = plain
+    # batch plans (host buffers in and out: positions cross PCIe inside the timed replay)
+    forms = {}
+    ref = None
+    for prec, tag in ((cc.FP64, "fp64"), (cc.MIXED, "mixed")):
+        plan = cc.GpuPlan(model, cfgs, precision=prec, tune=True)
+        _, pos, types, cells = cc._flatten(cfgs, plan.model_of, [model])
+        E, F, S = np.zeros(len(cfgs)), np.zeros(pos.shape[0]), np.zeros(9 * len(cfgs))
+        for _ in range(args.warmup):
+            plan.run_flat(pos, types, cells, E, F, S)
+        ts = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            plan.run_flat(pos, types, cells, E, F, S)
+            ts.append(time.perf_counter() - t0)
+        v3, v4, tms, counts = plan.info()
+        forms[tag] = {"value": natoms / np.median(ts), "ms_per_batch": 1e3 * float(np.median(ts)),
+                      "p99_ms": 1e3 * float(np.percentile(ts, 99)), "variant3": v3, "variant4": v4,
+                      "counts": counts}
+        if prec == cc.FP64:
+            ref = (E.copy(), F.copy())
+        else:
+            e_err = np.abs(E - ref[0]) * 0.0433641 / np.array([len(c[0]) for c in cfgs])
+            forms[tag]["energy_err_eV_per_atom_max"] = float(e_err.max())
+            forms[tag]["force_err_eV_per_A_max"] = float(np.abs(F - ref[1]).max() * 0.0433641)
+        plan.close()
+    out["forms"] = forms
+    out["value"] = forms["mixed"]["value"]
+    out["ms_per_step"] = forms["mixed"]["ms_per_batch"]
+    out["steps"], out["warmup"] = args.steps, args.warmup
+    out["dtype"] = "f32-mixed"
+    out["speedup_mixed_plan_vs_plain_fp64"] = forms["mixed"]["value"] / plain
+    out["speedup_fp64_plan_vs_plain_fp64"] = forms["fp64"]["value"] / plain
+    print(json.dumps(out))
+
+
+def bench_c3(args):
+    """32768-atom carbon at orders 16/10/4, FP64 and FP32-mixed."""
+    import chimes_calculator_amd as cc
+    from chimes_calculator_amd import workloads as W
+
+    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
+    mname, cfgs = W.workload("C3")
+    pf = W.MODELS[mname].write(os.path.join(ROOT, "_work", "bench_%s.txt" % mname))
+    model = cc.GpuModel(pf)
+    n = atoms_of(cfgs)
+    out = {"metric": "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency",
+           "unit": "atom-evals/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic: seeded jittered carbon lattice and seeded ChIMES model",
+           "config": {"workload": W.WORKLOADS["C3"], "natoms": n}}
+    res = {}
+    for prec, tag in ((cc.FP64, "fp64"), (cc.MIXED, "mixed")):
+        plan = cc.GpuPlan(model, cfgs, precision=prec, tune=True)
+        _, pos, types, cells = cc._flatten(cfgs, plan.model_of, [model])
+        E, F, S = np.zeros(1), np.zeros(pos.shape[0]), np.zeros(9)
+        for _ in range(args.warmup):
+            plan.run_flat(pos, types, cells, E, F, S)
+        ts = []
+        for _ in range(max(3, args.steps // 4)):
+            t0 = time.perf_counter()
+            plan.run_flat(pos, types, cells, E, F, S)
+            ts.append(time.perf_counter() - t0)
+        v3, v4, tms, counts = plan.info()
+        res[tag] = {"value": n / np.median(ts), "ms_per_step": 1e3 * float(np.median(ts)), "variant3": v3,
+                    "variant4": v4, "counts": counts, "E": E[0], "F": F.copy(), "S": S.copy()}
+        plan.close()
+    a, b = res["fp64"], res["mixed"]
+    err = {"energy_eV_per_atom": abs(b["E"] - a["E"]) * 0.0433641 / n,
+           "force_max_eV_per_A": float(np.abs(b["F"] - a["F"]).max() * 0.0433641),
+           "force_rms_eV_per_A": float(np.sqrt(np.mean((b["F"] - a["F"]) ** 2)) * 0.0433641),
+           "stress_max_rel": float(np.abs(b["S"] - a["S"]).max() / np.abs(a["S"]).max())}
+    for r in res.values():
+        del r["E"], r["F"], r["S"]
+    out["forms"] = res
+    out["mixed_error_vs_fp64"] = err
+    out["value"] = res["mixed"]["value"]
+    out["ms_per_step"] = res["mixed"]["ms_per_step"]
+    out["dtype"] = "f32-mixed"
+    print(json.dumps(out))
+
+
+def bench_c4(args):
+    raise SystemExit("C4 serving benchmark: see chimes_calculator_amd.serving")
 
 
 if __name__ == "__main__":

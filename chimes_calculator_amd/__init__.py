@@ -38,8 +38,10 @@ EXPORTS = {
     "chimes_gpu.h": ["chimes_gpu_model_load", "chimes_gpu_model_free", "chimes_gpu_last_error",
                      "chimes_gpu_model_ntypes", "chimes_gpu_model_type_index", "chimes_gpu_model_order",
                      "chimes_gpu_model_max_cutoff", "chimes_gpu_eval_batch", "chimes_gpu_eval_batch_device",
-                     "chimes_gpu_last_timing", "chimes_gpu_last_counts", "chimes_gpu_compute_clusters", "chimes_gpu_cluster_lists",
-                     "chimes_gpu_free"],
+                     "chimes_gpu_plan_create", "chimes_gpu_plan_run", "chimes_gpu_plan_create_device",
+                     "chimes_gpu_plan_run_device", "chimes_gpu_plan_info", "chimes_gpu_plan_free",
+                     "chimes_gpu_last_timing", "chimes_gpu_last_counts", "chimes_gpu_compute_clusters",
+                     "chimes_gpu_cluster_lists", "chimes_gpu_free"],
 }
 
 
@@ -78,6 +80,18 @@ def lib():
         L.chimes_gpu_eval_batch_device.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, C.c_void_p,
                                                    C.c_void_p, _dp, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                                    C.c_void_p]
+        L.chimes_gpu_plan_create.restype = C.c_void_p
+        L.chimes_gpu_plan_create.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _dp, _ip, _dp, C.c_int,
+                                             C.c_int, C.c_int, _dp, _dp, _dp]
+        L.chimes_gpu_plan_run.argtypes = [C.c_void_p, _dp, _ip, _dp, _dp, _dp, _dp]
+        L.chimes_gpu_plan_create_device.restype = C.c_void_p
+        L.chimes_gpu_plan_create_device.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, C.c_void_p,
+                                                    C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p]
+        L.chimes_gpu_plan_run_device.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.chimes_gpu_plan_info.argtypes = [C.c_void_p, _ip, _dp,
+                                           np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+        L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
         L.chimes_gpu_last_timing.argtypes = [C.c_void_p, _dp]
         L.chimes_gpu_last_counts.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.chimes_gpu_compute_clusters.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp]
@@ -194,6 +208,62 @@ def eval_batch(models, configs, model_of=None, small=False, precision=FP64):
     F = F.reshape(-1, 3)
     offs = np.concatenate([[0], np.cumsum(natoms)])
     return E, [F[offs[k]:offs[k + 1]] for k in range(nconf)], S.reshape(nconf, 9)
+
+
+VARIANTS = ("mfma64", "mfma32", "vector")
+
+
+class GpuPlan:
+    """A batch plan (chimes_gpu.h): one batch shape evaluated by replaying a
+    captured HIP graph.  Construction evaluates `configs` once (see
+    `first`); run() takes new configurations of the same shape."""
+
+    def __init__(self, models, configs, model_of=None, small=False, precision=FP64, tune=True):
+        self.L = lib()
+        if isinstance(models, GpuModel):
+            models = [models]
+        self.models = models
+        self.nconf = len(configs)
+        self.model_of = np.zeros(self.nconf, np.int32) if model_of is None else np.asarray(model_of, np.int32)
+        self.natoms, pos, types, cells = _flatten(configs, self.model_of, models)
+        self._arr = (C.c_void_p * len(models))(*[m.h for m in models])
+        E, F, S = np.zeros(self.nconf), np.zeros(pos.shape[0]), np.zeros(9 * self.nconf)
+        self.h = self.L.chimes_gpu_plan_create(self._arr, len(models), self.nconf, self.model_of, self.natoms, pos,
+                                               types, cells, int(small), int(precision), int(bool(tune)), E, F, S)
+        if not self.h:
+            raise ChimesError(_err())
+        self.first = self._split(E, F, S)
+
+    def _split(self, E, F, S):
+        F = F.reshape(-1, 3)
+        offs = np.concatenate([[0], np.cumsum(self.natoms)])
+        return E, [F[offs[k]:offs[k + 1]] for k in range(self.nconf)], S.reshape(self.nconf, 9)
+
+    def run(self, configs):
+        _, pos, types, cells = _flatten(configs, self.model_of, self.models)
+        E, F, S = np.zeros(self.nconf), np.zeros(pos.shape[0]), np.zeros(9 * self.nconf)
+        if self.L.chimes_gpu_plan_run(self.h, pos, types, cells, E, F, S):
+            raise ChimesError(_err())
+        return self._split(E, F, S)
+
+    def run_flat(self, pos, types, cells, E, F, S):
+        """Replays on pre-flattened arrays (pos[3n], types[n], cells[9c]) into E[c], F[3n], S[9c]."""
+        if self.L.chimes_gpu_plan_run(self.h, pos, types, cells, E, F, S):
+            raise ChimesError(_err())
+
+    def info(self):
+        """(3-body variant, 4-body variant, tuning ms per (body, variant), (pairs, triplets, quads))."""
+        v, t, c = np.zeros(2, np.int32), np.zeros(6), np.zeros(3, np.int64)
+        self.L.chimes_gpu_plan_info(self.h, v, t, c)
+        return VARIANTS[v[0]], VARIANTS[v[1]], t.reshape(2, 3), tuple(int(x) for x in c)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.chimes_gpu_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class ChimesSerial:

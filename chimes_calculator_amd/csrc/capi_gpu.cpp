@@ -16,6 +16,19 @@ struct chimes_gpu_model {
     std::unique_ptr<chx::DeviceModel> dm;
 };
 
+struct chimes_gpu_plan {
+    chx_plan *p = nullptr;
+    int nconf = 0;
+    long natoms = 0;
+    std::vector<int> nrep;
+    std::vector<double> vol, V;
+    int out_dev = 0;
+    ~chimes_gpu_plan()
+    {
+        if (p) chx_plan_destroy(p);
+    }
+};
+
 namespace {
 
 thread_local std::string g_err;
@@ -37,6 +50,36 @@ chx::EngineHandle *common_engine(chimes_gpu_model *const *models, int nmodels, s
         ms[k] = models[k]->dm->m;
     }
     return eh;
+}
+
+// Energy / (R+1)^3 and the 9-component stress (virial / volume) of each
+// configuration from the engine's raw outputs.
+void finish_host(int nconf, const int *nrep, const double *vol, const double *V, double *energy, double *stress)
+{
+    for (int c = 0; c < nconf; c++) {
+        energy[c] /= pow(nrep[c] + 1.0, 3.0);
+        const double *v = &V[6 * c];
+        double *s = stress + 9 * c;
+        s[0] = v[0]; s[1] = v[1]; s[2] = v[2];
+        s[3] = v[1]; s[4] = v[3]; s[5] = v[4];
+        s[6] = v[2]; s[7] = v[4]; s[8] = v[5];
+        for (int k = 0; k < 9; k++) s[k] /= vol[c];
+    }
+}
+
+int check_types(chimes_gpu_model *const *models, int nmodels, int nconf, const int *model_of, const int *natoms,
+                const int *types)
+{
+    for (int c = 0; c < nconf; c++)
+        if (model_of[c] < 0 || model_of[c] >= nmodels) return fail("model index out of range");
+    if (!types) return 0;
+    long a = 0;
+    for (int c = 0; c < nconf; c++) {
+        const int nt = models[model_of[c]]->dm->P.natmtyps;
+        for (int k = 0; k < natoms[c]; k++, a++)
+            if (types[a] < 0 || types[a] >= nt) return fail("atom type index out of range");
+    }
+    return 0;
 }
 
 void sort_rows(int *a, long n, int w)
@@ -91,14 +134,7 @@ int chimes_gpu_eval_batch(chimes_gpu_model *const *models, int nmodels, int ncon
     std::vector<chx_model *> ms;
     chx::EngineHandle *eh = common_engine(models, nmodels, ms);
     if (!eh) return fail("models must be loaded on the same device");
-    for (int c = 0; c < nconf; c++)
-        if (model_of[c] < 0 || model_of[c] >= nmodels) return fail("model index out of range");
-    long a = 0;
-    for (int c = 0; c < nconf; c++) {
-        const int nt = models[model_of[c]]->dm->P.natmtyps;
-        for (int k = 0; k < natoms[c]; k++, a++)
-            if (types[a] < 0 || types[a] >= nt) return fail("atom type index out of range");
-    }
+    if (check_types(models, nmodels, nconf, model_of, natoms, types)) return 1;
     chx_batch b{};
     b.nconf = nconf;
     b.natoms = natoms;
@@ -120,15 +156,7 @@ int chimes_gpu_eval_batch(chimes_gpu_model *const *models, int nmodels, int ncon
         std::lock_guard<std::mutex> lk(eh->mu);
         if (chx_evaluate(eh->eng, ms.data(), nmodels, &b, &o)) return fail(chx_last_error(eh->eng));
     }
-    for (int c = 0; c < nconf; c++) {
-        energy[c] /= pow(nrep[c] + 1.0, 3.0);
-        const double *v = &V[6 * c];
-        double *s = stress + 9 * c;
-        s[0] = v[0]; s[1] = v[1]; s[2] = v[2];
-        s[3] = v[1]; s[4] = v[3]; s[5] = v[4];
-        s[6] = v[2]; s[7] = v[4]; s[8] = v[5];
-        for (int k = 0; k < 9; k++) s[k] /= vol[c];
-    }
+    finish_host(nconf, nrep.data(), vol.data(), V.data(), energy, stress);
     return 0;
 }
 
@@ -159,6 +187,111 @@ int chimes_gpu_eval_batch_device(chimes_gpu_model *const *models, int nmodels, i
     if (chx_evaluate(eh->eng, ms.data(), nmodels, &b, &o)) return fail(chx_last_error(eh->eng));
     return 0;
 }
+
+chimes_gpu_plan *chimes_gpu_plan_create(chimes_gpu_model *const *models, int nmodels, int nconf,
+                                        const int *model_of, const int *natoms, const double *pos, const int *types,
+                                        const double *cells, int small, int precision, int tune, double *energy,
+                                        double *forces, double *stress)
+{
+    std::vector<chx_model *> ms;
+    chx::EngineHandle *eh = common_engine(models, nmodels, ms);
+    if (!eh) { fail("models must be loaded on the same device"); return nullptr; }
+    if (check_types(models, nmodels, nconf, model_of, natoms, types)) return nullptr;
+    std::unique_ptr<chimes_gpu_plan> P(new chimes_gpu_plan);
+    P->nconf = nconf;
+    P->nrep.resize(nconf);
+    P->vol.resize(nconf);
+    P->V.resize(6 * (size_t)nconf);
+    chx_batch b{};
+    b.nconf = nconf;
+    b.natoms = natoms;
+    b.pos = pos;
+    b.types = types;
+    b.cells = cells;
+    b.model = model_of;
+    b.small = small;
+    b.precision = precision;
+    chx_out o{};
+    o.energy = energy;
+    o.forces = forces;
+    o.virial = P->V.data();
+    o.volume = P->vol.data();
+    o.n_rep = P->nrep.data();
+    char err[512];
+    P->p = chx_plan_create(models[0]->dm->device, ms.data(), nmodels, &b, tune, &o, err, sizeof(err));
+    if (!P->p) { fail(err); return nullptr; }
+    finish_host(nconf, P->nrep.data(), P->vol.data(), P->V.data(), energy, stress);
+    return P.release();
+}
+
+int chimes_gpu_plan_run(chimes_gpu_plan *P, const double *pos, const int *types, const double *cells,
+                        double *energy, double *forces, double *stress)
+{
+    if (!P || P->out_dev) return fail("not a host-buffer plan");
+    chx_out o{};
+    o.energy = energy;
+    o.forces = forces;
+    o.virial = P->V.data();
+    o.volume = P->vol.data();
+    o.n_rep = P->nrep.data();
+    if (chx_plan_run(P->p, pos, types, cells, &o)) return fail(chx_plan_error(P->p));
+    finish_host(P->nconf, P->nrep.data(), P->vol.data(), P->V.data(), energy, stress);
+    return 0;
+}
+
+chimes_gpu_plan *chimes_gpu_plan_create_device(chimes_gpu_model *const *models, int nmodels, int nconf,
+                                               const int *model_of, const int *natoms, const double *pos_dev,
+                                               const int *types_dev, const double *cells, int small, int precision,
+                                               int tune, double *energy_dev, double *forces_dev,
+                                               double *virial_dev)
+{
+    std::vector<chx_model *> ms;
+    chx::EngineHandle *eh = common_engine(models, nmodels, ms);
+    if (!eh) { fail("models must be loaded on the same device"); return nullptr; }
+    if (check_types(models, nmodels, nconf, model_of, natoms, nullptr)) return nullptr;
+    std::unique_ptr<chimes_gpu_plan> P(new chimes_gpu_plan);
+    P->nconf = nconf;
+    P->out_dev = 1;
+    chx_batch b{};
+    b.nconf = nconf;
+    b.natoms = natoms;
+    b.pos = pos_dev;
+    b.types = types_dev;
+    b.cells = cells;
+    b.model = model_of;
+    b.small = small;
+    b.pos_dev = 1;
+    b.precision = precision;
+    chx_out o{};
+    o.energy = energy_dev;
+    o.forces = forces_dev;
+    o.virial = virial_dev;
+    o.out_dev = 1;
+    char err[512];
+    P->p = chx_plan_create(models[0]->dm->device, ms.data(), nmodels, &b, tune, &o, err, sizeof(err));
+    if (!P->p) { fail(err); return nullptr; }
+    return P.release();
+}
+
+int chimes_gpu_plan_run_device(chimes_gpu_plan *P, const double *cells, double *energy_dev, double *forces_dev,
+                               double *virial_dev)
+{
+    if (!P || !P->out_dev) return fail("not a device-buffer plan");
+    chx_out o{};
+    o.energy = energy_dev;
+    o.forces = forces_dev;
+    o.virial = virial_dev;
+    o.out_dev = 1;
+    if (chx_plan_run(P->p, nullptr, nullptr, cells, &o)) return fail(chx_plan_error(P->p));
+    return 0;
+}
+
+void chimes_gpu_plan_info(const chimes_gpu_plan *P, int variants[2], double tune_ms[6], long long counts[3])
+{
+    chx_plan_info(P->p, variants, tune_ms, counts);
+}
+
+void chimes_gpu_plan_free(chimes_gpu_plan *P) { delete P; }
 
 void chimes_gpu_last_timing(const chimes_gpu_model *m, double ms[4])
 {

@@ -93,6 +93,24 @@ int chx_eval_clusters(chx_engine *e, chx_model *m, int nb, int n, const double *
                       const int *types, double *energy, double *force, double *stress, double *last,
                       int *flags);
 
+// Batch plans: a batch shape (models, model_of, natoms, small, precision and,
+// when pos_dev / out_dev are set, the device buffers) fixed at creation.
+// Creation evaluates b once (results in o), optionally times the 3-/4-body
+// kernel variants on it and keeps the fastest (tune != 0), and captures the
+// whole pipeline - input copy, kernels, output copy - in one HIP graph.  Each
+// run replays the graph for new positions, types and cells; a run whose
+// shape or cluster counts no longer fit the captured sizes is evaluated
+// directly and captured again.  A plan owns its engine (stream and buffers).
+typedef struct chx_plan chx_plan;
+chx_plan *chx_plan_create(int device, chx_model *const *models, int nmodels, const chx_batch *b, int tune,
+                          chx_out *o, char *err, int errlen);
+int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double *cells, chx_out *o);
+void chx_plan_destroy(chx_plan *p);
+const char *chx_plan_error(chx_plan *p);
+// variants[2]: chosen variant of the 3- and 4-body kernels; tune_ms[6]: the
+// timed kernel milliseconds per (body, variant); counts[3]: as chx_last_counts.
+void chx_plan_info(chx_plan *p, int *variants, double *tune_ms, long long *counts);
+
 // Cluster counts of the last evaluate call: [0] pair entries, [1] triplets,
 // [2] quadruplets (all configurations, excluded types included).
 void chx_last_counts(chx_engine *e, long long *c3);

@@ -32,12 +32,11 @@ bool supported_D4(int D);
 void launch_eval2(long cap, hipStream_t st, const int *e_owner, const chx_cfg *cfgs, const chx_atom_s *as,
                   const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
                   unsigned long long *Oacc, int *cfg_stat, int *status);
-// Vector-unit kernel for nb = 3 or 4 bodies (4-body only for D > CHX_MFMA_MAX_D).
+// Vector-unit kernel for nb = 3 or 4 bodies.
 void launch_evalv(int nb, int D, bool mixed, const chx_list_args &la, hipStream_t st);
-// Matrix-core 3-body kernel; false if D is not handled by it.
-bool launch_eval3m(int D, bool mixed, const chx_list_args &la, hipStream_t st);
-// Matrix-core 4-body kernel; false if D is not handled by it.
-bool launch_eval4m(int D, bool mixed, const chx_list_args &la, hipStream_t st);
+// nb-body evaluation with the given kernel variant (CHX_VAR_*); the vector
+// kernel stands in where the matrix-core kernels do not take D.
+void launch_eval_body(int nb, int D, bool mixed, int variant, const chx_list_args &la, hipStream_t st);
 void launch_clusters(int nb, int D, int n, hipStream_t st, const double *dist, const double *drv, const int *types,
                      const chx_model_dev &M, double *E, double *F, double *S, double *last, int *flags);
 

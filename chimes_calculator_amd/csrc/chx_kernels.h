@@ -38,6 +38,8 @@ __global__ void k_enum(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, co
                        int2 *list3, int2 *list4, int *status, int ngl_max);
 __global__ void k_scan_block(const int *in, int *out, long n, int *bsum);
 __global__ void k_scan_add(int *out, long n, const int *bscan);
+__global__ void k_scan_one(const int *in, int *out, long n);
+__global__ void k_clear(const chx_clear_range *r, int nr);
 __global__ void k_scan_total(int *out, long n, const int *in);
 
 }  // namespace chx

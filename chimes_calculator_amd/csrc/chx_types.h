@@ -33,6 +33,18 @@ enum {
     CHX_STOP = CHX_BAD_NB | CHX_BAD_N34 | CHX_RETRY_ANY
 };
 
+// A range of ints zeroed by the clear kernel at the start of a run.
+struct chx_clear_range {
+    int *p;
+    long n;
+};
+
+// 3-/4-body evaluation kernel variants (chosen per body order by the plan
+// autotuner): matrix cores with 4 or 2 wavefronts per workgroup (64 or 32
+// clusters per tile), or the vector units.
+enum { CHX_VAR_MFMA4 = 0, CHX_VAR_MFMA2 = 1, CHX_VAR_VECTOR = 2, CHX_NVAR = 3 };
+#define CHX_SCAN1_MAX 32768    // scans up to this length run in one workgroup
+
 // Per-configuration frame data (one per configuration in a batch).
 struct chx_cfg {
     double h[9], ih[9];        // evaluation frame: replicated cell and inverse

@@ -233,6 +233,28 @@ struct chx_model {
     }
 };
 
+// Page-locked host buffer (hipHostMalloc), grown on demand.
+struct PinnedBuf {
+    char *p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t n)
+    {
+        if (n <= cap && p) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        const size_t c = std::max<size_t>(n + n / 4, 4096);
+        HIPCHK(hipHostMalloc((void **)&p, c, hipHostMallocDefault));
+        cap = c;
+    }
+};
+
 struct chx_engine {
     int device = 0;
     hipStream_t st = nullptr;
@@ -247,28 +269,46 @@ struct chx_engine {
     int G3 = 0, G4 = 0;
     std::vector<int> g3b, g4b;
     std::vector<chx_model *> models;
+    std::vector<chx_enum_model> emh;
+    std::vector<chx_model_dev> mdh;
+    std::vector<chx_group> g3h, g4h;
+    std::vector<int> D3s, D4s;
+    int ngl_max = 1;
+    bool mixed = false;
+    // kernel variants of the 3-/4-body evaluation (chosen by the plan autotuner)
+    int var3 = CHX_VAR_MFMA4, var4 = CHX_VAR_MFMA4;
+    bool capturing = false;  // inside a graph capture: no event records
     // buffer capacities (entries, list slots); grow on demand
     long ent_cap = 0, l3_cap = 0, l4_cap = 0;
-    // device buffers
-    DevBuf<chx_cfg> cfgs;
-    DevBuf<int> cfg_atom_off, cfg_in_off;
-    DevBuf<double> pos;
-    DevBuf<int> types;
+    // one host-to-device and one device-to-host copy per run
+    PinnedBuf in_h, out_h;
+    DevBuf<char> in_d, out_d;
+    size_t in_bytes = 0, out_bytes = 0;
+    size_t o_stat = 0, o_cstat = 0, o_E = 0, o_V = 0, o_F = 0;  // offsets in the output blob
+    // device pointers into the input blob
+    chx_cfg *cfgs = nullptr;
+    int *cfg_atom_off = nullptr, *cfg_in_off = nullptr;
+    chx_enum_model *em = nullptr;
+    chx_model_dev *mdev = nullptr;
+    chx_group *grp3 = nullptr, *grp4 = nullptr;
+    chx_clear_range *clear = nullptr;
+    int nclear = 0;
+    double *pos_in = nullptr;
+    int *types_in = nullptr;
+    // device pointers into the output blob
+    int *status = nullptr, *cfg_stat = nullptr;
+    double *E_out = nullptr, *V_out = nullptr, *F_out = nullptr;
+    // work buffers
     DevBuf<chx_atom_s> as;
     DevBuf<chx_atom_n> an;
     DevBuf<int> cell_cnt, cell_start, cell_cur, cell_atoms;
     DevBuf<int> nbr_cnt, nbr_off, e_owner;
     DevBuf<chx_entry_s> es;
     DevBuf<chx_entry_n> en;
-    DevBuf<int> cnt3, off3, cnt4, off4, status;
+    DevBuf<int> cnt3, off3, cnt4, off4;
     DevBuf<int> gb3, ge3, gb4, ge4;
-    DevBuf<chx_group> grp3, grp4;
     DevBuf<int2> list3, list4;
     DevBuf<unsigned long long> Facc, Oacc;
-    DevBuf<double> Fout, Ecfg, Vcfg;
-    DevBuf<int> cfg_stat;
-    DevBuf<chx_enum_model> em;
-    DevBuf<chx_model_dev> mdev;
     std::vector<DevBuf<int>> scan_tmp;
     // per-cluster API buffers
     DevBuf<double> cl_in, cl_out;
@@ -278,6 +318,10 @@ struct chx_engine {
     {
         if (scan_tmp.size() < 32) scan_tmp.resize(32);  // fixed: references below stay valid
         if (level >= 15) throw EngineError("scan too deep");
+        if (level == 0 && n <= CHX_SCAN1_MAX) {
+            hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, st, in, out, n);
+            return;
+        }
         const long nb = (n + 2047) / 2048;
         DevBuf<int> &bsum = scan_tmp[2 * level];
         DevBuf<int> &bscan = scan_tmp[2 * level + 1];
@@ -290,9 +334,9 @@ struct chx_engine {
         }
         hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, out, n, in);
     }
-    void zero_i(int *p, long n, int v = 0)
+    void record(int k)
     {
-        if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long>(nblk(n, 256), 4096)), dim3(256), 0, st, p, n, v);
+        if (!capturing) (void)hipEventRecord(ev[k], st);
     }
 };
 
@@ -586,104 +630,352 @@ void guess_capacities(chx_engine *e, chx_model *const *models)
     e->l4_cap = (e->l4_cap + 255) / 256 * 256;
 }
 
-// One pass of the pipeline on the engine's stream (no host synchronisation).
-void run_pipeline(chx_engine *e, const double *pos_d, const int *typ_d, bool mixed, int ngl_max,
-                  const std::vector<int> &D3s, const std::vector<int> &D4s, double *Fout, double *Ecfg,
-                  double *Vcfg)
+// Prepares a batch on the host: frames, groups, model tables and capacity
+// guesses (nothing is enqueued).
+void prepare(chx_engine *e, chx_model *const *models, int nmodels, const chx_batch *b)
+{
+    const int ncfg = b->nconf;
+    if (ncfg <= 0) throw EngineError("empty batch");
+    e->ncfg = ncfg;
+    e->models.assign(models, models + nmodels);
+    setup_frames(e, models, nmodels, b);
+    e->g3b.assign(nmodels, 0);
+    e->g4b.assign(nmodels, 0);
+    e->emh.assign(nmodels, chx_enum_model{});
+    e->mdh.assign(nmodels, chx_model_dev{});
+    e->g3h.clear();
+    e->g4h.clear();
+    e->D3s.clear();
+    e->D4s.clear();
+    e->ngl_max = 1;
+    for (int mi = 0; mi < nmodels; mi++) {
+        const chx_model *M = models[mi];
+        chx_enum_model &q = e->emh[mi];
+        q.nt = M->d.nt;
+        q.has3 = M->d.order[1] > 0 && M->d.ntrip > 0;
+        q.has4 = M->d.order[2] > 0 && M->d.nquad > 0;
+        q.ntrip = M->d.ntrip;
+        q.nquad = M->d.nquad;
+        q.g3 = (int)e->g3h.size();
+        q.g4 = (int)e->g4h.size();
+        q.tripmap = M->dev.tripmap;
+        q.quadmap = M->dev.quadmap;
+        e->g3b[mi] = q.g3;
+        e->g4b[mi] = q.g4;
+        for (int body = 3; body <= 4; body++) {
+            const bool has = body == 3 ? q.has3 : q.has4;
+            if (!has) continue;
+            const int nty = body == 3 ? M->d.ntrip : M->d.nquad;
+            const int D = body == 3 ? M->d.D3 : M->d.D4;
+            std::vector<chx_group> &gv = body == 3 ? e->g3h : e->g4h;
+            std::vector<int> &Ds = body == 3 ? e->D3s : e->D4s;
+            if (std::find(Ds.begin(), Ds.end(), D) == Ds.end()) Ds.push_back(D);
+            e->ngl_max = std::max(e->ngl_max, nty + 1);
+            for (int t = 0; t <= nty; t++) {
+                chx_group g{};
+                g.model = mi;
+                g.type = t < nty ? t : -1;
+                g.nt = M->d.nt;
+                g.D = D;
+                g.tersoff = M->d.tersoff;
+                g.fcut_var = M->d.fcut_var;
+                g.perm = body == 3 ? M->dev.trip_perm : M->dev.quad_perm;
+                if (t < nty) {
+                    const TypeLaunch &tl = body == 3 ? M->t3[t] : M->t4[t];
+                    g.C = tl.C;
+                    g.Cf = tl.Cf;
+                    g.Cm = tl.Cm;
+                    g.Cmf = tl.Cmf;
+                    for (int k = 0; k < (body == 3 ? 3 : 6); k++) g.s[k] = tl.s[k];
+                }
+                gv.push_back(g);
+            }
+        }
+        e->mdh[mi] = M->dev;
+    }
+    e->G3 = (int)e->g3h.size();
+    e->G4 = (int)e->g4h.size();
+    e->mixed = b->precision == CHX_MIXED;
+    guess_capacities(e, models);
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Sizes every buffer, lays out the input and output blobs and fills the host
+// input blob.  host_pos: positions/types are host arrays copied in the blob;
+// host_out: energy/virial/forces come back in the output blob.
+void layout(chx_engine *e, const chx_batch *b, bool host_out)
+{
+    const long N = e->N, Nin = e->Nin, ncell = e->ncell;
+    const int ncfg = e->ncfg, nmodels = (int)e->models.size(), G3 = e->G3, G4 = e->G4;
+    e->as.ensure(N);
+    e->an.ensure(N);
+    e->cell_cnt.ensure(ncell + 1);
+    e->cell_start.ensure(ncell + 1);
+    e->cell_cur.ensure(ncell + 1);
+    e->cell_atoms.ensure(N);
+    e->nbr_cnt.ensure(N + 1);
+    e->nbr_off.ensure(N + 1);
+    e->cnt3.ensure((long)G3 * N + 1);
+    e->off3.ensure((long)G3 * N + 2);
+    e->cnt4.ensure((long)G4 * N + 1);
+    e->off4.ensure((long)G4 * N + 2);
+    e->gb3.ensure(G3 + 1);
+    e->ge3.ensure(G3 + 1);
+    e->gb4.ensure(G4 + 1);
+    e->ge4.ensure(G4 + 1);
+    e->Facc.ensure(6 * N);
+    e->Oacc.ensure(14 * N);
+    // input blob
+    size_t off = 0;
+    const size_t o_cfg = off; off = align256(off + ncfg * sizeof(chx_cfg));
+    const size_t o_cao = off; off = align256(off + (ncfg + 1) * sizeof(int));
+    const size_t o_cio = off; off = align256(off + (ncfg + 1) * sizeof(int));
+    const size_t o_em = off; off = align256(off + nmodels * sizeof(chx_enum_model));
+    const size_t o_md = off; off = align256(off + nmodels * sizeof(chx_model_dev));
+    const size_t o_g3 = off; off = align256(off + (G3 + 1) * sizeof(chx_group));
+    const size_t o_g4 = off; off = align256(off + (G4 + 1) * sizeof(chx_group));
+    const size_t o_cl = off; off = align256(off + 8 * sizeof(chx_clear_range));
+    const bool host_pos = !b->pos_dev;
+    const size_t o_pos = off; if (host_pos) off = align256(off + 3 * Nin * sizeof(double));
+    const size_t o_typ = off; if (host_pos) off = align256(off + Nin * sizeof(int));
+    e->in_bytes = off;
+    e->in_h.ensure(off);
+    e->in_d.ensure(off);
+    char *hb = e->in_h.p, *db = e->in_d.p;
+    e->cfgs = (chx_cfg *)(db + o_cfg);
+    e->cfg_atom_off = (int *)(db + o_cao);
+    e->cfg_in_off = (int *)(db + o_cio);
+    e->em = (chx_enum_model *)(db + o_em);
+    e->mdev = (chx_model_dev *)(db + o_md);
+    e->grp3 = (chx_group *)(db + o_g3);
+    e->grp4 = (chx_group *)(db + o_g4);
+    e->clear = (chx_clear_range *)(db + o_cl);
+    e->pos_in = host_pos ? (double *)(db + o_pos) : const_cast<double *>(b->pos);
+    e->types_in = host_pos ? (int *)(db + o_typ) : const_cast<int *>(b->types);
+    memcpy(hb + o_cfg, e->cfg_h.data(), ncfg * sizeof(chx_cfg));
+    memcpy(hb + o_cao, e->cfg_atom_off_h.data(), (ncfg + 1) * sizeof(int));
+    memcpy(hb + o_cio, e->cfg_in_off_h.data(), (ncfg + 1) * sizeof(int));
+    memcpy(hb + o_em, e->emh.data(), nmodels * sizeof(chx_enum_model));
+    memcpy(hb + o_md, e->mdh.data(), nmodels * sizeof(chx_model_dev));
+    if (G3) memcpy(hb + o_g3, e->g3h.data(), G3 * sizeof(chx_group));
+    if (G4) memcpy(hb + o_g4, e->g4h.data(), G4 * sizeof(chx_group));
+    if (host_pos) {
+        memcpy(hb + o_pos, b->pos, 3 * Nin * sizeof(double));
+        memcpy(hb + o_typ, b->types, Nin * sizeof(int));
+    }
+    // output blob: status, per-configuration flags[, energy, virial, forces]
+    off = 0;
+    e->o_stat = off; off = align256(off + CHX_ST_WORDS * sizeof(int));
+    e->o_cstat = off; off = align256(off + 4 * ncfg * sizeof(int));
+    e->o_E = off; if (host_out) off = align256(off + ncfg * sizeof(double));
+    e->o_V = off; if (host_out) off = align256(off + 6 * ncfg * sizeof(double));
+    e->o_F = off; if (host_out) off = align256(off + 3 * Nin * sizeof(double));
+    e->out_bytes = off;
+    e->out_h.ensure(off);
+    e->out_d.ensure(off);
+    e->status = (int *)(e->out_d.p + e->o_stat);
+    e->cfg_stat = (int *)(e->out_d.p + e->o_cstat);
+    if (host_out) {
+        e->E_out = (double *)(e->out_d.p + e->o_E);
+        e->V_out = (double *)(e->out_d.p + e->o_V);
+        e->F_out = (double *)(e->out_d.p + e->o_F);
+    }
+}
+
+// Work buffers sized by the current capacities, and the clear list (the
+// ranges one kernel zeroes at the start of a run).
+void size_work(chx_engine *e)
+{
+    e->es.ensure(e->ent_cap);
+    e->en.ensure(e->ent_cap);
+    e->e_owner.ensure(e->ent_cap);
+    e->list3.ensure(e->l3_cap);
+    e->list4.ensure(e->l4_cap);
+    const long N = e->N;
+    chx_clear_range cl[8];
+    int n = 0;
+    cl[n++] = chx_clear_range{e->status, CHX_ST_WORDS};
+    cl[n++] = chx_clear_range{e->cfg_stat, 4L * e->ncfg};
+    cl[n++] = chx_clear_range{e->cell_cnt.p, (long)e->ncell + 1};
+    cl[n++] = chx_clear_range{e->cell_cur.p, (long)e->ncell + 1};
+    cl[n++] = chx_clear_range{e->cnt3.p, (long)e->G3 * N + 1};
+    cl[n++] = chx_clear_range{e->cnt4.p, (long)e->G4 * N + 1};
+    cl[n++] = chx_clear_range{(int *)e->Facc.p, 12 * N};
+    cl[n++] = chx_clear_range{(int *)e->Oacc.p, 28 * N};
+    e->nclear = n;
+    memcpy(e->in_h.p + ((char *)e->clear - e->in_d.p), cl, sizeof(cl));
+}
+
+// One pass of the pipeline on the engine's stream, between the blob copies
+// (no host synchronisation).
+void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
 {
     hipStream_t st = e->st;
     const long N = e->N, Nin = e->Nin, ncell = e->ncell;
-    const int ncfg = e->ncfg, G3 = e->G3, G4 = e->G4;
-    (void)hipEventRecord(e->ev[0], st);
-    e->zero_i(e->status.p, CHX_ST_WORDS);
-    e->zero_i(e->cell_cnt.p, ncell + 1);
-    e->zero_i(e->cell_cur.p, ncell + 1);
-    hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, ncfg, e->cfg_atom_off.p, e->cfgs.p,
-                       pos_d, typ_d, e->as.p, e->an.p, e->cell_cnt.p);
+    const int ncfg = e->ncfg, G3 = e->G3, G4 = e->G4, ngl_max = e->ngl_max;
+    const bool mixed = e->mixed;
+    e->record(0);
+    HIPCHK(hipMemcpyAsync(e->in_d.p, e->in_h.p, e->in_bytes, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, e->clear, e->nclear);
+    hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, ncfg, e->cfg_atom_off, e->cfgs,
+                       e->pos_in, e->types_in, e->as.p, e->an.p, e->cell_cnt.p);
     e->scan(e->cell_cnt.p, e->cell_start.p, ncell);
     hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, e->an.p, e->cell_start.p,
                        e->cell_cur.p, e->cell_atoms.p);
     // neighbour images, sorted per owner
-    hipLaunchKernelGGL((k_nbr<0>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
+    hipLaunchKernelGGL((k_nbr<0>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
                        e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr, (chx_entry_s *)nullptr,
-                       (chx_entry_n *)nullptr, (int *)nullptr, e->status.p);
+                       (chx_entry_n *)nullptr, (int *)nullptr, e->status);
     e->scan(e->nbr_cnt.p, e->nbr_off.p, N);
-    hipLaunchKernelGGL(k_check_entries, dim3(1), dim3(1), 0, st, e->nbr_off.p, (int)N, e->ent_cap, e->status.p);
-    hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
+    hipLaunchKernelGGL(k_check_entries, dim3(1), dim3(1), 0, st, e->nbr_off.p, (int)N, e->ent_cap, e->status);
+    hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
                        e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
-                       e->e_owner.p, e->status.p);
+                       e->e_owner.p, e->status);
     // 3-/4-body lists in type-aligned segments
     if (G3 + G4 > 0) {
-        const long n3c = (long)G3 * N + 1, n4c = (long)G4 * N + 1;
-        e->zero_i(e->cnt3.p, n3c);
-        e->zero_i(e->cnt4.p, n4c);
-        hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs.p, e->as.p, e->an.p,
-                           e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p, e->cnt4.p, (const int *)nullptr,
+        hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
+                           e->nbr_off.p, e->es.p, e->en.p, e->em, e->cnt3.p, e->cnt4.p, (const int *)nullptr,
                            (const int *)nullptr, (const int *)nullptr, (const int *)nullptr, (int2 *)nullptr,
-                           (int2 *)nullptr, e->status.p, ngl_max);
+                           (int2 *)nullptr, e->status, ngl_max);
         if (G3) {
             e->scan(e->cnt3.p, e->off3.p, (long)G3 * N);
             hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off3.p, N, G3, e->l3_cap, e->gb3.p,
-                               e->ge3.p, e->status.p, (int)CHX_ST_N3, (int)CHX_RETRY_L3);
+                               e->ge3.p, e->status, (int)CHX_ST_N3, (int)CHX_RETRY_L3);
         }
         if (G4) {
             e->scan(e->cnt4.p, e->off4.p, (long)G4 * N);
             hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off4.p, N, G4, e->l4_cap, e->gb4.p,
-                               e->ge4.p, e->status.p, (int)CHX_ST_N4, (int)CHX_RETRY_L4);
+                               e->ge4.p, e->status, (int)CHX_ST_N4, (int)CHX_RETRY_L4);
         }
-        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st,
-                           (int)N, e->cfgs.p, e->as.p, e->an.p, e->nbr_off.p, e->es.p, e->en.p, e->em.p, e->cnt3.p,
-                           e->cnt4.p, e->off3.p, e->off4.p, e->gb3.p, e->gb4.p, e->list3.p, e->list4.p,
-                           e->status.p, ngl_max);
+        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
+                           e->nbr_off.p, e->es.p, e->en.p, e->em, e->cnt3.p, e->cnt4.p, e->off3.p, e->off4.p,
+                           e->gb3.p, e->gb4.p, e->list3.p, e->list4.p, e->status, ngl_max);
     }
-    (void)hipEventRecord(e->ev[1], st);
+    e->record(1);
     // evaluation with exact accumulation
-    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long>(nblk(12 * N, 256), 4096)), dim3(256), 0, st,
-                       (int *)e->Facc.p, 12 * N, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long>(nblk(28 * N, 256), 4096)), dim3(256), 0, st,
-                       (int *)e->Oacc.p, 28 * N, 0);
-    e->zero_i(e->cfg_stat.p, 4 * ncfg);
-    launch_eval2(e->ent_cap, st, e->e_owner.p, e->cfgs.p, e->as.p, e->es.p, e->mdev.p, e->Facc.p, e->Oacc.p,
-                 e->cfg_stat.p, e->status.p);
-    (void)hipEventRecord(e->ev[2], st);
+    launch_eval2(e->ent_cap, st, e->e_owner.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p, e->Oacc.p,
+                 e->cfg_stat, e->status);
+    e->record(2);
     chx_list_args la{};
-    la.cfgs = e->cfgs.p;
+    la.cfgs = e->cfgs;
     la.as = e->as.p;
     la.nbr_off = e->nbr_off.p;
     la.es = e->es.p;
     la.Facc = e->Facc.p;
     la.Oacc = e->Oacc.p;
-    la.cfg_stat = e->cfg_stat.p;
-    la.status = e->status.p;
+    la.cfg_stat = e->cfg_stat;
+    la.status = e->status;
     if (G3) {
         chx_list_args l3 = la;
         l3.cap = e->l3_cap;
         l3.G = G3;
         l3.gbase = e->gb3.p;
         l3.gend = e->ge3.p;
-        l3.groups = e->grp3.p;
+        l3.groups = e->grp3;
         l3.list = e->list3.p;
-        for (int D : D3s)
-            if (!launch_eval3m(D, mixed, l3, st)) launch_evalv(3, D, mixed, l3, st);
+        for (int D : e->D3s) launch_eval_body(3, D, mixed, e->var3, l3, st);
     }
-    (void)hipEventRecord(e->ev[3], st);
+    e->record(3);
     if (G4) {
         chx_list_args l4 = la;
         l4.cap = e->l4_cap;
         l4.G = G4;
         l4.gbase = e->gb4.p;
         l4.gend = e->ge4.p;
-        l4.groups = e->grp4.p;
+        l4.groups = e->grp4;
         l4.list = e->list4.p;
-        for (int D : D4s)
-            if (!launch_eval4m(D, mixed, l4, st)) launch_evalv(4, D, mixed, l4, st);
+        for (int D : e->D4s) launch_eval_body(4, D, mixed, e->var4, l4, st);
     }
-    (void)hipEventRecord(e->ev[4], st);
-    hipLaunchKernelGGL(k_fin_forces, dim3(nblk(Nin, 256)), dim3(256), 0, st, (int)Nin, ncfg, e->cfg_in_off.p,
-                       e->cfgs.p, e->Facc.p, Fout);
-    hipLaunchKernelGGL(k_fin_cfg, dim3(ncfg), dim3(256), 0, st, e->cfgs.p, e->as.p, e->mdev.p, e->Oacc.p, Ecfg,
-                       Vcfg, e->status.p);
-    (void)hipEventRecord(e->ev[5], st);
+    e->record(4);
+    hipLaunchKernelGGL(k_fin_forces, dim3(nblk(Nin, 256)), dim3(256), 0, st, (int)Nin, ncfg, e->cfg_in_off,
+                       e->cfgs, e->Facc.p, Fout);
+    hipLaunchKernelGGL(k_fin_cfg, dim3(ncfg), dim3(256), 0, st, e->cfgs, e->as.p, e->mdev, e->Oacc.p, Ecfg, Vcfg,
+                       e->status);
+    HIPCHK(hipMemcpyAsync(e->out_h.p, e->out_d.p, e->out_bytes, hipMemcpyDeviceToHost, st));
+    e->record(5);
+}
+
+// Status of the last run (host copy): throws on errors, grows capacities and
+// returns true when the run must be repeated.
+bool check_status(chx_engine *e)
+{
+    const int *st_h = (const int *)(e->out_h.p + e->o_stat);
+    const int fl = st_h[CHX_ST_FLAGS];
+    if (fl & CHX_BAD_NB)
+        throw EngineError("an atom has " + std::to_string(st_h[CHX_ST_MAXNB]) + " neighbour images within the "
+                          "search cutoff; at most " + std::to_string(CHX_MAX_NB) + " are supported");
+    if (fl & CHX_BAD_N34)
+        throw EngineError("an atom has more than " + std::to_string(CHX_MAX_N3) + " images within the 3-body "
+                          "cutoff or more than " + std::to_string(CHX_MAX_N4) + " within the 4-body cutoff");
+    if (!(fl & CHX_RETRY_ANY)) {
+        if (fl & CHX_BAD_RANGE)
+            throw EngineError("a cluster energy, force or virial contribution is outside the accumulator range "
+                              "(|value| >= 2^42)");
+        e->counts[0] = st_h[CHX_ST_NENT];
+        e->counts[1] = e->G3 ? st_h[CHX_ST_C3] : 0;
+        e->counts[2] = e->G4 ? st_h[CHX_ST_C4] : 0;
+        return false;
+    }
+    if (fl & CHX_RETRY_ENT) e->ent_cap = std::max<long>(e->ent_cap, (long)(st_h[CHX_ST_NENT] * 1.2) + 1024);
+    if (fl & CHX_RETRY_L3) e->l3_cap = std::max<long>(e->l3_cap, (long)(st_h[CHX_ST_N3] * 1.2) + 4096);
+    if (fl & CHX_RETRY_L4) e->l4_cap = std::max<long>(e->l4_cap, (long)(st_h[CHX_ST_N4] * 1.2) + 4096);
+    e->l3_cap = (e->l3_cap + 255) / 256 * 256;
+    e->l4_cap = (e->l4_cap + 255) / 256 * 256;
+    return true;
+}
+
+void read_timing(chx_engine *e)
+{
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[5]);
+    e->timing[0] = ms;
+    (void)hipEventElapsedTime(&ms, e->ev[3], e->ev[4]);
+    e->timing[1] = ms;
+    (void)hipEventElapsedTime(&ms, e->ev[2], e->ev[3]);
+    e->timing[2] = ms;
+    (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
+    e->timing[3] = ms;
+}
+
+// Per-configuration results from the output blob into the caller's arrays.
+void deliver(chx_engine *e, chx_out *o)
+{
+    const int ncfg = e->ncfg;
+    const int *stat = (const int *)(e->out_h.p + e->o_cstat);
+    if (!o->out_dev) {
+        memcpy(o->energy, e->out_h.p + e->o_E, ncfg * sizeof(double));
+        memcpy(o->virial, e->out_h.p + e->o_V, 6 * ncfg * sizeof(double));
+        memcpy(o->forces, e->out_h.p + e->o_F, 3 * (size_t)e->Nin * sizeof(double));
+    }
+    for (int c = 0; c < ncfg; c++) {
+        if (o->volume) o->volume[c] = e->cfg_h[c].vol;
+        if (o->n_rep) o->n_rep[c] = e->cfg_h[c].n_rep;
+        if (o->badness) o->badness[c] = stat[4 * c];
+        if (o->counts) {
+            o->counts[4 * c + 0] = stat[4 * c + 1];
+            o->counts[4 * c + 1] = stat[4 * c + 2];
+            o->counts[4 * c + 2] = 0;
+            o->counts[4 * c + 3] = 0;
+        }
+    }
+}
+
+// Eager evaluation: runs the pipeline until no capacity is short.
+void evaluate_eager(chx_engine *e, chx_out *o)
+{
+    double *Fout = o->out_dev ? o->forces : e->F_out;
+    double *Ecfg = o->out_dev ? o->energy : e->E_out;
+    double *Vcfg = o->out_dev ? o->virial : e->V_out;
+    for (int attempt = 0;; attempt++) {
+        size_work(e);
+        run_pipeline(e, Fout, Ecfg, Vcfg);
+        HIPCHK(hipStreamSynchronize(e->st));
+        HIPCHK(hipGetLastError());
+        if (!check_status(e)) break;
+        if (attempt >= 3) throw EngineError("buffer sizing did not converge");
+    }
+    read_timing(e);
 }
 
 }  // namespace
@@ -694,191 +986,10 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
 {
     try {
         HIPCHK(hipSetDevice(e->device));
-        hipStream_t st = e->st;
-        const int ncfg = b->nconf;
-        if (ncfg <= 0) throw EngineError("empty batch");
-        e->ncfg = ncfg;
-        e->models.assign(models, models + nmodels);
-        setup_frames(e, models, nmodels, b);
-        const long N = e->N, Nin = e->Nin, ncell = e->ncell;
-        // ---------------- groups per model ---------------------------------
-        e->g3b.assign(nmodels, 0);
-        e->g4b.assign(nmodels, 0);
-        std::vector<chx_enum_model> emh(nmodels);
-        std::vector<chx_model_dev> mdh(nmodels);
-        std::vector<chx_group> g3h, g4h;
-        std::vector<int> D3s, D4s;
-        int ngl_max = 1;
-        for (int mi = 0; mi < nmodels; mi++) {
-            const chx_model *M = models[mi];
-            chx_enum_model &q = emh[mi];
-            q.nt = M->d.nt;
-            q.has3 = M->d.order[1] > 0 && M->d.ntrip > 0;
-            q.has4 = M->d.order[2] > 0 && M->d.nquad > 0;
-            q.ntrip = M->d.ntrip;
-            q.nquad = M->d.nquad;
-            q.g3 = (int)g3h.size();
-            q.g4 = (int)g4h.size();
-            q.tripmap = M->dev.tripmap;
-            q.quadmap = M->dev.quadmap;
-            e->g3b[mi] = q.g3;
-            e->g4b[mi] = q.g4;
-            for (int body = 3; body <= 4; body++) {
-                const bool has = body == 3 ? q.has3 : q.has4;
-                if (!has) continue;
-                const int nty = body == 3 ? M->d.ntrip : M->d.nquad;
-                const int D = body == 3 ? M->d.D3 : M->d.D4;
-                std::vector<chx_group> &gv = body == 3 ? g3h : g4h;
-                std::vector<int> &Ds = body == 3 ? D3s : D4s;
-                if (std::find(Ds.begin(), Ds.end(), D) == Ds.end()) Ds.push_back(D);
-                ngl_max = std::max(ngl_max, nty + 1);
-                for (int t = 0; t <= nty; t++) {
-                    chx_group g{};
-                    g.model = mi;
-                    g.type = t < nty ? t : -1;
-                    g.nt = M->d.nt;
-                    g.D = D;
-                    g.tersoff = M->d.tersoff;
-                    g.fcut_var = M->d.fcut_var;
-                    g.perm = body == 3 ? M->dev.trip_perm : M->dev.quad_perm;
-                    if (t < nty) {
-                        const TypeLaunch &tl = body == 3 ? M->t3[t] : M->t4[t];
-                        g.C = tl.C;
-                        g.Cf = tl.Cf;
-                        g.Cm = tl.Cm;
-                        g.Cmf = tl.Cmf;
-                        for (int k = 0; k < (body == 3 ? 3 : 6); k++) g.s[k] = tl.s[k];
-                    }
-                    gv.push_back(g);
-                }
-            }
-            mdh[mi] = M->dev;
-        }
-        e->G3 = (int)g3h.size();
-        e->G4 = (int)g4h.size();
-        const int G3 = e->G3, G4 = e->G4;
-        guess_capacities(e, models);
-        // ---------------- uploads and fixed-size buffers --------------------
-        e->cfgs.ensure(ncfg);
-        e->cfg_atom_off.ensure(ncfg + 1);
-        e->cfg_in_off.ensure(ncfg + 1);
-        e->em.ensure(nmodels);
-        e->mdev.ensure(nmodels);
-        e->grp3.ensure(G3 + 1);
-        e->grp4.ensure(G4 + 1);
-        HIPCHK(hipMemcpyAsync(e->cfgs.p, e->cfg_h.data(), ncfg * sizeof(chx_cfg), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(e->cfg_atom_off.p, e->cfg_atom_off_h.data(), (ncfg + 1) * sizeof(int),
-                              hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(e->cfg_in_off.p, e->cfg_in_off_h.data(), (ncfg + 1) * sizeof(int),
-                              hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(e->em.p, emh.data(), nmodels * sizeof(chx_enum_model), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(e->mdev.p, mdh.data(), nmodels * sizeof(chx_model_dev), hipMemcpyHostToDevice, st));
-        if (G3) HIPCHK(hipMemcpyAsync(e->grp3.p, g3h.data(), G3 * sizeof(chx_group), hipMemcpyHostToDevice, st));
-        if (G4) HIPCHK(hipMemcpyAsync(e->grp4.p, g4h.data(), G4 * sizeof(chx_group), hipMemcpyHostToDevice, st));
-        const double *pos_d;
-        const int *typ_d;
-        if (b->pos_dev) {
-            pos_d = b->pos;
-            typ_d = b->types;
-        } else {
-            e->pos.ensure(3 * Nin);
-            e->types.ensure(Nin);
-            HIPCHK(hipMemcpyAsync(e->pos.p, b->pos, 3 * Nin * sizeof(double), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(e->types.p, b->types, Nin * sizeof(int), hipMemcpyHostToDevice, st));
-            pos_d = e->pos.p;
-            typ_d = e->types.p;
-        }
-        e->as.ensure(N);
-        e->an.ensure(N);
-        e->cell_cnt.ensure(ncell + 1);
-        e->cell_start.ensure(ncell + 1);
-        e->cell_cur.ensure(ncell + 1);
-        e->cell_atoms.ensure(N);
-        e->nbr_cnt.ensure(N + 1);
-        e->nbr_off.ensure(N + 1);
-        e->status.ensure(CHX_ST_WORDS);
-        e->cnt3.ensure((long)G3 * N + 1);
-        e->off3.ensure((long)G3 * N + 2);
-        e->cnt4.ensure((long)G4 * N + 1);
-        e->off4.ensure((long)G4 * N + 2);
-        e->gb3.ensure(G3 + 1);
-        e->ge3.ensure(G3 + 1);
-        e->gb4.ensure(G4 + 1);
-        e->ge4.ensure(G4 + 1);
-        e->Facc.ensure(6 * N);
-        e->Oacc.ensure(14 * N);
-        e->cfg_stat.ensure(4 * ncfg);
-        e->Fout.ensure(3 * Nin);
-        e->Ecfg.ensure(ncfg);
-        e->Vcfg.ensure(6 * ncfg);
-        double *Fout = o->out_dev ? o->forces : e->Fout.p;
-        double *Ecfg = o->out_dev ? o->energy : e->Ecfg.p;
-        double *Vcfg = o->out_dev ? o->virial : e->Vcfg.p;
-        const bool mixed = b->precision == CHX_MIXED;
-        // ---------------- pipeline, repeated while a capacity is short ------
-        int st_h[CHX_ST_WORDS];
-        std::vector<int> stat(4 * ncfg);
-        for (int attempt = 0;; attempt++) {
-            e->es.ensure(e->ent_cap);
-            e->en.ensure(e->ent_cap);
-            e->e_owner.ensure(e->ent_cap);
-            e->list3.ensure(e->l3_cap);
-            e->list4.ensure(e->l4_cap);
-            run_pipeline(e, pos_d, typ_d, mixed, ngl_max, D3s, D4s, Fout, Ecfg, Vcfg);
-            HIPCHK(hipMemcpyAsync(st_h, e->status.p, sizeof(st_h), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            HIPCHK(hipGetLastError());
-            const int fl = st_h[CHX_ST_FLAGS];
-            if (fl & CHX_BAD_NB)
-                throw EngineError("an atom has " + std::to_string(st_h[CHX_ST_MAXNB]) + " neighbour images within "
-                                  "the search cutoff; at most " + std::to_string(CHX_MAX_NB) + " are supported");
-            if (fl & CHX_BAD_N34)
-                throw EngineError("an atom has more than " + std::to_string(CHX_MAX_N3) + " images within the "
-                                  "3-body cutoff or more than " + std::to_string(CHX_MAX_N4) + " within the 4-body "
-                                  "cutoff");
-            if (!(fl & CHX_RETRY_ANY)) {
-                if (fl & CHX_BAD_RANGE)
-                    throw EngineError("a cluster energy, force or virial contribution is outside the accumulator "
-                                      "range (|value| >= 2^42)");
-                break;
-            }
-            if (attempt >= 3) throw EngineError("buffer sizing did not converge");
-            if (fl & CHX_RETRY_ENT) e->ent_cap = std::max<long>(e->ent_cap, (long)(st_h[CHX_ST_NENT] * 1.2) + 1024);
-            if (fl & CHX_RETRY_L3) e->l3_cap = std::max<long>(e->l3_cap, (long)(st_h[CHX_ST_N3] * 1.2) + 4096);
-            if (fl & CHX_RETRY_L4) e->l4_cap = std::max<long>(e->l4_cap, (long)(st_h[CHX_ST_N4] * 1.2) + 4096);
-            e->l3_cap = (e->l3_cap + 255) / 256 * 256;
-            e->l4_cap = (e->l4_cap + 255) / 256 * 256;
-        }
-        HIPCHK(hipMemcpyAsync(stat.data(), e->cfg_stat.p, 4 * ncfg * sizeof(int), hipMemcpyDeviceToHost, st));
-        if (!o->out_dev) {
-            HIPCHK(hipMemcpyAsync(o->forces, Fout, 3 * Nin * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(o->energy, Ecfg, ncfg * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(o->virial, Vcfg, 6 * ncfg * sizeof(double), hipMemcpyDeviceToHost, st));
-        }
-        HIPCHK(hipStreamSynchronize(st));
-        e->counts[0] = st_h[CHX_ST_NENT];
-        e->counts[1] = G3 ? st_h[CHX_ST_C3] : 0;
-        e->counts[2] = G4 ? st_h[CHX_ST_C4] : 0;
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[5]);
-        e->timing[0] = ms;
-        (void)hipEventElapsedTime(&ms, e->ev[3], e->ev[4]);
-        e->timing[1] = ms;
-        (void)hipEventElapsedTime(&ms, e->ev[2], e->ev[3]);
-        e->timing[2] = ms;
-        (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
-        e->timing[3] = ms;
-        for (int c = 0; c < ncfg; c++) {
-            if (o->volume) o->volume[c] = e->cfg_h[c].vol;
-            if (o->n_rep) o->n_rep[c] = e->cfg_h[c].n_rep;
-            if (o->badness) o->badness[c] = stat[4 * c];
-            if (o->counts) {
-                o->counts[4 * c + 0] = stat[4 * c + 1];
-                o->counts[4 * c + 1] = stat[4 * c + 2];
-                o->counts[4 * c + 2] = 0;
-                o->counts[4 * c + 3] = 0;
-            }
-        }
+        prepare(e, models, nmodels, b);
+        layout(e, b, !o->out_dev);
+        evaluate_eager(e, o);
+        deliver(e, o);
         e->err.clear();
         return 0;
     } catch (const EngineError &x) {
@@ -887,6 +998,235 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
         return 1;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Batch plans: a fixed batch shape evaluated by replaying one captured graph.
+// ---------------------------------------------------------------------------
+struct chx_plan {
+    chx_engine *e = nullptr;
+    std::vector<chx_model *> models;
+    std::vector<int> natoms, model_of;
+    int nconf = 0, small = 0, precision = 0, pos_dev = 0, out_dev = 0;
+    const double *pos_dev_p = nullptr;
+    const int *types_dev_p = nullptr;
+    double *dev_out[3] = {nullptr, nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long sig[6] = {0, 0, 0, 0, 0, 0};  // N, Nin, ncell, ent_cap, l3_cap, l4_cap of the captured graph
+    double tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    std::string err;
+    ~chx_plan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e) chx_engine_destroy(e);
+    }
+};
+
+namespace {
+
+void plan_batch(chx_plan *p, const double *pos, const int *types, const double *cells, chx_batch &b)
+{
+    b.nconf = p->nconf;
+    b.natoms = p->natoms.data();
+    b.pos = p->pos_dev ? p->pos_dev_p : pos;
+    b.types = p->pos_dev ? p->types_dev_p : types;
+    b.cells = cells;
+    b.model = p->model_of.data();
+    b.small = p->small;
+    b.pos_dev = p->pos_dev;
+    b.precision = p->precision;
+}
+
+void plan_release_graph(chx_plan *p)
+{
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+// Captures the pipeline at the current sizes (buffers already grown by an
+// eager run, so no allocation happens while capturing).
+void plan_capture(chx_plan *p, chx_out *o)
+{
+    chx_engine *e = p->e;
+    plan_release_graph(p);
+    double *Fout = o->out_dev ? o->forces : e->F_out;
+    double *Ecfg = o->out_dev ? o->energy : e->E_out;
+    double *Vcfg = o->out_dev ? o->virial : e->V_out;
+    size_work(e);
+    e->capturing = true;
+    HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+    try {
+        run_pipeline(e, Fout, Ecfg, Vcfg);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(e->st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        e->capturing = false;
+        throw;
+    }
+    e->capturing = false;
+    HIPCHK(hipStreamEndCapture(e->st, &p->graph));
+    HIPCHK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    const long sig[6] = {e->N, e->Nin, e->ncell, e->ent_cap, e->l3_cap, e->l4_cap};
+    for (int k = 0; k < 6; k++) p->sig[k] = sig[k];
+}
+
+// Times each 3-/4-body kernel variant on the lists of the last eager run and
+// keeps the fastest per body order.
+void plan_tune(chx_plan *p)
+{
+    chx_engine *e = p->e;
+    hipEvent_t a, z;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&z));
+    for (int body = 3; body <= 4; body++) {
+        const int G = body == 3 ? e->G3 : e->G4;
+        if (!G) continue;
+        chx_list_args la{};
+        la.cfgs = e->cfgs;
+        la.as = e->as.p;
+        la.nbr_off = e->nbr_off.p;
+        la.es = e->es.p;
+        la.Facc = e->Facc.p;  // accumulators are overwritten by the next real run
+        la.Oacc = e->Oacc.p;
+        la.cfg_stat = e->cfg_stat;
+        la.status = e->status;
+        la.cap = body == 3 ? e->l3_cap : e->l4_cap;
+        la.G = G;
+        la.gbase = body == 3 ? e->gb3.p : e->gb4.p;
+        la.gend = body == 3 ? e->ge3.p : e->ge4.p;
+        la.groups = body == 3 ? e->grp3 : e->grp4;
+        la.list = body == 3 ? e->list3.p : e->list4.p;
+        const std::vector<int> &Ds = body == 3 ? e->D3s : e->D4s;
+        int best = CHX_VAR_MFMA4;
+        double best_ms = 1e30;
+        for (int var = 0; var < CHX_NVAR; var++) {
+            float best_rep = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                HIPCHK(hipEventRecord(a, e->st));
+                for (int D : Ds) launch_eval_body(body, D, e->mixed, var, la, e->st);
+                HIPCHK(hipEventRecord(z, e->st));
+                HIPCHK(hipEventSynchronize(z));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, a, z));
+                if (rep > 0) best_rep = std::min(best_rep, ms);
+            }
+            p->tune_ms[body - 3][var] = best_rep;
+            if (best_rep < best_ms) { best_ms = best_rep; best = var; }
+        }
+        if (body == 3) e->var3 = best; else e->var4 = best;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(z);
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+chx_plan *chx_plan_create(int device, chx_model *const *models, int nmodels, const chx_batch *b, int tune,
+                          chx_out *o, char *err, int errlen)
+{
+    chx_plan *p = new chx_plan;
+    try {
+        p->e = chx_engine_create(device);
+        if (!p->e) throw EngineError("cannot create an engine on this device");
+        chx_engine *e = p->e;
+        HIPCHK(hipSetDevice(e->device));
+        p->models.assign(models, models + nmodels);
+        p->nconf = b->nconf;
+        p->natoms.assign(b->natoms, b->natoms + b->nconf);
+        p->model_of.assign(b->model, b->model + b->nconf);
+        p->small = b->small;
+        p->precision = b->precision;
+        p->pos_dev = b->pos_dev;
+        p->out_dev = o->out_dev;
+        if (p->pos_dev) {
+            p->pos_dev_p = b->pos;
+            p->types_dev_p = b->types;
+        }
+        if (p->out_dev) {
+            p->dev_out[0] = o->energy;
+            p->dev_out[1] = o->forces;
+            p->dev_out[2] = o->virial;
+        }
+        prepare(e, models, nmodels, b);
+        layout(e, b, !o->out_dev);
+        evaluate_eager(e, o);
+        if (tune) {
+            plan_tune(p);
+            evaluate_eager(e, o);  // results of the chosen variants
+        }
+        deliver(e, o);
+        plan_capture(p, o);
+        return p;
+    } catch (const EngineError &x) {
+        if (err && errlen > 0) {
+            strncpy(err, x.msg.c_str(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        delete p;
+        return nullptr;
+    }
+}
+
+int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double *cells, chx_out *o)
+{
+    chx_engine *e = p->e;
+    try {
+        HIPCHK(hipSetDevice(e->device));
+        if (o->out_dev != p->out_dev) throw EngineError("output location differs from the plan's");
+        if (p->out_dev && (o->energy != p->dev_out[0] || o->forces != p->dev_out[1] || o->virial != p->dev_out[2]))
+            throw EngineError("device output buffers differ from the plan's");
+        chx_batch b{};
+        plan_batch(p, pos, types, cells, b);
+        prepare(e, p->models.data(), (int)p->models.size(), &b);
+        layout(e, &b, !o->out_dev);
+        const long sig[6] = {e->N, e->Nin, e->ncell, e->ent_cap, e->l3_cap, e->l4_cap};
+        bool same = p->exec != nullptr;
+        for (int k = 0; k < 6; k++) same = same && sig[k] == p->sig[k];
+        if (same) {
+            size_work(e);
+            HIPCHK(hipGraphLaunch(p->exec, e->st));
+            HIPCHK(hipStreamSynchronize(e->st));
+            HIPCHK(hipGetLastError());
+            if (!check_status(e)) {
+                deliver(e, o);
+                e->err.clear();
+                return 0;
+            }
+        }
+        // new shape or a capacity ran short: eager run, then capture again
+        evaluate_eager(e, o);
+        deliver(e, o);
+        plan_capture(p, o);
+        e->err.clear();
+        return 0;
+    } catch (const EngineError &x) {
+        p->err = x.msg;
+        (void)hipStreamSynchronize(e->st);
+        return 1;
+    }
+}
+
+void chx_plan_destroy(chx_plan *p) { delete p; }
+
+const char *chx_plan_error(chx_plan *p) { return p ? p->err.c_str() : "no plan"; }
+
+void chx_plan_info(chx_plan *p, int *variants, double *tune_ms, long long *counts)
+{
+    variants[0] = p->e->var3;
+    variants[1] = p->e->var4;
+    for (int b = 0; b < 2; b++)
+        for (int v = 0; v < 3; v++) tune_ms[3 * b + v] = p->tune_ms[b][v];
+    for (int k = 0; k < 3; k++) counts[k] = p->e->counts[k];
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int chx_export_lists(chx_engine *e, int cfg, long *n2, int **pairs, long *n3, int **trips, long *n4, int **quads)
 {

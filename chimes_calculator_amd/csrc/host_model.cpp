@@ -186,6 +186,7 @@ std::unique_ptr<DeviceModel> make_device_model(const ParamSet &P, int device, st
     std::unique_ptr<DeviceModel> dm(new DeviceModel);
     dm->P = P;
     dm->eh = eh;
+    dm->device = device;
     {
         std::lock_guard<std::mutex> lk(eh->mu);
         dm->m = chx_model_create(eh->eng, &d);

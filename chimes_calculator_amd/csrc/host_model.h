@@ -25,6 +25,7 @@ struct DeviceModel {
     ParamSet P;
     EngineHandle *eh = nullptr;
     chx_model *m = nullptr;
+    int device = 0;
     double rc[3] = {0, 0, 0};
     ~DeviceModel();
 };

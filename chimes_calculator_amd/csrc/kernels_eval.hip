@@ -263,8 +263,6 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
     const int g = find_group(gbase, G, t0);
     const chx_group &grp = groups[g];
     if (grp.type < 0 || grp.D != D) return;
-    if (NB == 4 && D <= CHX_MFMA_MAX_D) return;  // matrix-core kernels
-    if (NB == 3 && D <= CHX_MFMA3_MAX_D) return;
     const long c = t0 + lane;
     int owner = -1;
     double q7[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -372,8 +370,8 @@ struct mfma_shape {
     static constexpr int RB = MP / 16;                  // 16-row output blocks
 };
 
-template <int D, typename R>
-__global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict__ gbase,
+template <int D, typename R, int WPB>
+__global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__restrict__ gbase,
                                                     const int *__restrict__ gend,
                                                     const chx_group *__restrict__ groups,
                                                     const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
@@ -390,28 +388,28 @@ __global__ __launch_bounds__(256, 2) void k_eval4m(int G, const int *__restrict_
     __shared__ R Cl[LM::size];
     // staged polynomials; after the contraction the same memory holds the force table
     union stage_t {
-        struct { R T[4][16 * TS], dT[4][16 * TS]; } p;
+        struct { R T[WPB][16 * TS], dT[WPB][16 * TS]; } p;
         fx_table<256> tab;
     };
     __shared__ stage_t st;
-    R (&Ts)[4][16 * TS] = st.p.T;
-    R (&dTs)[4][16 * TS] = st.p.dT;
-    __shared__ double Gm[4][16][6][3];  // r, fcut, fcut' per pair
-    __shared__ double Pm[4][16][4][3];  // member positions (evaluation-frame fractional)
+    R (&Ts)[WPB][16 * TS] = st.p.T;
+    R (&dTs)[WPB][16 * TS] = st.p.dT;
+    __shared__ double Gm[WPB][16][6][3];  // r, fcut, fcut' per pair
+    __shared__ double Pm[WPB][16][4][3];  // member positions (evaluation-frame fractional)
     if (block_stop(status, CHX_STOP)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int j = lane & 15, sub = lane >> 4;
-    const long nbt = status[CHX_ST_N4] / 64;
+    const long nbt = status[CHX_ST_N4] / (16 * WPB);
     {
         const long bt = blockIdx.x;
         if (bt >= nbt) return;
-        const long t0 = bt * 64;
+        const long t0 = bt * (16 * WPB);
         const int g = find_group(gbase, G, t0);
         const chx_group &grp = groups[g];
         if (grp.type < 0 || grp.D != D) return;
         {
             const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
-            for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 256) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
+            for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 64 * WPB) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
         }
         // ---- stage: geometry (all four lanes of a cluster), polynomials of pairs sub and sub + 4
         const long c = t0 + wv * 16 + j;
@@ -616,8 +614,8 @@ struct mfma3_shape {
     static constexpr int RB = D;
 };
 
-template <int D, typename R>
-__global__ __launch_bounds__(256, 2) void k_eval3m(int G, const int *__restrict__ gbase,
+template <int D, typename R, int WPB>
+__global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__restrict__ gbase,
                                                     const int *__restrict__ gend,
                                                     const chx_group *__restrict__ groups,
                                                     const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
@@ -635,27 +633,27 @@ __global__ __launch_bounds__(256, 2) void k_eval3m(int G, const int *__restrict_
     __shared__ R Cl[LM::size];
     // staged polynomials; after the contraction the same memory holds the force table
     union stage_t {
-        struct { R T[4][16 * TS], dT[4][16 * TS]; } p;
+        struct { R T[WPB][16 * TS], dT[WPB][16 * TS]; } p;
         fx_table<256> tab;
     };
     __shared__ stage_t st;
-    R (&Ts)[4][16 * TS] = st.p.T;
-    R (&dTs)[4][16 * TS] = st.p.dT;
-    __shared__ double Gm[4][16][3][3];  // r, fcut, fcut' per pair
-    __shared__ double Pm[4][16][3][3];  // member positions
+    R (&Ts)[WPB][16 * TS] = st.p.T;
+    R (&dTs)[WPB][16 * TS] = st.p.dT;
+    __shared__ double Gm[WPB][16][3][3];  // r, fcut, fcut' per pair
+    __shared__ double Pm[WPB][16][3][3];  // member positions
     if (block_stop(status, CHX_STOP)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int j = lane & 15, sub = lane >> 4;
-    const long nbt = status[CHX_ST_N3] / 64;
+    const long nbt = status[CHX_ST_N3] / (16 * WPB);
     const long bt = blockIdx.x;
     if (bt >= nbt) return;
-    const long t0 = bt * 64;
+    const long t0 = bt * (16 * WPB);
     const int g = find_group(gbase, G, t0);
     const chx_group &grp = groups[g];
     if (grp.type < 0 || grp.D != D) return;
     {
         const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
-        for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 256) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
+        for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 64 * WPB) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
     }
     // ---- stage: geometry (all lanes of a cluster), lane sub < 3 the polynomials of pair sub
     const long c = t0 + wv * 16 + j;
@@ -1035,48 +1033,60 @@ void launch_evalv(int nb, int D, bool mixed, const chx_list_args &la, hipStream_
 #undef X
 }
 
+template <int WPB>
 bool launch_eval3m(int D, bool mixed, const chx_list_args &la, hipStream_t st)
 {
     if (D > CHX_MFMA3_MAX_D) return false;
     if (la.cap <= 0 || la.G <= 0) return true;
-    const dim3 grid((unsigned)std::max<long>(1, la.cap / 64)), block(256);
-#define X(d)                                                                                                    \
-    if (D == d) {                                                                                               \
-        if (mixed)                                                                                              \
-            hipLaunchKernelGGL((k_eval3m<d, float>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups,     \
-                               la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,        \
-                               la.status);                                                                      \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_eval3m<d, double>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups,    \
-                               la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,        \
-                               la.status);                                                                      \
-        return true;                                                                                            \
+    const dim3 grid((unsigned)std::max<long>(1, la.cap / (16 * WPB))), block(64 * WPB);
+#define X(d)                                                                                                   \
+    if (D == d) {                                                                                              \
+        if (mixed)                                                                                             \
+            hipLaunchKernelGGL((k_eval3m<d, float, WPB>), grid, block, 0, st, la.G, la.gbase, la.gend,          \
+                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.cfg_stat, la.status);                                                        \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_eval3m<d, double, WPB>), grid, block, 0, st, la.G, la.gbase, la.gend,         \
+                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.cfg_stat, la.status);                                                        \
+        return true;                                                                                           \
     }
     CHX_D3_CASES(X)
 #undef X
     return false;
 }
 
+template <int WPB>
 bool launch_eval4m(int D, bool mixed, const chx_list_args &la, hipStream_t st)
 {
     if (D > CHX_MFMA_MAX_D) return false;
     if (la.cap <= 0 || la.G <= 0) return true;
-    const dim3 grid((unsigned)std::max<long>(1, la.cap / 64)), block(256);
-#define X(d)                                                                                                    \
-    if (D == d) {                                                                                               \
-        if (mixed)                                                                                              \
-            hipLaunchKernelGGL((k_eval4m<d, float>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups,     \
-                               la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,        \
-                               la.status);                                                                      \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_eval4m<d, double>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups,    \
-                               la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,        \
-                               la.status);                                                                      \
-        return true;                                                                                            \
+    const dim3 grid((unsigned)std::max<long>(1, la.cap / (16 * WPB))), block(64 * WPB);
+#define X(d)                                                                                                   \
+    if (D == d) {                                                                                              \
+        if (mixed)                                                                                             \
+            hipLaunchKernelGGL((k_eval4m<d, float, WPB>), grid, block, 0, st, la.G, la.gbase, la.gend,          \
+                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.cfg_stat, la.status);                                                        \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_eval4m<d, double, WPB>), grid, block, 0, st, la.G, la.gbase, la.gend,         \
+                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.cfg_stat, la.status);                                                        \
+        return true;                                                                                           \
     }
     CHX_D4M_CASES(X)
 #undef X
     return false;
+}
+
+void launch_eval_body(int nb, int D, bool mixed, int variant, const chx_list_args &la, hipStream_t st)
+{
+    bool done = false;
+    if (variant == CHX_VAR_MFMA4)
+        done = nb == 3 ? launch_eval3m<4>(D, mixed, la, st) : launch_eval4m<4>(D, mixed, la, st);
+    else if (variant == CHX_VAR_MFMA2)
+        done = nb == 3 ? launch_eval3m<2>(D, mixed, la, st) : launch_eval4m<2>(D, mixed, la, st);
+    if (!done) launch_evalv(nb, D, mixed, la, st);
 }
 
 void launch_clusters(int nb, int D, int n, hipStream_t st, const double *dist, const double *drv, const int *types,

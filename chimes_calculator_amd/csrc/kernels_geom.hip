@@ -514,6 +514,44 @@ __global__ __launch_bounds__(256) void k_scan_block(const int *in, int *out, lon
     if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
 }
 
+// Exclusive scan of a short array (n <= CHX_SCAN1_MAX) in one workgroup of
+// 1024 threads, each summing a contiguous chunk; out[n] = total.
+__global__ __launch_bounds__(1024) void k_scan_one(const int *in, int *out, long n)
+{
+    __shared__ int s[1024];
+    const long per = (n + 1023) / 1024;
+    const long b = threadIdx.x * per;
+    const long e = b + per < n ? b + per : n;
+    int t = 0;
+    for (long i = b; i < e; i++) t += in[i];
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int x = (int)threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+        __syncthreads();
+        s[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - t;
+    for (long i = b; i < e; i++) {
+        const int v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) out[n] = s[1023];
+}
+
+// Zeroes every listed range (one launch for all counters and accumulators).
+__global__ void k_clear(const chx_clear_range *__restrict__ r, int nr)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x, gs = (long)gridDim.x * blockDim.x;
+    for (int k = 0; k < nr; k++) {
+        int *p = r[k].p;
+        const long n = r[k].n;
+        for (long i = g; i < n; i += gs) p[i] = 0;
+    }
+}
+
 __global__ void k_scan_add(int *out, long n, const int *bscan)
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;

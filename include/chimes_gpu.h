@@ -52,6 +52,41 @@ int chimes_gpu_eval_batch_device(chimes_gpu_model *const *models, int nmodels, i
                                  const double *cells, int small, int precision, double *energy_dev,
                                  double *forces_dev, double *virial_dev);
 
+/* Batch plans: one batch shape (models, model_of, natoms, small, precision)
+ * evaluated repeatedly by replaying a single captured HIP graph (input copy,
+ * every kernel, output copy) - the form for thousands of uneven
+ * configurations evaluated again and again.  Creation evaluates the given
+ * inputs once (results in energy/forces/stress, as chimes_gpu_eval_batch) and,
+ * when tune != 0, times the 3-/4-body kernel variants (matrix cores with 64-
+ * or 32-cluster tiles, or vector units) on this batch and keeps the fastest.
+ * chimes_gpu_plan_run takes new positions, types and cells of the same shape;
+ * a run whose cluster counts outgrow the captured buffers is evaluated
+ * directly and captured again.  A plan owns its GPU stream and buffers; calls
+ * on one plan must not overlap. */
+typedef struct chimes_gpu_plan chimes_gpu_plan;
+chimes_gpu_plan *chimes_gpu_plan_create(chimes_gpu_model *const *models, int nmodels, int nconf,
+                                        const int *model_of, const int *natoms, const double *pos, const int *types,
+                                        const double *cells, int small, int precision, int tune, double *energy,
+                                        double *forces, double *stress);
+int chimes_gpu_plan_run(chimes_gpu_plan *p, const double *pos, const int *types, const double *cells,
+                        double *energy, double *forces, double *stress);
+/* Device-buffer plans: pos_dev/types_dev and the outputs (as in
+ * chimes_gpu_eval_batch_device) are fixed at creation; each run re-reads the
+ * same input buffers and takes the cells (host) of this run. */
+chimes_gpu_plan *chimes_gpu_plan_create_device(chimes_gpu_model *const *models, int nmodels, int nconf,
+                                               const int *model_of, const int *natoms, const double *pos_dev,
+                                               const int *types_dev, const double *cells, int small, int precision,
+                                               int tune, double *energy_dev, double *forces_dev,
+                                               double *virial_dev);
+int chimes_gpu_plan_run_device(chimes_gpu_plan *p, const double *cells, double *energy_dev, double *forces_dev,
+                               double *virial_dev);
+/* variants[2]: kernel variant of the 3- and 4-body terms (0 matrix cores,
+ * 64-cluster tiles; 1 matrix cores, 32-cluster tiles; 2 vector units);
+ * tune_ms[6]: timed milliseconds per (body, variant), 0 if not tuned;
+ * counts[3]: pair images, triplets, quadruplets of the last run. */
+void chimes_gpu_plan_info(const chimes_gpu_plan *p, int variants[2], double tune_ms[6], long long counts[3]);
+void chimes_gpu_plan_free(chimes_gpu_plan *p);
+
 /* Milliseconds of the last evaluation on the model's device, from HIP events:
  * [0] pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] lists. */
 void chimes_gpu_last_timing(const chimes_gpu_model *m, double ms[4]);
