@@ -40,6 +40,9 @@ EXPORTS = {
                      "chimes_gpu_model_max_cutoff", "chimes_gpu_eval_batch", "chimes_gpu_eval_batch_device",
                      "chimes_gpu_plan_create", "chimes_gpu_plan_run", "chimes_gpu_plan_create_device",
                      "chimes_gpu_plan_run_device", "chimes_gpu_plan_info", "chimes_gpu_plan_free",
+                     "chimes_gpu_server_create", "chimes_gpu_server_submit", "chimes_gpu_server_wait",
+                     "chimes_gpu_server_result", "chimes_gpu_server_stats", "chimes_gpu_server_last_error",
+                     "chimes_gpu_server_free",
                      "chimes_gpu_last_timing", "chimes_gpu_last_counts", "chimes_gpu_compute_clusters",
                      "chimes_gpu_cluster_lists", "chimes_gpu_free"],
 }
@@ -92,6 +95,17 @@ def lib():
         L.chimes_gpu_plan_info.argtypes = [C.c_void_p, _ip, _dp,
                                            np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
+        L.chimes_gpu_server_create.restype = C.c_void_p
+        L.chimes_gpu_server_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double]
+        L.chimes_gpu_server_submit.restype = C.c_long
+        L.chimes_gpu_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _ip, _dp, C.c_int]
+        L.chimes_gpu_server_wait.restype = C.c_long
+        L.chimes_gpu_server_wait.argtypes = [C.c_void_p, C.c_long, C.c_double]
+        L.chimes_gpu_server_result.argtypes = [C.c_void_p, C.c_long, C.POINTER(C.c_double), _dp, _dp, _dp]
+        L.chimes_gpu_server_stats.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64,
+                                                                                  flags="C_CONTIGUOUS")]
+        L.chimes_gpu_server_last_error.restype = C.c_char_p
+        L.chimes_gpu_server_free.argtypes = [C.c_void_p]
         L.chimes_gpu_last_timing.argtypes = [C.c_void_p, _dp]
         L.chimes_gpu_last_counts.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.chimes_gpu_compute_clusters.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp]
@@ -260,6 +274,61 @@ class GpuPlan:
     def close(self):
         if getattr(self, "h", None):
             self.L.chimes_gpu_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class GpuServer:
+    """Continuous-batching request loop over resident models (chimes_gpu.h).
+    submit() returns a request id at once; wait()/result() collect it."""
+
+    def __init__(self, models, precision=FP64, nslots=2, max_batch_atoms=65536, hbm_budget_gb=16.0):
+        self.L = lib()
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m.h for m in self.models])
+        self.h = self.L.chimes_gpu_server_create(arr, len(self.models), int(precision), int(nslots),
+                                                 int(max_batch_atoms), float(hbm_budget_gb))
+        if not self.h:
+            raise ChimesError(self.L.chimes_gpu_server_last_error().decode())
+        self.natoms = {}
+
+    def submit(self, model, config, small=False):
+        pos, types, cell = config
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        idx = self.models[model].types_to_index(types)
+        cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+        rid = self.L.chimes_gpu_server_submit(self.h, int(model), len(idx), pos, idx, cell, int(small))
+        if rid < 0:
+            raise ChimesError(self.L.chimes_gpu_server_last_error().decode())
+        self.natoms[rid] = len(idx)
+        return rid
+
+    def wait(self, rid=0, timeout=60.0):
+        """Id of a completed request (rid, or any if rid == 0); 0 on timeout."""
+        r = self.L.chimes_gpu_server_wait(self.h, int(rid), float(timeout))
+        if r < 0:
+            raise ChimesError(self.L.chimes_gpu_server_last_error().decode())
+        return r
+
+    def result(self, rid):
+        """(energy, forces[n, 3], stress[9], (t_submit, t_start, t_done) in ms)."""
+        n = self.natoms.pop(rid)
+        E = C.c_double()
+        F, S, T = np.zeros(3 * n), np.zeros(9), np.zeros(3)
+        if self.L.chimes_gpu_server_result(self.h, int(rid), C.byref(E), F, S, T):
+            raise ChimesError(self.L.chimes_gpu_server_last_error().decode())
+        return E.value, F.reshape(n, 3), S, tuple(T)
+
+    def stats(self):
+        s = np.zeros(4, np.int64)
+        self.L.chimes_gpu_server_stats(self.h, s)
+        return dict(requests=int(s[0]), batches=int(s[1]), atoms=int(s[2]), max_batch=int(s[3]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.chimes_gpu_server_free(self.h)
             self.h = None
 
     def __del__(self):

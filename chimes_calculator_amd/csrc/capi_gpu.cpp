@@ -12,9 +12,6 @@
 
 #include "host_model.h"
 
-struct chimes_gpu_model {
-    std::unique_ptr<chx::DeviceModel> dm;
-};
 
 struct chimes_gpu_plan {
     chx_plan *p = nullptr;

@@ -35,3 +35,8 @@ struct DeviceModel {
 std::unique_ptr<DeviceModel> make_device_model(const ParamSet &P, int device, std::string &err);
 
 }  // namespace chx
+
+// The public handle of chimes_gpu.h.
+struct chimes_gpu_model {
+    std::unique_ptr<chx::DeviceModel> dm;
+};

@@ -87,6 +87,33 @@ int chimes_gpu_plan_run_device(chimes_gpu_plan *p, const double *cells, double *
 void chimes_gpu_plan_info(const chimes_gpu_plan *p, int variants[2], double tune_ms[6], long long counts[3]);
 void chimes_gpu_plan_free(chimes_gpu_plan *p);
 
+/* Serving: a continuous-batching request loop over resident models.  Each of
+ * nslots workers owns a GPU stream and device buffers (kept between batches)
+ * and repeatedly evaluates every pending request that fits max_batch_atoms
+ * and the HBM budget (estimated device bytes of all requests in flight) as one
+ * multi-model batch; batches of different slots overlap on the device.
+ * Requests are single configurations with host inputs (copied at submit).
+ * Every result is bit-identical to evaluating the request alone. */
+typedef struct chimes_gpu_server chimes_gpu_server;
+chimes_gpu_server *chimes_gpu_server_create(chimes_gpu_model *const *models, int nmodels, int precision,
+                                            int nslots, long max_batch_atoms, double hbm_budget_gb);
+/* Returns the request id (> 0), or -1 on error (chimes_gpu_server_last_error). */
+long chimes_gpu_server_submit(chimes_gpu_server *s, int model, int natoms, const double *pos, const int *types,
+                              const double *cell, int small);
+/* Waits up to timeout_s for request id (id > 0) or for any request (id == 0)
+ * to complete; returns its id, 0 on timeout, -1 on error. */
+long chimes_gpu_server_wait(chimes_gpu_server *s, long id, double timeout_s);
+/* Results of a completed request (as chimes_gpu_eval_batch for one
+ * configuration) and its times in ms since server creation: submitted,
+ * started, done.  Releases the request; nonzero if it failed. */
+int chimes_gpu_server_result(chimes_gpu_server *s, long id, double *energy, double *forces, double *stress,
+                             double times_ms[3]);
+/* [0] requests served, [1] batches, [2] atoms, [3] largest batch (requests). */
+void chimes_gpu_server_stats(chimes_gpu_server *s, long long stats[4]);
+const char *chimes_gpu_server_last_error(void);
+/* Finishes the pending requests and stops the workers. */
+void chimes_gpu_server_free(chimes_gpu_server *s);
+
 /* Milliseconds of the last evaluation on the model's device, from HIP events:
  * [0] pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] lists. */
 void chimes_gpu_last_timing(const chimes_gpu_model *m, double ms[4]);
