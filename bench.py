@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4"], default="C1")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB before timing (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every configuration of this workload fits one MI355X and the "
@@ -154,6 +156,11 @@ def bench_c1(args):
     for _ in range(args.warmup):
         step()
     hip.sync()
+    if args.calib_copy_mib > 0:  # a plain copy of known size, for HBM counter calibration
+        nb = args.calib_copy_mib << 20
+        a, b = hip.alloc(nb), hip.alloc(nb)
+        assert hip.L.hipMemcpy(b, a, nb, 3) == 0
+        hip.sync()
     t4, tall = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
