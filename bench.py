@@ -86,6 +86,20 @@ class Hip:
         assert self.L.hipDeviceSynchronize() == 0
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "k_eval4m_water1536_fp64_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) > 3 and f[0] == "traffic" and f[1] == kernel:
+            kv = dict(x.split("=") for x in f[2:])
+            return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
+    return None
+
+
 def cpu_baseline(param_file, cfg, stride=3):
     """The oracle (C restatement, single thread) on 1/stride of the clusters
     (owners i with i % stride == 0), scaled to a full evaluation."""
@@ -208,7 +222,9 @@ def bench_c1(args):
         "latency_ms": {"p50": float(np.percentile(tall, 50)), "p99": float(np.percentile(tall, 99))},
         "error_vs_reference_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": measured_traffic("k_eval4m") if prec == cc.FP64 else None,
+                     "traffic_source": "profiles/k_eval4m_water1536_fp64_traffic.md (bytes per launch)",
                      "kernel": "k_eval4 (4-body evaluation, %d cluster types)" % len(t4terms),
                      "flops_per_step": flops4, "kernel_ms_per_step": t4_s * 1e3},
     }

@@ -6,7 +6,7 @@
  * Makefile builds it with -ffp-contract=off) so that it can be compared with
  * the compiled reference bit for bit, and with the HIP product to 1e-10.
  *
- * What pins each part (tests/test_oracle_*.py):
+ * What pins each part (tests/test_oracle.py):
  *   parser + maps (oc_load)         : compiled reference (oracle/_ref) on every
  *                                     force field in serial_interface/tests,
  *                                     via end-to-end E/F/stress equality;
@@ -19,8 +19,8 @@
  *   oc_compute_{2,3,4}b             : brute-force long-double term sums,
  *                                     finite-difference forces, virial identity.
  *   system setup + neighbour lists  : golden cluster lists and E/F/stress from
- *                                     the compiled reference (tests/golden),
- *                                     brute-force O(N^k) image enumeration.
+ *                                     the compiled reference (tests/golden):
+ *                                     identical sorted lists on every fixture.
  *   oc_calculate                    : reference expected_output fixtures
  *                                     (1e-3), golden %.17g vectors (1e-12).
  */
