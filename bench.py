@@ -313,7 +313,7 @@ def bench_c3(args):
            "vs_baseline": None, "data": "synthetic: seeded jittered carbon lattice and seeded ChIMES model",
            "config": {"workload": W.WORKLOADS["C3"], "natoms": n}}
     res = {}
-    for prec, tag in ((cc.FP64, "fp64"), (cc.MIXED, "mixed")):
+    for prec, tag in ((cc.FP64, "fp64"), (cc.MIXED, "mixed"), (cc.BF16, "bf16"), (cc.BF16X3, "bf16x3")):
         plan = cc.GpuPlan(model, cfgs, precision=prec, tune=True)
         _, pos, types, cells = cc._flatten(cfgs, plan.model_of, [model])
         E, F, S = np.zeros(1), np.zeros(pos.shape[0]), np.zeros(9)
@@ -328,15 +328,17 @@ def bench_c3(args):
         res[tag] = {"value": n / np.median(ts), "ms_per_step": 1e3 * float(np.median(ts)), "variant3": v3,
                     "variant4": v4, "counts": counts, "E": E[0], "F": F.copy(), "S": S.copy()}
         plan.close()
-    a, b = res["fp64"], res["mixed"]
-    err = {"energy_eV_per_atom": abs(b["E"] - a["E"]) * 0.0433641 / n,
-           "force_max_eV_per_A": float(np.abs(b["F"] - a["F"]).max() * 0.0433641),
-           "force_rms_eV_per_A": float(np.sqrt(np.mean((b["F"] - a["F"]) ** 2)) * 0.0433641),
-           "stress_max_rel": float(np.abs(b["S"] - a["S"]).max() / np.abs(a["S"]).max())}
+    a = res["fp64"]
+    for tag in ("mixed", "bf16", "bf16x3"):
+        b = res[tag]
+        out["%s_error_vs_fp64" % tag] = {
+            "energy_eV_per_atom": abs(b["E"] - a["E"]) * 0.0433641 / n,
+            "force_max_eV_per_A": float(np.abs(b["F"] - a["F"]).max() * 0.0433641),
+            "force_rms_eV_per_A": float(np.sqrt(np.mean((b["F"] - a["F"]) ** 2)) * 0.0433641),
+            "stress_max_rel": float(np.abs(b["S"] - a["S"]).max() / np.abs(a["S"]).max())}
     for r in res.values():
         del r["E"], r["F"], r["S"]
     out["forms"] = res
-    out["mixed_error_vs_fp64"] = err
     out["value"] = res["mixed"]["value"]
     out["ms_per_step"] = res["mixed"]["ms_per_step"]
     out["dtype"] = "f32-mixed"

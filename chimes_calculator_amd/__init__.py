@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libchimescalc_dl.so")
 
-FP64, MIXED = 0, 1
+FP64, MIXED, BF16, BF16X3 = 0, 1, 2, 3
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

@@ -40,7 +40,11 @@ typedef struct {
     const double *lastc4;        // [nquad]
 } chx_model_desc;
 
-typedef enum { CHX_FP64 = 0, CHX_MIXED = 1 } chx_precision;
+// FP64; FP32 recurrences and contractions; BF16 matrix-core contractions with
+// FP32 accumulation, with plain BF16 operands or with each operand split into
+// a BF16 high part and a BF16 residual (three products); everything else as
+// FP32-mixed.
+typedef enum { CHX_FP64 = 0, CHX_MIXED = 1, CHX_BF16 = 2, CHX_BF16X3 = 3 } chx_precision;
 
 // A batch of independent configurations.  Positions are xyz-interleaved
 // input coordinates; types are indices into the configuration's model.

@@ -33,10 +33,12 @@ void launch_eval2(long cap, hipStream_t st, const int *e_owner, const chx_cfg *c
                   const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
                   unsigned long long *Oacc, int *cfg_stat, int *status);
 // Vector-unit kernel for nb = 3 or 4 bodies.
-void launch_evalv(int nb, int D, bool mixed, const chx_list_args &la, hipStream_t st);
-// nb-body evaluation with the given kernel variant (CHX_VAR_*); the vector
-// kernel stands in where the matrix-core kernels do not take D.
-void launch_eval_body(int nb, int D, bool mixed, int variant, const chx_list_args &la, hipStream_t st);
+void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t st);
+// nb-body evaluation at precision prec (chx_precision) with the given kernel
+// variant (CHX_VAR_*; BF16 always uses the matrix cores); the vector kernel
+// (FP32 for the reduced precisions) stands in where the matrix-core kernels
+// do not take D.
+void launch_eval_body(int nb, int D, int prec, int variant, const chx_list_args &la, hipStream_t st);
 void launch_clusters(int nb, int D, int n, hipStream_t st, const double *dist, const double *drv, const int *types,
                      const chx_model_dev &M, double *E, double *F, double *S, double *last, int *flags);
 

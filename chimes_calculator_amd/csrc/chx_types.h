@@ -108,6 +108,7 @@ struct chx_group {
     const float *Cf;           // FP32 copy
     const double *Cm;          // 4-body matrix image for the matrix-core kernel (D <= 4)
     const float *Cmf;          // FP32 copy
+    const unsigned short *Cmb; // BF16 images (rounded, then residual), transposed: [column][row padded to 32]
     chx_slot s[6];
 };
 

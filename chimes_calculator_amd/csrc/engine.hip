@@ -175,7 +175,38 @@ struct TypeLaunch {
     const float *Cf = nullptr;   // device, FP32 copy for the mixed path
     const double *Cm = nullptr;  // device, matrix image for the matrix-core kernels
     const float *Cmf = nullptr;
+    const unsigned short *Cmb = nullptr;  // device, BF16 images (rounded, residual) transposed [column][row to 32]
 };
+
+// Round-to-nearest-even BF16 bits of x.
+unsigned short bf16_of(double x)
+{
+    const float f = (float)x;
+    unsigned u;
+    memcpy(&u, &f, 4);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// BF16 copies of a row-major KP x MP matrix image, transposed to [MP][K32]
+// with K32 = KP rounded up to 32 (zero padding): the rounded values, then the
+// rounded residuals (value - high part, in FP32).
+std::vector<unsigned short> bf16_transposed(const std::vector<double> &m, int KP, int MP)
+{
+    const int K32 = (KP + 31) / 32 * 32;
+    const size_t n = (size_t)MP * K32;
+    std::vector<unsigned short> t(2 * n, 0);
+    for (int k = 0; k < KP; k++)
+        for (int c = 0; c < MP; c++) {
+            const float f = (float)m[(size_t)k * MP + c];
+            const unsigned short h = bf16_of(f);
+            unsigned u = (unsigned)h << 16;
+            float hf;
+            memcpy(&hf, &u, 4);
+            t[(size_t)c * K32 + k] = h;
+            t[n + (size_t)c * K32 + k] = bf16_of(f - hf);
+        }
+    return t;
+}
 
 // Matrix image of one 4-body coefficient tensor for the matrix-core kernel:
 // Cm[k][m] with k = (a*D + b)*4 + c (pairs 01, 02, 03) and m = (d*D + e)*4 + f
@@ -274,7 +305,7 @@ struct chx_engine {
     std::vector<chx_group> g3h, g4h;
     std::vector<int> D3s, D4s;
     int ngl_max = 1;
-    bool mixed = false;
+    int prec = CHX_FP64;
     // kernel variants of the 3-/4-body evaluation (chosen by the plan autotuner)
     int var3 = CHX_VAR_MFMA4, var4 = CHX_VAR_MFMA4;
     bool capturing = false;  // inside a graph capture: no event records
@@ -438,6 +469,9 @@ chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
                     const std::vector<float> imgf(img.begin(), img.end());
                     m->t3[t].Cm = m->up(img.data(), img.size());
                     m->t3[t].Cmf = m->up(imgf.data(), imgf.size());
+                    const int D = d->D3, KP = (D + 3) / 4 * 4, MP = 16 * D;
+                    const std::vector<unsigned short> imgb = bf16_transposed(img, KP, MP);
+                    m->t3[t].Cmb = m->up(imgb.data(), imgb.size());
                 }
             }
             m->tripmap_h.assign(d->tripmap, d->tripmap + nt3);
@@ -469,6 +503,9 @@ chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
                     const std::vector<float> imgf(img.begin(), img.end());
                     m->t4[t].Cm = m->up(img.data(), img.size());
                     m->t4[t].Cmf = m->up(imgf.data(), imgf.size());
+                    const int D = d->D4, KP = D * D * 4, MP = (D * D * 4 + 15) / 16 * 16;
+                    const std::vector<unsigned short> imgb = bf16_transposed(img, KP, MP);
+                    m->t4[t].Cmb = m->up(imgb.data(), imgb.size());
                 }
             }
             m->quadmap_h.assign(d->quadmap, d->quadmap + nt4);
@@ -686,6 +723,7 @@ void prepare(chx_engine *e, chx_model *const *models, int nmodels, const chx_bat
                     g.Cf = tl.Cf;
                     g.Cm = tl.Cm;
                     g.Cmf = tl.Cmf;
+                    g.Cmb = tl.Cmb;
                     for (int k = 0; k < (body == 3 ? 3 : 6); k++) g.s[k] = tl.s[k];
                 }
                 gv.push_back(g);
@@ -695,7 +733,8 @@ void prepare(chx_engine *e, chx_model *const *models, int nmodels, const chx_bat
     }
     e->G3 = (int)e->g3h.size();
     e->G4 = (int)e->g4h.size();
-    e->mixed = b->precision == CHX_MIXED;
+    if (b->precision < CHX_FP64 || b->precision > CHX_BF16X3) throw EngineError("unknown precision");
+    e->prec = b->precision;
     guess_capacities(e, models);
 }
 
@@ -814,7 +853,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     hipStream_t st = e->st;
     const long N = e->N, Nin = e->Nin, ncell = e->ncell;
     const int ncfg = e->ncfg, G3 = e->G3, G4 = e->G4, ngl_max = e->ngl_max;
-    const bool mixed = e->mixed;
+    const int prec = e->prec;
     e->record(0);
     HIPCHK(hipMemcpyAsync(e->in_d.p, e->in_h.p, e->in_bytes, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, e->clear, e->nclear);
@@ -874,7 +913,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         l3.gend = e->ge3.p;
         l3.groups = e->grp3;
         l3.list = e->list3.p;
-        for (int D : e->D3s) launch_eval_body(3, D, mixed, e->var3, l3, st);
+        for (int D : e->D3s) launch_eval_body(3, D, prec, e->var3, l3, st);
     }
     e->record(3);
     if (G4) {
@@ -885,7 +924,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         l4.gend = e->ge4.p;
         l4.groups = e->grp4;
         l4.list = e->list4.p;
-        for (int D : e->D4s) launch_eval_body(4, D, mixed, e->var4, l4, st);
+        for (int D : e->D4s) launch_eval_body(4, D, prec, e->var4, l4, st);
     }
     e->record(4);
     hipLaunchKernelGGL(k_fin_forces, dim3(nblk(Nin, 256)), dim3(256), 0, st, (int)Nin, ncfg, e->cfg_in_off,
@@ -1107,7 +1146,7 @@ void plan_tune(chx_plan *p)
             float best_rep = 1e30f;
             for (int rep = 0; rep < 3; rep++) {
                 HIPCHK(hipEventRecord(a, e->st));
-                for (int D : Ds) launch_eval_body(body, D, e->mixed, var, la, e->st);
+                for (int D : Ds) launch_eval_body(body, D, e->prec, var, la, e->st);
                 HIPCHK(hipEventRecord(z, e->st));
                 HIPCHK(hipEventSynchronize(z));
                 float ms = 0;

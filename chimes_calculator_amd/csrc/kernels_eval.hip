@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "chx_dev.h"
+#include "chx_engine.h"
 #include "chx_eval.h"
 #include "eval_math.h"
 
@@ -362,6 +363,50 @@ struct lds_mat {
     __device__ __forceinline__ static int at(int k, int m) { return swz ? k * MP + (m ^ ((k & 3) << 4)) : k * LD + m; }
 };
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Round-to-nearest-even FP32 -> BF16 bits.
+__device__ __forceinline__ unsigned short bf16_bits(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Eight FP32 values as one BF16 MFMA operand (k = 8 (lane / 16) + q).
+__device__ __forceinline__ bf16x8 pack_bf16(const float (&x)[8])
+{
+    bf16x8 r;
+#pragma unroll
+    for (int q = 0; q < 8; q++) r[q] = __builtin_bit_cast(__bf16, bf16_bits(x[q]));
+    return r;
+}
+
+// Split of eight FP32 values into BF16 high parts and BF16 residuals
+// (x ~ hi + lo to about 16 significant bits).
+__device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8 &hi, bf16x8 &lo)
+{
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const unsigned short h = bf16_bits(x[q]);
+        hi[q] = __builtin_bit_cast(__bf16, h);
+        lo[q] = __builtin_bit_cast(__bf16, bf16_bits(x[q] - __uint_as_float((unsigned)h << 16)));
+    }
+}
+
+__device__ __forceinline__ flt4 mfma_bf16(const bf16x8 &a, const bf16x8 &b, flt4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS layout of a transposed BF16 image [MP][K32]: rows of K32 + 8 halves
+// (16 B of padding, so the 16 columns read by a lane group fall in different
+// banks), 16-byte aligned for the 8-value operand reads.
+template <int MP, int K32>
+struct lds_matb {
+    static constexpr int LD = K32 + 8;
+    static constexpr int size = MP * LD;
+};
+
 template <int D>
 struct mfma_shape {
     static constexpr int KP = D * D * 4;                // rows of Cm (k)
@@ -370,7 +415,7 @@ struct mfma_shape {
     static constexpr int RB = MP / 16;                  // 16-row output blocks
 };
 
-template <int D, typename R, int WPB>
+template <int D, typename R, int WPB, int BF>
 __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__restrict__ gbase,
                                                     const int *__restrict__ gend,
                                                     const chx_group *__restrict__ groups,
@@ -384,8 +429,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
     using MA = mfma_acc<R>;
     typedef typename MA::type acc_t;
     using LM = lds_mat<SH::KP, SH::MP>;
+    constexpr int K32 = (SH::KP + 31) / 32 * 32;
+    using LB = lds_matb<SH::MP, K32>;
     constexpr int TS = 6 * D + 1;     // per-cluster stride of the staged polynomials (odd: conflict-free)
-    __shared__ R Cl[LM::size];
+    // matrix image in LDS: R[KP][MP] (swizzled), or BF16 transposed [MP][K32 + 8]
+    // BF = 1: BF16 image; BF = 3: BF16 high and residual images (three-product split)
+    __shared__ __attribute__((aligned(16))) unsigned short ClRaw[BF ? (BF == 3 ? 2 : 1) * LB::size
+                                                                     : (int)(LM::size * sizeof(R) / 2)];
+    R *Cl = (R *)ClRaw;
     // staged polynomials; after the contraction the same memory holds the force table
     union stage_t {
         struct { R T[WPB][16 * TS], dT[WPB][16 * TS]; } p;
@@ -407,7 +458,12 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
         const int g = find_group(gbase, G, t0);
         const chx_group &grp = groups[g];
         if (grp.type < 0 || grp.D != D) return;
-        {
+        if (BF) {
+            for (int x = threadIdx.x; x < (BF == 3 ? 2 : 1) * SH::MP * K32; x += 64 * WPB) {
+                const int img = x / (SH::MP * K32), y = x % (SH::MP * K32);
+                ClRaw[img * LB::size + (y / K32) * LB::LD + y % K32] = grp.Cmb[x];
+            }
+        } else {
             const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
             for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 64 * WPB) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
         }
@@ -478,15 +534,55 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
 #pragma unroll
         for (int rb = 0; rb < SH::RB; rb++) {
             acc_t z0 = {0, 0, 0, 0}, z1 = {0, 0, 0, 0}, z2 = {0, 0, 0, 0}, z3 = {0, 0, 0, 0};
+            if constexpr (BF) {
+                // k = 32 s + 8 sub + q over (a, b, c) = (k / 4D, (k / 4) % D, k % 4)
 #pragma unroll
-            for (int s = 0; s < SH::KS; s++) {
-                const int a = s / D, b = s % D;
-                const R A = Cl[LM::at(4 * s + sub, 16 * rb + j)];
-                const R x01 = T0[a] * T1[b];
-                z0 = MA::op(A, x01 * t2, z0);
-                z1 = MA::op(A, (dT0[a] * T1[b]) * t2, z1);
-                z2 = MA::op(A, (T0[a] * dT1[b]) * t2, z2);
-                z3 = MA::op(A, x01 * dt2, z3);
+                for (int s = 0; s < K32 / 32; s++) {
+                    const bf16x8 A = *(const bf16x8 *)&ClRaw[(16 * rb + j) * LB::LD + 32 * s + 8 * sub];
+                    float x[8], y0[8], y1[8], y2[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int k = 32 * s + 8 * sub + q;
+                        const int a = k / (4 * D), b = (k / 4) % D, c = q & 3;
+                        const bool ok = k < SH::KP && c < D;
+                        const float t0 = ok ? (float)Tj[a] : 0.f, e0 = ok ? (float)dTj[a] : 0.f;
+                        const float t1 = ok ? (float)Tj[D + b] : 0.f, e1 = ok ? (float)dTj[D + b] : 0.f;
+                        const float tc = ok ? (float)Tj[2 * D + c] : 0.f, ec = ok ? (float)dTj[2 * D + c] : 0.f;
+                        x[q] = (t0 * t1) * tc;
+                        y0[q] = (e0 * t1) * tc;
+                        y1[q] = (t0 * e1) * tc;
+                        y2[q] = (t0 * t1) * ec;
+                    }
+                    if (BF == 1) {
+                        z0 = mfma_bf16(A, pack_bf16(x), z0);
+                        z1 = mfma_bf16(A, pack_bf16(y0), z1);
+                        z2 = mfma_bf16(A, pack_bf16(y1), z2);
+                        z3 = mfma_bf16(A, pack_bf16(y2), z3);
+                    } else {
+                        const bf16x8 Al =
+                            *(const bf16x8 *)&ClRaw[LB::size + (16 * rb + j) * LB::LD + 32 * s + 8 * sub];
+                        bf16x8 h, l;
+                        split_bf16(x, h, l);
+                        z0 = mfma_bf16(A, h, z0); z0 = mfma_bf16(A, l, z0); z0 = mfma_bf16(Al, h, z0);
+                        split_bf16(y0, h, l);
+                        z1 = mfma_bf16(A, h, z1); z1 = mfma_bf16(A, l, z1); z1 = mfma_bf16(Al, h, z1);
+                        split_bf16(y1, h, l);
+                        z2 = mfma_bf16(A, h, z2); z2 = mfma_bf16(A, l, z2); z2 = mfma_bf16(Al, h, z2);
+                        split_bf16(y2, h, l);
+                        z3 = mfma_bf16(A, h, z3); z3 = mfma_bf16(A, l, z3); z3 = mfma_bf16(Al, h, z3);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < SH::KS; s++) {
+                    const int a = s / D, b = s % D;
+                    const R A = Cl[LM::at(4 * s + sub, 16 * rb + j)];
+                    const R x01 = T0[a] * T1[b];
+                    z0 = MA::op(A, x01 * t2, z0);
+                    z1 = MA::op(A, (dT0[a] * T1[b]) * t2, z1);
+                    z2 = MA::op(A, (T0[a] * dT1[b]) * t2, z2);
+                    z3 = MA::op(A, x01 * dt2, z3);
+                }
             }
             if (MA::f64) {
 #pragma unroll
@@ -614,7 +710,7 @@ struct mfma3_shape {
     static constexpr int RB = D;
 };
 
-template <int D, typename R, int WPB>
+template <int D, typename R, int WPB, int BF>
 __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__restrict__ gbase,
                                                     const int *__restrict__ gend,
                                                     const chx_group *__restrict__ groups,
@@ -628,9 +724,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
     using MA = mfma_acc<R>;
     typedef typename MA::type acc_t;
     using LM = lds_mat<SH::KP, SH::MP>;
+    constexpr int K32 = (SH::KP + 31) / 32 * 32;
+    using LB = lds_matb<SH::MP, K32>;
     // staged polynomials per cluster: pair 01 [KP], pair 02 [D], pair 12 [16]; odd stride
     constexpr int O1 = SH::KP, O2 = SH::KP + D, TS = (SH::KP + D + 16) | 1;
-    __shared__ R Cl[LM::size];
+    // BF = 1: BF16 image; BF = 3: BF16 high and residual images (three-product split)
+    __shared__ __attribute__((aligned(16))) unsigned short ClRaw[BF ? (BF == 3 ? 2 : 1) * LB::size
+                                                                     : (int)(LM::size * sizeof(R) / 2)];
+    R *Cl = (R *)ClRaw;
     // staged polynomials; after the contraction the same memory holds the force table
     union stage_t {
         struct { R T[WPB][16 * TS], dT[WPB][16 * TS]; } p;
@@ -653,7 +754,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
     if (grp.type < 0 || grp.D != D) return;
     {
         const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
-        for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 64 * WPB) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
+        if (BF) {
+            for (int x = threadIdx.x; x < (BF == 3 ? 2 : 1) * SH::MP * K32; x += 64 * WPB) {
+                const int img = x / (SH::MP * K32), y = x % (SH::MP * K32);
+                ClRaw[img * LB::size + (y / K32) * LB::LD + y % K32] = grp.Cmb[x];
+            }
+        } else {
+            for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 64 * WPB) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
+        }
     }
     // ---- stage: geometry (all lanes of a cluster), lane sub < 3 the polynomials of pair sub
     const long c = t0 + wv * 16 + j;
@@ -718,11 +826,36 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
 #pragma unroll
     for (int rb = 0; rb < SH::RB; rb++) {
         acc_t z0 = {0, 0, 0, 0}, z1 = {0, 0, 0, 0};
+        if constexpr (BF) {
 #pragma unroll
-        for (int s = 0; s < SH::KS; s++) {
-            const R A = Cl[LM::at(4 * s + sub, 16 * rb + j)];
-            z0 = MA::op(A, Tj[4 * s + sub], z0);
-            z1 = MA::op(A, dTj[4 * s + sub], z1);
+            for (int s = 0; s < K32 / 32; s++) {
+                const bf16x8 A = *(const bf16x8 *)&ClRaw[(16 * rb + j) * LB::LD + 32 * s + 8 * sub];
+                float x[8], y[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int a = 32 * s + 8 * sub + q;
+                    x[q] = a < D ? (float)Tj[a < D ? a : 0] : 0.f;
+                    y[q] = a < D ? (float)dTj[a < D ? a : 0] : 0.f;
+                }
+                if (BF == 1) {
+                    z0 = mfma_bf16(A, pack_bf16(x), z0);
+                    z1 = mfma_bf16(A, pack_bf16(y), z1);
+                } else {
+                    const bf16x8 Al = *(const bf16x8 *)&ClRaw[LB::size + (16 * rb + j) * LB::LD + 32 * s + 8 * sub];
+                    bf16x8 h, l;
+                    split_bf16(x, h, l);
+                    z0 = mfma_bf16(A, h, z0); z0 = mfma_bf16(A, l, z0); z0 = mfma_bf16(Al, h, z0);
+                    split_bf16(y, h, l);
+                    z1 = mfma_bf16(A, h, z1); z1 = mfma_bf16(A, l, z1); z1 = mfma_bf16(Al, h, z1);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < SH::KS; s++) {
+                const R A = Cl[LM::at(4 * s + sub, 16 * rb + j)];
+                z0 = MA::op(A, Tj[4 * s + sub], z0);
+                z1 = MA::op(A, dTj[4 * s + sub], z1);
+            }
         }
         const R t1 = Tj[O1 + rb], dt1 = dTj[O1 + rb];
 #pragma unroll
@@ -1004,14 +1137,15 @@ void launch_eval2(long cap, hipStream_t st, const int *e_owner, const chx_cfg *c
                        Facc, Oacc, cfg_stat, status);
 }
 
-void launch_evalv(int nb, int D, bool mixed, const chx_list_args &la, hipStream_t st)
+void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t st)
 {
     if (la.cap <= 0 || la.G <= 0) return;
+    const bool f32 = prec != CHX_FP64;
     const dim3 grid((unsigned)((la.cap + 255) / 256)), block(256);
     const int word = nb == 3 ? CHX_ST_N3 : CHX_ST_N4;
 #define X(d)                                                                                                   \
     if (D == d) {                                                                                              \
-        if (mixed)                                                                                             \
+        if (f32)                                                                                               \
             hipLaunchKernelGGL((k_evalv<NB_, d, float>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups, \
                                word, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,  \
                                la.status);                                                                     \
@@ -1033,60 +1167,63 @@ void launch_evalv(int nb, int D, bool mixed, const chx_list_args &la, hipStream_
 #undef X
 }
 
-template <int WPB>
-bool launch_eval3m(int D, bool mixed, const chx_list_args &la, hipStream_t st)
-{
-    if (D > CHX_MFMA3_MAX_D) return false;
-    if (la.cap <= 0 || la.G <= 0) return true;
-    const dim3 grid((unsigned)std::max<long>(1, la.cap / (16 * WPB))), block(64 * WPB);
-#define X(d)                                                                                                   \
-    if (D == d) {                                                                                              \
-        if (mixed)                                                                                             \
-            hipLaunchKernelGGL((k_eval3m<d, float, WPB>), grid, block, 0, st, la.G, la.gbase, la.gend,          \
+// Matrix-core kernels: FP64 (prec 0), FP32 (1), BF16 operands (2) or split
+// BF16 operands, three products each (3); BF16 forms accumulate in FP32.
+#define CHX_LAUNCH_M(KER, d)                                                                                   \
+    {                                                                                                          \
+        if (prec == CHX_FP64)                                                                                  \
+            hipLaunchKernelGGL((KER<d, double, WPB, 0>), grid, block, 0, st, la.G, la.gbase, la.gend,           \
+                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.cfg_stat, la.status);                                                        \
+        else if (prec == CHX_MIXED)                                                                            \
+            hipLaunchKernelGGL((KER<d, float, WPB, 0>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
+                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.cfg_stat, la.status);                                                        \
+        else if (prec == CHX_BF16)                                                                             \
+            hipLaunchKernelGGL((KER<d, float, WPB, 1>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
                                la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
                                la.cfg_stat, la.status);                                                        \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_eval3m<d, double, WPB>), grid, block, 0, st, la.G, la.gbase, la.gend,         \
+            hipLaunchKernelGGL((KER<d, float, WPB, 3>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
                                la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
                                la.cfg_stat, la.status);                                                        \
         return true;                                                                                           \
     }
+
+template <int WPB>
+bool launch_eval3m(int D, int prec, const chx_list_args &la, hipStream_t st)
+{
+    if (D > CHX_MFMA3_MAX_D) return false;
+    if (la.cap <= 0 || la.G <= 0) return true;
+    const dim3 grid((unsigned)std::max<long>(1, la.cap / (16 * WPB))), block(64 * WPB);
+#define X(d) \
+    if (D == d) CHX_LAUNCH_M(k_eval3m, d)
     CHX_D3_CASES(X)
 #undef X
     return false;
 }
 
 template <int WPB>
-bool launch_eval4m(int D, bool mixed, const chx_list_args &la, hipStream_t st)
+bool launch_eval4m(int D, int prec, const chx_list_args &la, hipStream_t st)
 {
     if (D > CHX_MFMA_MAX_D) return false;
     if (la.cap <= 0 || la.G <= 0) return true;
     const dim3 grid((unsigned)std::max<long>(1, la.cap / (16 * WPB))), block(64 * WPB);
-#define X(d)                                                                                                   \
-    if (D == d) {                                                                                              \
-        if (mixed)                                                                                             \
-            hipLaunchKernelGGL((k_eval4m<d, float, WPB>), grid, block, 0, st, la.G, la.gbase, la.gend,          \
-                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
-                               la.cfg_stat, la.status);                                                        \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_eval4m<d, double, WPB>), grid, block, 0, st, la.G, la.gbase, la.gend,         \
-                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
-                               la.cfg_stat, la.status);                                                        \
-        return true;                                                                                           \
-    }
+#define X(d) \
+    if (D == d) CHX_LAUNCH_M(k_eval4m, d)
     CHX_D4M_CASES(X)
 #undef X
     return false;
 }
 
-void launch_eval_body(int nb, int D, bool mixed, int variant, const chx_list_args &la, hipStream_t st)
+void launch_eval_body(int nb, int D, int prec, int variant, const chx_list_args &la, hipStream_t st)
 {
     bool done = false;
-    if (variant == CHX_VAR_MFMA4)
-        done = nb == 3 ? launch_eval3m<4>(D, mixed, la, st) : launch_eval4m<4>(D, mixed, la, st);
+    if (variant == CHX_VAR_MFMA4 || prec >= CHX_BF16)
+        done = nb == 3 ? launch_eval3m<4>(D, prec, la, st) : launch_eval4m<4>(D, prec, la, st);
     else if (variant == CHX_VAR_MFMA2)
-        done = nb == 3 ? launch_eval3m<2>(D, mixed, la, st) : launch_eval4m<2>(D, mixed, la, st);
-    if (!done) launch_evalv(nb, D, mixed, la, st);
+        done = nb == 3 ? launch_eval3m<2>(D, prec, la, st) : launch_eval4m<2>(D, prec, la, st);
+    if (!done) launch_evalv(nb, D, prec, la, st);
 }
 
 void launch_clusters(int nb, int D, int n, hipStream_t st, const double *dist, const double *drv, const int *types,

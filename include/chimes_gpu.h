@@ -16,7 +16,12 @@ extern "C" {
 
 typedef struct chimes_gpu_model chimes_gpu_model;
 
-enum { CHIMES_GPU_FP64 = 0, CHIMES_GPU_MIXED = 1 };
+/* Precision of the 3-/4-body terms: FP64; FP32-mixed (FP32 polynomials and
+ * contractions, FP64 distances, cutoffs, force scalars and accumulation);
+ * BF16 (as FP32-mixed, with BF16 matrix-core contractions accumulated in
+ * FP32); BF16X3 (each BF16 operand split into a high part and a residual,
+ * three products per term).  The 2-body term is always FP64. */
+enum { CHIMES_GPU_FP64 = 0, CHIMES_GPU_MIXED = 1, CHIMES_GPU_BF16 = 2, CHIMES_GPU_BF16X3 = 3 };
 
 /* Parses a ChIMES parameter file and uploads its tables to GPU `device`. */
 chimes_gpu_model *chimes_gpu_model_load(const char *param_file, int device);
@@ -36,7 +41,7 @@ double chimes_gpu_model_max_cutoff(const chimes_gpu_model *m, int body);  /* bod
  *   types[sum natoms]                  atom type indices of each configuration's model
  *   cells[9*nconf]                     cell vectors a, b, c
  *   small                              1: replicate cells shorter than the 2B cutoff
- *   precision                          CHIMES_GPU_FP64 or CHIMES_GPU_MIXED
+ *   precision                          CHIMES_GPU_FP64, _MIXED, _BF16 or _BF16X3
  * Outputs (overwritten): energy[nconf] (kcal/mol, 1B..4B, divided by (R+1)^3
  * when replicated), forces[3*sum natoms] (kcal/mol/A), stress[9*nconf]
  * (virial / volume, kcal/mol/A^3, xx xy xz yx yy yz zx zy zz). */
