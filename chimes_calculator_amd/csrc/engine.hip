@@ -1142,7 +1142,9 @@ void plan_tune(chx_plan *p)
         const std::vector<int> &Ds = body == 3 ? e->D3s : e->D4s;
         int best = CHX_VAR_MFMA4;
         double best_ms = 1e30;
-        for (int var = 0; var < CHX_NVAR; var++) {
+        // the BF16 forms are defined by their matrix-core contraction: no vector variant
+        const int nvar = e->prec >= CHX_BF16 ? CHX_VAR_VECTOR : CHX_NVAR;
+        for (int var = 0; var < nvar; var++) {
             float best_rep = 1e30f;
             for (int rep = 0; rep < 3; rep++) {
                 HIPCHK(hipEventRecord(a, e->st));

@@ -1219,7 +1219,7 @@ bool launch_eval4m(int D, int prec, const chx_list_args &la, hipStream_t st)
 void launch_eval_body(int nb, int D, int prec, int variant, const chx_list_args &la, hipStream_t st)
 {
     bool done = false;
-    if (variant == CHX_VAR_MFMA4 || prec >= CHX_BF16)
+    if (variant == CHX_VAR_MFMA4 || (variant == CHX_VAR_VECTOR && prec >= CHX_BF16))
         done = nb == 3 ? launch_eval3m<4>(D, prec, la, st) : launch_eval4m<4>(D, prec, la, st);
     else if (variant == CHX_VAR_MFMA2)
         done = nb == 3 ? launch_eval3m<2>(D, prec, la, st) : launch_eval4m<2>(D, prec, la, st);
