@@ -149,6 +149,7 @@ __device__ __forceinline__ void fx_runs3(unsigned long long *acc, int key, const
 // distinct atom.  CAP must exceed the number of contributions per flush.
 template <int CAP>
 struct fx_table {
+    static_assert((CAP & (CAP - 1)) == 0, "table capacity must be a power of two");
     int key[CAP];
     unsigned long long val[CAP][6];  // (hi, lo) per component
 

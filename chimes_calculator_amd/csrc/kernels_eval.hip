@@ -66,7 +66,8 @@ __device__ __forceinline__ int find_group(const int *__restrict__ gbase, int G, 
 
 // ---------------------------------------------------------------------------
 // 2-body: one thread per neighbour entry.  F_owner += f dr, F_image -= f dr,
-// virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).
+// virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).  The workgroup's
+// forces are merged per atom in an LDS table before they reach memory.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owner, const chx_cfg *__restrict__ cfgs,
                                                 const chx_atom_s *__restrict__ as,
@@ -74,7 +75,10 @@ __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owne
                                                 const chx_model_dev *__restrict__ models, unsigned long long *Facc,
                                                 unsigned long long *Oacc, int *cfg_stat, int *status)
 {
+    __shared__ fx_table<512> tab;
     if (block_stop(status, CHX_STOP)) return;
+    tab.clear();
+    __syncthreads();
     const long nent = status[CHX_ST_NENT];
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int owner = -1, k1 = -1;
@@ -106,9 +110,11 @@ __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owne
         q[6] = -(fs * (dr[2] * dr[2]));
         note_flags(cfg_stat, ai.cfg, flags);
     }
-    fx_runs3<64>(Facc, owner, F0, bad);
-    fx_runs3<64>(Facc, k1, F1, bad);
+    tab.add(owner, F0, bad);
+    tab.add(k1, F1, bad);
     fx_owner<7>(Oacc, owner, q, bad);
+    __syncthreads();
+    tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
 }
 
@@ -241,8 +247,8 @@ __device__ __forceinline__ void cluster_forces(const double *h, const double (&p
 
 // ---------------------------------------------------------------------------
 // 3- and 4-body on the vector units: one thread per cluster, one wavefront per
-// 64-cluster tile of one group (any supported D; the 4-body matrix-core kernel
-// below takes D <= 4).
+// 64-cluster tile of one group (any supported D); the workgroup's member
+// forces are merged per atom in an LDS table before they reach memory.
 // ---------------------------------------------------------------------------
 template <int NB, int D, typename R>
 __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
@@ -254,16 +260,19 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
 {
     using PQ = pairs_of<NB>;
     constexpr int P = PQ::P;
+    __shared__ fx_table<1024> tab;  // >= 256 clusters x NB members
     if (block_stop(status, CHX_STOP)) return;
+    tab.clear();
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     // wave index made explicitly uniform, so the group, its slots and its
     // coefficients are read through the scalar unit
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const long t0 = ((long)blockIdx.x * 4 + wave) * 64;
-    if (t0 >= status[total_word]) return;
-    const int g = find_group(gbase, G, t0);
+    bool wave_on = t0 < status[total_word];
+    const int g = wave_on ? find_group(gbase, G, t0) : 0;
     const chx_group &grp = groups[g];
-    if (grp.type < 0 || grp.D != D) return;
+    wave_on = wave_on && grp.type >= 0 && grp.D == D;
     const long c = t0 + lane;
     int owner = -1;
     double q7[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -272,7 +281,7 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
 #pragma unroll
     for (int m = 0; m < NB; m++) { key[m] = -1; Fr[m][0] = Fr[m][1] = Fr[m][2] = 0.0; }
     bool bad = false;
-    if (c < gend[g]) {
+    if (wave_on && c < gend[g]) {
         const int2 rec = list[c];
         double p[NB][3], r[P];
         int par[NB], cfg, pm[NB];
@@ -306,8 +315,10 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
         }
     }
 #pragma unroll
-    for (int m = 0; m < NB; m++) fx_runs3<64>(Facc, key[m], Fr[m], bad);
+    for (int m = 0; m < NB; m++) tab.add(key[m], Fr[m], bad);
     fx_owner<7>(Oacc, owner, q7, bad);
+    __syncthreads();
+    tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
 }
 
