@@ -51,92 +51,68 @@ __device__ __forceinline__ void fx_atomic(unsigned long long *acc, double v, boo
     if (lo) atomicAdd(acc + 1, (unsigned long long)lo);
 }
 
-// Adds K per-lane values into the accumulators of each lane's owner
-// (acc[(K*owner + k)*2 + {0,1}]); owner < 0 contributes nothing.  Must be
-// reached by all 64 lanes.  When every contributing lane has the same owner
-// the integer parts are first summed across the wave (exactly), then added
-// once.
-template <int K>
-__device__ __forceinline__ void fx_owner(unsigned long long *acc, int owner, const double (&q)[K], bool &bad)
-{
-    long long hi[K], lo[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        hi[k] = 0;
-        lo[k] = 0;
-        if (owner >= 0) fx_split(q[k], hi[k], lo[k], bad);
-    }
-    const unsigned long long act = __ballot(owner >= 0);
-    if (act == 0) return;
-    const int o0 = __shfl(owner, __builtin_ctzll(act), 64);
-    if (__ballot(owner >= 0 && owner != o0) == 0) {
-#pragma unroll
-        for (int w = 32; w >= 1; w >>= 1)
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                hi[k] += __shfl_xor(hi[k], w, 64);
-                lo[k] += __shfl_xor(lo[k], w, 64);
-            }
-        if ((threadIdx.x & 63) == 0)
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                unsigned long long *a = acc + 2L * ((long)K * o0 + k);
-                if (hi[k]) atomicAdd(a, (unsigned long long)hi[k]);
-                if (lo[k]) atomicAdd(a + 1, (unsigned long long)lo[k]);
-            }
-    } else if (owner >= 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            unsigned long long *a = acc + 2L * ((long)K * owner + k);
-            if (hi[k]) atomicAdd(a, (unsigned long long)hi[k]);
-            if (lo[k]) atomicAdd(a + 1, (unsigned long long)lo[k]);
-        }
-    }
-}
+// Segmented sums over runs of equal keys in groups of W consecutive lanes
+// (W = 16 or 64).  Clusters and pair entries are stored sorted by owner, so
+// the owners of a wavefront's lanes come in a few runs: each run's integer
+// parts are summed exactly across lanes and only the run's last lane issues
+// global atomics.  Must be reached by all lanes of the wavefront.
 
-// Adds a 3-vector per lane into acc[key] (acc[(3*key + k)*2 + {0,1}]) for
-// groups of W consecutive lanes (W = 16 or 64) whose keys come in runs: the
-// integer parts are summed over each run of equal keys (a segmented scan,
-// exact), and only the last lane of a run issues the atomics.  key < 0
-// contributes nothing.  Must be reached by all lanes of the wavefront.
+// first lane (within the group) of this lane's run
 template <int W>
-__device__ __forceinline__ void fx_runs3(unsigned long long *acc, int key, const double (&v)[3], bool &bad)
+__device__ __forceinline__ int fx_run_first(int key)
 {
-    long long hi[3], lo[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        hi[k] = 0;
-        lo[k] = 0;
-        if (key >= 0) fx_split(v[k], hi[k], lo[k], bad);
-    }
     const int l = threadIdx.x & (W - 1);
     const int kprev = __shfl_up(key, 1, W);
-    const int knext = __shfl_down(key, 1, W);
-    int seg = (l == 0 || kprev != key) ? l : 0;  // first lane of this lane's run
+    int seg = (l == 0 || kprev != key) ? l : 0;
 #pragma unroll
     for (int d = 1; d < W; d <<= 1) {
         const int t = __shfl_up(seg, d, W);
         if (l >= d && t > seg) seg = t;
     }
+    return seg;
+}
+
+// inclusive segmented scan of K integers: lane l ends with the sum over
+// lanes seg..l of its run
+template <int W, int K>
+__device__ __forceinline__ void fx_run_scan(long long (&x)[K], int seg)
+{
+    const int l = threadIdx.x & (W - 1);
 #pragma unroll
     for (int d = 1; d < W; d <<= 1) {
-        long long th[3], tl[3];
+        long long t[K];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            th[k] = __shfl_up(hi[k], d, W);
-            tl[k] = __shfl_up(lo[k], d, W);
-        }
+        for (int k = 0; k < K; k++) t[k] = __shfl_up(x[k], d, W);
         if (l - d >= seg)
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                hi[k] += th[k];
-                lo[k] += tl[k];
-            }
+            for (int k = 0; k < K; k++) x[k] += t[k];
     }
+}
+
+// Adds the owner's force f into Facc[key] (acc[(3*key + k)*2 + {0,1}]) and
+// the energy and virial terms q into Oacc[key] (acc[(7*key + k)*2 + {0,1}]),
+// one atomic pair per nonzero component and run.  key < 0 contributes nothing.
+template <int W>
+__device__ __forceinline__ void fx_owner_runs(unsigned long long *Facc, unsigned long long *Oacc, int key,
+                                              const double (&f)[3], const double (&q)[7], bool &bad)
+{
+    if (__ballot(key >= 0) == 0) return;
+    long long hi[10], lo[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        hi[k] = 0;
+        lo[k] = 0;
+        if (key >= 0) fx_split(k < 3 ? f[k] : q[k - 3], hi[k], lo[k], bad);
+    }
+    const int seg = fx_run_first<W>(key);
+    fx_run_scan<W, 10>(hi, seg);
+    fx_run_scan<W, 10>(lo, seg);
+    const int l = threadIdx.x & (W - 1);
+    const int knext = __shfl_down(key, 1, W);
     if (key >= 0 && (l == W - 1 || knext != key)) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            unsigned long long *a = acc + 2L * (3L * key + k);
+        for (int k = 0; k < 10; k++) {
+            unsigned long long *a = k < 3 ? Facc + 2L * (3L * key + k) : Oacc + 2L * (7L * key + (k - 3));
             if (hi[k]) atomicAdd(a, (unsigned long long)hi[k]);
             if (lo[k]) atomicAdd(a + 1, (unsigned long long)lo[k]);
         }

@@ -136,6 +136,11 @@ __device__ __forceinline__ void contract3(const R *__restrict__ C, const R (&T)[
                                           A &S, A (&G)[3])
 {
     A s0 = 0, g00 = 0, g01 = 0, g02 = 0;
+    // T[0] and dT[0] are consumed from the front of rotating copies: the a0
+    // loop is not unrolled, and indexing by a0 would place them in scratch
+    R t0v[D], d0v[D];
+#pragma unroll
+    for (int u = 0; u < D; u++) { t0v[u] = T[0][u]; d0v[u] = dT[0][u]; }
 #pragma unroll 1
     for (int a0 = 0; a0 < D; a0++) {
         A s1 = 0, g11 = 0, g12 = 0;
@@ -152,10 +157,13 @@ __device__ __forceinline__ void contract3(const R *__restrict__ C, const R (&T)[
             g11 = fma((A)dT[1][a1], (A)z, g11);
             g12 = fma((A)T[1][a1], (A)z2, g12);
         }
-        s0 = fma((A)T[0][a0], s1, s0);
-        g00 = fma((A)dT[0][a0], s1, g00);
-        g01 = fma((A)T[0][a0], g11, g01);
-        g02 = fma((A)T[0][a0], g12, g02);
+        const A t0 = (A)t0v[0], dt0 = (A)d0v[0];
+        s0 = fma(t0, s1, s0);
+        g00 = fma(dt0, s1, g00);
+        g01 = fma(t0, g11, g01);
+        g02 = fma(t0, g12, g02);
+#pragma unroll
+        for (int u = 0; u + 1 < D; u++) { t0v[u] = t0v[u + 1]; d0v[u] = d0v[u + 1]; }
     }
     S = s0;
     G[0] = g00; G[1] = g01; G[2] = g02;
@@ -169,9 +177,16 @@ __device__ __forceinline__ void contract4(const R *__restrict__ C, const R (&T)[
                                           A &S, A (&G)[6])
 {
     A s0 = 0, h0[6] = {0, 0, 0, 0, 0, 0};
+    // rotating copies of T[0], dT[0], T[1], dT[1], as in contract3
+    R t0v[D], d0v[D];
+#pragma unroll
+    for (int u = 0; u < D; u++) { t0v[u] = T[0][u]; d0v[u] = dT[0][u]; }
 #pragma unroll 1
     for (int a0 = 0; a0 < D; a0++) {
         A s1 = 0, h1[5] = {0, 0, 0, 0, 0};
+        R t1v[D], d1v[D];
+#pragma unroll
+        for (int u = 0; u < D; u++) { t1v[u] = T[1][u]; d1v[u] = dT[1][u]; }
 #pragma unroll 1
         for (int a1 = 0; a1 < D; a1++) {
             R s2 = 0, h2[4] = {0, 0, 0, 0};
@@ -204,15 +219,21 @@ __device__ __forceinline__ void contract4(const R *__restrict__ C, const R (&T)[
 #pragma unroll
                 for (int q = 0; q < 3; q++) h2[q + 1] = fma(T[2][a2], h3[q], h2[q + 1]);
             }
-            s1 = fma((A)T[1][a1], (A)s2, s1);
-            h1[0] = fma((A)dT[1][a1], (A)s2, h1[0]);
+            const A t1 = (A)t1v[0], dt1 = (A)d1v[0];
+            s1 = fma(t1, (A)s2, s1);
+            h1[0] = fma(dt1, (A)s2, h1[0]);
 #pragma unroll
-            for (int q = 0; q < 4; q++) h1[q + 1] = fma((A)T[1][a1], (A)h2[q], h1[q + 1]);
+            for (int q = 0; q < 4; q++) h1[q + 1] = fma(t1, (A)h2[q], h1[q + 1]);
+#pragma unroll
+            for (int u = 0; u + 1 < D; u++) { t1v[u] = t1v[u + 1]; d1v[u] = d1v[u + 1]; }
         }
-        s0 = fma((A)T[0][a0], s1, s0);
-        h0[0] = fma((A)dT[0][a0], s1, h0[0]);
+        const A t0 = (A)t0v[0], dt0 = (A)d0v[0];
+        s0 = fma(t0, s1, s0);
+        h0[0] = fma(dt0, s1, h0[0]);
 #pragma unroll
-        for (int q = 0; q < 5; q++) h0[q + 1] = fma((A)T[0][a0], h1[q], h0[q + 1]);
+        for (int q = 0; q < 5; q++) h0[q + 1] = fma(t0, h1[q], h0[q + 1]);
+#pragma unroll
+        for (int u = 0; u + 1 < D; u++) { t0v[u] = t0v[u + 1]; d0v[u] = d0v[u + 1]; }
     }
     S = s0;
 #pragma unroll

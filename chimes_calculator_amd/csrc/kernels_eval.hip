@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owne
                                                 const chx_model_dev *__restrict__ models, unsigned long long *Facc,
                                                 unsigned long long *Oacc, int *cfg_stat, int *status)
 {
-    __shared__ fx_table<512> tab;
+    __shared__ fx_table<256> tab;  // neighbour side of the block's 256 entries
     if (block_stop(status, CHX_STOP)) return;
     tab.clear();
     __syncthreads();
@@ -110,9 +110,8 @@ __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owne
         q[6] = -(fs * (dr[2] * dr[2]));
         note_flags(cfg_stat, ai.cfg, flags);
     }
-    tab.add(owner, F0, bad);
     tab.add(k1, F1, bad);
-    fx_owner<7>(Oacc, owner, q, bad);
+    fx_owner_runs<64>(Facc, Oacc, owner, F0, q, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -260,7 +259,7 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
 {
     using PQ = pairs_of<NB>;
     constexpr int P = PQ::P;
-    __shared__ fx_table<1024> tab;  // >= 256 clusters x NB members
+    __shared__ fx_table<(NB == 3 ? 512 : 1024)> tab;  // >= 256 clusters x (NB - 1) non-owner members
     if (block_stop(status, CHX_STOP)) return;
     tab.clear();
     __syncthreads();
@@ -315,8 +314,9 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
         }
     }
 #pragma unroll
-    for (int m = 0; m < NB; m++) tab.add(key[m], Fr[m], bad);
-    fx_owner<7>(Oacc, owner, q7, bad);
+    for (int m = 1; m < NB; m++) tab.add(key[m], Fr[m], bad);
+    // runtime member 0 is the owner
+    fx_owner_runs<64>(Facc, Oacc, owner, Fr[0], q7, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -697,8 +697,16 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
             }
             owner = rec.x;
         }
-        st.tab.add(act ? key_s : -1, F, bad);
-        fx_owner<7>(Oacc, owner, q7, bad);
+        // energy and virial of each cluster summed over its four lanes; the
+        // owner's force (runtime member 0, lane sub 0) and these go to the
+        // owner's accumulators per run of equal owners, the other members'
+        // forces through the LDS table
+#pragma unroll
+        for (int w = 16; w <= 32; w <<= 1)
+#pragma unroll
+            for (int k = 0; k < 7; k++) q7[k] += __shfl_xor(q7[k], w, 64);
+        st.tab.add((act && sub != 0) ? key_s : -1, F, bad);
+        fx_owner_runs<16>(Facc, Oacc, sub == 0 ? owner : -1, F, q7, bad);
         __syncthreads();
         st.tab.flush(Facc);
         if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -933,8 +941,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
         }
         owner = rec.x;
     }
-    st.tab.add((act && sub < 3) ? key_s : -1, F, bad);
-    fx_owner<7>(Oacc, owner, q7, bad);
+    // as in the 4-body kernel: per-cluster energy and virial over the lanes,
+    // owner terms per run, other members through the LDS table
+#pragma unroll
+    for (int w = 16; w <= 32; w <<= 1)
+#pragma unroll
+        for (int k = 0; k < 7; k++) q7[k] += __shfl_xor(q7[k], w, 64);
+    st.tab.add((act && sub != 0 && sub < 3) ? key_s : -1, F, bad);
+    fx_owner_runs<16>(Facc, Oacc, sub == 0 ? owner : -1, F, q7, bad);
     __syncthreads();
     st.tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
