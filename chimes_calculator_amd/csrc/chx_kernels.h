@@ -31,11 +31,27 @@ __global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, con
 __global__ void k_check_entries(const int *nbr_off, int natoms, long cap, int *status);
 __global__ void k_group_bases(const int *off, long N, int G, long cap, int *base, int *end, int *status, int word,
                               int retry_bit);
+// arguments of the enumeration kernels
+struct chx_enum_args {
+    int natoms;
+    const chx_cfg *cfgs;
+    const chx_atom_s *as;
+    const chx_atom_n *an;
+    const int *nbr_off;
+    const chx_entry_s *es;
+    const chx_entry_n *en;
+    const chx_enum_model *em;
+    int *cnt3, *cnt4;              // MODE 0 output: per (group, owner) counts [G][N]
+    const int *off3, *off4;        // MODE 1: exclusive scans of the counts
+    const int *gb3, *gb4;          // MODE 1: aligned group bases
+    int2 *list3, *list4;           // MODE 1 output
+    int *big;                      // [natoms]: owners queued for k_enum_big
+    int *status;
+};
 template <int MODE>
-__global__ void k_enum(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
-                       const int *nbr_off, const chx_entry_s *es, const chx_entry_n *en, const chx_enum_model *em,
-                       int *cnt3, int *cnt4, const int *off3, const int *off4, const int *gb3, const int *gb4,
-                       int2 *list3, int2 *list4, int *status, int ngl_max);
+__global__ void k_enum(chx_enum_args x);
+template <int MODE>
+__global__ void k_enum_big(chx_enum_args x);
 __global__ void k_scan_block(const int *in, int *out, long n, int *bsum);
 __global__ void k_scan_add(int *out, long n, const int *bscan);
 __global__ void k_scan_one(const int *in, int *out, long n);

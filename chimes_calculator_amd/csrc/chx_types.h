@@ -6,6 +6,8 @@
 #define CHX_MAX_NB 512         // neighbour images per owner atom within the search cutoff
 #define CHX_MAX_N3 512         // images per owner within the 3-body cutoff
 #define CHX_MAX_N4 256         // images per owner within the 4-body cutoff
+#define CHX_NBR_CELLS 128      // searched cells per owner for the flattened candidate loop
+#define CHX_ENUM_SMALL 128     // owners with at most this many images use the small-LDS enumeration
 #define CHX_MAX_CLUSTER_TYPES 64
 #define CHX_LIST_ALIGN 64      // 3-/4-body list segments of one cluster type start on this multiple
 #define CHX_MFMA_MAX_D 4       // 4-body dense dimension handled by the matrix-core kernel
@@ -20,6 +22,7 @@ enum {
     CHX_ST_N4 = 4,             // quadruplet list slots needed (aligned)
     CHX_ST_C3 = 5,             // triplets listed
     CHX_ST_C4 = 6,             // quadruplets listed
+    CHX_ST_NBIG = 7,           // owners queued for the large enumeration kernel
     CHX_ST_WORDS = 8
 };
 enum {

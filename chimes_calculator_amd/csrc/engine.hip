@@ -333,7 +333,7 @@ struct chx_engine {
     DevBuf<chx_atom_s> as;
     DevBuf<chx_atom_n> an;
     DevBuf<int> cell_cnt, cell_start, cell_cur, cell_atoms;
-    DevBuf<int> nbr_cnt, nbr_off, e_owner;
+    DevBuf<int> nbr_cnt, nbr_off, e_owner, big;
     DevBuf<chx_entry_s> es;
     DevBuf<chx_entry_n> en;
     DevBuf<int> cnt3, off3, cnt4, off4;
@@ -754,6 +754,7 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->cell_cur.ensure(ncell + 1);
     e->cell_atoms.ensure(N);
     e->nbr_cnt.ensure(N + 1);
+    e->big.ensure(N + 1);
     e->nbr_off.ensure(N + 1);
     e->cnt3.ensure((long)G3 * N + 1);
     e->off3.ensure((long)G3 * N + 2);
@@ -852,7 +853,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
 {
     hipStream_t st = e->st;
     const long N = e->N, Nin = e->Nin, ncell = e->ncell;
-    const int ncfg = e->ncfg, G3 = e->G3, G4 = e->G4, ngl_max = e->ngl_max;
+    const int ncfg = e->ncfg, G3 = e->G3, G4 = e->G4;
     const int prec = e->prec;
     e->record(0);
     HIPCHK(hipMemcpyAsync(e->in_d.p, e->in_h.p, e->in_bytes, hipMemcpyHostToDevice, st));
@@ -873,10 +874,29 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
                        e->e_owner.p, e->status);
     // 3-/4-body lists in type-aligned segments
     if (G3 + G4 > 0) {
-        hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
-                           e->nbr_off.p, e->es.p, e->en.p, e->em, e->cnt3.p, e->cnt4.p, (const int *)nullptr,
-                           (const int *)nullptr, (const int *)nullptr, (const int *)nullptr, (int2 *)nullptr,
-                           (int2 *)nullptr, e->status, ngl_max);
+        chx_enum_args ea{};
+        ea.natoms = (int)N;
+        ea.cfgs = e->cfgs;
+        ea.as = e->as.p;
+        ea.an = e->an.p;
+        ea.nbr_off = e->nbr_off.p;
+        ea.es = e->es.p;
+        ea.en = e->en.p;
+        ea.em = e->em;
+        ea.cnt3 = e->cnt3.p;
+        ea.cnt4 = e->cnt4.p;
+        ea.off3 = e->off3.p;
+        ea.off4 = e->off4.p;
+        ea.gb3 = e->gb3.p;
+        ea.gb4 = e->gb4.p;
+        ea.list3 = e->list3.p;
+        ea.list4 = e->list4.p;
+        ea.big = e->big.p;
+        ea.status = e->status;
+        // owners with many images are queued by k_enum<0> for the large-LDS kernel
+        const dim3 gbig(1280);
+        hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, ea);
+        hipLaunchKernelGGL((k_enum_big<0>), gbig, dim3(64), 0, st, ea);
         if (G3) {
             e->scan(e->cnt3.p, e->off3.p, (long)G3 * N);
             hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off3.p, N, G3, e->l3_cap, e->gb3.p,
@@ -887,9 +907,8 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
             hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off4.p, N, G4, e->l4_cap, e->gb4.p,
                                e->ge4.p, e->status, (int)CHX_ST_N4, (int)CHX_RETRY_L4);
         }
-        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
-                           e->nbr_off.p, e->es.p, e->en.p, e->em, e->cnt3.p, e->cnt4.p, e->off3.p, e->off4.p,
-                           e->gb3.p, e->gb4.p, e->list3.p, e->list4.p, e->status, ngl_max);
+        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st, ea);
+        hipLaunchKernelGGL((k_enum_big<1>), gbig, dim3(64), 0, st, ea);
     }
     e->record(1);
     // evaluation with exact accumulation

@@ -147,53 +147,126 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
                                               chx_entry_n *en, int *e_owner, int *status)
 {
     __shared__ long long keys[4][MODE ? CHX_MAX_NB : 1];
+    // searched cells of each wave's owner: inclusive prefix of their atom
+    // counts, first cell_atoms slot, packed image shift
+    __shared__ int cpre[4][CHX_NBR_CELLS], cbeg[4][CHX_NBR_CELLS], cshf[4][CHX_NBR_CELLS];
     if (MODE == 1 && block_stop(status, CHX_STOP)) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + wv;
+    const bool own = i < natoms;
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
     int n = 0;
-    if (i < natoms) {
-        const chx_atom_n ni = an[i];
-        const chx_cfg &C = cfgs[as[i].cfg];
+    chx_atom_n ni{};
+    const chx_cfg &C = cfgs[own ? as[i].cfg : 0];
+    int ci[3] = {0, 0, 0}, nc = 0;
+    // searched cell c = (dx, dy, dz) offsets from the owner's cell, x fastest:
+    // the wrapped cell, the image shift, and whether the shift is inside the
+    // ghost layers
+    auto searched_cell = [&](int dx, int dy, int dz, int &cell, int (&sh)[3]) {
+        const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
+        int wc[3];
+        bool ok = true;
+        for (int d = 0; d < 3; d++) {
+            int q = u[d] % C.m[d];
+            if (q < 0) q += C.m[d];
+            wc[d] = q;
+            sh[d] = (u[d] - q) / C.m[d];
+            if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
+        }
+        cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
+        return ok;
+    };
+    // one candidate per lane: image (a, sh) of the owner's list?
+    auto take = [&](bool valid, int a, const int (&sh)[3], int spack) {
+        bool in = false;
+        if (valid) {
+            const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
+            if (!(a < i || (a == i && zero))) {
+                double vn[3];
+                in = nbr_image(C, ni.u, an[a].u, sh, zero, vn);
+            }
+        }
+        const unsigned long long b = __ballot(in);
+        if (MODE == 1 && in) {
+            const int idx = n + __popcll(b & below);
+            if (idx < CHX_MAX_NB) keys[wv][idx] = ((long long)a << 24) | spack;
+        }
+        n += __popcll(b);
+    };
+    if (own) {
+        ni = an[i];
         const int lc = ni.cell - C.cell_off;
-        const int ci[3] = {lc % C.m[0], (lc / C.m[0]) % C.m[1], lc / (C.m[0] * C.m[1])};
-        for (int dz = -C.reach[2]; dz <= C.reach[2]; dz++)
-            for (int dy = -C.reach[1]; dy <= C.reach[1]; dy++)
-                for (int dx = -C.reach[0]; dx <= C.reach[0]; dx++) {
-                    const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
-                    int wc[3], sh[3];
-                    bool ok = true;
-                    for (int d = 0; d < 3; d++) {
-                        int q = u[d] % C.m[d];
-                        if (q < 0) q += C.m[d];
-                        wc[d] = q;
-                        sh[d] = (u[d] - q) / C.m[d];
-                        if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
-                    }
-                    if (!ok) continue;
-                    const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
-                    const int cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
-                    const int cs = cell_start[cell], ce = cell_start[cell + 1];
-                    const int spack = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
-                    for (int p0 = cs; p0 < ce; p0 += 64) {
-                        const int p = p0 + lane;
-                        bool in = false;
-                        int a = 0;
-                        if (p < ce) {
-                            a = cell_atoms[p];
-                            if (!(a < i || (a == i && zero))) {
-                                double vn[3];
-                                in = nbr_image(C, ni.u, an[a].u, sh, zero, vn);
-                            }
-                        }
-                        const unsigned long long b = __ballot(in);
-                        if (MODE == 1 && in) {
-                            const int idx = n + __popcll(b & below);
-                            if (idx < CHX_MAX_NB) keys[wv][idx] = ((long long)a << 24) | spack;
-                        }
-                        n += __popcll(b);
+        ci[0] = lc % C.m[0];
+        ci[1] = (lc / C.m[0]) % C.m[1];
+        ci[2] = lc / (C.m[0] * C.m[1]);
+        const int w0 = 2 * C.reach[0] + 1, w1 = 2 * C.reach[1] + 1;
+        nc = w0 * w1 * (2 * C.reach[2] + 1);
+        if (nc <= CHX_NBR_CELLS) {
+            int carry = 0;
+            for (int c0 = 0; c0 < nc; c0 += 64) {
+                const int c = c0 + lane;
+                int cnt = 0, cs = 0, spk = 0;
+                if (c < nc) {
+                    int cell, sh[3];
+                    if (searched_cell(c % w0 - C.reach[0], (c / w0) % w1 - C.reach[1], c / (w0 * w1) - C.reach[2],
+                                      cell, sh)) {
+                        cs = cell_start[cell];
+                        cnt = cell_start[cell + 1] - cs;
+                        spk = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
                     }
                 }
+                int v = cnt;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int t = __shfl_up(v, d, 64);
+                    if (lane >= d) v += t;
+                }
+                if (c < nc) {
+                    cpre[wv][c] = carry + v;
+                    cbeg[wv][c] = cs;
+                    cshf[wv][c] = spk;
+                }
+                carry += __shfl(v, 63, 64);
+            }
+        }
+    }
+    __syncthreads();
+    if (own) {
+        if (nc <= CHX_NBR_CELLS) {
+            // the candidates of all searched cells as one index range over the lanes
+            const int T = cpre[wv][nc - 1];
+            for (int t0 = 0; t0 < T; t0 += 64) {
+                const int t = t0 + lane;
+                int a = 0, spk = 0, sh[3] = {0, 0, 0};
+                if (t < T) {
+                    int lo = 0, hi = nc - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (cpre[wv][mid] > t) hi = mid; else lo = mid + 1;
+                    }
+                    const int before = lo ? cpre[wv][lo - 1] : 0;
+                    a = cell_atoms[cbeg[wv][lo] + (t - before)];
+                    spk = cshf[wv][lo];
+                    sh[0] = (spk & 255) - 128;
+                    sh[1] = ((spk >> 8) & 255) - 128;
+                    sh[2] = ((spk >> 16) & 255) - 128;
+                }
+                take(t < T, a, sh, spk);
+            }
+        } else {
+            for (int dz = -C.reach[2]; dz <= C.reach[2]; dz++)
+                for (int dy = -C.reach[1]; dy <= C.reach[1]; dy++)
+                    for (int dx = -C.reach[0]; dx <= C.reach[0]; dx++) {
+                        int cell, sh[3];
+                        if (!searched_cell(dx, dy, dz, cell, sh)) continue;
+                        const int cs = cell_start[cell], ce = cell_start[cell + 1];
+                        const int spack = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
+                        for (int p0 = cs; p0 < ce; p0 += 64) {
+                            const int p = p0 + lane;
+                            take(p < ce, p < ce ? cell_atoms[p] : 0, sh, spack);
+                        }
+                    }
+        }
         if (MODE == 0 && lane == 0) {
             nbr_cnt[i] = n;
             atomicMax(&status[CHX_ST_MAXNB], n);
@@ -202,9 +275,7 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
     }
     if (MODE == 0) return;
     __syncthreads();
-    if (i >= natoms || n > CHX_MAX_NB) return;
-    const chx_atom_n ni = an[i];
-    const chx_cfg &C = cfgs[as[i].cfg];
+    if (!own || n > CHX_MAX_NB) return;
     const long off = nbr_off[i];
     for (int j = lane; j < n; j += 64) {
         const long long k = keys[wv][j];
@@ -278,43 +349,37 @@ __global__ void k_group_bases(const int *__restrict__ off, long N, int G, long c
 // {j, k}, parent(j), parent(k) <= parent(l), all six distances < rc4.
 // Candidates are the images within rc3 (S3) of the owner; quadruplets use
 // adjacency bit masks over the images within rc4 (S4).
+//
+// The LDS working set is sized for C3 / C4 candidates.  k_enum uses the small
+// size (occupancy); an owner with more than CHX_ENUM_SMALL images is queued
+// in MODE 0 and handled, in both modes, by k_enum_big with the full size.
 // ---------------------------------------------------------------------------
-#define EW4 (CHX_MAX_N4 / 32)
-
+template <int C3, int C4>
 struct EnumShared {
-    double vn3[CHX_MAX_N3][3];
-    int idx3[CHX_MAX_N3];
-    int par3[CHX_MAX_N3];
-    int typ3[CHX_MAX_N3];
-    int gst3[CHX_MAX_N3];
-    int idx4[CHX_MAX_N4];      // S4 member -> S3 index
-    int gst4[CHX_MAX_N4];
-    unsigned A4[CHX_MAX_N4][EW4];
+    static constexpr int EW4 = C4 / 32;
+    double vn3[C3][3];
+    int idx3[C3];
+    int par3[C3];
+    int typ3[C3];
+    int idx4[C4];      // S4 member -> S3 index
+    int gst4[C4];
+    unsigned A4[C4][EW4];
     int cnt[CHX_MAX_CLUSTER_TYPES + 1];
     int n3, n4;
 };
 
-template <int MODE>
-__global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restrict__ cfgs,
-                                             const chx_atom_s *__restrict__ as, const chx_atom_n *__restrict__ an,
-                                             const int *__restrict__ nbr_off, const chx_entry_s *__restrict__ es,
-                                             const chx_entry_n *__restrict__ en,
-                                             const chx_enum_model *__restrict__ em, int *cnt3, int *cnt4,
-                                             const int *__restrict__ off3, const int *__restrict__ off4,
-                                             const int *__restrict__ gb3, const int *__restrict__ gb4,
-                                             int2 *list3, int2 *list4, int *status, int ngl_max)
+template <int MODE, int C3, int C4>
+__device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumShared<C3, C4> &sh)
 {
-    __shared__ EnumShared sh;
-    if (block_stop(status, CHX_STOP)) return;
-    const int i = blockIdx.x;
+    constexpr int EW4 = EnumShared<C3, C4>::EW4;
     const int lane = threadIdx.x;
-    const int off = nbr_off[i];
-    const int nb = nbr_off[i + 1] - off;
-    const chx_atom_s ai = as[i];
-    const chx_atom_n ni = an[i];
-    const chx_cfg &C = cfgs[ai.cfg];
-    const chx_enum_model &M = em[C.model];
-    const long N = natoms;
+    const int off = x.nbr_off[i];
+    const int nb = x.nbr_off[i + 1] - off;
+    const chx_atom_s ai = x.as[i];
+    const chx_atom_n ni = x.an[i];
+    const chx_cfg &C = x.cfgs[ai.cfg];
+    const chx_enum_model &M = x.em[C.model];
+    const long N = x.natoms;
     const double rc3 = C.rc3, rc4 = C.rc4;
     const bool any3 = M.has3 && rc3 > 0.0, any4 = M.has4 && rc4 > 0.0;
     const double rcS = rc3 > rc4 ? rc3 : rc4;  // many-body candidate radius
@@ -327,7 +392,7 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
         double d = 1e300;
         chx_entry_n e;
         if (j < nb) {
-            e = en[off + j];
+            e = x.en[off + j];
             d = frac_dist(C.hn, ni.u, e.v);
         }
         const bool in3 = j < nb && d < rcS, in4 = j < nb && d < rc4;
@@ -336,15 +401,15 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
         const int n3b = sh.n3, n4b = sh.n4;
         if (in3) {
             const int a = n3b + __popcll(b3 & below);
-            if (a < CHX_MAX_N3) {
+            if (a < C3) {
                 sh.vn3[a][0] = e.v[0]; sh.vn3[a][1] = e.v[1]; sh.vn3[a][2] = e.v[2];
                 sh.idx3[a] = j;
-                const chx_entry_s f = es[off + j];
+                const chx_entry_s f = x.es[off + j];
                 sh.par3[a] = f.parent;
                 sh.typ3[a] = f.type;
                 if (in4) {
                     const int u = n4b + __popcll(b4 & below);
-                    if (u < CHX_MAX_N4) sh.idx4[u] = a;
+                    if (u < C4) sh.idx4[u] = a;
                 }
             }
         }
@@ -353,14 +418,9 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
         __syncthreads();
     }
     const int n3 = sh.n3, n4 = sh.n4;
-    if (n3 > CHX_MAX_N3 || n4 > CHX_MAX_N4) {
-        if (lane == 0) atomicOr(&status[0], 2);
+    if (n3 > C3 || n4 > C4) {
+        if (lane == 0) atomicOr(&x.status[CHX_ST_FLAGS], CHX_BAD_N34);
         return;
-    }
-    for (int a = lane; a < n3; a += 64) {
-        int g = a;
-        while (g > 0 && sh.par3[g - 1] == sh.par3[a]) g--;
-        sh.gst3[a] = g;
     }
     for (int u = lane; u < n4; u += 64) {
         int g = u;
@@ -385,90 +445,117 @@ __global__ __launch_bounds__(64) void k_enum(int natoms, const chx_cfg *__restri
         }
     }
     __syncthreads();
-    const int W4 = (n4 + 31) >> 5;
     const int nt = M.nt;
     const int ti = ai.type;
-    for (int body = 3; body <= 4; body++) {
-        if ((body == 3 && !any3) || (body == 4 && !any4)) continue;
-        const int ngl = body == 3 ? M.ntrip + 1 : M.nquad + 1;
-        const int gbase = body == 3 ? M.g3 : M.g4;
-        const int *cmap = body == 3 ? M.tripmap : M.quadmap;
-        int *cntv = body == 3 ? cnt3 : cnt4;
-        const int *offv = body == 3 ? off3 : off4;
-        int2 *list = body == 3 ? list3 : list4;
-        const int *gbv = body == 3 ? gb3 : gb4;
-        const int nrow = body == 3 ? n3 : n4;
-        for (int round = 0; round * 64 < nrow; round++) {
-            const int r = round * 64 + lane;
-            {
-                if (r >= nrow) continue;
-                if (body == 3) {
-                    const int a = r;
-                    const int j = sh.idx3[a];
-                    const int tj = sh.typ3[a];
-                    for (int b = sh.gst3[a]; b < n3; b++) {
-                        if (b == a) continue;
-                        if (!(frac_dist(C.hn, sh.vn3[a], sh.vn3[b]) < rc3)) continue;
-                        const int k = sh.idx3[b];
-                        int g = cmap[(ti * nt + tj) * nt + sh.typ3[b]];
-                        if (g < 0) g = ngl - 1;
-                        const int slot = atomicAdd(&sh.cnt[g], 1);
-                        if (MODE == 1) {
-                            const long gg = gbase + g;
-                            const long pos = gbv[gg] + (offv[gg * N + i] - offv[gg * N]) + slot;
-                            list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10));
-                        }
-                    }
-                } else {
-                    const int u = r;
-                    const int au = sh.idx4[u];
-                    const int j = sh.idx3[au];
-                    const int tj = sh.typ3[au];
-                    const int gu = sh.gst4[u];
-                    for (int w = 0; w < W4; w++) {
-                        unsigned mk = sh.A4[u][w];
-                        const int lo = w * 32;
-                        if (gu > lo) mk &= (gu - lo >= 32) ? 0u : (~0u << (gu - lo));
-                        if (u >= lo && u < lo + 32) mk &= ~(1u << (u - lo));
-                        while (mk) {
-                            const int v = lo + __ffs(mk) - 1;
-                            mk &= mk - 1;
-                            const int av = sh.idx4[v];
-                            const int k = sh.idx3[av];
-                            const int tk = sh.typ3[av];
-                            const int gv = sh.gst4[v];
-                            for (int x = 0; x < W4; x++) {
-                                unsigned ml = sh.A4[u][x] & sh.A4[v][x];
-                                const int lo2 = x * 32;
-                                if (gv > lo2) ml &= (gv - lo2 >= 32) ? 0u : (~0u << (gv - lo2));
-                                if (u >= lo2 && u < lo2 + 32) ml &= ~(1u << (u - lo2));
-                                if (v >= lo2 && v < lo2 + 32) ml &= ~(1u << (v - lo2));
-                                while (ml) {
-                                    const int wl = lo2 + __ffs(ml) - 1;
-                                    ml &= ml - 1;
-                                    const int aw = sh.idx4[wl];
-                                    const int l = sh.idx3[aw];
-                                    int g = cmap[((ti * nt + tj) * nt + tk) * nt + sh.typ3[aw]];
-                                    if (g < 0) g = ngl - 1;
-                                    const int slot = atomicAdd(&sh.cnt[g], 1);
-                                    if (MODE == 1) {
-                                        const long gg = gbase + g;
-                                        const long pos = gbv[gg] + (offv[gg * N + i] - offv[gg * N]) + slot;
-                                        list[pos] = make_int2(i, (j + 1) | ((k + 1) << 10) | ((l + 1) << 20));
-                                    }
-                                }
+    if (any3) {
+        const int ngl = M.ntrip + 1;
+        const long gbase = M.g3;
+        // every unordered pair u < v of S3 once, spread over the lanes: (j, k)
+        // = (u, v) always, and (v, u) too when both are images of one atom
+        // (parent(j) <= parent(k) in both orders); the distance is symmetric
+        const int npair = n3 * (n3 - 1) / 2;
+        for (int p = lane; p < npair; p += 64) {
+            int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
+            while (v * (v - 1) / 2 > p) v--;
+            while ((v + 1) * v / 2 <= p) v++;
+            const int u = p - v * (v - 1) / 2;
+            if (!(frac_dist(C.hn, sh.vn3[u], sh.vn3[v]) < rc3)) continue;
+            const int nrec = sh.par3[u] == sh.par3[v] ? 2 : 1;
+            for (int q = 0; q < nrec; q++) {
+                const int a = q ? v : u, b = q ? u : v;
+                int g = M.tripmap[(ti * nt + sh.typ3[a]) * nt + sh.typ3[b]];
+                if (g < 0) g = ngl - 1;
+                const int slot = atomicAdd(&sh.cnt[g], 1);
+                if (MODE == 1) {
+                    const long gg = gbase + g;
+                    const long pos = x.gb3[gg] + (x.off3[gg * N + i] - x.off3[gg * N]) + slot;
+                    x.list3[pos] = make_int2(i, (sh.idx3[a] + 1) | ((sh.idx3[b] + 1) << 10));
+                }
+            }
+        }
+        __syncthreads();
+        for (int g = lane; g < ngl; g += 64) {
+            if (MODE == 0) x.cnt3[(gbase + g) * N + i] = sh.cnt[g];
+            sh.cnt[g] = 0;
+        }
+        __syncthreads();
+    }
+    if (any4) {
+        const int ngl = M.nquad + 1;
+        const long gbase = M.g4;
+        const int W4 = (n4 + 31) >> 5;
+        for (int u = lane; u < n4; u += 64) {
+            const int au = sh.idx4[u];
+            const int j = sh.idx3[au];
+            const int tj = sh.typ3[au];
+            const int gu = sh.gst4[u];
+            for (int w = 0; w < W4; w++) {
+                unsigned mk = sh.A4[u][w];
+                const int lo = w * 32;
+                if (gu > lo) mk &= (gu - lo >= 32) ? 0u : (~0u << (gu - lo));
+                if (u >= lo && u < lo + 32) mk &= ~(1u << (u - lo));
+                while (mk) {
+                    const int v = lo + __ffs(mk) - 1;
+                    mk &= mk - 1;
+                    const int av = sh.idx4[v];
+                    const int k = sh.idx3[av];
+                    const int tk = sh.typ3[av];
+                    const int gv = sh.gst4[v];
+                    for (int y = 0; y < W4; y++) {
+                        unsigned ml = sh.A4[u][y] & sh.A4[v][y];
+                        const int lo2 = y * 32;
+                        if (gv > lo2) ml &= (gv - lo2 >= 32) ? 0u : (~0u << (gv - lo2));
+                        if (u >= lo2 && u < lo2 + 32) ml &= ~(1u << (u - lo2));
+                        if (v >= lo2 && v < lo2 + 32) ml &= ~(1u << (v - lo2));
+                        while (ml) {
+                            const int wl = lo2 + __ffs(ml) - 1;
+                            ml &= ml - 1;
+                            const int aw = sh.idx4[wl];
+                            const int l = sh.idx3[aw];
+                            int g = M.quadmap[((ti * nt + tj) * nt + tk) * nt + sh.typ3[aw]];
+                            if (g < 0) g = ngl - 1;
+                            const int slot = atomicAdd(&sh.cnt[g], 1);
+                            if (MODE == 1) {
+                                const long gg = gbase + g;
+                                const long pos = x.gb4[gg] + (x.off4[gg * N + i] - x.off4[gg * N]) + slot;
+                                x.list4[pos] = make_int2(i, (j + 1) | ((k + 1) << 10) | ((l + 1) << 20));
                             }
                         }
                     }
                 }
             }
-            __syncthreads();
         }
         __syncthreads();
         for (int g = lane; g < ngl; g += 64) {
-            if (MODE == 0) cntv[(long)(gbase + g) * N + i] = sh.cnt[g];
+            if (MODE == 0) x.cnt4[(gbase + g) * N + i] = sh.cnt[g];
             sh.cnt[g] = 0;
         }
+        __syncthreads();
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64) void k_enum(chx_enum_args x)
+{
+    __shared__ EnumShared<CHX_ENUM_SMALL, CHX_ENUM_SMALL> sh;
+    if (block_stop(x.status, CHX_STOP)) return;
+    const int i = blockIdx.x;
+    if (x.nbr_off[i + 1] - x.nbr_off[i] > CHX_ENUM_SMALL) {
+        if (MODE == 0 && threadIdx.x == 0) x.big[atomicAdd(&x.status[CHX_ST_NBIG], 1)] = i;
+        return;
+    }
+    enum_owner<MODE, CHX_ENUM_SMALL, CHX_ENUM_SMALL>(x, i, sh);
+}
+
+// Owners queued by k_enum<0>, any number of them, over a fixed grid.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_enum_big(chx_enum_args x)
+{
+    __shared__ EnumShared<CHX_MAX_N3, CHX_MAX_N4> sh;
+    if (block_stop(x.status, CHX_STOP)) return;
+    const int nbig = x.status[CHX_ST_NBIG];
+    for (int t = blockIdx.x; t < nbig; t += gridDim.x) {
+        enum_owner<MODE, CHX_MAX_N3, CHX_MAX_N4>(x, x.big[t], sh);
         __syncthreads();
     }
 }
@@ -477,12 +564,10 @@ template __global__ void k_nbr<0>(int, const chx_cfg *, const chx_atom_s *, cons
                                   const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, int *);
 template __global__ void k_nbr<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
                                   const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, int *);
-template __global__ void k_enum<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                   const chx_entry_s *, const chx_entry_n *, const chx_enum_model *, int *, int *,
-                                   const int *, const int *, const int *, const int *, int2 *, int2 *, int *, int);
-template __global__ void k_enum<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                   const chx_entry_s *, const chx_entry_n *, const chx_enum_model *, int *, int *,
-                                   const int *, const int *, const int *, const int *, int2 *, int2 *, int *, int);
+template __global__ void k_enum<0>(chx_enum_args);
+template __global__ void k_enum<1>(chx_enum_args);
+template __global__ void k_enum_big<0>(chx_enum_args);
+template __global__ void k_enum_big<1>(chx_enum_args);
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of int32 (out[n] = total), 2048 elements per block.

@@ -222,6 +222,10 @@ MODELS = {
     # C3: carbon at scale, orders 16/10/4, rc 4.5 / 4.0 / 3.5
     "carbon_16_10_4": Model("carbon_16_10_4", ["C"], (16, 10, 4), (4.5, 4.0, 3.5),
                             _rin_all(["C"], 1.0), sigma=0.05, seed=0xC0DE03),
+    # test case, not a workload: a 7.5 A 2-body cutoff at 3.5 g/cc gives each
+    # owner ~155 half-list images (crowded-owner enumeration path)
+    "carbon_wide": Model("carbon_wide", ["C"], (8, 5, 3), (7.5, 4.0, 3.5),
+                         _rin_all(["C"], 1.0), sigma=0.05, seed=0xC0DE05),
 }
 
 
@@ -248,6 +252,29 @@ def carbon_lattice(nside=4, a=2.1525, jitter=0.25, seed=0xC0DE00, min_dist=1.1):
         if nside > 8 or _min_image_min_dist(pos, L) >= min_dist:
             break
     return pos, ["C"] * len(pos), np.diag([L, L, L])
+
+
+def dense_carbon(aspect=(1.0, 1.0, 1.0), natoms=64, density=3.5, dmin=1.2, seed=0xC0DE05):
+    """natoms carbon atoms at seeded random sites (minimum image distance >=
+    dmin) in an orthorhombic box of the given density whose edge lengths are
+    in the ratio aspect.  Returns (pos, types, cell)."""
+    rng = np.random.default_rng(seed)
+    vol = natoms * MASS["C"] / density * AMU_PER_A3_TO_GCC
+    a = np.asarray(aspect, dtype=float)
+    Lv = a * (vol / a.prod()) ** (1.0 / 3.0)
+    pos = []
+    while len(pos) < natoms:
+        p = rng.uniform(0.0, 1.0, 3) * Lv
+        ok = True
+        for q in pos:
+            d = p - q
+            d -= Lv * np.round(d / Lv)
+            if np.sqrt((d * d).sum()) < dmin:
+                ok = False
+                break
+        if ok:
+            pos.append(p)
+    return np.array(pos), ["C"] * natoms, np.diag(Lv)
 
 
 def water_box(nmol=512, L=24.8, roh=0.97, angle=104.5, doo=2.5, dmin=1.0, seed=0xC0DE01):

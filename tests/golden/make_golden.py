@@ -9,7 +9,9 @@ Runs only where the reference checkout and oracle/_ref/libchimes_ref.so exist
                         lists, as computed by the reference itself
   synth_cases.json      the same for the seeded synthetic workloads (C0 with
                         full cluster lists, C1, and the first configurations
-                        of the C2 batch)
+                        of the C2 batch), and ("dense") for two 64-atom carbon
+                        boxes under a 7.5 A 2-body cutoff, a cube and a thin
+                        slab, whose owners have ~155 half-list images
   cluster_vectors.json  single-cluster compute_2B/3B/4B input/output vectors
 """
 import gzip
@@ -128,6 +130,18 @@ def main():
                 if "C1|0" in prev:
                     out["C1|0"] = prev["C1|0"]
         with open(os.path.join(HERE, "synth_cases.json"), "w") as f:
+            json.dump(out, f)
+    if "dense" in which:
+        tmp = os.path.join(ROOT, "_work", "golden_models")
+        os.makedirs(tmp, exist_ok=True)
+        path = os.path.join(HERE, "synth_cases.json")
+        out = json.load(open(path))
+        pf = W.MODELS["carbon_wide"].write(os.path.join(tmp, "carbon_wide.txt"))
+        for tag, aspect in (("cube", (1.0, 1.0, 1.0)), ("slab", (4.0, 0.5, 0.5))):
+            res = Reference(pf, 0).calculate(*W.dense_carbon(aspect), lists=True)
+            out["DENSE|" + tag] = record(res)
+            print("dense", tag, res.energy, len(res.pairs), len(res.trips), len(res.quads), flush=True)
+        with open(path, "w") as f:
             json.dump(out, f)
     if "vectors" in which:
         import ctypes as C

@@ -108,3 +108,19 @@ def test_c2_batch_subset(cc, model_dir, synth_cases):
         rec = synth_cases["C2|%d" % k]
         check_efs("C2[%d]" % k, E[k], F[k], S[k], rec)
         check_lists(m, rec, conf=k)
+
+
+def test_dense_owners(cc, model_dir, synth_cases):
+    """Owners with ~155 half-list images, more than the small enumeration
+    working set holds: a cube (every searched cell in one flattened range)
+    and a thin slab (more searched cells than that range takes), in one
+    batch."""
+    from chimes_calculator_amd import workloads as W
+    m = cc.GpuModel(os.path.join(model_dir, "carbon_wide.txt"))
+    tags = ("cube", "slab")
+    cfgs = [W.dense_carbon((1.0, 1.0, 1.0)), W.dense_carbon((4.0, 0.5, 0.5))]
+    E, F, S = cc.eval_batch(m, cfgs)
+    for k, tag in enumerate(tags):
+        rec = synth_cases["DENSE|" + tag]
+        check_efs("dense " + tag, E[k], F[k], S[k], rec)
+        check_lists(m, rec, conf=k)
