@@ -24,8 +24,10 @@ struct chx_list_args {
 
 __global__ void k_fin_forces(int natoms_in, int ncfg, const int *cfg_in_off, const chx_cfg *cfgs,
                              const unsigned long long *Facc, double *Fout);
-__global__ void k_fin_cfg(const chx_cfg *cfgs, const chx_atom_s *as, const chx_model_dev *models,
-                          const unsigned long long *Oacc, double *Ecfg, double *Vcfg, int *status);
+__global__ void k_fin_part(int ncfg, const int *cfg_item_off, const chx_cfg *cfgs, const chx_atom_s *as,
+                           const chx_model_dev *models, const unsigned long long *Oacc, long long *part,
+                           int *status);
+__global__ void k_fin_cfg(int ncfg, const int *cfg_item_off, const long long *part, double *Ecfg, double *Vcfg);
 
 bool supported_D3(int D);
 bool supported_D4(int D);

@@ -6,6 +6,7 @@
 #define CHX_MAX_NB 512         // neighbour images per owner atom within the search cutoff
 #define CHX_MAX_N3 512         // images per owner within the 3-body cutoff
 #define CHX_MAX_N4 256         // images per owner within the 4-body cutoff
+#define CHX_FIN_CHUNK 2048     // owners per block of the energy/virial reduction
 #define CHX_NBR_CELLS 128      // searched cells per owner for the flattened candidate loop
 #define CHX_ENUM_SMALL 128     // owners with at most this many images use the small-LDS enumeration
 #define CHX_MAX_CLUSTER_TYPES 64

@@ -296,6 +296,8 @@ struct chx_engine {
     // batch state
     std::vector<chx_cfg> cfg_h;
     std::vector<int> cfg_atom_off_h, cfg_in_off_h;
+    std::vector<int> cfg_item_off_h;   // energy/virial reduction items (owner chunks) per configuration
+    int nitems = 0;
     int N = 0, Nin = 0, ncfg = 0, ncell = 0;
     int G3 = 0, G4 = 0;
     std::vector<int> g3b, g4b;
@@ -318,7 +320,7 @@ struct chx_engine {
     size_t o_stat = 0, o_cstat = 0, o_E = 0, o_V = 0, o_F = 0;  // offsets in the output blob
     // device pointers into the input blob
     chx_cfg *cfgs = nullptr;
-    int *cfg_atom_off = nullptr, *cfg_in_off = nullptr;
+    int *cfg_atom_off = nullptr, *cfg_in_off = nullptr, *cfg_item_off = nullptr;
     chx_enum_model *em = nullptr;
     chx_model_dev *mdev = nullptr;
     chx_group *grp3 = nullptr, *grp4 = nullptr;
@@ -340,6 +342,7 @@ struct chx_engine {
     DevBuf<int> gb3, ge3, gb4, ge4;
     DevBuf<int2> list3, list4;
     DevBuf<unsigned long long> Facc, Oacc;
+    DevBuf<long long> part;
     std::vector<DevBuf<int>> scan_tmp;
     // per-cluster API buffers
     DevBuf<double> cl_in, cl_out;
@@ -530,7 +533,8 @@ void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const ch
     e->cfg_h.assign(ncfg, chx_cfg{});
     e->cfg_atom_off_h.assign(ncfg + 1, 0);
     e->cfg_in_off_h.assign(ncfg + 1, 0);
-    long N = 0, Nin = 0, ncell = 0;
+    e->cfg_item_off_h.assign(ncfg + 1, 0);
+    long N = 0, Nin = 0, ncell = 0, items = 0;
     for (int c = 0; c < ncfg; c++) {
         chx_cfg &C = e->cfg_h[c];
         const int mi = b->model[c];
@@ -628,12 +632,16 @@ void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const ch
         C.model = mi;
         e->cfg_atom_off_h[c] = (int)N;
         e->cfg_in_off_h[c] = (int)Nin;
+        e->cfg_item_off_h[c] = (int)items;
+        items += std::max<long>(1, (C.natoms + CHX_FIN_CHUNK - 1) / CHX_FIN_CHUNK);
         N += C.natoms;
         Nin += nin;
         ncell += cells;
     }
     e->cfg_atom_off_h[ncfg] = (int)N;
     e->cfg_in_off_h[ncfg] = (int)Nin;
+    e->cfg_item_off_h[ncfg] = (int)items;
+    e->nitems = (int)items;
     if (N >= (1L << 28)) throw EngineError("batch too large");
     e->N = (int)N;
     e->Nin = (int)Nin;
@@ -766,11 +774,13 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->ge4.ensure(G4 + 1);
     e->Facc.ensure(6 * N);
     e->Oacc.ensure(14 * N);
+    e->part.ensure(14L * e->nitems);
     // input blob
     size_t off = 0;
     const size_t o_cfg = off; off = align256(off + ncfg * sizeof(chx_cfg));
     const size_t o_cao = off; off = align256(off + (ncfg + 1) * sizeof(int));
     const size_t o_cio = off; off = align256(off + (ncfg + 1) * sizeof(int));
+    const size_t o_cfi = off; off = align256(off + (ncfg + 1) * sizeof(int));
     const size_t o_em = off; off = align256(off + nmodels * sizeof(chx_enum_model));
     const size_t o_md = off; off = align256(off + nmodels * sizeof(chx_model_dev));
     const size_t o_g3 = off; off = align256(off + (G3 + 1) * sizeof(chx_group));
@@ -786,6 +796,7 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->cfgs = (chx_cfg *)(db + o_cfg);
     e->cfg_atom_off = (int *)(db + o_cao);
     e->cfg_in_off = (int *)(db + o_cio);
+    e->cfg_item_off = (int *)(db + o_cfi);
     e->em = (chx_enum_model *)(db + o_em);
     e->mdev = (chx_model_dev *)(db + o_md);
     e->grp3 = (chx_group *)(db + o_g3);
@@ -796,6 +807,7 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     memcpy(hb + o_cfg, e->cfg_h.data(), ncfg * sizeof(chx_cfg));
     memcpy(hb + o_cao, e->cfg_atom_off_h.data(), (ncfg + 1) * sizeof(int));
     memcpy(hb + o_cio, e->cfg_in_off_h.data(), (ncfg + 1) * sizeof(int));
+    memcpy(hb + o_cfi, e->cfg_item_off_h.data(), (ncfg + 1) * sizeof(int));
     memcpy(hb + o_em, e->emh.data(), nmodels * sizeof(chx_enum_model));
     memcpy(hb + o_md, e->mdh.data(), nmodels * sizeof(chx_model_dev));
     if (G3) memcpy(hb + o_g3, e->g3h.data(), G3 * sizeof(chx_group));
@@ -948,8 +960,10 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     e->record(4);
     hipLaunchKernelGGL(k_fin_forces, dim3(nblk(Nin, 256)), dim3(256), 0, st, (int)Nin, ncfg, e->cfg_in_off,
                        e->cfgs, e->Facc.p, Fout);
-    hipLaunchKernelGGL(k_fin_cfg, dim3(ncfg), dim3(256), 0, st, e->cfgs, e->as.p, e->mdev, e->Oacc.p, Ecfg, Vcfg,
-                       e->status);
+    hipLaunchKernelGGL(k_fin_part, dim3(e->nitems), dim3(256), 0, st, ncfg, e->cfg_item_off, e->cfgs, e->as.p,
+                       e->mdev, e->Oacc.p, e->part.p, e->status);
+    hipLaunchKernelGGL(k_fin_cfg, dim3(nblk(ncfg, 256)), dim3(256), 0, st, ncfg, e->cfg_item_off, e->part.p,
+                       Ecfg, Vcfg);
     HIPCHK(hipMemcpyAsync(e->out_h.p, e->out_d.p, e->out_bytes, hipMemcpyDeviceToHost, st));
     e->record(5);
 }
@@ -1070,7 +1084,7 @@ struct chx_plan {
     double *dev_out[3] = {nullptr, nullptr, nullptr};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    long sig[6] = {0, 0, 0, 0, 0, 0};  // N, Nin, ncell, ent_cap, l3_cap, l4_cap of the captured graph
+    long sig[7] = {0, 0, 0, 0, 0, 0, 0};  // N, Nin, ncell, ent_cap, l3_cap, l4_cap, nitems of the captured graph
     double tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
     std::string err;
     ~chx_plan()
@@ -1128,8 +1142,8 @@ void plan_capture(chx_plan *p, chx_out *o)
     e->capturing = false;
     HIPCHK(hipStreamEndCapture(e->st, &p->graph));
     HIPCHK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
-    const long sig[6] = {e->N, e->Nin, e->ncell, e->ent_cap, e->l3_cap, e->l4_cap};
-    for (int k = 0; k < 6; k++) p->sig[k] = sig[k];
+    const long sig[7] = {e->N, e->Nin, e->ncell, e->ent_cap, e->l3_cap, e->l4_cap, e->nitems};
+    for (int k = 0; k < 7; k++) p->sig[k] = sig[k];
 }
 
 // Times each 3-/4-body kernel variant on the lists of the last eager run and
@@ -1244,9 +1258,9 @@ int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double 
         plan_batch(p, pos, types, cells, b);
         prepare(e, p->models.data(), (int)p->models.size(), &b);
         layout(e, &b, !o->out_dev);
-        const long sig[6] = {e->N, e->Nin, e->ncell, e->ent_cap, e->l3_cap, e->l4_cap};
+        const long sig[7] = {e->N, e->Nin, e->ncell, e->ent_cap, e->l3_cap, e->l4_cap, e->nitems};
         bool same = p->exec != nullptr;
-        for (int k = 0; k < 6; k++) same = same && sig[k] == p->sig[k];
+        for (int k = 0; k < 7; k++) same = same && sig[k] == p->sig[k];
         if (same) {
             size_work(e);
             HIPCHK(hipGraphLaunch(p->exec, e->st));

@@ -983,18 +983,32 @@ __global__ void k_fin_forces(int natoms_in, int ncfg, const int *__restrict__ cf
     for (int k = 0; k < 3; k++) Fout[3L * a + k] = f[k].value();
 }
 
-__global__ __launch_bounds__(256) void k_fin_cfg(const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
-                                                  const chx_model_dev *__restrict__ models,
-                                                  const unsigned long long *__restrict__ Oacc, double *Ecfg,
-                                                  double *Vcfg, int *status)
+// Energy and virial, first pass: one block per chunk of CHX_FIN_CHUNK owners
+// of one configuration (item; cfg_item_off[c] is the first item of c), exact
+// partial sums normalised to part[item][7][2] = (hi, lo < 2^40).
+__global__ __launch_bounds__(256) void k_fin_part(int ncfg, const int *__restrict__ cfg_item_off,
+                                                   const chx_cfg *__restrict__ cfgs,
+                                                   const chx_atom_s *__restrict__ as,
+                                                   const chx_model_dev *__restrict__ models,
+                                                   const unsigned long long *__restrict__ Oacc, long long *part,
+                                                   int *status)
 {
     __shared__ long long sh_hi[7][256];
     __shared__ unsigned long long sh_lo[7][256];
-    const chx_cfg &C = cfgs[blockIdx.x];
+    const int item = blockIdx.x;
+    int lo = 0, hi = ncfg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cfg_item_off[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const chx_cfg &C = cfgs[lo];
     const chx_model_dev &M = models[C.model];
+    const long r0 = C.atom_off + (long)(item - cfg_item_off[lo]) * CHX_FIN_CHUNK;
+    const long r1 = r0 + CHX_FIN_CHUNK < (long)C.atom_off + C.natoms ? r0 + CHX_FIN_CHUNK
+                                                                      : (long)C.atom_off + C.natoms;
     fx_sum acc[7];
     bool bad = false;
-    for (int r = C.atom_off + threadIdx.x; r < C.atom_off + C.natoms; r += 256) {
+    for (long r = r0 + threadIdx.x; r < r1; r += 256) {
 #pragma unroll
         for (int k = 0; k < 7; k++) acc[k].add(Oacc + 2L * (7L * r + k));
         acc[0].add_value(M.eoff[as[r].type], bad);
@@ -1014,19 +1028,27 @@ __global__ __launch_bounds__(256) void k_fin_cfg(const chx_cfg *__restrict__ cfg
             }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        fx_sum t;
-        t.hi = sh_hi[0][0];
-        t.lo = sh_lo[0][0];
-        Ecfg[blockIdx.x] = t.value();
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-            t.hi = sh_hi[1 + k][0];
-            t.lo = sh_lo[1 + k][0];
-            Vcfg[6L * blockIdx.x + k] = t.value();
-        }
+    if (threadIdx.x < 7) {
+        const int k = threadIdx.x;
+        part[14L * item + 2 * k] = sh_hi[k][0] + (long long)(sh_lo[k][0] >> 40);
+        part[14L * item + 2 * k + 1] = (long long)(sh_lo[k][0] & 0xffffffffffull);
     }
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
+}
+
+// Second pass: one thread per configuration sums its items' partials.
+__global__ void k_fin_cfg(int ncfg, const int *__restrict__ cfg_item_off, const long long *__restrict__ part,
+                          double *Ecfg, double *Vcfg)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncfg) return;
+    fx_sum t[7];
+    for (int it = cfg_item_off[c]; it < cfg_item_off[c + 1]; it++)
+#pragma unroll
+        for (int k = 0; k < 7; k++) t[k].add((const unsigned long long *)part + 14L * it + 2 * k);
+    Ecfg[c] = t[0].value();
+#pragma unroll
+    for (int k = 0; k < 6; k++) Vcfg[6L * c + k] = t[1 + k].value();
 }
 
 // ---------------------------------------------------------------------------

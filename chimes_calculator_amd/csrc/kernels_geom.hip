@@ -269,7 +269,9 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
         }
         if (MODE == 0 && lane == 0) {
             nbr_cnt[i] = n;
-            atomicMax(&status[CHX_ST_MAXNB], n);
+            // the word only grows, so a count not above its current value
+            // needs no atomic (one address for all owners)
+            if (n > __atomic_load_n(&status[CHX_ST_MAXNB], __ATOMIC_RELAXED)) atomicMax(&status[CHX_ST_MAXNB], n);
             if (n > CHX_MAX_NB) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_NB);
         }
     }
@@ -365,6 +367,8 @@ struct EnumShared {
     int gst4[C4];
     unsigned A4[C4][EW4];
     int cnt[CHX_MAX_CLUSTER_TYPES + 1];
+    int wb[CHX_MAX_CLUSTER_TYPES + 1];   // MODE 1: list index of the owner's first record per group
+    int map[256];                         // type tuple -> group, for nt <= 4
     int n3, n4;
 };
 
@@ -447,9 +451,23 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     __syncthreads();
     const int nt = M.nt;
     const int ti = ai.type;
+    // group of a type tuple from LDS when the map is small, and the owner's
+    // write position per group
+    const bool lmap = nt <= 4;
+    auto stage = [&](const int *map, int nmap, const int *gb, const int *offv, long gbase, int ngl) {
+        if (lmap)
+            for (int t = lane; t < nmap; t += 64) sh.map[t] = map[t];
+        if (MODE == 1)
+            for (int g = lane; g < ngl; g += 64) {
+                const long gg = gbase + g;
+                sh.wb[g] = gb[gg] + (offv[gg * N + i] - offv[gg * N]);
+            }
+        __syncthreads();
+    };
     if (any3) {
         const int ngl = M.ntrip + 1;
         const long gbase = M.g3;
+        stage(M.tripmap, nt * nt * nt, x.gb3, x.off3, gbase, ngl);
         // every unordered pair u < v of S3 once, spread over the lanes: (j, k)
         // = (u, v) always, and (v, u) too when both are images of one atom
         // (parent(j) <= parent(k) in both orders); the distance is symmetric
@@ -463,14 +481,11 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
             const int nrec = sh.par3[u] == sh.par3[v] ? 2 : 1;
             for (int q = 0; q < nrec; q++) {
                 const int a = q ? v : u, b = q ? u : v;
-                int g = M.tripmap[(ti * nt + sh.typ3[a]) * nt + sh.typ3[b]];
+                const int t = (ti * nt + sh.typ3[a]) * nt + sh.typ3[b];
+                int g = lmap ? sh.map[t] : M.tripmap[t];
                 if (g < 0) g = ngl - 1;
                 const int slot = atomicAdd(&sh.cnt[g], 1);
-                if (MODE == 1) {
-                    const long gg = gbase + g;
-                    const long pos = x.gb3[gg] + (x.off3[gg * N + i] - x.off3[gg * N]) + slot;
-                    x.list3[pos] = make_int2(i, (sh.idx3[a] + 1) | ((sh.idx3[b] + 1) << 10));
-                }
+                if (MODE == 1) x.list3[sh.wb[g] + slot] = make_int2(i, (sh.idx3[a] + 1) | ((sh.idx3[b] + 1) << 10));
             }
         }
         __syncthreads();
@@ -484,6 +499,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         const int ngl = M.nquad + 1;
         const long gbase = M.g4;
         const int W4 = (n4 + 31) >> 5;
+        stage(M.quadmap, nt * nt * nt * nt, x.gb4, x.off4, gbase, ngl);
         for (int u = lane; u < n4; u += 64) {
             const int au = sh.idx4[u];
             const int j = sh.idx3[au];
@@ -512,14 +528,12 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
                             ml &= ml - 1;
                             const int aw = sh.idx4[wl];
                             const int l = sh.idx3[aw];
-                            int g = M.quadmap[((ti * nt + tj) * nt + tk) * nt + sh.typ3[aw]];
+                            const int t = ((ti * nt + tj) * nt + tk) * nt + sh.typ3[aw];
+                            int g = lmap ? sh.map[t] : M.quadmap[t];
                             if (g < 0) g = ngl - 1;
                             const int slot = atomicAdd(&sh.cnt[g], 1);
-                            if (MODE == 1) {
-                                const long gg = gbase + g;
-                                const long pos = x.gb4[gg] + (x.off4[gg * N + i] - x.off4[gg * N]) + slot;
-                                x.list4[pos] = make_int2(i, (j + 1) | ((k + 1) << 10) | ((l + 1) << 20));
-                            }
+                            if (MODE == 1)
+                                x.list4[sh.wb[g] + slot] = make_int2(i, (j + 1) | ((k + 1) << 10) | ((l + 1) << 20));
                         }
                     }
                 }
