@@ -338,7 +338,7 @@ struct chx_engine {
     DevBuf<int> nbr_cnt, nbr_off, e_owner, big;
     DevBuf<chx_entry_s> es;
     DevBuf<chx_entry_n> en;
-    DevBuf<int> cnt3, off3, cnt4, off4;
+    DevBuf<int> cnt34, off34;   // per (group, owner) counts and their scan: 3-body groups, then 4-body
     DevBuf<int> gb3, ge3, gb4, ge4;
     DevBuf<int2> list3, list4;
     DevBuf<unsigned long long> Facc, Oacc;
@@ -764,10 +764,8 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->nbr_cnt.ensure(N + 1);
     e->big.ensure(N + 1);
     e->nbr_off.ensure(N + 1);
-    e->cnt3.ensure((long)G3 * N + 1);
-    e->off3.ensure((long)G3 * N + 2);
-    e->cnt4.ensure((long)G4 * N + 1);
-    e->off4.ensure((long)G4 * N + 2);
+    e->cnt34.ensure((long)(G3 + G4) * N + 1);
+    e->off34.ensure((long)(G3 + G4) * N + 2);
     e->gb3.ensure(G3 + 1);
     e->ge3.ensure(G3 + 1);
     e->gb4.ensure(G4 + 1);
@@ -851,8 +849,7 @@ void size_work(chx_engine *e)
     cl[n++] = chx_clear_range{e->cfg_stat, 4L * e->ncfg};
     cl[n++] = chx_clear_range{e->cell_cnt.p, (long)e->ncell + 1};
     cl[n++] = chx_clear_range{e->cell_cur.p, (long)e->ncell + 1};
-    cl[n++] = chx_clear_range{e->cnt3.p, (long)e->G3 * N + 1};
-    cl[n++] = chx_clear_range{e->cnt4.p, (long)e->G4 * N + 1};
+    cl[n++] = chx_clear_range{e->cnt34.p, (long)(e->G3 + e->G4) * N + 1};
     cl[n++] = chx_clear_range{(int *)e->Facc.p, 12 * N};
     cl[n++] = chx_clear_range{(int *)e->Oacc.p, 28 * N};
     e->nclear = n;
@@ -895,10 +892,10 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         ea.es = e->es.p;
         ea.en = e->en.p;
         ea.em = e->em;
-        ea.cnt3 = e->cnt3.p;
-        ea.cnt4 = e->cnt4.p;
-        ea.off3 = e->off3.p;
-        ea.off4 = e->off4.p;
+        ea.cnt3 = e->cnt34.p;
+        ea.cnt4 = e->cnt34.p + (long)G3 * N;
+        ea.off3 = e->off34.p;
+        ea.off4 = e->off34.p + (long)G3 * N;
         ea.gb3 = e->gb3.p;
         ea.gb4 = e->gb4.p;
         ea.list3 = e->list3.p;
@@ -909,16 +906,11 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         const dim3 gbig(1280);
         hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, ea);
         hipLaunchKernelGGL((k_enum_big<0>), gbig, dim3(64), 0, st, ea);
-        if (G3) {
-            e->scan(e->cnt3.p, e->off3.p, (long)G3 * N);
-            hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off3.p, N, G3, e->l3_cap, e->gb3.p,
-                               e->ge3.p, e->status, (int)CHX_ST_N3, (int)CHX_RETRY_L3);
-        }
-        if (G4) {
-            e->scan(e->cnt4.p, e->off4.p, (long)G4 * N);
-            hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(1), 0, st, e->off4.p, N, G4, e->l4_cap, e->gb4.p,
-                               e->ge4.p, e->status, (int)CHX_ST_N4, (int)CHX_RETRY_L4);
-        }
+        // one scan over both bodies' counts: segments and positions use
+        // differences of the scan only
+        e->scan(e->cnt34.p, e->off34.p, (long)(G3 + G4) * N);
+        hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(64), 0, st, e->off34.p, N, G3, G4, e->l3_cap, e->l4_cap,
+                           e->gb3.p, e->ge3.p, e->gb4.p, e->ge4.p, e->status);
         hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st, ea);
         hipLaunchKernelGGL((k_enum_big<1>), gbig, dim3(64), 0, st, ea);
     }
@@ -1343,7 +1335,7 @@ int chx_export_lists(chx_engine *e, int cfg, long *n2, int **pairs, long *n3, in
             if (has && G > 0) {
                 const int gb = body == 3 ? e->g3b[C.model] : e->g4b[C.model];
                 const int ng = (body == 3 ? M->d.ntrip : M->d.nquad) + 1;
-                const int *offv = body == 3 ? e->off3.p : e->off4.p;
+                const int *offv = body == 3 ? e->off34.p : e->off34.p + (long)e->G3 * N;
                 const int *gbase = body == 3 ? e->gb3.p : e->gb4.p;
                 const int2 *lst = body == 3 ? e->list3.p : e->list4.p;
                 for (int g = gb; g < gb + ng; g++) {

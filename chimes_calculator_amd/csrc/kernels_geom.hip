@@ -319,25 +319,33 @@ __global__ void k_check_entries(const int *__restrict__ nbr_off, int natoms, lon
     if (tot > cap) atomicOr(&status[CHX_ST_FLAGS], CHX_RETRY_ENT);
 }
 
-// Aligned segment of each (model, cluster type) group in a 3-/4-body list:
-// base[g] = sum over h < g of the group sizes rounded up to CHX_LIST_ALIGN,
-// end[g] = base[g] + size of g.  off is the exclusive scan of the per-(group,
-// owner) counts laid out [G][N] (off[G*N] = total).
-__global__ void k_group_bases(const int *__restrict__ off, long N, int G, long cap, int *base, int *end,
-                              int *status, int word, int retry_bit)
+// Aligned segment of each (model, cluster type) group in the 3- and 4-body
+// lists (thread 0: triplets, thread 1: quadruplets): base[g] = sum over h < g
+// of the group sizes rounded up to CHX_LIST_ALIGN, end[g] = base[g] + size of
+// g.  off is the exclusive scan of the per-(group, owner) counts laid out
+// [G3 + G4][N]; a group's size is a difference of two scan values.
+__global__ void k_group_bases(const int *__restrict__ off, long N, int G3, int G4, long cap3, long cap4,
+                              int *base3, int *end3, int *base4, int *end4, int *status)
 {
-    if (threadIdx.x || blockIdx.x) return;
-    status[word + (CHX_ST_C3 - CHX_ST_N3)] = off[(long)G * N];
+    if (blockIdx.x || threadIdx.x > 1) return;
+    const bool four = threadIdx.x == 1;
+    const int G = four ? G4 : G3;
+    if (G == 0) return;
+    const int *o = four ? off + (long)G3 * N : off;
+    int *base = four ? base4 : base3, *end = four ? end4 : end3;
+    const long cap = four ? cap4 : cap3;
+    const int word = four ? CHX_ST_N4 : CHX_ST_N3;
+    status[four ? CHX_ST_C4 : CHX_ST_C3] = o[(long)G * N] - o[0];
     long b = 0;
     for (int g = 0; g < G; g++) {
-        const long sz = (long)off[(long)(g + 1) * N] - off[(long)g * N];
+        const long sz = (long)o[(long)(g + 1) * N] - o[(long)g * N];
         base[g] = (int)b;
         end[g] = (int)(b + sz);
         b += (sz + CHX_LIST_ALIGN - 1) / CHX_LIST_ALIGN * CHX_LIST_ALIGN;
     }
     base[G] = (int)b;
     status[word] = (int)b;
-    if (b > cap || b >= (1L << 31) - CHX_LIST_ALIGN) atomicOr(&status[CHX_ST_FLAGS], retry_bit);
+    if (b > cap || b >= (1L << 31) - CHX_LIST_ALIGN) atomicOr(&status[CHX_ST_FLAGS], four ? CHX_RETRY_L4 : CHX_RETRY_L3);
 }
 
 // ---------------------------------------------------------------------------
@@ -500,41 +508,40 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         const long gbase = M.g4;
         const int W4 = (n4 + 31) >> 5;
         stage(M.quadmap, nt * nt * nt * nt, x.gb4, x.off4, gbase, ngl);
-        for (int u = lane; u < n4; u += 64) {
-            const int au = sh.idx4[u];
-            const int j = sh.idx3[au];
-            const int tj = sh.typ3[au];
-            const int gu = sh.gst4[u];
-            for (int w = 0; w < W4; w++) {
-                unsigned mk = sh.A4[u][w];
-                const int lo = w * 32;
-                if (gu > lo) mk &= (gu - lo >= 32) ? 0u : (~0u << (gu - lo));
-                if (u >= lo && u < lo + 32) mk &= ~(1u << (u - lo));
-                while (mk) {
-                    const int v = lo + __ffs(mk) - 1;
-                    mk &= mk - 1;
-                    const int av = sh.idx4[v];
-                    const int k = sh.idx3[av];
-                    const int tk = sh.typ3[av];
-                    const int gv = sh.gst4[v];
-                    for (int y = 0; y < W4; y++) {
-                        unsigned ml = sh.A4[u][y] & sh.A4[v][y];
-                        const int lo2 = y * 32;
-                        if (gv > lo2) ml &= (gv - lo2 >= 32) ? 0u : (~0u << (gv - lo2));
-                        if (u >= lo2 && u < lo2 + 32) ml &= ~(1u << (u - lo2));
-                        if (v >= lo2 && v < lo2 + 32) ml &= ~(1u << (v - lo2));
-                        while (ml) {
-                            const int wl = lo2 + __ffs(ml) - 1;
-                            ml &= ml - 1;
-                            const int aw = sh.idx4[wl];
-                            const int l = sh.idx3[aw];
-                            const int t = ((ti * nt + tj) * nt + tk) * nt + sh.typ3[aw];
-                            int g = lmap ? sh.map[t] : M.quadmap[t];
-                            if (g < 0) g = ngl - 1;
-                            const int slot = atomicAdd(&sh.cnt[g], 1);
-                            if (MODE == 1)
-                                x.list4[sh.wb[g] + slot] = make_int2(i, (j + 1) | ((k + 1) << 10) | ((l + 1) << 20));
-                        }
+        // every adjacent unordered pair u < v of S4 once, spread over the
+        // lanes, as (j, k) = (u, v) and, for two images of one atom, also
+        // (v, u); then l over the common neighbours with parent(k) <= parent(l)
+        const int npair = n4 * (n4 - 1) / 2;
+        for (int p = lane; p < npair; p += 64) {
+            int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
+            while (v * (v - 1) / 2 > p) v--;
+            while ((v + 1) * v / 2 <= p) v++;
+            const int u = p - v * (v - 1) / 2;
+            if (!((sh.A4[u][v >> 5] >> (v & 31)) & 1u)) continue;
+            const int nrec = sh.par3[sh.idx4[u]] == sh.par3[sh.idx4[v]] ? 2 : 1;
+            for (int q = 0; q < nrec; q++) {
+                const int ua = q ? v : u, vb = q ? u : v;
+                const int au = sh.idx4[ua], av = sh.idx4[vb];
+                const int j = sh.idx3[au], k = sh.idx3[av];
+                const int tjk = (ti * nt + sh.typ3[au]) * nt + sh.typ3[av];
+                const int gv = sh.gst4[vb];
+                for (int y = 0; y < W4; y++) {
+                    unsigned ml = sh.A4[ua][y] & sh.A4[vb][y];
+                    const int lo2 = y * 32;
+                    if (gv > lo2) ml &= (gv - lo2 >= 32) ? 0u : (~0u << (gv - lo2));
+                    if (ua >= lo2 && ua < lo2 + 32) ml &= ~(1u << (ua - lo2));
+                    if (vb >= lo2 && vb < lo2 + 32) ml &= ~(1u << (vb - lo2));
+                    while (ml) {
+                        const int wl = lo2 + __ffs(ml) - 1;
+                        ml &= ml - 1;
+                        const int aw = sh.idx4[wl];
+                        const int t = tjk * nt + sh.typ3[aw];
+                        int g = lmap ? sh.map[t] : M.quadmap[t];
+                        if (g < 0) g = ngl - 1;
+                        const int slot = atomicAdd(&sh.cnt[g], 1);
+                        if (MODE == 1)
+                            x.list4[sh.wb[g] + slot] =
+                                make_int2(i, (j + 1) | ((k + 1) << 10) | ((sh.idx3[aw] + 1) << 20));
                     }
                 }
             }
@@ -617,27 +624,40 @@ __global__ __launch_bounds__(256) void k_scan_block(const int *in, int *out, lon
 // 1024 threads, each summing a contiguous chunk; out[n] = total.
 __global__ __launch_bounds__(1024) void k_scan_one(const int *in, int *out, long n)
 {
-    __shared__ int s[1024];
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const long per = (n + 1023) / 1024;
-    const long b = threadIdx.x * per;
+    const long b = tid * per;
     const long e = b + per < n ? b + per : n;
     int t = 0;
     for (long i = b; i < e; i++) t += in[i];
-    s[threadIdx.x] = t;
+    // inclusive scan of the thread totals: within each wavefront, then over
+    // the 16 wavefront totals
+    int v = t;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int x = (int)threadIdx.x >= d ? s[threadIdx.x - d] : 0;
-        __syncthreads();
-        s[threadIdx.x] += x;
-        __syncthreads();
+    if (w == 0) {
+        int q = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int u = __shfl_up(q, d, 64);
+            if (lane >= d) q += u;
+        }
+        if (lane < 16) wsum[lane] = q;
     }
-    int run = s[threadIdx.x] - t;
+    __syncthreads();
+    int run = v - t + (w ? wsum[w - 1] : 0);
     for (long i = b; i < e; i++) {
-        const int v = in[i];
+        const int x = in[i];
         out[i] = run;
-        run += v;
+        run += x;
     }
-    if (threadIdx.x == 1023) out[n] = s[1023];
+    if (tid == 1023) out[n] = wsum[15];
 }
 
 // Zeroes every listed range (one launch for all counters and accumulators).
