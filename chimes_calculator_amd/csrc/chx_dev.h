@@ -119,6 +119,45 @@ __device__ __forceinline__ void fx_owner_runs(unsigned long long *Facc, unsigned
     }
 }
 
+// Owner terms of tiles in which the four lanes l, l + 16, l + 32, l + 48
+// (sub = lane / 16) hold one cluster and the same key: sub 0 adds the owner's
+// force f into Facc[key], subs 1, 2, 3 add the energy and virial terms q
+// (components 0-2, 3-4 and 5-6) into Oacc[key]; each sub sums over runs of
+// equal keys in its 16 lanes, so a lane scans three values, not ten.
+__device__ __forceinline__ void fx_owner_runs4(unsigned long long *Facc, unsigned long long *Oacc, int key,
+                                               const double (&f)[3], const double (&q)[7], bool &bad)
+{
+    if (__ballot(key >= 0) == 0) return;
+    const int sub = (threadIdx.x >> 4) & 3;
+    double v[3];
+    v[0] = sub == 0 ? f[0] : (sub == 1 ? q[0] : (sub == 2 ? q[3] : q[5]));
+    v[1] = sub == 0 ? f[1] : (sub == 1 ? q[1] : (sub == 2 ? q[4] : q[6]));
+    v[2] = sub == 0 ? f[2] : (sub == 1 ? q[2] : 0.0);
+    long long hi[3], lo[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        hi[k] = 0;
+        lo[k] = 0;
+        if (key >= 0) fx_split(v[k], hi[k], lo[k], bad);
+    }
+    const int seg = fx_run_first<16>(key);
+    fx_run_scan<16, 3>(hi, seg);
+    fx_run_scan<16, 3>(lo, seg);
+    const int l = threadIdx.x & 15;
+    const int knext = __shfl_down(key, 1, 16);
+    if (key >= 0 && (l == 15 || knext != key)) {
+        unsigned long long *a = sub == 0 ? Facc + 6L * key
+                                         : Oacc + 14L * key + (sub == 1 ? 0 : (sub == 2 ? 6 : 10));
+        const int n = sub < 2 ? 3 : 2;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (k >= n) break;
+            if (hi[k]) atomicAdd(a + 2 * k, (unsigned long long)hi[k]);
+            if (lo[k]) atomicAdd(a + 2 * k + 1, (unsigned long long)lo[k]);
+        }
+    }
+}
+
 // Per-workgroup LDS table that merges the force contributions of a tile by
 // atom before they reach global memory: open addressing on the atom index,
 // exact integer LDS atomics, then one global atomic pair per component and

@@ -426,6 +426,32 @@ struct mfma_shape {
     static constexpr int RB = MP / 16;                  // 16-row output blocks
 };
 
+// A block's share of a coefficient image on its way to LDS: every thread
+// issues all of its loads first and stores them later, so that the loads are
+// in flight together and overlap the cluster geometry.
+template <typename R, int N, int NT>
+struct mat_stage {
+    static constexpr int PER = (N + NT - 1) / NT;
+    R v[PER];
+    __device__ __forceinline__ void load(const R *__restrict__ src)
+    {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int x = (int)threadIdx.x + u * NT;
+            v[u] = x < N ? src[x] : (R)0;
+        }
+    }
+    template <class AT>
+    __device__ __forceinline__ void store(AT at)
+    {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int x = (int)threadIdx.x + u * NT;
+            if (x < N) at(x) = v[u];
+        }
+    }
+};
+
 template <int D, typename R, int WPB, int BF>
 __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__restrict__ gbase,
                                                     const int *__restrict__ gend,
@@ -469,14 +495,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
         const int g = find_group(gbase, G, t0);
         const chx_group &grp = groups[g];
         if (grp.type < 0 || grp.D != D) return;
-        if (BF) {
+        mat_stage<R, BF ? 1 : SH::KP * SH::MP, 64 * WPB> cm;
+        if constexpr (BF) {
             for (int x = threadIdx.x; x < (BF == 3 ? 2 : 1) * SH::MP * K32; x += 64 * WPB) {
                 const int img = x / (SH::MP * K32), y = x % (SH::MP * K32);
                 ClRaw[img * LB::size + (y / K32) * LB::LD + y % K32] = grp.Cmb[x];
             }
         } else {
-            const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
-            for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 64 * WPB) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
+            cm.load(sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf);
         }
         // ---- stage: geometry (all four lanes of a cluster), polynomials of pairs sub and sub + 4
         const long c = t0 + wv * 16 + j;
@@ -491,6 +517,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
                 rec = list[c];
                 act = cluster_geometry<4>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, perm);
             }
+            if constexpr (!BF) cm.store([&](int x) -> R & { return Cl[LM::at(x / SH::MP, x % SH::MP)]; });
             double pm[3] = {p[0][0], p[0][1], p[0][2]};
 #pragma unroll
             for (int m = 1; m < 4; m++)
@@ -706,7 +733,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
 #pragma unroll
             for (int k = 0; k < 7; k++) q7[k] += __shfl_xor(q7[k], w, 64);
         st.tab.add((act && sub != 0) ? key_s : -1, F, bad);
-        fx_owner_runs<16>(Facc, Oacc, sub == 0 ? owner : -1, F, q7, bad);
+        fx_owner_runs4(Facc, Oacc, owner, F, q7, bad);
         __syncthreads();
         st.tab.flush(Facc);
         if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -771,16 +798,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
     const int g = find_group(gbase, G, t0);
     const chx_group &grp = groups[g];
     if (grp.type < 0 || grp.D != D) return;
-    {
-        const R *src = sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf;
-        if (BF) {
-            for (int x = threadIdx.x; x < (BF == 3 ? 2 : 1) * SH::MP * K32; x += 64 * WPB) {
-                const int img = x / (SH::MP * K32), y = x % (SH::MP * K32);
-                ClRaw[img * LB::size + (y / K32) * LB::LD + y % K32] = grp.Cmb[x];
-            }
-        } else {
-            for (int x = threadIdx.x; x < SH::KP * SH::MP; x += 64 * WPB) Cl[LM::at(x / SH::MP, x % SH::MP)] = src[x];
+    mat_stage<R, BF ? 1 : SH::KP * SH::MP, 64 * WPB> cm;
+    if constexpr (BF) {
+        for (int x = threadIdx.x; x < (BF == 3 ? 2 : 1) * SH::MP * K32; x += 64 * WPB) {
+            const int img = x / (SH::MP * K32), y = x % (SH::MP * K32);
+            ClRaw[img * LB::size + (y / K32) * LB::LD + y % K32] = grp.Cmb[x];
         }
+    } else {
+        cm.load(sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf);
     }
     // ---- stage: geometry (all lanes of a cluster), lane sub < 3 the polynomials of pair sub
     const long c = t0 + wv * 16 + j;
@@ -795,6 +820,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
             rec = list[c];
             act = cluster_geometry<3>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, perm);
         }
+        if constexpr (!BF) cm.store([&](int x) -> R & { return Cl[LM::at(x / SH::MP, x % SH::MP)]; });
         if (act)
 #pragma unroll
             for (int a = 0; a < 3; a++)
@@ -900,7 +926,6 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
     double q7[7] = {0, 0, 0, 0, 0, 0, 0};
     double F[3] = {0.0, 0.0, 0.0};
     bool bad = false;
-    int owner = -1;
     if (act && sub < 3) {
         double r[3], f[3], fd[3], E, fs[3], Gd[3];
 #pragma unroll
@@ -939,7 +964,6 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
             q7[0] = E;
             note_flags(cfg_stat, cfg, flags);
         }
-        owner = rec.x;
     }
     // as in the 4-body kernel: per-cluster energy and virial over the lanes,
     // owner terms per run, other members through the LDS table
@@ -948,7 +972,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
 #pragma unroll
         for (int k = 0; k < 7; k++) q7[k] += __shfl_xor(q7[k], w, 64);
     st.tab.add((act && sub != 0 && sub < 3) ? key_s : -1, F, bad);
-    fx_owner_runs<16>(Facc, Oacc, sub == 0 ? owner : -1, F, q7, bad);
+    fx_owner_runs4(Facc, Oacc, act ? rec.x : -1, F, q7, bad);
     __syncthreads();
     st.tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
