@@ -374,6 +374,22 @@ struct lds_mat {
     __device__ __forceinline__ static int at(int k, int m) { return swz ? k * MP + (m ^ ((k & 3) << 4)) : k * LD + m; }
 };
 
+// Square P x P image (P a multiple of 16) read both by rows and by columns:
+// in a 16-lane group either the column (16 consecutive m, one k) or the row
+// (16 consecutive k, one m) varies, and XOR-ing m's low four bits with k's
+// keeps both patterns on distinct banks; for P a multiple of 64 the k & 3
+// part also moves the four rows of one MFMA k-group to different 16-column
+// blocks.
+template <int P>
+struct lds_sq {
+    static_assert(P % 16 == 0, "square image size must be a multiple of 16");
+    static constexpr int size = P * P;
+    __device__ __forceinline__ static int at(int k, int m)
+    {
+        return k * P + (m ^ ((P % 64 == 0 ? (k & 3) << 4 : 0) ^ (k & 15)));
+    }
+};
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // Round-to-nearest-even FP32 -> BF16 bits.
@@ -429,16 +445,17 @@ struct mfma_shape {
 // A block's share of a coefficient image on its way to LDS: every thread
 // issues all of its loads first and stores them later, so that the loads are
 // in flight together and overlap the cluster geometry.
-template <typename R, int N, int NT>
+template <typename R, int N, int NT, int NSRC = N>
 struct mat_stage {
     static constexpr int PER = (N + NT - 1) / NT;
     R v[PER];
+    // src holds the first NSRC of the N elements; the rest are zeros
     __device__ __forceinline__ void load(const R *__restrict__ src)
     {
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int x = (int)threadIdx.x + u * NT;
-            v[u] = x < N ? src[x] : (R)0;
+            v[u] = x < NSRC ? src[x] : (R)0;
         }
     }
     template <class AT>
@@ -465,14 +482,15 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
     using SH = mfma_shape<D>;
     using MA = mfma_acc<R>;
     typedef typename MA::type acc_t;
-    using LM = lds_mat<SH::KP, SH::MP>;
+    using LQ = lds_sq<SH::MP>;
     constexpr int K32 = (SH::KP + 31) / 32 * 32;
     using LB = lds_matb<SH::MP, K32>;
     constexpr int TS = 6 * D + 1;     // per-cluster stride of the staged polynomials (odd: conflict-free)
-    // matrix image in LDS: R[KP][MP] (swizzled), or BF16 transposed [MP][K32 + 8]
-    // BF = 1: BF16 image; BF = 3: BF16 high and residual images (three-product split)
+    // matrix image in LDS: R[MP][MP] (rows k >= KP zero, swizzled), or BF16
+    // transposed [MP][K32 + 8]; BF = 1: BF16 image; BF = 3: BF16 high and
+    // residual images (three-product split)
     __shared__ __attribute__((aligned(16))) unsigned short ClRaw[BF ? (BF == 3 ? 2 : 1) * LB::size
-                                                                     : (int)(LM::size * sizeof(R) / 2)];
+                                                                     : (int)(LQ::size * sizeof(R) / 2)];
     R *Cl = (R *)ClRaw;
     // staged polynomials; after the contraction the same memory holds the force table
     union stage_t {
@@ -495,7 +513,8 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
         const int g = find_group(gbase, G, t0);
         const chx_group &grp = groups[g];
         if (grp.type < 0 || grp.D != D) return;
-        mat_stage<R, BF ? 1 : SH::KP * SH::MP, 64 * WPB> cm;
+        // rows KP .. MP - 1 of the square image read as zeros (src holds KP x MP)
+        mat_stage<R, BF ? 1 : SH::MP * SH::MP, 64 * WPB, BF ? 1 : SH::KP * SH::MP> cm;
         if constexpr (BF) {
             for (int x = threadIdx.x; x < (BF == 3 ? 2 : 1) * SH::MP * K32; x += 64 * WPB) {
                 const int img = x / (SH::MP * K32), y = x % (SH::MP * K32);
@@ -517,7 +536,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
                 rec = list[c];
                 act = cluster_geometry<4>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, perm);
             }
-            if constexpr (!BF) cm.store([&](int x) -> R & { return Cl[LM::at(x / SH::MP, x % SH::MP)]; });
+            if constexpr (!BF) cm.store([&](int x) -> R & { return Cl[LQ::at(x / SH::MP, x % SH::MP)]; });
             double pm[3] = {p[0][0], p[0][1], p[0][2]};
 #pragma unroll
             for (int m = 1; m < 4; m++)
@@ -569,10 +588,77 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
         const R t2 = sub < D ? Tj[2 * D + sub] : (R)0, dt2 = sub < D ? dTj[2 * D + sub] : (R)0;
         const R t5 = sub < D ? Tj[5 * D + sub] : (R)0, dt5 = sub < D ? dTj[5 * D + sub] : (R)0;
         R S = 0, Gs[6] = {0, 0, 0, 0, 0, 0};
+        if constexpr (!BF) {
+            // Two products per 16-row block of the square image C[abc][def]:
+            // Z = C^T X (rows def) gives the energy and the gradients of pairs
+            // 12, 13, 23 against Y and its derivatives; W = C Y (rows abc)
+            // gives those of pairs 01, 02, 03 against the derivatives of X.
+            R T2[D], dT2[D];
 #pragma unroll
-        for (int rb = 0; rb < SH::RB; rb++) {
-            acc_t z0 = {0, 0, 0, 0}, z1 = {0, 0, 0, 0}, z2 = {0, 0, 0, 0}, z3 = {0, 0, 0, 0};
-            if constexpr (BF) {
+            for (int x = 0; x < D; x++) { T2[x] = Tj[2 * D + x]; dT2[x] = dTj[2 * D + x]; }
+#pragma unroll
+            for (int rb = 0; rb < SH::RB; rb++) {
+                acc_t z = {0, 0, 0, 0}, w = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < SH::KS; s++) {
+                    const int a = s / D, b = s % D;
+                    const R Az = Cl[LQ::at(4 * s + sub, 16 * rb + j)];
+                    const R Aw = Cl[LQ::at(16 * rb + j, 4 * s + sub)];
+                    z = MA::op(Az, (T0[a] * T1[b]) * t2, z);
+                    w = MA::op(Aw, (T3[a] * T4[b]) * t5, w);
+                }
+                if (MA::f64) {
+                    // output row 16 rb + sub + 4 v: (d, e) or (a, b) = 4 rb + v, f or c = sub
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int de = 4 * rb + v;
+                        if (de >= D * D) continue;
+                        const int d = de / D, e = de % D;
+                        const R t34 = T3[d] * T4[e];
+                        S += z[v] * (t34 * t5);
+                        Gs[3] += z[v] * ((dT3[d] * T4[e]) * t5);
+                        Gs[4] += z[v] * ((T3[d] * dT4[e]) * t5);
+                        Gs[5] += z[v] * (t34 * dt5);
+                        Gs[0] += w[v] * ((dT0[d] * T1[e]) * t2);
+                        Gs[1] += w[v] * ((T0[d] * dT1[e]) * t2);
+                        Gs[2] += w[v] * ((T0[d] * T1[e]) * dt2);
+                    }
+                } else {
+                    // output row 16 rb + 4 sub + v: (d, e) or (a, b) = 4 rb + sub, f or c = v
+                    R q34 = 0, q3 = 0, q4 = 0, p01 = 0, p0 = 0, p1 = 0;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int de = 4 * rb + u;
+                        if (de >= D * D) continue;
+                        const int d = de / D, e = de % D;
+                        if (sub == u) {
+                            q34 = T3[d] * T4[e];
+                            q3 = dT3[d] * T4[e];
+                            q4 = T3[d] * dT4[e];
+                            p01 = T0[d] * T1[e];
+                            p0 = dT0[d] * T1[e];
+                            p1 = T0[d] * dT1[e];
+                        }
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        if (v >= D) continue;
+                        S += z[v] * (q34 * T5[v]);
+                        Gs[3] += z[v] * (q3 * T5[v]);
+                        Gs[4] += z[v] * (q4 * T5[v]);
+                        Gs[5] += z[v] * (q34 * dT5[v]);
+                        Gs[0] += w[v] * (p0 * T2[v]);
+                        Gs[1] += w[v] * (p1 * T2[v]);
+                        Gs[2] += w[v] * (p01 * dT2[v]);
+                    }
+                }
+                // keep the operand loads of the next row block out of this one
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+#pragma unroll
+            for (int rb = 0; rb < SH::RB; rb++) {
+                acc_t z0 = {0, 0, 0, 0}, z1 = {0, 0, 0, 0}, z2 = {0, 0, 0, 0}, z3 = {0, 0, 0, 0};
                 // k = 32 s + 8 sub + q over (a, b, c) = (k / 4D, (k / 4) % D, k % 4)
 #pragma unroll
                 for (int s = 0; s < K32 / 32; s++) {
@@ -610,35 +696,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
                         z3 = mfma_bf16(A, h, z3); z3 = mfma_bf16(A, l, z3); z3 = mfma_bf16(Al, h, z3);
                     }
                 }
-            } else {
-#pragma unroll
-                for (int s = 0; s < SH::KS; s++) {
-                    const int a = s / D, b = s % D;
-                    const R A = Cl[LM::at(4 * s + sub, 16 * rb + j)];
-                    const R x01 = T0[a] * T1[b];
-                    z0 = MA::op(A, x01 * t2, z0);
-                    z1 = MA::op(A, (dT0[a] * T1[b]) * t2, z1);
-                    z2 = MA::op(A, (T0[a] * dT1[b]) * t2, z2);
-                    z3 = MA::op(A, x01 * dt2, z3);
-                }
-            }
-            if (MA::f64) {
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int de = 4 * rb + v;
-                    if (de >= D * D) continue;
-                    const int d = de / D, e = de % D;
-                    const R t34 = T3[d] * T4[e];
-                    const R y = t34 * t5;
-                    S += z0[v] * y;
-                    Gs[0] += z1[v] * y;
-                    Gs[1] += z2[v] * y;
-                    Gs[2] += z3[v] * y;
-                    Gs[3] += z0[v] * ((dT3[d] * T4[e]) * t5);
-                    Gs[4] += z0[v] * ((T3[d] * dT4[e]) * t5);
-                    Gs[5] += z0[v] * (t34 * dt5);
-                }
-            } else {
+                // fold against Y (FP32 output layout: row 16 rb + 4 sub + v)
                 R q34 = 0, q3 = 0, q4 = 0;
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
@@ -663,9 +721,8 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
                     Gs[4] += z0[v] * (q4 * T5[v]);
                     Gs[5] += z0[v] * (q34 * dT5[v]);
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
-            // keep the operand loads of the next row block out of this one
-            __builtin_amdgcn_sched_barrier(0);
         }
         // sum over the four lanes of each cluster
 #pragma unroll
