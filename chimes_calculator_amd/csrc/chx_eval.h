@@ -22,12 +22,10 @@ struct chx_list_args {
     int *cfg_stat, *status;
 };
 
-__global__ void k_fin_forces(int natoms_in, int ncfg, const int *cfg_in_off, const chx_cfg *cfgs,
-                             const unsigned long long *Facc, double *Fout);
-__global__ void k_fin_part(int ncfg, const int *cfg_item_off, const chx_cfg *cfgs, const chx_atom_s *as,
-                           const chx_model_dev *models, const unsigned long long *Oacc, long long *part,
-                           int *status);
-__global__ void k_fin_cfg(int ncfg, const int *cfg_item_off, const long long *part, double *Ecfg, double *Vcfg);
+__global__ void k_fin(int ncfg, int nitems, const int *cfg_item_off, int natoms_in, const int *cfg_in_off,
+                      const chx_cfg *cfgs, const chx_atom_s *as, const chx_model_dev *models,
+                      const unsigned long long *Facc, const unsigned long long *Oacc, long long *part, int *done,
+                      double *Fout, double *Ecfg, double *Vcfg, int *status);
 
 bool supported_D3(int D);
 bool supported_D4(int D);

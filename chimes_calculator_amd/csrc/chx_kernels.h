@@ -27,8 +27,7 @@ __global__ void k_cell_fill(int natoms, const chx_atom_n *an, const int *cell_st
 template <int MODE>
 __global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
                       const int *cell_start, const int *cell_atoms, int *nbr_cnt, const int *nbr_off,
-                      chx_entry_s *es, chx_entry_n *en, int *e_owner, int *status);
-__global__ void k_check_entries(const int *nbr_off, int natoms, long cap, int *status);
+                      chx_entry_s *es, chx_entry_n *en, int *e_owner, long ent_cap, int *status);
 __global__ void k_group_bases(const int *off, long N, int G3, int G4, long cap3, long cap4, int *base3, int *end3,
                               int *base4, int *end4, int *status);
 // arguments of the enumeration kernels

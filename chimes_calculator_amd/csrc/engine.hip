@@ -10,7 +10,7 @@
 //                 aligned so evaluation tiles share coefficients
 //   k_eval2/3/4   cluster energies, forces and virials, accumulated exactly
 //                 in fixed point per atom and per owner
-//   k_fin_*       forces per input atom, energy and virial per configuration
+//   k_fin         forces per input atom, energy and virial per configuration
 // Buffers are sized by capacity: a run that finds a capacity short sets a
 // status bit, skips the rest and is repeated with the size it reported, so a
 // run needs one host synchronisation (at the end).
@@ -343,6 +343,7 @@ struct chx_engine {
     DevBuf<int2> list3, list4;
     DevBuf<unsigned long long> Facc, Oacc;
     DevBuf<long long> part;
+    DevBuf<int> done;  // per configuration: reduction items finished
     std::vector<DevBuf<int>> scan_tmp;
     // per-cluster API buffers
     DevBuf<double> cl_in, cl_out;
@@ -773,6 +774,7 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->Facc.ensure(6 * N);
     e->Oacc.ensure(14 * N);
     e->part.ensure(14L * e->nitems);
+    e->done.ensure(ncfg + 1);
     // input blob
     size_t off = 0;
     const size_t o_cfg = off; off = align256(off + ncfg * sizeof(chx_cfg));
@@ -852,6 +854,7 @@ void size_work(chx_engine *e)
     cl[n++] = chx_clear_range{e->cnt34.p, (long)(e->G3 + e->G4) * N + 1};
     cl[n++] = chx_clear_range{(int *)e->Facc.p, 12 * N};
     cl[n++] = chx_clear_range{(int *)e->Oacc.p, 28 * N};
+    cl[n++] = chx_clear_range{e->done.p, (long)e->ncfg + 1};
     e->nclear = n;
     memcpy(e->in_h.p + ((char *)e->clear - e->in_d.p), cl, sizeof(cl));
 }
@@ -875,12 +878,11 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     // neighbour images, sorted per owner
     hipLaunchKernelGGL((k_nbr<0>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
                        e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr, (chx_entry_s *)nullptr,
-                       (chx_entry_n *)nullptr, (int *)nullptr, e->status);
+                       (chx_entry_n *)nullptr, (int *)nullptr, e->ent_cap, e->status);
     e->scan(e->nbr_cnt.p, e->nbr_off.p, N);
-    hipLaunchKernelGGL(k_check_entries, dim3(1), dim3(1), 0, st, e->nbr_off.p, (int)N, e->ent_cap, e->status);
     hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
                        e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
-                       e->e_owner.p, e->status);
+                       e->e_owner.p, e->ent_cap, e->status);
     // 3-/4-body lists in type-aligned segments
     if (G3 + G4 > 0) {
         chx_enum_args ea{};
@@ -950,12 +952,9 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         for (int D : e->D4s) launch_eval_body(4, D, prec, e->var4, l4, st);
     }
     e->record(4);
-    hipLaunchKernelGGL(k_fin_forces, dim3(nblk(Nin, 256)), dim3(256), 0, st, (int)Nin, ncfg, e->cfg_in_off,
-                       e->cfgs, e->Facc.p, Fout);
-    hipLaunchKernelGGL(k_fin_part, dim3(e->nitems), dim3(256), 0, st, ncfg, e->cfg_item_off, e->cfgs, e->as.p,
-                       e->mdev, e->Oacc.p, e->part.p, e->status);
-    hipLaunchKernelGGL(k_fin_cfg, dim3(nblk(ncfg, 256)), dim3(256), 0, st, ncfg, e->cfg_item_off, e->part.p,
-                       Ecfg, Vcfg);
+    hipLaunchKernelGGL(k_fin, dim3(e->nitems + nblk(Nin, 256)), dim3(256), 0, st, ncfg, e->nitems,
+                       e->cfg_item_off, (int)Nin, e->cfg_in_off, e->cfgs, e->as.p, e->mdev, e->Facc.p, e->Oacc.p,
+                       e->part.p, e->done.p, Fout, Ecfg, Vcfg, e->status);
     HIPCHK(hipMemcpyAsync(e->out_h.p, e->out_d.p, e->out_bytes, hipMemcpyDeviceToHost, st));
     e->record(5);
 }

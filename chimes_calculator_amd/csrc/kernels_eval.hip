@@ -1036,55 +1036,59 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
 }
 
 // ---------------------------------------------------------------------------
-// Totals.  Force on input atom a: the exact sum over its replicas' force
-// accumulators.  Energy and virial of a configuration: the exact sum over its
-// owners plus the 1-body offsets (compute_1B, chimesFF.cpp:1636-1641).
+// Totals, one launch.  Blocks [0, nitems): energy and virial, one chunk of
+// CHX_FIN_CHUNK owners of one configuration each (item; cfg_item_off[c] is
+// the first item of c), exact partial sums normalised to part[item][7][2] =
+// (hi, lo < 2^40) plus the 1-body offsets (compute_1B, chimesFF.cpp:1636-
+// 1641); the last block of a configuration to finish (done[c], zeroed before
+// the run) adds its items' partials and writes the totals.  Blocks from
+// nitems on: force on input atom a, the exact sum over its replicas' force
+// accumulators.
 // ---------------------------------------------------------------------------
-__global__ void k_fin_forces(int natoms_in, int ncfg, const int *__restrict__ cfg_in_off,
-                             const chx_cfg *__restrict__ cfgs, const unsigned long long *__restrict__ Facc,
-                             double *Fout)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natoms_in) return;
-    int lo = 0, hi = ncfg - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cfg_in_off[mid] <= a) lo = mid; else hi = mid - 1;
-    }
-    const chx_cfg &C = cfgs[lo];
-    const int la = a - C.atom_in_off;
-    const int ncopy = (C.n_rep + 1) * (C.n_rep + 1) * (C.n_rep + 1);
-    fx_sum f[3];
-    for (int rho = 0; rho < ncopy; rho++) {
-        const long r = C.atom_off + (long)rho * C.natoms_in + la;
-#pragma unroll
-        for (int k = 0; k < 3; k++) f[k].add(Facc + 2L * (3L * r + k));
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) Fout[3L * a + k] = f[k].value();
-}
-
-// Energy and virial, first pass: one block per chunk of CHX_FIN_CHUNK owners
-// of one configuration (item; cfg_item_off[c] is the first item of c), exact
-// partial sums normalised to part[item][7][2] = (hi, lo < 2^40).
-__global__ __launch_bounds__(256) void k_fin_part(int ncfg, const int *__restrict__ cfg_item_off,
-                                                   const chx_cfg *__restrict__ cfgs,
-                                                   const chx_atom_s *__restrict__ as,
-                                                   const chx_model_dev *__restrict__ models,
-                                                   const unsigned long long *__restrict__ Oacc, long long *part,
-                                                   int *status)
+__global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__restrict__ cfg_item_off,
+                                              int natoms_in, const int *__restrict__ cfg_in_off,
+                                              const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
+                                              const chx_model_dev *__restrict__ models,
+                                              const unsigned long long *__restrict__ Facc,
+                                              const unsigned long long *__restrict__ Oacc, long long *part,
+                                              int *done, double *Fout, double *Ecfg, double *Vcfg, int *status)
 {
     __shared__ long long sh_hi[7][256];
     __shared__ unsigned long long sh_lo[7][256];
-    const int item = blockIdx.x;
-    int lo = 0, hi = ncfg - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cfg_item_off[mid] <= item) lo = mid; else hi = mid - 1;
+    __shared__ int last;
+    if ((int)blockIdx.x >= nitems) {
+        const int a = ((int)blockIdx.x - nitems) * 256 + threadIdx.x;
+        if (a >= natoms_in) return;
+        int lo = 0, hi = ncfg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cfg_in_off[mid] <= a) lo = mid; else hi = mid - 1;
+        }
+        const chx_cfg &C = cfgs[lo];
+        const int la = a - C.atom_in_off;
+        const int ncopy = (C.n_rep + 1) * (C.n_rep + 1) * (C.n_rep + 1);
+        fx_sum f[3];
+        for (int rho = 0; rho < ncopy; rho++) {
+            const long r = C.atom_off + (long)rho * C.natoms_in + la;
+#pragma unroll
+            for (int k = 0; k < 3; k++) f[k].add(Facc + 2L * (3L * r + k));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) Fout[3L * a + k] = f[k].value();
+        return;
     }
-    const chx_cfg &C = cfgs[lo];
+    const int item = blockIdx.x;
+    int c = 0;
+    {
+        int hi = ncfg - 1;
+        while (c < hi) {
+            const int mid = (c + hi + 1) >> 1;
+            if (cfg_item_off[mid] <= item) c = mid; else hi = mid - 1;
+        }
+    }
+    const chx_cfg &C = cfgs[c];
     const chx_model_dev &M = models[C.model];
-    const long r0 = C.atom_off + (long)(item - cfg_item_off[lo]) * CHX_FIN_CHUNK;
+    const long r0 = C.atom_off + (long)(item - cfg_item_off[c]) * CHX_FIN_CHUNK;
     const long r1 = r0 + CHX_FIN_CHUNK < (long)C.atom_off + C.natoms ? r0 + CHX_FIN_CHUNK
                                                                       : (long)C.atom_off + C.natoms;
     fx_sum acc[7];
@@ -1109,27 +1113,42 @@ __global__ __launch_bounds__(256) void k_fin_part(int ncfg, const int *__restric
             }
         __syncthreads();
     }
+    if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
+    const int nit = cfg_item_off[c + 1] - cfg_item_off[c];
     if (threadIdx.x < 7) {
         const int k = threadIdx.x;
-        part[14L * item + 2 * k] = sh_hi[k][0] + (long long)(sh_lo[k][0] >> 40);
-        part[14L * item + 2 * k + 1] = (long long)(sh_lo[k][0] & 0xffffffffffull);
+        const long long h = sh_hi[k][0] + (long long)(sh_lo[k][0] >> 40);
+        const long long l = (long long)(sh_lo[k][0] & 0xffffffffffull);
+        if (nit == 1) {
+            fx_sum t;
+            t.hi = h;
+            t.lo = (unsigned long long)l;
+            if (k == 0) Ecfg[c] = t.value(); else Vcfg[6L * c + k - 1] = t.value();
+        } else {
+            __atomic_store_n(&part[14L * item + 2 * k], h, __ATOMIC_RELAXED);
+            __atomic_store_n(&part[14L * item + 2 * k + 1], l, __ATOMIC_RELAXED);
+        }
     }
-    if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
-}
-
-// Second pass: one thread per configuration sums its items' partials.
-__global__ void k_fin_cfg(int ncfg, const int *__restrict__ cfg_item_off, const long long *__restrict__ part,
-                          double *Ecfg, double *Vcfg)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncfg) return;
-    fx_sum t[7];
-    for (int it = cfg_item_off[c]; it < cfg_item_off[c + 1]; it++)
-#pragma unroll
-        for (int k = 0; k < 7; k++) t[k].add((const unsigned long long *)part + 14L * it + 2 * k);
-    Ecfg[c] = t[0].value();
-#pragma unroll
-    for (int k = 0; k < 6; k++) Vcfg[6L * c + k] = t[1 + k].value();
+    if (nit == 1) return;
+    // publish this item's partials, then count it; the last item of the
+    // configuration sums all of them
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&done[c], 1) == nit - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < 7) {
+        const int k = threadIdx.x;
+        fx_sum t;
+        for (int it = cfg_item_off[c]; it < cfg_item_off[c + 1]; it++) {
+            const unsigned long long v[2] = {
+                (unsigned long long)__atomic_load_n(&part[14L * it + 2 * k], __ATOMIC_RELAXED),
+                (unsigned long long)__atomic_load_n(&part[14L * it + 2 * k + 1], __ATOMIC_RELAXED)};
+            t.add(v);
+        }
+        if (k == 0) Ecfg[c] = t.value(); else Vcfg[6L * c + k - 1] = t.value();
+    }
 }
 
 // ---------------------------------------------------------------------------

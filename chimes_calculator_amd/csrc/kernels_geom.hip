@@ -144,13 +144,22 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
                                               const chx_atom_s *__restrict__ as, const chx_atom_n *__restrict__ an,
                                               const int *__restrict__ cell_start, const int *__restrict__ cell_atoms,
                                               int *nbr_cnt, const int *__restrict__ nbr_off, chx_entry_s *es,
-                                              chx_entry_n *en, int *e_owner, int *status)
+                                              chx_entry_n *en, int *e_owner, long ent_cap, int *status)
 {
     __shared__ long long keys[4][MODE ? CHX_MAX_NB : 1];
     // searched cells of each wave's owner: inclusive prefix of their atom
     // counts, first cell_atoms slot, packed image shift
     __shared__ int cpre[4][CHX_NBR_CELLS], cbeg[4][CHX_NBR_CELLS], cshf[4][CHX_NBR_CELLS];
-    if (MODE == 1 && block_stop(status, CHX_STOP)) return;
+    if (MODE == 1) {
+        if (block_stop(status, CHX_STOP)) return;
+        // entry total against the entry buffer: every block decides alike
+        const int tot = nbr_off[natoms];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status[CHX_ST_NENT] = tot;
+            if (tot > ent_cap) atomicOr(&status[CHX_ST_FLAGS], CHX_RETRY_ENT);
+        }
+        if (tot > ent_cap) return;
+    }
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + wv;
     const bool own = i < natoms;
@@ -308,15 +317,6 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
         en[off + rk] = f;
         e_owner[off + rk] = i;
     }
-}
-
-// Entry total against the entry buffer capacity.
-__global__ void k_check_entries(const int *__restrict__ nbr_off, int natoms, long cap, int *status)
-{
-    if (threadIdx.x || blockIdx.x) return;
-    const int tot = nbr_off[natoms];
-    status[CHX_ST_NENT] = tot;
-    if (tot > cap) atomicOr(&status[CHX_ST_FLAGS], CHX_RETRY_ENT);
 }
 
 // Aligned segment of each (model, cluster type) group in the 3- and 4-body
@@ -582,9 +582,9 @@ __global__ __launch_bounds__(64) void k_enum_big(chx_enum_args x)
 }
 
 template __global__ void k_nbr<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, int *);
+                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, long, int *);
 template __global__ void k_nbr<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, int *);
+                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, long, int *);
 template __global__ void k_enum<0>(chx_enum_args);
 template __global__ void k_enum<1>(chx_enum_args);
 template __global__ void k_enum_big<0>(chx_enum_args);
@@ -624,20 +624,26 @@ __global__ __launch_bounds__(256) void k_scan_block(const int *in, int *out, lon
 // 1024 threads, each summing a contiguous chunk; out[n] = total.
 __global__ __launch_bounds__(1024) void k_scan_one(const int *in, int *out, long n)
 {
+    constexpr int PER = CHX_SCAN1_MAX / 1024;
     __shared__ int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const long per = (n + 1023) / 1024;
-    const long b = tid * per;
-    const long e = b + per < n ? b + per : n;
+    const int per = (int)((n + 1023) / 1024);
+    const long b = (long)tid * per;
+    // the thread's segment in registers: all loads in flight together
+    int x[PER];
     int t = 0;
-    for (long i = b; i < e; i++) t += in[i];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        x[u] = (u < per && b + u < n) ? in[b + u] : 0;
+        t += x[u];
+    }
     // inclusive scan of the thread totals: within each wavefront, then over
     // the 16 wavefront totals
     int v = t;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const int u = __shfl_up(v, d, 64);
-        if (lane >= d) v += u;
+        const int q = __shfl_up(v, d, 64);
+        if (lane >= d) v += q;
     }
     if (lane == 63) wsum[w] = v;
     __syncthreads();
@@ -645,17 +651,17 @@ __global__ __launch_bounds__(1024) void k_scan_one(const int *in, int *out, long
         int q = lane < 16 ? wsum[lane] : 0;
 #pragma unroll
         for (int d = 1; d < 16; d <<= 1) {
-            const int u = __shfl_up(q, d, 64);
-            if (lane >= d) q += u;
+            const int y = __shfl_up(q, d, 64);
+            if (lane >= d) q += y;
         }
         if (lane < 16) wsum[lane] = q;
     }
     __syncthreads();
     int run = v - t + (w ? wsum[w - 1] : 0);
-    for (long i = b; i < e; i++) {
-        const int x = in[i];
-        out[i] = run;
-        run += x;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        if (u < per && b + u < n) out[b + u] = run;
+        run += x[u];
     }
     if (tid == 1023) out[n] = wsum[15];
 }
