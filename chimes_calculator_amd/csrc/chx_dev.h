@@ -57,35 +57,81 @@ __device__ __forceinline__ void fx_atomic(unsigned long long *acc, double v, boo
 // parts are summed exactly across lanes and only the run's last lane issues
 // global atomics.  Must be reached by all lanes of the wavefront.
 
-// first lane (within the group) of this lane's run
+// Lane movement within rows of 16 lanes (DPP row_shr:D, zero where the
+// source lane is outside the row), and a value of a fixed lane read into a
+// scalar register: the scans below use no LDS.
+template <int D>
+__device__ __forceinline__ int row_shr_i32(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x110 + D, 0xf, 0xf, true);
+}
+template <int D>
+__device__ __forceinline__ long long row_shr_i64(long long x)
+{
+    const int lo = row_shr_i32<D>((int)(unsigned long long)x);
+    const int hi = row_shr_i32<D>((int)((unsigned long long)x >> 32));
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ long long readlane_i64(long long x, int l)
+{
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)x, l);
+    const int hi = __builtin_amdgcn_readlane((int)((unsigned long long)x >> 32), l);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// first lane (within the group) of this lane's run: a running maximum of the
+// run starts, within rows of 16 lanes, then across rows (W = 64)
 template <int W>
 __device__ __forceinline__ int fx_run_first(int key)
 {
+    static_assert(W == 16 || W == 64, "run groups of 16 or 64 lanes");
     const int l = threadIdx.x & (W - 1);
     const int kprev = __shfl_up(key, 1, W);
     int seg = (l == 0 || kprev != key) ? l : 0;
-#pragma unroll
-    for (int d = 1; d < W; d <<= 1) {
-        const int t = __shfl_up(seg, d, W);
-        if (l >= d && t > seg) seg = t;
+    seg = max(seg, row_shr_i32<1>(seg));
+    seg = max(seg, row_shr_i32<2>(seg));
+    seg = max(seg, row_shr_i32<4>(seg));
+    seg = max(seg, row_shr_i32<8>(seg));
+    if constexpr (W == 64) {
+        const int r = l >> 4;
+        const int s15 = __builtin_amdgcn_readlane(seg, 15);
+        const int s31 = max(__builtin_amdgcn_readlane(seg, 31), s15);
+        const int s47 = max(__builtin_amdgcn_readlane(seg, 47), s31);
+        seg = max(seg, r == 0 ? 0 : (r == 1 ? s15 : (r == 2 ? s31 : s47)));
     }
     return seg;
 }
 
 // inclusive segmented scan of K integers: lane l ends with the sum over
-// lanes seg..l of its run
+// lanes seg..l of its run.  Within rows of 16 lanes by row shifts; then each
+// row adds the row totals before it that lie in its run (W = 64).
 template <int W, int K>
 __device__ __forceinline__ void fx_run_scan(long long (&x)[K], int seg)
 {
     const int l = threadIdx.x & (W - 1);
+    const int lr = threadIdx.x & 15;
+#define CHX_RUN_STEP(D)                                                  \
+    _Pragma("unroll") for (int k = 0; k < K; k++)                        \
+    {                                                                    \
+        const long long t = row_shr_i64<D>(x[k]);                        \
+        if (lr >= D && l - D >= seg) x[k] += t;                          \
+    }
+    CHX_RUN_STEP(1)
+    CHX_RUN_STEP(2)
+    CHX_RUN_STEP(4)
+    CHX_RUN_STEP(8)
+#undef CHX_RUN_STEP
+    if constexpr (W == 64) {
+        const int r = l >> 4;
 #pragma unroll
-    for (int d = 1; d < W; d <<= 1) {
-        long long t[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) t[k] = __shfl_up(x[k], d, W);
-        if (l - d >= seg)
-#pragma unroll
-            for (int k = 0; k < K; k++) x[k] += t[k];
+        for (int k = 0; k < K; k++) {
+            const long long R0 = readlane_i64(x[k], 15), R1 = readlane_i64(x[k], 31), R2 = readlane_i64(x[k], 47);
+            const long long Rm1 = r == 1 ? R0 : (r == 2 ? R1 : R2);  // row r - 1
+            const long long Rm2 = r == 2 ? R0 : R1;                   // row r - 2
+            if (r >= 1 && seg <= 16 * r - 1) x[k] += Rm1;
+            if (r >= 2 && seg <= 16 * r - 17) x[k] += Rm2;
+            if (r >= 3 && seg <= 16 * r - 33) x[k] += R0;
+        }
     }
 }
 
@@ -97,25 +143,22 @@ __device__ __forceinline__ void fx_owner_runs(unsigned long long *Facc, unsigned
                                               const double (&f)[3], const double (&q)[7], bool &bad)
 {
     if (__ballot(key >= 0) == 0) return;
-    long long hi[10], lo[10];
-#pragma unroll
-    for (int k = 0; k < 10; k++) {
-        hi[k] = 0;
-        lo[k] = 0;
-        if (key >= 0) fx_split(k < 3 ? f[k] : q[k - 3], hi[k], lo[k], bad);
-    }
     const int seg = fx_run_first<W>(key);
-    fx_run_scan<W, 10>(hi, seg);
-    fx_run_scan<W, 10>(lo, seg);
     const int l = threadIdx.x & (W - 1);
     const int knext = __shfl_down(key, 1, W);
-    if (key >= 0 && (l == W - 1 || knext != key)) {
+    const bool last = key >= 0 && (l == W - 1 || knext != key);
+    // one value at a time (bounded register use): split, scan, add
 #pragma unroll
-        for (int k = 0; k < 10; k++) {
+    for (int k = 0; k < 10; k++) {
+        long long x[2] = {0, 0};
+        if (key >= 0) fx_split(k < 3 ? f[k] : q[k - 3], x[0], x[1], bad);
+        fx_run_scan<W, 2>(x, seg);
+        if (last) {
             unsigned long long *a = k < 3 ? Facc + 2L * (3L * key + k) : Oacc + 2L * (7L * key + (k - 3));
-            if (hi[k]) atomicAdd(a, (unsigned long long)hi[k]);
-            if (lo[k]) atomicAdd(a + 1, (unsigned long long)lo[k]);
+            if (x[0]) atomicAdd(a, (unsigned long long)x[0]);
+            if (x[1]) atomicAdd(a + 1, (unsigned long long)x[1]);
         }
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
