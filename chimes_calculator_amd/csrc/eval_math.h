@@ -131,6 +131,11 @@ __device__ __forceinline__ void eval_pair(double r, const chx_slot &s, const dou
 // ---------------------------------------------------------------------------
 // Dense contraction of the 3-body form (pairs q = 0, 1, 2).
 // ---------------------------------------------------------------------------
+// Coefficient tensors are read-only for a whole launch: reading them through
+// the constant address space lets wave-uniform reads use scalar loads.
+template <typename R>
+using const_ptr = const __attribute__((address_space(4))) R *;
+
 template <int D, typename R, typename A>
 __device__ __forceinline__ void contract3(const R *__restrict__ C, const R (&T)[3][D], const R (&dT)[3][D],
                                           A &S, A (&G)[3])
@@ -146,7 +151,7 @@ __device__ __forceinline__ void contract3(const R *__restrict__ C, const R (&T)[
         A s1 = 0, g11 = 0, g12 = 0;
 #pragma unroll
         for (int a1 = 0; a1 < D; a1++) {
-            const R *c = C + (a0 * D + a1) * D;
+            const_ptr<R> c = (const_ptr<R>)C + (a0 * D + a1) * D;
             R z = 0, z2 = 0;
 #pragma unroll
             for (int a2 = 0; a2 < D; a2++) {
@@ -198,7 +203,7 @@ __device__ __forceinline__ void contract4(const R *__restrict__ C, const R (&T)[
                     R s4 = 0, h4[2] = {0, 0};
 #pragma unroll
                     for (int a4 = 0; a4 < D; a4++) {
-                        const R *c = C + ((((a0 * D + a1) * D + a2) * D + a3) * D + a4) * D;
+                        const_ptr<R> c = (const_ptr<R>)C + ((((a0 * D + a1) * D + a2) * D + a3) * D + a4) * D;
                         R z = 0, z5 = 0;
 #pragma unroll
                         for (int a5 = 0; a5 < D; a5++) {
