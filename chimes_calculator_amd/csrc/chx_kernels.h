@@ -28,6 +28,10 @@ template <int MODE>
 __global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
                       const int *cell_start, const int *cell_atoms, int *nbr_cnt, const int *nbr_off,
                       chx_entry_s *es, chx_entry_n *en, int *e_owner, long ent_cap, int *status);
+template <int MODE>
+__global__ void k_nbr_wide(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
+                           const int *cell_start, const int *cell_atoms, int *nbr_cnt, const int *nbr_off,
+                           chx_entry_s *es, chx_entry_n *en, int *e_owner, long ent_cap, int *status);
 __global__ void k_group_bases(const int *off, long N, int G3, int G4, long cap3, long cap4, int *base3, int *end3,
                               int *base4, int *end4, int *status);
 // arguments of the enumeration kernels
@@ -47,7 +51,7 @@ struct chx_enum_args {
     int *big;                      // [natoms]: owners queued for k_enum_big
     int *status;
 };
-template <int MODE>
+template <int MODE, int NT>
 __global__ void k_enum(chx_enum_args x);
 template <int MODE>
 __global__ void k_enum_big(chx_enum_args x);

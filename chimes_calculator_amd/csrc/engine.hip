@@ -876,13 +876,25 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, e->an.p, e->cell_start.p,
                        e->cell_cur.p, e->cell_atoms.p);
     // neighbour images, sorted per owner
-    hipLaunchKernelGGL((k_nbr<0>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
-                       e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr, (chx_entry_s *)nullptr,
-                       (chx_entry_n *)nullptr, (int *)nullptr, e->ent_cap, e->status);
+    // small batches: one owner per workgroup (four wavefronts), else one per wavefront
+    const bool wide = N <= CHX_ENUM_WIDE_MAX;
+    if (wide)
+        hipLaunchKernelGGL((k_nbr_wide<0>), dim3((unsigned)N), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
+                           e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr,
+                           (chx_entry_s *)nullptr, (chx_entry_n *)nullptr, (int *)nullptr, e->ent_cap, e->status);
+    else
+        hipLaunchKernelGGL((k_nbr<0>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
+                           e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr,
+                           (chx_entry_s *)nullptr, (chx_entry_n *)nullptr, (int *)nullptr, e->ent_cap, e->status);
     e->scan(e->nbr_cnt.p, e->nbr_off.p, N);
-    hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
-                       e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
-                       e->e_owner.p, e->ent_cap, e->status);
+    if (wide)
+        hipLaunchKernelGGL((k_nbr_wide<1>), dim3((unsigned)N), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
+                           e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
+                           e->e_owner.p, e->ent_cap, e->status);
+    else
+        hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
+                           e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
+                           e->e_owner.p, e->ent_cap, e->status);
     // 3-/4-body lists in type-aligned segments
     if (G3 + G4 > 0) {
         chx_enum_args ea{};
@@ -906,14 +918,17 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         ea.status = e->status;
         // owners with many images are queued by k_enum<0> for the large-LDS kernel
         const dim3 gbig(1280);
-        hipLaunchKernelGGL((k_enum<0>), dim3((unsigned)N), dim3(64), 0, st, ea);
+        // small batches: four wavefronts share an owner's enumeration (latency)
+        if (wide) hipLaunchKernelGGL((k_enum<0, 256>), dim3((unsigned)N), dim3(256), 0, st, ea);
+        else hipLaunchKernelGGL((k_enum<0, 64>), dim3((unsigned)N), dim3(64), 0, st, ea);
         hipLaunchKernelGGL((k_enum_big<0>), gbig, dim3(64), 0, st, ea);
         // one scan over both bodies' counts: segments and positions use
         // differences of the scan only
         e->scan(e->cnt34.p, e->off34.p, (long)(G3 + G4) * N);
         hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(64), 0, st, e->off34.p, N, G3, G4, e->l3_cap, e->l4_cap,
                            e->gb3.p, e->ge3.p, e->gb4.p, e->ge4.p, e->status);
-        hipLaunchKernelGGL((k_enum<1>), dim3((unsigned)N), dim3(64), 0, st, ea);
+        if (wide) hipLaunchKernelGGL((k_enum<1, 256>), dim3((unsigned)N), dim3(256), 0, st, ea);
+        else hipLaunchKernelGGL((k_enum<1, 64>), dim3((unsigned)N), dim3(64), 0, st, ea);
         hipLaunchKernelGGL((k_enum_big<1>), gbig, dim3(64), 0, st, ea);
     }
     e->record(1);

@@ -319,6 +319,183 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
     }
 }
 
+// The same lists for small batches: one owner per workgroup, the searched
+// candidates spread over its four wavefronts (latency-bound otherwise).
+// Cells beyond the flattened range are searched by wavefront 0 alone.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__restrict__ cfgs,
+                                                   const chx_atom_s *__restrict__ as,
+                                                   const chx_atom_n *__restrict__ an,
+                                                   const int *__restrict__ cell_start,
+                                                   const int *__restrict__ cell_atoms, int *nbr_cnt,
+                                                   const int *__restrict__ nbr_off, chx_entry_s *es, chx_entry_n *en,
+                                                   int *e_owner, long ent_cap, int *status)
+{
+    __shared__ long long keys[MODE ? CHX_MAX_NB : 1];
+    __shared__ int cpre[CHX_NBR_CELLS], cbeg[CHX_NBR_CELLS], cshf[CHX_NBR_CELLS];
+    __shared__ int ncount;
+    if (MODE == 1) {
+        if (block_stop(status, CHX_STOP)) return;
+        const int tot = nbr_off[natoms];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status[CHX_ST_NENT] = tot;
+            if (tot > ent_cap) atomicOr(&status[CHX_ST_FLAGS], CHX_RETRY_ENT);
+        }
+        if (tot > ent_cap) return;
+    }
+    const int i = blockIdx.x;
+    if (i >= natoms) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const chx_atom_n ni = an[i];
+    const chx_cfg &C = cfgs[as[i].cfg];
+    const int lc = ni.cell - C.cell_off;
+    const int ci[3] = {lc % C.m[0], (lc / C.m[0]) % C.m[1], lc / (C.m[0] * C.m[1])};
+    const int w0 = 2 * C.reach[0] + 1, w1 = 2 * C.reach[1] + 1;
+    const int nc = w0 * w1 * (2 * C.reach[2] + 1);
+    auto searched_cell = [&](int dx, int dy, int dz, int &cell, int (&sh)[3]) {
+        const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
+        int wc[3];
+        bool ok = true;
+        for (int d = 0; d < 3; d++) {
+            int q = u[d] % C.m[d];
+            if (q < 0) q += C.m[d];
+            wc[d] = q;
+            sh[d] = (u[d] - q) / C.m[d];
+            if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
+        }
+        cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
+        return ok;
+    };
+    // candidates of this wavefront's lanes: appended to the owner's key list
+    auto take = [&](bool valid, int a, const int (&sh)[3], int spack) {
+        bool in = false;
+        if (valid) {
+            const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
+            if (!(a < i || (a == i && zero))) {
+                double vn[3];
+                in = nbr_image(C, ni.u, an[a].u, sh, zero, vn);
+            }
+        }
+        const unsigned long long b = __ballot(in);
+        if (b == 0) return;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&ncount, __popcll(b));
+        base = __shfl(base, 0, 64);
+        if (MODE == 1 && in) {
+            const int idx = base + __popcll(b & below);
+            if (idx < CHX_MAX_NB) keys[idx] = ((long long)a << 24) | spack;
+        }
+    };
+    if (tid == 0) ncount = 0;
+    if (nc <= CHX_NBR_CELLS && wv == 0) {
+        int carry = 0;
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+            const int c = c0 + lane;
+            int cnt = 0, cs = 0, spk = 0;
+            if (c < nc) {
+                int cell, sh[3];
+                if (searched_cell(c % w0 - C.reach[0], (c / w0) % w1 - C.reach[1], c / (w0 * w1) - C.reach[2], cell,
+                                  sh)) {
+                    cs = cell_start[cell];
+                    cnt = cell_start[cell + 1] - cs;
+                    spk = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
+                }
+            }
+            int v = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int t = __shfl_up(v, d, 64);
+                if (lane >= d) v += t;
+            }
+            if (c < nc) {
+                cpre[c] = carry + v;
+                cbeg[c] = cs;
+                cshf[c] = spk;
+            }
+            carry += __shfl(v, 63, 64);
+        }
+    }
+    __syncthreads();
+    if (nc <= CHX_NBR_CELLS) {
+        const int T = cpre[nc - 1];
+        for (int t0 = wv * 64; t0 < T; t0 += 256) {
+            const int t = t0 + lane;
+            int a = 0, spk = 0, sh[3] = {0, 0, 0};
+            if (t < T) {
+                int lo = 0, hi = nc - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cpre[mid] > t) hi = mid; else lo = mid + 1;
+                }
+                const int before = lo ? cpre[lo - 1] : 0;
+                a = cell_atoms[cbeg[lo] + (t - before)];
+                spk = cshf[lo];
+                sh[0] = (spk & 255) - 128;
+                sh[1] = ((spk >> 8) & 255) - 128;
+                sh[2] = ((spk >> 16) & 255) - 128;
+            }
+            take(t < T, a, sh, spk);
+        }
+    } else if (wv == 0) {
+        for (int dz = -C.reach[2]; dz <= C.reach[2]; dz++)
+            for (int dy = -C.reach[1]; dy <= C.reach[1]; dy++)
+                for (int dx = -C.reach[0]; dx <= C.reach[0]; dx++) {
+                    int cell, sh[3];
+                    if (!searched_cell(dx, dy, dz, cell, sh)) continue;
+                    const int cs = cell_start[cell], ce = cell_start[cell + 1];
+                    const int spack = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
+                    for (int p0 = cs; p0 < ce; p0 += 64) {
+                        const int p = p0 + lane;
+                        take(p < ce, p < ce ? cell_atoms[p] : 0, sh, spack);
+                    }
+                }
+    }
+    __syncthreads();
+    const int n = ncount;
+    if (MODE == 0) {
+        if (tid == 0) {
+            nbr_cnt[i] = n;
+            // the word only grows, so a count not above its current value
+            // needs no atomic (one address for all owners)
+            if (n > __atomic_load_n(&status[CHX_ST_MAXNB], __ATOMIC_RELAXED)) atomicMax(&status[CHX_ST_MAXNB], n);
+            if (n > CHX_MAX_NB) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_NB);
+        }
+        return;
+    }
+    if (n > CHX_MAX_NB) return;
+    const long off = nbr_off[i];
+    for (int j = tid; j < n; j += 256) {
+        const long long k = keys[j];
+        int rk = 0;
+        for (int q = 0; q < n; q++) rk += keys[q] < k;
+        const int a = (int)(k >> 24);
+        const int spack = (int)(k & 0xffffff);
+        const int sh[3] = {(spack & 255) - 128, ((spack >> 8) & 255) - 128, ((spack >> 16) & 255) - 128};
+        const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
+        chx_entry_n f;
+        nbr_image(C, ni.u, an[a].u, sh, zero, f.v);
+        f.shift = spack;
+        f.pad = 0;
+        const chx_atom_s sa = as[a];
+        chx_entry_s e;
+        if (zero) {
+            e.v[0] = sa.u[0]; e.v[1] = sa.u[1]; e.v[2] = sa.u[2];
+        } else {
+            double t[3] = {sa.u[0], sa.u[1], sa.u[2]};
+            t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
+            double xg[3];
+            mv(C.h, t[0], t[1], t[2], xg);
+            mv(C.ih, xg[0], xg[1], xg[2], e.v);
+        }
+        e.parent = a;
+        e.type = sa.type;
+        es[off + rk] = e;
+        en[off + rk] = f;
+        e_owner[off + rk] = i;
+    }
+}
+
 // Aligned segment of each (model, cluster type) group in the 3- and 4-body
 // lists (thread 0: triplets, thread 1: quadruplets): base[g] = sum over h < g
 // of the group sizes rounded up to CHX_LIST_ALIGN, end[g] = base[g] + size of
@@ -377,14 +554,16 @@ struct EnumShared {
     int cnt[CHX_MAX_CLUSTER_TYPES + 1];
     int wb[CHX_MAX_CLUSTER_TYPES + 1];   // MODE 1: list index of the owner's first record per group
     int map[256];                         // type tuple -> group, for nt <= 4
+    int wc3[4], wc4[4];                   // per-wavefront candidate counts of one compaction pass
     int n3, n4;
 };
 
-template <int MODE, int C3, int C4>
+template <int MODE, int C3, int C4, int NT>
 __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumShared<C3, C4> &sh)
 {
+    static_assert(NT == 64 || NT == 256, "one or four wavefronts per owner");
     constexpr int EW4 = EnumShared<C3, C4>::EW4;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, wl = lane & 63, wv = lane >> 6;
     const int off = x.nbr_off[i];
     const int nb = x.nbr_off[i + 1] - off;
     const chx_atom_s ai = x.as[i];
@@ -398,8 +577,9 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     if (!any3 && !any4) return;
     if (lane == 0) { sh.n3 = 0; sh.n4 = 0; }
     __syncthreads();
-    // compact the images within rc3 / rc4, in sorted order
-    for (int base = 0; base < nb; base += 64) {
+    // compact the images within rc3 / rc4, in sorted order (positions: the
+    // running count, the counts of the lower wavefronts, the lower lanes)
+    for (int base = 0; base < nb; base += NT) {
         const int j = base + lane;
         double d = 1e300;
         chx_entry_n e;
@@ -409,8 +589,18 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         }
         const bool in3 = j < nb && d < rcS, in4 = j < nb && d < rc4;
         const unsigned long long b3 = __ballot(in3), b4 = __ballot(in4);
-        const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-        const int n3b = sh.n3, n4b = sh.n4;
+        const unsigned long long below = wl ? (~0ull >> (64 - wl)) : 0ull;
+        int n3b = sh.n3, n4b = sh.n4, t3 = __popcll(b3), t4 = __popcll(b4);
+        if constexpr (NT > 64) {
+            if (wl == 0) { sh.wc3[wv] = t3; sh.wc4[wv] = t4; }
+            __syncthreads();
+            t3 = t4 = 0;
+            for (int w = 0; w < NT / 64; w++) {
+                if (w < wv) { n3b += sh.wc3[w]; n4b += sh.wc4[w]; }
+                t3 += sh.wc3[w];
+                t4 += sh.wc4[w];
+            }
+        }
         if (in3) {
             const int a = n3b + __popcll(b3 & below);
             if (a < C3) {
@@ -426,7 +616,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
             }
         }
         __syncthreads();
-        if (lane == 0) { sh.n3 = n3b + __popcll(b3); sh.n4 = n4b + __popcll(b4); }
+        if (lane == 0) { sh.n3 += t3; sh.n4 += t4; }
         __syncthreads();
     }
     const int n3 = sh.n3, n4 = sh.n4;
@@ -434,18 +624,18 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         if (lane == 0) atomicOr(&x.status[CHX_ST_FLAGS], CHX_BAD_N34);
         return;
     }
-    for (int u = lane; u < n4; u += 64) {
+    for (int u = lane; u < n4; u += NT) {
         int g = u;
         const int p = sh.par3[sh.idx4[u]];
         while (g > 0 && sh.par3[sh.idx4[g - 1]] == p) g--;
         sh.gst4[u] = g;
         for (int w = 0; w < EW4; w++) sh.A4[u][w] = 0u;
     }
-    for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += 64) sh.cnt[g] = 0;
+    for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += NT) sh.cnt[g] = 0;
     __syncthreads();
     if (any4) {
         const int npair = n4 * (n4 - 1) / 2;
-        for (int p = lane; p < npair; p += 64) {
+        for (int p = lane; p < npair; p += NT) {
             int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
             while (v * (v - 1) / 2 > p) v--;
             while ((v + 1) * v / 2 <= p) v++;
@@ -464,9 +654,9 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     const bool lmap = nt <= 4;
     auto stage = [&](const int *map, int nmap, const int *gb, const int *offv, long gbase, int ngl) {
         if (lmap)
-            for (int t = lane; t < nmap; t += 64) sh.map[t] = map[t];
+            for (int t = lane; t < nmap; t += NT) sh.map[t] = map[t];
         if (MODE == 1)
-            for (int g = lane; g < ngl; g += 64) {
+            for (int g = lane; g < ngl; g += NT) {
                 const long gg = gbase + g;
                 sh.wb[g] = gb[gg] + (offv[gg * N + i] - offv[gg * N]);
             }
@@ -480,7 +670,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         // = (u, v) always, and (v, u) too when both are images of one atom
         // (parent(j) <= parent(k) in both orders); the distance is symmetric
         const int npair = n3 * (n3 - 1) / 2;
-        for (int p = lane; p < npair; p += 64) {
+        for (int p = lane; p < npair; p += NT) {
             int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
             while (v * (v - 1) / 2 > p) v--;
             while ((v + 1) * v / 2 <= p) v++;
@@ -497,7 +687,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
             }
         }
         __syncthreads();
-        for (int g = lane; g < ngl; g += 64) {
+        for (int g = lane; g < ngl; g += NT) {
             if (MODE == 0) x.cnt3[(gbase + g) * N + i] = sh.cnt[g];
             sh.cnt[g] = 0;
         }
@@ -512,7 +702,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         // lanes, as (j, k) = (u, v) and, for two images of one atom, also
         // (v, u); then l over the common neighbours with parent(k) <= parent(l)
         const int npair = n4 * (n4 - 1) / 2;
-        for (int p = lane; p < npair; p += 64) {
+        for (int p = lane; p < npair; p += NT) {
             int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
             while (v * (v - 1) / 2 > p) v--;
             while ((v + 1) * v / 2 <= p) v++;
@@ -547,7 +737,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
             }
         }
         __syncthreads();
-        for (int g = lane; g < ngl; g += 64) {
+        for (int g = lane; g < ngl; g += NT) {
             if (MODE == 0) x.cnt4[(gbase + g) * N + i] = sh.cnt[g];
             sh.cnt[g] = 0;
         }
@@ -555,8 +745,8 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(64) void k_enum(chx_enum_args x)
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_enum(chx_enum_args x)
 {
     __shared__ EnumShared<CHX_ENUM_SMALL, CHX_ENUM_SMALL> sh;
     if (block_stop(x.status, CHX_STOP)) return;
@@ -565,7 +755,7 @@ __global__ __launch_bounds__(64) void k_enum(chx_enum_args x)
         if (MODE == 0 && threadIdx.x == 0) x.big[atomicAdd(&x.status[CHX_ST_NBIG], 1)] = i;
         return;
     }
-    enum_owner<MODE, CHX_ENUM_SMALL, CHX_ENUM_SMALL>(x, i, sh);
+    enum_owner<MODE, CHX_ENUM_SMALL, CHX_ENUM_SMALL, NT>(x, i, sh);
 }
 
 // Owners queued by k_enum<0>, any number of them, over a fixed grid.
@@ -576,7 +766,7 @@ __global__ __launch_bounds__(64) void k_enum_big(chx_enum_args x)
     if (block_stop(x.status, CHX_STOP)) return;
     const int nbig = x.status[CHX_ST_NBIG];
     for (int t = blockIdx.x; t < nbig; t += gridDim.x) {
-        enum_owner<MODE, CHX_MAX_N3, CHX_MAX_N4>(x, x.big[t], sh);
+        enum_owner<MODE, CHX_MAX_N3, CHX_MAX_N4, 64>(x, x.big[t], sh);
         __syncthreads();
     }
 }
@@ -585,8 +775,16 @@ template __global__ void k_nbr<0>(int, const chx_cfg *, const chx_atom_s *, cons
                                   const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, long, int *);
 template __global__ void k_nbr<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
                                   const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, long, int *);
-template __global__ void k_enum<0>(chx_enum_args);
-template __global__ void k_enum<1>(chx_enum_args);
+template __global__ void k_nbr_wide<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *,
+                                       const int *, const int *, int *, const int *, chx_entry_s *, chx_entry_n *,
+                                       int *, long, int *);
+template __global__ void k_nbr_wide<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *,
+                                       const int *, const int *, int *, const int *, chx_entry_s *, chx_entry_n *,
+                                       int *, long, int *);
+template __global__ void k_enum<0, 64>(chx_enum_args);
+template __global__ void k_enum<1, 64>(chx_enum_args);
+template __global__ void k_enum<0, 256>(chx_enum_args);
+template __global__ void k_enum<1, 256>(chx_enum_args);
 template __global__ void k_enum_big<0>(chx_enum_args);
 template __global__ void k_enum_big<1>(chx_enum_args);
 
