@@ -169,6 +169,8 @@ def bench_c1(args):
         if rc:
             raise cc.ChimesError(cc._err())
 
+    # (1) direct evaluations: each call runs the pipeline with HIP events
+    # around the kernel groups (the 4-body kernel's time for the roofline)
     for _ in range(args.warmup):
         step()
     hip.sync()
@@ -185,8 +187,39 @@ def bench_c1(args):
         t4.append(tm[1])
         tall.append(tm[0])
     hip.sync()
-    dt = time.perf_counter() - t0
+    dt_direct = time.perf_counter() - t0
     counts = model.counts()
+    # (2) the same evaluation as a batch plan: autotuned kernels, one captured
+    # graph replayed per step (the repeated-evaluation form: an MD loop)
+    L.chimes_gpu_plan_create_device.restype = C.c_void_p
+    plan = L.chimes_gpu_plan_create_device(arr, 1, 1, model_of, natoms, d_pos, d_typ, cells, 0, prec, 1, d_E, d_F,
+                                           d_V)
+    if not plan:
+        raise cc.ChimesError(cc._err())
+    L.chimes_gpu_plan_run_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    cells_c = cells.ctypes.data_as(C.c_void_p)
+
+    def pstep():
+        if L.chimes_gpu_plan_run_device(plan, cells_c, d_E, d_F, d_V):
+            raise cc.ChimesError(cc._err())
+
+    for _ in range(args.warmup):
+        pstep()
+    hip.sync()
+    tp = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        pstep()
+        tp.append(time.perf_counter() - t1)
+    hip.sync()
+    dt = time.perf_counter() - t0
+    vv, tune_ms, pcounts = (C.c_int * 2)(), (C.c_double * 6)(), (C.c_longlong * 3)()
+    L.chimes_gpu_plan_info.argtypes = [C.c_void_p] * 4
+    L.chimes_gpu_plan_info(plan, vv, tune_ms, pcounts)
+    L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
+    L.chimes_gpu_plan_free(plan)
+    variants = ("mfma64", "mfma32", "vector")
 
     # error of the measured result against the reference's own FP64 values
     E = hip.download(d_E, (1,), np.float64)[0]
@@ -221,7 +254,12 @@ def bench_c1(args):
         "config": {"workload": "O/H 1536 atoms (512 H2O-like, L=24.8 A), 2B+3B+4B orders 12/8/4, "
                                "rc 5.0/4.5/4.0 A, single system, FP64",
                    "natoms": n, "pairs": counts[0], "triplets": counts[1], "quadruplets": counts[2]},
-        "latency_ms": {"p50": float(np.percentile(tall, 50)), "p99": float(np.percentile(tall, 99))},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(tp, 50)), "p99": 1e3 * float(np.percentile(tp, 99))},
+        "forms": {"plan": {"value": n * args.steps / dt, "ms_per_step": dt / args.steps * 1e3,
+                           "variant3": variants[vv[0]], "variant4": variants[vv[1]]},
+                  "direct": {"value": n * args.steps / dt_direct, "ms_per_step": dt_direct / args.steps * 1e3,
+                             "pipeline_ms_p50": float(np.percentile(tall, 50)), "variant3": "vector",
+                             "variant4": "mfma64"}},
         "error_vs_reference_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak,

@@ -309,7 +309,9 @@ struct chx_engine {
     int ngl_max = 1;
     int prec = CHX_FP64;
     // kernel variants of the 3-/4-body evaluation (chosen by the plan autotuner)
-    int var3 = CHX_VAR_MFMA4, var4 = CHX_VAR_MFMA4;
+    // untuned defaults: the vector-unit 3-body kernel and the 64-cluster
+    // matrix-core 4-body kernel (the fastest on the C1-C3 workloads)
+    int var3 = CHX_VAR_VECTOR, var4 = CHX_VAR_MFMA4;
     bool capturing = false;  // inside a graph capture: no event records
     // buffer capacities (entries, list slots); grow on demand
     long ent_cap = 0, l3_cap = 0, l4_cap = 0;
