@@ -617,7 +617,9 @@ void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const ch
                              V / mag3(ab[0], ab[1], ab[2])};
         long cells = 1;
         for (int k = 0; k < 3; k++) {
-            int m = (int)floor(w[k] / (C.search * (1.0 + 1e-6)));
+            // cells at least half the search radius wide: 5 x 5 x 5 searched
+            // cells hold ~2.4x fewer candidates than 3 x 3 x 3 of full width
+            int m = (int)floor(2.0 * w[k] / (C.search * (1.0 + 1e-6)));
             if (m < 1) m = 1;
             if (m > 256) m = 256;
             const double delta = C.search / w[k] * (1.0 + 1e-8) + 1e-12;
