@@ -24,13 +24,14 @@ struct chx_list_args {
 
 __global__ void k_fin(int ncfg, int nitems, const int *cfg_item_off, int natoms_in, const int *cfg_in_off,
                       const chx_cfg *cfgs, const chx_atom_s *as, const chx_model_dev *models,
-                      const unsigned long long *Facc, const unsigned long long *Oacc, long long *part, int *done,
+                      const unsigned long long *Facc, const unsigned long long *Oacc, const int *nbr_cnt,
+                      long long *part, int *done,
                       double *Fout, double *Ecfg, double *Vcfg, int *status);
 
 bool supported_D3(int D);
 bool supported_D4(int D);
-void launch_eval2(long cap, hipStream_t st, const int *e_owner, const chx_cfg *cfgs, const chx_atom_s *as,
-                  const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
+void launch_eval2(long nslot, int nb_cap, hipStream_t st, const int *nbr_cnt, const chx_cfg *cfgs,
+                  const chx_atom_s *as, const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
                   unsigned long long *Oacc, int *cfg_stat, int *status);
 // Vector-unit kernel for nb = 3 or 4 bodies.
 void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t st);

@@ -24,14 +24,12 @@ __global__ void k_fill_f64(double *p, long n);
 __global__ void k_prep(int natoms, int ncfg, const int *cfg_atom_off, const chx_cfg *cfgs, const double *pos,
                        const int *types, chx_atom_s *as, chx_atom_n *an, int *cell_count);
 __global__ void k_cell_fill(int natoms, const chx_atom_n *an, const int *cell_start, int *cursor, int *cell_atoms);
-template <int MODE>
 __global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
-                      const int *cell_start, const int *cell_atoms, int *nbr_cnt, const int *nbr_off,
-                      chx_entry_s *es, chx_entry_n *en, int *e_owner, long ent_cap, int *status);
-template <int MODE>
+                      const int *cell_start, const int *cell_atoms, int *nbr_cnt, int *nbr_off, chx_entry_s *es,
+                      chx_entry_n *en, int nb_cap, int *status);
 __global__ void k_nbr_wide(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
-                           const int *cell_start, const int *cell_atoms, int *nbr_cnt, const int *nbr_off,
-                           chx_entry_s *es, chx_entry_n *en, int *e_owner, long ent_cap, int *status);
+                           const int *cell_start, const int *cell_atoms, int *nbr_cnt, int *nbr_off,
+                           chx_entry_s *es, chx_entry_n *en, int nb_cap, int *status);
 __global__ void k_group_bases(const int *off, long N, int G3, int G4, long cap3, long cap4, int *base3, int *end3,
                               int *base4, int *end4, int *status);
 // arguments of the enumeration kernels
@@ -41,6 +39,7 @@ struct chx_enum_args {
     const chx_atom_s *as;
     const chx_atom_n *an;
     const int *nbr_off;
+    const int *nbr_cnt;
     const chx_entry_s *es;
     const chx_entry_n *en;
     const chx_enum_model *em;

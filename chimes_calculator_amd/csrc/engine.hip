@@ -314,7 +314,8 @@ struct chx_engine {
     int var3 = CHX_VAR_VECTOR, var4 = CHX_VAR_MFMA4;
     bool capturing = false;  // inside a graph capture: no event records
     // buffer capacities (entries, list slots); grow on demand
-    long ent_cap = 0, l3_cap = 0, l4_cap = 0;
+    long l3_cap = 0, l4_cap = 0;
+    int nb_cap = 0;  // neighbour images per owner slab
     // one host-to-device and one device-to-host copy per run
     PinnedBuf in_h, out_h;
     DevBuf<char> in_d, out_d;
@@ -337,7 +338,7 @@ struct chx_engine {
     DevBuf<chx_atom_s> as;
     DevBuf<chx_atom_n> an;
     DevBuf<int> cell_cnt, cell_start, cell_cur, cell_atoms;
-    DevBuf<int> nbr_cnt, nbr_off, e_owner, big;
+    DevBuf<int> nbr_cnt, nbr_off, big;
     DevBuf<chx_entry_s> es;
     DevBuf<chx_entry_n> en;
     DevBuf<int> cnt34, off34;   // per (group, owner) counts and their scan: 3-body groups, then 4-body
@@ -658,12 +659,12 @@ void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const ch
 void guess_capacities(chx_engine *e, chx_model *const *models)
 {
     const double pi = 3.141592653589793;
-    double ent = 0, l3 = 0, l4 = 0;
+    double nbmax = 0, l3 = 0, l4 = 0;
     for (const chx_cfg &C : e->cfg_h) {
         const chx_model *M = models[C.model];
         const double rho = C.natoms / C.vol;
         const double nb = 0.5 * rho * 4.0 / 3.0 * pi * C.search * C.search * C.search;
-        ent += C.natoms * (1.3 * nb + 8);
+        nbmax = std::max(nbmax, 1.4 * nb + 12);
         if (M->d.order[1] > 0) {
             const double n3 = 0.5 * rho * 4.0 / 3.0 * pi * C.rc3 * C.rc3 * C.rc3;
             l3 += C.natoms * (0.4 * n3 * n3 + 4);
@@ -673,7 +674,7 @@ void guess_capacities(chx_engine *e, chx_model *const *models)
             l4 += C.natoms * (0.08 * n4 * n4 * n4 + 4);
         }
     }
-    e->ent_cap = std::max<long>(e->ent_cap, (long)ent + 1024);
+    e->nb_cap = std::max(e->nb_cap, std::min(CHX_MAX_NB, ((int)nbmax + 7) / 8 * 8));
     e->l3_cap = std::max<long>(e->l3_cap, (long)l3 + (long)CHX_LIST_ALIGN * (e->G3 + 1));
     e->l4_cap = std::max<long>(e->l4_cap, (long)l4 + (long)CHX_LIST_ALIGN * (e->G4 + 1));
     e->l3_cap = (e->l3_cap + 255) / 256 * 256;
@@ -843,9 +844,9 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
 // ranges one kernel zeroes at the start of a run).
 void size_work(chx_engine *e)
 {
-    e->es.ensure(e->ent_cap);
-    e->en.ensure(e->ent_cap);
-    e->e_owner.ensure(e->ent_cap);
+    if ((long)e->N * e->nb_cap >= (1L << 31)) throw EngineError("batch too large for the neighbour slabs");
+    e->es.ensure((long)e->N * e->nb_cap);
+    e->en.ensure((long)e->N * e->nb_cap);
     e->list3.ensure(e->l3_cap);
     e->list4.ensure(e->l4_cap);
     const long N = e->N;
@@ -883,22 +884,13 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     // small batches: one owner per workgroup (four wavefronts), else one per wavefront
     const bool wide = N <= CHX_ENUM_WIDE_MAX;
     if (wide)
-        hipLaunchKernelGGL((k_nbr_wide<0>), dim3((unsigned)N), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
-                           e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr,
-                           (chx_entry_s *)nullptr, (chx_entry_n *)nullptr, (int *)nullptr, e->ent_cap, e->status);
-    else
-        hipLaunchKernelGGL((k_nbr<0>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
-                           e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, (const int *)nullptr,
-                           (chx_entry_s *)nullptr, (chx_entry_n *)nullptr, (int *)nullptr, e->ent_cap, e->status);
-    e->scan(e->nbr_cnt.p, e->nbr_off.p, N);
-    if (wide)
-        hipLaunchKernelGGL((k_nbr_wide<1>), dim3((unsigned)N), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
+        hipLaunchKernelGGL(k_nbr_wide, dim3((unsigned)N), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
                            e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
-                           e->e_owner.p, e->ent_cap, e->status);
+                           e->nb_cap, e->status);
     else
-        hipLaunchKernelGGL((k_nbr<1>), dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
+        hipLaunchKernelGGL(k_nbr, dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
                            e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
-                           e->e_owner.p, e->ent_cap, e->status);
+                           e->nb_cap, e->status);
     // 3-/4-body lists in type-aligned segments
     if (G3 + G4 > 0) {
         chx_enum_args ea{};
@@ -907,6 +899,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         ea.as = e->as.p;
         ea.an = e->an.p;
         ea.nbr_off = e->nbr_off.p;
+        ea.nbr_cnt = e->nbr_cnt.p;
         ea.es = e->es.p;
         ea.en = e->en.p;
         ea.em = e->em;
@@ -937,7 +930,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     }
     e->record(1);
     // evaluation with exact accumulation
-    launch_eval2(e->ent_cap, st, e->e_owner.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p, e->Oacc.p,
+    launch_eval2(N * e->nb_cap, e->nb_cap, st, e->nbr_cnt.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p, e->Oacc.p,
                  e->cfg_stat, e->status);
     e->record(2);
     chx_list_args la{};
@@ -973,7 +966,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     e->record(4);
     hipLaunchKernelGGL(k_fin, dim3(e->nitems + nblk(Nin, 256)), dim3(256), 0, st, ncfg, e->nitems,
                        e->cfg_item_off, (int)Nin, e->cfg_in_off, e->cfgs, e->as.p, e->mdev, e->Facc.p, e->Oacc.p,
-                       e->part.p, e->done.p, Fout, Ecfg, Vcfg, e->status);
+                       e->nbr_cnt.p, e->part.p, e->done.p, Fout, Ecfg, Vcfg, e->status);
     HIPCHK(hipMemcpyAsync(e->out_h.p, e->out_d.p, e->out_bytes, hipMemcpyDeviceToHost, st));
     e->record(5);
 }
@@ -999,7 +992,8 @@ bool check_status(chx_engine *e)
         e->counts[2] = e->G4 ? st_h[CHX_ST_C4] : 0;
         return false;
     }
-    if (fl & CHX_RETRY_ENT) e->ent_cap = std::max<long>(e->ent_cap, (long)(st_h[CHX_ST_NENT] * 1.2) + 1024);
+    if (fl & CHX_RETRY_ENT)
+        e->nb_cap = std::max(e->nb_cap, std::min(CHX_MAX_NB, (st_h[CHX_ST_MAXNB] + 8 + 7) / 8 * 8));
     if (fl & CHX_RETRY_L3) e->l3_cap = std::max<long>(e->l3_cap, (long)(st_h[CHX_ST_N3] * 1.2) + 4096);
     if (fl & CHX_RETRY_L4) e->l4_cap = std::max<long>(e->l4_cap, (long)(st_h[CHX_ST_N4] * 1.2) + 4096);
     e->l3_cap = (e->l3_cap + 255) / 256 * 256;
@@ -1094,7 +1088,7 @@ struct chx_plan {
     double *dev_out[3] = {nullptr, nullptr, nullptr};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    long sig[7] = {0, 0, 0, 0, 0, 0, 0};  // N, Nin, ncell, ent_cap, l3_cap, l4_cap, nitems of the captured graph
+    long sig[7] = {0, 0, 0, 0, 0, 0, 0};  // N, Nin, ncell, nb_cap, l3_cap, l4_cap, nitems of the captured graph
     double tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
     std::string err;
     ~chx_plan()
@@ -1152,7 +1146,7 @@ void plan_capture(chx_plan *p, chx_out *o)
     e->capturing = false;
     HIPCHK(hipStreamEndCapture(e->st, &p->graph));
     HIPCHK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
-    const long sig[7] = {e->N, e->Nin, e->ncell, e->ent_cap, e->l3_cap, e->l4_cap, e->nitems};
+    const long sig[7] = {e->N, e->Nin, e->ncell, (long)e->nb_cap, e->l3_cap, e->l4_cap, e->nitems};
     for (int k = 0; k < 7; k++) p->sig[k] = sig[k];
 }
 
@@ -1268,7 +1262,7 @@ int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double 
         plan_batch(p, pos, types, cells, b);
         prepare(e, p->models.data(), (int)p->models.size(), &b);
         layout(e, &b, !o->out_dev);
-        const long sig[7] = {e->N, e->Nin, e->ncell, e->ent_cap, e->l3_cap, e->l4_cap, e->nitems};
+        const long sig[7] = {e->N, e->Nin, e->ncell, (long)e->nb_cap, e->l3_cap, e->l4_cap, e->nitems};
         bool same = p->exec != nullptr;
         for (int k = 0; k < 7; k++) same = same && sig[k] == p->sig[k];
         if (same) {
@@ -1319,18 +1313,22 @@ int chx_export_lists(chx_engine *e, int cfg, long *n2, int **pairs, long *n3, in
         if (cfg < 0 || cfg >= e->ncfg) throw EngineError("configuration index out of range");
         const chx_cfg &C = e->cfg_h[cfg];
         const long N = e->N;
-        std::vector<int> off(C.natoms + 1);
-        HIPCHK(hipMemcpy(off.data(), e->nbr_off.p + C.atom_off, (C.natoms + 1) * sizeof(int), hipMemcpyDeviceToHost));
-        const long e0 = off[0], ne = off[C.natoms] - off[0];
-        std::vector<chx_entry_s> es(ne);
-        std::vector<chx_entry_n> en(ne);
-        if (ne) {
-            HIPCHK(hipMemcpy(es.data(), e->es.p + e0, ne * sizeof(chx_entry_s), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(en.data(), e->en.p + e0, ne * sizeof(chx_entry_n), hipMemcpyDeviceToHost));
+        // the configuration's owners' slabs, and their counts
+        std::vector<int> cnt(C.natoms);
+        HIPCHK(hipMemcpy(cnt.data(), e->nbr_cnt.p + C.atom_off, C.natoms * sizeof(int), hipMemcpyDeviceToHost));
+        const long cap = e->nb_cap, e0 = (long)C.atom_off * cap, nslab = (long)C.natoms * cap;
+        std::vector<long> off(C.natoms + 1, 0);
+        for (int i = 0; i < C.natoms; i++) off[i + 1] = off[i] + cnt[i];
+        const long ne = off[C.natoms];
+        std::vector<chx_entry_s> es(nslab);
+        std::vector<chx_entry_n> en(nslab);
+        if (nslab) {
+            HIPCHK(hipMemcpy(es.data(), e->es.p + e0, nslab * sizeof(chx_entry_s), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(en.data(), e->en.p + e0, nslab * sizeof(chx_entry_n), hipMemcpyDeviceToHost));
         }
         auto key = [&](int owner_local, int code, int *o) {
             if (code == 0) { o[0] = owner_local; o[1] = o[2] = o[3] = 0; return; }
-            const long idx = off[owner_local] - e0 + code - 1;
+            const long idx = (long)owner_local * cap + code - 1;
             o[0] = es[idx].parent - C.atom_off;
             const int s = en[idx].shift;
             o[1] = (s & 255) - 128;
@@ -1340,8 +1338,8 @@ int chx_export_lists(chx_engine *e, int cfg, long *n2, int **pairs, long *n3, in
         *n2 = ne;
         *pairs = (int *)malloc(sizeof(int) * 5 * std::max<long>(ne, 1));
         for (int i = 0; i < C.natoms; i++)
-            for (int s = 1; s <= off[i + 1] - off[i]; s++) {
-                int *o = *pairs + 5 * (off[i] - e0 + s - 1);
+            for (int s = 1; s <= cnt[i]; s++) {
+                int *o = *pairs + 5 * (off[i] + s - 1);
                 o[0] = i;
                 key(i, s, o + 1);
             }

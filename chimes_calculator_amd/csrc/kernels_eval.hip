@@ -69,7 +69,8 @@ __device__ __forceinline__ int find_group(const int *__restrict__ gbase, int G, 
 // virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).  The workgroup's
 // forces are merged per atom in an LDS table before they reach memory.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owner, const chx_cfg *__restrict__ cfgs,
+__global__ __launch_bounds__(256, 3) void k_eval2(long nslot, int nb_cap, const int *__restrict__ nbr_cnt,
+                                                const chx_cfg *__restrict__ cfgs,
                                                 const chx_atom_s *__restrict__ as,
                                                 const chx_entry_s *__restrict__ es,
                                                 const chx_model_dev *__restrict__ models, unsigned long long *Facc,
@@ -79,13 +80,19 @@ __global__ __launch_bounds__(256, 3) void k_eval2(const int *__restrict__ e_owne
     if (block_stop(status, CHX_STOP)) return;
     tab.clear();
     __syncthreads();
-    const long nent = status[CHX_ST_NENT];
+    // entry slot e: owner e / nb_cap, image e % nb_cap of its slab
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int io = 0;
+    bool valid = false;
+    if (e < nslot) {
+        io = (int)(e / nb_cap);
+        valid = (int)(e - (long)io * nb_cap) < nbr_cnt[io];
+    }
     int owner = -1, k1 = -1;
     double q[7] = {0, 0, 0, 0, 0, 0, 0}, F0[3] = {0, 0, 0}, F1[3] = {0, 0, 0};
     bool bad = false;
-    if (e < nent) {
-        const int i = e_owner[e];
+    if (valid) {
+        const int i = io;
         const chx_atom_s ai = as[i];
         const chx_entry_s g = es[e];
         const chx_cfg &C = cfgs[ai.cfg];
@@ -1050,12 +1057,13 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
                                               const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
                                               const chx_model_dev *__restrict__ models,
                                               const unsigned long long *__restrict__ Facc,
-                                              const unsigned long long *__restrict__ Oacc, long long *part,
-                                              int *done, double *Fout, double *Ecfg, double *Vcfg, int *status)
+                                              const unsigned long long *__restrict__ Oacc,
+                                              const int *__restrict__ nbr_cnt, long long *part, int *done,
+                                              double *Fout, double *Ecfg, double *Vcfg, int *status)
 {
     __shared__ long long sh_hi[7][256];
     __shared__ unsigned long long sh_lo[7][256];
-    __shared__ int last;
+    __shared__ int last, nimg;
     if ((int)blockIdx.x >= nitems) {
         const int a = ((int)blockIdx.x - nitems) * 256 + threadIdx.x;
         if (a >= natoms_in) return;
@@ -1093,11 +1101,16 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
                                                                       : (long)C.atom_off + C.natoms;
     fx_sum acc[7];
     bool bad = false;
+    int ni = 0;  // neighbour images of these owners (reported count)
+    if (threadIdx.x == 0) nimg = 0;
     for (long r = r0 + threadIdx.x; r < r1; r += 256) {
 #pragma unroll
         for (int k = 0; k < 7; k++) acc[k].add(Oacc + 2L * (7L * r + k));
         acc[0].add_value(M.eoff[as[r].type], bad);
+        ni += nbr_cnt[r];
     }
+    __syncthreads();
+    if (ni) atomicAdd(&nimg, ni);
 #pragma unroll
     for (int k = 0; k < 7; k++) {
         sh_hi[k][threadIdx.x] = acc[k].hi + (long long)(acc[k].lo >> 40);
@@ -1114,6 +1127,7 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
         __syncthreads();
     }
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
+    if (threadIdx.x == 0 && nimg) atomicAdd(&status[CHX_ST_NENT], nimg);
     const int nit = cfg_item_off[c + 1] - cfg_item_off[c];
     if (threadIdx.x < 7) {
         const int k = threadIdx.x;
@@ -1275,13 +1289,14 @@ __global__ void k_clusters(int n, const double *__restrict__ dist, const double 
 bool supported_D3(int D) { return D >= 2 && D <= 13; }
 bool supported_D4(int D) { return D >= 2 && D <= 6; }
 
-void launch_eval2(long cap, hipStream_t st, const int *e_owner, const chx_cfg *cfgs, const chx_atom_s *as,
+void launch_eval2(long nslot, int nb_cap, hipStream_t st, const int *nbr_cnt, const chx_cfg *cfgs,
+                  const chx_atom_s *as,
                   const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
                   unsigned long long *Oacc, int *cfg_stat, int *status)
 {
-    if (cap <= 0) return;
-    hipLaunchKernelGGL(k_eval2, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, e_owner, cfgs, as, es, models,
-                       Facc, Oacc, cfg_stat, status);
+    if (nslot <= 0) return;
+    hipLaunchKernelGGL(k_eval2, dim3((unsigned)((nslot + 255) / 256)), dim3(256), 0, st, nslot, nb_cap, nbr_cnt, cfgs,
+                       as, es, models, Facc, Oacc, cfg_stat, status);
 }
 
 void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t st)

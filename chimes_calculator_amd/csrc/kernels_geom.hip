@@ -135,31 +135,34 @@ __device__ __forceinline__ bool nbr_image(const chx_cfg &C, const double *ui, co
 // g != i and |r_g - r_i| < search in the neighbour frame
 // (serial_chimes_interface.cpp:682-731).  One wavefront per owner, four
 // owners per workgroup; lanes stride over the atoms of each searched cell.
-// MODE 0 counts.  MODE 1 collects the (parent, shift) keys in LDS, ranks
-// them and writes each owner's images sorted by key, so every list and every
-// later traversal is in one fixed order.
+// The (parent, shift) keys are collected in LDS, ranked, and each owner's
+// images written sorted by key into its slab, so every list and every later
+// traversal is in one fixed order.
 // ---------------------------------------------------------------------------
-template <int MODE>
+// Owner i's image count and slab: entries of owner i live at i * nb_cap
+// (a capacity that grows on demand), so no count pass or scan is needed.
+__device__ __forceinline__ void nbr_slab_note(int i, int n, int nb_cap, int *nbr_cnt, int *nbr_off, int *status)
+{
+    nbr_cnt[i] = n;
+    nbr_off[i] = i * nb_cap;
+    // the word only grows, so a count not above its current value needs no
+    // atomic (one address for all owners)
+    if (n > __atomic_load_n(&status[CHX_ST_MAXNB], __ATOMIC_RELAXED)) atomicMax(&status[CHX_ST_MAXNB], n);
+    if (n > CHX_MAX_NB) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_NB);
+    else if (n > nb_cap) atomicOr(&status[CHX_ST_FLAGS], CHX_RETRY_ENT);
+}
+
 __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restrict__ cfgs,
                                               const chx_atom_s *__restrict__ as, const chx_atom_n *__restrict__ an,
                                               const int *__restrict__ cell_start, const int *__restrict__ cell_atoms,
-                                              int *nbr_cnt, const int *__restrict__ nbr_off, chx_entry_s *es,
-                                              chx_entry_n *en, int *e_owner, long ent_cap, int *status)
+                                              int *nbr_cnt, int *nbr_off, chx_entry_s *es, chx_entry_n *en,
+                                              int nb_cap, int *status)
 {
-    __shared__ long long keys[4][MODE ? CHX_MAX_NB : 1];
+    __shared__ long long keys[4][CHX_MAX_NB];
     // searched cells of each wave's owner: inclusive prefix of their atom
     // counts, first cell_atoms slot, packed image shift
     __shared__ int cpre[4][CHX_NBR_CELLS], cbeg[4][CHX_NBR_CELLS], cshf[4][CHX_NBR_CELLS];
-    if (MODE == 1) {
-        if (block_stop(status, CHX_STOP)) return;
-        // entry total against the entry buffer: every block decides alike
-        const int tot = nbr_off[natoms];
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            status[CHX_ST_NENT] = tot;
-            if (tot > ent_cap) atomicOr(&status[CHX_ST_FLAGS], CHX_RETRY_ENT);
-        }
-        if (tot > ent_cap) return;
-    }
+    if (block_stop(status, CHX_STOP)) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + wv;
     const bool own = i < natoms;
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
             }
         }
         const unsigned long long b = __ballot(in);
-        if (MODE == 1 && in) {
+        if (in) {
             const int idx = n + __popcll(b & below);
             if (idx < CHX_MAX_NB) keys[wv][idx] = ((long long)a << 24) | spack;
         }
@@ -276,18 +279,11 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
                         }
                     }
         }
-        if (MODE == 0 && lane == 0) {
-            nbr_cnt[i] = n;
-            // the word only grows, so a count not above its current value
-            // needs no atomic (one address for all owners)
-            if (n > __atomic_load_n(&status[CHX_ST_MAXNB], __ATOMIC_RELAXED)) atomicMax(&status[CHX_ST_MAXNB], n);
-            if (n > CHX_MAX_NB) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_NB);
-        }
+        if (lane == 0) nbr_slab_note(i, n, nb_cap, nbr_cnt, nbr_off, status);
     }
-    if (MODE == 0) return;
     __syncthreads();
-    if (!own || n > CHX_MAX_NB) return;
-    const long off = nbr_off[i];
+    if (!own || n > CHX_MAX_NB || n > nb_cap) return;
+    const long off = (long)i * nb_cap;
     for (int j = lane; j < n; j += 64) {
         const long long k = keys[wv][j];
         int rk = 0;
@@ -315,34 +311,23 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
         e.type = sa.type;
         es[off + rk] = e;
         en[off + rk] = f;
-        e_owner[off + rk] = i;
     }
 }
 
 // The same lists for small batches: one owner per workgroup, the searched
 // candidates spread over its four wavefronts (latency-bound otherwise).
 // Cells beyond the flattened range are searched by wavefront 0 alone.
-template <int MODE>
 __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__restrict__ cfgs,
                                                    const chx_atom_s *__restrict__ as,
                                                    const chx_atom_n *__restrict__ an,
                                                    const int *__restrict__ cell_start,
-                                                   const int *__restrict__ cell_atoms, int *nbr_cnt,
-                                                   const int *__restrict__ nbr_off, chx_entry_s *es, chx_entry_n *en,
-                                                   int *e_owner, long ent_cap, int *status)
+                                                   const int *__restrict__ cell_atoms, int *nbr_cnt, int *nbr_off,
+                                                   chx_entry_s *es, chx_entry_n *en, int nb_cap, int *status)
 {
-    __shared__ long long keys[MODE ? CHX_MAX_NB : 1];
+    __shared__ long long keys[CHX_MAX_NB];
     __shared__ int cpre[CHX_NBR_CELLS], cbeg[CHX_NBR_CELLS], cshf[CHX_NBR_CELLS];
     __shared__ int ncount;
-    if (MODE == 1) {
-        if (block_stop(status, CHX_STOP)) return;
-        const int tot = nbr_off[natoms];
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            status[CHX_ST_NENT] = tot;
-            if (tot > ent_cap) atomicOr(&status[CHX_ST_FLAGS], CHX_RETRY_ENT);
-        }
-        if (tot > ent_cap) return;
-    }
+    if (block_stop(status, CHX_STOP)) return;
     const int i = blockIdx.x;
     if (i >= natoms) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -382,7 +367,7 @@ __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__r
         int base = 0;
         if (lane == 0) base = atomicAdd(&ncount, __popcll(b));
         base = __shfl(base, 0, 64);
-        if (MODE == 1 && in) {
+        if (in) {
             const int idx = base + __popcll(b & below);
             if (idx < CHX_MAX_NB) keys[idx] = ((long long)a << 24) | spack;
         }
@@ -453,18 +438,9 @@ __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__r
     }
     __syncthreads();
     const int n = ncount;
-    if (MODE == 0) {
-        if (tid == 0) {
-            nbr_cnt[i] = n;
-            // the word only grows, so a count not above its current value
-            // needs no atomic (one address for all owners)
-            if (n > __atomic_load_n(&status[CHX_ST_MAXNB], __ATOMIC_RELAXED)) atomicMax(&status[CHX_ST_MAXNB], n);
-            if (n > CHX_MAX_NB) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_NB);
-        }
-        return;
-    }
-    if (n > CHX_MAX_NB) return;
-    const long off = nbr_off[i];
+    if (tid == 0) nbr_slab_note(i, n, nb_cap, nbr_cnt, nbr_off, status);
+    if (n > CHX_MAX_NB || n > nb_cap) return;
+    const long off = (long)i * nb_cap;
     for (int j = tid; j < n; j += 256) {
         const long long k = keys[j];
         int rk = 0;
@@ -492,7 +468,6 @@ __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__r
         e.type = sa.type;
         es[off + rk] = e;
         en[off + rk] = f;
-        e_owner[off + rk] = i;
     }
 }
 
@@ -565,7 +540,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     constexpr int EW4 = EnumShared<C3, C4>::EW4;
     const int lane = threadIdx.x, wl = lane & 63, wv = lane >> 6;
     const int off = x.nbr_off[i];
-    const int nb = x.nbr_off[i + 1] - off;
+    const int nb = x.nbr_cnt[i];
     const chx_atom_s ai = x.as[i];
     const chx_atom_n ni = x.an[i];
     const chx_cfg &C = x.cfgs[ai.cfg];
@@ -751,7 +726,7 @@ __global__ __launch_bounds__(NT) void k_enum(chx_enum_args x)
     __shared__ EnumShared<CHX_ENUM_SMALL, CHX_ENUM_SMALL> sh;
     if (block_stop(x.status, CHX_STOP)) return;
     const int i = blockIdx.x;
-    if (x.nbr_off[i + 1] - x.nbr_off[i] > CHX_ENUM_SMALL) {
+    if (x.nbr_cnt[i] > CHX_ENUM_SMALL) {
         if (MODE == 0 && threadIdx.x == 0) x.big[atomicAdd(&x.status[CHX_ST_NBIG], 1)] = i;
         return;
     }
@@ -771,16 +746,6 @@ __global__ __launch_bounds__(64) void k_enum_big(chx_enum_args x)
     }
 }
 
-template __global__ void k_nbr<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, long, int *);
-template __global__ void k_nbr<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *, const int *,
-                                  const int *, int *, const int *, chx_entry_s *, chx_entry_n *, int *, long, int *);
-template __global__ void k_nbr_wide<0>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *,
-                                       const int *, const int *, int *, const int *, chx_entry_s *, chx_entry_n *,
-                                       int *, long, int *);
-template __global__ void k_nbr_wide<1>(int, const chx_cfg *, const chx_atom_s *, const chx_atom_n *,
-                                       const int *, const int *, int *, const int *, chx_entry_s *, chx_entry_n *,
-                                       int *, long, int *);
 template __global__ void k_enum<0, 64>(chx_enum_args);
 template __global__ void k_enum<1, 64>(chx_enum_args);
 template __global__ void k_enum<0, 256>(chx_enum_args);
