@@ -16,7 +16,7 @@ struct chx_list_args {
     const int2 *list;
     const chx_cfg *cfgs;
     const chx_atom_s *as;
-    const int *nbr_off;
+    int nb_cap;  // neighbour images per owner slab
     const chx_entry_s *es;
     unsigned long long *Facc, *Oacc;
     int *cfg_stat, *status;

@@ -936,7 +936,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     chx_list_args la{};
     la.cfgs = e->cfgs;
     la.as = e->as.p;
-    la.nbr_off = e->nbr_off.p;
+    la.nb_cap = e->nb_cap;
     la.es = e->es.p;
     la.Facc = e->Facc.p;
     la.Oacc = e->Oacc.p;
@@ -1164,7 +1164,7 @@ void plan_tune(chx_plan *p)
         chx_list_args la{};
         la.cfgs = e->cfgs;
         la.as = e->as.p;
-        la.nbr_off = e->nbr_off.p;
+        la.nb_cap = e->nb_cap;
         la.es = e->es.p;
         la.Facc = e->Facc.p;  // accumulators are overwritten by the next real run
         la.Oacc = e->Oacc.p;

@@ -77,9 +77,12 @@ __global__ __launch_bounds__(256, 3) void k_eval2(long nslot, int nb_cap, const 
                                                 unsigned long long *Oacc, int *cfg_stat, int *status)
 {
     __shared__ fx_table<256> tab;  // neighbour side of the block's 256 entries
-    if (block_stop(status, CHX_STOP)) return;
+    // status read overlapped with the table clear; one barrier for both
+    __shared__ int flags_s;
+    if (threadIdx.x == 0) flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
     tab.clear();
     __syncthreads();
+    if (flags_s & CHX_STOP) return;
     // entry slot e: owner e / nb_cap, image e % nb_cap of its slab
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int io = 0;
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(256, 3) void k_eval2(long nslot, int nb_cap, const 
 // Member positions, types and replicated atom indices of a cluster record.
 template <int NB>
 __device__ __forceinline__ void load_members(const int2 rec, const chx_atom_s *__restrict__ as,
-                                             const int *__restrict__ nbr_off, const chx_entry_s *__restrict__ es,
+                                             int nb_cap, const chx_entry_s *__restrict__ es,
                                              double (&p)[NB][3], int (&t)[NB], int (&par)[NB], int &cfg)
 {
     const int i = rec.x;
@@ -136,7 +139,7 @@ __device__ __forceinline__ void load_members(const int2 rec, const chx_atom_s *_
     p[0][0] = ai.u[0]; p[0][1] = ai.u[1]; p[0][2] = ai.u[2];
     t[0] = ai.type;
     par[0] = i;
-    const int off = nbr_off[i] - 1;
+    const long off = (long)i * nb_cap - 1;  // owner i's slab
 #pragma unroll
     for (int m = 1; m < NB; m++) {
         const int s = (rec.y >> (10 * (m - 1))) & 1023;
@@ -198,14 +201,14 @@ struct pairs_of<4> {
 template <int NB>
 __device__ __forceinline__ bool cluster_geometry(const chx_group &grp, const int2 rec,
                                                  const chx_cfg *__restrict__ cfgs,
-                                                 const chx_atom_s *__restrict__ as, const int *__restrict__ nbr_off,
+                                                 const chx_atom_s *__restrict__ as, int nb_cap,
                                                  const chx_entry_s *__restrict__ es, double (&p)[NB][3],
                                                  int (&par)[NB], double (&r)[pairs_of<NB>::P], int &cfg,
                                                  int (&pm)[NB])
 {
     using PQ = pairs_of<NB>;
     int t[NB];
-    load_members<NB>(rec, as, nbr_off, es, p, t, par, cfg);
+    load_members<NB>(rec, as, nb_cap, es, p, t, par, cfg);
     int tup = 0;
 #pragma unroll
     for (int a = 0; a < NB; a++) tup = tup * grp.nt + t[a];
@@ -260,16 +263,19 @@ template <int NB, int D, typename R>
 __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
                                                 const chx_group *__restrict__ groups, int total_word,
                                                 const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
-                                                const chx_atom_s *__restrict__ as, const int *__restrict__ nbr_off,
+                                                const chx_atom_s *__restrict__ as, int nb_cap,
                                                 const chx_entry_s *__restrict__ es, unsigned long long *Facc,
                                                 unsigned long long *Oacc, int *cfg_stat, int *status)
 {
     using PQ = pairs_of<NB>;
     constexpr int P = PQ::P;
     __shared__ fx_table<(NB == 3 ? 512 : 1024)> tab;  // >= 256 clusters x (NB - 1) non-owner members
-    if (block_stop(status, CHX_STOP)) return;
+    // status read overlapped with the table clear; one barrier for both
+    __shared__ int flags_s;
+    if (threadIdx.x == 0) flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
     tab.clear();
     __syncthreads();
+    if (flags_s & CHX_STOP) return;
     const int lane = threadIdx.x & 63;
     // wave index made explicitly uniform, so the group, its slots and its
     // coefficients are read through the scalar unit
@@ -291,7 +297,7 @@ __global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gb
         const int2 rec = list[c];
         double p[NB][3], r[P];
         int par[NB], cfg, pm[NB];
-        if (cluster_geometry<NB>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, pm)) {
+        if (cluster_geometry<NB>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, pm)) {
             R T[P][D], dT[P][D];
             double f[P], fd[P];
             int flags = 0;
@@ -482,7 +488,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
                                                     const chx_group *__restrict__ groups,
                                                     const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
                                                     const chx_atom_s *__restrict__ as,
-                                                    const int *__restrict__ nbr_off,
+                                                    int nb_cap,
                                                     const chx_entry_s *__restrict__ es, unsigned long long *Facc,
                                                     unsigned long long *Oacc, int *cfg_stat, int *status)
 {
@@ -541,7 +547,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
             int par[4], perm[4];
             if (act) {
                 rec = list[c];
-                act = cluster_geometry<4>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, perm);
+                act = cluster_geometry<4>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, perm);
             }
             if constexpr (!BF) cm.store([&](int x) -> R & { return Cl[LQ::at(x / SH::MP, x % SH::MP)]; });
             double pm[3] = {p[0][0], p[0][1], p[0][2]};
@@ -826,7 +832,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
                                                     const chx_group *__restrict__ groups,
                                                     const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
                                                     const chx_atom_s *__restrict__ as,
-                                                    const int *__restrict__ nbr_off,
+                                                    int nb_cap,
                                                     const chx_entry_s *__restrict__ es, unsigned long long *Facc,
                                                     unsigned long long *Oacc, int *cfg_stat, int *status)
 {
@@ -882,7 +888,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
         int par[3], perm[3];
         if (act) {
             rec = list[c];
-            act = cluster_geometry<3>(grp, rec, cfgs, as, nbr_off, es, p, par, r, cfg, perm);
+            act = cluster_geometry<3>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, perm);
         }
         if constexpr (!BF) cm.store([&](int x) -> R & { return Cl[LM::at(x / SH::MP, x % SH::MP)]; });
         if (act)
@@ -1309,11 +1315,11 @@ void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t 
     if (D == d) {                                                                                              \
         if (f32)                                                                                               \
             hipLaunchKernelGGL((k_evalv<NB_, d, float>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups, \
-                               word, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc, la.cfg_stat,  \
+                               word, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc, la.cfg_stat,  \
                                la.status);                                                                     \
         else                                                                                                   \
             hipLaunchKernelGGL((k_evalv<NB_, d, double>), grid, block, 0, st, la.G, la.gbase, la.gend,          \
-                               la.groups, word, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,  \
+                               la.groups, word, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,  \
                                la.cfg_stat, la.status);                                                        \
         return;                                                                                                \
     }
@@ -1335,19 +1341,19 @@ void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t 
     {                                                                                                          \
         if (prec == CHX_FP64)                                                                                  \
             hipLaunchKernelGGL((KER<d, double, WPB, 0>), grid, block, 0, st, la.G, la.gbase, la.gend,           \
-                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,        \
                                la.cfg_stat, la.status);                                                        \
         else if (prec == CHX_MIXED)                                                                            \
             hipLaunchKernelGGL((KER<d, float, WPB, 0>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
-                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,        \
                                la.cfg_stat, la.status);                                                        \
         else if (prec == CHX_BF16)                                                                             \
             hipLaunchKernelGGL((KER<d, float, WPB, 1>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
-                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,        \
                                la.cfg_stat, la.status);                                                        \
         else                                                                                                   \
             hipLaunchKernelGGL((KER<d, float, WPB, 3>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
-                               la.groups, la.list, la.cfgs, la.as, la.nbr_off, la.es, la.Facc, la.Oacc,        \
+                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,        \
                                la.cfg_stat, la.status);                                                        \
         return true;                                                                                           \
     }
