@@ -74,3 +74,36 @@ def test_per_cluster_c_abi(ref_dir, vectors):
             checked += 1
     print("per-cluster vectors checked:", checked)
     assert checked >= 200
+
+
+
+def test_batched_clusters(ref_dir, vectors):
+    """chimes_gpu_compute_clusters: every golden cluster of a (force field,
+    body count) set in one call, against the reference's compute_nB outputs
+    at the same bars as the single-cluster ABI."""
+    import chimes_calculator_amd as cc
+    from oracle_bindings import Oracle
+    checked = 0
+    for key, rows in sorted(vectors.items()):
+        pf, nb = key.split("|")
+        nb = int(nb)
+        path = os.path.join(ref_dir, pf)
+        names = Oracle(path).type_names()
+        m = cc.GpuModel(path)
+        dist = np.array([row["dx"] for row in rows], dtype=np.float64)
+        dr = np.array([row["dr"] for row in rows], dtype=np.float64)
+        types = np.stack([m.types_to_index([names[t] for t in row["types"]]) for row in rows])
+        E, F, S = m.compute_clusters(nb, dist, dr, types)
+        for k, row in enumerate(rows):
+            e0 = f64(row["energy"])
+            f0 = np.array([f64(v) for v in row["force"]])
+            s0 = np.array([f64(v) for v in row["stress"]])
+            assert abs(E[k] - e0) <= 1e-10 * abs(e0) + 1e-14, (key, k, E[k], e0)
+            fs = max(np.abs(f0).max(), 1e-300)
+            assert np.abs(F[k].reshape(-1) - f0).max() <= 1e-10 * fs + 1e-12, (key, k)
+            ss = max(np.abs(s0).max(), 1e-300)
+            assert np.abs(S[k] - s0).max() <= 1e-10 * ss + 1e-12, (key, k)
+            checked += 1
+        m.close()
+    print("batched cluster vectors checked:", checked)
+    assert checked >= 200
