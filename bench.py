@@ -121,7 +121,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4"], default="C1")
+    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4", "K"], default="C1",
+                    help="K: batched explicit 3-/4-body clusters (chimes_gpu_compute_clusters)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--plan-only", action="store_true",
@@ -139,6 +140,8 @@ def main():
         return bench_c3(args)
     if args.workload == "C4":
         return bench_c4(args)
+    if args.workload == "K":
+        return bench_clusters(args)
     bench_c1(args)
 
 
@@ -342,6 +345,67 @@ def bench_c2(args):
     if plain:
         for tag in forms:
             out["speedup_%s_plan_vs_plain_fp64" % tag] = forms[tag]["value"] / plain
+    print(json.dumps(out))
+
+
+def random_clusters(rng, nb, n, rmin, rmax, ntypes):
+    """n clusters of nb atoms with every pair distance in (rmin, rmax): pair
+    distances (order 01 02 03 12 13 23), pair vectors x_b - x_a, types."""
+    pairs = [(a, b) for a in range(nb) for b in range(a + 1, nb)]
+    out = np.zeros((0, nb, 3))
+    while len(out) < n:
+        x = rng.uniform(-rmax / 2, rmax / 2, (2 * n, nb, 3))
+        d = np.stack([np.linalg.norm(x[:, b] - x[:, a], axis=1) for a, b in pairs], 1)
+        out = np.concatenate([out, x[((d > rmin) & (d < rmax)).all(1)]])[:n]
+    dr = np.stack([out[:, b] - out[:, a] for a, b in pairs], 1)
+    dist = np.linalg.norm(dr, axis=2)
+    return dist, dr.reshape(n, -1), rng.integers(0, ntypes, (n, nb)).astype(np.int32)
+
+
+def bench_clusters(args):
+    """Batched explicit clusters (the per-cluster API over arrays): random
+    water-model 3- and 4-body clusters, host arrays in and out."""
+    import chimes_calculator_amd as cc
+    from chimes_calculator_amd import workloads as W
+
+    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
+    m = W.MODELS["water_2b3b4b"]
+    pf = m.write(os.path.join(ROOT, "_work", "bench_water_2b3b4b.txt"))
+    model = cc.GpuModel(pf)
+    rng = np.random.default_rng(0xC0DE05)
+    n = 1 << 20
+    res = {}
+    for nb, rc in ((3, m.rout[1]), (4, m.rout[2])):
+        dist, dr, types = random_clusters(rng, nb, n, 1.0, rc, 2)
+        for _ in range(args.warmup):
+            model.compute_clusters(nb, dist, dr, types)
+        ts = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            model.compute_clusters(nb, dist, dr, types)
+            ts.append(time.perf_counter() - t0)
+        res["%d-body" % nb] = {"clusters_per_s": n / float(np.median(ts)), "ms_per_call": 1e3 * float(np.median(ts)),
+                              "clusters_per_call": n}
+    out = {"metric": "explicit clusters/s (E+F+stress per cluster, host arrays in and out)",
+           "value": res["4-body"]["clusters_per_s"], "unit": "clusters/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["4-body"]["ms_per_call"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: seeded random O/H clusters, pair distances in (1.0, rc) A",
+           "config": {"workload": "2^20 clusters per call, water model 12/8/4, rc 5.0/4.5/4.0"}, "forms": res}
+    if not args.no_cpu:
+        # the oracle's per-cluster routine, one C call per cluster, on a sample
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bindings import Oracle
+        import ctypes as C
+        o = Oracle(pf)
+        dist, dr, types = random_clusters(rng, 4, 20000, 1.0, m.rout[2], 2)
+        f, st, e = np.zeros(12), np.zeros(9), C.c_double(0.0)
+        t0 = time.perf_counter()
+        for k in range(len(dist)):
+            o.L.oc_compute_4b(o.h, dist[k], dr[k], types[k], f, st, C.byref(e))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": len(dist) / dt, "unit": "clusters/s", "cores": 1, "kind": "port",
+                               "sample": "oracle oc_compute_4b on 20,000 of the same kind of clusters, one call each"}
     print(json.dumps(out))
 
 
