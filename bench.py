@@ -125,6 +125,8 @@ def main():
                     help="K: batched explicit 3-/4-body clusters (chimes_gpu_compute_clusters)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--direct-only", action="store_true",
+                    help="C1: time direct evaluations only (no plan), e.g. for a kernel trace of plain steps")
     ap.add_argument("--plan-only", action="store_true",
                     help="C2: only the batch plan of --precision (no plain per-configuration path), for profiling")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -194,34 +196,37 @@ def bench_c1(args):
     counts = model.counts()
     # (2) the same evaluation as a batch plan: autotuned kernels, one captured
     # graph replayed per step (the repeated-evaluation form: an MD loop)
-    L.chimes_gpu_plan_create_device.restype = C.c_void_p
-    plan = L.chimes_gpu_plan_create_device(arr, 1, 1, model_of, natoms, d_pos, d_typ, cells, 0, prec, 1, d_E, d_F,
-                                           d_V)
-    if not plan:
-        raise cc.ChimesError(cc._err())
-    L.chimes_gpu_plan_run_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-    cells_c = cells.ctypes.data_as(C.c_void_p)
-
-    def pstep():
-        if L.chimes_gpu_plan_run_device(plan, cells_c, d_E, d_F, d_V):
+    if args.direct_only:
+        dt, tp, vv = dt_direct, [1e-3 * x for x in tall], (2, 0)
+    else:
+        L.chimes_gpu_plan_create_device.restype = C.c_void_p
+        plan = L.chimes_gpu_plan_create_device(arr, 1, 1, model_of, natoms, d_pos, d_typ, cells, 0, prec, 1, d_E, d_F,
+                                               d_V)
+        if not plan:
             raise cc.ChimesError(cc._err())
+        L.chimes_gpu_plan_run_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        cells_c = cells.ctypes.data_as(C.c_void_p)
 
-    for _ in range(args.warmup):
-        pstep()
-    hip.sync()
-    tp = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t1 = time.perf_counter()
-        pstep()
-        tp.append(time.perf_counter() - t1)
-    hip.sync()
-    dt = time.perf_counter() - t0
-    vv, tune_ms, pcounts = (C.c_int * 2)(), (C.c_double * 6)(), (C.c_longlong * 3)()
-    L.chimes_gpu_plan_info.argtypes = [C.c_void_p] * 4
-    L.chimes_gpu_plan_info(plan, vv, tune_ms, pcounts)
-    L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
-    L.chimes_gpu_plan_free(plan)
+        def pstep():
+            if L.chimes_gpu_plan_run_device(plan, cells_c, d_E, d_F, d_V):
+                raise cc.ChimesError(cc._err())
+
+        for _ in range(args.warmup):
+            pstep()
+        hip.sync()
+        tp = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            pstep()
+            tp.append(time.perf_counter() - t1)
+        hip.sync()
+        dt = time.perf_counter() - t0
+        vv, tune_ms, pcounts = (C.c_int * 2)(), (C.c_double * 6)(), (C.c_longlong * 3)()
+        L.chimes_gpu_plan_info.argtypes = [C.c_void_p] * 4
+        L.chimes_gpu_plan_info(plan, vv, tune_ms, pcounts)
+        L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
+        L.chimes_gpu_plan_free(plan)
     variants = ("mfma64", "mfma32", "vector")
 
     # error of the measured result against the reference's own FP64 values
@@ -258,8 +263,9 @@ def bench_c1(args):
                                "rc 5.0/4.5/4.0 A, single system, FP64",
                    "natoms": n, "pairs": counts[0], "triplets": counts[1], "quadruplets": counts[2]},
         "latency_ms": {"p50": 1e3 * float(np.percentile(tp, 50)), "p99": 1e3 * float(np.percentile(tp, 99))},
-        "forms": {"plan": {"value": n * args.steps / dt, "ms_per_step": dt / args.steps * 1e3,
-                           "variant3": variants[vv[0]], "variant4": variants[vv[1]]},
+        "forms": {"plan" if not args.direct_only else "direct (timed)": {
+                      "value": n * args.steps / dt, "ms_per_step": dt / args.steps * 1e3,
+                      "variant3": variants[vv[0]], "variant4": variants[vv[1]]},
                   "direct": {"value": n * args.steps / dt_direct, "ms_per_step": dt_direct / args.steps * 1e3,
                              "pipeline_ms_p50": float(np.percentile(tall, 50)), "variant3": "vector",
                              "variant4": "mfma64"}},
