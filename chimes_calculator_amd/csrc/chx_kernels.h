@@ -30,8 +30,16 @@ __global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, con
 __global__ void k_nbr_wide(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
                            const int *cell_start, const int *cell_atoms, int *nbr_cnt, int *nbr_off,
                            chx_entry_s *es, chx_entry_n *en, int nb_cap, int *status);
-__global__ void k_group_bases(const int *off, long N, int G3, int G4, long cap3, long cap4, int *base3, int *end3,
-                              int *base4, int *end4, int *status);
+// Aligned group segments from the scan of the per-(group, owner) counts.
+struct chx_bases_args {
+    const int *off;  // the scan ([G3 + G4][N] + total); null: no bases
+    long N;
+    int G3, G4;
+    long cap3, cap4;
+    int *base3, *end3, *base4, *end4;
+    int *status;
+};
+__global__ void k_group_bases(chx_bases_args a);
 // arguments of the enumeration kernels
 struct chx_enum_args {
     int natoms;
@@ -48,6 +56,7 @@ struct chx_enum_args {
     const int *gb3, *gb4;          // MODE 1: aligned group bases
     int2 *list3, *list4;           // MODE 1 output
     int *big;                      // [natoms]: owners queued for k_enum_big
+    int big_on;                    // k_enum_big runs in this pipeline (else a queued owner asks for a retry)
     int *status;
 };
 template <int MODE, int NT>
@@ -56,7 +65,7 @@ template <int MODE>
 __global__ void k_enum_big(chx_enum_args x);
 __global__ void k_scan_block(const int *in, int *out, long n, int *bsum);
 __global__ void k_scan_add(int *out, long n, const int *bscan);
-__global__ void k_scan_one(const int *in, int *out, long n);
+__global__ void k_scan_one(const int *in, int *out, long n, chx_bases_args gb);
 __global__ void k_clear(const chx_clear_range *r, int nr);
 __global__ void k_scan_total(int *out, long n, const int *in);
 

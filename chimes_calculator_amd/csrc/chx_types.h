@@ -34,7 +34,8 @@ enum {
     CHX_RETRY_L3 = 8,          // triplet list buffer too small
     CHX_RETRY_L4 = 16,         // quadruplet list buffer too small
     CHX_BAD_RANGE = 32,        // a contribution outside the fixed-point range
-    CHX_RETRY_ANY = CHX_RETRY_ENT | CHX_RETRY_L3 | CHX_RETRY_L4,
+    CHX_RETRY_BIG = 64,        // an owner needs the crowded-owner enumeration, which was not launched
+    CHX_RETRY_ANY = CHX_RETRY_ENT | CHX_RETRY_L3 | CHX_RETRY_L4 | CHX_RETRY_BIG,
     CHX_STOP = CHX_BAD_NB | CHX_BAD_N34 | CHX_RETRY_ANY
 };
 

@@ -316,6 +316,7 @@ struct chx_engine {
     // buffer capacities (entries, list slots); grow on demand
     long l3_cap = 0, l4_cap = 0;
     int nb_cap = 0;  // neighbour images per owner slab
+    bool big_on = false;  // launch the crowded-owner enumeration (set once an owner needed it)
     // one host-to-device and one device-to-host copy per run
     PinnedBuf in_h, out_h;
     DevBuf<char> in_d, out_d;
@@ -352,12 +353,15 @@ struct chx_engine {
     DevBuf<double> cl_in, cl_out;
     DevBuf<int> cl_types, cl_flags;
 
-    void scan(const int *in, int *out, long n, int level = 0)
+    // Exclusive scan of n ints (out[n] = total).  With gb.off set (to out),
+    // the group bases are computed too: in the scan's own launch when it is a
+    // single workgroup, else by a k_group_bases launch after it.
+    void scan(const int *in, int *out, long n, int level = 0, chx_bases_args gb = chx_bases_args{})
     {
         if (scan_tmp.size() < 32) scan_tmp.resize(32);  // fixed: references below stay valid
         if (level >= 15) throw EngineError("scan too deep");
         if (level == 0 && n <= CHX_SCAN1_MAX) {
-            hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, st, in, out, n);
+            hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, st, in, out, n, gb);
             return;
         }
         const long nb = (n + 2047) / 2048;
@@ -371,6 +375,7 @@ struct chx_engine {
             hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, st, out, n, bscan.p);
         }
         hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, out, n, in);
+        if (gb.off) hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(64), 0, st, gb);
     }
     void record(int k)
     {
@@ -912,21 +917,34 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         ea.list3 = e->list3.p;
         ea.list4 = e->list4.p;
         ea.big = e->big.p;
+        ea.big_on = e->big_on;
         ea.status = e->status;
-        // owners with many images are queued by k_enum<0> for the large-LDS kernel
+        // owners with many images are queued by k_enum<0> for the large-LDS
+        // kernel, which is launched once a batch has needed it (until then a
+        // queued owner asks for a retry)
         const dim3 gbig(1280);
         // small batches: four wavefronts share an owner's enumeration (latency)
         if (wide) hipLaunchKernelGGL((k_enum<0, 256>), dim3((unsigned)N), dim3(256), 0, st, ea);
         else hipLaunchKernelGGL((k_enum<0, 64>), dim3((unsigned)N), dim3(64), 0, st, ea);
-        hipLaunchKernelGGL((k_enum_big<0>), gbig, dim3(64), 0, st, ea);
+        if (e->big_on) hipLaunchKernelGGL((k_enum_big<0>), gbig, dim3(64), 0, st, ea);
         // one scan over both bodies' counts: segments and positions use
         // differences of the scan only
-        e->scan(e->cnt34.p, e->off34.p, (long)(G3 + G4) * N);
-        hipLaunchKernelGGL(k_group_bases, dim3(1), dim3(64), 0, st, e->off34.p, N, G3, G4, e->l3_cap, e->l4_cap,
-                           e->gb3.p, e->ge3.p, e->gb4.p, e->ge4.p, e->status);
+        chx_bases_args gb{};
+        gb.off = e->off34.p;
+        gb.N = N;
+        gb.G3 = G3;
+        gb.G4 = G4;
+        gb.cap3 = e->l3_cap;
+        gb.cap4 = e->l4_cap;
+        gb.base3 = e->gb3.p;
+        gb.end3 = e->ge3.p;
+        gb.base4 = e->gb4.p;
+        gb.end4 = e->ge4.p;
+        gb.status = e->status;
+        e->scan(e->cnt34.p, e->off34.p, (long)(G3 + G4) * N, 0, gb);
         if (wide) hipLaunchKernelGGL((k_enum<1, 256>), dim3((unsigned)N), dim3(256), 0, st, ea);
         else hipLaunchKernelGGL((k_enum<1, 64>), dim3((unsigned)N), dim3(64), 0, st, ea);
-        hipLaunchKernelGGL((k_enum_big<1>), gbig, dim3(64), 0, st, ea);
+        if (e->big_on) hipLaunchKernelGGL((k_enum_big<1>), gbig, dim3(64), 0, st, ea);
     }
     e->record(1);
     // evaluation with exact accumulation
@@ -992,6 +1010,7 @@ bool check_status(chx_engine *e)
         e->counts[2] = e->G4 ? st_h[CHX_ST_C4] : 0;
         return false;
     }
+    if (fl & CHX_RETRY_BIG) e->big_on = true;
     if (fl & CHX_RETRY_ENT)
         e->nb_cap = std::max(e->nb_cap, std::min(CHX_MAX_NB, (st_h[CHX_ST_MAXNB] + 8 + 7) / 8 * 8));
     if (fl & CHX_RETRY_L3) e->l3_cap = std::max<long>(e->l3_cap, (long)(st_h[CHX_ST_N3] * 1.2) + 4096);

@@ -476,11 +476,12 @@ __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__r
 // of the group sizes rounded up to CHX_LIST_ALIGN, end[g] = base[g] + size of
 // g.  off is the exclusive scan of the per-(group, owner) counts laid out
 // [G3 + G4][N]; a group's size is a difference of two scan values.
-__global__ void k_group_bases(const int *__restrict__ off, long N, int G3, int G4, long cap3, long cap4,
-                              int *base3, int *end3, int *base4, int *end4, int *status)
+// Thread 0: triplet groups, thread 1: quadruplet groups.
+__device__ __forceinline__ void group_bases_of(const int *__restrict__ off, long N, int G3, int G4, long cap3,
+                                               long cap4, int *base3, int *end3, int *base4, int *end4, int *status,
+                                               int t)
 {
-    if (blockIdx.x || threadIdx.x > 1) return;
-    const bool four = threadIdx.x == 1;
+    const bool four = t == 1;
     const int G = four ? G4 : G3;
     if (G == 0) return;
     const int *o = four ? off + (long)G3 * N : off;
@@ -498,6 +499,12 @@ __global__ void k_group_bases(const int *__restrict__ off, long N, int G3, int G
     base[G] = (int)b;
     status[word] = (int)b;
     if (b > cap || b >= (1L << 31) - CHX_LIST_ALIGN) atomicOr(&status[CHX_ST_FLAGS], four ? CHX_RETRY_L4 : CHX_RETRY_L3);
+}
+
+__global__ void k_group_bases(chx_bases_args a)
+{
+    if (blockIdx.x || threadIdx.x > 1) return;
+    group_bases_of(a.off, a.N, a.G3, a.G4, a.cap3, a.cap4, a.base3, a.end3, a.base4, a.end4, a.status, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -727,7 +734,10 @@ __global__ __launch_bounds__(NT) void k_enum(chx_enum_args x)
     if (block_stop(x.status, CHX_STOP)) return;
     const int i = blockIdx.x;
     if (x.nbr_cnt[i] > CHX_ENUM_SMALL) {
-        if (MODE == 0 && threadIdx.x == 0) x.big[atomicAdd(&x.status[CHX_ST_NBIG], 1)] = i;
+        if (MODE == 0 && threadIdx.x == 0) {
+            x.big[atomicAdd(&x.status[CHX_ST_NBIG], 1)] = i;
+            if (!x.big_on) atomicOr(&x.status[CHX_ST_FLAGS], CHX_RETRY_BIG);
+        }
         return;
     }
     enum_owner<MODE, CHX_ENUM_SMALL, CHX_ENUM_SMALL, NT>(x, i, sh);
@@ -785,7 +795,7 @@ __global__ __launch_bounds__(256) void k_scan_block(const int *in, int *out, lon
 
 // Exclusive scan of a short array (n <= CHX_SCAN1_MAX) in one workgroup of
 // 1024 threads, each summing a contiguous chunk; out[n] = total.
-__global__ __launch_bounds__(1024) void k_scan_one(const int *in, int *out, long n)
+__global__ __launch_bounds__(1024) void k_scan_one(const int *in, int *out, long n, chx_bases_args gb)
 {
     constexpr int PER = CHX_SCAN1_MAX / 1024;
     static_assert(PER % 4 == 0, "segments of whole int4");
@@ -850,6 +860,12 @@ __global__ __launch_bounds__(1024) void k_scan_one(const int *in, int *out, long
         }
     }
     if (tid == 1023) out[n] = wsum[15];
+    // the group bases of the 3-/4-body count scan, in the same launch
+    if (gb.off == nullptr) return;
+    __syncthreads();
+    if (tid < 2)
+        group_bases_of(out, gb.N, gb.G3, gb.G4, gb.cap3, gb.cap4, gb.base3, gb.end3, gb.base4, gb.end4, gb.status,
+                       tid);
 }
 
 // Zeroes every listed range (one launch for all counters and accumulators).

@@ -142,3 +142,28 @@ def test_tiled_extensivity(cc, model_dir, synth_cases):
     assert abs(E[0] - 64 * e0) <= 1e-10 * abs(64 * e0), (E[0], 64 * e0)
     assert_close("tiled forces", F[0], np.tile(f0, (64, 1)), 1e-10, atol=FATOL)
     assert_close("tiled stress", S[0], s0, 1e-10)
+
+
+def test_clustered_capacity_retry(cc, model_dir):
+    """64 carbon atoms packed in an 8 A corner of a 20 A box: the capacity
+    estimates from the mean density fall short (neighbour slabs and list
+    segments), the pipeline reruns with the sizes it measured, and the result
+    equals the oracle's."""
+    from chimes_calculator_amd import workloads as W
+    from oracle_bindings import Oracle
+    rng = np.random.default_rng(0xC0DE06)
+    pos = []
+    while len(pos) < 64:
+        p = rng.uniform(0.0, 8.0, 3)
+        if all(np.linalg.norm(p - q) >= 1.2 for q in pos):
+            pos.append(p)
+    pos = np.array(pos) + 2.0
+    cell = np.diag([20.0, 20.0, 20.0])
+    types = ["C"] * 64
+    pf = os.path.join(model_dir, "carbon_2b3b.txt")
+    m = cc.GpuModel(pf)
+    E, F, S = cc.eval_batch(m, [(pos, types, cell)])
+    r = Oracle(pf).calculate(pos, types, cell)
+    assert abs(E[0] - r.energy) <= RTOL * abs(r.energy), (E[0], r.energy)
+    assert_close("clustered forces", F[0], r.forces, RTOL, atol=FATOL)
+    assert_close("clustered stress", S[0], r.stress, RTOL)
