@@ -239,6 +239,29 @@ struct fx_table {
             if (lo) atomicAdd(&val[s][2 * k + 1], (unsigned long long)lo);
         }
     }
+    // as add, but when every slot is taken by other atoms the contribution
+    // goes straight to the global accumulators acc[(3*atom + k)*2 + {0,1}]
+    __device__ __forceinline__ void add_or_global(int atom, const double (&v)[3], unsigned long long *acc, bool &bad)
+    {
+        if (atom < 0) return;
+        unsigned h = ((unsigned)atom * 2654435761u) >> 8;
+        int s = (int)(h & (CAP - 1));
+        bool found = false;
+        for (int probe = 0; probe < CAP; probe++) {
+            const int prev = atomicCAS(&key[s], -1, atom);
+            if (prev == -1 || prev == atom) { found = true; break; }
+            s = (s + 1) & (CAP - 1);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (v[k] == 0.0) continue;
+            if (!found) { fx_atomic(acc + 2L * (3L * atom + k), v[k], bad); continue; }
+            long long hi, lo;
+            fx_split(v[k], hi, lo, bad);
+            if (hi) atomicAdd(&val[s][2 * k], (unsigned long long)hi);
+            if (lo) atomicAdd(&val[s][2 * k + 1], (unsigned long long)lo);
+        }
+    }
     // all threads of the block, after a barrier
     __device__ __forceinline__ void flush(unsigned long long *acc)
     {

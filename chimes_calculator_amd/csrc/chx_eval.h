@@ -30,7 +30,7 @@ __global__ void k_fin(int ncfg, int nitems, const int *cfg_item_off, int natoms_
 
 bool supported_D3(int D);
 bool supported_D4(int D);
-void launch_eval2(long nslot, int nb_cap, hipStream_t st, const int *nbr_cnt, const chx_cfg *cfgs,
+void launch_eval2(int natoms, int nb_cap, hipStream_t st, const int *nbr_cnt, const chx_cfg *cfgs,
                   const chx_atom_s *as, const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
                   unsigned long long *Oacc, int *cfg_stat, int *status);
 // Vector-unit kernel for nb = 3 or 4 bodies.

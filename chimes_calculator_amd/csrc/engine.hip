@@ -948,7 +948,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     }
     e->record(1);
     // evaluation with exact accumulation
-    launch_eval2(N * e->nb_cap, e->nb_cap, st, e->nbr_cnt.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p, e->Oacc.p,
+    launch_eval2((int)N, e->nb_cap, st, e->nbr_cnt.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p, e->Oacc.p,
                  e->cfg_stat, e->status);
     e->record(2);
     chx_list_args la{};

@@ -65,63 +65,73 @@ __device__ __forceinline__ int find_group(const int *__restrict__ gbase, int G, 
 }
 
 // ---------------------------------------------------------------------------
-// 2-body: one thread per neighbour entry.  F_owner += f dr, F_image -= f dr,
-// virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).  The workgroup's
-// forces are merged per atom in an LDS table before they reach memory.
+// 2-body: sixteen lanes per owner atom over its image slab (four owners per
+// wavefront, sixteen per workgroup).  F_owner += f dr, F_image -= f dr,
+// virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).  The owner's
+// force, energy and virial are summed in double in a fixed order (lane l of
+// the sixteen takes images l, l + 16, ... of the slab, which is sorted by
+// image key, and a fixed butterfly adds the sixteen lanes), so the sum
+// depends only on the owner's own images, never on the batch around it, and
+// is then added exactly.  Image forces are merged in the workgroup's LDS
+// table.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256, 3) void k_eval2(long nslot, int nb_cap, const int *__restrict__ nbr_cnt,
-                                                const chx_cfg *__restrict__ cfgs,
-                                                const chx_atom_s *__restrict__ as,
-                                                const chx_entry_s *__restrict__ es,
-                                                const chx_model_dev *__restrict__ models, unsigned long long *Facc,
-                                                unsigned long long *Oacc, int *cfg_stat, int *status)
+__global__ __launch_bounds__(256, 3) void k_eval2(int natoms, int nb_cap, const int *__restrict__ nbr_cnt,
+                                               const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
+                                               const chx_entry_s *__restrict__ es,
+                                               const chx_model_dev *__restrict__ models, unsigned long long *Facc,
+                                               unsigned long long *Oacc, int *cfg_stat, int *status)
 {
-    __shared__ fx_table<256> tab;  // neighbour side of the block's 256 entries
+    __shared__ fx_table<512> tab;  // image side of the workgroup's sixteen owners
     // status read overlapped with the table clear; one barrier for both
     __shared__ int flags_s;
     if (threadIdx.x == 0) flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
     tab.clear();
     __syncthreads();
     if (flags_s & CHX_STOP) return;
-    // entry slot e: owner e / nb_cap, image e % nb_cap of its slab
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int io = 0;
-    bool valid = false;
-    if (e < nslot) {
-        io = (int)(e / nb_cap);
-        valid = (int)(e - (long)io * nb_cap) < nbr_cnt[io];
-    }
-    int owner = -1, k1 = -1;
-    double q[7] = {0, 0, 0, 0, 0, 0, 0}, F0[3] = {0, 0, 0}, F1[3] = {0, 0, 0};
+    const int sub = threadIdx.x & 15;
+    const int i = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // owner force, energy, virial
     bool bad = false;
-    if (valid) {
-        const int i = io;
+    if (i < natoms) {
+        const int n = nbr_cnt[i];
         const chx_atom_s ai = as[i];
-        const chx_entry_s g = es[e];
         const chx_cfg &C = cfgs[ai.cfg];
         const chx_model_dev &M = models[C.model];
-        double dr[3];
-        const double r = pair_vec(C.h, ai.u, g.v, dr);
-        const int pt = M.pairmap[ai.type * M.nt + g.type];
-        double E, fs;
         int flags = 0;
-        eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
-                  M.pen_dist, M.pen_scale, E, fs, flags);
-        F0[0] = fs * dr[0]; F0[1] = fs * dr[1]; F0[2] = fs * dr[2];
-        F1[0] = -F0[0]; F1[1] = -F0[1]; F1[2] = -F0[2];
-        k1 = g.parent;
-        owner = i;
-        q[0] = E;
-        q[1] = -(fs * (dr[0] * dr[0]));
-        q[2] = -(fs * (dr[0] * dr[1]));
-        q[3] = -(fs * (dr[0] * dr[2]));
-        q[4] = -(fs * (dr[1] * dr[1]));
-        q[5] = -(fs * (dr[1] * dr[2]));
-        q[6] = -(fs * (dr[2] * dr[2]));
+        for (int j = sub; j < n; j += 16) {
+            const chx_entry_s g = es[(long)i * nb_cap + j];
+            double dr[3];
+            const double r = pair_vec(C.h, ai.u, g.v, dr);
+            const int pt = M.pairmap[ai.type * M.nt + g.type];
+            double E, fs;
+            eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
+                      M.pen_dist, M.pen_scale, E, fs, flags);
+            const double F0[3] = {fs * dr[0], fs * dr[1], fs * dr[2]};
+            const double F1[3] = {-F0[0], -F0[1], -F0[2]};
+            tab.add_or_global(g.parent, F1, Facc, bad);
+            acc[0] += F0[0];
+            acc[1] += F0[1];
+            acc[2] += F0[2];
+            acc[3] += E;
+            acc[4] -= fs * (dr[0] * dr[0]);
+            acc[5] -= fs * (dr[0] * dr[1]);
+            acc[6] -= fs * (dr[0] * dr[2]);
+            acc[7] -= fs * (dr[1] * dr[1]);
+            acc[8] -= fs * (dr[1] * dr[2]);
+            acc[9] -= fs * (dr[2] * dr[2]);
+        }
         note_flags(cfg_stat, ai.cfg, flags);
+#pragma unroll
+        for (int w = 8; w >= 1; w >>= 1)
+#pragma unroll
+            for (int k = 0; k < 10; k++) acc[k] += __shfl_xor(acc[k], w, 64);
+        if (sub == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) fx_atomic(Facc + 2L * (3L * i + k), acc[k], bad);
+#pragma unroll
+            for (int k = 0; k < 7; k++) fx_atomic(Oacc + 2L * (7L * i + k), acc[3 + k], bad);
+        }
     }
-    tab.add(k1, F1, bad);
-    fx_owner_runs<64>(Facc, Oacc, owner, F0, q, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -1295,13 +1305,12 @@ __global__ void k_clusters(int n, const double *__restrict__ dist, const double 
 bool supported_D3(int D) { return D >= 2 && D <= 13; }
 bool supported_D4(int D) { return D >= 2 && D <= 6; }
 
-void launch_eval2(long nslot, int nb_cap, hipStream_t st, const int *nbr_cnt, const chx_cfg *cfgs,
-                  const chx_atom_s *as,
-                  const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
+void launch_eval2(int natoms, int nb_cap, hipStream_t st, const int *nbr_cnt, const chx_cfg *cfgs,
+                  const chx_atom_s *as, const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
                   unsigned long long *Oacc, int *cfg_stat, int *status)
 {
-    if (nslot <= 0) return;
-    hipLaunchKernelGGL(k_eval2, dim3((unsigned)((nslot + 255) / 256)), dim3(256), 0, st, nslot, nb_cap, nbr_cnt, cfgs,
+    if (natoms <= 0) return;
+    hipLaunchKernelGGL(k_eval2, dim3((unsigned)((natoms + 15) / 16)), dim3(256), 0, st, natoms, nb_cap, nbr_cnt, cfgs,
                        as, es, models, Facc, Oacc, cfg_stat, status);
 }
 
