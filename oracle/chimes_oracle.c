@@ -734,6 +734,14 @@ void oc_fcut(const oc_params *P, double dx, double rc, double *fcut, double *fcu
 /* ------------------------------------------------------------------------ */
 
 /* compute_2B (chimesFF.cpp:1657-1775) incl. get_penalty (chimesFF.h:388-419) */
+/* chimesFF::compute_1B (chimesFF.cpp:1636-1641) */
+void oc_compute_1b(const oc_params *P, int t, double *energy) { *energy += P->eoff[t]; }
+
+double oc_mass(const oc_params *P, int t) { return P->mass[t]; }
+
+/* Outer 2-body cutoff of a type pair (chimesFF::get_cutoff_2B, chimesFF.cpp:2886-2901). */
+double oc_cutoff_2b(const oc_params *P, int ti, int tj) { return P->rout[P->pairmap[ti * P->nt + tj]]; }
+
 void oc_compute_2b(const oc_params *P, double dx, const double dr[3], const int t[2],
                    double force[6], double stress[6], double *energy)
 {

@@ -57,6 +57,13 @@ int oc_calculate(const oc_params *P, int natom, const double *x, const double *y
 
 void oc_lists_free(oc_lists *L);
 
+/* 1-body energy offset of type t, added onto *energy (chimesFF.cpp:1636-1641). */
+void oc_compute_1b(const oc_params *P, int t, double *energy);
+/* Mass of type t as read from the parameter file (chimesFF.cpp:354-1570). */
+double oc_mass(const oc_params *P, int t);
+/* Outer 2-body cutoff of a type pair (chimesFF.cpp:2886-2901). */
+double oc_cutoff_2b(const oc_params *P, int ti, int tj);
+
 /* Single-cluster evaluators (chimesFF.cpp:1657-1775, 1944-2176, 2509-2872).
  * Type arguments are type indices.  force / stress are packed as in the
  * reference (force[natoms*3], stress[6]) and are accumulated onto. */
@@ -66,6 +73,19 @@ void oc_compute_3b(const oc_params *P, const double dx[3], const double dr[9],
                    const int t[3], double force[9], double stress[6], double *energy);
 void oc_compute_4b(const oc_params *P, const double dx[6], const double dr[18],
                    const int t[4], double force[12], double stress[6], double *energy);
+
+/* LAMMPS pair style on one process (chimes_lmp_oracle.c): many-body lists
+ * from a full neighbour list with ghosts (pair_chimes.cpp:321-478; trips as
+ * (i, j, k), quads as (i, j, k, l), free with oc_lmp_free) and the compute
+ * loops with ev_tally_mb (pair_chimes.cpp:480-831, pair.cpp:1425-1482). */
+int oc_lmp_build_lists(const oc_params *P, const double *x, const int *tag, int inum, const int *ilist,
+                       const int *numneigh, const int *nbr_off, const int *nbr, double skin, long *n3, int **trips,
+                       long *n4, int **quads);
+void oc_lmp_compute(const oc_params *P, const double *x, const int *ctype, const int *tag, int inum,
+                    const int *ilist, const int *numneigh, const int *nbr_off, const int *nbr, long n3,
+                    const int *trips, long n4, const int *quads, int eflag_global, int eflag_atom, int vflag_global,
+                    int vflag_atom, double *f, double *eng, double *virial, double *eatom, double *vatom);
+void oc_lmp_free(void *p);
 
 /* Chebyshev polynomials of the Morse-transformed distance and the cutoff
  * function, exposed for unit tests (chimesFF.h:347-386, 542-628,

@@ -67,6 +67,18 @@ def oracle_lib():
                               C.POINTER(C.c_double)]
         L.oc_last_penalty_count.restype = C.c_long
         L.oc_last_out_of_range_count.restype = C.c_long
+        L.oc_mass.restype = C.c_double
+        L.oc_mass.argtypes = [C.c_void_p, C.c_int]
+        L.oc_cutoff_2b.restype = C.c_double
+        L.oc_cutoff_2b.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        _lp = C.POINTER(C.c_long)
+        _ipp = C.POINTER(C.POINTER(C.c_int))
+        L.oc_lmp_build_lists.argtypes = [C.c_void_p, _dp, _ip, C.c_int, _ip, _ip, _ip, _ip, C.c_double,
+                                         _lp, _ipp, _lp, _ipp]
+        L.oc_lmp_compute.argtypes = [C.c_void_p, _dp, _ip, _ip, C.c_int, _ip, _ip, _ip, _ip, C.c_long, _ip,
+                                     C.c_long, _ip, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
+                                     C.POINTER(C.c_double), _dp, _dp, _dp]
+        L.oc_lmp_free.argtypes = [C.c_void_p]
         _oracle = L
     return _oracle
 
@@ -127,6 +139,49 @@ class Oracle:
             res.meta = (L.n_atoms, L.n_layers, L.n_rep)
             self.L.oc_lists_free(C.byref(L))
         return res
+
+
+class LmpResult:
+    def __init__(self, f, eng, virial, eatom, vatom):
+        self.f, self.eng, self.virial, self.eatom, self.vatom = f, eng, virial, eatom, vatom
+
+
+class LmpOracle(Oracle):
+    """The C restatement of the LAMMPS pair style (pair_chimes.cpp) on one
+    process, over an LmpSystem (tests/lmp_setup.py)."""
+
+    def masses(self):
+        return [self.L.oc_mass(self.h, t) for t in range(self.L.oc_ntypes(self.h))]
+
+    def rc2(self):
+        nt = self.L.oc_ntypes(self.h)
+        return np.array([[self.L.oc_cutoff_2b(self.h, a, b) for b in range(nt)] for a in range(nt)])
+
+    def build_lists(self, S):
+        n3, n4 = C.c_long(0), C.c_long(0)
+        p3, p4 = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        rc = self.L.oc_lmp_build_lists(self.h, S.x.reshape(-1), S.tag, S.inum, S.ilist, S.numneigh, S.nbr_off,
+                                       S.nbr, S.skin, C.byref(n3), C.byref(p3), C.byref(n4), C.byref(p4))
+        if rc != 0:
+            raise RuntimeError("oracle list build failed")
+        trips = _keys(p3, n3.value, 3)
+        quads = _keys(p4, n4.value, 4)
+        self.L.oc_lmp_free(C.cast(p3, C.c_void_p))
+        self.L.oc_lmp_free(C.cast(p4, C.c_void_p))
+        return trips, quads
+
+    def compute(self, S, trips, quads, eflag_global=1, eflag_atom=1, vflag_global=1, vflag_atom=1):
+        trips = np.ascontiguousarray(trips, np.int32).reshape(-1)
+        quads = np.ascontiguousarray(quads, np.int32).reshape(-1)
+        f = np.zeros(S.nall * 3)
+        virial, eatom, vatom = np.zeros(6), np.zeros(S.nall), np.zeros(S.nall * 6)
+        eng = C.c_double(0.0)
+        self.L.oc_lmp_compute(self.h, S.x.reshape(-1), S.ctype, S.tag, S.inum, S.ilist, S.numneigh, S.nbr_off,
+                              S.nbr, len(trips) // 3, trips if len(trips) else np.zeros(3, np.int32),
+                              len(quads) // 4, quads if len(quads) else np.zeros(4, np.int32),
+                              eflag_global, eflag_atom, vflag_global, vflag_atom, f, C.byref(eng), virial, eatom,
+                              vatom)
+        return LmpResult(f.reshape(-1, 3), eng.value, virial, eatom, vatom.reshape(-1, 6))
 
 
 _ref = None
