@@ -45,6 +45,8 @@ EXPORTS = {
                      "chimes_gpu_server_free",
                      "chimes_gpu_last_timing", "chimes_gpu_last_counts", "chimes_gpu_compute_clusters",
                      "chimes_gpu_cluster_lists", "chimes_gpu_free"],
+    "chimes_lmp.h": ["chimes_lmp_create", "chimes_lmp_free", "chimes_lmp_build_mb_neighlists",
+                     "chimes_lmp_compute", "chimes_lmp_mb_lists", "chimes_lmp_timing"],
 }
 
 
@@ -113,6 +115,17 @@ def lib():
         L.chimes_gpu_cluster_lists.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), pp,
                                                C.POINTER(C.c_long), pp, C.POINTER(C.c_long), pp]
         L.chimes_gpu_free.argtypes = [C.c_void_p]
+        ipp = C.POINTER(C.POINTER(C.c_int))
+        L.chimes_lmp_create.restype = C.c_void_p
+        L.chimes_lmp_create.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.chimes_lmp_free.argtypes = [C.c_void_p]
+        L.chimes_lmp_build_mb_neighlists.argtypes = [C.c_void_p, C.c_int, _dp, _ip, C.c_int, _ip, _ip, ipp,
+                                                     C.c_double]
+        L.chimes_lmp_compute.argtypes = [C.c_void_p, C.c_int, _dp, _ip, _ip, C.c_int, _ip, _ip, ipp, C.c_int,
+                                         C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
+                                         C.POINTER(C.c_double), _dp, _dp, _dp]
+        L.chimes_lmp_mb_lists.argtypes = [C.c_void_p, C.POINTER(C.c_long), pp, C.POINTER(C.c_long), pp]
+        L.chimes_lmp_timing.argtypes = [C.c_void_p, _dp]
         _lib = L
     return _lib
 
@@ -193,6 +206,94 @@ class GpuModel:
             self.L.chimes_gpu_free(C.cast(p, C.c_void_p))
             out.append(a)
         return tuple(out)
+
+
+class LmpPair:
+    """The LAMMPS pair style chimesFF on the GPU (chimes_lmp.h), with the
+    arrays PairCHIMES::compute reads (etc/lmp/src/pair_chimes.cpp:480-831).
+
+    masses: {LAMMPS type: mass}.  Neighbour rows are given packed: the row of
+    owner ilist[ii] is nbr[nbr_off[ii]:nbr_off[ii + 1]] (firstneigh[ilist[ii]]
+    in LAMMPS); numneigh is indexed by atom like LAMMPS's.
+    """
+
+    def __init__(self, model, masses):
+        self.L = lib()
+        self.model = model
+        nt = max(masses)
+        m = np.zeros(nt + 1)
+        for t, v in masses.items():
+            m[t] = v
+        self.h = self.L.chimes_lmp_create(model.h, nt, m)
+        if not self.h:
+            raise ChimesError(_err())
+
+    @staticmethod
+    def _rows(nall, ilist, nbr_off, nbr):
+        rows = (C.POINTER(C.c_int) * nall)()
+        base = nbr.ctypes.data
+        for ii, i in enumerate(ilist):
+            rows[int(i)] = C.cast(base + 4 * int(nbr_off[ii]), C.POINTER(C.c_int))
+        return C.cast(rows, C.POINTER(C.POINTER(C.c_int))), rows
+
+    def compute(self, x, type, tag, ilist, numneigh, nbr_off, nbr, ago=0, skin=1.0, eflag_global=1,
+                eflag_atom=1, vflag_global=1, vflag_atom=1):
+        """Returns (f[nall, 3], eng, virial[6], eatom[nall], vatom[nall, 6])."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        nall = x.size // 3
+        type, tag = (np.ascontiguousarray(a, np.int32) for a in (type, tag))
+        ilist, numneigh = (np.ascontiguousarray(a, np.int32) for a in (ilist, numneigh))
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        if nbr.size == 0:
+            nbr = np.zeros(1, np.int32)
+        fn, keep = self._rows(nall, ilist, nbr_off, nbr)
+        f, virial, eatom, vatom = np.zeros(nall * 3), np.zeros(6), np.zeros(nall), np.zeros(nall * 6)
+        eng = C.c_double(0.0)
+        if self.L.chimes_lmp_compute(self.h, nall, x, type, tag, len(ilist), ilist, numneigh, fn, int(ago),
+                                     float(skin), eflag_global, eflag_atom, vflag_global, vflag_atom, f,
+                                     C.byref(eng), virial, eatom, vatom):
+            raise ChimesError(_err())
+        del keep
+        return f.reshape(-1, 3), eng.value, virial, eatom, vatom.reshape(-1, 6)
+
+    def build_mb_neighlists(self, x, tag, ilist, numneigh, nbr_off, nbr, skin):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        nall = x.size // 3
+        tag, ilist, numneigh = (np.ascontiguousarray(a, np.int32) for a in (tag, ilist, numneigh))
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        if nbr.size == 0:
+            nbr = np.zeros(1, np.int32)
+        fn, keep = self._rows(nall, ilist, nbr_off, nbr)
+        if self.L.chimes_lmp_build_mb_neighlists(self.h, nall, x, tag, len(ilist), ilist, numneigh, fn,
+                                                 float(skin)):
+            raise ChimesError(_err())
+        del keep
+
+    def mb_lists(self):
+        """(trips[n3, 3], quads[n4, 4]) of the last build, in the reference's order."""
+        n3, n4 = C.c_long(), C.c_long()
+        p3, p4 = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        if self.L.chimes_lmp_mb_lists(self.h, C.byref(n3), C.byref(p3), C.byref(n4), C.byref(p4)):
+            raise ChimesError(_err())
+        out = []
+        for p, n, w in ((p3, n3.value, 3), (p4, n4.value, 4)):
+            a = np.ctypeslib.as_array(p, shape=(max(n, 1) * w,))[: n * w].reshape(n, w).copy()
+            self.L.chimes_gpu_free(C.cast(p, C.c_void_p))
+            out.append(a)
+        return tuple(out)
+
+    def timing(self):
+        t = np.zeros(4)
+        self.L.chimes_lmp_timing(self.h, t)
+        return t
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.chimes_lmp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def _flatten(configs, model_of, models):

@@ -333,3 +333,85 @@ int chimes_gpu_cluster_lists(chimes_gpu_model *m, int conf, long *n2, int **pair
 void chimes_gpu_free(void *p) { chx_free(p); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// chimes_lmp.h: the LAMMPS pair style
+// ---------------------------------------------------------------------------
+#include "chimes_lmp.h"
+
+struct chimes_lmp {
+    chx_lmp *p = nullptr;
+    ~chimes_lmp()
+    {
+        if (p) chx_lmp_destroy(p);
+    }
+};
+
+extern "C" {
+
+chimes_lmp *chimes_lmp_create(chimes_gpu_model *m, int ntypes, const double *mass)
+{
+    if (!m || ntypes <= 0 || !mass) {
+        fail("model, atom types and masses are required");
+        return nullptr;
+    }
+    const std::vector<double> &cm = m->dm->P.masses;
+    std::vector<int> ct(ntypes, -1);
+    for (int t = 1; t <= ntypes; t++) {
+        for (size_t j = 0; j < cm.size(); j++)
+            if (std::fabs(mass[t] - cm[j]) < 1e-3) {
+                ct[t - 1] = (int)j;
+                break;
+            }
+        if (ct[t - 1] < 0) {
+            fail("LAMMPS atom type " + std::to_string(t) + " matches no ChIMES atom type by mass");
+            return nullptr;
+        }
+    }
+    char err[256] = {0};
+    chx_lmp *p = chx_lmp_create(m->dm->m, ntypes, ct.data(), err, sizeof err);
+    if (!p) {
+        fail(err);
+        return nullptr;
+    }
+    chimes_lmp *h = new chimes_lmp();
+    h->p = p;
+    return h;
+}
+
+void chimes_lmp_free(chimes_lmp *p) { delete p; }
+
+int chimes_lmp_build_mb_neighlists(chimes_lmp *p, int nall, const double *x, const int *tag, int inum,
+                                   const int *ilist, const int *numneigh, int *const *firstneigh, double skin)
+{
+    if (!p) return fail("no handle");
+    if (chx_lmp_build_lists(p->p, nall, x, tag, inum, ilist, numneigh, (const int *const *)firstneigh, skin))
+        return fail(chx_lmp_error(p->p));
+    return 0;
+}
+
+int chimes_lmp_compute(chimes_lmp *p, int nall, const double *x, const int *type, const int *tag, int inum,
+                       const int *ilist, const int *numneigh, int *const *firstneigh, int ago, double skin,
+                       int eflag_global, int eflag_atom, int vflag_global, int vflag_atom, double *f,
+                       double *eng_vdwl, double *virial, double *eatom, double *vatom)
+{
+    if (!p) return fail("no handle");
+    if (chx_lmp_compute(p->p, nall, x, type, tag, inum, ilist, numneigh, (const int *const *)firstneigh, ago, skin,
+                        eflag_global, eflag_atom, vflag_global, vflag_atom, f, eng_vdwl, virial, eatom, vatom))
+        return fail(chx_lmp_error(p->p));
+    return 0;
+}
+
+int chimes_lmp_mb_lists(chimes_lmp *p, long *n3, int **trips, long *n4, int **quads)
+{
+    if (!p) return fail("no handle");
+    if (chx_lmp_lists(p->p, n3, trips, n4, quads)) return fail(chx_lmp_error(p->p));
+    return 0;
+}
+
+void chimes_lmp_timing(const chimes_lmp *p, double ms[4])
+{
+    if (p) chx_lmp_timing(p->p, ms);
+}
+
+}  // extern "C"

@@ -123,6 +123,30 @@ void chx_last_counts(chx_engine *e, long long *c3);
 // [0] whole pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] enumeration.
 void chx_last_timing(chx_engine *e, double *ms4);
 
+// LAMMPS pair style (etc/lmp/src/pair_chimes.cpp) over one process's local
+// and ghost atoms.  chimes_type[t - 1] is the ChIMES type of LAMMPS type t.
+// x[nall*3], type[nall] (LAMMPS types), tag[nall], ilist[inum] and the
+// neighbour rows firstneigh[i][0 .. numneigh[i]) of the full list with ghosts
+// are host arrays.  compute rebuilds the many-body lists when ago == 0 (or
+// none exist) and adds onto f[nall*3], *eng, virial[6] (xx yy zz xy xz yz),
+// eatom[nall] and vatom[nall*6] as the flags ask.
+typedef struct chx_lmp chx_lmp;
+chx_lmp *chx_lmp_create(chx_model *m, int ntypes, const int *chimes_type, char *err, int errlen);
+void chx_lmp_destroy(chx_lmp *p);
+const char *chx_lmp_error(chx_lmp *p);
+int chx_lmp_build_lists(chx_lmp *p, int nall, const double *x, const int *tag, int inum, const int *ilist,
+                        const int *numneigh, const int *const *firstneigh, double skin);
+int chx_lmp_compute(chx_lmp *p, int nall, const double *x, const int *type, const int *tag, int inum,
+                    const int *ilist, const int *numneigh, const int *const *firstneigh, int ago, double skin,
+                    int eflag_global, int eflag_atom, int vflag_global, int vflag_atom, double *f, double *eng,
+                    double *virial, double *eatom, double *vatom);
+// Many-body lists of the last build: (i, j, k) and (i, j, k, l) atom indices
+// in the reference's loop order; malloc'd, free with chx_free.
+int chx_lmp_lists(chx_lmp *p, long *n3, int **trips, long *n4, int **quads);
+// Milliseconds of the last compute (HIP events): [0] input copies, [1] list
+// build, [2] 1-/2-body, [3] 3-/4-body, read-back of tallies.
+void chx_lmp_timing(chx_lmp *p, double *ms4);
+
 #ifdef __cplusplus
 }
 #endif

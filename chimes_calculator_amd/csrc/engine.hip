@@ -29,6 +29,7 @@
 #include "chx_engine.h"
 #include "chx_eval.h"
 #include "chx_kernels.h"
+#include "chx_lmp.h"
 
 using namespace chx;
 
@@ -1443,6 +1444,322 @@ int chx_eval_clusters(chx_engine *e, chx_model *m, int nb, int n, const double *
         e->err = x.msg;
         return 1;
     }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// LAMMPS pair style (etc/lmp/src/pair_chimes.cpp) on one process's local and
+// ghost atoms.  The neighbour rows (LAMMPS's firstneigh) are packed on the
+// host after every index has been checked, so no kernel reads outside its
+// arrays; the many-body lists stay on the device between rebuilds.
+// ---------------------------------------------------------------------------
+struct chx_lmp {
+    chx_model *m = nullptr;
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[5];
+    std::string err;
+    std::vector<int> chimes_type;
+    std::vector<int> off_h, nbr_h, ctype_h, cnt_h;
+    std::vector<long long> scan_h;
+    std::vector<double> out_h;
+    DevBuf<double> x, out;
+    DevBuf<int> ctype, tag, ilist, nbr_off, nbr, cnt, trips, quads, status;
+    DevBuf<long long> off;
+    DevBuf<unsigned long long> acc;
+    long n3 = 0, n4 = 0;
+    bool have_lists = false;
+    int nall = 0, inum = 0;
+    int lists_nall = -1, lists_inum = -1;  // shape the lists were built for
+    double ms[4] = {0, 0, 0, 0};
+    ~chx_lmp()
+    {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            for (hipEvent_t &e : ev) (void)hipEventDestroy(e);
+            (void)hipStreamDestroy(st);
+        }
+    }
+    chx_lmp_args args() const
+    {
+        chx_lmp_args A{};
+        A.inum = inum;
+        A.nall = nall;
+        A.x = x.p;
+        A.ctype = ctype.p;
+        A.tag = tag.p;
+        A.ilist = ilist.p;
+        A.nbr_off = nbr_off.p;
+        A.nbr = nbr.p;
+        A.status = status.p;
+        return A;
+    }
+};
+
+namespace {
+
+// LAMMPS keeps special-bond bits above NEIGHMASK in neighbour indices
+constexpr int LMP_NEIGHMASK = 0x3FFFFFFF;
+
+void lmp_upload(chx_lmp *p, int nall, const double *x, const int *type, const int *tag, int inum, const int *ilist,
+                const int *numneigh, const int *const *firstneigh)
+{
+    if (nall <= 0 || inum < 0 || inum > nall) throw EngineError("atom counts out of range");
+    if (!x || !tag || (inum > 0 && (!ilist || !numneigh || !firstneigh))) throw EngineError("missing input array");
+    p->off_h.assign((size_t)inum + 1, 0);
+    p->nbr_h.clear();
+    for (int ii = 0; ii < inum; ii++) {
+        const int i = ilist[ii];
+        if (i < 0 || i >= nall) throw EngineError("ilist entry out of range");
+        const int n = numneigh[i];
+        if (n < 0) throw EngineError("negative neighbour count");
+        const int *row = firstneigh[i];
+        if (n > 0 && !row) throw EngineError("missing neighbour row");
+        for (int q = 0; q < n; q++) {
+            const int j = row[q] & LMP_NEIGHMASK;
+            if (j >= nall) throw EngineError("neighbour index out of range");
+            p->nbr_h.push_back(j);
+        }
+        if ((long)p->nbr_h.size() >= (1L << 31)) throw EngineError("too many neighbour entries");
+        p->off_h[ii + 1] = (int)p->nbr_h.size();
+    }
+    hipStream_t st = p->st;
+    p->nall = nall;
+    p->inum = inum;
+    p->x.ensure((size_t)nall * 3);
+    p->tag.ensure(nall);
+    p->ilist.ensure(std::max(inum, 1));
+    p->nbr_off.ensure((size_t)inum + 1);
+    p->nbr.ensure(std::max<size_t>(p->nbr_h.size(), 1));
+    HIPCHK(hipMemcpyAsync(p->x.p, x, sizeof(double) * nall * 3, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->tag.p, tag, sizeof(int) * nall, hipMemcpyHostToDevice, st));
+    if (inum > 0) HIPCHK(hipMemcpyAsync(p->ilist.p, ilist, sizeof(int) * inum, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->nbr_off.p, p->off_h.data(), sizeof(int) * (inum + 1), hipMemcpyHostToDevice, st));
+    if (!p->nbr_h.empty())
+        HIPCHK(hipMemcpyAsync(p->nbr.p, p->nbr_h.data(), sizeof(int) * p->nbr_h.size(), hipMemcpyHostToDevice, st));
+    p->ctype.ensure(nall);
+    if (type) {
+        const int nt = (int)p->chimes_type.size();
+        p->ctype_h.resize(nall);
+        for (int a = 0; a < nall; a++) {
+            if (type[a] < 1 || type[a] > nt) throw EngineError("atom type out of range");
+            p->ctype_h[a] = p->chimes_type[type[a] - 1];
+        }
+        HIPCHK(hipMemcpyAsync(p->ctype.p, p->ctype_h.data(), sizeof(int) * nall, hipMemcpyHostToDevice, st));
+    }
+    p->status.ensure(CHX_LMP_ST_WORDS);
+    HIPCHK(hipMemsetAsync(p->status.p, 0, sizeof(int) * CHX_LMP_ST_WORDS, st));
+}
+
+void lmp_check_status(chx_lmp *p, const int *s)
+{
+    if (s[CHX_LMP_ST_FLAGS] & CHX_LMP_BAD_CAND)
+        throw EngineError("an owner has " + std::to_string(s[CHX_LMP_ST_MAXCAND]) +
+                          " neighbours within the padded many-body cutoffs (at most " +
+                          std::to_string(CHX_LMP_CAND) + ")");
+    if (s[CHX_LMP_ST_FLAGS] & CHX_LMP_BAD_RANGE) throw EngineError("a contribution is out of the accumulator range");
+}
+
+// build_mb_neighlists (pair_chimes.cpp:321-478) on the uploaded arrays
+void lmp_build(chx_lmp *p, double skin)
+{
+    const chx_model *m = p->m;
+    p->n3 = p->n4 = 0;
+    p->have_lists = false;  // until this build completes
+    p->lists_nall = p->nall;
+    p->lists_inum = p->inum;
+    const int inum = p->inum;
+    if ((m->d.order[1] == 0 && m->d.order[2] == 0) || inum == 0) {
+        p->have_lists = true;
+        return;
+    }
+    const double pad3 = m->rc[1] + skin, pad4 = m->rc[2] + skin;
+    const int has4 = m->d.order[2] > 0;
+    chx_lmp_args A = p->args();
+    hipStream_t st = p->st;
+    p->cnt.ensure((size_t)2 * inum + CHX_LMP_ST_WORDS);
+    launch_lmp_lists(0, A, pad3, pad4, has4, p->cnt.p, p->cnt.p + inum, nullptr, nullptr, nullptr, nullptr, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(p->cnt.p + 2 * inum, p->status.p, sizeof(int) * CHX_LMP_ST_WORDS, hipMemcpyDeviceToDevice,
+                          st));
+    p->cnt_h.resize((size_t)2 * inum + CHX_LMP_ST_WORDS);
+    HIPCHK(hipMemcpyAsync(p->cnt_h.data(), p->cnt.p, sizeof(int) * p->cnt_h.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    lmp_check_status(p, p->cnt_h.data() + 2 * inum);
+    p->scan_h.resize((size_t)2 * inum);
+    long long t3 = 0, t4 = 0;
+    for (int ii = 0; ii < inum; ii++) {
+        p->scan_h[ii] = t3;
+        p->scan_h[inum + ii] = t4;
+        t3 += p->cnt_h[ii];
+        t4 += p->cnt_h[inum + ii];
+    }
+    if (t3 * 3 >= (1LL << 31) || t4 * 4 >= (1LL << 31)) throw EngineError("many-body lists too long");
+    p->n3 = (long)t3;
+    p->n4 = (long)t4;
+    if (t3 + t4 == 0) {
+        p->have_lists = true;
+        return;
+    }
+    p->off.ensure((size_t)2 * inum);
+    p->trips.ensure(std::max<size_t>((size_t)t3 * 3, 3));
+    p->quads.ensure(std::max<size_t>((size_t)t4 * 4, 4));
+    HIPCHK(hipMemcpyAsync(p->off.p, p->scan_h.data(), sizeof(long long) * 2 * inum, hipMemcpyHostToDevice, st));
+    launch_lmp_lists(1, A, pad3, pad4, has4, nullptr, nullptr, p->off.p, p->off.p + inum, p->trips.p, p->quads.p,
+                     st);
+    HIPCHK(hipGetLastError());
+    p->have_lists = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+chx_lmp *chx_lmp_create(chx_model *m, int ntypes, const int *chimes_type, char *err, int errlen)
+{
+    chx_lmp *p = new chx_lmp();
+    try {
+        if (!m) throw EngineError("no model");
+        if (ntypes <= 0 || !chimes_type) throw EngineError("no atom types");
+        for (int t = 0; t < ntypes; t++)
+            if (chimes_type[t] < 0 || chimes_type[t] >= m->d.nt) throw EngineError("ChIMES type index out of range");
+        p->m = m;
+        p->device = m->eng->device;
+        p->chimes_type.assign(chimes_type, chimes_type + ntypes);
+        HIPCHK(hipSetDevice(p->device));
+        HIPCHK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
+        for (hipEvent_t &e : p->ev) HIPCHK(hipEventCreate(&e));
+        return p;
+    } catch (const EngineError &x) {
+        if (err && errlen > 0) snprintf(err, errlen, "%s", x.msg.c_str());
+        delete p;
+        return nullptr;
+    }
+}
+
+void chx_lmp_destroy(chx_lmp *p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    delete p;
+}
+
+const char *chx_lmp_error(chx_lmp *p) { return p ? p->err.c_str() : "no handle"; }
+
+int chx_lmp_build_lists(chx_lmp *p, int nall, const double *x, const int *tag, int inum, const int *ilist,
+                        const int *numneigh, const int *const *firstneigh, double skin)
+{
+    try {
+        HIPCHK(hipSetDevice(p->device));
+        lmp_upload(p, nall, x, nullptr, tag, inum, ilist, numneigh, firstneigh);
+        lmp_build(p, skin);
+        HIPCHK(hipStreamSynchronize(p->st));
+        return 0;
+    } catch (const EngineError &e) {
+        p->err = e.msg;
+        p->have_lists = false;
+        return 1;
+    }
+}
+
+int chx_lmp_compute(chx_lmp *p, int nall, const double *x, const int *type, const int *tag, int inum,
+                    const int *ilist, const int *numneigh, const int *const *firstneigh, int ago, double skin,
+                    int eflag_global, int eflag_atom, int vflag_global, int vflag_atom, double *f, double *eng,
+                    double *virial, double *eatom, double *vatom)
+{
+    try {
+        HIPCHK(hipSetDevice(p->device));
+        if (!type || !f) throw EngineError("missing input array");
+        if ((eflag_global && !eng) || (vflag_global && !virial) || (eflag_atom && !eatom) ||
+            (vflag_atom && !vatom))
+            throw EngineError("missing tally array");
+        hipStream_t st = p->st;
+        HIPCHK(hipEventRecord(p->ev[0], st));
+        lmp_upload(p, nall, x, type, tag, inum, ilist, numneigh, firstneigh);
+        HIPCHK(hipEventRecord(p->ev[1], st));
+        if (ago == 0 || !p->have_lists || p->lists_inum != inum || p->lists_nall != nall) lmp_build(p, skin);
+        HIPCHK(hipEventRecord(p->ev[2], st));
+        const chx_model *m = p->m;
+        // accumulators: forces, per-atom energy, per-atom virial, per-owner global
+        const size_t nF = (size_t)nall * 6, nE = eflag_atom ? (size_t)nall * 2 : 0,
+                     nV = vflag_atom ? (size_t)nall * 12 : 0,
+                     nG = (eflag_global || vflag_global) ? (size_t)nall * 14 : 0;
+        p->acc.ensure(nF + nE + nV + nG);
+        HIPCHK(hipMemsetAsync(p->acc.p, 0, sizeof(unsigned long long) * (nF + nE + nV + nG), st));
+        chx_lmp_args A = p->args();
+        A.Facc = p->acc.p;
+        A.Eacc = nE ? A.Facc + nF : nullptr;
+        A.Vacc = nV ? A.Facc + nF + nE : nullptr;
+        A.Gacc = nG ? A.Facc + nF + nE + nV : nullptr;
+        A.eflag_global = eflag_global;
+        A.vflag_global = vflag_global;
+        launch_lmp_pair(A, m->dev, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(p->ev[3], st));
+        if (m->d.order[1] > 0 && p->n3 > 0) launch_lmp_clusters(3, m->d.D3, p->n3, p->trips.p, A, m->dev, st);
+        if (m->d.order[2] > 0 && p->n4 > 0) launch_lmp_clusters(4, m->d.D4, p->n4, p->quads.p, A, m->dev, st);
+        HIPCHK(hipGetLastError());
+        const size_t nout = (size_t)nall * 3 + (nE ? nall : 0) + (nV ? (size_t)nall * 6 : 0) + 7;
+        p->out.ensure(nout + CHX_LMP_ST_WORDS);
+        double *F = p->out.p, *Ea = F + (size_t)nall * 3, *Va = Ea + (nE ? nall : 0),
+               *G = Va + (nV ? (size_t)nall * 6 : 0);
+        launch_lmp_out(A, F, nE ? Ea : nullptr, nV ? Va : nullptr, G, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(p->out.p + nout, p->status.p, sizeof(int) * CHX_LMP_ST_WORDS, hipMemcpyDeviceToDevice,
+                              st));
+        HIPCHK(hipEventRecord(p->ev[4], st));
+        p->out_h.resize(nout + CHX_LMP_ST_WORDS);
+        HIPCHK(hipMemcpyAsync(p->out_h.data(), p->out.p, sizeof(double) * p->out_h.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        int s[CHX_LMP_ST_WORDS];
+        memcpy(s, p->out_h.data() + nout, sizeof(s));
+        lmp_check_status(p, s);
+        const double *o = p->out_h.data();
+        for (size_t k = 0; k < (size_t)nall * 3; k++) f[k] += o[k];
+        o += (size_t)nall * 3;
+        if (nE) {
+            for (int a = 0; a < nall; a++) eatom[a] += o[a];
+            o += nall;
+        }
+        if (nV) {
+            for (size_t k = 0; k < (size_t)nall * 6; k++) vatom[k] += o[k];
+            o += (size_t)nall * 6;
+        }
+        if (eflag_global) *eng += o[0];
+        if (vflag_global)
+            for (int k = 0; k < 6; k++) virial[k] += o[1 + k];
+        float t[4];
+        for (int k = 0; k < 4; k++) HIPCHK(hipEventElapsedTime(&t[k], p->ev[k], p->ev[k + 1]));
+        for (int k = 0; k < 4; k++) p->ms[k] = t[k];
+        return 0;
+    } catch (const EngineError &e) {
+        p->err = e.msg;
+        return 1;
+    }
+}
+
+int chx_lmp_lists(chx_lmp *p, long *n3, int **trips, long *n4, int **quads)
+{
+    try {
+        HIPCHK(hipSetDevice(p->device));
+        if (!p->have_lists) throw EngineError("no many-body lists built");
+        *n3 = p->n3;
+        *n4 = p->n4;
+        *trips = (int *)malloc(sizeof(int) * std::max<long>(3 * p->n3, 1));
+        *quads = (int *)malloc(sizeof(int) * std::max<long>(4 * p->n4, 1));
+        if (p->n3) HIPCHK(hipMemcpy(*trips, p->trips.p, sizeof(int) * 3 * p->n3, hipMemcpyDeviceToHost));
+        if (p->n4) HIPCHK(hipMemcpy(*quads, p->quads.p, sizeof(int) * 4 * p->n4, hipMemcpyDeviceToHost));
+        return 0;
+    } catch (const EngineError &e) {
+        p->err = e.msg;
+        return 1;
+    }
+}
+
+void chx_lmp_timing(chx_lmp *p, double *ms4)
+{
+    for (int k = 0; k < 4; k++) ms4[k] = p->ms[k];
 }
 
 }  // extern "C"

@@ -22,6 +22,7 @@
 #include "chx_engine.h"
 #include "chx_eval.h"
 #include "eval_math.h"
+#include "chx_cluster.h"
 
 #include <algorithm>
 
@@ -1197,97 +1198,20 @@ __global__ void k_clusters(int n, const double *__restrict__ dist, const double 
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     constexpr int P = NB * (NB - 1) / 2;
-    int pa[P], pb[P];
-    {
-        int q = 0;
-        for (int a = 0; a < NB; a++)
-            for (int b = a + 1; b < NB; b++) { pa[q] = a; pb[q] = b; q++; }
-    }
-    const int *ty = types + (long)c * NB;
-    double Ec = 0.0;
-    double fs_rt[P];
-    for (int p = 0; p < P; p++) fs_rt[p] = 0.0;
-    int flags = 0;
-    bool evaluated = false;
-    if constexpr (NB == 2) {
-        const int pt = M.pairmap[ty[0] * M.nt + ty[1]];
-        double fs;
-        const double r = dist[c];
-        eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
-                  M.pen_dist, M.pen_scale, Ec, fs, flags);
-        fs_rt[0] = fs;
-        evaluated = r < M.slots2[pt].rout;
-        if (evaluated) {
-            double ls = 1.0;
-            if (r - M.pen_dist < M.slots2[pt].rin) {
-                const double rp = M.slots2[pt].rin + M.pen_dist - r;
-                if (rp > 0.0) ls = (-3.0 * rp * rp * M.pen_scale) / r;
-            }
-            last[c] = ls;
-        }
-    } else {
-        int tup = 0;
-        for (int a = 0; a < NB; a++) tup = tup * M.nt + ty[a];
-        const int tau = NB == 3 ? M.tripmap[tup] : M.quadmap[tup];
-        if (tau >= 0) {
-            const int *perm = (NB == 3 ? M.trip_perm : M.quad_perm) + tup * NB;
-            const chx_slot *sl = (NB == 3 ? M.slots3 : M.slots4) + tau * P;
-            int rt[P];
-            double rq[P];
-            bool in = true;
-            for (int q = 0; q < P; q++) {
-                const int a = perm[pa[q]], b = perm[pb[q]];
-                const int lo = a < b ? a : b, hi = a < b ? b : a;
-                int idx = 0;
-                for (int x = 0; x < lo; x++) idx += NB - 1 - x;
-                idx += hi - lo - 1;
-                rt[q] = idx;
-                rq[q] = dist[(long)c * P + idx];
-                in = in && rq[q] < sl[q].rout;
-            }
-            if (in) {
-                evaluated = true;
-                double T[P][D], dT[P][D], f[P], fd[P];
-                for (int q = 0; q < P; q++) {
-                    cheby<D, double>(rq[q], sl[q], T[q], dT[q], flags);
-                    fcut(rq[q], sl[q].rout, M.tersoff, M.fcut_var, f[q], fd[q]);
-                }
-                double S, G[P], fs[P];
-                if constexpr (NB == 3) contract3<D, double, double>(M.C3 + (long)tau * D * D * D, T, dT, S, G);
-                else contract4<D, double, double>(M.C4 + (long)tau * D * D * D * D * D * D, T, dT, S, G);
-                cluster_scalars<P, double>(rq, f, fd, S, G, Ec, fs);
-                for (int q = 0; q < P; q++) fs_rt[rt[q]] = fs[q];
-                // last term of the list, in runtime pair order
-                const int *lp = (NB == 3 ? M.last3 : M.last4) + tau * P;
-                const double lc = (NB == 3 ? M.lastc3 : M.lastc4)[tau];
-                for (int q = 0; q < P; q++) {
-                    double v = lc * (f[q] * dT[q][lp[q]] + fd[q] * T[q][lp[q]]);
-                    double other = 1.0;
-                    for (int k = 0; k < P; k++)
-                        if (k != q) other *= f[k];
-                    v *= other / rq[q];
-                    for (int k = 0; k < P; k++)
-                        if (k != q) v *= T[k][lp[k]];
-                    last[(long)c * P + rt[q]] = v;
-                }
-            }
-        }
-    }
-    double fo[NB][3];
-    for (int a = 0; a < NB; a++) fo[a][0] = fo[a][1] = fo[a][2] = 0.0;
-    double So[6] = {0, 0, 0, 0, 0, 0};
+    int ty[NB];
+    double rq[P], dr[P][3];
+#pragma unroll
+    for (int a = 0; a < NB; a++) ty[a] = types[(long)c * NB + a];
+#pragma unroll
     for (int p = 0; p < P; p++) {
-        const double fs = fs_rt[p];
-        const double *d = drv + ((long)c * P + p) * 3;
-        fo[pa[p]][0] += fs * d[0]; fo[pa[p]][1] += fs * d[1]; fo[pa[p]][2] += fs * d[2];
-        fo[pb[p]][0] -= fs * d[0]; fo[pb[p]][1] -= fs * d[1]; fo[pb[p]][2] -= fs * d[2];
-        So[0] -= fs * (d[0] * d[0]);
-        So[1] -= fs * (d[0] * d[1]);
-        So[2] -= fs * (d[0] * d[2]);
-        So[3] -= fs * (d[1] * d[1]);
-        So[4] -= fs * (d[1] * d[2]);
-        So[5] -= fs * (d[2] * d[2]);
+        rq[p] = dist[(long)c * P + p];
+#pragma unroll
+        for (int k = 0; k < 3; k++) dr[p][k] = drv[((long)c * P + p) * 3 + k];
     }
+    double Ec, fo[NB][3], So[6];
+    int flags = 0;
+    bool evaluated;
+    explicit_cluster<NB, D>(M, ty, rq, dr, Ec, fo, So, flags, evaluated, last + (long)c * P);
     E[c] = Ec;
     for (int a = 0; a < NB; a++)
         for (int k = 0; k < 3; k++) F[((long)c * NB + a) * 3 + k] = fo[a][k];
@@ -1298,8 +1222,6 @@ __global__ void k_clusters(int n, const double *__restrict__ dist, const double 
 // ---------------------------------------------------------------------------
 // Host-side launchers: dispatch on the dense dimension D and precision.
 // ---------------------------------------------------------------------------
-#define CHX_D3_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13)
-#define CHX_D4_CASES(X) X(2) X(3) X(4) X(5) X(6)
 #define CHX_D4M_CASES(X) X(2) X(3) X(4)
 
 bool supported_D3(int D) { return D >= 2 && D <= 13; }

@@ -1,0 +1,380 @@
+// LAMMPS pair style chimesFF (etc/lmp/src/pair_chimes.cpp) on the GPU, one
+// process's local and ghost atoms per call:
+//   k_lmp_pair      1- and 2-body loops of PairCHIMES::compute (567-656)
+//   k_lmp_lists     PairCHIMES::build_mb_neighlists (321-478)
+//   k_lmp_clusters  3- and 4-body loops (673-829) over those lists
+//   k_lmp_out       fixed-point accumulators -> forces and per-atom tallies
+//   k_lmp_global    global energy and virial
+// Every tally of Pair::ev_tally_mb (etc/lmp/etc/pair.cpp:1425-1482) is added
+// in the exact fixed-point form of the batch pipeline, so results do not
+// depend on the order of the lists or on the kernel schedule.
+#include <hip/hip_runtime.h>
+
+#include "chx_cluster.h"
+#include "chx_dev.h"
+#include "chx_lmp.h"
+
+namespace chx {
+
+// get_dist (pair_chimes.cpp:303-312): dr = x_b - x_a, same operation order
+__device__ __forceinline__ double lmp_dist(const double *a, const double *b, double (&dr)[3])
+{
+    dr[0] = b[0] - a[0];
+    dr[1] = b[1] - a[1];
+    dr[2] = b[2] - a[2];
+    return sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+}
+
+// ChIMES stress (xx xy xz yy yz zz) -> LAMMPS virial order (xx yy zz xy xz yz)
+__device__ __forceinline__ void lmp_order(const double (&s)[6], double (&v)[6])
+{
+    v[0] = s[0]; v[1] = s[3]; v[2] = s[5];
+    v[3] = s[1]; v[4] = s[2]; v[5] = s[4];
+}
+
+// ---------------------------------------------------------------------------
+// 1- and 2-body: sixteen lanes per owner ilist[ii]; lane l takes row entries
+// l, l + 16, ...  Pairs with tag[j] > tag[i] only (pair_chimes.cpp:603).  The
+// owner's force and tallies are summed over its lanes in double in that fixed
+// order and added exactly once; the partner's are added per pair.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lmp_pair(chx_lmp_args A, chx_model_dev M)
+{
+    const int sub = threadIdx.x & 15;
+    const int ii = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+    if (ii >= A.inum) return;  // whole groups of sixteen leave together
+    const int i = A.ilist[ii];
+    const int itag = A.tag[i], ti = A.ctype[i];
+    const double xi[3] = {A.x[3L * i], A.x[3L * i + 1], A.x[3L * i + 2]};
+    // owner force, owner energy share, owner virial share, global energy, global virial
+    double fi[3] = {0, 0, 0}, ea = 0.0, va[6] = {0, 0, 0, 0, 0, 0}, ge = 0.0, gv[6] = {0, 0, 0, 0, 0, 0};
+    bool bad = false;
+    int flags = 0;
+    const int beg = A.nbr_off[ii], end = A.nbr_off[ii + 1];
+    for (int q = beg + sub; q < end; q += 16) {
+        const int j = A.nbr[q];
+        if (A.tag[j] <= itag) continue;
+        const double xj[3] = {A.x[3L * j], A.x[3L * j + 1], A.x[3L * j + 2]};
+        double dr[3];
+        const double r = lmp_dist(xi, xj, dr);
+        const int pt = M.pairmap[ti * M.nt + A.ctype[j]];
+        double E, fs;
+        eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
+                  M.pen_dist, M.pen_scale, E, fs, flags);
+        if (r >= M.slots2[pt].rout) continue;
+        const double F[3] = {fs * dr[0], fs * dr[1], fs * dr[2]};
+        const double s[6] = {-(fs * (dr[0] * dr[0])), -(fs * (dr[0] * dr[1])), -(fs * (dr[0] * dr[2])),
+                             -(fs * (dr[1] * dr[1])), -(fs * (dr[1] * dr[2])), -(fs * (dr[2] * dr[2]))};
+        double v[6];
+        lmp_order(s, v);
+        for (int k = 0; k < 3; k++) {
+            fi[k] += F[k];
+            fx_atomic(A.Facc + 2L * (3L * j + k), -F[k], bad);
+        }
+        ge += E;
+        ea += E / 2;
+        if (A.Eacc) fx_atomic(A.Eacc + 2L * j, E / 2, bad);
+        for (int k = 0; k < 6; k++) {
+            gv[k] += v[k];
+            va[k] += v[k] / 2;
+            if (A.Vacc) fx_atomic(A.Vacc + 2L * (6L * j + k), v[k] / 2, bad);
+        }
+    }
+#pragma unroll
+    for (int w = 8; w >= 1; w >>= 1) {
+        for (int k = 0; k < 3; k++) fi[k] += __shfl_xor(fi[k], w, 64);
+        ea += __shfl_xor(ea, w, 64);
+        ge += __shfl_xor(ge, w, 64);
+        for (int k = 0; k < 6; k++) {
+            va[k] += __shfl_xor(va[k], w, 64);
+            gv[k] += __shfl_xor(gv[k], w, 64);
+        }
+    }
+    if (sub == 0) {
+        const double e1 = M.eoff[ti];  // compute_1B, tallied for one atom
+        for (int k = 0; k < 3; k++) fx_atomic(A.Facc + 2L * (3L * i + k), fi[k], bad);
+        if (A.Eacc) {
+            fx_atomic(A.Eacc + 2L * i, e1, bad);
+            fx_atomic(A.Eacc + 2L * i, ea, bad);
+        }
+        if (A.Vacc)
+            for (int k = 0; k < 6; k++) fx_atomic(A.Vacc + 2L * (6L * i + k), va[k], bad);
+        if (A.Gacc) {
+            if (A.eflag_global) {
+                fx_atomic(A.Gacc + 2L * (7L * i), e1, bad);
+                fx_atomic(A.Gacc + 2L * (7L * i), ge, bad);
+            }
+            if (A.vflag_global)
+                for (int k = 0; k < 6; k++) fx_atomic(A.Gacc + 2L * (7L * i + 1 + k), gv[k], bad);
+        }
+    }
+    if (bad) atomicOr(&A.status[CHX_LMP_ST_FLAGS], CHX_LMP_BAD_RANGE);
+}
+
+// ---------------------------------------------------------------------------
+// Many-body lists, one wavefront per owner ilist[ii].  The owner's row is
+// filtered to J = {j : j != i, tag[j] >= tag[i], d_ij below either padded
+// cutoff}, kept in row order in LDS; every k and l of the reference's loops
+// lies in J.  Triplets come from the (j, k) pairs of J x J in row-major order,
+// quadruplets from each such pair within the 4-body cutoff followed by every
+// l of J, so ballot compaction writes them in the reference's loop order.
+// MODE 0 counts per owner, MODE 1 writes at the scanned offsets.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(64) void k_lmp_lists(chx_lmp_args A, double pad3, double pad4, int has4, int *cnt3,
+                                                  int *cnt4, const long long *off3, const long long *off4,
+                                                  int *trips, int *quads)
+{
+    __shared__ int cj[CHX_LMP_CAND], ctag[CHX_LMP_CAND];
+    __shared__ double cx[CHX_LMP_CAND][3], cd[CHX_LMP_CAND];
+    const int ii = blockIdx.x, lane = threadIdx.x;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int i = A.ilist[ii], itag = A.tag[i];
+    const double xi[3] = {A.x[3L * i], A.x[3L * i + 1], A.x[3L * i + 2]};
+    const int beg = A.nbr_off[ii], n = A.nbr_off[ii + 1] - beg;
+    int m = 0;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+        bool v = false;
+        int j = 0;
+        double xj[3] = {0, 0, 0}, d = 0.0;
+        if (b0 + lane < n) {
+            j = A.nbr[beg + b0 + lane];
+            if (j != i && A.tag[j] >= itag) {
+                xj[0] = A.x[3L * j]; xj[1] = A.x[3L * j + 1]; xj[2] = A.x[3L * j + 2];
+                double dr[3];
+                d = lmp_dist(xi, xj, dr);
+                v = !(d >= pad3 && d >= pad4);
+            }
+        }
+        const unsigned long long bal = __ballot(v);
+        const int r = m + __popcll(bal & below);
+        if (v && r < CHX_LMP_CAND) {
+            cj[r] = j;
+            ctag[r] = A.tag[j];
+            cx[r][0] = xj[0]; cx[r][1] = xj[1]; cx[r][2] = xj[2];
+            cd[r] = d;
+        }
+        m += __popcll(bal);
+    }
+    if (m > CHX_LMP_CAND) {
+        if (lane == 0) {
+            atomicOr(&A.status[CHX_LMP_ST_FLAGS], CHX_LMP_BAD_CAND);
+            atomicMax(&A.status[CHX_LMP_ST_MAXCAND], m);
+        }
+        return;
+    }
+    __syncthreads();
+    long long c3 = 0, c4 = 0;
+    const long long o3 = MODE ? off3[ii] : 0, o4 = MODE ? off4[ii] : 0;
+    const int total = m * m;
+    for (int p0 = 0; p0 < total; p0 += 64) {
+        const int p = p0 + lane;
+        bool v3 = false, v4 = false;
+        int a = 0, b = 0;
+        if (p < total) {
+            a = p / m;
+            b = p - a * m;
+            if (cj[a] != cj[b] && ctag[b] >= ctag[a]) {
+                double dr[3];
+                const double dab = lmp_dist(cx[a], cx[b], dr);
+                const double da = cd[a], db = cd[b];
+                v3 = da < pad3 && db < pad3 && dab < pad3;
+                v4 = has4 && da < pad4 && db < pad4 && dab < pad4;
+            }
+        }
+        const unsigned long long b3 = __ballot(v3);
+        if (MODE && v3) {
+            int *t = trips + 3 * (o3 + c3 + __popcll(b3 & below));
+            t[0] = i; t[1] = cj[a]; t[2] = cj[b];
+        }
+        c3 += __popcll(b3);
+        unsigned long long b4 = __ballot(v4);
+        while (b4) {
+            const int t = __ffsll((long long)b4) - 1;
+            b4 &= b4 - 1;
+            const int pt = p0 + t, at = pt / m, bt = pt - at * m;
+            const int ja = cj[at], jb = cj[bt], tb = ctag[bt];
+            for (int q0 = 0; q0 < m; q0 += 64) {
+                const int q = q0 + lane;
+                bool v = false;
+                if (q < m) {
+                    const int l = cj[q];
+                    if (l != ja && l != jb && ctag[q] >= tb && cd[q] < pad4) {
+                        double dr[3];
+                        v = lmp_dist(cx[at], cx[q], dr) < pad4 && lmp_dist(cx[bt], cx[q], dr) < pad4;
+                    }
+                }
+                const unsigned long long bl = __ballot(v);
+                if (MODE && v) {
+                    int *u = quads + 4 * (o4 + c4 + __popcll(bl & below));
+                    u[0] = i; u[1] = ja; u[2] = jb; u[3] = cj[q];
+                }
+                c4 += __popcll(bl);
+            }
+        }
+    }
+    if (!MODE && lane == 0) {
+        cnt3[ii] = (int)c3;
+        cnt4[ii] = (int)c4;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3- and 4-body: one thread per listed cluster.  Pair distances and vectors
+// as the reference forms them (ij, ik, jk / ij, ik, il, jk, jl, kl; x_b - x_a),
+// the terms of compute_3B / compute_4B, then the forces of every member and
+// the tallies of ev_tally_mb (energy and virial shared equally by the
+// members; the global part kept per owner atom).
+// ---------------------------------------------------------------------------
+template <int NB, int D>
+__global__ __launch_bounds__(128) void k_lmp_clusters(long n, const int *__restrict__ list, chx_lmp_args A,
+                                                      chx_model_dev M)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    constexpr int P = NB * (NB - 1) / 2;
+    int mem[NB], ty[NB];
+    double xm[NB][3];
+#pragma unroll
+    for (int a = 0; a < NB; a++) {
+        mem[a] = list[c * NB + a];
+        ty[a] = A.ctype[mem[a]];
+        for (int k = 0; k < 3; k++) xm[a][k] = A.x[3L * mem[a] + k];
+    }
+    double dist[P], dr[P][3];
+    {
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < NB; a++)
+#pragma unroll
+            for (int b = a + 1; b < NB; b++, q++) dist[q] = lmp_dist(xm[a], xm[b], dr[q]);
+    }
+    double Ec, fo[NB][3], So[6];
+    int flags = 0;
+    bool evaluated;
+    explicit_cluster<NB, D>(M, ty, dist, dr, Ec, fo, So, flags, evaluated, nullptr);
+    if (!evaluated) return;
+    bool bad = false;
+    double v[6];
+    lmp_order(So, v);
+#pragma unroll
+    for (int a = 0; a < NB; a++) {
+        for (int k = 0; k < 3; k++) fx_atomic(A.Facc + 2L * (3L * mem[a] + k), fo[a][k], bad);
+        if (A.Eacc) fx_atomic(A.Eacc + 2L * mem[a], Ec / NB, bad);
+        if (A.Vacc)
+            for (int k = 0; k < 6; k++) fx_atomic(A.Vacc + 2L * (6L * mem[a] + k), v[k] / NB, bad);
+    }
+    if (A.Gacc) {
+        if (A.eflag_global) fx_atomic(A.Gacc + 2L * (7L * mem[0]), Ec, bad);
+        if (A.vflag_global)
+            for (int k = 0; k < 6; k++) fx_atomic(A.Gacc + 2L * (7L * mem[0] + 1 + k), v[k], bad);
+    }
+    if (bad) atomicOr(&A.status[CHX_LMP_ST_FLAGS], CHX_LMP_BAD_RANGE);
+}
+
+// ---------------------------------------------------------------------------
+// Accumulators -> doubles: one thread per atom; block 0 also sums the global
+// energy and virial over the owner atoms (integer sums, exact).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lmp_out(chx_lmp_args A, double *F, double *eatom, double *vatom)
+{
+    const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A.nall) return;
+    for (int k = 0; k < 3; k++) {
+        fx_sum t;
+        t.add(A.Facc + 2 * (3 * a + k));
+        F[3 * a + k] = t.value();
+    }
+    if (A.Eacc) {
+        fx_sum t;
+        t.add(A.Eacc + 2 * a);
+        eatom[a] = t.value();
+    }
+    if (A.Vacc)
+        for (int k = 0; k < 6; k++) {
+            fx_sum t;
+            t.add(A.Vacc + 2 * (6 * a + k));
+            vatom[6 * a + k] = t.value();
+        }
+}
+
+__global__ __launch_bounds__(256) void k_lmp_global(chx_lmp_args A, double *glob)
+{
+    __shared__ long long sh[256], sl[256];
+    for (int k = 0; k < 7; k++) {
+        long long hi = 0, lo = 0;
+        for (long a = threadIdx.x; a < A.nall; a += 256) {
+            const unsigned long long *g = A.Gacc + 2 * (7 * a + k);
+            hi += (long long)g[0] + (long long)(g[1] >> 40);
+            lo += (long long)(g[1] & 0xffffffffffull);
+        }
+        sh[threadIdx.x] = hi;
+        sl[threadIdx.x] = lo;
+        __syncthreads();
+        for (int w = 128; w >= 1; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                sh[threadIdx.x] += sh[threadIdx.x + w];
+                sl[threadIdx.x] += sl[threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            fx_sum t;
+            t.hi = sh[0];
+            t.lo = (unsigned long long)sl[0];
+            glob[k] = t.value();
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_lmp_pair(const chx_lmp_args &A, const chx_model_dev &M, hipStream_t st)
+{
+    if (A.inum <= 0) return;
+    hipLaunchKernelGGL(k_lmp_pair, dim3((unsigned)((A.inum + 15) / 16)), dim3(256), 0, st, A, M);
+}
+
+void launch_lmp_lists(int mode, const chx_lmp_args &A, double pad3, double pad4, int has4, int *cnt3, int *cnt4,
+                      const long long *off3, const long long *off4, int *trips, int *quads, hipStream_t st)
+{
+    if (A.inum <= 0) return;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_lmp_lists<0>, dim3((unsigned)A.inum), dim3(64), 0, st, A, pad3, pad4, has4, cnt3,
+                           cnt4, off3, off4, trips, quads);
+    else
+        hipLaunchKernelGGL(k_lmp_lists<1>, dim3((unsigned)A.inum), dim3(64), 0, st, A, pad3, pad4, has4, cnt3,
+                           cnt4, off3, off4, trips, quads);
+}
+
+void launch_lmp_clusters(int nb, int D, long n, const int *list, const chx_lmp_args &A, const chx_model_dev &M,
+                         hipStream_t st)
+{
+    if (n <= 0) return;
+    const dim3 grid((unsigned)((n + 127) / 128)), block(128);
+#define X(d)                                                                                         \
+    if (D == d) {                                                                                    \
+        hipLaunchKernelGGL((k_lmp_clusters<3, d>), grid, block, 0, st, n, list, A, M);             \
+        return;                                                                                      \
+    }
+    if (nb == 3) { CHX_D3_CASES(X) }
+#undef X
+#define X(d)                                                                                         \
+    if (D == d) {                                                                                    \
+        hipLaunchKernelGGL((k_lmp_clusters<4, d>), grid, block, 0, st, n, list, A, M);             \
+        return;                                                                                      \
+    }
+    if (nb == 4) { CHX_D4_CASES(X) }
+#undef X
+}
+
+void launch_lmp_out(const chx_lmp_args &A, double *F, double *eatom, double *vatom, double *glob, hipStream_t st)
+{
+    if (A.nall > 0)
+        hipLaunchKernelGGL(k_lmp_out, dim3((unsigned)((A.nall + 255) / 256)), dim3(256), 0, st, A, F, eatom, vatom);
+    if (A.Gacc) hipLaunchKernelGGL(k_lmp_global, dim3(1), dim3(256), 0, st, A, glob);
+}
+
+}  // namespace chx

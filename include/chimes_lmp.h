@@ -1,0 +1,59 @@
+/*
+ * The LAMMPS pair style chimesFF (etc/lmp/src/pair_chimes.cpp) evaluated on
+ * the GPU: what PairCHIMES::compute does with one process's local and ghost
+ * atoms, behind plain arrays a pair style can hand over unchanged (x, type,
+ * tag, and the full neighbour list with ghosts: inum, ilist, numneigh,
+ * firstneigh).  Results are added onto the caller's force and tally arrays,
+ * as the reference's loops and Pair::ev_tally_mb (etc/lmp/etc/pair.cpp:
+ * 1425-1482) add onto LAMMPS's.  Every function returns 0 on success and a
+ * nonzero code on error (message in chimes_gpu_last_error()); none exits.
+ */
+#ifndef CHIMES_LMP_H
+#define CHIMES_LMP_H
+
+#include "chimes_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chimes_lmp chimes_lmp;
+
+/* PairCHIMES::coeff and set_chimes_type (pair_chimes.cpp:176-236, 833-879):
+ * LAMMPS types 1..ntypes are matched to the model's atom types by mass
+ * within 1e-3; mass[1..ntypes] as LAMMPS's atom->mass (mass[0] unused).
+ * Every type must match: the reference only warns when one does not, after
+ * which its type table is shifted for the later types. */
+chimes_lmp *chimes_lmp_create(chimes_gpu_model *m, int ntypes, const double *mass);
+void chimes_lmp_free(chimes_lmp *p);
+
+/* PairCHIMES::build_mb_neighlists (pair_chimes.cpp:321-478): the 3- and
+ * 4-mer lists within the largest 3-/4-body cutoffs plus the neighbour skin,
+ * kept on the GPU for the following compute calls. */
+int chimes_lmp_build_mb_neighlists(chimes_lmp *p, int nall, const double *x, const int *tag, int inum,
+                                   const int *ilist, const int *numneigh, int *const *firstneigh, double skin);
+
+/* PairCHIMES::compute (pair_chimes.cpp:480-831).  x[nall*3] (local atoms,
+ * then ghosts), type[nall] (LAMMPS types), tag[nall]; the lists are rebuilt
+ * first when ago == 0 (neighbor->ago) or none exist.  Adds onto f[nall*3]
+ * and, as the flags ask, *eng_vdwl, virial[6] (xx yy zz xy xz yz),
+ * eatom[nall], vatom[nall*6]; ghost forces are left for LAMMPS's reverse
+ * communication. */
+int chimes_lmp_compute(chimes_lmp *p, int nall, const double *x, const int *type, const int *tag, int inum,
+                       const int *ilist, const int *numneigh, int *const *firstneigh, int ago, double skin,
+                       int eflag_global, int eflag_atom, int vflag_global, int vflag_atom, double *f,
+                       double *eng_vdwl, double *virial, double *eatom, double *vatom);
+
+/* The many-body lists of the last build as atom indices in the reference's
+ * loop order: trips (i, j, k), quads (i, j, k, l); free with chimes_gpu_free. */
+int chimes_lmp_mb_lists(chimes_lmp *p, long *n3, int **trips, long *n4, int **quads);
+
+/* Milliseconds of the last compute (HIP events): [0] input copies, [1] list
+ * build (0 when the lists were reused), [2] 1- and 2-body, [3] 3-/4-body and
+ * the read-back of the results. */
+void chimes_lmp_timing(const chimes_lmp *p, double ms[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
