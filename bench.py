@@ -17,6 +17,11 @@ Other forms (one JSON line each):
                  the error of the mixed result per quantity.
   --workload C4  Poisson request stream served with continuous batching:
                  p50/p99 latency and throughput, FP64 and FP32-mixed.
+  --workload K   batched explicit 3-/4-body clusters (per-cluster API).
+  --workload L   the LAMMPS pair style (chimes_lmp.h): 4096 local carbon atoms
+                 (the reference's 512-atom diamond test box repeated 2x2x2),
+                 Carbon-2.0 2+3+4-body, skin 1.5 A; steps reuse the many-body
+                 lists as LAMMPS does between rebuilds (host arrays in and out).
 """
 import argparse
 import ctypes as C
@@ -121,8 +126,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4", "K"], default="C1",
-                    help="K: batched explicit 3-/4-body clusters (chimes_gpu_compute_clusters)")
+    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4", "K", "L"], default="C1",
+                    help="K: batched explicit 3-/4-body clusters (chimes_gpu_compute_clusters); "
+                         "L: the LAMMPS pair style (chimes_lmp.h)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--direct-only", action="store_true",
@@ -144,6 +150,8 @@ def main():
         return bench_c4(args)
     if args.workload == "K":
         return bench_clusters(args)
+    if args.workload == "L":
+        return bench_lmp(args)
     bench_c1(args)
 
 
@@ -412,6 +420,110 @@ def bench_clusters(args):
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": len(dist) / dt, "unit": "clusters/s", "cores": 1, "kind": "port",
                                "sample": "oracle oc_compute_4b on 20,000 of the same kind of clusters, one call each"}
+    print(json.dumps(out))
+
+
+def lmp_inputs(reps, skin):
+    """The reference's 512-atom diamond LAMMPS test box (tests/golden/lmp),
+    repeated reps^3 times, as one process's local atoms, ghosts and full
+    neighbour list."""
+    import gzip
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from lmp_setup import LmpSystem, read_lammps_data
+    G = os.path.join(ROOT, "tests", "golden", "lmp")
+    rec = json.load(open(os.path.join(G, "ev_tally_mb.json")))
+    pf = os.path.join(tempfile.mkdtemp(), "carbon.txt")
+    with gzip.open(os.path.join(G, rec["params"])) as a, open(pf, "wb") as b:
+        b.write(a.read())
+    x, typ, tag, lo, hi, masses = read_lammps_data(os.path.join(G, rec["data"]))
+    L = hi - lo
+    shifts = [np.array([i, j, k]) * L for i in range(reps) for j in range(reps) for k in range(reps)]
+    X = np.vstack([x + s for s in shifts])
+    T = np.tile(typ, len(shifts))
+    Tg = np.arange(1, len(X) + 1, dtype=np.int32)
+    return pf, masses, X, T, Tg, lo, lo + reps * L, LmpSystem
+
+
+def bench_lmp(args):
+    """The LAMMPS pair style: PairCHIMES::compute's work on one process's
+    local and ghost atoms, as a LAMMPS step hands it over."""
+    import chimes_calculator_amd as cc
+
+    skin = 1.5
+    pf, masses, X, T, Tg, lo, hi, LmpSystem = lmp_inputs(2, skin)
+    model = cc.GpuModel(pf)
+    rc2 = np.array([[model.max_cutoff(2)]])
+    S = LmpSystem(X, T, Tg, lo, hi, [0], rc2, skin)
+    P = cc.LmpPair(model, masses)
+    args_c = (S.x, S.type, S.tag, S.ilist, S.numneigh, S.nbr_off, S.nbr)
+    t0 = time.perf_counter()
+    P.compute(*args_c, ago=0, skin=skin)
+    t_build_call = time.perf_counter() - t0
+    t_build = P.timing()
+    g3, g4 = P.mb_lists()
+    # prebuilt row pointers, as a C++ caller passes LAMMPS's firstneigh as is
+    L = cc.lib()
+    nall = S.nall
+    fn, keep = cc.LmpPair._rows(nall, S.ilist, S.nbr_off, S.nbr)
+    f, vir, ea, va = np.zeros(nall * 3), np.zeros(6), np.zeros(nall), np.zeros(nall * 6)
+    eng = C.c_double(0.0)
+    xs = np.ascontiguousarray(S.x.reshape(-1))
+
+    def step(ago, fl):
+        if L.chimes_lmp_compute(P.h, nall, xs, S.type, S.tag, S.inum, S.ilist, S.numneigh, fn, ago, skin,
+                                fl[0], fl[1], fl[2], fl[3], f, C.byref(eng), vir, ea, va):
+            raise cc.ChimesError(cc._err())
+
+    # tally flags (eflag_global, eflag_atom, vflag_global, vflag_atom) of
+    # three kinds of LAMMPS step: forces only, a thermo step (energy and
+    # pressure), and a step whose computes ask for per-atom energy and stress
+    kinds = {"forces": (0, 0, 0, 0), "forces_energy_virial": (1, 0, 1, 0), "all_tallies": (1, 1, 1, 1)}
+    forms = {}
+    for name, fl in kinds.items():
+        for _ in range(args.warmup):
+            step(1, fl)
+        ts, kt = [], []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            step(1, fl)
+            ts.append(time.perf_counter() - t0)
+            kt.append(P.timing())
+        msk = 1e3 * float(np.median(ts))
+        kt = np.median(np.array(kt), 0)
+        forms[name] = {"value": S.inum / (msk * 1e-3), "ms_per_step": msk,
+                       "phases_ms": {"positions_in": float(kt[0]), "pair_1_2_body": float(kt[2]),
+                                     "many_body_and_readback": float(kt[3])}}
+    ms = forms["forces_energy_virial"]["ms_per_step"]
+    kt = forms["forces_energy_virial"]["phases_ms"]
+    out = {"metric": "atom-evals/s (LAMMPS pair style: E, F, virial; lists reused between rebuilds)",
+           "value": S.inum / (ms * 1e-3), "unit": "atom-evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "the reference's 512-atom diamond LAMMPS test box repeated 2x2x2 (4096 local atoms)",
+           "config": {"workload": "4096 local carbon atoms + %d ghosts, Carbon-2.0 2+3+4-body, skin %.1f A, "
+                                  "full neighbour list with ghosts" % (nall - S.inum, skin),
+                      "local_atoms": int(S.inum), "ghost_atoms": int(nall - S.inum),
+                      "neighbour_entries": int(len(S.nbr)), "triplets": int(len(g3)), "quadruplets": int(len(g4))},
+           "phases_ms": kt, "forms": forms,
+           "rebuild_step_ms": {"call": 1e3 * t_build_call, "lists": float(t_build[1])}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bindings import LmpOracle
+        from lmp_setup import read_lammps_data, match_types
+        O = LmpOracle(pf)
+        pf1, masses1, x1, t1, g1, lo1, hi1, _ = lmp_inputs(1, skin)
+        S1 = LmpSystem(x1, t1, g1, lo1, hi1, match_types(masses1, O.masses()), O.rc2(), skin)
+        tr, qu = O.build_lists(S1)
+        t0 = time.perf_counter()
+        O.compute(S1, tr, qu)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": S1.inum / dt, "unit": "atom-evals/s", "cores": 1, "kind": "port",
+                               "sample": "the oracle's restatement of PairCHIMES::compute on the 512-atom box "
+                                         "(one step, lists prebuilt)"}
+    del keep
+    P.close()
+    model.close()
     print(json.dumps(out))
 
 

@@ -91,8 +91,18 @@ __device__ __forceinline__ void explicit_cluster(const chx_model_dev &M, const i
                     fcut(rq[q], sl[q].rout, M.tersoff, M.fcut_var, f[q], fd[q]);
                 }
                 double S, G[P], fs[P];
-                if constexpr (NB == 3) contract3<D, double, double>(M.C3 + (long)tau * D * D * D, T, dT, S, G);
-                else contract4<D, double, double>(M.C4 + (long)tau * D * D * D * D * D * D, T, dT, S, G);
+                // when every active lane has the same cluster type (always for
+                // one-element models) the coefficient address is wave-uniform
+                // and the contraction reads it through scalar loads
+                const int tu = __builtin_amdgcn_readfirstlane(tau);
+                const long csz = NB == 3 ? (long)D * D * D : (long)D * D * D * D * D * D;
+                if (__ballot(tau != tu) == 0) {
+                    if constexpr (NB == 3) contract3<D, double, double>(M.C3 + tu * csz, T, dT, S, G);
+                    else contract4<D, double, double>(M.C4 + tu * csz, T, dT, S, G);
+                } else {
+                    if constexpr (NB == 3) contract3<D, double, double>(M.C3 + tau * csz, T, dT, S, G);
+                    else contract4<D, double, double>(M.C4 + tau * csz, T, dT, S, G);
+                }
                 cluster_scalars<P, double>(rq, f, fd, S, G, Ec, fs);
 #pragma unroll
                 for (int q = 0; q < P; q++)

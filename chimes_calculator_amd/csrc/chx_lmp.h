@@ -36,7 +36,9 @@ void launch_lmp_lists(int mode, const chx_lmp_args &A, double pad3, double pad4,
                       const long long *off3, const long long *off4, int *trips, int *quads, hipStream_t st);
 void launch_lmp_clusters(int nb, int D, long n, const int *list, const chx_lmp_args &A, const chx_model_dev &M,
                          hipStream_t st);
-// F[nall*3], eatom[nall] (if Eacc), vatom[nall*6] (if Vacc), glob[7] (if Gacc)
-void launch_lmp_out(const chx_lmp_args &A, double *F, double *eatom, double *vatom, double *glob, hipStream_t st);
+// F[nall*3], eatom[nall] (if Eacc), vatom[nall*6] (if Vacc), glob[7] (if
+// Gacc; gsum[14] and *done zeroed scratch of the global reduction)
+void launch_lmp_out(const chx_lmp_args &A, double *F, double *eatom, double *vatom, double *glob,
+                    unsigned long long *gsum, int *done, hipStream_t st);
 
 }  // namespace chx

@@ -1676,17 +1676,27 @@ int chx_lmp_compute(chx_lmp *p, int nall, const double *x, const int *type, cons
             throw EngineError("missing tally array");
         hipStream_t st = p->st;
         HIPCHK(hipEventRecord(p->ev[0], st));
-        lmp_upload(p, nall, x, type, tag, inum, ilist, numneigh, firstneigh);
+        // Between rebuilds (ago > 0) LAMMPS keeps its neighbour list, tags,
+        // types and ilist unchanged, so only the positions are sent again.
+        const bool rebuild = ago == 0 || !p->have_lists || p->lists_inum != inum || p->lists_nall != nall;
+        if (rebuild) {
+            lmp_upload(p, nall, x, type, tag, inum, ilist, numneigh, firstneigh);
+        } else {
+            if (!x) throw EngineError("missing input array");
+            HIPCHK(hipMemcpyAsync(p->x.p, x, sizeof(double) * nall * 3, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemsetAsync(p->status.p, 0, sizeof(int) * CHX_LMP_ST_WORDS, st));
+        }
         HIPCHK(hipEventRecord(p->ev[1], st));
-        if (ago == 0 || !p->have_lists || p->lists_inum != inum || p->lists_nall != nall) lmp_build(p, skin);
+        if (rebuild) lmp_build(p, skin);
         HIPCHK(hipEventRecord(p->ev[2], st));
         const chx_model *m = p->m;
         // accumulators: forces, per-atom energy, per-atom virial, per-owner global
         const size_t nF = (size_t)nall * 6, nE = eflag_atom ? (size_t)nall * 2 : 0,
                      nV = vflag_atom ? (size_t)nall * 12 : 0,
                      nG = (eflag_global || vflag_global) ? (size_t)nall * 14 : 0;
-        p->acc.ensure(nF + nE + nV + nG);
-        HIPCHK(hipMemsetAsync(p->acc.p, 0, sizeof(unsigned long long) * (nF + nE + nV + nG), st));
+        // plus the global reduction's sums (14 words) and block counter (1)
+        p->acc.ensure(nF + nE + nV + nG + 15);
+        HIPCHK(hipMemsetAsync(p->acc.p, 0, sizeof(unsigned long long) * (nF + nE + nV + nG + 15), st));
         chx_lmp_args A = p->args();
         A.Facc = p->acc.p;
         A.Eacc = nE ? A.Facc + nF : nullptr;
@@ -1704,7 +1714,8 @@ int chx_lmp_compute(chx_lmp *p, int nall, const double *x, const int *type, cons
         p->out.ensure(nout + CHX_LMP_ST_WORDS);
         double *F = p->out.p, *Ea = F + (size_t)nall * 3, *Va = Ea + (nE ? nall : 0),
                *G = Va + (nV ? (size_t)nall * 6 : 0);
-        launch_lmp_out(A, F, nE ? Ea : nullptr, nV ? Va : nullptr, G, st);
+        unsigned long long *gsum = p->acc.p + nF + nE + nV + nG;
+        launch_lmp_out(A, F, nE ? Ea : nullptr, nV ? Va : nullptr, G, gsum, (int *)(gsum + 14), st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(p->out.p + nout, p->status.p, sizeof(int) * CHX_LMP_ST_WORDS, hipMemcpyDeviceToDevice,
                               st));

@@ -10,6 +10,8 @@
 // depend on the order of the lists or on the kernel schedule.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "chx_cluster.h"
 #include "chx_dev.h"
 #include "chx_lmp.h"
@@ -226,18 +228,32 @@ __global__ __launch_bounds__(64) void k_lmp_lists(chx_lmp_args A, double pad3, d
 // the tallies of ev_tally_mb (energy and virial shared equally by the
 // members; the global part kept per owner atom).
 // ---------------------------------------------------------------------------
+// One value of the owner atom: summed over the wavefront's runs of equal keys
+// (the lists are sorted by owner) and added by each run's last lane at a.
+__device__ __forceinline__ void lmp_run_add(int key, int seg, bool last, double v, unsigned long long *a, bool &bad)
+{
+    long long x[2] = {0, 0};
+    if (key >= 0) fx_split(v, x[0], x[1], bad);
+    fx_run_scan<64, 2>(x, seg);
+    if (last) {
+        if (x[0]) atomicAdd(a, (unsigned long long)x[0]);
+        if (x[1]) atomicAdd(a + 1, (unsigned long long)x[1]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 template <int NB, int D>
 __global__ __launch_bounds__(128) void k_lmp_clusters(long n, const int *__restrict__ list, chx_lmp_args A,
                                                       chx_model_dev M)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    const bool valid = c < n;
     constexpr int P = NB * (NB - 1) / 2;
     int mem[NB], ty[NB];
     double xm[NB][3];
 #pragma unroll
     for (int a = 0; a < NB; a++) {
-        mem[a] = list[c * NB + a];
+        mem[a] = valid ? list[c * NB + a] : 0;
         ty[a] = A.ctype[mem[a]];
         for (int k = 0; k < 3; k++) xm[a][k] = A.x[3L * mem[a] + k];
     }
@@ -249,25 +265,40 @@ __global__ __launch_bounds__(128) void k_lmp_clusters(long n, const int *__restr
 #pragma unroll
             for (int b = a + 1; b < NB; b++, q++) dist[q] = lmp_dist(xm[a], xm[b], dr[q]);
     }
-    double Ec, fo[NB][3], So[6];
+    double Ec = 0.0, fo[NB][3], So[6];
     int flags = 0;
-    bool evaluated;
-    explicit_cluster<NB, D>(M, ty, dist, dr, Ec, fo, So, flags, evaluated, nullptr);
-    if (!evaluated) return;
+    bool evaluated = false;
+    if (valid) explicit_cluster<NB, D>(M, ty, dist, dr, Ec, fo, So, flags, evaluated, nullptr);
     bool bad = false;
     double v[6];
     lmp_order(So, v);
+    // members 1.. : exact atomics per cluster
+    if (evaluated) {
 #pragma unroll
-    for (int a = 0; a < NB; a++) {
-        for (int k = 0; k < 3; k++) fx_atomic(A.Facc + 2L * (3L * mem[a] + k), fo[a][k], bad);
-        if (A.Eacc) fx_atomic(A.Eacc + 2L * mem[a], Ec / NB, bad);
-        if (A.Vacc)
-            for (int k = 0; k < 6; k++) fx_atomic(A.Vacc + 2L * (6L * mem[a] + k), v[k] / NB, bad);
+        for (int a = 1; a < NB; a++) {
+            for (int k = 0; k < 3; k++) fx_atomic(A.Facc + 2L * (3L * mem[a] + k), fo[a][k], bad);
+            if (A.Eacc) fx_atomic(A.Eacc + 2L * mem[a], Ec / NB, bad);
+            if (A.Vacc)
+                for (int k = 0; k < 6; k++) fx_atomic(A.Vacc + 2L * (6L * mem[a] + k), v[k] / NB, bad);
+        }
     }
-    if (A.Gacc) {
-        if (A.eflag_global) fx_atomic(A.Gacc + 2L * (7L * mem[0]), Ec, bad);
-        if (A.vflag_global)
-            for (int k = 0; k < 6; k++) fx_atomic(A.Gacc + 2L * (7L * mem[0] + 1 + k), v[k], bad);
+    // member 0, the owner: runs of equal owners over the wavefront
+    const int key = evaluated ? mem[0] : -1;
+    if (__ballot(key >= 0)) {
+        const int seg = fx_run_first<64>(key);
+        const int l = threadIdx.x & 63;
+        const int knext = __shfl_down(key, 1, 64);
+        const bool last = key >= 0 && (l == 63 || knext != key);
+        const long o = key >= 0 ? key : 0;
+        for (int k = 0; k < 3; k++) lmp_run_add(key, seg, last, fo[0][k], A.Facc + 2L * (3L * o + k), bad);
+        if (A.Eacc) lmp_run_add(key, seg, last, Ec / NB, A.Eacc + 2L * o, bad);
+        if (A.Vacc)
+            for (int k = 0; k < 6; k++) lmp_run_add(key, seg, last, v[k] / NB, A.Vacc + 2L * (6L * o + k), bad);
+        if (A.Gacc) {
+            if (A.eflag_global) lmp_run_add(key, seg, last, Ec, A.Gacc + 2L * (7L * o), bad);
+            if (A.vflag_global)
+                for (int k = 0; k < 6; k++) lmp_run_add(key, seg, last, v[k], A.Gacc + 2L * (7L * o + 1 + k), bad);
+        }
     }
     if (bad) atomicOr(&A.status[CHX_LMP_ST_FLAGS], CHX_LMP_BAD_RANGE);
 }
@@ -298,12 +329,17 @@ __global__ __launch_bounds__(256) void k_lmp_out(chx_lmp_args A, double *F, doub
         }
 }
 
-__global__ __launch_bounds__(256) void k_lmp_global(chx_lmp_args A, double *glob)
+// global energy and virial: each block sums a strided share of the owner
+// atoms' words (integer sums, exact) and adds them into gsum[7][2]; the
+// last block to finish converts them
+__global__ __launch_bounds__(256) void k_lmp_global(chx_lmp_args A, unsigned long long *gsum, int *done, double *glob)
 {
     __shared__ long long sh[256], sl[256];
+    __shared__ bool is_last;
+    const long stride = (long)gridDim.x * 256;
     for (int k = 0; k < 7; k++) {
         long long hi = 0, lo = 0;
-        for (long a = threadIdx.x; a < A.nall; a += 256) {
+        for (long a = (long)blockIdx.x * 256 + threadIdx.x; a < A.nall; a += stride) {
             const unsigned long long *g = A.Gacc + 2 * (7 * a + k);
             hi += (long long)g[0] + (long long)(g[1] >> 40);
             lo += (long long)(g[1] & 0xffffffffffull);
@@ -319,12 +355,24 @@ __global__ __launch_bounds__(256) void k_lmp_global(chx_lmp_args A, double *glob
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            fx_sum t;
-            t.hi = sh[0];
-            t.lo = (unsigned long long)sl[0];
-            glob[k] = t.value();
+            if (sh[0]) atomicAdd(gsum + 2 * k, (unsigned long long)sh[0]);
+            if (sl[0]) atomicAdd(gsum + 2 * k + 1, (unsigned long long)sl[0]);
         }
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        is_last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (is_last && threadIdx.x < 7) {
+        __threadfence();
+        const unsigned long long h = __atomic_load_n(gsum + 2 * threadIdx.x, __ATOMIC_RELAXED);
+        const unsigned long long l = __atomic_load_n(gsum + 2 * threadIdx.x + 1, __ATOMIC_RELAXED);
+        fx_sum t;
+        t.hi = (long long)h + (long long)(l >> 40);
+        t.lo = l & 0xffffffffffull;
+        glob[threadIdx.x] = t.value();
     }
 }
 
@@ -370,11 +418,15 @@ void launch_lmp_clusters(int nb, int D, long n, const int *list, const chx_lmp_a
 #undef X
 }
 
-void launch_lmp_out(const chx_lmp_args &A, double *F, double *eatom, double *vatom, double *glob, hipStream_t st)
+void launch_lmp_out(const chx_lmp_args &A, double *F, double *eatom, double *vatom, double *glob,
+                    unsigned long long *gsum, int *done, hipStream_t st)
 {
     if (A.nall > 0)
         hipLaunchKernelGGL(k_lmp_out, dim3((unsigned)((A.nall + 255) / 256)), dim3(256), 0, st, A, F, eatom, vatom);
-    if (A.Gacc) hipLaunchKernelGGL(k_lmp_global, dim3(1), dim3(256), 0, st, A, glob);
+    if (A.Gacc) {
+        const unsigned nb = (unsigned)std::min<long>(256, std::max<long>(1, ((long)A.nall + 1023) / 1024));
+        hipLaunchKernelGGL(k_lmp_global, dim3(nb), dim3(256), 0, st, A, gsum, done, glob);
+    }
 }
 
 }  // namespace chx

@@ -35,7 +35,9 @@ int chimes_lmp_build_mb_neighlists(chimes_lmp *p, int nall, const double *x, con
 
 /* PairCHIMES::compute (pair_chimes.cpp:480-831).  x[nall*3] (local atoms,
  * then ghosts), type[nall] (LAMMPS types), tag[nall]; the lists are rebuilt
- * first when ago == 0 (neighbor->ago) or none exist.  Adds onto f[nall*3]
+ * first when ago == 0 (neighbor->ago) or none exist.  Between rebuilds only x
+ * is read again: LAMMPS keeps the neighbour list, tags, types and ilist
+ * unchanged until it rebuilds the list.  Adds onto f[nall*3]
  * and, as the flags ask, *eng_vdwl, virial[6] (xx yy zz xy xz yz),
  * eatom[nall], vatom[nall*6]; ghost forces are left for LAMMPS's reverse
  * communication. */
