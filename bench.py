@@ -18,6 +18,9 @@ Other forms (one JSON line each):
   --workload C4  Poisson request stream served with continuous batching:
                  p50/p99 latency and throughput, FP64 and FP32-mixed.
   --workload K   batched explicit 3-/4-body clusters (per-cluster API).
+  --workload F   cluster fingerprint histogram (calc_histograms.cpp): all
+                 117k triplets of a 128-atom C/O configuration against each
+                 other (6.8e9 cluster pairs), edges in host memory.
   --workload L   the LAMMPS pair style (chimes_lmp.h): 4096 local carbon atoms
                  (the reference's 512-atom diamond test box repeated 2x2x2),
                  Carbon-2.0 2+3+4-body, skin 1.5 A; steps reuse the many-body
@@ -126,9 +129,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4", "K", "L"], default="C1",
+    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4", "K", "L", "F"], default="C1",
                     help="K: batched explicit 3-/4-body clusters (chimes_gpu_compute_clusters); "
-                         "L: the LAMMPS pair style (chimes_lmp.h)")
+                         "L: the LAMMPS pair style (chimes_lmp.h); F: fingerprint histogram")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--direct-only", action="store_true",
@@ -152,6 +155,8 @@ def main():
         return bench_clusters(args)
     if args.workload == "L":
         return bench_lmp(args)
+    if args.workload == "F":
+        return bench_fp(args)
     bench_c1(args)
 
 
@@ -523,6 +528,95 @@ def bench_lmp(args):
                                          "(one step, lists prebuilt)"}
     del keep
     P.close()
+    model.close()
+    print(json.dumps(out))
+
+
+def fp_rows():
+    """Triplet rows (distances ij ik jk, types) of a 128-atom C/O configuration
+    under the LAMMPS neighbour rules, and the force field's path."""
+    import gzip
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from lmp_setup import LmpSystem
+    from oracle_bindings import read_xyz
+    import chimes_calculator_amd as cc
+    G = os.path.join(ROOT, "tests", "golden", "ref_inputs")
+    d = tempfile.mkdtemp()
+
+    def ungz(name):
+        out = os.path.join(d, name[:-3])
+        with gzip.open(os.path.join(G, name)) as a, open(out, "wb") as b:
+            b.write(a.read())
+        return out
+
+    pf = ungz("published_params.CO2400K.2+3+4b.Tersoff.special.offsets.txt.gz")
+    pos, names, cell = read_xyz(ungz("CO.9GPa_2400K.OUTCAR_#000.xyz.gz"))
+    Lv = np.diag(np.asarray(cell, float).reshape(3, 3))
+    pos = np.mod(pos, Lv)
+    model = cc.GpuModel(pf)
+    ct = model.types_to_index(names)
+    nt = model.ntypes
+    rc2 = np.full((nt, nt), model.max_cutoff(2))
+    S = LmpSystem(pos, ct + 1, np.arange(1, len(pos) + 1, dtype=np.int32), np.zeros(3), Lv,
+                  np.arange(nt, dtype=np.int32), rc2, 0.5)
+    # triplets as build_mb_neighlists lists them (tag order), within the 3-body cutoff plus the skin
+    rc3 = model.max_cutoff(3) + 0.5
+    T = []
+    for ii in range(S.inum):
+        i = S.ilist[ii]
+        r = S.nbr[S.nbr_off[ii]:S.nbr_off[ii + 1]]
+        d_i = np.linalg.norm(S.x[r] - S.x[i], axis=1)
+        J = r[(S.tag[r] >= S.tag[i]) & (d_i < rc3)]
+        for a in range(len(J)):
+            for b in range(len(J)):
+                j, k = J[a], J[b]
+                if j != k and S.tag[k] >= S.tag[j] and np.linalg.norm(S.x[k] - S.x[j]) < rc3:
+                    T.append((i, j, k))
+    T = np.array(T, np.int32)
+    dist = np.stack([np.linalg.norm(S.x[T[:, b]] - S.x[T[:, a]], axis=1) for a, b in ((0, 1), (0, 2), (1, 2))], 1)
+    return model, dist, S.ctype[T]
+
+
+def bench_fp(args):
+    """Cluster fingerprint histogram: every triplet against every later one."""
+    model, dist, types = fp_rows()
+    # excluded cluster types are not dumped: keep the tuples the model evaluates
+    keep = np.ones(len(types), bool)
+    for t in set(map(tuple, types)):
+        if model.L.chimes_gpu_fp_edges(model.h, 3, 1, np.zeros(3), np.asarray(t, np.int32), np.zeros(3)):
+            keep &= ~np.all(types == np.asarray(t), axis=1)
+    dist, types = dist[keep], types[keep]
+    t0 = time.perf_counter()
+    E = model.fp_edges(3, dist, types)
+    t_edges = time.perf_counter() - t0
+    n = len(E)
+    for _ in range(args.warmup):
+        model.fp_histogram(3, E, nbin=100)
+    ts = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        counts, ns, binw = model.fp_histogram(3, E, nbin=100)
+        ts.append(time.perf_counter() - t0)
+    ms = 1e3 * float(np.median(ts))
+    out = {"metric": "cluster pairs binned/s (fingerprint histogram, 3-body, one set against itself)",
+           "value": ns / (ms * 1e-3), "unit": "cluster-pairs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "triplets of the reference's 128-atom C/O test configuration (CO.9GPa_2400K #000), "
+                   "CO2400K force field, LAMMPS tag rule, skin 0.5 A",
+           "config": {"workload": "%d triplets, all pairs j > i (%d samples), 100 bins" % (n, ns),
+                      "clusters": n, "samples": int(ns)},
+           "edges_ms": 1e3 * t_edges}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bindings import fp_hist
+        m = 12000
+        t0 = time.perf_counter()
+        _, ns_cpu, _ = fp_hist(3, E[:m], nbin=100)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": ns_cpu / dt, "unit": "cluster-pairs/s", "cores": 1, "kind": "port",
+                               "sample": "the oracle's gen_flat_hists on the first %d triplets (%d pairs)" % (m, ns_cpu)}
     model.close()
     print(json.dumps(out))
 

@@ -44,7 +44,8 @@ EXPORTS = {
                      "chimes_gpu_server_result", "chimes_gpu_server_stats", "chimes_gpu_server_last_error",
                      "chimes_gpu_server_free",
                      "chimes_gpu_last_timing", "chimes_gpu_last_counts", "chimes_gpu_compute_clusters",
-                     "chimes_gpu_cluster_lists", "chimes_gpu_free"],
+                     "chimes_gpu_cluster_lists", "chimes_gpu_free", "chimes_gpu_fp_edges",
+                     "chimes_gpu_fp_histogram"],
     "chimes_lmp.h": ["chimes_lmp_create", "chimes_lmp_free", "chimes_lmp_build_mb_neighlists",
                      "chimes_lmp_compute", "chimes_lmp_mb_lists", "chimes_lmp_timing"],
 }
@@ -115,6 +116,10 @@ def lib():
         L.chimes_gpu_cluster_lists.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), pp,
                                                C.POINTER(C.c_long), pp, C.POINTER(C.c_long), pp]
         L.chimes_gpu_free.argtypes = [C.c_void_p]
+        L.chimes_gpu_fp_edges.argtypes = [C.c_void_p, C.c_int, C.c_long, _dp, _ip, _dp]
+        _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.chimes_gpu_fp_histogram.argtypes = [C.c_void_p, C.c_int, C.c_long, _dp, C.c_long, _dp, C.c_int, C.c_int,
+                                              C.c_double, C.c_double, _lp, C.POINTER(C.c_longlong)]
         ipp = C.POINTER(C.POINTER(C.c_int))
         L.chimes_lmp_create.restype = C.c_void_p
         L.chimes_lmp_create.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -192,6 +197,36 @@ class GpuModel:
         if self.L.chimes_gpu_compute_clusters(self.h, nb, n, dist.reshape(-1), dr, types, E, F, S):
             raise ChimesError(_err())
         return E, F.reshape(n, nb, 3), S.reshape(n, 6)
+
+    def fp_edges(self, nb, dist, types):
+        """Sorted Morse-transformed edges of dumped cluster rows (calc_histograms.cpp:140-249)."""
+        dist = np.ascontiguousarray(dist, np.float64).reshape(-1)
+        types = np.ascontiguousarray(types, np.int32).reshape(-1)
+        n = types.size // nb
+        out = np.zeros(max(dist.size, 1))
+        if self.L.chimes_gpu_fp_edges(self.h, nb, n, dist if dist.size else np.zeros(1), types
+                                      if types.size else np.zeros(1, np.int32), out):
+            raise ChimesError(_err())
+        return out[: dist.size].reshape(n, -1)
+
+    def fp_histogram(self, nb, edges1, edges2=None, nbin=100, maxd=None):
+        """(counts[nbin], nsamples, binw) of the cluster-distance histogram of
+        gen_flat_hists (calc_histograms.cpp:351-489); edges2=None: one set,
+        pairs j > i.  maxd defaults to the reference's sqrt(P * 2^2) + 1."""
+        P = nb * (nb - 1) // 2
+        if maxd is None:
+            maxd = (P * 4.0) ** 0.5 + 1.0
+        binw = maxd / nbin
+        e1 = np.ascontiguousarray(edges1, np.float64).reshape(-1)
+        same = edges2 is None
+        e2 = e1 if same else np.ascontiguousarray(edges2, np.float64).reshape(-1)
+        counts = np.zeros(nbin, np.int64)
+        ns = C.c_longlong(0)
+        if self.L.chimes_gpu_fp_histogram(self.h, nb, e1.size // P, e1 if e1.size else np.zeros(1), e2.size // P,
+                                          e2 if e2.size else np.zeros(1), int(same), nbin, binw, maxd, counts,
+                                          C.byref(ns)):
+            raise ChimesError(_err())
+        return counts, ns.value, binw
 
     def cluster_lists(self, conf=0):
         """Sorted canonical cluster keys of configuration `conf` of the last evaluation."""

@@ -123,6 +123,12 @@ void chx_last_counts(chx_engine *e, long long *c3);
 // [0] whole pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] enumeration.
 void chx_last_timing(chx_engine *e, double *ms4);
 
+// Cluster fingerprint histogram (calc_histograms.cpp:351-489): counts[nbin]
+// of the distances between the np-edge clusters e1[n1] and e2[n2] (host
+// arrays; same != 0: one set, pairs j > i).  ms: kernel milliseconds.
+int chx_fp_histogram(chx_engine *e, int np, long n1, const double *e1, long n2, const double *e2, int same,
+                     int nbin, double binw, double maxd, long long *counts, double *ms);
+
 // LAMMPS pair style (etc/lmp/src/pair_chimes.cpp) over one process's local
 // and ghost atoms.  chimes_type[t - 1] is the ChIMES type of LAMMPS type t.
 // x[nall*3], type[nall] (LAMMPS types), tag[nall], ilist[inum] and the

@@ -29,6 +29,7 @@
 #include "chx_engine.h"
 #include "chx_eval.h"
 #include "chx_kernels.h"
+#include "chx_fp.h"
 #include "chx_lmp.h"
 
 using namespace chx;
@@ -1774,3 +1775,46 @@ void chx_lmp_timing(chx_lmp *p, double *ms4)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Cluster fingerprint histogram (chimesFF/src/FP/calc_histograms.cpp:351-489)
+// ---------------------------------------------------------------------------
+extern "C" int chx_fp_histogram(chx_engine *e, int np, long n1, const double *e1, long n2, const double *e2,
+                                int same, int nbin, double binw, double maxd, long long *counts, double *ms)
+{
+    try {
+        HIPCHK(hipSetDevice(e->device));
+        if (np != 1 && np != 3 && np != 6) throw EngineError("clusters must have 1, 3 or 6 edges");
+        if (nbin <= 0 || nbin > CHX_FP_MAX_BINS)
+            throw EngineError("number of bins must be in 1.." + std::to_string(CHX_FP_MAX_BINS));
+        if (!(binw > 0.0)) throw EngineError("bin width must be positive");
+        if (n1 < 0 || n2 < 0 || (same && n1 != n2)) throw EngineError("cluster counts out of range");
+        hipStream_t st = e->st;
+        DevBuf<double> d1, d2;
+        DevBuf<unsigned long long> cnt;
+        d1.ensure(std::max<size_t>((size_t)n1 * np, 1));
+        if (!same) d2.ensure(std::max<size_t>((size_t)n2 * np, 1));
+        cnt.ensure((size_t)nbin + 1);
+        const double *g2 = same ? d1.p : d2.p;
+        if (n1) HIPCHK(hipMemcpyAsync(d1.p, e1, sizeof(double) * n1 * np, hipMemcpyHostToDevice, st));
+        if (!same && n2) HIPCHK(hipMemcpyAsync(d2.p, e2, sizeof(double) * n2 * np, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * (nbin + 1), st));
+        HIPCHK(hipEventRecord(e->ev[0], st));
+        if (!launch_fp_hist(np, n1, d1.p, n2, g2, same, nbin, binw, maxd, cnt.p, (int *)(cnt.p + nbin), st))
+            throw EngineError("cluster sets too large for one histogram launch");
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e->ev[1], st));
+        std::vector<unsigned long long> h((size_t)nbin + 1);
+        HIPCHK(hipMemcpyAsync(h.data(), cnt.p, sizeof(unsigned long long) * (nbin + 1), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (h[nbin]) throw EngineError("a cluster distance falls outside the histogram");
+        for (int b = 0; b < nbin; b++) counts[b] = (long long)h[b];
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));
+        if (ms) *ms = t;
+        return 0;
+    } catch (const EngineError &x) {
+        e->err = x.msg;
+        return 1;
+    }
+}

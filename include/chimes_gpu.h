@@ -141,6 +141,25 @@ int chimes_gpu_cluster_lists(chimes_gpu_model *m, int conf, long *n2, int **pair
                              long *n4, int **quads);
 void chimes_gpu_free(void *p);
 
+/* Cluster fingerprints (chimesFF/src/FP/calc_histograms.cpp).
+ * chimes_gpu_fp_edges: rows of a cluster dump - the nb(nb-1)/2 pair
+ * distances dist[n*P] (pair order 01 02 03 12 13 23) and the nb type indices
+ * types[n*nb] of each cluster, as chimesFF's FINGERPRINT mode records them
+ * (chimesFF.cpp:1694-1703, 2013-2020, 2603-2612) - become edges[n*P], the
+ * sorted Morse-transformed distances with the cutoffs of each pair's slot
+ * (read_flat_clusters, calc_histograms.cpp:140-249).
+ * chimes_gpu_fp_histogram: counts[nbin] of the distances between every
+ * cluster of edges1[n1] and every cluster of edges2[n2] - or, with same != 0,
+ * every pair j > i of edges1 - binned as get_bin does with bin width binw and
+ * top value maxd (gen_flat_hists, calc_histograms.cpp:351-489; the reference
+ * divides the counts by nsamples when it writes the .hist file).  A distance
+ * outside the nbin bins (at most 8192) is an error; the reference leaves the
+ * case bin == nbin unchecked. */
+int chimes_gpu_fp_edges(chimes_gpu_model *m, int nb, long n, const double *dist, const int *types, double *edges);
+int chimes_gpu_fp_histogram(chimes_gpu_model *m, int nb, long n1, const double *edges1, long n2,
+                            const double *edges2, int same, int nbin, double binw, double maxd, long long *counts,
+                            long long *nsamples);
+
 #ifdef __cplusplus
 }
 #endif

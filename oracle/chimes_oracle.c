@@ -739,6 +739,41 @@ void oc_compute_1b(const oc_params *P, int t, double *energy) { *energy += P->eo
 
 double oc_mass(const oc_params *P, int t) { return P->mass[t]; }
 
+/* Inner and outer cutoff of runtime pair p of an nb-body cluster of the
+ * given member types: the pair's own cutoffs for nb = 2, else the cluster
+ * type's cutoffs at the parameter slot the pair maps to (the cutoffs
+ * compute_3B/4B test, chimesFF.cpp:1990-2011, 2560-2600).  Nonzero for an
+ * excluded cluster type. */
+int oc_cluster_pair_cutoffs(const oc_params *P, int nb, const int *t, int p, double *rin, double *rout)
+{
+    const int nt = P->nt;
+    if (nb == 2) {
+        const int pair = P->pairmap[t[0] * nt + t[1]];
+        *rin = P->rin[pair];
+        *rout = P->rout[pair];
+        return 0;
+    }
+    if (nb == 3) {
+        const int tix = (t[0] * nt + t[1]) * nt + t[2];
+        const int trip = P->tripmap[tix];
+        if (trip < 0) return 1;
+        const int m = P->trip_pm[tix * 3 + p];
+        *rin = P->cut3[trip][0][m];
+        *rout = P->cut3[trip][1][m];
+        return 0;
+    }
+    const int tix = ((t[0] * nt + t[1]) * nt + t[2]) * nt + t[3];
+    const int quad = P->quadmap[tix];
+    if (quad < 0) return 1;
+    const int m = P->quad_pm[tix * 6 + p];
+    *rin = P->cut4[quad][0][m];
+    *rout = P->cut4[quad][1][m];
+    return 0;
+}
+
+/* Morse lambda of the pair type of two atom types (chimesFF.cpp:354-1570). */
+double oc_morse_lambda(const oc_params *P, int ti, int tj) { return P->lam[P->pairmap[ti * P->nt + tj]]; }
+
 /* Outer 2-body cutoff of a type pair (chimesFF::get_cutoff_2B, chimesFF.cpp:2886-2901). */
 double oc_cutoff_2b(const oc_params *P, int ti, int tj) { return P->rout[P->pairmap[ti * P->nt + tj]]; }
 

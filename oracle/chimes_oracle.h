@@ -74,6 +74,19 @@ void oc_compute_3b(const oc_params *P, const double dx[3], const double dr[9],
 void oc_compute_4b(const oc_params *P, const double dx[6], const double dr[18],
                    const int t[4], double force[12], double stress[6], double *energy);
 
+/* Inner/outer cutoff of runtime pair p of a cluster of types t (nonzero for
+ * an excluded cluster type) and the Morse lambda of a type pair. */
+int oc_cluster_pair_cutoffs(const oc_params *P, int nb, const int *t, int p, double *rin, double *rout);
+double oc_morse_lambda(const oc_params *P, int ti, int tj);
+
+/* Cluster fingerprint histograms (chimes_fp_oracle.c, calc_histograms.cpp):
+ * dumped rows (nb(nb-1)/2 distances, nb types) -> sorted transformed edges,
+ * and the histogram of cluster-cluster distances (same != 0: pairs j > i of
+ * one set).  Nonzero on an excluded cluster type or a bin out of range. */
+int oc_fp_edges(const oc_params *P, int nb, long n, const double *dist, const int *types, double *edges);
+int oc_fp_hist(int np, long n1, const double *e1, long n2, const double *e2, int same, int nbin, double binw,
+               double maxd, long long *counts, long long *nsamples);
+
 /* LAMMPS pair style on one process (chimes_lmp_oracle.c): many-body lists
  * from a full neighbour list with ghosts (pair_chimes.cpp:321-478; trips as
  * (i, j, k), quads as (i, j, k, l), free with oc_lmp_free) and the compute

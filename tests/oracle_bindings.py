@@ -79,6 +79,12 @@ def oracle_lib():
                                      C.c_long, _ip, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
                                      C.POINTER(C.c_double), _dp, _dp, _dp]
         L.oc_lmp_free.argtypes = [C.c_void_p]
+        L.oc_cluster_pair_cutoffs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double)]
+        L.oc_fp_edges.argtypes = [C.c_void_p, C.c_int, C.c_long, _dp, _ip, _dp]
+        _llp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.oc_fp_hist.argtypes = [C.c_int, C.c_long, _dp, C.c_long, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 _llp, C.POINTER(C.c_longlong)]
         _oracle = L
     return _oracle
 
@@ -139,6 +145,34 @@ class Oracle:
             res.meta = (L.n_atoms, L.n_layers, L.n_rep)
             self.L.oc_lists_free(C.byref(L))
         return res
+
+
+def fp_edges(oracle, nb, dist, types):
+    """The oracle's read_flat_clusters edges of dumped rows."""
+    dist = np.ascontiguousarray(dist, np.float64).reshape(-1)
+    types = np.ascontiguousarray(types, np.int32).reshape(-1)
+    n = types.size // nb
+    out = np.zeros(max(dist.size, 1))
+    if oracle.L.oc_fp_edges(oracle.h, nb, n, dist if dist.size else np.zeros(1), types, out):
+        raise RuntimeError("oracle fp_edges failed")
+    return out[: dist.size].reshape(n, -1)
+
+
+def fp_hist(nb, e1, e2=None, nbin=100, maxd=None):
+    """The oracle's gen_flat_hists: (counts, nsamples, binw)."""
+    L = oracle_lib()
+    P = nb * (nb - 1) // 2
+    if maxd is None:
+        maxd = (P * 4.0) ** 0.5 + 1.0
+    binw = maxd / nbin
+    a = np.ascontiguousarray(e1, np.float64).reshape(-1)
+    same = e2 is None
+    b = a if same else np.ascontiguousarray(e2, np.float64).reshape(-1)
+    counts = np.zeros(nbin, np.int64)
+    ns = C.c_longlong(0)
+    if L.oc_fp_hist(P, a.size // P, a, b.size // P, b, int(same), nbin, binw, maxd, counts, C.byref(ns)):
+        raise RuntimeError("oracle histogram: distance outside the bins")
+    return counts, ns.value, binw
 
 
 class LmpResult:
