@@ -1675,6 +1675,7 @@ int chx_lmp_compute(chx_lmp *p, int nall, const double *x, const int *type, cons
         if ((eflag_global && !eng) || (vflag_global && !virial) || (eflag_atom && !eatom) ||
             (vflag_atom && !vatom))
             throw EngineError("missing tally array");
+        if (nall == 0 && inum == 0) return 0;  // a process without atoms: the loops do nothing
         hipStream_t st = p->st;
         HIPCHK(hipEventRecord(p->ev[0], st));
         // Between rebuilds (ago > 0) LAMMPS keeps its neighbour list, tags,

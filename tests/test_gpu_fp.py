@@ -105,6 +105,12 @@ def test_fp_histogram_large_and_errors(rows):
     B = np.sort(rng.uniform(-1, 1, (70001, 3)), 1)
     counts, ns, _ = m.fp_histogram(3, A[:50000], B, nbin=777)
     assert ns == 50000 * 70001 and counts.sum() == ns
+    counts, ns, _ = m.fp_histogram(3, A[:0], nbin=100)
+    assert ns == 0 and not counts.any()
+    counts, ns, _ = m.fp_histogram(3, A[:1], nbin=100)
+    assert ns == 0 and not counts.any()
+    counts, ns, _ = m.fp_histogram(3, A[:0], B[:10], nbin=100)
+    assert ns == 0 and not counts.any()
     with pytest.raises(ChimesError, match="bins"):
         m.fp_histogram(3, A[:100], nbin=9000)
     with pytest.raises(ChimesError, match="outside the histogram"):

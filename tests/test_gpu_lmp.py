@@ -162,3 +162,23 @@ def test_lmp_unmatched_type_is_an_error(tmp_path):
     with pytest.raises(ChimesError, match="matches no ChIMES atom type"):
         LmpPair(model, {1: 12.0107, 2: 39.948})
     model.close()
+
+
+def test_lmp_empty_and_ghost_only(tmp_path):
+    """A process without atoms, and one whose atoms are all ghosts (inum = 0),
+    add nothing, as the reference's loops do."""
+    from chimes_calculator_amd import LmpPair
+    rec = json.load(open(os.path.join(G, "ev_tally_mb.json")))
+    params = _ungz(os.path.join(G, rec["params"]), tmp_path)
+    model = _gpu(params)
+    P = LmpPair(model, {1: 12.0107})
+    e = np.zeros(0, np.int32)
+    f, eng, vir, ea, va = P.compute(np.zeros((0, 3)), e, e, e, e, np.zeros(1, np.int32), e, ago=0, skin=1.0)
+    assert f.size == 0 and eng == 0.0 and not vir.any()
+    x = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0]])
+    one = np.ones(2, np.int32)
+    f, eng, vir, ea, va = P.compute(x, one, np.array([1, 2], np.int32), e, np.zeros(2, np.int32),
+                                    np.zeros(1, np.int32), e, ago=0, skin=1.0)
+    assert not f.any() and eng == 0.0 and not vir.any() and not ea.any() and not va.any()
+    P.close()
+    model.close()
