@@ -114,12 +114,18 @@ def test_lmp_small_box_images(tmp_path, L, natoms, skin):
     _run_case(params, S, masses, ago_steps=2, seed=natoms)
 
 
-def test_lmp_two_types_special_cutoffs(tmp_path):
-    ff = "published_params.CO2400K.2+3+4b.Tersoff.special.offsets.txt.gz"
+@pytest.mark.parametrize("ff,xyz", [
+    # two types, special 3-/4-body cutoffs, energy offsets
+    ("published_params.CO2400K.2+3+4b.Tersoff.special.offsets.txt.gz", "CO.9GPa_2400K.OUTCAR_#000.xyz.gz"),
+    # two types (N, H), 2+3+4-body, special cutoffs, offsets, 512 atoms
+    ("published_params.HN3.2+3+4b.Tersoff.special.offsets.txt.gz", "HN3.2gcc_3000K.OUTCAR_#000.xyz.gz"),
+    # one type, 2+3-body only (no quadruplets), cubic smoothing
+    ("published_params.liqC.2+3b.cubic.txt.gz", "liqC.2.5gcc_6000K.OUTCAR_#000.xyz.gz"),
+])
+def test_lmp_force_fields(tmp_path, ff, xyz):
     params = _ungz(os.path.join(GOLDEN, "ref_inputs", ff), tmp_path)
     O = LmpOracle(params)
-    pos, names, cell = read_xyz(_ungz(os.path.join(GOLDEN, "ref_inputs", "CO.9GPa_2400K.OUTCAR_#000.xyz.gz"),
-                                      tmp_path))
+    pos, names, cell = read_xyz(_ungz(os.path.join(GOLDEN, "ref_inputs", xyz), tmp_path))
     Lv = np.diag(np.asarray(cell, float).reshape(3, 3))
     pos = np.mod(np.asarray(pos, float), Lv)
     tn = O.type_names()
@@ -129,7 +135,7 @@ def test_lmp_two_types_special_cutoffs(tmp_path):
     S = LmpSystem(pos, ct + 1, np.arange(1, len(pos) + 1, dtype=np.int32), np.zeros(3), Lv,
                   np.arange(len(tn), dtype=np.int32), O.rc2(), 1.0, shuffle_seed=3)
     _, n3, n4 = _run_case(params, S, masses, ago_steps=1, seed=5)
-    assert n3 > 0 and n4 > 0
+    assert n3 > 0 and (n4 > 0) == (O.order(4) > 0)
 
 
 def test_lmp_tally_flags(tmp_path):
