@@ -1800,18 +1800,28 @@ extern "C" int chx_fp_histogram(chx_engine *e, int np, long n1, const double *e1
         if (n1) HIPCHK(hipMemcpyAsync(d1.p, e1, sizeof(double) * n1 * np, hipMemcpyHostToDevice, st));
         if (!same && n2) HIPCHK(hipMemcpyAsync(d2.p, e2, sizeof(double) * n2 * np, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * (nbin + 1), st));
-        HIPCHK(hipEventRecord(e->ev[0], st));
+        // events of its own: the engine's time the last evaluation
+        struct Ev {
+            hipEvent_t e = nullptr;
+            ~Ev()
+            {
+                if (e) (void)hipEventDestroy(e);
+            }
+        } t0, t1;
+        HIPCHK(hipEventCreate(&t0.e));
+        HIPCHK(hipEventCreate(&t1.e));
+        HIPCHK(hipEventRecord(t0.e, st));
         if (!launch_fp_hist(np, n1, d1.p, n2, g2, same, nbin, binw, maxd, cnt.p, (int *)(cnt.p + nbin), st))
             throw EngineError("cluster sets too large for one histogram launch");
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(e->ev[1], st));
+        HIPCHK(hipEventRecord(t1.e, st));
         std::vector<unsigned long long> h((size_t)nbin + 1);
         HIPCHK(hipMemcpyAsync(h.data(), cnt.p, sizeof(unsigned long long) * (nbin + 1), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (h[nbin]) throw EngineError("a cluster distance falls outside the histogram");
         for (int b = 0; b < nbin; b++) counts[b] = (long long)h[b];
         float t = 0.f;
-        HIPCHK(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));
+        HIPCHK(hipEventElapsedTime(&t, t0.e, t1.e));
         if (ms) *ms = t;
         return 0;
     } catch (const EngineError &x) {

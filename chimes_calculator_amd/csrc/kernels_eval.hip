@@ -67,7 +67,8 @@ __device__ __forceinline__ int find_group(const int *__restrict__ gbase, int G, 
 
 // ---------------------------------------------------------------------------
 // 2-body: sixteen lanes per owner atom over its image slab (four owners per
-// wavefront, sixteen per workgroup).  F_owner += f dr, F_image -= f dr,
+// wavefront; OWN = 4 or 16 owners per workgroup, the small shape for batches
+// too small to fill the device with the large one).  F_owner += f dr, F_image -= f dr,
 // virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).  The owner's
 // force, energy and virial are summed in double in a fixed order (lane l of
 // the sixteen takes images l, l + 16, ... of the slab, which is sorted by
@@ -76,13 +77,14 @@ __device__ __forceinline__ int find_group(const int *__restrict__ gbase, int G, 
 // is then added exactly.  Image forces are merged in the workgroup's LDS
 // table.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256, 3) void k_eval2(int natoms, int nb_cap, const int *__restrict__ nbr_cnt,
+template <int OWN>
+__global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) void k_eval2(int natoms, int nb_cap, const int *__restrict__ nbr_cnt,
                                                const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
                                                const chx_entry_s *__restrict__ es,
                                                const chx_model_dev *__restrict__ models, unsigned long long *Facc,
                                                unsigned long long *Oacc, int *cfg_stat, int *status)
 {
-    __shared__ fx_table<512> tab;  // image side of the workgroup's sixteen owners
+    __shared__ fx_table<OWN == 4 ? 256 : 512> tab;  // image side of the workgroup's owners
     // status read overlapped with the table clear; one barrier for both
     __shared__ int flags_s;
     if (threadIdx.x == 0) flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256, 3) void k_eval2(int natoms, int nb_cap, const 
     __syncthreads();
     if (flags_s & CHX_STOP) return;
     const int sub = threadIdx.x & 15;
-    const int i = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
+    const int i = (int)(blockIdx.x * OWN + (threadIdx.x >> 4));
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // owner force, energy, virial
     bool bad = false;
     if (i < natoms) {
@@ -1232,8 +1234,14 @@ void launch_eval2(int natoms, int nb_cap, hipStream_t st, const int *nbr_cnt, co
                   unsigned long long *Oacc, int *cfg_stat, int *status)
 {
     if (natoms <= 0) return;
-    hipLaunchKernelGGL(k_eval2, dim3((unsigned)((natoms + 15) / 16)), dim3(256), 0, st, natoms, nb_cap, nbr_cnt, cfgs,
-                       as, es, models, Facc, Oacc, cfg_stat, status);
+    // results do not depend on the shape: per-owner sums run over that owner's
+    // sixteen lanes and image forces are exact sums
+    if (natoms < 16384)
+        hipLaunchKernelGGL(k_eval2<4>, dim3((unsigned)((natoms + 3) / 4)), dim3(64), 0, st, natoms, nb_cap, nbr_cnt,
+                           cfgs, as, es, models, Facc, Oacc, cfg_stat, status);
+    else
+        hipLaunchKernelGGL(k_eval2<16>, dim3((unsigned)((natoms + 15) / 16)), dim3(256), 0, st, natoms, nb_cap,
+                           nbr_cnt, cfgs, as, es, models, Facc, Oacc, cfg_stat, status);
 }
 
 void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t st)

@@ -126,21 +126,25 @@ def test_dense_owners(cc, model_dir, synth_cases):
         check_lists(m, rec, conf=k)
 
 
-def test_tiled_extensivity(cc, model_dir, synth_cases):
-    """C0 tiled 4 x 4 x 4 (4096 atoms, more than one energy-reduction chunk):
-    energy 64 x the reference's C0 energy, each image's forces the
-    reference's C0 forces, stress the C0 stress (per-volume, so equal)."""
+@pytest.mark.parametrize("reps", [(4, 4, 4), (8, 8, 4)])
+def test_tiled_extensivity(cc, model_dir, synth_cases, reps):
+    """C0 tiled 4 x 4 x 4 (4096 atoms, more than one energy-reduction chunk)
+    and 8 x 8 x 4 (16384 atoms, the 2-body kernel's large-workgroup shape):
+    energy n x the reference's C0 energy, each image's forces the reference's
+    C0 forces, stress the C0 stress (per-volume, so equal)."""
     from chimes_calculator_amd import workloads as W
     mname, cfgs = W.workload("C0")
     pos, types, cell = cfgs[0]
     L = np.diag(cell)
-    shifts = np.array([(i, j, k) for i in range(4) for j in range(4) for k in range(4)], dtype=float)
+    shifts = np.array([(i, j, k) for i in range(reps[0]) for j in range(reps[1]) for k in range(reps[2])],
+                      dtype=float)
+    n = len(shifts)
     big = (shifts[:, None, :] * L[None, None, :] + pos[None, :, :]).reshape(-1, 3)
     m = cc.GpuModel(os.path.join(model_dir, mname + ".txt"))
-    E, F, S = cc.eval_batch(m, [(big, types * 64, np.diag(4 * L))])
+    E, F, S = cc.eval_batch(m, [(big, types * n, np.diag(np.array(reps) * L))])
     e0, f0, s0 = golden_arrays(synth_cases["C0|0"])
-    assert abs(E[0] - 64 * e0) <= 1e-10 * abs(64 * e0), (E[0], 64 * e0)
-    assert_close("tiled forces", F[0], np.tile(f0, (64, 1)), 1e-10, atol=FATOL)
+    assert abs(E[0] - n * e0) <= 1e-10 * abs(n * e0), (E[0], n * e0)
+    assert_close("tiled forces", F[0], np.tile(f0, (n, 1)), 1e-10, atol=FATOL)
     assert_close("tiled stress", S[0], s0, 1e-10)
 
 
