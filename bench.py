@@ -462,10 +462,7 @@ def bench_lmp(args):
     S = LmpSystem(X, T, Tg, lo, hi, [0], rc2, skin)
     P = cc.LmpPair(model, masses)
     args_c = (S.x, S.type, S.tag, S.ilist, S.numneigh, S.nbr_off, S.nbr)
-    t0 = time.perf_counter()
     P.compute(*args_c, ago=0, skin=skin)
-    t_build_call = time.perf_counter() - t0
-    t_build = P.timing()
     g3, g4 = P.mb_lists()
     # prebuilt row pointers, as a C++ caller passes LAMMPS's firstneigh as is
     L = cc.lib()
@@ -499,6 +496,14 @@ def bench_lmp(args):
         forms[name] = {"value": S.inum / (msk * 1e-3), "ms_per_step": msk,
                        "phases_ms": {"positions_in": float(kt[0]), "pair_1_2_body": float(kt[2]),
                                      "many_body_and_readback": float(kt[3])}}
+    # a rebuild step (ago == 0): row packing and checks, list build, evaluation
+    tb, lb = [], []
+    for _ in range(max(3, args.steps // 4)):
+        t0 = time.perf_counter()
+        step(0, kinds["forces_energy_virial"])
+        tb.append(time.perf_counter() - t0)
+        lb.append(P.timing()[1])
+    t_build_call, t_build_lists = float(np.median(tb)), float(np.median(lb))
     ms = forms["forces_energy_virial"]["ms_per_step"]
     kt = forms["forces_energy_virial"]["phases_ms"]
     out = {"metric": "atom-evals/s (LAMMPS pair style: E, F, virial; lists reused between rebuilds)",
@@ -511,7 +516,7 @@ def bench_lmp(args):
                       "local_atoms": int(S.inum), "ghost_atoms": int(nall - S.inum),
                       "neighbour_entries": int(len(S.nbr)), "triplets": int(len(g3)), "quadruplets": int(len(g4))},
            "phases_ms": kt, "forms": forms,
-           "rebuild_step_ms": {"call": 1e3 * t_build_call, "lists": float(t_build[1])}}
+           "rebuild_step_ms": {"call": 1e3 * t_build_call, "lists": t_build_lists}}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_bindings import LmpOracle

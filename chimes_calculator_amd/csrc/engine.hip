@@ -1508,23 +1508,32 @@ void lmp_upload(chx_lmp *p, int nall, const double *x, const int *type, const in
 {
     if (nall <= 0 || inum < 0 || inum > nall) throw EngineError("atom counts out of range");
     if (!x || !tag || (inum > 0 && (!ilist || !numneigh || !firstneigh))) throw EngineError("missing input array");
+    // row offsets first (checking owners and counts), then the rows
     p->off_h.assign((size_t)inum + 1, 0);
-    p->nbr_h.clear();
+    long total = 0;
     for (int ii = 0; ii < inum; ii++) {
         const int i = ilist[ii];
         if (i < 0 || i >= nall) throw EngineError("ilist entry out of range");
         const int n = numneigh[i];
         if (n < 0) throw EngineError("negative neighbour count");
-        const int *row = firstneigh[i];
-        if (n > 0 && !row) throw EngineError("missing neighbour row");
+        if (n > 0 && !firstneigh[i]) throw EngineError("missing neighbour row");
+        total += n;
+        if (total >= (1L << 31)) throw EngineError("too many neighbour entries");
+        p->off_h[ii + 1] = (int)total;
+    }
+    p->nbr_h.resize((size_t)total);
+    int *dst = p->nbr_h.data();
+    bool bad_index = false;
+    for (int ii = 0; ii < inum; ii++) {
+        const int *row = firstneigh[ilist[ii]];
+        const int n = p->off_h[ii + 1] - p->off_h[ii];
         for (int q = 0; q < n; q++) {
             const int j = row[q] & LMP_NEIGHMASK;
-            if (j >= nall) throw EngineError("neighbour index out of range");
-            p->nbr_h.push_back(j);
+            bad_index |= j >= nall;
+            *dst++ = j;
         }
-        if ((long)p->nbr_h.size() >= (1L << 31)) throw EngineError("too many neighbour entries");
-        p->off_h[ii + 1] = (int)p->nbr_h.size();
     }
+    if (bad_index) throw EngineError("neighbour index out of range");
     hipStream_t st = p->st;
     p->nall = nall;
     p->inum = inum;
