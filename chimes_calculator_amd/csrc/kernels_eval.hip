@@ -270,10 +270,12 @@ __device__ __forceinline__ void cluster_forces(const double *h, const double (&p
 // ---------------------------------------------------------------------------
 // 3- and 4-body on the vector units: one thread per cluster, one wavefront per
 // 64-cluster tile of one group (any supported D); the workgroup's member
-// forces are merged per atom in an LDS table before they reach memory.
+// forces are merged per atom in an LDS table before they reach memory.  The
+// FP64 4-body kernels for D <= 3 are held to two wavefronts per SIMD (left
+// alone the compiler gives them one; at two they still need no scratch).
 // ---------------------------------------------------------------------------
 template <int NB, int D, typename R>
-__global__ __launch_bounds__(256) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && D <= 3 && sizeof(R) == 8 ? 2 : 1))) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
                                                 const chx_group *__restrict__ groups, int total_word,
                                                 const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
                                                 const chx_atom_s *__restrict__ as, int nb_cap,
