@@ -1195,7 +1195,7 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
 // (chimesFF.cpp:1745-1774).
 // ---------------------------------------------------------------------------
 template <int NB, int D>
-__global__ void k_clusters(int n, const double *__restrict__ dist, const double *__restrict__ drv,
+__global__ __launch_bounds__(128) void k_clusters(int n, const double *__restrict__ dist, const double *__restrict__ drv,
                            const int *__restrict__ types, chx_model_dev M, double *E, double *F, double *Sout,
                            double *last, int *flags_out)
 {

@@ -243,7 +243,7 @@ __device__ __forceinline__ void lmp_run_add(int key, int seg, bool last, double 
 }
 
 template <int NB, int D>
-__global__ __launch_bounds__(128) void k_lmp_clusters(long n, const int *__restrict__ list, chx_lmp_args A,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_lmp_clusters(long n, const int *__restrict__ list, chx_lmp_args A,
                                                       chx_model_dev M)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
