@@ -270,12 +270,14 @@ __device__ __forceinline__ void cluster_forces(const double *h, const double (&p
 // ---------------------------------------------------------------------------
 // 3- and 4-body on the vector units: one thread per cluster, one wavefront per
 // 64-cluster tile of one group (any supported D); the workgroup's member
-// forces are merged per atom in an LDS table before they reach memory.  The
-// FP64 4-body kernels for D <= 3 are held to two wavefronts per SIMD (left
-// alone the compiler gives them one; at two they still need no scratch).
+// forces are merged per atom in an LDS table before they reach memory.
+// Occupancy targets, measured: the FP64 4-body kernels for D <= 3 are held to
+// two wavefronts per SIMD (left alone the compiler gives them one; at two
+// they need no scratch), the FP64 3-body kernels for D <= 8 to three (12-36
+// bytes of scratch, faster end to end).
 // ---------------------------------------------------------------------------
 template <int NB, int D, typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && D <= 3 && sizeof(R) == 8 ? 2 : 1))) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && D <= 3 && sizeof(R) == 8 ? 2 : (NB == 3 && D <= 8 && sizeof(R) == 8 ? 3 : 1)))) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
                                                 const chx_group *__restrict__ groups, int total_word,
                                                 const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
                                                 const chx_atom_s *__restrict__ as, int nb_cap,
