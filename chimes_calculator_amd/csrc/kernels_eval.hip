@@ -274,11 +274,11 @@ __device__ __forceinline__ void cluster_forces(const double *h, const double (&p
 // Occupancy targets, measured: the FP64 4-body kernels for D <= 3 are held to
 // two wavefronts per SIMD (left alone the compiler gives them one; at two
 // they need no scratch); to three: the FP64 3-body kernels for D <= 8, the
-// FP32 3-body kernels for D <= 10 and the FP32 4-body kernels for D <= 3 (at
-// most 36 bytes of scratch, faster end to end).
+// FP32 3-body kernels for D <= 10 and the FP32 4-body kernels for D <= 4 (at
+// most 84 bytes of scratch, faster end to end).
 // ---------------------------------------------------------------------------
 template <int NB, int D, typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && D <= 3 && sizeof(R) == 8 ? 2 : ((NB == 3 && D <= (sizeof(R) == 8 ? 8 : 10)) || (NB == 4 && D <= 3) ? 3 : 1)))) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && D <= 3 && sizeof(R) == 8 ? 2 : ((NB == 3 && D <= (sizeof(R) == 8 ? 8 : 10)) || (NB == 4 && D <= (sizeof(R) == 8 ? 3 : 4)) ? 3 : 1)))) void k_evalv(int G, const int *__restrict__ gbase, const int *__restrict__ gend,
                                                 const chx_group *__restrict__ groups, int total_word,
                                                 const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
                                                 const chx_atom_s *__restrict__ as, int nb_cap,
