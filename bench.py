@@ -1,20 +1,24 @@
 """Benchmark: atom-evaluations per second of the ChIMES evaluation path.
 
-Default workload (--workload C1): the plain FP64 single-system case, 1536
-O/H atoms (512 H2O-like molecules, L = 24.8 A), 2B+3B+4B at orders 12/8/4,
-rc 5.0/4.5/4.0 A.  One step = one full evaluation (ghost images, cluster
-lists, 2/3/4-body energies, forces and stress) with inputs already resident
-in GPU memory.  Prints one JSON line with the metric, a roofline entry for
-the dominant kernel (the 4-body evaluation, algorithmic FLOPs by the term
-model) and the CPU baseline (the oracle's C restatement on a sample of the
-same workload).
+Default workload (--workload C2): 4096 independent C/O configurations, 8-512
+atoms each (489,549 atoms), densities 1.5-3.5 g/cc, 2B/3B/4B orders 12/7/3,
+rc 4.5/4.0/3.5 A.  One step = one full evaluation of the whole batch (ghost
+images, cluster lists, 1-4-body energies, forces and stress of every
+configuration) by replaying the batch plan's captured graph, with positions,
+types and outputs resident in GPU memory.  The headline value is the
+FP32-mixed plan; the same line carries the FP64 plan, the plain FP64 drop-in
+path (one calculate() per configuration), the error of the measured result
+against the reference's FP64 goldens, a roofline entry for the batch's
+dominant kernel (term-model FLOPs over its HIP-event-timed duration), the CPU
+baseline (the oracle's C restatement, single-threaded and as one process per
+host core over the same configurations), and nested lines for the 1536-atom
+O/H box (C1) and the served request stream (C4, p50/p99 latency).
 
 Other forms (one JSON line each):
-  --workload C2  4096 uneven C/O configurations: the plain FP64 path (one
-                 drop-in calculate() per configuration) against batch plans
-                 (one captured-graph replay per batch), FP64 and FP32-mixed.
+  --workload C1  1536 O/H atoms (512 H2O-like, L = 24.8 A), 12/8/4, rc
+                 5.0/4.5/4.0 A, FP64: direct evaluations and an autotuned plan.
   --workload C3  32768-atom carbon, orders 16/10/4: FP64 and FP32-mixed, with
-                 the error of the mixed result per quantity.
+                 the error of each form against the FP64 golden.
   --workload C4  Poisson request stream served with continuous batching:
                  p50/p99 latency and throughput, FP64 and FP32-mixed.
   --workload K   batched explicit 3-/4-body clusters (per-cluster API).
@@ -27,10 +31,15 @@ Other forms (one JSON line each):
                  lists as LAMMPS does between rebuilds (host arrays in and out).
 """
 import argparse
+import atexit
 import ctypes as C
+import gzip
 import json
+import multiprocessing
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -39,7 +48,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12   # MI355X dense FP64 (vector and matrix), FLOP/s
-PEAK_FP32 = 157.3e12  # MI355X FP32, FLOP/s
+PEAK_FP32 = 157.3e12  # MI355X FP32 (vector and matrix), FLOP/s
+EV = 0.0433641        # eV per kcal/mol
+METRIC = "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency"
+_TMP = []
+
+
+def model_file(name):
+    """Parameter file of a synthetic workload model, written once into a
+    temporary directory removed at exit."""
+    from chimes_calculator_amd import workloads as W
+    if not _TMP:
+        _TMP.append(tempfile.mkdtemp(prefix="chimes_bench_"))
+        atexit.register(shutil.rmtree, _TMP[0], True)
+    path = os.path.join(_TMP[0], name + ".txt")
+    if not os.path.exists(path):
+        W.MODELS[name].write(path)
+    return path
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def term_flops(nterms, order, npairs):
@@ -94,17 +129,27 @@ class Hip:
         assert self.L.hipDeviceSynchronize() == 0
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "k_eval4m_water1536_fp64_traffic.md")
+def profile_figures(summary, kernel):
+    """key=value figures of `kernel` from a committed profile summary (lines
+    'traffic <kernel> fetch_bytes=.. write_bytes=..' and 'counters <kernel>
+    k=v ...'), or {}."""
+    path = os.path.join(ROOT, "profiles", summary)
+    out = {}
     if not os.path.exists(path):
-        return None
+        return out
     for line in open(path):
         f = line.split()
-        if len(f) > 3 and f[0] == "traffic" and f[1] == kernel:
-            kv = dict(x.split("=") for x in f[2:])
-            return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
+        if len(f) > 2 and f[0] in ("traffic", "counters") and f[1] == kernel:
+            out.update({k: float(v) for k, v in (x.split("=") for x in f[2:])})
+    return out
+
+
+def measured_traffic(kernel, summary="k_eval4m_water1536_fp64_traffic.md"):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated), or None."""
+    f = profile_figures(summary, kernel)
+    if "fetch_bytes" in f and "write_bytes" in f:
+        return f["fetch_bytes"] + f["write_bytes"]
     return None
 
 
@@ -129,11 +174,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4", "K", "L", "F"], default="C1",
+    ap.add_argument("--workload", choices=["C1", "C2", "C3", "C4", "K", "L", "F"], default="C2",
                     help="K: batched explicit 3-/4-body clusters (chimes_gpu_compute_clusters); "
                          "L: the LAMMPS pair style (chimes_lmp.h); F: fingerprint histogram")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="C2 headline precision (default mixed); C1 precision (default fp64)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-nested", action="store_true", help="C2: skip the nested C1 and C4 lines")
     ap.add_argument("--direct-only", action="store_true",
                     help="C1: time direct evaluations only (no plan), e.g. for a kernel trace of plain steps")
     ap.add_argument("--plan-only", action="store_true",
@@ -145,6 +192,8 @@ def main():
         print("bench.py measures one GPU: every configuration of this workload fits one MI355X and the "
               "multi-GPU decomposition is not part of this build (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.precision is None:
+        args.precision = "mixed" if args.workload == "C2" else "fp64"
     if args.workload == "C2":
         return bench_c2(args)
     if args.workload == "C3":
@@ -161,12 +210,17 @@ def main():
 
 
 def bench_c1(args):
+    print(json.dumps(c1_form(args, cpu=not args.no_cpu)))
+
+
+def c1_form(args, cpu=False, precision=None):
+    """The 1536-atom O/H box, FP64: direct evaluations (HIP-event timed
+    kernels) and an autotuned device-buffer plan (the value)."""
     import chimes_calculator_amd as cc
     from chimes_calculator_amd import workloads as W
 
-    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
     mname, cfgs = W.workload("C1")
-    pf = W.MODELS[mname].write(os.path.join(ROOT, "_work", "bench_%s.txt" % mname))
+    pf = model_file(mname)
     model = cc.GpuModel(pf)
     pos, types, cell = cfgs[0]
     n = len(pos)
@@ -177,7 +231,7 @@ def bench_c1(args):
     natoms = np.array([n], np.int32)
     model_of = np.zeros(1, np.int32)
     cells = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
-    prec = cc.FP64 if args.precision == "fp64" else cc.MIXED
+    prec = cc.MIXED if (precision or args.precision) == "mixed" else cc.FP64
     arr = (C.c_void_p * 1)(model.h)
     L = cc.lib()
 
@@ -260,7 +314,7 @@ def bench_c1(args):
     achieved = flops4 / t4_s
     peak = PEAK_FP64 if prec == cc.FP64 else PEAK_FP32
     out = {
-        "metric": "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency",
+        "metric": METRIC,
         "value": n * args.steps / dt,
         "unit": "atom-evals/s",
         "n_gpus": 1,
@@ -290,32 +344,193 @@ def bench_c1(args):
                      "kernel": "k_eval4 (4-body evaluation, %d cluster types)" % len(t4terms),
                      "flops_per_step": flops4, "kernel_ms_per_step": t4_s * 1e3},
     }
-    if not args.no_cpu:
+    if cpu:
         out["cpu_baseline"] = cpu_baseline(pf, cfgs[0])
-    print(json.dumps(out))
+    return out
 
 
 def atoms_of(cfgs):
     return sum(len(c[0]) for c in cfgs)
 
 
+_C2 = {}
+
+
+def _c2_cpu_worker(job):
+    """The oracle on configurations idx of the batch until budget_s has
+    passed; returns (atoms evaluated, seconds)."""
+    pf, idx, budget_s = job
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bindings import Oracle
+    cfgs = _C2["cfgs"]
+    o = Oracle(pf)
+    atoms, t0 = 0, time.perf_counter()
+    for k in idx:
+        o.calculate(*cfgs[k])
+        atoms += len(cfgs[k][0])
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    return atoms, time.perf_counter() - t0
+
+
+def c2_cpu_baseline(pf, cfgs, budget_s=10.0):
+    """The oracle (C restatement of the reference's serial path, -O2, one
+    thread per process) over the batch's configurations in order: once in
+    one process, and as one process per host core (at most 16) taking every
+    P-th configuration, each for about budget_s seconds.  Runs before the GPU
+    is touched (the worker processes are forked)."""
+    _C2["cfgs"] = cfgs
+    n = len(cfgs)
+    a1, t1 = _c2_cpu_worker((pf, range(n), budget_s))
+    P = max(1, min(16, os.cpu_count() or 1))
+    ctx = multiprocessing.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(P) as pool:
+        res = pool.map(_c2_cpu_worker, [(pf, range(w, n, P), budget_s) for w in range(P)])
+    wall = time.perf_counter() - t0
+    aP = sum(r[0] for r in res)
+    return {"value": aP / max(r[1] for r in res), "unit": "atom-evals/s", "cores": P, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": "the oracle's calculate() on the C/O batch's configurations, %d processes (one per "
+                      "core) taking every %d-th configuration for %.0f s each: %d atoms (%.1f s wall)"
+                      % (P, P, budget_s, aP, wall),
+            "single_thread": {"value": a1 / t1, "cores": 1,
+                              "sample": "configurations 0.. in order for %.0f s: %d atoms" % (budget_s, a1)}}
+
+
+def c2_goldens_error(E, F, V, cfgs):
+    """Error of a batch result (raw energies, forces, virials of the first
+    configurations) against the reference's FP64 goldens (24 configurations)."""
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "synth_cases.json")))
+    de = df = ds = 0.0
+    off = 0
+    for k in range(24):
+        n = len(cfgs[k][0])
+        rec = gold["C2|%d" % k]
+        e0 = float(rec["energy"])
+        f0 = np.array([[float(v) for v in r] for r in rec["forces"]])
+        s0 = np.array([float(v) for v in rec["stress"]])
+        vol = abs(np.linalg.det(np.asarray(cfgs[k][2], float).reshape(3, 3)))
+        v = V[6 * k: 6 * k + 6]
+        s9 = np.array([v[0], v[1], v[2], v[1], v[3], v[4], v[2], v[4], v[5]]) / vol
+        de = max(de, abs(E[k] - e0) * EV / n)
+        df = max(df, float(np.abs(F[off: off + n] - f0).max()) * EV)
+        ds = max(ds, float(np.abs(s9 - s0).max() / np.abs(s0).max()))
+        off += n
+    return {"energy_eV_per_atom_max": de, "force_max_eV_per_A": df, "stress_max_rel": ds,
+            "configurations": 24, "reference": "tests/golden/synth_cases.json (compiled reference, FP64)"}
+
+
+def c2_plan(cc, L, hip, model, cfgs, prec, steps, warmup):
+    """A device-buffer batch plan of the whole batch (autotuned): K replays
+    bracketed by device synchronisations.  Returns the form's figures and
+    the downloaded outputs of the last replay."""
+    nconf = len(cfgs)
+    model_of = np.zeros(nconf, np.int32)
+    natoms, pos, types, cells = cc._flatten(cfgs, model_of, [model])
+    N = pos.size // 3
+    d_pos, d_typ = hip.upload(pos), hip.upload(types)
+    d_E, d_F, d_V = hip.alloc(8 * nconf), hip.alloc(8 * 3 * N), hip.alloc(8 * 6 * nconf)
+    arr = (C.c_void_p * 1)(model.h)
+    t0 = time.perf_counter()
+    plan = L.chimes_gpu_plan_create_device(arr, 1, nconf, model_of, natoms, d_pos, d_typ, cells, 0, prec, 1, d_E,
+                                           d_F, d_V)
+    if not plan:
+        raise cc.ChimesError(cc._err())
+    t_create = time.perf_counter() - t0
+    cells_c = cells.ctypes.data_as(C.c_void_p)
+
+    def run():
+        if L.chimes_gpu_plan_run_device(plan, cells_c, d_E, d_F, d_V):
+            raise cc.ChimesError(cc._err())
+
+    for _ in range(warmup):
+        run()
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    hip.sync()
+    dt = time.perf_counter() - t0
+    vv, tune_ms, counts = (C.c_int * 2)(), (C.c_double * 6)(), (C.c_longlong * 3)()
+    L.chimes_gpu_plan_info(plan, vv, tune_ms, counts)
+    L.chimes_gpu_plan_free(plan)
+    E = hip.download(d_E, (nconf,), np.float64)
+    F = hip.download(d_F, (N, 3), np.float64)
+    V = hip.download(d_V, (6 * nconf,), np.float64)
+    for p in (d_pos, d_typ, d_E, d_F, d_V):
+        hip.L.hipFree(p)
+    form = {"value": N * steps / dt, "ms_per_step": dt / steps * 1e3, "variant3": cc.VARIANTS[vv[0]],
+            "variant4": cc.VARIANTS[vv[1]], "tune_ms": [round(x, 4) for x in tune_ms],
+            "plan_create_s": t_create}
+    return form, (E, F, V), (vv[0], vv[1]), tuple(counts)
+
+
+def c2_kernel_timing(cc, L, hip, model, cfgs, prec, variants, reps=5):
+    """Per-kernel-group milliseconds of the batch (HIP events on the engine's
+    stream around the 3- and 4-body evaluation launches) from direct device
+    evaluations with the plan's kernel variants."""
+    nconf = len(cfgs)
+    model_of = np.zeros(nconf, np.int32)
+    natoms, pos, types, cells = cc._flatten(cfgs, model_of, [model])
+    N = pos.size // 3
+    d_pos, d_typ = hip.upload(pos), hip.upload(types)
+    d_E, d_F, d_V = hip.alloc(8 * nconf), hip.alloc(8 * 3 * N), hip.alloc(8 * 6 * nconf)
+    arr = (C.c_void_p * 1)(model.h)
+    old = cc.get_variants()
+    cc.set_variants(*variants)
+    ts = []
+    for k in range(reps + 2):
+        if L.chimes_gpu_eval_batch_device(arr, 1, nconf, model_of, natoms, d_pos, d_typ, cells, 0, prec, d_E, d_F,
+                                          d_V):
+            raise cc.ChimesError(cc._err())
+        if k >= 2:
+            ts.append(model.timing())
+    cc.set_variants(*old)
+    for p in (d_pos, d_typ, d_E, d_F, d_V):
+        hip.L.hipFree(p)
+    t = np.mean(np.array(ts), 0)
+    return {"pipeline": t[0], "4body": t[1], "3body": t[2], "lists": t[3]}
+
+
 def bench_c2(args):
-    """4096 uneven configurations: plain FP64 drop-in path per configuration
-    against batch plans replayed from one captured graph."""
+    """4096 uneven configurations as one batch plan (device buffers): the
+    FP32-mixed plan (headline), the FP64 plan, the plain FP64 drop-in path."""
     import chimes_calculator_amd as cc
     from chimes_calculator_amd import workloads as W
 
-    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
     mname, cfgs = W.workload("C2")
-    pf = W.MODELS[mname].write(os.path.join(ROOT, "_work", "bench_%s.txt" % mname))
-    model = cc.GpuModel(pf)
+    pf = model_file(mname)
     natoms = atoms_of(cfgs)
-    out = {"metric": "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency",
-           "unit": "atom-evals/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "data": "synthetic: seeded C/O configurations and seeded ChIMES model",
-           "config": {"workload": W.WORKLOADS["C2"], "nconf": len(cfgs), "natoms": natoms}}
-    plain = None
+    cpu = None if args.no_cpu else c2_cpu_baseline(pf, cfgs)  # before the GPU is touched
+    model = cc.GpuModel(pf)
+    L = cc.lib()
+    L.chimes_gpu_plan_info.argtypes = [C.c_void_p] * 4
+    L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
+    L.chimes_gpu_plan_run_device.argtypes = [C.c_void_p] * 5
+    hip = Hip()
+    head_prec = cc.MIXED if args.precision == "mixed" else cc.FP64
+    forms = {}
+    head, outs, variants, counts = c2_plan(cc, L, hip, model, cfgs, head_prec, args.steps, args.warmup)
+    err = c2_goldens_error(*outs, cfgs)
     if not args.plan_only:
+        other = cc.FP64 if head_prec == cc.MIXED else cc.MIXED
+        f2, o2, _, _ = c2_plan(cc, L, hip, model, cfgs, other, args.steps, args.warmup)
+        f2["error_vs_reference_fp64"] = c2_goldens_error(*o2, cfgs)
+        forms["fp64_plan" if other == cc.FP64 else "mixed_plan"] = f2
+        # the same plan with host buffers: positions in and results out cross PCIe in every replay
+        plan = cc.GpuPlan(model, cfgs, precision=head_prec, tune=True)
+        _, pos, types, cells = cc._flatten(cfgs, plan.model_of, [model])
+        E, F, S = np.zeros(len(cfgs)), np.zeros(pos.shape[0]), np.zeros(9 * len(cfgs))
+        for _ in range(args.warmup):
+            plan.run_flat(pos, types, cells, E, F, S)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.run_flat(pos, types, cells, E, F, S)
+        dt = time.perf_counter() - t0
+        plan.close()
+        forms["host_buffer_plan"] = {"value": natoms * args.steps / dt, "ms_per_step": dt / args.steps * 1e3,
+                                     "note": "positions, types and results cross PCIe inside each replay"}
         # plain FP64 GPU path: the drop-in calculate() once per configuration (host buffers)
         s = cc.ChimesSerial(small=False)
         s.init_chimes(pf)
@@ -326,44 +541,63 @@ def bench_c2(args):
             s.calculate_chimes(len(p), p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), t, c[0], c[1], c[2])
         plain = natoms / (time.perf_counter() - t0)
         s.close()
-        out["plain_fp64_per_config"] = plain
-    # batch plans (host buffers in and out: positions cross PCIe inside the timed replay)
-    forms = {}
-    ref = None
-    todo = ((cc.FP64, "fp64"), (cc.MIXED, "mixed"))
-    if args.plan_only:
-        todo = tuple(x for x in todo if x[1] == args.precision)
-    for prec, tag in todo:
-        plan = cc.GpuPlan(model, cfgs, precision=prec, tune=True)
-        _, pos, types, cells = cc._flatten(cfgs, plan.model_of, [model])
-        E, F, S = np.zeros(len(cfgs)), np.zeros(pos.shape[0]), np.zeros(9 * len(cfgs))
-        for _ in range(args.warmup):
-            plan.run_flat(pos, types, cells, E, F, S)
-        ts = []
-        for _ in range(args.steps):
-            t0 = time.perf_counter()
-            plan.run_flat(pos, types, cells, E, F, S)
-            ts.append(time.perf_counter() - t0)
-        v3, v4, tms, counts = plan.info()
-        forms[tag] = {"value": natoms / np.median(ts), "ms_per_batch": 1e3 * float(np.median(ts)),
-                      "p99_ms": 1e3 * float(np.percentile(ts, 99)), "variant3": v3, "variant4": v4,
-                      "counts": counts}
-        if prec == cc.FP64:
-            ref = (E.copy(), F.copy())
-        elif ref is not None:
-            e_err = np.abs(E - ref[0]) * 0.0433641 / np.array([len(c[0]) for c in cfgs])
-            forms[tag]["energy_err_eV_per_atom_max"] = float(e_err.max())
-            forms[tag]["force_err_eV_per_A_max"] = float(np.abs(F - ref[1]).max() * 0.0433641)
-        plan.close()
-    out["forms"] = forms
-    head = forms.get("mixed", forms.get("fp64"))
-    out["value"] = head["value"]
-    out["ms_per_step"] = head["ms_per_batch"]
-    out["steps"], out["warmup"] = args.steps, args.warmup
-    out["dtype"] = "f32-mixed" if "mixed" in forms else "f64"
-    if plain:
-        for tag in forms:
-            out["speedup_%s_plan_vs_plain_fp64" % tag] = forms[tag]["value"] / plain
+        forms["plain_fp64_per_config"] = {"value": plain, "note": "calculate_chimes_instance once per "
+                                          "configuration, host buffers (the drop-in path)"}
+        key = "mixed_plan" if head_prec == cc.FP64 else "fp64_plan"
+        fp64v = head["value"] if head_prec == cc.FP64 else forms[key]["value"]
+        mixv = head["value"] if head_prec == cc.MIXED else forms[key]["value"]
+        forms["speedup_mixed_plan_vs_plain_fp64"] = mixv / plain
+        forms["speedup_mixed_plan_vs_fp64_plan"] = mixv / fp64v
+    # roofline of the batch's dominant kernel (the larger of the 3- and 4-body
+    # evaluations), HIP-event timed on the engine's stream
+    kt = c2_kernel_timing(cc, L, hip, model, cfgs, head_prec, variants)
+    t3, t4 = param_terms(pf)
+    d3, d4 = model.order(3), model.order(4)
+    body = 3 if kt["3body"] >= kt["4body"] else 4
+    var = variants[body - 3]
+    f32 = head_prec == cc.MIXED
+    if var == 2 or (body == 4 and d4 > 4):
+        kname = "k_evalv<%d, %d, %s>" % (body, d3 if body == 3 else d4, "float" if f32 else "double")
+        unit = "vector ALU"
+    else:
+        kname = "%s<%d, %s, %d, 0>" % ("k_eval3m" if body == 3 else "k_eval4m", d3 if body == 3 else d4,
+                                       "float" if f32 else "double", 4 if var == 0 else 2)
+        unit = "matrix cores"
+    nclus = counts[1] if body == 3 else counts[2]
+    terms = (t3 if body == 3 else t4)[0]
+    flops = nclus * term_flops(terms, d3 if body == 3 else d4, 3 if body == 3 else 6)
+    ms = kt["3body"] if body == 3 else kt["4body"]
+    peak = PEAK_FP32 if f32 else PEAK_FP64
+    prof = "%s_co4096_%s_counters.md" % ("k_evalv3" if body == 3 else "k_evalv4", "mixed" if f32 else "fp64")
+    figs = profile_figures(prof, kname.replace(" ", ""))
+    roof = {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": flops / (ms * 1e-3) / peak,
+            "traffic": (figs["fetch_bytes"] + figs["write_bytes"]) if "fetch_bytes" in figs else None,
+            "kernel": "%s (%d-body evaluation on the %s)" % (kname, body, unit),
+            "compute_unit": unit, "flops_per_launch": flops, "kernel_ms": ms,
+            "clusters_per_launch": nclus, "term_flops_per_cluster": flops / max(nclus, 1),
+            "kernel_groups_ms": kt, "counters_source": "profiles/" + prof if figs else None,
+            "counters": figs or None}
+    out = {"metric": METRIC, "value": head["value"], "unit": "atom-evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32-mixed" if f32 else "f64",
+           "data": "synthetic: seeded jittered-lattice C/O configurations and seeded random-coefficient ChIMES model",
+           "config": {"workload": "C/O batch: 4096 configurations of 8-512 atoms (489,549 atoms), 1.5-3.5 g/cc, "
+                                  "2B/3B/4B orders 12/7/3, rc 4.5/4.0/3.5 A, one batch plan (captured graph), "
+                                  "inputs and outputs in HBM, %s" % ("FP32-mixed" if f32 else "FP64"),
+                      "nconf": len(cfgs), "natoms": natoms, "pairs": counts[0], "triplets": counts[1],
+                      "quadruplets": counts[2]},
+           "error_vs_reference_fp64": err,
+           "variants": {"3body": head["variant3"], "4body": head["variant4"], "tune_ms": head["tune_ms"]},
+           "forms": forms, "roofline": roof}
+    if not args.plan_only and not args.no_nested:
+        out["forms"]["C1"] = c1_form(args, precision="fp64")
+        out["forms"]["C4"] = c4_form(cc, steps_scale=1.0)
+        out["latency_ms"] = {"p50": out["forms"]["C4"]["mixed_rate200"]["p50_ms"],
+                             "p99": out["forms"]["C4"]["mixed_rate200"]["p99_ms"],
+                             "stream": "C4, 200 requests/s, FP32-mixed"}
+    if cpu:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 
@@ -387,9 +621,8 @@ def bench_clusters(args):
     import chimes_calculator_amd as cc
     from chimes_calculator_amd import workloads as W
 
-    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
     m = W.MODELS["water_2b3b4b"]
-    pf = m.write(os.path.join(ROOT, "_work", "bench_water_2b3b4b.txt"))
+    pf = model_file("water_2b3b4b")
     model = cc.GpuModel(pf)
     rng = np.random.default_rng(0xC0DE05)
     n = 1 << 20
@@ -631,9 +864,8 @@ def bench_c3(args):
     import chimes_calculator_amd as cc
     from chimes_calculator_amd import workloads as W
 
-    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
     mname, cfgs = W.workload("C3")
-    pf = W.MODELS[mname].write(os.path.join(ROOT, "_work", "bench_%s.txt" % mname))
+    pf = model_file(mname)
     model = cc.GpuModel(pf)
     n = atoms_of(cfgs)
     out = {"metric": "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency",
@@ -695,15 +927,36 @@ def serve_stream(cc, srv, names, reqs, rate_scale=1.0):
     return np.array(lat), atoms / ((last - first) * 1e-3)
 
 
+def c4_form(cc, steps_scale=1.0):
+    """The served request stream (256 requests at 200/s, two resident models),
+    FP64 and FP32-mixed: latency percentiles; and a burst's throughput."""
+    from chimes_calculator_amd import workloads as W
+    names = ["co_2b3b4b", "carbon_2b3b"]
+    models = [cc.GpuModel(model_file(n)) for n in names]
+    reqs = W.serving_stream(nreq=256, rate=200.0)
+    forms = {"requests": len(reqs), "atoms": sum(len(r[2][0]) for r in reqs)}
+    for prec, tag in ((cc.FP64, "fp64"), (cc.MIXED, "mixed")):
+        for scale in (1.0, 1e9):
+            srv = cc.GpuServer(models, precision=prec, nslots=2, max_batch_atoms=65536, hbm_budget_gb=32.0)
+            lat, thr = serve_stream(cc, srv, names, reqs, scale)
+            st = srv.stats()
+            srv.close()
+            key = "%s_rate200" % tag if scale < 1e6 else "%s_burst" % tag
+            forms[key] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                          "atom_evals_per_s": thr, "batches": st["batches"] - 8, "max_batch": st["max_batch"]}
+    for m in models:
+        m.close()
+    return forms
+
+
 def bench_c4(args):
     """Poisson request stream (two resident models, 16-4096 atoms) served
     with continuous batching: latency percentiles and throughput."""
     import chimes_calculator_amd as cc
     from chimes_calculator_amd import workloads as W
 
-    os.makedirs(os.path.join(ROOT, "_work"), exist_ok=True)
     names = ["co_2b3b4b", "carbon_2b3b"]
-    models = [cc.GpuModel(W.MODELS[n].write(os.path.join(ROOT, "_work", "bench_%s.txt" % n))) for n in names]
+    models = [cc.GpuModel(model_file(n)) for n in names]
     reqs = W.serving_stream(nreq=256, rate=200.0)
     out = {"metric": "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency",
            "unit": "atom-evals/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",

@@ -40,9 +40,10 @@ EXPORTS = {
                      "chimes_gpu_model_max_cutoff", "chimes_gpu_eval_batch", "chimes_gpu_eval_batch_device",
                      "chimes_gpu_plan_create", "chimes_gpu_plan_run", "chimes_gpu_plan_create_device",
                      "chimes_gpu_plan_run_device", "chimes_gpu_plan_info", "chimes_gpu_plan_free",
+                     "chimes_gpu_set_variants", "chimes_gpu_get_variants", "chimes_gpu_plan_set_variants",
                      "chimes_gpu_server_create", "chimes_gpu_server_submit", "chimes_gpu_server_wait",
-                     "chimes_gpu_server_result", "chimes_gpu_server_stats", "chimes_gpu_server_last_error",
-                     "chimes_gpu_server_free",
+                     "chimes_gpu_server_result", "chimes_gpu_server_stats", "chimes_gpu_server_memory",
+                     "chimes_gpu_server_failures", "chimes_gpu_server_last_error", "chimes_gpu_server_free",
                      "chimes_gpu_last_timing", "chimes_gpu_last_counts", "chimes_gpu_compute_clusters",
                      "chimes_gpu_cluster_lists", "chimes_gpu_free", "chimes_gpu_fp_edges",
                      "chimes_gpu_fp_histogram"],
@@ -98,6 +99,9 @@ def lib():
         L.chimes_gpu_plan_info.argtypes = [C.c_void_p, _ip, _dp,
                                            np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
+        L.chimes_gpu_set_variants.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.chimes_gpu_get_variants.argtypes = [C.c_int, _ip]
+        L.chimes_gpu_plan_set_variants.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.chimes_gpu_server_create.restype = C.c_void_p
         L.chimes_gpu_server_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double]
         L.chimes_gpu_server_submit.restype = C.c_long
@@ -107,6 +111,9 @@ def lib():
         L.chimes_gpu_server_result.argtypes = [C.c_void_p, C.c_long, C.POINTER(C.c_double), _dp, _dp, _dp]
         L.chimes_gpu_server_stats.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64,
                                                                                   flags="C_CONTIGUOUS")]
+        L.chimes_gpu_server_memory.argtypes = [C.c_void_p, _dp]
+        L.chimes_gpu_server_failures.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64,
+                                                                                     flags="C_CONTIGUOUS")]
         L.chimes_gpu_server_last_error.restype = C.c_char_p
         L.chimes_gpu_server_free.argtypes = [C.c_void_p]
         L.chimes_gpu_last_timing.argtypes = [C.c_void_p, _dp]
@@ -363,6 +370,20 @@ def eval_batch(models, configs, model_of=None, small=False, precision=FP64):
 VARIANTS = ("mfma64", "mfma32", "vector")
 
 
+def set_variants(var3=-1, var4=-1, device=0):
+    """Kernel variants of the 3- and 4-body terms (indices into VARIANTS, -1
+    keeps) for every later evaluation on `device` outside plans."""
+    if lib().chimes_gpu_set_variants(int(device), int(var3), int(var4)):
+        raise ChimesError(_err())
+
+
+def get_variants(device=0):
+    v = np.zeros(2, np.int32)
+    if lib().chimes_gpu_get_variants(int(device), v):
+        raise ChimesError(_err())
+    return int(v[0]), int(v[1])
+
+
 class GpuPlan:
     """A batch plan (chimes_gpu.h): one batch shape evaluated by replaying a
     captured HIP graph.  Construction evaluates `configs` once (see
@@ -401,8 +422,14 @@ class GpuPlan:
         if self.L.chimes_gpu_plan_run(self.h, pos, types, cells, E, F, S):
             raise ChimesError(_err())
 
+    def set_variants(self, var3=-1, var4=-1):
+        """Forces the kernel variants of this plan (the next run captures again)."""
+        if self.L.chimes_gpu_plan_set_variants(self.h, int(var3), int(var4)):
+            raise ChimesError(_err())
+
     def info(self):
-        """(3-body variant, 4-body variant, tuning ms per (body, variant), (pairs, triplets, quads))."""
+        """(3-body variant, 4-body variant, tuning ms per (body, variant) - negative
+        for a variant the tuner rejected - , (pairs, triplets, quads))."""
         v, t, c = np.zeros(2, np.int32), np.zeros(6), np.zeros(3, np.int64)
         self.L.chimes_gpu_plan_info(self.h, v, t, c)
         return VARIANTS[v[0]], VARIANTS[v[1]], t.reshape(2, 3), tuple(int(x) for x in c)
@@ -450,17 +477,31 @@ class GpuServer:
 
     def result(self, rid):
         """(energy, forces[n, 3], stress[9], (t_submit, t_start, t_done) in ms)."""
-        n = self.natoms.pop(rid)
+        n = self.natoms[rid]
         E = C.c_double()
         F, S, T = np.zeros(3 * n), np.zeros(9), np.zeros(3)
         if self.L.chimes_gpu_server_result(self.h, int(rid), C.byref(E), F, S, T):
-            raise ChimesError(self.L.chimes_gpu_server_last_error().decode())
+            msg = self.L.chimes_gpu_server_last_error().decode()
+            if msg != "request not complete":  # a failed request is released too
+                self.natoms.pop(rid, None)
+            raise ChimesError(msg)
+        self.natoms.pop(rid, None)
         return E.value, F.reshape(n, 3), S, tuple(T)
 
     def stats(self):
         s = np.zeros(4, np.int64)
         self.L.chimes_gpu_server_stats(self.h, s)
-        return dict(requests=int(s[0]), batches=int(s[1]), atoms=int(s[2]), max_batch=int(s[3]))
+        f = np.zeros(2, np.int64)
+        self.L.chimes_gpu_server_failures(self.h, f)
+        return dict(requests=int(s[0]), batches=int(s[1]), atoms=int(s[2]), max_batch=int(s[3]),
+                    failed=int(f[0]), isolated_batches=int(f[1]))
+
+    def memory(self):
+        """Bytes: budget, peak estimated in flight, peak held by the slots; the
+        largest needed/estimated ratio of a batch; batches above their estimate."""
+        m = np.zeros(5)
+        self.L.chimes_gpu_server_memory(self.h, m)
+        return dict(budget=m[0], peak_inflight=m[1], peak_held=m[2], worst_ratio=m[3], underestimated=int(m[4]))
 
     def close(self):
         if getattr(self, "h", None):

@@ -15,6 +15,8 @@
 
 struct chimes_gpu_plan {
     chx_plan *p = nullptr;
+    std::vector<chimes_gpu_model *> models;  // for the type checks of each run
+    std::vector<int> model_of, natoms_of;
     int nconf = 0;
     long natoms = 0;
     std::vector<int> nrep;
@@ -196,6 +198,9 @@ chimes_gpu_plan *chimes_gpu_plan_create(chimes_gpu_model *const *models, int nmo
     if (check_types(models, nmodels, nconf, model_of, natoms, types)) return nullptr;
     std::unique_ptr<chimes_gpu_plan> P(new chimes_gpu_plan);
     P->nconf = nconf;
+    P->models.assign(models, models + nmodels);
+    P->model_of.assign(model_of, model_of + nconf);
+    P->natoms_of.assign(natoms, natoms + nconf);
     P->nrep.resize(nconf);
     P->vol.resize(nconf);
     P->V.resize(6 * (size_t)nconf);
@@ -225,6 +230,8 @@ int chimes_gpu_plan_run(chimes_gpu_plan *P, const double *pos, const int *types,
                         double *energy, double *forces, double *stress)
 {
     if (!P || P->out_dev) return fail("not a host-buffer plan");
+    if (check_types(P->models.data(), (int)P->models.size(), P->nconf, P->model_of.data(), P->natoms_of.data(), types))
+        return 1;
     chx_out o{};
     o.energy = energy;
     o.forces = forces;
@@ -289,6 +296,33 @@ void chimes_gpu_plan_info(const chimes_gpu_plan *P, int variants[2], double tune
 }
 
 void chimes_gpu_plan_free(chimes_gpu_plan *P) { delete P; }
+
+int chimes_gpu_plan_set_variants(chimes_gpu_plan *P, int var3, int var4)
+{
+    if (!P) return fail("no plan");
+    if (chx_plan_set_variants(P->p, var3, var4)) return fail(chx_plan_error(P->p));
+    return 0;
+}
+
+int chimes_gpu_set_variants(int device, int var3, int var4)
+{
+    std::string err;
+    chx::EngineHandle *eh = chx::engine_for_device(device, err);
+    if (!eh) return fail(err);
+    std::lock_guard<std::mutex> lk(eh->mu);
+    if (chx_set_variants(eh->eng, var3, var4)) return fail(chx_last_error(eh->eng));
+    return 0;
+}
+
+int chimes_gpu_get_variants(int device, int variants[2])
+{
+    std::string err;
+    chx::EngineHandle *eh = chx::engine_for_device(device, err);
+    if (!eh) return fail(err);
+    std::lock_guard<std::mutex> lk(eh->mu);
+    chx_get_variants(eh->eng, variants);
+    return 0;
+}
 
 void chimes_gpu_last_timing(const chimes_gpu_model *m, double ms[4])
 {

@@ -112,12 +112,35 @@ int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double 
 void chx_plan_destroy(chx_plan *p);
 const char *chx_plan_error(chx_plan *p);
 // variants[2]: chosen variant of the 3- and 4-body kernels; tune_ms[6]: the
-// timed kernel milliseconds per (body, variant); counts[3]: as chx_last_counts.
+// timed kernel milliseconds per (body, variant), negated for a variant whose
+// results disagreed with the reference evaluation; counts[3]: as
+// chx_last_counts.
 void chx_plan_info(chx_plan *p, int *variants, double *tune_ms, long long *counts);
 
 // Cluster counts of the last evaluate call: [0] pair entries, [1] triplets,
 // [2] quadruplets (all configurations, excluded types included).
 void chx_last_counts(chx_engine *e, long long *c3);
+
+// Device bytes of the engine's work buffers: held_need[0] allocated,
+// held_need[1] asked for by the last evaluation (its own need).
+void chx_engine_bytes(chx_engine *e, size_t *held_need);
+// Frees the work buffers (allocated again on demand) and forgets the grown
+// capacities.
+void chx_engine_trim(chx_engine *e);
+// Forgets the grown capacities: the next evaluation sizes its buffers from its
+// own estimates (allocations are kept and reused when large enough).
+void chx_engine_reset_capacities(chx_engine *e);
+
+// Kernel variants of the 3- and 4-body terms for the engine's later
+// evaluations (CHX_VAR_*: matrix cores with 64- or 32-cluster tiles, vector
+// units; -1 keeps the current one).  A matrix-core request for a dense
+// dimension above the matrix-core kernels' range, and a vector request of a
+// BF16 form, fall back as launch_eval_body does.  Nonzero for an unknown
+// variant.
+int chx_set_variants(chx_engine *e, int var3, int var4);
+void chx_get_variants(chx_engine *e, int *v2);
+// The same for a plan; the next run evaluates directly and captures again.
+int chx_plan_set_variants(chx_plan *p, int var3, int var4);
 
 // Timing of the last evaluate call (HIP events on the engine's stream, ms):
 // [0] whole pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] enumeration.

@@ -51,13 +51,15 @@ template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t cap = 0;
+    size_t need = 0;  // elements the last ensure() asked for
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), cap(o.cap), need(o.need) { o.p = nullptr; o.cap = 0; o.need = 0; }
     ~DevBuf() { release(); }
     void ensure(size_t n)
     {
+        need = n;
         if (n <= cap && p) return;
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -71,6 +73,7 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        need = 0;
     }
 };
 
@@ -383,6 +386,15 @@ struct chx_engine {
     {
         if (!capturing) (void)hipEventRecord(ev[k], st);
     }
+    // Applies f to every device work buffer.
+    template <typename F>
+    void each_buf(F f)
+    {
+        f(as); f(an); f(cell_cnt); f(cell_start); f(cell_cur); f(cell_atoms); f(nbr_cnt); f(nbr_off); f(big);
+        f(es); f(en); f(cnt34); f(off34); f(gb3); f(ge3); f(gb4); f(ge4); f(list3); f(list4); f(Facc); f(Oacc);
+        f(part); f(done); f(in_d); f(out_d); f(cl_in); f(cl_out); f(cl_types); f(cl_flags);
+        for (auto &b : scan_tmp) f(b);
+    }
 };
 
 extern "C" {
@@ -422,6 +434,49 @@ void chx_last_counts(chx_engine *e, long long *c3)
 void chx_last_timing(chx_engine *e, double *ms4)
 {
     for (int k = 0; k < 4; k++) ms4[k] = e->timing[k];
+}
+
+void chx_engine_bytes(chx_engine *e, size_t *held_need)
+{
+    size_t held = 0, need = 0;
+    e->each_buf([&](const auto &b) {
+        held += b.cap * sizeof(*b.p);
+        need += b.need * sizeof(*b.p);
+    });
+    held_need[0] = held;
+    held_need[1] = need;
+}
+
+void chx_engine_trim(chx_engine *e)
+{
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->st);
+    e->each_buf([](auto &b) { b.release(); });
+    e->nb_cap = 0;
+    e->l3_cap = e->l4_cap = 0;
+}
+
+void chx_engine_reset_capacities(chx_engine *e)
+{
+    e->nb_cap = 0;
+    e->l3_cap = e->l4_cap = 0;
+}
+
+int chx_set_variants(chx_engine *e, int var3, int var4)
+{
+    if (var3 < -1 || var3 >= CHX_NVAR || var4 < -1 || var4 >= CHX_NVAR) {
+        e->err = "unknown kernel variant";
+        return 1;
+    }
+    if (var3 >= 0) e->var3 = var3;
+    if (var4 >= 0) e->var4 = var4;
+    return 0;
+}
+
+void chx_get_variants(chx_engine *e, int *v2)
+{
+    v2[0] = e->var3;
+    v2[1] = e->var4;
 }
 
 chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
@@ -1171,9 +1226,75 @@ void plan_capture(chx_plan *p, chx_out *o)
     for (int k = 0; k < 7; k++) p->sig[k] = sig[k];
 }
 
-// Times each 3-/4-body kernel variant on the lists of the last eager run and
-// keeps the fastest per body order.
-void plan_tune(chx_plan *p)
+// Results of the last eager run (raw energy, virial and forces), read from
+// the output blob or from the caller's device buffers.
+struct Snap {
+    std::vector<double> E, V, F;
+};
+
+void snapshot(chx_engine *e, const chx_out *o, Snap &s)
+{
+    const int ncfg = e->ncfg;
+    const long Nin = e->Nin;
+    s.E.resize(ncfg);
+    s.V.resize(6 * (size_t)ncfg);
+    s.F.resize(3 * (size_t)Nin);
+    if (o->out_dev) {
+        HIPCHK(hipMemcpyAsync(s.E.data(), o->energy, ncfg * sizeof(double), hipMemcpyDeviceToHost, e->st));
+        HIPCHK(hipMemcpyAsync(s.V.data(), o->virial, 6 * ncfg * sizeof(double), hipMemcpyDeviceToHost, e->st));
+        HIPCHK(hipMemcpyAsync(s.F.data(), o->forces, 3 * Nin * sizeof(double), hipMemcpyDeviceToHost, e->st));
+        HIPCHK(hipStreamSynchronize(e->st));
+    } else {
+        memcpy(s.E.data(), e->out_h.p + e->o_E, ncfg * sizeof(double));
+        memcpy(s.V.data(), e->out_h.p + e->o_V, 6 * ncfg * sizeof(double));
+        memcpy(s.F.data(), e->out_h.p + e->o_F, 3 * Nin * sizeof(double));
+    }
+}
+
+// Whether the results a of a kernel variant agree with the reference results
+// r within the bars of precision prec, per configuration (n = its replicated
+// atoms; energies and virials are raw sums in kcal/mol, forces kcal/mol/A):
+//   FP64   energy 1e-10 relative, forces 1e-10 of the largest + 1e-9,
+//          virial 1e-10 of the largest component;
+//   MIXED  energy 1e-6 eV/atom, forces 1e-4 eV/A (r is the FP64 result),
+//          virial 1e-5 of the largest component;
+//   BF16 forms: 1e-6 of each quantity's scale (r is the same form's result
+//          with the default variants: the variants differ only in tiling).
+bool agrees(const chx_engine *e, const Snap &a, const Snap &r, int prec)
+{
+    const double kcal_per_ev = 1.0 / 0.0433641;
+    for (int c = 0; c < e->ncfg; c++) {
+        const double n = e->cfg_h[c].natoms;
+        const long f0 = 3L * e->cfg_h[c].atom_in_off, f1 = f0 + 3L * e->cfg_h[c].natoms_in;
+        double fmax = 0, vmax = 0, df = 0, dv = 0;
+        for (long k = f0; k < f1; k++) {
+            fmax = std::max(fmax, fabs(r.F[k]));
+            df = std::max(df, fabs(a.F[k] - r.F[k]));
+        }
+        for (int k = 0; k < 6; k++) {
+            vmax = std::max(vmax, fabs(r.V[6 * c + k]));
+            dv = std::max(dv, fabs(a.V[6 * c + k] - r.V[6 * c + k]));
+        }
+        const double de = fabs(a.E[c] - r.E[c]);
+        bool ok;
+        if (prec == CHX_FP64)
+            ok = de <= 1e-10 * fabs(r.E[c]) + 1e-12 * n && df <= 1e-10 * fmax + 1e-9 && dv <= 1e-10 * vmax + 1e-9;
+        else if (prec == CHX_MIXED)
+            ok = de <= 1e-6 * kcal_per_ev * n && df <= 1e-4 * kcal_per_ev && dv <= 1e-5 * vmax + 1e-6 * n;
+        else
+            ok = de <= 1e-6 * (fabs(r.E[c]) + n) && df <= 1e-6 * (fmax + 1.0) && dv <= 1e-6 * (vmax + n);
+        if (!ok) return false;
+    }
+    return true;
+}
+
+// Times each 3-/4-body kernel variant on the lists of the last eager run, then
+// keeps per body order the fastest variant whose results agree with a
+// reference evaluation (the default variants; in FP64 for the FP32-mixed
+// form) within the precision's bars.  A variant that disagrees is never
+// chosen; its timing is stored negated.  Ends with an eager run of the chosen
+// variants (results in o).
+void plan_tune(chx_plan *p, chx_out *o)
 {
     chx_engine *e = p->e;
     hipEvent_t a, z;
@@ -1198,8 +1319,6 @@ void plan_tune(chx_plan *p)
         la.groups = body == 3 ? e->grp3 : e->grp4;
         la.list = body == 3 ? e->list3.p : e->list4.p;
         const std::vector<int> &Ds = body == 3 ? e->D3s : e->D4s;
-        int best = CHX_VAR_MFMA4;
-        double best_ms = 1e30;
         // the BF16 forms are defined by their matrix-core contraction: no vector variant
         const int nvar = e->prec >= CHX_BF16 ? CHX_VAR_VECTOR : CHX_NVAR;
         for (int var = 0; var < nvar; var++) {
@@ -1214,13 +1333,47 @@ void plan_tune(chx_plan *p)
                 if (rep > 0) best_rep = std::min(best_rep, ms);
             }
             p->tune_ms[body - 3][var] = best_rep;
-            if (best_rep < best_ms) { best_ms = best_rep; best = var; }
         }
-        if (body == 3) e->var3 = best; else e->var4 = best;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(z);
     HIPCHK(hipGetLastError());
+    // reference results: the default variants, in FP64 for the mixed form
+    const int prec = e->prec;
+    Snap ref, cand;
+    if (prec == CHX_MIXED) e->prec = CHX_FP64;
+    try {
+        evaluate_eager(e, o);
+    } catch (...) {
+        e->prec = prec;
+        throw;
+    }
+    e->prec = prec;
+    snapshot(e, o, ref);
+    for (int body = 3; body <= 4; body++) {
+        const int G = body == 3 ? e->G3 : e->G4;
+        if (!G) continue;
+        int &var = body == 3 ? e->var3 : e->var4;
+        const int def = var;
+        const int nvar = prec >= CHX_BF16 ? CHX_VAR_VECTOR : CHX_NVAR;
+        std::vector<int> order;
+        for (int v = 0; v < nvar; v++) order.push_back(v);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return p->tune_ms[body - 3][x] < p->tune_ms[body - 3][y]; });
+        int chosen = -1;
+        for (int v : order) {
+            var = v;
+            evaluate_eager(e, o);
+            snapshot(e, o, cand);
+            if (agrees(e, cand, ref, prec)) {
+                chosen = v;
+                break;
+            }
+            p->tune_ms[body - 3][v] = -fabs(p->tune_ms[body - 3][v]);
+        }
+        var = chosen >= 0 ? chosen : def;
+    }
+    evaluate_eager(e, o);  // results of the chosen variants
 }
 
 }  // namespace
@@ -1254,10 +1407,7 @@ chx_plan *chx_plan_create(int device, chx_model *const *models, int nmodels, con
         prepare(e, models, nmodels, b);
         layout(e, b, !o->out_dev);
         evaluate_eager(e, o);
-        if (tune) {
-            plan_tune(p);
-            evaluate_eager(e, o);  // results of the chosen variants
-        }
+        if (tune) plan_tune(p, o);
         deliver(e, o);
         plan_capture(p, o);
         return p;
@@ -1311,6 +1461,16 @@ int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double 
 }
 
 void chx_plan_destroy(chx_plan *p) { delete p; }
+
+int chx_plan_set_variants(chx_plan *p, int var3, int var4)
+{
+    if (chx_set_variants(p->e, var3, var4)) {
+        p->err = p->e->err;
+        return 1;
+    }
+    plan_release_graph(p);  // the next run evaluates directly and captures again
+    return 0;
+}
 
 const char *chx_plan_error(chx_plan *p) { return p ? p->err.c_str() : "no plan"; }
 

@@ -355,6 +355,25 @@ def serving_stream(nreq=256, rate=200.0, nmin=16, nmax=4096, seed=0xC0DE04):
     return reqs
 
 
+# Test cases, not workloads: two-type C/O models whose 3-/4-body dense
+# dimensions (D3, D4) together cover every dimension the evaluation kernels
+# are instantiated for (3-body 2..13, 4-body 2..6), each with one small
+# jittered-lattice configuration.
+DSCAN = [(2, 2), (3, 3), (4, 4), (5, 5), (6, 6), (7, 2), (8, 3), (9, 4), (10, 2), (11, 3), (12, 4), (13, 2)]
+
+
+def dscan_model(d3, d4):
+    return Model("dscan_%d_%d" % (d3, d4), ["C", "O"], (8, d3, d4), (4.5, 4.0, 3.5), _rin_all(["C", "O"], 0.6),
+                 sigma=(0.05, 0.5, 2.0), seed=0xC0DE10 + 16 * d3 + d4)
+
+
+def dscan_config(d3, d4):
+    """One C/O configuration for the (d3, d4) model: 96 atoms at 2.4 g/cc
+    (48 for the large 4-body dimensions, whose reference evaluation is slow)."""
+    rng = np.random.default_rng(0xC0DE20 + 16 * d3 + d4)
+    return co_config(rng, 48 if d4 >= 5 else 96, 2.4, 0.5)
+
+
 WORKLOADS = {
     "C0": "carbon 64 atoms, L=8.61 A, 2B+3B 12/7, rc 4.0/3.5, CPU-runnable",
     "C1": "O/H 1536 atoms (512 H2O-like), L=24.8 A, 2B+3B+4B 12/8/4, rc 5.0/4.5/4.0",

@@ -87,10 +87,27 @@ int chimes_gpu_plan_run_device(chimes_gpu_plan *p, const double *cells, double *
                                double *virial_dev);
 /* variants[2]: kernel variant of the 3- and 4-body terms (0 matrix cores,
  * 64-cluster tiles; 1 matrix cores, 32-cluster tiles; 2 vector units);
- * tune_ms[6]: timed milliseconds per (body, variant), 0 if not tuned;
+ * tune_ms[6]: timed milliseconds per (body, variant), 0 if not tuned, negated
+ * for a variant the tuner rejected: the tuner keeps the fastest variant whose
+ * results agree with a reference evaluation of the batch (the default
+ * variants, in FP64 for the FP32-mixed form) within the precision's bars;
  * counts[3]: pair images, triplets, quadruplets of the last run. */
 void chimes_gpu_plan_info(const chimes_gpu_plan *p, int variants[2], double tune_ms[6], long long counts[3]);
 void chimes_gpu_plan_free(chimes_gpu_plan *p);
+
+/* Kernel variants of the 3- and 4-body terms (numbered as in
+ * chimes_gpu_plan_info; -1 keeps the current one).  chimes_gpu_set_variants
+ * sets them for every later evaluation on `device` that does not go through
+ * a plan (chimes_gpu_eval_batch*, calculate_chimes*, the chimesFF classes);
+ * the defaults are 2 (vector units) for the 3-body and 0 (matrix cores,
+ * 64-cluster tiles) for the 4-body term.  chimes_gpu_plan_set_variants sets
+ * them for a plan, whose next run evaluates directly and captures again.  A
+ * matrix-core variant for a dense dimension above the matrix-core kernels'
+ * range (4-body D > 4) runs on the vector units, and a vector-unit request of
+ * a BF16 form on the matrix cores.  Nonzero for an unknown variant. */
+int chimes_gpu_set_variants(int device, int var3, int var4);
+int chimes_gpu_get_variants(int device, int variants[2]);
+int chimes_gpu_plan_set_variants(chimes_gpu_plan *p, int var3, int var4);
 
 /* Serving: a continuous-batching request loop over resident models.  Each of
  * nslots workers owns a GPU stream and device buffers (kept between batches)
@@ -98,7 +115,9 @@ void chimes_gpu_plan_free(chimes_gpu_plan *p);
  * and the HBM budget (estimated device bytes of all requests in flight) as one
  * multi-model batch; batches of different slots overlap on the device.
  * Requests are single configurations with host inputs (copied at submit).
- * Every result is bit-identical to evaluating the request alone. */
+ * Every result is bit-identical to evaluating the request alone.  A batch that
+ * fails is evaluated again request by request, so an error fails only the
+ * request that causes it. */
 typedef struct chimes_gpu_server chimes_gpu_server;
 chimes_gpu_server *chimes_gpu_server_create(chimes_gpu_model *const *models, int nmodels, int precision,
                                             int nslots, long max_batch_atoms, double hbm_budget_gb);
@@ -115,6 +134,16 @@ int chimes_gpu_server_result(chimes_gpu_server *s, long id, double *energy, doub
                              double times_ms[3]);
 /* [0] requests served, [1] batches, [2] atoms, [3] largest batch (requests). */
 void chimes_gpu_server_stats(chimes_gpu_server *s, long long stats[4]);
+/* Memory accounting in bytes: [0] the budget; [1] the largest estimated
+ * device bytes of the batches in flight together; [2] the largest device
+ * bytes held by the slots' buffers together (a slot releases its buffers
+ * when the slots hold more than the budget); [3] the largest ratio of the
+ * bytes a batch's evaluation asked for to its estimate; [4] batches that
+ * asked for more than their estimate. */
+void chimes_gpu_server_memory(chimes_gpu_server *s, double out[5]);
+/* [0] failed requests, [1] batches that failed and were evaluated again
+ * request by request (so that an error fails only its own request). */
+void chimes_gpu_server_failures(chimes_gpu_server *s, long long out[2]);
 const char *chimes_gpu_server_last_error(void);
 /* Finishes the pending requests and stops the workers. */
 void chimes_gpu_server_free(chimes_gpu_server *s);

@@ -45,6 +45,28 @@ def model_dir(tmp_path_factory):
 
 
 @pytest.fixture(scope="session")
+def dscan_dir(tmp_path_factory):
+    """Parameter files of the dense-dimension scan models (workloads.DSCAN)."""
+    from chimes_calculator_amd import workloads as W
+    d = tmp_path_factory.mktemp("dscan")
+    for d3, d4 in W.DSCAN:
+        W.dscan_model(d3, d4).write(os.path.join(d, "dscan_%d_%d.txt" % (d3, d4)))
+    return str(d)
+
+
+@pytest.fixture(scope="session")
+def dscan_cases():
+    with open(os.path.join(GOLDEN, "dscan_cases.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="session")
+def c3_case():
+    with gzip.open(os.path.join(GOLDEN, "c3_case.json.gz"), "rt") as f:
+        return json.load(f)["C3|0"]
+
+
+@pytest.fixture(scope="session")
 def ref_cases():
     with open(os.path.join(GOLDEN, "ref_cases.json")) as f:
         return json.load(f)
@@ -59,6 +81,34 @@ def synth_cases():
 def golden_arrays(rec):
     return (float(rec["energy"]), np.array([[float(v) for v in r] for r in rec["forces"]]),
             np.array([float(v) for v in rec["stress"]]))
+
+
+EV = 0.0433641       # eV per kcal/mol
+MIXED_E_TOL = 1e-6   # eV/atom
+MIXED_F_TOL = 1e-4   # eV/A
+MIXED_S_TOL = 1e-5   # of the largest stress component
+
+
+def mixed_errors(E, F, S, rec, natoms):
+    """Errors of one configuration's result against its FP64 golden record:
+    energy in eV/atom, largest force component error in eV/A, stress relative
+    to the largest golden component."""
+    e0, f0, s0 = golden_arrays(rec)
+    de = abs(E - e0) * EV / natoms
+    df = float(np.abs(np.asarray(F) - f0).max()) * EV
+    ds = float(np.abs(np.asarray(S) - s0).max()) / max(np.abs(s0).max(), 1e-300)
+    return de, df, ds
+
+
+def assert_mixed(tag, E, F, S, rec, natoms):
+    """The FP32-mixed bars against the FP64 golden: energy within 1e-6 eV/atom,
+    every force component within 1e-4 eV/A, stress within 1e-5 of its
+    largest component."""
+    de, df, ds = mixed_errors(E, F, S, rec, natoms)
+    assert de <= MIXED_E_TOL, "%s: energy error %.3g eV/atom" % (tag, de)
+    assert df <= MIXED_F_TOL, "%s: force error %.3g eV/A" % (tag, df)
+    assert ds <= MIXED_S_TOL, "%s: stress error %.3g (relative)" % (tag, ds)
+    return de, df, ds
 
 
 def assert_close(name, got, ref, rtol, scale=None, atol=0.0):

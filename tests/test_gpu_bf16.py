@@ -1,4 +1,5 @@
-"""BF16 matrix-core contractions: error against FP64, reported per quantity.
+"""BF16 matrix-core contractions: error against the reference's FP64
+goldens, reported per quantity.
 
 Plain BF16 keeps 8 mantissa bits, so it has no 1e-6 eV/atom bar: on the
 reference's fitted force fields (large, cancelling 3-/4-body coefficients)
@@ -12,9 +13,10 @@ import os
 import numpy as np
 import pytest
 
-pytestmark = pytest.mark.gpu
+from conftest import mixed_errors
+from oracle_bindings import read_xyz
 
-EV = 0.0433641
+pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
@@ -23,38 +25,41 @@ def cc():
     return cc
 
 
-def errors(cc, m, cfgs, prec, small=False):
-    E64, F64, _ = cc.eval_batch(m, cfgs, precision=cc.FP64, small=small)
-    Eb, Fb, _ = cc.eval_batch(m, cfgs, precision=prec, small=small)
-    n = np.array([len(c[0]) for c in cfgs])
-    eb = float((np.abs(Eb - E64) * EV / n).max())
-    fb = max(float(np.abs(a - b).max()) for a, b in zip(Fb, F64)) * EV
-    return eb, fb
+def errors(cc, m, cfgs, recs, prec, small=False):
+    E, F, S = cc.eval_batch(m, cfgs, precision=prec, small=small)
+    errs = np.array([mixed_errors(E[k], F[k], S[k], recs[k], len(cfgs[k][0])) for k in range(len(cfgs))])
+    return errs.max(0)
 
 
 def test_bf16_reference_models(cc, ref_dir, ref_cases):
-    from oracle_bindings import read_xyz
-    for key in ref_cases:
+    for key, rec in ref_cases.items():
         pf, xyz, small = key.split("|")
         pos, types, cell = read_xyz(os.path.join(ref_dir, xyz))
         m = cc.GpuModel(os.path.join(ref_dir, pf))
         cfg = [(pos, types, cell)]
-        eb, fb = errors(cc, m, cfg, cc.BF16, small=bool(int(small)))
-        e3, f3 = errors(cc, m, cfg, cc.BF16X3, small=bool(int(small)))
-        print("%s bf16: energy %.3g eV/atom, force %.3g eV/A; bf16x3: energy %.3g, force %.3g"
-              % (key, eb, fb, e3, f3))
+        eb, fb, sb = errors(cc, m, cfg, [rec], cc.BF16, small=bool(int(small)))
+        e3, f3, s3 = errors(cc, m, cfg, [rec], cc.BF16X3, small=bool(int(small)))
+        print("%s bf16: energy %.3g eV/atom, force %.3g eV/A, stress %.3g; bf16x3: %.3g, %.3g, %.3g"
+              % (key, eb, fb, sb, e3, f3, s3))
         assert eb <= 0.5 and fb <= 20.0
         assert e3 <= 2e-3 and f3 <= 0.1
         assert e3 <= eb / 30 + 1e-12 and f3 <= fb / 30 + 1e-12
 
 
-@pytest.mark.parametrize("wl", ["C0", "C1", "C2"])
-def test_bf16_workloads(cc, model_dir, wl):
+@pytest.mark.parametrize("wl,n", [("C0", 1), ("C1", 1), ("C2", 24), ("C3", 1)])
+def test_bf16_workloads(cc, model_dir, synth_cases, c3_case, wl, n):
     from chimes_calculator_amd import workloads as W
     mname, cfgs = W.workload(wl)
     m = cc.GpuModel(os.path.join(model_dir, mname + ".txt"))
-    eb, fb = errors(cc, m, cfgs[:64], cc.BF16)
-    e3, f3 = errors(cc, m, cfgs[:64], cc.BF16X3)
-    print("%s bf16: energy %.3g eV/atom, force %.3g eV/A; bf16x3: energy %.3g, force %.3g" % (wl, eb, fb, e3, f3))
+    if wl == "C3":
+        recs = [c3_case]
+    elif wl == "C0":
+        recs = [synth_cases["C0|0"]]
+    else:
+        recs = [synth_cases["%s|%d" % (wl, k)] for k in range(n)]
+    eb, fb, sb = errors(cc, m, cfgs[:n], recs, cc.BF16)
+    e3, f3, s3 = errors(cc, m, cfgs[:n], recs, cc.BF16X3)
+    print("%s bf16: energy %.3g eV/atom, force %.3g eV/A, stress %.3g; bf16x3: %.3g, %.3g, %.3g"
+          % (wl, eb, fb, sb, e3, f3, s3))
     assert eb <= 1e-5 and fb <= 1e-3
     assert e3 <= 1e-7 and f3 <= 1e-5

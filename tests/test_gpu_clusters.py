@@ -17,13 +17,6 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 
-def f64(v):
-    v = v.strip()
-    if v.startswith("np.float64("):
-        v = v[len("np.float64("):-1]
-    return float(v)
-
-
 @pytest.fixture(scope="module")
 def vectors():
     with open(os.path.join(GOLDEN, "cluster_vectors.json")) as f:
@@ -61,9 +54,9 @@ def test_per_cluster_c_abi(ref_dir, vectors):
                 L.chimes_compute_3b_props(D3(*dx), D9(*dr), tn, force, stress, C.byref(epot))
             else:
                 L.chimes_compute_4b_props(D6(*dx), (C.c_double * 18)(*dr), tn, force, stress, C.byref(epot))
-            e0 = f64(row["energy"])
-            f0 = np.array([f64(v) for v in row["force"]])
-            s0 = np.array([f64(v) for v in row["stress"]])
+            e0 = float(row["energy"])
+            f0 = np.array(row["force"], dtype=np.float64)
+            s0 = np.array(row["stress"], dtype=np.float64)
             f1 = np.array(force[:])
             s1 = np.array(stress[:6])
             assert abs(epot.value - e0) <= 1e-10 * abs(e0) + 1e-14, (key, epot.value, e0)
@@ -95,9 +88,9 @@ def test_batched_clusters(ref_dir, vectors):
         types = np.stack([m.types_to_index([names[t] for t in row["types"]]) for row in rows])
         E, F, S = m.compute_clusters(nb, dist, dr, types)
         for k, row in enumerate(rows):
-            e0 = f64(row["energy"])
-            f0 = np.array([f64(v) for v in row["force"]])
-            s0 = np.array([f64(v) for v in row["stress"]])
+            e0 = float(row["energy"])
+            f0 = np.array(row["force"], dtype=np.float64)
+            s0 = np.array(row["stress"], dtype=np.float64)
             assert abs(E[k] - e0) <= 1e-10 * abs(e0) + 1e-14, (key, k, E[k], e0)
             fs = max(np.abs(f0).max(), 1e-300)
             assert np.abs(F[k].reshape(-1) - f0).max() <= 1e-10 * fs + 1e-12, (key, k)

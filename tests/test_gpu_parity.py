@@ -171,3 +171,14 @@ def test_clustered_capacity_retry(cc, model_dir):
     assert abs(E[0] - r.energy) <= RTOL * abs(r.energy), (E[0], r.energy)
     assert_close("clustered forces", F[0], r.forces, RTOL, atol=FATOL)
     assert_close("clustered stress", S[0], r.stress, RTOL)
+
+
+def test_c3_carbon(cc, model_dir, c3_case):
+    """The 32768-atom carbon box at 2B/3B/4B orders 16/10/4 (3-body D = 10,
+    4-body D = 4, 2-body order 16)."""
+    from chimes_calculator_amd import workloads as W
+    mname, cfgs = W.workload("C3")
+    m = cc.GpuModel(os.path.join(model_dir, mname + ".txt"))
+    E, F, S = cc.eval_batch(m, cfgs)
+    check_efs("C3", E[0], F[0], S[0], c3_case)
+    check_lists(m, c3_case)
