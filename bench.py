@@ -509,6 +509,13 @@ def bench_c2(args):
     L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
     L.chimes_gpu_plan_run_device.argtypes = [C.c_void_p] * 5
     hip = Hip()
+    if args.calib_copy_mib > 0:  # a plain copy of known size, for HBM counter calibration
+        nb = args.calib_copy_mib << 20
+        a, b = hip.alloc(nb), hip.alloc(nb)
+        assert hip.L.hipMemcpy(b, a, nb, 3) == 0
+        hip.sync()
+        hip.L.hipFree(a)
+        hip.L.hipFree(b)
     head_prec = cc.MIXED if args.precision == "mixed" else cc.FP64
     forms = {}
     head, outs, variants, counts = c2_plan(cc, L, hip, model, cfgs, head_prec, args.steps, args.warmup)

@@ -48,7 +48,7 @@ EXPORTS = {
                      "chimes_gpu_cluster_lists", "chimes_gpu_free", "chimes_gpu_fp_edges",
                      "chimes_gpu_fp_histogram"],
     "chimes_lmp.h": ["chimes_lmp_create", "chimes_lmp_free", "chimes_lmp_build_mb_neighlists",
-                     "chimes_lmp_compute", "chimes_lmp_mb_lists", "chimes_lmp_timing"],
+                     "chimes_lmp_compute", "chimes_lmp_mb_lists", "chimes_lmp_fp_rows", "chimes_lmp_timing"],
 }
 
 
@@ -138,6 +138,8 @@ def lib():
                                          C.POINTER(C.c_double), _dp, _dp, _dp]
         L.chimes_lmp_mb_lists.argtypes = [C.c_void_p, C.POINTER(C.c_long), pp, C.POINTER(C.c_long), pp]
         L.chimes_lmp_timing.argtypes = [C.c_void_p, _dp]
+        L.chimes_lmp_fp_rows.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long),
+                                         C.POINTER(C.POINTER(C.c_double))]
         _lib = L
     return _lib
 
@@ -328,6 +330,20 @@ class LmpPair:
         t = np.zeros(4)
         self.L.chimes_lmp_timing(self.h, t)
         return t
+
+    def fp_rows(self, nb):
+        """(distances[n, P], types[n, nb]) of the FINGERPRINT rows of the last
+        compute, made on the GPU (chimes_lmp_fp_rows)."""
+        n = C.c_long()
+        p = C.POINTER(C.c_double)()
+        if self.L.chimes_lmp_fp_rows(self.h, nb, C.byref(n), C.byref(p)):
+            raise ChimesError(_err())
+        P = nb * (nb - 1) // 2
+        if n.value == 0:
+            return np.zeros((0, P)), np.zeros((0, nb), np.int32)
+        a = np.ctypeslib.as_array(p, shape=(n.value * (P + nb),)).reshape(n.value, P + nb).copy()
+        self.L.chimes_gpu_free(C.cast(p, C.c_void_p))
+        return a[:, :P], a[:, P:].astype(np.int32)
 
     def close(self):
         if getattr(self, "h", None):

@@ -443,6 +443,13 @@ int chimes_lmp_mb_lists(chimes_lmp *p, long *n3, int **trips, long *n4, int **qu
     return 0;
 }
 
+int chimes_lmp_fp_rows(chimes_lmp *p, int nb, long *nrows, double **rows)
+{
+    if (!p) return fail("no pair style");
+    if (chx_lmp_fp_rows(p->p, nb, nrows, rows)) return fail(chx_lmp_error(p->p));
+    return 0;
+}
+
 void chimes_lmp_timing(const chimes_lmp *p, double ms[4])
 {
     if (p) chx_lmp_timing(p->p, ms);
@@ -453,57 +460,25 @@ void chimes_lmp_timing(const chimes_lmp *p, double ms[4])
 // ---------------------------------------------------------------------------
 // Cluster fingerprints (chimesFF/src/FP/calc_histograms.cpp)
 // ---------------------------------------------------------------------------
-namespace {
-
-// transform (calc_histograms.cpp:126-134): the Morse-transformed distance,
-// 1 at the inner cutoff and -1 at the outer one
-double fp_transform(double rcin, double rcout, double lambda, double rij)
-{
-    const double x_min = exp(-1 * rcin / lambda);
-    const double x_max = exp(-1 * rcout / lambda);
-    const double x_avg = 0.5 * (x_max + x_min);
-    double x_diff = 0.5 * (x_max - x_min);
-    x_diff *= -1.0;
-    return (exp(-1 * rij / lambda) - x_avg) / x_diff;
-}
-
-}  // namespace
-
 extern "C" {
 
 int chimes_gpu_fp_edges(chimes_gpu_model *m, int nb, long n, const double *dist, const int *types, double *edges)
 {
     if (!m || nb < 2 || nb > 4 || n < 0) return fail("cluster size must be 2, 3 or 4");
     const chx::ParamSet &P = m->dm->P;
-    const int nt = P.natmtyps, np = nb * (nb - 1) / 2;
-    static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3}, pa3[3] = {0, 0, 1}, pb3[3] = {1, 2, 2};
+    const int nt = P.natmtyps;
     for (long c = 0; c < n; c++) {
         const int *t = types + c * nb;
         for (int a = 0; a < nb; a++)
             if (t[a] < 0 || t[a] >= nt) return fail("atom type index out of range");
         int tuple = 0;
         for (int a = 0; a < nb; a++) tuple = tuple * nt + t[a];
-        const chx::ClusterType *ct = nullptr;
-        const int *mapped = nullptr;
-        if (nb == 3 || nb == 4) {
-            const int tau = nb == 3 ? P.tripmap[tuple] : P.quadmap[tuple];
-            if (tau < 0) return fail("cluster of an excluded type");
-            ct = nb == 3 ? &P.trip[tau] : &P.quad[tau];
-            mapped = (nb == 3 ? P.trip_mapped.data() : P.quad_mapped.data()) + (size_t)tuple * np;
-        }
-        double *e = edges + c * np;
-        for (int p = 0; p < np; p++) {
-            const int a = nb == 3 ? pa3[p] : pa[p], b = nb == 3 ? pb3[p] : pb[p];
-            const int pair = P.pairmap[t[a] * nt + t[b]];
-            double rin = P.pair_rin[pair], rout = P.pair_rout[pair];
-            if (ct) {
-                rin = ct->rin[mapped[p]];
-                rout = ct->rout[mapped[p]];
-            }
-            e[p] = fp_transform(rin, rout, P.morse[pair], dist[c * np + p]);
-        }
-        std::sort(e, e + np);
+        if ((nb == 3 && P.tripmap[tuple] < 0) || (nb == 4 && P.quadmap[tuple] < 0))
+            return fail("cluster of an excluded type");
     }
+    chx::EngineHandle *eh = m->dm->eh;
+    std::lock_guard<std::mutex> lk(eh->mu);
+    if (chx_fp_edges(eh->eng, m->dm->m, nb, n, dist, types, edges)) return fail(chx_last_error(eh->eng));
     return 0;
 }
 

@@ -21,6 +21,49 @@ __device__ __forceinline__ double pick(const double (&a)[P], int idx)
     return v;
 }
 
+// Canonical pair slot q of a cluster type and the runtime pair rt it takes:
+// canonical member a is runtime member perm[a] (first-match permutation,
+// chimesFF.h:480-526), runtime pairs in the order 01 02 03 12 13 23.
+template <int NB>
+__device__ __forceinline__ int runtime_pair(const int *perm, int q)
+{
+    constexpr int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    constexpr int pa3[3] = {0, 0, 1}, pb3[3] = {1, 2, 2};
+    const int a = perm[NB == 3 ? pa3[q] : pa[q]], b = perm[NB == 3 ? pb3[q] : pb[q]];
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    int idx = 0;
+    for (int x = 0; x < lo; x++) idx += NB - 1 - x;
+    return idx + hi - lo - 1;
+}
+
+// Whether compute_nB evaluates a cluster of types ty with runtime pair
+// distances dist: its type is not excluded and every pair is below the outer
+// cutoff of its slot (chimesFF.cpp:1691-1692, 1989-2012, 2575-2600).
+template <int NB>
+__device__ __forceinline__ bool cluster_inside(const chx_model_dev &M, const int (&ty)[NB],
+                                               const double (&dist)[NB * (NB - 1) / 2])
+{
+    constexpr int P = NB * (NB - 1) / 2;
+    if constexpr (NB == 2) {
+        return dist[0] < M.slots2[M.pairmap[ty[0] * M.nt + ty[1]]].rout;
+    } else {
+        int tup = 0;
+        for (int a = 0; a < NB; a++) tup = tup * M.nt + ty[a];
+        const int tau = NB == 3 ? M.tripmap[tup] : M.quadmap[tup];
+        if (tau < 0) return false;
+        const int *perm = (NB == 3 ? M.trip_perm : M.quad_perm) + tup * NB;
+        const chx_slot *sl = (NB == 3 ? M.slots3 : M.slots4) + tau * P;
+        bool in = true;
+        for (int q = 0; q < P; q++) {
+            const int rt = runtime_pair<NB>(perm, q);
+            double r = dist[0];
+            for (int t = 1; t < P; t++) r = rt == t ? dist[t] : r;
+            in = in && r < sl[q].rout;
+        }
+        return in;
+    }
+}
+
 // dist[P] / drv[P][3]: pair distances and vectors x_b - x_a in runtime pair
 // order (01 02 03 12 13 23), ty[NB] the members' type indices.  Returns the
 // energy Ec, member forces fo, the stress So (xx xy xz yy yz zz, -sum f dr dr),

@@ -146,6 +146,10 @@ int chx_plan_set_variants(chx_plan *p, int var3, int var4);
 // [0] whole pipeline, [1] 4-body kernels, [2] 3-body kernels, [3] enumeration.
 void chx_last_timing(chx_engine *e, double *ms4);
 
+// Sorted Morse-transformed edges of cluster rows on the device
+// (read_flat_clusters, calc_histograms.cpp:140-249): dist[n*P], types[n*nb]
+// (checked by the caller) -> edges[n*P] (host arrays).
+int chx_fp_edges(chx_engine *e, chx_model *m, int nb, long n, const double *dist, const int *types, double *edges);
 // Cluster fingerprint histogram (calc_histograms.cpp:351-489): counts[nbin]
 // of the distances between the np-edge clusters e1[n1] and e2[n2] (host
 // arrays; same != 0: one set, pairs j > i).  ms: kernel milliseconds.
@@ -172,6 +176,10 @@ int chx_lmp_compute(chx_lmp *p, int nall, const double *x, const int *type, cons
 // Many-body lists of the last build: (i, j, k) and (i, j, k, l) atom indices
 // in the reference's loop order; malloc'd, free with chx_free.
 int chx_lmp_lists(chx_lmp *p, long *n3, int **trips, long *n4, int **quads);
+// Fingerprint rows of the last compute (FINGERPRINT build of pair_chimes.cpp):
+// nb(nb-1)/2 pair distances then nb types per row, malloc'd, in the loops'
+// order; free with chx_free.
+int chx_lmp_fp_rows(chx_lmp *p, int nb, long *nrows, double **rows);
 // Milliseconds of the last compute (HIP events): [0] input copies, [1] list
 // build, [2] 1-/2-body, [3] 3-/4-body, read-back of tallies.
 void chx_lmp_timing(chx_lmp *p, double *ms4);

@@ -36,6 +36,12 @@ void launch_lmp_lists(int mode, const chx_lmp_args &A, double pad3, double pad4,
                       const long long *off3, const long long *off4, int *trips, int *quads, hipStream_t st);
 void launch_lmp_clusters(int nb, int D, long n, const int *list, const chx_lmp_args &A, const chx_model_dev &M,
                          hipStream_t st);
+// Fingerprint rows of nb-body clusters (n loop entries: list entries, or for
+// nb = 2 the packed neighbour-row entries): mode 0 counts rows per block of
+// 256 entries into blk_cnt, mode 1 writes them from blk_off (rows of
+// nb(nb-1)/2 distances then nb types).
+void launch_lmp_fp_rows(int nb, int mode, long n, const int *list, const chx_lmp_args &A, const chx_model_dev &M,
+                        int *blk_cnt, const long long *blk_off, double *rows, hipStream_t st);
 // F[nall*3], eatom[nall] (if Eacc), vatom[nall*6] (if Vacc), glob[7] (if
 // Gacc; gsum[14] and *done zeroed scratch of the global reduction)
 void launch_lmp_out(const chx_lmp_args &A, double *F, double *eatom, double *vatom, double *glob,

@@ -1626,8 +1626,12 @@ struct chx_lmp {
     std::vector<long long> scan_h;
     std::vector<double> out_h;
     DevBuf<double> x, out;
-    DevBuf<int> ctype, tag, ilist, nbr_off, nbr, cnt, trips, quads, status;
-    DevBuf<long long> off;
+    DevBuf<int> ctype, tag, ilist, nbr_off, nbr, cnt, trips, quads, status, fp_cnt;
+    DevBuf<long long> off, fp_off;
+    DevBuf<double> fp_rows;
+    std::vector<int> fp_cnt_h;
+    std::vector<long long> fp_off_h;
+    int types_nall = -1;  // atoms whose ChIMES types are on the device
     DevBuf<unsigned long long> acc;
     long n3 = 0, n4 = 0;
     bool have_lists = false;
@@ -1717,6 +1721,9 @@ void lmp_upload(chx_lmp *p, int nall, const double *x, const int *type, const in
             p->ctype_h[a] = p->chimes_type[type[a] - 1];
         }
         HIPCHK(hipMemcpyAsync(p->ctype.p, p->ctype_h.data(), sizeof(int) * nall, hipMemcpyHostToDevice, st));
+        p->types_nall = nall;
+    } else if (p->types_nall != nall) {
+        p->types_nall = -1;
     }
     p->status.ensure(CHX_LMP_ST_WORDS);
     HIPCHK(hipMemsetAsync(p->status.p, 0, sizeof(int) * CHX_LMP_ST_WORDS, st));
@@ -1921,6 +1928,61 @@ int chx_lmp_compute(chx_lmp *p, int nall, const double *x, const int *type, cons
     }
 }
 
+int chx_lmp_fp_rows(chx_lmp *p, int nb, long *nrows, double **rows)
+{
+    *nrows = 0;
+    *rows = nullptr;
+    try {
+        HIPCHK(hipSetDevice(p->device));
+        if (nb < 2 || nb > 4) throw EngineError("cluster size must be 2, 3 or 4");
+        if (p->types_nall != p->nall || p->nall <= 0)
+            throw EngineError("no compute has run on these atoms: rows follow the last compute");
+        const chx_model *m = p->m;
+        if ((nb == 3 && m->d.order[1] == 0) || (nb == 4 && m->d.order[2] == 0)) return 0;
+        if (nb > 2 && (!p->have_lists || p->lists_nall != p->nall || p->lists_inum != p->inum))
+            throw EngineError("no many-body lists for these atoms");
+        const long n = nb == 2 ? (long)p->off_h[p->inum] : (nb == 3 ? p->n3 : p->n4);
+        if (n == 0) return 0;
+        const int *list = nb == 3 ? p->trips.p : (nb == 4 ? p->quads.p : nullptr);
+        const int w = nb * (nb - 1) / 2 + nb;
+        const long nb_blk = (n + 255) / 256;
+        hipStream_t st = p->st;
+        const chx_lmp_args A = p->args();
+        p->fp_cnt.ensure(nb_blk);
+        p->fp_off.ensure(nb_blk);
+        launch_lmp_fp_rows(nb, 0, n, list, A, m->dev, p->fp_cnt.p, nullptr, nullptr, st);
+        HIPCHK(hipGetLastError());
+        p->fp_cnt_h.resize(nb_blk);
+        p->fp_off_h.resize(nb_blk);
+        HIPCHK(hipMemcpyAsync(p->fp_cnt_h.data(), p->fp_cnt.p, sizeof(int) * nb_blk, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        long long total = 0;
+        for (long b = 0; b < nb_blk; b++) {
+            p->fp_off_h[b] = total;
+            total += p->fp_cnt_h[b];
+        }
+        if (total == 0) return 0;
+        double *out = (double *)malloc(sizeof(double) * total * w);
+        if (!out) throw EngineError("out of host memory for the rows");
+        p->fp_rows.ensure((size_t)total * w);
+        HIPCHK(hipMemcpyAsync(p->fp_off.p, p->fp_off_h.data(), sizeof(long long) * nb_blk, hipMemcpyHostToDevice,
+                              st));
+        launch_lmp_fp_rows(nb, 1, n, list, A, m->dev, nullptr, p->fp_off.p, p->fp_rows.p, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, p->fp_rows.p, sizeof(double) * total * w, hipMemcpyDeviceToHost, st));
+        if (hipStreamSynchronize(st) != hipSuccess) {
+            free(out);
+            throw EngineError("row read-back failed");
+        }
+        *nrows = (long)total;
+        *rows = out;
+        return 0;
+    } catch (const EngineError &x) {
+        p->err = x.msg;
+        return 1;
+    }
+}
+
 int chx_lmp_lists(chx_lmp *p, long *n3, int **trips, long *n4, int **quads)
 {
     try {
@@ -1949,6 +2011,33 @@ void chx_lmp_timing(chx_lmp *p, double *ms4)
 // ---------------------------------------------------------------------------
 // Cluster fingerprint histogram (chimesFF/src/FP/calc_histograms.cpp:351-489)
 // ---------------------------------------------------------------------------
+extern "C" int chx_fp_edges(chx_engine *e, chx_model *m, int nb, long n, const double *dist, const int *types,
+                            double *edges)
+{
+    try {
+        HIPCHK(hipSetDevice(e->device));
+        if (nb < 2 || nb > 4 || n < 0) throw EngineError("cluster size must be 2, 3 or 4");
+        if ((nb == 3 && m->d.order[1] == 0) || (nb == 4 && m->d.order[2] == 0))
+            throw EngineError("model has no terms of this body order");
+        if (n == 0) return 0;
+        const int P = nb * (nb - 1) / 2;
+        hipStream_t st = e->st;
+        e->cl_in.ensure((size_t)n * P * 2);
+        e->cl_types.ensure((size_t)n * nb);
+        double *d_dist = e->cl_in.p, *d_edges = e->cl_in.p + (size_t)n * P;
+        HIPCHK(hipMemcpyAsync(d_dist, dist, sizeof(double) * n * P, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(e->cl_types.p, types, sizeof(int) * n * nb, hipMemcpyHostToDevice, st));
+        launch_fp_edges(nb, n, d_dist, e->cl_types.p, m->dev, d_edges, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(edges, d_edges, sizeof(double) * n * P, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    } catch (const EngineError &x) {
+        e->err = x.msg;
+        return 1;
+    }
+}
+
 extern "C" int chx_fp_histogram(chx_engine *e, int np, long n1, const double *e1, long n2, const double *e2,
                                 int same, int nbin, double binw, double maxd, long long *counts, double *ms)
 {

@@ -15,9 +15,86 @@
 
 #include <algorithm>
 
+#include "chx_cluster.h"
 #include "chx_fp.h"
 
 namespace chx {
+
+// ---------------------------------------------------------------------------
+// read_flat_clusters (calc_histograms.cpp:140-249) on the device: each cluster
+// row (runtime pair distances, member types) becomes its NP Morse-transformed
+// edge lengths, sorted ascending.  Pair p of the row uses the inner and outer
+// cutoffs of its slot in the cluster type (the 2-body pair's own for NB = 2)
+// and the Morse lambda of its pair type; transform() (calc_histograms.cpp:
+// 126-134) is kept operation by operation (this file is built without FMA
+// contraction).  The caller has checked the types (range, no excluded type).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double fp_transform(double rcin, double rcout, double lambda, double rij)
+{
+    const double x_min = exp(-1 * rcin / lambda);
+    const double x_max = exp(-1 * rcout / lambda);
+    const double x_avg = 0.5 * (x_max + x_min);
+    double x_diff = 0.5 * (x_max - x_min);
+    x_diff *= -1.0;
+    return (exp(-1 * rij / lambda) - x_avg) / x_diff;
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_fp_edges(long n, const double *__restrict__ dist,
+                                                  const int *__restrict__ types, chx_model_dev M, double *edges)
+{
+    constexpr int P = NB * (NB - 1) / 2;
+    constexpr int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    constexpr int pa3[3] = {0, 0, 1}, pb3[3] = {1, 2, 2};
+    const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    int ty[NB];
+#pragma unroll
+    for (int a = 0; a < NB; a++) ty[a] = types[c * NB + a];
+    double e[P];
+    int tup = 0;
+#pragma unroll
+    for (int a = 0; a < NB; a++) tup = tup * M.nt + ty[a];
+    const int tau = NB == 2 ? 0 : (NB == 3 ? M.tripmap[tup] : M.quadmap[tup]);
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int a = NB == 3 ? pa3[p] : pa[p], b = NB == 3 ? pb3[p] : pb[p];
+        const chx_slot &s2 = M.slots2[M.pairmap[ty[a] * M.nt + ty[b]]];
+        double rin = s2.rin, rout = s2.rout;
+        if constexpr (NB > 2) {
+            // the canonical slot whose runtime pair is p
+            const int *perm = (NB == 3 ? M.trip_perm : M.quad_perm) + tup * NB;
+            const chx_slot *sl = (NB == 3 ? M.slots3 : M.slots4) + tau * P;
+            for (int q = 0; q < P; q++)
+                if (runtime_pair<NB>(perm, q) == p) {
+                    rin = sl[q].rin;
+                    rout = sl[q].rout;
+                }
+        }
+        e[p] = fp_transform(rin, rout, s2.lam, dist[c * P + p]);
+    }
+    // ascending (insertion order of equal values is immaterial)
+#pragma unroll
+    for (int i = 1; i < P; i++)
+#pragma unroll
+        for (int j = i; j > 0; j--) {
+            const double lo = e[j - 1] < e[j] ? e[j - 1] : e[j], hi = e[j - 1] < e[j] ? e[j] : e[j - 1];
+            e[j - 1] = lo;
+            e[j] = hi;
+        }
+#pragma unroll
+    for (int p = 0; p < P; p++) edges[c * P + p] = e[p];
+}
+
+void launch_fp_edges(int nb, long n, const double *dist, const int *types, const chx_model_dev &M, double *edges,
+                     hipStream_t st)
+{
+    if (n <= 0) return;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (nb == 2) hipLaunchKernelGGL(k_fp_edges<2>, grid, block, 0, st, n, dist, types, M, edges);
+    else if (nb == 3) hipLaunchKernelGGL(k_fp_edges<3>, grid, block, 0, st, n, dist, types, M, edges);
+    else hipLaunchKernelGGL(k_fp_edges<4>, grid, block, 0, st, n, dist, types, M, edges);
+}
 
 template <int NP, bool PERWAVE>
 __global__ __launch_bounds__(256) void k_fp_hist(long n1, const double *__restrict__ e1, long n2,

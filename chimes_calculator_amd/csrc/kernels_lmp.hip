@@ -379,6 +379,99 @@ __global__ __launch_bounds__(256) void k_lmp_global(chx_lmp_args A, unsigned lon
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Cluster rows of the FINGERPRINT build (pair_chimes.cpp:526-535, 625-640,
+// 703-714, 777-790, recorded by chimesFF.cpp:1694-1703, 2013-2022,
+// 2603-2613): a cluster of the pair style's loops is recorded when its
+// members are in order (2-body: i < j by atom index, on pairs with tag[j] >
+// tag[i]; 3-body: tag[i] < tag[j], tag[i] < tag[k], tag[j] < tag[k]; 4-body:
+// tag[i] < tag[j] < tag[k] < tag[l]) and compute_nB evaluates it (type not
+// excluded, every pair below its slot's outer cutoff).  Row: the pair
+// distances in runtime order (get_dist's arithmetic), then the member types
+// as doubles.  One thread per loop entry (2-body: per packed neighbour-row
+// entry), 256 entries per block: MODE 0 counts each block's rows, MODE 1
+// writes them from the block's offset in entry order, so rows come in the
+// loops' order.
+// ---------------------------------------------------------------------------
+template <int NB, int MODE>
+__global__ __launch_bounds__(256) void k_lmp_fp_rows(long n, const int *__restrict__ list, chx_lmp_args A,
+                                                     chx_model_dev M, int *blk_cnt, const long long *blk_off,
+                                                     double *rows)
+{
+    constexpr int P = NB * (NB - 1) / 2;
+    constexpr int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    constexpr int pa3[3] = {0, 0, 1}, pb3[3] = {1, 2, 2};
+    __shared__ int wsum[4];
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    bool keep = false;
+    int ty[NB];
+    double dist[P];
+    if (e < n) {
+        int m[NB];
+        if constexpr (NB == 2) {
+            int lo = 0, hi = A.inum - 1;  // owner: the last row starting at or before e
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (A.nbr_off[mid] <= e) lo = mid; else hi = mid - 1;
+            }
+            m[0] = A.ilist[lo];
+            m[1] = A.nbr[e];
+            keep = A.tag[m[1]] > A.tag[m[0]] && m[0] < m[1];
+        } else {
+#pragma unroll
+            for (int a = 0; a < NB; a++) m[a] = list[e * NB + a];
+            const int *t = A.tag;
+            if constexpr (NB == 3) keep = t[m[0]] < t[m[1]] && t[m[0]] < t[m[2]] && t[m[1]] < t[m[2]];
+            else keep = t[m[0]] < t[m[1]] && t[m[1]] < t[m[2]] && t[m[2]] < t[m[3]];
+        }
+        if (keep) {
+#pragma unroll
+            for (int a = 0; a < NB; a++) ty[a] = A.ctype[m[a]];
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const int a = NB == 3 ? pa3[q] : pa[q], b = NB == 3 ? pb3[q] : pb[q];
+                double dr[3];
+                dist[q] = lmp_dist(A.x + 3L * m[a], A.x + 3L * m[b], dr);
+            }
+            keep = cluster_inside<NB>(M, ty, dist);
+        }
+    }
+    const unsigned long long bal = __ballot(keep);
+    const int before = __popcll(bal & (lane ? (~0ull >> (64 - lane)) : 0ull));
+    if (lane == 0) wsum[wv] = __popcll(bal);
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        base += w < wv ? wsum[w] : 0;
+        total += wsum[w];
+    }
+    if (MODE == 0) {
+        if (threadIdx.x == 0) blk_cnt[blockIdx.x] = total;
+        return;
+    }
+    if (!keep) return;
+    double *row = rows + (blk_off[blockIdx.x] + base + before) * (P + NB);
+#pragma unroll
+    for (int q = 0; q < P; q++) row[q] = dist[q];
+#pragma unroll
+    for (int a = 0; a < NB; a++) row[P + a] = (double)ty[a];
+}
+
+void launch_lmp_fp_rows(int nb, int mode, long n, const int *list, const chx_lmp_args &A, const chx_model_dev &M,
+                        int *blk_cnt, const long long *blk_off, double *rows, hipStream_t st)
+{
+    if (n <= 0) return;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+#define L(NB_, MODE_) \
+    hipLaunchKernelGGL((k_lmp_fp_rows<NB_, MODE_>), grid, block, 0, st, n, list, A, M, blk_cnt, blk_off, rows)
+    if (nb == 2) { if (mode) L(2, 1); else L(2, 0); }
+    else if (nb == 3) { if (mode) L(3, 1); else L(3, 0); }
+    else { if (mode) L(4, 1); else L(4, 0); }
+#undef L
+}
+
 void launch_lmp_pair(const chx_lmp_args &A, const chx_model_dev &M, hipStream_t st)
 {
     if (A.inum <= 0) return;

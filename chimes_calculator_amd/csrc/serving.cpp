@@ -88,7 +88,8 @@ void size_request(const chx::DeviceModel &m, Request &r)
 // Device bytes an engine asks for when it evaluates these requests as one
 // batch: the work buffers of engine.hip's layout() and size_work() at the
 // estimated capacities (neighbour slabs at the batch's largest per-owner
-// capacity), the blobs, and a 25% margin for capacities grown by a retry.
+// capacity), the blobs, and a 50% margin for capacities that a retry grows
+// past the density estimate (an owner with more images than the estimate).
 size_t batch_bytes(const std::vector<Request *> &b, int groups)
 {
     double N = 0, Nin = 0, nb_cap = 8, l3 = 0, l4 = 0, cells = 0;
@@ -111,7 +112,7 @@ size_t batch_bytes(const std::vector<Request *> &b, int groups)
                             + 8.0;           // scan partials
     const double bytes = N * per_atom + 8.0 * (l3 + l4) + 16.0 * cells + 40.0 * Nin  // blobs: positions, types, forces
                          + 1024.0 * ncfg + 65536.0;
-    return (size_t)(1.25 * bytes);
+    return (size_t)(1.5 * bytes);
 }
 
 }  // namespace

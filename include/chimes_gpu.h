@@ -174,9 +174,11 @@ void chimes_gpu_free(void *p);
  * chimes_gpu_fp_edges: rows of a cluster dump - the nb(nb-1)/2 pair
  * distances dist[n*P] (pair order 01 02 03 12 13 23) and the nb type indices
  * types[n*nb] of each cluster, as chimesFF's FINGERPRINT mode records them
- * (chimesFF.cpp:1694-1703, 2013-2020, 2603-2612) - become edges[n*P], the
- * sorted Morse-transformed distances with the cutoffs of each pair's slot
- * (read_flat_clusters, calc_histograms.cpp:140-249).
+ * (chimesFF.cpp:1694-1703, 2013-2020, 2603-2612; chimes_lmp_fp_rows makes
+ * them on the GPU) - become edges[n*P], the sorted Morse-transformed
+ * distances with the cutoffs of each pair's slot (read_flat_clusters,
+ * calc_histograms.cpp:140-249), computed on the GPU (the exponentials are
+ * the device library's: an edge can differ from a host libm's by an ulp).
  * chimes_gpu_fp_histogram: counts[nbin] of the distances between every
  * cluster of edges1[n1] and every cluster of edges2[n2] - or, with same != 0,
  * every pair j > i of edges1 - binned as get_bin does with bin width binw and

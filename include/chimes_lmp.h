@@ -50,6 +50,20 @@ int chimes_lmp_compute(chimes_lmp *p, int nall, const double *x, const int *type
  * loop order: trips (i, j, k), quads (i, j, k, l); free with chimes_gpu_free. */
 int chimes_lmp_mb_lists(chimes_lmp *p, long *n3, int **trips, long *n4, int **quads);
 
+/* The cluster rows of the pair style's FINGERPRINT build for the last
+ * compute (pair_chimes.cpp:526-535, 625-640, 703-714, 777-790, recorded by
+ * chimesFF.cpp:1694-1703, 2013-2022, 2603-2613), made on the GPU from the
+ * device-resident neighbour rows and many-body lists: nb = 2, 3 or 4; a
+ * cluster is recorded when its members are in order (2-body: i < j by atom
+ * index, among the pairs tag[j] > tag[i]; 3-body: tag[i] < tag[j], tag[i] <
+ * tag[k], tag[j] < tag[k]; 4-body: tag[i] < tag[j] < tag[k] < tag[l]) and
+ * compute_nB evaluates it (type not excluded, every pair below its slot's
+ * outer cutoff).  Each row holds the nb(nb-1)/2 pair distances (pair order
+ * 01 02 03 12 13 23) and then the nb ChIMES type indices, as doubles, the
+ * values writeClusterDataComp prints; rows come in the loops' order.  *rows
+ * is malloc'd (free with chimes_gpu_free); chimes_gpu_fp_edges takes them. */
+int chimes_lmp_fp_rows(chimes_lmp *p, int nb, long *nrows, double **rows);
+
 /* Milliseconds of the last compute (HIP events): [0] input copies, [1] list
  * build (0 when the lists were reused), [2] 1- and 2-body, [3] 3-/4-body and
  * the read-back of the results. */

@@ -210,4 +210,63 @@ void oc_lmp_compute(const oc_params *P, const double *x, const int *ctype, const
         }
 }
 
+/* The cluster rows of the pair style's FINGERPRINT build: PairCHIMES::compute
+ * passes a cluster to the row record only when its members are in order
+ * (2-body: i < j by atom index, pair_chimes.cpp:625-626; 3-body: tag[i] <
+ * tag[j], tag[i] < tag[k], tag[j] < tag[k], 703-704; 4-body: tag[i] < tag[j] <
+ * tag[k] < tag[l], 777-778), and compute_nB records it after its exclusion
+ * and cutoff checks (chimesFF.cpp:1691-1703, 1989-2022, 2575-2613): the pair
+ * distances in runtime pair order, then the member types.  Rows come in the
+ * loops' order.  Returns the number of rows; *rows (np + nb doubles each) is
+ * malloc'd (free with oc_lmp_free), NULL when there are none. */
+long oc_lmp_fp_rows(const oc_params *P, int nb, const double *x, const int *ctype, const int *tag, int inum,
+                    const int *ilist, const int *numneigh, const int *nbr_off, const int *nbr, long n3,
+                    const int *trips, long n4, const int *quads, double **rows)
+{
+    static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    static const int pa3[3] = {0, 0, 1}, pb3[3] = {1, 2, 2};
+    const int np = nb * (nb - 1) / 2, w = np + nb;
+    long n = 0, cap = 0;
+    double *out = NULL, dr[3], d[6], rin, rout;
+    int m[4], t[4];
+    const long nlist = nb == 2 ? 0 : (nb == 3 ? n3 : n4);
+    const int *list = nb == 3 ? trips : quads;
+    long total = 0;
+    if (nb == 2)
+        for (int ii = 0; ii < inum; ii++) total += numneigh[ilist[ii]];
+    else
+        total = nlist;
+    for (long e = 0, ii = 0, jj = 0; e < total; e++) {
+        if (nb == 2) {
+            while (jj >= numneigh[ilist[ii]]) { ii++; jj = 0; }
+            m[0] = ilist[ii];
+            m[1] = nbr[nbr_off[ii] + jj];
+            jj++;
+            if (tag[m[1]] <= tag[m[0]] || !(m[0] < m[1])) continue;
+        } else {
+            for (int a = 0; a < nb; a++) m[a] = list[e * nb + a];
+            if (nb == 3 && !(tag[m[0]] < tag[m[1]] && tag[m[0]] < tag[m[2]] && tag[m[1]] < tag[m[2]])) continue;
+            if (nb == 4 && !(tag[m[0]] < tag[m[1]] && tag[m[1]] < tag[m[2]] && tag[m[2]] < tag[m[3]])) continue;
+        }
+        for (int a = 0; a < nb; a++) t[a] = ctype[m[a]];
+        int inside = 1;
+        for (int q = 0; q < np && inside; q++) {
+            const int a = nb == 3 ? pa3[q] : pa[q], b = nb == 3 ? pb3[q] : pb[q];
+            d[q] = lmp_dist(x, m[a], m[b], dr);
+            if (oc_cluster_pair_cutoffs(P, nb, t, q, &rin, &rout) != 0) inside = 0;  /* excluded type */
+            else if (d[q] >= rout) inside = 0;
+        }
+        if (!inside) continue;
+        if (n == cap) {
+            cap = cap ? 2 * cap : 4096;
+            out = (double *)realloc(out, sizeof(double) * cap * w);
+        }
+        for (int q = 0; q < np; q++) out[n * w + q] = d[q];
+        for (int a = 0; a < nb; a++) out[n * w + np + a] = (double)t[a];
+        n++;
+    }
+    *rows = out;
+    return n;
+}
+
 void oc_lmp_free(void *p) { free(p); }

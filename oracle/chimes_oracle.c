@@ -1338,6 +1338,16 @@ static void key_of(const sysm *S, int g, int *out)
     out[3] = t % w - S->n_layers;
 }
 
+static int g_exact_sums = 0;
+
+/* Compensated accumulation for oc_calculate (not a reference behaviour): the
+   reference adds every term straight into running double totals
+   (chimesFF.cpp:1736-1741, 2103-2168), whose rounding grows with the number of
+   terms; with this mode on, the energy and stress terms of each cluster are
+   summed from zero and the per-cluster sums added in long double, so the
+   totals carry the rounding of the per-term products only. */
+void oc_set_exact_sums(int on) { g_exact_sums = on != 0; }
+
 /* serial_chimes_interface::calculate (serial_chimes_interface.cpp:960-1124) */
 int oc_calculate(const oc_params *P, int natom, const double *x, const double *y, const double *z,
                  const char *const *types, const double ca[3], const double cb[3],
@@ -1378,6 +1388,13 @@ int oc_calculate(const oc_params *P, int natom, const double *x, const double *y
     double sc[6] = {0, 0, 0, 0, 0, 0};
     double E = *energy;
     double f2[6], f3[9], f4[12], r[3];
+    /* compensated mode: each cluster's energy and stress terms are summed
+       from zero in double, then added into long double totals */
+    long double EL = 0.0L, SL[6] = {0, 0, 0, 0, 0, 0};
+    double Ec, scc[6];
+#define OC_BEGIN() double *sc_ = g_exact_sums ? scc : sc, *E_ = g_exact_sums ? &Ec : &E; \
+    if (g_exact_sums) { Ec = 0.0; memset(scc, 0, sizeof scc); }
+#define OC_END() if (g_exact_sums) { EL += Ec; for (int k_ = 0; k_ < 6; k_++) SL[k_] += scc[k_]; }
     for (int i = 0; i < S.n_atoms; i++) {
         E += P->eoff[S.type[i]];
         if (i % owner_stride) continue;
@@ -1386,7 +1403,9 @@ int oc_calculate(const oc_params *P, int natom, const double *x, const double *y
             double d = get_dist(&S, i, jj, r);
             int t[2] = {S.type[i], S.type[jj]};
             memset(f2, 0, sizeof f2);
-            oc_compute_2b(P, d, r, t, f2, sc, &E);
+            OC_BEGIN();
+            oc_compute_2b(P, d, r, t, f2, sc_, E_);
+            OC_END();
             int a = S.rep_parent[i], b = S.rep_parent[S.parent[jj]];
             fx[a] += f2[0]; fy[a] += f2[1]; fz[a] += f2[2];
             fx[b] += f2[3]; fy[b] += f2[4]; fz[b] += f2[5];
@@ -1402,7 +1421,9 @@ int oc_calculate(const oc_params *P, int natom, const double *x, const double *y
             dx[2] = get_dist(&S, m[1], m[2], dr + 6);
             int t[3] = {S.type[m[0]], S.type[m[1]], S.type[m[2]]};
             memset(f3, 0, sizeof f3);
-            oc_compute_3b(P, dx, dr, t, f3, sc, &E);
+            OC_BEGIN();
+            oc_compute_3b(P, dx, dr, t, f3, sc_, E_);
+            OC_END();
             for (int a = 0; a < 3; a++) {
                 int o = S.rep_parent[S.parent[m[a]]];
                 fx[o] += f3[3 * a]; fy[o] += f3[3 * a + 1]; fz[o] += f3[3 * a + 2];
@@ -1417,12 +1438,20 @@ int oc_calculate(const oc_params *P, int natom, const double *x, const double *y
             for (int p = 0; p < 6; p++) dx[p] = get_dist(&S, m[pa[p]], m[pb[p]], dr + 3 * p);
             int t[4] = {S.type[m[0]], S.type[m[1]], S.type[m[2]], S.type[m[3]]};
             memset(f4, 0, sizeof f4);
-            oc_compute_4b(P, dx, dr, t, f4, sc, &E);
+            OC_BEGIN();
+            oc_compute_4b(P, dx, dr, t, f4, sc_, E_);
+            OC_END();
             for (int a = 0; a < 4; a++) {
                 int o = S.rep_parent[S.parent[m[a]]];
                 fx[o] += f4[3 * a]; fy[o] += f4[3 * a + 1]; fz[o] += f4[3 * a + 2];
             }
         }
+#undef OC_BEGIN
+#undef OC_END
+    if (g_exact_sums) {
+        E = (double)((long double)E + EL);
+        for (int k = 0; k < 6; k++) sc[k] = (double)SL[k];
+    }
     E /= pow(S.n_rep + 1.0, 3.0);
     *energy = E;
     stress[0] = sc[0]; stress[1] = sc[1]; stress[2] = sc[2];

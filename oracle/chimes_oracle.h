@@ -98,6 +98,9 @@ void oc_lmp_compute(const oc_params *P, const double *x, const int *ctype, const
                     const int *ilist, const int *numneigh, const int *nbr_off, const int *nbr, long n3,
                     const int *trips, long n4, const int *quads, int eflag_global, int eflag_atom, int vflag_global,
                     int vflag_atom, double *f, double *eng, double *virial, double *eatom, double *vatom);
+long oc_lmp_fp_rows(const oc_params *P, int nb, const double *x, const int *ctype, const int *tag, int inum,
+                    const int *ilist, const int *numneigh, const int *nbr_off, const int *nbr, long n3,
+                    const int *trips, long n4, const int *quads, double **rows);
 void oc_lmp_free(void *p);
 
 /* Chebyshev polynomials of the Morse-transformed distance and the cutoff
@@ -106,6 +109,10 @@ void oc_lmp_free(void *p);
 void oc_cheby(double dx, double lambda, double rin, double rout, int order,
               double *Tn, double *Tnd);
 void oc_fcut(const oc_params *P, double dx, double rout, double *fcut, double *fcutd);
+
+/* Compensated energy and stress sums in oc_calculate (per-cluster sums added
+   in long double); off by default, when the sums follow the reference's order. */
+void oc_set_exact_sums(int on);
 
 /* Penalty / out-of-range event counters of the last oc_calculate call. */
 long oc_last_penalty_count(void);

@@ -122,3 +122,39 @@ def assert_close(name, got, ref, rtol, scale=None, atol=0.0):
     assert err <= rtol * s + atol, "%s: max error %.3e (scale %.3e) exceeds %.1e relative + %.1e" % (
         name, err, s, rtol, atol)
     return err / s
+
+
+DRIVERS = os.path.join(TESTS, "drivers")
+LIBDIR = os.path.join(ROOT, "chimes_calculator_amd", "lib")
+
+
+def build_driver(name, outdir):
+    """Compiles tests/drivers/<name>.{c,cpp} against include/ and links it to
+    the built libchimescalc.so (rpath set); returns the executable's path."""
+    import subprocess
+    src = os.path.join(DRIVERS, name + (".cpp" if os.path.exists(os.path.join(DRIVERS, name + ".cpp")) else ".c"))
+    exe = os.path.join(str(outdir), name)
+    cc = ["g++", "-std=c++11"] if src.endswith(".cpp") else ["gcc", "-std=c99"]
+    cmd = cc + ["-O1", "-I", os.path.join(ROOT, "include"), "-o", exe, src, "-L", LIBDIR, "-lchimescalc",
+                "-Wl,-rpath," + LIBDIR]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, "compile %s failed:\n%s" % (name, r.stderr)
+    return exe
+
+
+def parse_driver_output(text):
+    """(energy, ntypes or None, stress[9], forces[n, 3]) from a driver's output."""
+    e, nt, st, F = None, None, None, []
+    for line in text.splitlines():
+        f = line.split()
+        if not f:
+            continue
+        if f[0] == "energy":
+            e = float(f[1])
+        elif f[0] == "ntypes":
+            nt = int(f[1])
+        elif f[0] == "stress":
+            st = np.array([float(v) for v in f[1:10]])
+        elif f[0] == "force":
+            F.append([float(v) for v in f[1:4]])
+    return e, nt, st, np.array(F)

@@ -17,12 +17,17 @@ Runs only where the reference checkout and oracle/_ref/libchimes_ref.so exist
                         configurations of the models that cover every dense
                         dimension of the evaluation kernels (workloads.DSCAN)
   c3_case.json.gz       the 32768-atom carbon box: E, F, stress and list
-                        hashes
+                        hashes; and ("C3exact") the energy and stress of the
+                        oracle with compensated sums (oc_set_exact_sums): the
+                        reference adds ~1e10 per-term stress values into
+                        running double totals, whose rounding (~2e-10 of this
+                        box's small, cancelling stress) the compensated sum
+                        does not carry
   c4_cases.json.gz      a sample of the serving test's request stream
                         (workloads.serving_stream(48, nmax=4096, seed
                         0xC0DE14), requests C4_SAMPLE): E, F, stress
 
-Usage: make_golden.py [ref] [synth [C1]] [dense] [vectors] [dscan] [C3] [C4]
+Usage: make_golden.py [ref] [synth [C1]] [dense] [vectors] [dscan] [C3] [C3exact] [C4]
 """
 import gzip
 import hashlib
@@ -39,7 +44,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
-from oracle_bindings import Reference, read_xyz, ref_available, sort_keys  # noqa: E402
+from oracle_bindings import Oracle, Reference, read_xyz, ref_available, sort_keys  # noqa: E402
 from chimes_calculator_amd import workloads as W  # noqa: E402
 
 REFT = "/root/reference/serial_interface/tests"
@@ -206,6 +211,18 @@ def run(which, tmp):
         with gzip.open(os.path.join(HERE, "c3_case.json.gz"), "wt", compresslevel=9) as f:
             json.dump({"C3|0": record(res)}, f)
         print("C3", res.energy, len(res.pairs), len(res.trips), len(res.quads), flush=True)
+    if "C3exact" in which:
+        m, cfgs = W.workload("C3")
+        pf = W.MODELS[m].write(os.path.join(tmp, m + ".txt"))
+        path = os.path.join(HERE, "c3_case.json.gz")
+        with gzip.open(path, "rt") as f:
+            out = json.load(f)
+        res = Oracle(pf).calculate(*cfgs[0], exact_sums=True)
+        out["C3|0"]["energy_exact"] = repr(float(res.energy))
+        out["C3|0"]["stress_exact"] = [repr(float(v)) for v in res.stress]
+        with gzip.open(path, "wt", compresslevel=9) as f:
+            json.dump(out, f)
+        print("C3 compensated", res.energy, res.stress[0], flush=True)
     if "C4" in which:
         reqs = W.serving_stream(nreq=48, nmax=4096, seed=0xC0DE14)
         out = {}

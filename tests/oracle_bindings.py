@@ -48,6 +48,7 @@ def oracle_lib():
         L = C.CDLL(ORACLE_SO)
         L.oc_load.restype = C.c_void_p
         L.oc_load.argtypes = [C.c_char_p]
+        L.oc_set_exact_sums.argtypes = [C.c_int]
         L.oc_free.argtypes = [C.c_void_p]
         L.oc_order.argtypes = [C.c_void_p, C.c_int]
         L.oc_max_cutoff.restype = C.c_double
@@ -79,6 +80,9 @@ def oracle_lib():
                                      C.c_long, _ip, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
                                      C.POINTER(C.c_double), _dp, _dp, _dp]
         L.oc_lmp_free.argtypes = [C.c_void_p]
+        L.oc_lmp_fp_rows.restype = C.c_long
+        L.oc_lmp_fp_rows.argtypes = [C.c_void_p, C.c_int, _dp, _ip, _ip, C.c_int, _ip, _ip, _ip, _ip, C.c_long, _ip,
+                                     C.c_long, _ip, C.POINTER(C.POINTER(C.c_double))]
         L.oc_cluster_pair_cutoffs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, C.POINTER(C.c_double),
                                               C.POINTER(C.c_double)]
         L.oc_fp_edges.argtypes = [C.c_void_p, C.c_int, C.c_long, _dp, _ip, _dp]
@@ -121,7 +125,9 @@ class Oracle:
     def type_names(self):
         return [self.L.oc_type_name(self.h, t).decode() for t in range(self.L.oc_ntypes(self.h))]
 
-    def calculate(self, pos, types, cell, small=False, lists=False, owner_stride=1):
+    def calculate(self, pos, types, cell, small=False, lists=False, owner_stride=1, exact_sums=False):
+        """exact_sums: per-cluster energy and stress sums added in long double
+        (oc_set_exact_sums) instead of the reference's running double totals."""
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         n = pos.shape[0]
         x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
@@ -132,9 +138,13 @@ class Oracle:
         e = C.c_double(0.0)
         L = _Lists()
         ts = _cstrs(types)
-        rc = self.L.oc_calculate(self.h, n, x, y, z, C.cast(ts, C.c_void_p), ca, cb, cc,
-                                 int(bool(small)), C.byref(e), fx, fy, fz, st,
-                                 C.byref(L) if lists else None, int(owner_stride))
+        self.L.oc_set_exact_sums(int(bool(exact_sums)))
+        try:
+            rc = self.L.oc_calculate(self.h, n, x, y, z, C.cast(ts, C.c_void_p), ca, cb, cc,
+                                     int(bool(small)), C.byref(e), fx, fy, fz, st,
+                                     C.byref(L) if lists else None, int(owner_stride))
+        finally:
+            self.L.oc_set_exact_sums(0)
         if rc != 0:
             raise RuntimeError("oracle calculate failed with code %d" % rc)
         res = Result(e.value, np.stack([fx, fy, fz], 1), st)
@@ -203,6 +213,22 @@ class LmpOracle(Oracle):
         self.L.oc_lmp_free(C.cast(p3, C.c_void_p))
         self.L.oc_lmp_free(C.cast(p4, C.c_void_p))
         return trips, quads
+
+    def fp_rows(self, S, trips, quads, nb):
+        """(distances[n, P], types[n, nb]) of the FINGERPRINT rows
+        (oc_lmp_fp_rows) over the system's pairs and the given lists."""
+        trips = np.ascontiguousarray(trips, np.int32).reshape(-1)
+        quads = np.ascontiguousarray(quads, np.int32).reshape(-1)
+        p = C.POINTER(C.c_double)()
+        n = self.L.oc_lmp_fp_rows(self.h, nb, S.x.reshape(-1), S.ctype, S.tag, S.inum, S.ilist, S.numneigh,
+                                  S.nbr_off, S.nbr, len(trips) // 3, trips if len(trips) else np.zeros(3, np.int32),
+                                  len(quads) // 4, quads if len(quads) else np.zeros(4, np.int32), C.byref(p))
+        P = nb * (nb - 1) // 2
+        if n == 0:
+            return np.zeros((0, P)), np.zeros((0, nb), np.int32)
+        a = np.ctypeslib.as_array(p, shape=(n * (P + nb),)).reshape(n, P + nb).copy()
+        self.L.oc_lmp_free(C.cast(p, C.c_void_p))
+        return a[:, :P], a[:, P:].astype(np.int32)
 
     def compute(self, S, trips, quads, eflag_global=1, eflag_atom=1, vflag_global=1, vflag_atom=1):
         trips = np.ascontiguousarray(trips, np.int32).reshape(-1)

@@ -87,3 +87,69 @@ def test_fp_hist_brute_force(P, same):
     assert ns == n == (300 * 299 // 2 if same else 300 * 250)
     assert counts.sum() == ns
     assert np.array_equal(counts, want)
+
+
+def _co_system(O, skin=0.5):
+    from lmp_setup import LmpSystem
+    from oracle_bindings import read_xyz
+    d = tempfile.mkdtemp()
+    xyz = os.path.join(d, "co.xyz")
+    with gzip.open(os.path.join(GOLDEN, "ref_inputs", "CO.9GPa_2400K.OUTCAR_#000.xyz.gz")) as a, open(xyz, "wb") as b:
+        b.write(a.read())
+    pos, names, cell = read_xyz(xyz)
+    Lv = np.diag(np.asarray(cell, float).reshape(3, 3))
+    pos = np.mod(pos, Lv)
+    tn = O.type_names()
+    ct = np.array([tn.index(n) for n in names], np.int32)
+    return LmpSystem(pos, ct + 1, np.arange(1, len(pos) + 1, dtype=np.int32), np.zeros(3), Lv,
+                     np.arange(len(tn), dtype=np.int32), O.rc2(), skin)
+
+
+def _dist(x, a, b):
+    d = x[b] - x[a]
+    return np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+
+
+def test_fp_rows_oracle():
+    """oc_lmp_fp_rows against the row rule restated here: the pair style's
+    loops (2-body pairs tag[j] > tag[i]; the 3-/4-mer lists), kept when the
+    members are in order (2-body i < j; strictly increasing tags) and every
+    pair is inside its slot's outer cutoff (type not excluded); distances
+    as get_dist forms them; rows in loop order."""
+    from oracle_bindings import LmpOracle
+    f = tempfile.NamedTemporaryFile(suffix=".txt", delete=False)
+    f.write(gzip.open(os.path.join(GOLDEN, "ref_inputs", FF)).read())
+    f.close()
+    O = LmpOracle(f.name)
+    S = _co_system(O)
+    trips, quads = O.build_lists(S)
+    pairs = np.array([(i, j) for ii, i in enumerate(S.ilist) for j in S.nbr[S.nbr_off[ii]:S.nbr_off[ii + 1]]
+                      if S.tag[j] > S.tag[i]], np.int32)
+    tag = S.tag
+    for nb, lst in ((2, pairs), (3, trips), (4, quads)):
+        if nb == 2:
+            order = lst[:, 0] < lst[:, 1]
+        elif nb == 3:
+            order = (tag[lst[:, 0]] < tag[lst[:, 1]]) & (tag[lst[:, 0]] < tag[lst[:, 2]]) & \
+                    (tag[lst[:, 1]] < tag[lst[:, 2]])
+        else:
+            order = (tag[lst[:, 0]] < tag[lst[:, 1]]) & (tag[lst[:, 1]] < tag[lst[:, 2]]) & \
+                    (tag[lst[:, 2]] < tag[lst[:, 3]])
+        lst = lst[order]
+        idx = [(a, b) for a in range(nb) for b in range(a + 1, nb)]
+        dist = np.stack([_dist(S.x, lst[:, a], lst[:, b]) for a, b in idx], 1)
+        types = S.ctype[lst]
+        keep = np.ones(len(lst), bool)
+        for k, t in enumerate(types):
+            for p in range(len(idx)):
+                import ctypes as C
+                rin, rout = C.c_double(), C.c_double()
+                if O.L.oc_cluster_pair_cutoffs(O.h, nb, np.asarray(t, np.int32), p, C.byref(rin), C.byref(rout)):
+                    keep[k] = False
+                    break
+                if dist[k, p] >= rout.value:
+                    keep[k] = False
+                    break
+        got_d, got_t = O.fp_rows(S, trips, quads, nb)
+        assert len(got_d) > 50, nb
+        assert np.array_equal(got_d, dist[keep]) and np.array_equal(got_t, types[keep]), nb

@@ -175,10 +175,23 @@ def test_clustered_capacity_retry(cc, model_dir):
 
 def test_c3_carbon(cc, model_dir, c3_case):
     """The 32768-atom carbon box at 2B/3B/4B orders 16/10/4 (3-body D = 10,
-    4-body D = 4, 2-body order 16)."""
+    4-body D = 4, 2-body order 16): energy and forces at 1e-10 of the
+    reference's, lists identical.  This box's stress is small against its
+    terms (cancellation), and the reference adds ~1e10 per-term values into
+    running double totals, whose rounding reaches ~2e-10 of it; the stress is
+    held to 1e-10 of the oracle's compensated sum of the same terms
+    (per-cluster sums added in long double), and the reference's own value
+    to the same sum within 1e-9."""
     from chimes_calculator_amd import workloads as W
     mname, cfgs = W.workload("C3")
     m = cc.GpuModel(os.path.join(model_dir, mname + ".txt"))
     E, F, S = cc.eval_batch(m, cfgs)
-    check_efs("C3", E[0], F[0], S[0], c3_case)
+    e0, f0, s0 = golden_arrays(c3_case)
+    sx = np.array([float(v) for v in c3_case["stress_exact"]])
+    print("C3 stress: GPU vs compensated %.3g, reference vs compensated %.3g (relative)"
+          % (np.abs(S[0] - sx).max() / np.abs(sx).max(), np.abs(s0 - sx).max() / np.abs(sx).max()))
+    assert abs(E[0] - e0) <= RTOL * abs(e0), "C3: energy %.17g vs %.17g" % (E[0], e0)
+    assert_close("C3 forces", F[0], f0, RTOL, atol=FATOL)
+    assert_close("C3 stress", S[0], sx, RTOL)
+    assert_close("C3 reference stress", s0, sx, 1e-9)
     check_lists(m, c3_case)

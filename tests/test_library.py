@@ -48,3 +48,13 @@ def test_no_cpu_fallback():
     env = dict(os.environ, HIP_VISIBLE_DEVICES="", ROCR_VISIBLE_DEVICES="")
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=120)
     assert "ERR" in out.stdout
+
+
+@pytest.mark.parametrize("name", ["serial_cpp_main", "serial_c_main"])
+def test_caller_programs_build(tmp_path, name):
+    """A C++ program of the serial_chimes_interface class and a C program of
+    the system C ABI compile against include/ and link libchimescalc.so
+    alone (they run on the GPU in test_gpu_boundary.py)."""
+    from conftest import build_driver
+    exe = build_driver(name, tmp_path)
+    assert os.access(exe, os.X_OK)
