@@ -45,6 +45,7 @@ EXPORTS = {
                      "chimes_gpu_server_result", "chimes_gpu_server_stats", "chimes_gpu_server_memory",
                      "chimes_gpu_server_failures", "chimes_gpu_server_last_error", "chimes_gpu_server_free",
                      "chimes_gpu_last_timing", "chimes_gpu_last_counts", "chimes_gpu_compute_clusters",
+                     "chimes_gpu_compute_clusters_static",
                      "chimes_gpu_cluster_lists", "chimes_gpu_free", "chimes_gpu_fp_edges",
                      "chimes_gpu_fp_histogram"],
     "chimes_lmp.h": ["chimes_lmp_create", "chimes_lmp_free", "chimes_lmp_build_mb_neighlists",
@@ -119,6 +120,8 @@ def lib():
         L.chimes_gpu_last_timing.argtypes = [C.c_void_p, _dp]
         L.chimes_gpu_last_counts.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.chimes_gpu_compute_clusters.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp]
+        L.chimes_gpu_compute_clusters_static.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp,
+                                                         _dp]
         pp = C.POINTER(C.POINTER(C.c_int))
         L.chimes_gpu_cluster_lists.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), pp,
                                                C.POINTER(C.c_long), pp, C.POINTER(C.c_long), pp]

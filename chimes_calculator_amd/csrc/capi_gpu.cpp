@@ -352,6 +352,22 @@ int chimes_gpu_compute_clusters(chimes_gpu_model *m, int nb, int n, const double
     return 0;
 }
 
+int chimes_gpu_compute_clusters_static(chimes_gpu_model *m, int nb, int n, const double *dist, const double *dr,
+                                       const int *types, double *energy, double *force, double *stress)
+{
+    if (!m) return fail("no model");
+    if (nb < 2 || nb > 4) return fail("cluster size must be 2, 3 or 4");
+    if (n < 0 || n > CHIMES_GPU_STATIC_MAX_CLUSTERS) return fail("cluster count out of range");
+    const int nt = m->dm->P.natmtyps;
+    for (long k = 0; k < (long)n * nb; k++)
+        if (types[k] < 0 || types[k] >= nt) return fail("atom type index out of range");
+    chx::EngineHandle *eh = m->dm->eh;
+    std::lock_guard<std::mutex> lk(eh->mu);
+    if (chx_eval_clusters_static(eh->eng, m->dm->m, nb, n, dist, dr, types, energy, force, stress))
+        return fail(chx_last_error(eh->eng));
+    return 0;
+}
+
 int chimes_gpu_cluster_lists(chimes_gpu_model *m, int conf, long *n2, int **pairs, long *n3, int **trips,
                              long *n4, int **quads)
 {

@@ -97,6 +97,13 @@ int chx_eval_clusters(chx_engine *e, chx_model *m, int nb, int n, const double *
                       const int *types, double *energy, double *force, double *stress, double *last,
                       int *flags);
 
+// The same without allocating: inputs and outputs are staged in the
+// library's static device scratch array (at most CHX_STATIC_MAX_CLUSTERS
+// clusters; its address on each device is resolved when the device's first
+// engine is created).  Calls that overlap on one device race on the array.
+int chx_eval_clusters_static(chx_engine *e, chx_model *m, int nb, int n, const double *dist, const double *dr,
+                             const int *types, double *energy, double *force, double *stress);
+
 // Batch plans: a batch shape (models, model_of, natoms, small, precision and,
 // when pos_dev / out_dev are set, the device buffers) fixed at creation.
 // Creation evaluates b once (results in o), optionally times the 3-/4-body

@@ -14,6 +14,7 @@
 #define CHX_LIST_ALIGN 64      // 3-/4-body list segments of one cluster type start on this multiple
 #define CHX_MFMA_MAX_D 4       // 4-body dense dimension handled by the matrix-core kernel
 #define CHX_MFMA3_MAX_D 13     // 3-body dense dimension handled by the matrix-core kernel
+#define CHX_STATIC_MAX_CLUSTERS 131072  // clusters per call of the allocation-free explicit-cluster entry
 
 // Status words of one evaluation (device int array, read back at the end).
 enum {

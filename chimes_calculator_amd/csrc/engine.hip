@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -248,6 +249,40 @@ std::vector<double> trip_matrix_image(const double *C, int D)
 
 }  // namespace
 
+// Scratch of the allocation-free explicit-cluster entry point: inputs and
+// outputs of at most CHX_STATIC_MAX_CLUSTERS clusters.  Each device holds its
+// own copy for the life of the process; host code uses the address that
+// hipGetSymbolAddress returns for the current device (scratch_init, once per
+// device at engine creation), never the array's name.
+constexpr size_t CHX_SCRATCH_BYTES = (size_t)CHX_STATIC_MAX_CLUSTERS * 416 + 8 * 256;
+static __device__ __align__(256) char scratch[CHX_SCRATCH_BYTES];
+
+namespace {
+constexpr int CHX_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[CHX_MAX_DEVICES] = {};
+hipError_t g_scratch_rc[CHX_MAX_DEVICES] = {};
+
+// Resolves the scratch array's address on `device` (which must be current).
+void scratch_init(int device)
+{
+    if (device < 0 || device >= CHX_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return;
+    void *p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_rc[device] = rc;
+    if (rc == hipSuccess) g_scratch[device] = p;
+}
+
+void *scratch_ptr(int device)
+{
+    if (device < 0 || device >= CHX_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[device];
+}
+}  // namespace
+
 struct chx_model {
     chx_engine *eng = nullptr;
     chx_model_desc d{};
@@ -405,6 +440,7 @@ chx_engine *chx_engine_create(int device)
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return nullptr;
     if (device < 0 || device >= n) return nullptr;
     if (hipSetDevice(device) != hipSuccess) return nullptr;
+    scratch_init(device);
     chx_engine *e = new chx_engine;
     e->device = device;
     if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) { delete e; return nullptr; }
@@ -1563,6 +1599,59 @@ int chx_export_lists(chx_engine *e, int cfg, long *n2, int **pairs, long *n3, in
             if (body == 3) { *n3 = (long)out.size() / w; *trips = buf; }
             else { *n4 = (long)out.size() / w; *quads = buf; }
         }
+        return 0;
+    } catch (const EngineError &x) {
+        e->err = x.msg;
+        return 1;
+    }
+}
+
+int chx_eval_clusters_static(chx_engine *e, chx_model *m, int nb, int n, const double *dist, const double *dr,
+                             const int *types, double *energy, double *force, double *stress)
+{
+    try {
+        HIPCHK(hipSetDevice(e->device));
+        if (nb < 2 || nb > 4) throw EngineError("cluster size must be 2, 3 or 4");
+        if (n < 0 || n > CHX_STATIC_MAX_CLUSTERS)
+            throw EngineError("at most " + std::to_string(CHX_STATIC_MAX_CLUSTERS) +
+                              " clusters per call of the allocation-free entry point");
+        if (n == 0) return 0;
+        if ((nb == 3 && (m->d.order[1] == 0 || !m->dev.C3)) || (nb == 4 && (m->d.order[2] == 0 || !m->dev.C4)))
+            throw EngineError("model has no terms of this body order");
+        char *base = (char *)scratch_ptr(e->device);
+        if (!base) throw EngineError("the device scratch array has no address on this device");
+        const int P = nb * (nb - 1) / 2;
+        hipStream_t st = e->st;
+        // scratch layout (each part 256-byte aligned): distances, pair vectors,
+        // types, energy, forces, stress, last-term scalars, flags
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            char *p = base + off;
+            off += (bytes + 255) & ~(size_t)255;
+            return p;
+        };
+        double *d_dist = (double *)take(sizeof(double) * n * P);
+        double *d_dr = (double *)take(sizeof(double) * n * P * 3);
+        int *d_types = (int *)take(sizeof(int) * n * nb);
+        double *oE = (double *)take(sizeof(double) * n);
+        double *oF = (double *)take(sizeof(double) * n * nb * 3);
+        double *oS = (double *)take(sizeof(double) * n * 6);
+        double *oL = (double *)take(sizeof(double) * n * P);
+        int *oflags = (int *)take(sizeof(int) * n);
+        if (off > CHX_SCRATCH_BYTES) throw EngineError("scratch layout exceeds the scratch array");
+        HIPCHK(hipMemcpyAsync(d_dist, dist, sizeof(double) * n * P, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_dr, dr, sizeof(double) * n * P * 3, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_types, types, sizeof(int) * n * nb, hipMemcpyHostToDevice, st));
+        // the last-term scalars keep their value for a cluster that is not
+        // evaluated: they start at zero, as chimes_gpu_compute_clusters's do
+        hipLaunchKernelGGL(k_fill_f64, dim3(nblk((long)n * P, 256)), dim3(256), 0, st, oL, (long)n * P);
+        launch_clusters(nb, nb == 3 ? m->d.D3 : (nb == 4 ? m->d.D4 : 2), n, st, d_dist, d_dr, d_types, m->dev, oE,
+                        oF, oS, oL, oflags);
+        HIPCHK(hipMemcpyAsync(energy, oE, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(force, oF, sizeof(double) * n * nb * 3, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(stress, oS, sizeof(double) * n * 6, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipGetLastError());
         return 0;
     } catch (const EngineError &x) {
         e->err = x.msg;

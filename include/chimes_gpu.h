@@ -163,6 +163,19 @@ void chimes_gpu_last_counts(const chimes_gpu_model *m, long long counts[3]);
 int chimes_gpu_compute_clusters(chimes_gpu_model *m, int nb, int n, const double *dist, const double *dr,
                                 const int *types, double *energy, double *force, double *stress);
 
+/* Allocation-free chimes_gpu_compute_clusters.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as chimes_gpu_compute_clusters; the clusters'
+ * inputs and outputs are staged in a static device array of the library
+ * (about 52 MiB, held on each GPU the library runs on until the process
+ * ends), so n is at most CHIMES_GPU_STATIC_MAX_CLUSTERS (a larger n is an
+ * error) and no call allocates.  All calls on one GPU share that array: calls
+ * that overlap on one GPU race. */
+#define CHIMES_GPU_STATIC_MAX_CLUSTERS 131072
+int chimes_gpu_compute_clusters_static(chimes_gpu_model *m, int nb, int n, const double *dist, const double *dr,
+                                       const int *types, double *energy, double *force, double *stress);
+
 /* Cluster lists of configuration `conf` of the last evaluation, as sorted
  * canonical keys: pairs (i, a, sx, sy, sz), triplets (i, a_j, s_j, a_k, s_k),
  * quadruplets (i, a_j, s_j, a_k, s_k, a_l, s_l); free with chimes_gpu_free. */
