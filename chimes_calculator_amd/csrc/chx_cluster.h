@@ -131,7 +131,7 @@ __device__ __forceinline__ void explicit_cluster(const chx_model_dev &M, const i
                 double T[P][D], dT[P][D], f[P], fd[P];
                 for (int q = 0; q < P; q++) {
                     cheby<D, double>(rq[q], sl[q], T[q], dT[q], flags);
-                    fcut(rq[q], sl[q].rout, M.tersoff, M.fcut_var, f[q], fd[q]);
+                    fcut(rq[q], sl[q], M.tersoff, M.fcut_var, f[q], fd[q]);
                 }
                 double S, G[P], fs[P];
                 // when every active lane has the same cluster type (always for

@@ -100,8 +100,13 @@ struct chx_entry_n {
 };
 
 // Per-slot constants of one cluster type (or pair type).
+// Cutoffs and Morse constants of one pair slot, with the reciprocals the
+// kernels multiply by (1 / lambda, 1 / xdiff, 1 / rout and, for the Tersoff
+// cutoff, 1 / (rout - thr), the width of the ramp that starts at
+// thr = rout - fcut_var * rout).
 struct chx_slot {
     double rin, rout, lam, xavg, xdiff;
+    double ilam, ixdiff, irout, iramp;
 };
 
 // One (model, cluster type) group of the 3- or 4-body list.  type < 0 marks

@@ -235,6 +235,21 @@ std::vector<double> quad_matrix_image(const double *C, int D)
     return m;
 }
 
+// A pair slot from (rin, rout, lambda, xavg, xdiff) and the model's Tersoff
+// ramp fraction, with the reciprocals the kernels multiply by.
+chx_slot make_slot(const double *q, double fcut_var)
+{
+    chx_slot s{q[0], q[1], q[2], q[3], q[4], 0, 0, 0, 0};
+    s.ilam = 1.0 / s.lam;
+    s.ixdiff = 1.0 / s.xdiff;
+    s.irout = 1.0 / s.rout;
+    // Tersoff ramp: from thr = rout - fcut_var * rout up to rout
+    const double thr = s.rout - fcut_var * s.rout;
+    const double width = s.rout - thr;
+    s.iramp = width != 0.0 ? 1.0 / width : 0.0;
+    return s;
+}
+
 // Matrix image of one 3-body coefficient tensor: Cm3[a][16 b + c], a padded to
 // a multiple of 4, c to 16, zero padding.
 std::vector<double> trip_matrix_image(const double *C, int D)
@@ -356,6 +371,7 @@ struct chx_engine {
     // buffer capacities (entries, list slots); grow on demand
     long l3_cap = 0, l4_cap = 0;
     int nb_cap = 0;  // neighbour images per owner slab
+    long cap_key[4] = {-1, -1, -1, -1};  // batch shape the capacities were measured on
     bool big_on = false;  // launch the crowded-owner enumeration (set once an owner needed it)
     // one host-to-device and one device-to-host copy per run
     PinnedBuf in_h, out_h;
@@ -490,12 +506,14 @@ void chx_engine_trim(chx_engine *e)
     e->each_buf([](auto &b) { b.release(); });
     e->nb_cap = 0;
     e->l3_cap = e->l4_cap = 0;
+    for (int k = 0; k < 4; k++) e->cap_key[k] = -1;
 }
 
 void chx_engine_reset_capacities(chx_engine *e)
 {
     e->nb_cap = 0;
     e->l3_cap = e->l4_cap = 0;
+    for (int k = 0; k < 4; k++) e->cap_key[k] = -1;
 }
 
 int chx_set_variants(chx_engine *e, int var3, int var4)
@@ -541,7 +559,7 @@ chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
         std::vector<chx_slot> s2(d->npairs);
         for (int p = 0; p < d->npairs; p++) {
             const double *q = d->pair_slots + 5 * p;
-            s2[p] = chx_slot{q[0], q[1], q[2], q[3], q[4]};
+            s2[p] = make_slot(q, d->fcut_var);
         }
         v.slots2 = m->up(s2.data(), s2.size());
         v.c2 = m->up(d->c2, (size_t)d->npairs * (d->order[0] + 1));
@@ -556,7 +574,7 @@ chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
             std::vector<chx_slot> s3(d->ntrip * 3);
             for (int k = 0; k < d->ntrip * 3; k++) {
                 const double *q = d->trip_slots + 5 * k;
-                s3[k] = chx_slot{q[0], q[1], q[2], q[3], q[4]};
+                s3[k] = make_slot(q, d->fcut_var);
             }
             v.slots3 = m->up(s3.data(), s3.size());
             v.C3 = m->up(d->C3, d->ntrip * cs);
@@ -590,7 +608,7 @@ chx_model *chx_model_create(chx_engine *e, const chx_model_desc *d)
             std::vector<chx_slot> s4(d->nquad * 6);
             for (int k = 0; k < d->nquad * 6; k++) {
                 const double *q = d->quad_slots + 5 * k;
-                s4[k] = chx_slot{q[0], q[1], q[2], q[3], q[4]};
+                s4[k] = make_slot(q, d->fcut_var);
             }
             v.slots4 = m->up(s4.data(), s4.size());
             v.C4 = m->up(d->C4, d->nquad * cs);
@@ -772,11 +790,37 @@ void guess_capacities(chx_engine *e, chx_model *const *models)
             l4 += C.natoms * (0.08 * n4 * n4 * n4 + 4);
         }
     }
-    e->nb_cap = std::max(e->nb_cap, std::min(CHX_MAX_NB, ((int)nbmax + 7) / 8 * 8));
-    e->l3_cap = std::max<long>(e->l3_cap, (long)l3 + (long)CHX_LIST_ALIGN * (e->G3 + 1));
-    e->l4_cap = std::max<long>(e->l4_cap, (long)l4 + (long)CHX_LIST_ALIGN * (e->G4 + 1));
+    // A batch of the shape of the last one (same atoms, configurations and
+    // groups: a plan, or repeated evaluations of one system) keeps the
+    // capacities its runs measured; another shape starts from the estimate,
+    // so a large batch's capacities are not carried into small ones (their
+    // launch grids follow the capacities).
+    const long key[4] = {e->N, e->ncfg, e->G3, e->G4};
+    bool same = true;
+    for (int k = 0; k < 4; k++) same = same && key[k] == e->cap_key[k];
+    if (!same) {
+        for (int k = 0; k < 4; k++) e->cap_key[k] = key[k];
+        e->nb_cap = std::min(CHX_MAX_NB, ((int)nbmax + 7) / 8 * 8);
+        e->l3_cap = (long)l3 + (long)CHX_LIST_ALIGN * (e->G3 + 1);
+        e->l4_cap = (long)l4 + (long)CHX_LIST_ALIGN * (e->G4 + 1);
+    }
     e->l3_cap = (e->l3_cap + 255) / 256 * 256;
     e->l4_cap = (e->l4_cap + 255) / 256 * 256;
+}
+
+// After a complete run: capacities brought down to what the run used plus a
+// margin (never below it), so the list kernels' launch grids follow the
+// measured cluster counts rather than the density estimate.  A later run
+// that needs more reports it and is repeated with the size it needs.
+void tighten_capacities(chx_engine *e, double margin)
+{
+    const int *st = (const int *)(e->out_h.p + e->o_stat);
+    const int nb = std::min(CHX_MAX_NB, ((int)(st[CHX_ST_MAXNB] * margin) + 8 + 7) / 8 * 8);
+    const long n3 = ((long)(st[CHX_ST_N3] * margin) + 1024 + 255) / 256 * 256;
+    const long n4 = ((long)(st[CHX_ST_N4] * margin) + 1024 + 255) / 256 * 256;
+    e->nb_cap = std::min(e->nb_cap, std::max(nb, 8));
+    if (e->G3) e->l3_cap = std::min(e->l3_cap, n3);
+    if (e->G4) e->l4_cap = std::min(e->l4_cap, n4);
 }
 
 // Prepares a batch on the host: frames, groups, model tables and capacity
@@ -1178,6 +1222,7 @@ int chx_evaluate(chx_engine *e, chx_model *const *models, int nmodels, const chx
         layout(e, b, !o->out_dev);
         evaluate_eager(e, o);
         deliver(e, o);
+        tighten_capacities(e, 1.1);  // for the next evaluation of this shape
         e->err.clear();
         return 0;
     } catch (const EngineError &x) {
@@ -1445,6 +1490,7 @@ chx_plan *chx_plan_create(int device, chx_model *const *models, int nmodels, con
         evaluate_eager(e, o);
         if (tune) plan_tune(p, o);
         deliver(e, o);
+        tighten_capacities(e, 1.02);  // the captured grids follow this batch's counts
         plan_capture(p, o);
         return p;
     } catch (const EngineError &x) {
@@ -1486,6 +1532,7 @@ int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double 
         // new shape or a capacity ran short: eager run, then capture again
         evaluate_eager(e, o);
         deliver(e, o);
+        tighten_capacities(e, 1.02);
         plan_capture(p, o);
         e->err.clear();
         return 0;

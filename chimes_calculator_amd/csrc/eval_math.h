@@ -29,9 +29,9 @@ __device__ __forceinline__ void cheby(double r, const chx_slot &s, R (&T)[D], R 
     double dx = r;
     const bool oor = dx < s.rin;
     if (oor) dx = s.rin;
-    const double ex = exp(-1 * dx / s.lam);
-    const double xd = (ex - s.xavg) / s.xdiff;
-    const double dxdr_d = (-ex / s.lam) / s.xdiff;
+    const double ex = exp(-dx * s.ilam);
+    const double xd = (ex - s.xavg) * s.ixdiff;
+    const double dxdr_d = (-ex * s.ilam) * s.ixdiff;
     const R x = (R)xd, dxdr = (R)dxdr_d, two_x = (R)(2.0 * xd);
     R U[D];
     T[0] = (R)1;
@@ -57,23 +57,26 @@ __device__ __forceinline__ void cheby(double r, const chx_slot &s, R (&T)[D], R 
     }
 }
 
-// get_fcut: cubic (1 - r/rc)^3 or Tersoff's sine ramp.
-__device__ __forceinline__ void fcut(double r, double rc, int tersoff, double fvar, double &f, double &fd)
+// get_fcut: cubic (1 - r/rc)^3 or Tersoff's sine ramp, rc = the slot's outer
+// cutoff; divisions by rc and by the ramp width are products with the
+// slot's reciprocals.
+__device__ __forceinline__ void fcut(double r, const chx_slot &s, int tersoff, double fvar, double &f, double &fd)
 {
+    const double rc = s.rout;
     if (!tersoff) {
-        const double f0 = 1.0 - r / rc;
+        const double f0 = 1.0 - r * s.irout;
         f = f0 * f0 * f0;
-        fd = f0 * f0 * (-1.0 * 3.0 / rc);
+        fd = f0 * f0 * (-3.0 * s.irout);
     } else {
         const double thr = rc - fvar * rc;
         if (r < thr) { f = 1.0; fd = 0.0; }
         else if (r > rc) { f = 0.0; fd = 0.0; }
         else {
-            const double a = (r - thr) / (rc - thr) * CHX_PI + CHX_PI / 2.0;
+            const double a = (r - thr) * s.iramp * CHX_PI + CHX_PI / 2.0;
             double sa, ca;
             sincos(a, &sa, &ca);
             f = 0.5 + 0.5 * sa;
-            fd = 0.5 * ca * (CHX_PI / (rc - thr));
+            fd = 0.5 * ca * (CHX_PI * s.iramp);
         }
     }
 }
@@ -91,9 +94,9 @@ __device__ __forceinline__ void eval_pair(double r, const chx_slot &s, const dou
     double dx = r;
     const bool oor = dx < s.rin;
     if (oor) dx = s.rin;
-    const double ex = exp(-1 * dx / s.lam);
-    const double x = (ex - s.xavg) / s.xdiff;
-    const double dxdr = (-ex / s.lam) / s.xdiff;
+    const double ex = exp(-dx * s.ilam);
+    const double x = (ex - s.xavg) * s.ixdiff;
+    const double dxdr = (-ex * s.ilam) * s.ixdiff;
     const double two_x = 2.0 * x;
     double Tm2 = 1.0, Tm1 = x, Um2 = 1.0, Um1 = two_x;
     double sT = c2[1] * x, sdT = c2[1] * dxdr;
@@ -114,7 +117,7 @@ __device__ __forceinline__ void eval_pair(double r, const chx_slot &s, const dou
         sdT *= damp;
     }
     double f, fd;
-    fcut(r, s.rout, tersoff, fvar, f, fd);
+    fcut(r, s, tersoff, fvar, f, fd);
     const double rinv = (r > 0.0) ? 1.0 / r : 1e20;
     E = f * sT;
     fs = (f * sdT + fd * sT) * rinv;

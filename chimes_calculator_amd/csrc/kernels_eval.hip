@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
 #pragma unroll
             for (int q = 0; q < P; q++) {
                 cheby<D, R>(r[q], grp.s[q], T[q], dT[q], flags);
-                fcut(r[q], grp.s[q].rout, grp.tersoff, grp.fcut_var, f[q], fd[q]);
+                fcut(r[q], grp.s[q], grp.tersoff, grp.fcut_var, f[q], fd[q]);
             }
             const R *Cd = sizeof(R) == 8 ? (const R *)grp.C : (const R *)grp.Cf;
             double S, Gq[P], E, fs[P];
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
                 if (act) {
                     const chx_slot sl = grp.s[q];
                     cheby<D, R>(rq, sl, Tq, dTq, flags);
-                    fcut(rq, sl.rout, grp.tersoff, grp.fcut_var, fq, fdq);
+                    fcut(rq, sl, grp.tersoff, grp.fcut_var, fq, fdq);
                 } else {
 #pragma unroll
                     for (int x = 0; x < D; x++) { Tq[x] = (R)0; dTq[x] = (R)0; }
@@ -927,7 +927,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
             if (act) {
                 const chx_slot sl = grp.s[sub];
                 cheby<D, R>(rq, sl, Tq, dTq, flags);
-                fcut(rq, sl.rout, grp.tersoff, grp.fcut_var, fq, fdq);
+                fcut(rq, sl, grp.tersoff, grp.fcut_var, fq, fdq);
             } else {
 #pragma unroll
                 for (int x = 0; x < D; x++) { Tq[x] = (R)0; dTq[x] = (R)0; }

@@ -97,6 +97,35 @@ def test_c1_water(cc, model_dir, synth_cases):
     check_lists(m, rec)
 
 
+def test_tersoff_wide_ramp(cc, tmp_path):
+    """Carbon 2.0 Small (Tersoff ramp over 75% of the cutoff, even-only 2-body
+    powers, special 3- and 4-body cutoffs) on the 512-atom diamond of
+    etc/lmp/tests/test_suite-ev_tally_mb, jittered: the serial path against
+    the oracle.  The fixtures above all use a ramp fraction of 0.5, where
+    the ramp's width and its start coincide."""
+    import gzip
+
+    from conftest import GOLDEN
+    from lmp_setup import read_lammps_data
+    from oracle_bindings import Oracle
+    src = os.path.join(GOLDEN, "lmp", "published_params.Carbon-2.0.Small.2+3+4b.Tersoff.txt.gz")
+    params = os.path.join(str(tmp_path), "carbon_small.txt")
+    with gzip.open(src) as a, open(params, "wb") as b:
+        b.write(a.read())
+    x, _, _, lo, hi, _ = read_lammps_data(os.path.join(GOLDEN, "lmp", "diamond_3.21gcc_512atoms.data.in.gz"))
+    rng = np.random.default_rng(0xC0DE31)
+    pos = x - lo + rng.uniform(-0.08, 0.08, x.shape)
+    cell = np.diag(hi - lo)
+    types = ["C"] * len(pos)
+    R = Oracle(params).calculate(pos, types, cell)
+    m = cc.GpuModel(params)
+    E, F, S = cc.eval_batch(m, [(pos, types, cell)])
+    m.close()
+    assert abs(E[0] - R.energy) <= RTOL * abs(R.energy), "energy %.17g vs %.17g" % (E[0], R.energy)
+    assert_close("forces", F[0], R.forces, RTOL, atol=FATOL)
+    assert_close("stress", S[0], R.stress, RTOL)
+
+
 def test_c2_batch_subset(cc, model_dir, synth_cases):
     """The first 24 configurations of the C2 batch, evaluated as one batch."""
     from chimes_calculator_amd import workloads as W
