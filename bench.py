@@ -49,6 +49,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12   # MI355X dense FP64 (vector and matrix), FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X FP32 (vector and matrix), FLOP/s
+PEAK_HBM = 8.0e12     # MI355X HBM3E, bytes/s
 EV = 0.0433641        # eV per kcal/mol
 METRIC = "atom-evals/s (E+F+stress) at stated error vs FP64; p50/p99 request latency"
 _TMP = []
@@ -577,14 +578,25 @@ def bench_c2(args):
     peak = PEAK_FP32 if f32 else PEAK_FP64
     prof = "%s_co4096_%s_counters.md" % ("k_evalv3" if body == 3 else "k_evalv4", "mixed" if f32 else "fp64")
     figs = profile_figures(prof, kname.replace(" ", ""))
-    roof = {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": flops / (ms * 1e-3) / peak,
-            "traffic": (figs["fetch_bytes"] + figs["write_bytes"]) if "fetch_bytes" in figs else None,
+    traffic = (figs["fetch_bytes"] + figs["write_bytes"]) if "fetch_bytes" in figs else None
+    # bound from the committed counters: HBM when the measured traffic would
+    # take over half the launch at 8 TB/s, else the arithmetic units
+    bound = "hbm" if traffic is not None and traffic / PEAK_HBM > 0.5 * ms * 1e-3 else "mfma"
+    roof = {"bound": bound, "achieved": flops / (ms * 1e-3) / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": flops / (ms * 1e-3) / peak, "traffic": traffic,
             "kernel": "%s (%d-body evaluation on the %s)" % (kname, body, unit),
             "compute_unit": unit, "flops_per_launch": flops, "kernel_ms": ms,
             "clusters_per_launch": nclus, "term_flops_per_cluster": flops / max(nclus, 1),
             "kernel_groups_ms": kt, "counters_source": "profiles/" + prof if figs else None,
             "counters": figs or None}
+    if traffic is not None:
+        roof["hbm_TBps_at_measured_traffic"] = traffic / (ms * 1e-3) / 1e12
+    if unit == "vector ALU":
+        # FP32 arithmetic is built without packed-FP32 instructions, so each
+        # FP32 operation issues at the FP64 vector rate (78.6 TFLOP/s)
+        roof["frac_of_unpacked_vector_peak"] = flops / (ms * 1e-3) / PEAK_FP64
+        if "VALUBusy_pct" in figs:
+            roof["valu_issue_busy_pct"] = figs["VALUBusy_pct"]
     out = {"metric": METRIC, "value": head["value"], "unit": "atom-evals/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32-mixed" if f32 else "f64",

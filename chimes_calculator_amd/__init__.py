@@ -49,7 +49,8 @@ EXPORTS = {
                      "chimes_gpu_cluster_lists", "chimes_gpu_free", "chimes_gpu_fp_edges",
                      "chimes_gpu_fp_histogram"],
     "chimes_lmp.h": ["chimes_lmp_create", "chimes_lmp_free", "chimes_lmp_build_mb_neighlists",
-                     "chimes_lmp_compute", "chimes_lmp_mb_lists", "chimes_lmp_fp_rows", "chimes_lmp_timing"],
+                     "chimes_lmp_compute", "chimes_lmp_mb_lists", "chimes_lmp_fp_rows", "chimes_lmp_timing",
+                     "chimes_lmp_set_tabulation"],
 }
 
 
@@ -141,6 +142,7 @@ def lib():
                                          C.POINTER(C.c_double), _dp, _dp, _dp]
         L.chimes_lmp_mb_lists.argtypes = [C.c_void_p, C.POINTER(C.c_long), pp, C.POINTER(C.c_long), pp]
         L.chimes_lmp_timing.argtypes = [C.c_void_p, _dp]
+        L.chimes_lmp_set_tabulation.argtypes = [C.c_void_p, C.c_int]
         L.chimes_lmp_fp_rows.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long),
                                          C.POINTER(C.POINTER(C.c_double))]
         _lib = L
@@ -273,6 +275,12 @@ class LmpPair:
             m[t] = v
         self.h = self.L.chimes_lmp_create(model.h, nt, m)
         if not self.h:
+            raise ChimesError(_err())
+
+    def set_tabulation(self, on=True):
+        """As the pair style built with -DTABULATION: 2-/3-body interactions
+        from the tables the parameter file names (chimes_lmp_set_tabulation)."""
+        if self.L.chimes_lmp_set_tabulation(self.h, int(bool(on))):
             raise ChimesError(_err())
 
     @staticmethod

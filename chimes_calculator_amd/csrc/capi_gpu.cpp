@@ -388,9 +388,11 @@ void chimes_gpu_free(void *p) { chx_free(p); }
 // chimes_lmp.h: the LAMMPS pair style
 // ---------------------------------------------------------------------------
 #include "chimes_lmp.h"
+#include "tables.h"
 
 struct chimes_lmp {
     chx_lmp *p = nullptr;
+    const chx::ParamSet *P = nullptr;  // the model's parameter set (table names)
     ~chimes_lmp()
     {
         if (p) chx_lmp_destroy(p);
@@ -426,7 +428,20 @@ chimes_lmp *chimes_lmp_create(chimes_gpu_model *m, int ntypes, const double *mas
     }
     chimes_lmp *h = new chimes_lmp();
     h->p = p;
+    h->P = &m->dm->P;
     return h;
+}
+
+int chimes_lmp_set_tabulation(chimes_lmp *p, int on)
+{
+    if (!p) return fail("no pair style");
+    chx::TableSet T;
+    if (on) {
+        std::string err;
+        if (!chx::load_tables(*p->P, T, err)) return fail(err);
+    }
+    if (chx_lmp_set_tables(p->p, &T)) return fail(chx_lmp_error(p->p));
+    return 0;
 }
 
 void chimes_lmp_free(chimes_lmp *p) { delete p; }

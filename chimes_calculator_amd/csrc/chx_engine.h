@@ -193,4 +193,11 @@ void chx_lmp_timing(chx_lmp *p, double *ms4);
 
 #ifdef __cplusplus
 }
+
+namespace chx {
+struct TableSet;
+}
+// The tables of the tabulated 2-/3-body path (a -DTABULATION build of the
+// pair style) for the following compute calls; null or empty: the polynomials.
+int chx_lmp_set_tables(chx_lmp *p, const chx::TableSet *T);
 #endif

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "chx_tab.h"
 #include "chx_types.h"
 
 namespace chx {
@@ -12,7 +13,7 @@ namespace chx {
 constexpr int CHX_LMP_CAND = 512;
 // status words of the adapter
 enum : int { CHX_LMP_ST_FLAGS = 0, CHX_LMP_ST_MAXCAND = 1, CHX_LMP_ST_WORDS = 4 };
-enum : int { CHX_LMP_BAD_CAND = 1, CHX_LMP_BAD_RANGE = 2 };
+enum : int { CHX_LMP_BAD_CAND = 1, CHX_LMP_BAD_RANGE = 2, CHX_LMP_BAD_TAB2 = 4, CHX_LMP_BAD_TAB3 = 8 };
 
 struct chx_lmp_args {
     int inum, nall;
@@ -30,12 +31,13 @@ struct chx_lmp_args {
     int eflag_global, vflag_global;
 };
 
-void launch_lmp_pair(const chx_lmp_args &A, const chx_model_dev &M, hipStream_t st);
+// T: the tables of a -DTABULATION build (null or empty: the polynomials)
+void launch_lmp_pair(const chx_lmp_args &A, const chx_model_dev &M, const chx_tabs_dev *T, hipStream_t st);
 // mode 0: per-owner counts into cnt3 / cnt4; mode 1: lists written at off3 / off4
 void launch_lmp_lists(int mode, const chx_lmp_args &A, double pad3, double pad4, int has4, int *cnt3, int *cnt4,
                       const long long *off3, const long long *off4, int *trips, int *quads, hipStream_t st);
 void launch_lmp_clusters(int nb, int D, long n, const int *list, const chx_lmp_args &A, const chx_model_dev &M,
-                         hipStream_t st);
+                         const chx_tabs_dev *T, hipStream_t st);
 // Fingerprint rows of nb-body clusters (n loop entries: list entries, or for
 // nb = 2 the packed neighbour-row entries): mode 0 counts rows per block of
 // 256 entries into blk_cnt, mode 1 writes them from blk_off (rows of

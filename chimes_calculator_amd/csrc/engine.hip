@@ -32,6 +32,7 @@
 #include "chx_kernels.h"
 #include "chx_fp.h"
 #include "chx_lmp.h"
+#include "tables.h"
 
 using namespace chx;
 
@@ -1769,6 +1770,9 @@ struct chx_lmp {
     std::vector<long long> fp_off_h;
     int types_nall = -1;  // atoms whose ChIMES types are on the device
     DevBuf<unsigned long long> acc;
+    // tables of the tabulated 2-/3-body path (chx_lmp_set_tables)
+    DevBuf<char> tab_buf;
+    chx_tabs_dev tabs{};
     long n3 = 0, n4 = 0;
     bool have_lists = false;
     int nall = 0, inum = 0;
@@ -1872,6 +1876,10 @@ void lmp_check_status(chx_lmp *p, const int *s)
                           " neighbours within the padded many-body cutoffs (at most " +
                           std::to_string(CHX_LMP_CAND) + ")");
     if (s[CHX_LMP_ST_FLAGS] & CHX_LMP_BAD_RANGE) throw EngineError("a contribution is out of the accumulator range");
+    if (s[CHX_LMP_ST_FLAGS] & CHX_LMP_BAD_TAB2)
+        throw EngineError("a pair distance lies at or below the second point of its 2-body table");
+    if (s[CHX_LMP_ST_FLAGS] & CHX_LMP_BAD_TAB3)
+        throw EngineError("a triplet lies outside its 3-body table (or its type has no table)");
 }
 
 // build_mb_neighlists (pair_chimes.cpp:321-478) on the uploaded arrays
@@ -1960,6 +1968,80 @@ void chx_lmp_destroy(chx_lmp *p)
 
 const char *chx_lmp_error(chx_lmp *p) { return p ? p->err.c_str() : "no handle"; }
 
+extern "C++" int chx_lmp_set_tables(chx_lmp *p, const TableSet *T)
+{
+    try {
+        HIPCHK(hipSetDevice(p->device));
+        HIPCHK(hipStreamSynchronize(p->st));
+        p->tabs = chx_tabs_dev{};
+        p->tab_buf.release();
+        if (!T || (T->t2.empty() && T->t3.empty())) return 0;
+        const chx_model *m = p->m;
+        if (!T->t2.empty() && (int)T->t2.size() != m->d.npairs) throw EngineError("one 2-body table per pair type");
+        // one device buffer: offsets and descriptors, then the columns
+        std::vector<char> h;
+        auto put = [&](const void *src, size_t bytes) {
+            const size_t off = align256(h.size());
+            h.resize(off + std::max<size_t>(bytes, 1));
+            if (bytes) memcpy(h.data() + off, src, bytes);
+            return off;
+        };
+        std::vector<int> off2{0}, n3;
+        std::vector<double> r2, e2, f2, dr3, rij3, rik3, rjk3, v3;
+        std::vector<long> o3, o3ik, o3jk;
+        for (const Table2 &t : T->t2) {
+            r2.insert(r2.end(), t.r.begin(), t.r.end());
+            e2.insert(e2.end(), t.e.begin(), t.e.end());
+            f2.insert(f2.end(), t.f.begin(), t.f.end());
+            off2.push_back((int)r2.size());
+        }
+        for (const Table3 &t : T->t3) {
+            const size_t n = (size_t)t.n;
+            n3.push_back(t.n);
+            dr3.push_back(t.dr);
+            o3.push_back((long)rij3.size());
+            o3ik.push_back((long)rik3.size());
+            o3jk.push_back((long)rjk3.size());
+            rij3.insert(rij3.end(), t.rij.begin(), t.rij.begin() + n * n * n);
+            rik3.insert(rik3.end(), t.rik.begin(), t.rik.begin() + n * n);
+            rjk3.insert(rjk3.end(), t.rjk.begin(), t.rjk.begin() + n);
+            v3.insert(v3.end(), t.v.begin(), t.v.begin() + 4 * n * n * n);
+        }
+        const size_t a_off2 = put(off2.data(), off2.size() * sizeof(int));
+        const size_t a_r2 = put(r2.data(), r2.size() * 8), a_e2 = put(e2.data(), e2.size() * 8),
+                     a_f2 = put(f2.data(), f2.size() * 8);
+        const size_t a_n3 = put(n3.data(), n3.size() * sizeof(int)), a_dr3 = put(dr3.data(), dr3.size() * 8);
+        const size_t a_o3 = put(o3.data(), o3.size() * sizeof(long)), a_o3ik = put(o3ik.data(), o3ik.size() * sizeof(long)),
+                     a_o3jk = put(o3jk.data(), o3jk.size() * sizeof(long));
+        const size_t a_rij = put(rij3.data(), rij3.size() * 8), a_rik = put(rik3.data(), rik3.size() * 8),
+                     a_rjk = put(rjk3.data(), rjk3.size() * 8), a_v3 = put(v3.data(), v3.size() * 8);
+        p->tab_buf.ensure(h.size());
+        HIPCHK(hipMemcpy(p->tab_buf.p, h.data(), h.size(), hipMemcpyHostToDevice));
+        char *d = p->tab_buf.p;
+        chx_tabs_dev &D = p->tabs;
+        D.ntab2 = (int)T->t2.size();
+        D.ntab3 = (int)T->t3.size();
+        D.off2 = (const int *)(d + a_off2);
+        D.r2 = (const double *)(d + a_r2);
+        D.e2 = (const double *)(d + a_e2);
+        D.f2 = (const double *)(d + a_f2);
+        D.n3 = (const int *)(d + a_n3);
+        D.dr3 = (const double *)(d + a_dr3);
+        D.o3 = (const long *)(d + a_o3);
+        D.o3ik = (const long *)(d + a_o3ik);
+        D.o3jk = (const long *)(d + a_o3jk);
+        D.rij3 = (const double *)(d + a_rij);
+        D.rik3 = (const double *)(d + a_rik);
+        D.rjk3 = (const double *)(d + a_rjk);
+        D.v3 = (const double *)(d + a_v3);
+        return 0;
+    } catch (const EngineError &e) {
+        p->err = e.msg;
+        p->tabs = chx_tabs_dev{};
+        return 1;
+    }
+}
+
 int chx_lmp_build_lists(chx_lmp *p, int nall, const double *x, const int *tag, int inum, const int *ilist,
                         const int *numneigh, const int *const *firstneigh, double skin)
 {
@@ -2018,11 +2100,13 @@ int chx_lmp_compute(chx_lmp *p, int nall, const double *x, const int *type, cons
         A.Gacc = nG ? A.Facc + nF + nE + nV : nullptr;
         A.eflag_global = eflag_global;
         A.vflag_global = vflag_global;
-        launch_lmp_pair(A, m->dev, st);
+        launch_lmp_pair(A, m->dev, &p->tabs, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(p->ev[3], st));
-        if (m->d.order[1] > 0 && p->n3 > 0) launch_lmp_clusters(3, m->d.D3, p->n3, p->trips.p, A, m->dev, st);
-        if (m->d.order[2] > 0 && p->n4 > 0) launch_lmp_clusters(4, m->d.D4, p->n4, p->quads.p, A, m->dev, st);
+        if (m->d.order[1] > 0 && p->n3 > 0)
+            launch_lmp_clusters(3, m->d.D3, p->n3, p->trips.p, A, m->dev, &p->tabs, st);
+        if (m->d.order[2] > 0 && p->n4 > 0)
+            launch_lmp_clusters(4, m->d.D4, p->n4, p->quads.p, A, m->dev, &p->tabs, st);
         HIPCHK(hipGetLastError());
         const size_t nout = (size_t)nall * 3 + (nE ? nall : 0) + (nV ? (size_t)nall * 6 : 0) + 7;
         p->out.ensure(nout + CHX_LMP_ST_WORDS);

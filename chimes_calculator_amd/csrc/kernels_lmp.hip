@@ -15,6 +15,7 @@
 #include "chx_cluster.h"
 #include "chx_dev.h"
 #include "chx_lmp.h"
+#include "chx_tab.h"
 
 namespace chx {
 
@@ -38,9 +39,12 @@ __device__ __forceinline__ void lmp_order(const double (&s)[6], double (&v)[6])
 // 1- and 2-body: sixteen lanes per owner ilist[ii]; lane l takes row entries
 // l, l + 16, ...  Pairs with tag[j] > tag[i] only (pair_chimes.cpp:603).  The
 // owner's force and tallies are summed over its lanes in double in that fixed
-// order and added exactly once; the partner's are added per pair.
+// order and added exactly once; the partner's are added per pair.  TAB: the
+// pair's energy and force scalar come from its table (compute_2B_tab,
+// chimesFF.cpp:1782-1856: cutoff test, table lookups, no penalty term).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lmp_pair(chx_lmp_args A, chx_model_dev M)
+template <bool TAB>
+__global__ __launch_bounds__(256) void k_lmp_pair(chx_lmp_args A, chx_model_dev M, chx_tabs_dev T)
 {
     const int sub = threadIdx.x & 15;
     const int ii = (int)(blockIdx.x * 16 + (threadIdx.x >> 4));
@@ -50,7 +54,7 @@ __global__ __launch_bounds__(256) void k_lmp_pair(chx_lmp_args A, chx_model_dev 
     const double xi[3] = {A.x[3L * i], A.x[3L * i + 1], A.x[3L * i + 2]};
     // owner force, owner energy share, owner virial share, global energy, global virial
     double fi[3] = {0, 0, 0}, ea = 0.0, va[6] = {0, 0, 0, 0, 0, 0}, ge = 0.0, gv[6] = {0, 0, 0, 0, 0, 0};
-    bool bad = false;
+    bool bad = false, tab_err = false;
     int flags = 0;
     const int beg = A.nbr_off[ii], end = A.nbr_off[ii + 1];
     for (int q = beg + sub; q < end; q += 16) {
@@ -61,9 +65,14 @@ __global__ __launch_bounds__(256) void k_lmp_pair(chx_lmp_args A, chx_model_dev 
         const double r = lmp_dist(xi, xj, dr);
         const int pt = M.pairmap[ti * M.nt + A.ctype[j]];
         double E, fs;
-        eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
-                  M.pen_dist, M.pen_scale, E, fs, flags);
-        if (r >= M.slots2[pt].rout) continue;
+        if constexpr (TAB) {
+            if (r >= M.slots2[pt].rout) continue;
+            tab2_lookup(T, pt, r, E, fs, tab_err);
+        } else {
+            eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
+                      M.pen_dist, M.pen_scale, E, fs, flags);
+            if (r >= M.slots2[pt].rout) continue;
+        }
         const double F[3] = {fs * dr[0], fs * dr[1], fs * dr[2]};
         const double s[6] = {-(fs * (dr[0] * dr[0])), -(fs * (dr[0] * dr[1])), -(fs * (dr[0] * dr[2])),
                              -(fs * (dr[1] * dr[1])), -(fs * (dr[1] * dr[2])), -(fs * (dr[2] * dr[2]))};
@@ -111,6 +120,7 @@ __global__ __launch_bounds__(256) void k_lmp_pair(chx_lmp_args A, chx_model_dev 
         }
     }
     if (bad) atomicOr(&A.status[CHX_LMP_ST_FLAGS], CHX_LMP_BAD_RANGE);
+    if (tab_err) atomicOr(&A.status[CHX_LMP_ST_FLAGS], CHX_LMP_BAD_TAB2);
 }
 
 // ---------------------------------------------------------------------------
@@ -242,9 +252,49 @@ __device__ __forceinline__ void lmp_run_add(int key, int seg, bool last, double 
     __builtin_amdgcn_sched_barrier(0);
 }
 
-template <int NB, int D>
+// compute_3B_tab (chimesFF.cpp:2183-2313): exclusion and the cutoffs of the
+// mapped slots as compute_3B, then the table at the runtime pair distances
+// (ij, ik, jk; get_tab_3B's sort is not executed, 2459-2470); forces and
+// stress of ij, ik, jk with the distance-tensor products, in that order.
+__device__ __forceinline__ void tab3_cluster(const chx_model_dev &M, const chx_tabs_dev &T, const int (&ty)[3],
+                                             const double (&dist)[3], const double (&dr)[3][3], double &E,
+                                             double (&fo)[3][3], double (&S)[6], bool &evaluated, bool &err)
+{
+    E = 0.0;
+    for (int a = 0; a < 3; a++) fo[a][0] = fo[a][1] = fo[a][2] = 0.0;
+    for (int k = 0; k < 6; k++) S[k] = 0.0;
+    evaluated = false;
+    const int tau = M.tripmap[(ty[0] * M.nt + ty[1]) * M.nt + ty[2]];
+    if (tau < 0 || !cluster_inside<3>(M, ty, dist)) return;
+    if (tau >= T.ntab3) {
+        err = true;
+        return;
+    }
+    double v[4];
+    tab3_lookup(T, tau, dist[0], dist[1], dist[2], v, err);
+    evaluated = true;
+    E = v[0];
+    const int ma[3] = {0, 0, 1}, mb[3] = {1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double f = v[1 + q];
+        const double *d = dr[q];
+        for (int k = 0; k < 3; k++) {
+            fo[ma[q]][k] += f * d[k];
+            fo[mb[q]][k] -= f * d[k];
+        }
+        S[0] -= f * (d[0] * d[0]);
+        S[1] -= f * (d[0] * d[1]);
+        S[2] -= f * (d[0] * d[2]);
+        S[3] -= f * (d[1] * d[1]);
+        S[4] -= f * (d[1] * d[2]);
+        S[5] -= f * (d[2] * d[2]);
+    }
+}
+
+template <int NB, int D, bool TAB>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_lmp_clusters(long n, const int *__restrict__ list, chx_lmp_args A,
-                                                      chx_model_dev M)
+                                                      chx_model_dev M, chx_tabs_dev T)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = c < n;
@@ -267,8 +317,13 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     double Ec = 0.0, fo[NB][3], So[6];
     int flags = 0;
-    bool evaluated = false;
-    if (valid) explicit_cluster<NB, D>(M, ty, dist, dr, Ec, fo, So, flags, evaluated, nullptr);
+    bool evaluated = false, tab_err = false;
+    if constexpr (TAB) {
+        if (valid) tab3_cluster(M, T, ty, dist, dr, Ec, fo, So, evaluated, tab_err);
+    } else {
+        if (valid) explicit_cluster<NB, D>(M, ty, dist, dr, Ec, fo, So, flags, evaluated, nullptr);
+    }
+    if (tab_err) atomicOr(&A.status[CHX_LMP_ST_FLAGS], CHX_LMP_BAD_TAB3);
     bool bad = false;
     double v[6];
     lmp_order(So, v);
@@ -472,10 +527,12 @@ void launch_lmp_fp_rows(int nb, int mode, long n, const int *list, const chx_lmp
 #undef L
 }
 
-void launch_lmp_pair(const chx_lmp_args &A, const chx_model_dev &M, hipStream_t st)
+void launch_lmp_pair(const chx_lmp_args &A, const chx_model_dev &M, const chx_tabs_dev *T, hipStream_t st)
 {
     if (A.inum <= 0) return;
-    hipLaunchKernelGGL(k_lmp_pair, dim3((unsigned)((A.inum + 15) / 16)), dim3(256), 0, st, A, M);
+    const dim3 grid((unsigned)((A.inum + 15) / 16)), block(256);
+    if (T && T->ntab2) hipLaunchKernelGGL(k_lmp_pair<true>, grid, block, 0, st, A, M, *T);
+    else hipLaunchKernelGGL(k_lmp_pair<false>, grid, block, 0, st, A, M, chx_tabs_dev{});
 }
 
 void launch_lmp_lists(int mode, const chx_lmp_args &A, double pad3, double pad4, int has4, int *cnt3, int *cnt4,
@@ -491,20 +548,24 @@ void launch_lmp_lists(int mode, const chx_lmp_args &A, double pad3, double pad4,
 }
 
 void launch_lmp_clusters(int nb, int D, long n, const int *list, const chx_lmp_args &A, const chx_model_dev &M,
-                         hipStream_t st)
+                         const chx_tabs_dev *T, hipStream_t st)
 {
     if (n <= 0) return;
     const dim3 grid((unsigned)((n + 127) / 128)), block(128);
+    if (nb == 3 && T && T->ntab3) {
+        hipLaunchKernelGGL((k_lmp_clusters<3, 0, true>), grid, block, 0, st, n, list, A, M, *T);
+        return;
+    }
 #define X(d)                                                                                         \
     if (D == d) {                                                                                    \
-        hipLaunchKernelGGL((k_lmp_clusters<3, d>), grid, block, 0, st, n, list, A, M);             \
+        hipLaunchKernelGGL((k_lmp_clusters<3, d, false>), grid, block, 0, st, n, list, A, M, chx_tabs_dev{}); \
         return;                                                                                      \
     }
     if (nb == 3) { CHX_D3_CASES(X) }
 #undef X
 #define X(d)                                                                                         \
     if (D == d) {                                                                                    \
-        hipLaunchKernelGGL((k_lmp_clusters<4, d>), grid, block, 0, st, n, list, A, M);             \
+        hipLaunchKernelGGL((k_lmp_clusters<4, d, false>), grid, block, 0, st, n, list, A, M, chx_tabs_dev{}); \
         return;                                                                                      \
     }
     if (nb == 4) { CHX_D4_CASES(X) }

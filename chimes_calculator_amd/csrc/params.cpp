@@ -109,6 +109,9 @@ void read_cluster_block(ParamSet &P, Lines &L, int nb)
         if (has(line, kcount)) types.resize(toi(tokens(line), 3));
         if (has(line, kblock)) {
             std::vector<std::string> t = tokens(L.next());
+            // eight items on a triplet's INDEX line: it names a 3-body table
+            // (chimesFF.cpp:882-897), kept in reading order
+            if (nb == 3 && t.size() == 8) P.tab3_files.push_back(t[7]);
             int ci = toi(t, 1);
             if (ci < 0 || ci >= (int)types.size()) throw Fail{"cluster type index out of range"};
             ClusterType &T = types[ci];
@@ -339,7 +342,17 @@ bool load_params(const std::string &path, ParamSet &P, std::string &err)
             if (L.eof) throw Fail{"unexpected end of parameter file"};
             if (has(line, "ENDFILE")) break;
             if (has(line, "PAIRTYPE PARAMS:")) {
-                int p = toi(tokens(line), 2);
+                // seven items: the entry also names a 2-body table
+                // (chimesFF.cpp:719-745), kept in reading order
+                const std::vector<std::string> pt = tokens(line);
+                if (pt.size() == 7) {
+                    if (P.tab2_plain) P.tab2_mixed = true;
+                    P.tab2_files.push_back(pt[6]);
+                } else {
+                    if (!P.tab2_files.empty()) P.tab2_mixed = true;
+                    P.tab2_plain = true;
+                }
+                int p = toi(pt, 2);
                 if (p < 0 || p >= P.npairs()) throw Fail{"pair parameter index out of range"};
                 L.next();
                 P.pow2[p].clear();

@@ -41,6 +41,12 @@ struct ParamSet {
     double fcut_var = 0.0;
     double penalty_dist = 0.01;          // chimesFF.cpp:110
     double penalty_scale = 1.0e4;        // chimesFF.cpp:111
+    // tables named by the parameter file for the tabulated 2-/3-body path of
+    // the LAMMPS pair style (chimesFF.cpp:715-745, 876-918), reading order;
+    // the polynomial coefficients of those entries are read as well
+    std::vector<std::string> tab2_files, tab3_files;
+    bool tab2_plain = false;             // a 2-body entry without a table
+    bool tab2_mixed = false;             // tabulated and plain 2-body entries
     // many-body
     std::vector<ClusterType> trip, quad;
     std::vector<int> tripmap, quadmap;   // [tuple] -> cluster type (negative: excluded)

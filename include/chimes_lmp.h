@@ -27,6 +27,21 @@ typedef struct chimes_lmp chimes_lmp;
 chimes_lmp *chimes_lmp_create(chimes_gpu_model *m, int ntypes, const double *mass);
 void chimes_lmp_free(chimes_lmp *p);
 
+/* The pair style as built with -DTABULATION (pair_chimes.cpp:620-623,
+ * 697-700; chimesFF.cpp:198-352, 715-745, 876-918, 1777-1924, 2178-2484).
+ * on = 1: the 2-body interactions are read from the tables the parameter
+ * file names on its "PAIRTYPE PARAMS:" lines of seven items (compute_2B_tab:
+ * a local cubic spline per pair type, no penalty term), the 3-body ones from
+ * the tables named on "INDEX:" lines of eight items (compute_3B_tab: a
+ * Catmull-Rom tricubic at the runtime pair distances); the table files
+ * <name>.energy and <name>.force are read now, relative to the parameter
+ * file's directory.  A body order whose entries name no table stays
+ * polynomial; 4-body is always polynomial.  on = 0: polynomials throughout.
+ * Errors: unreadable or malformed tables; at compute, a pair at or below the
+ * second point of its table or a triplet past the end of its table (where
+ * the reference exits or reads past its arrays). */
+int chimes_lmp_set_tabulation(chimes_lmp *p, int on);
+
 /* PairCHIMES::build_mb_neighlists (pair_chimes.cpp:321-478): the 3- and
  * 4-mer lists within the largest 3-/4-body cutoffs plus the neighbour skin,
  * kept on the GPU for the following compute calls. */

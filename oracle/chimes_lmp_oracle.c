@@ -144,6 +144,17 @@ static void ev_tally_mb(int n, const int *atoms, double evdwl, const double s[6]
         }
 }
 
+static const oc_tables *g_tab;
+static long g_tab_errors;
+
+void oc_lmp_use_tables(const oc_tables *T)
+{
+    g_tab = T;
+    g_tab_errors = 0;
+}
+
+long oc_lmp_tab_errors(void) { return g_tab_errors; }
+
 /* PairCHIMES::compute (pair_chimes.cpp:480-831) over given many-body lists.
  * f[nall][3], eng, virial[6], eatom[nall], vatom[nall][6] are accumulated
  * onto, as the reference's force and tally arrays. */
@@ -168,7 +179,8 @@ void oc_lmp_compute(const oc_params *P, const double *x, const int *ctype, const
             memset(fo, 0, sizeof fo);
             memset(s, 0, sizeof s);
             e = 0.0;
-            oc_compute_2b(P, dist[0], dr, t, fo, s, &e);
+            if (oc_tab_bodies(g_tab) & 1) g_tab_errors += oc_compute_2b_tab(P, g_tab, dist[0], dr, t, fo, s, &e) != 0;
+            else oc_compute_2b(P, dist[0], dr, t, fo, s, &e);
             for (int d = 0; d < 3; d++) {
                 f[3 * i + d] += fo[d];
                 f[3 * j + d] += fo[3 + d];
@@ -188,7 +200,8 @@ void oc_lmp_compute(const oc_params *P, const double *x, const int *ctype, const
             memset(fo, 0, sizeof fo);
             memset(s, 0, sizeof s);
             e = 0.0;
-            oc_compute_3b(P, dist, dr, t, fo, s, &e);
+            if (oc_tab_bodies(g_tab) & 2) g_tab_errors += oc_compute_3b_tab(P, g_tab, dist, dr, t, fo, s, &e) != 0;
+            else oc_compute_3b(P, dist, dr, t, fo, s, &e);
             for (int a = 0; a < 3; a++)
                 for (int d = 0; d < 3; d++) f[3 * m[a] + d] += fo[3 * a + d];
             ev_tally_mb(3, m, e, s, eflag_global, eflag_atom, vflag_global, vflag_atom, eng, virial, eatom, vatom);

@@ -771,6 +771,17 @@ int oc_cluster_pair_cutoffs(const oc_params *P, int nb, const int *t, int p, dou
     return 0;
 }
 
+int oc_npairs(const oc_params *P) { return P->npairs; }
+
+/* Parameter pair index of two atom types (atom_int_pair_map). */
+int oc_pair_type(const oc_params *P, int ti, int tj) { return P->pairmap[ti * P->nt + tj]; }
+
+/* Triplet type of three atom types (atom_int_trip_map); negative: excluded. */
+int oc_trip_type(const oc_params *P, const int t[3])
+{
+    return P->tripmap[(t[0] * P->nt + t[1]) * P->nt + t[2]];
+}
+
 /* Morse lambda of the pair type of two atom types (chimesFF.cpp:354-1570). */
 double oc_morse_lambda(const oc_params *P, int ti, int tj) { return P->lam[P->pairmap[ti * P->nt + tj]]; }
 

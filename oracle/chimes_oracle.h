@@ -118,6 +118,35 @@ void oc_set_exact_sums(int on);
 long oc_last_penalty_count(void);
 long oc_last_out_of_range_count(void);
 
+int oc_npairs(const oc_params *P);
+int oc_pair_type(const oc_params *P, int ti, int tj);
+int oc_trip_type(const oc_params *P, const int t[3]);
+
+/* Tabulated 2-/3-body interactions of the -DTABULATION build
+ * (chimes_tab_oracle.c).  oc_tab_load reads the tables a parameter file
+ * names (NULL on error, message in oc_tab_error()); oc_tab_bodies: bit 0 the
+ * 2-body, bit 1 the 3-body interactions are tabulated.  The compute functions
+ * return nonzero where the reference exits or reads past its table. */
+typedef struct oc_tables oc_tables;
+oc_tables *oc_tab_load(const oc_params *P, const char *param_path);
+void oc_tab_free(oc_tables *T);
+const char *oc_tab_error(void);
+int oc_tab_bodies(const oc_tables *T);
+int oc_tab_3b_count(const oc_tables *T);
+double oc_tab_2b_value(const oc_tables *T, int pair, double rij, int for_energy, int *err);
+void oc_tab_3b_interp(const oc_tables *T, int trip, double rij, double rik, double rjk, double out[4], int *err);
+int oc_compute_2b_tab(const oc_params *P, const oc_tables *T, double dx, const double dr[3], const int t[2],
+                      double force[6], double stress[6], double *energy);
+int oc_compute_3b_tab(const oc_params *P, const oc_tables *T, const double dx[3], const double dr[9],
+                      const int t[3], double force[9], double stress[6], double *energy);
+
+/* The pair style's loops (oc_lmp_compute) with the tabulated interactions
+ * of T (NULL: the polynomial ones), as pair_chimes.cpp:620-623, 697-700 of a
+ * -DTABULATION build; oc_lmp_tab_errors counts calls that returned an error
+ * since the last oc_lmp_use_tables. */
+void oc_lmp_use_tables(const oc_tables *T);
+long oc_lmp_tab_errors(void);
+
 #ifdef __cplusplus
 }
 #endif

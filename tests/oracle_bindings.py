@@ -13,6 +13,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libchimes_ref.so")
+REF_TAB_SO = os.path.join(ROOT, "oracle", "_ref", "libchimes_ref_tab.so")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -89,6 +90,20 @@ def oracle_lib():
         _llp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
         L.oc_fp_hist.argtypes = [C.c_int, C.c_long, _dp, C.c_long, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
                                  _llp, C.POINTER(C.c_longlong)]
+        L.oc_tab_load.restype = C.c_void_p
+        L.oc_tab_load.argtypes = [C.c_void_p, C.c_char_p]
+        L.oc_tab_free.argtypes = [C.c_void_p]
+        L.oc_tab_error.restype = C.c_char_p
+        L.oc_tab_bodies.argtypes = [C.c_void_p]
+        L.oc_tab_2b_value.restype = C.c_double
+        L.oc_tab_2b_value.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]
+        L.oc_tab_3b_interp.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
+                                       C.POINTER(C.c_int)]
+        L.oc_compute_2b_tab.argtypes = [C.c_void_p, C.c_void_p, C.c_double, _dp, _ip, _dp, _dp,
+                                        C.POINTER(C.c_double)]
+        L.oc_compute_3b_tab.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        L.oc_lmp_use_tables.argtypes = [C.c_void_p]
+        L.oc_lmp_tab_errors.restype = C.c_long
         _oracle = L
     return _oracle
 
@@ -185,6 +200,49 @@ def fp_hist(nb, e1, e2=None, nbin=100, maxd=None):
     return counts, ns.value, binw
 
 
+class TabOracle:
+    """The tabulated 2-/3-body restatement (chimes_tab_oracle.c) over the
+    tables a parameter file names, with the oracle's parameter set."""
+
+    def __init__(self, oracle, param_file):
+        self.O = oracle
+        self.L = oracle.L
+        self.h = self.L.oc_tab_load(oracle.h, param_file.encode())
+        if not self.h:
+            raise ValueError(self.L.oc_tab_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oc_tab_free(self.h)
+            self.h = None
+
+    def bodies(self):
+        return self.L.oc_tab_bodies(self.h)
+
+    def value_2b(self, pair, r, for_energy):
+        err = C.c_int(0)
+        v = self.L.oc_tab_2b_value(self.h, pair, r, int(for_energy), C.byref(err))
+        return v, err.value
+
+    def interp_3b(self, trip, rij, rik, rjk):
+        out, err = np.zeros(4), C.c_int(0)
+        self.L.oc_tab_3b_interp(self.h, trip, rij, rik, rjk, out, C.byref(err))
+        return out, err.value
+
+    def compute_2b(self, dx, dr, t):
+        f, s, e = np.zeros(6), np.zeros(6), C.c_double(0.0)
+        rc = self.L.oc_compute_2b_tab(self.O.h, self.h, dx, np.ascontiguousarray(dr, np.float64),
+                                      np.ascontiguousarray(t, np.int32), f, s, C.byref(e))
+        return f, s, e.value, rc
+
+    def compute_3b(self, dx, dr, t):
+        f, s, e = np.zeros(9), np.zeros(6), C.c_double(0.0)
+        rc = self.L.oc_compute_3b_tab(self.O.h, self.h, np.ascontiguousarray(dx, np.float64),
+                                      np.ascontiguousarray(dr, np.float64), np.ascontiguousarray(t, np.int32), f, s,
+                                      C.byref(e))
+        return f, s, e.value, rc
+
+
 class LmpResult:
     def __init__(self, f, eng, virial, eatom, vatom):
         self.f, self.eng, self.virial, self.eatom, self.vatom = f, eng, virial, eatom, vatom
@@ -230,7 +288,17 @@ class LmpOracle(Oracle):
         self.L.oc_lmp_free(C.cast(p, C.c_void_p))
         return a[:, :P], a[:, P:].astype(np.int32)
 
-    def compute(self, S, trips, quads, eflag_global=1, eflag_atom=1, vflag_global=1, vflag_atom=1):
+    def compute(self, S, trips, quads, eflag_global=1, eflag_atom=1, vflag_global=1, vflag_atom=1, tables=None):
+        """tables: a TabOracle whose tabulated interactions replace the
+        polynomial ones (a -DTABULATION build of the pair style)."""
+        self.L.oc_lmp_use_tables(tables.h if tables is not None else None)
+        try:
+            return self._compute(S, trips, quads, eflag_global, eflag_atom, vflag_global, vflag_atom)
+        finally:
+            self.tab_errors = self.L.oc_lmp_tab_errors()
+            self.L.oc_lmp_use_tables(None)
+
+    def _compute(self, S, trips, quads, eflag_global, eflag_atom, vflag_global, vflag_atom):
         trips = np.ascontiguousarray(trips, np.int32).reshape(-1)
         quads = np.ascontiguousarray(quads, np.int32).reshape(-1)
         f = np.zeros(S.nall * 3)
@@ -311,6 +379,72 @@ class Reference:
             for p in (p2, p3, p4):
                 self.L.ref_free(C.cast(p, C.c_void_p))
         return res
+
+
+_ref_tab = None
+
+
+def ref_tab_available():
+    return os.path.exists(REF_TAB_SO)
+
+
+def ref_tab_lib():
+    global _ref_tab
+    if _ref_tab is None:
+        L = C.CDLL(REF_TAB_SO)
+        L.reft_open.restype = C.c_void_p
+        L.reft_open.argtypes = [C.c_char_p, C.c_char_p]
+        L.reft_close.argtypes = [C.c_void_p]
+        L.reft_flags.argtypes = [C.c_void_p]
+        L.reft_get_tab_2b.restype = C.c_double
+        L.reft_get_tab_2b.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.reft_interp_3b.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+        L.reft_compute_2b.argtypes = [C.c_void_p, C.c_double, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        L.reft_compute_3b.argtypes = [C.c_void_p, _dp, _dp, _ip, _dp, _dp, C.POINTER(C.c_double)]
+        _ref_tab = L
+    return _ref_tab
+
+
+class ReferenceTab:
+    """The reference's chimesFF compiled with -DTABULATION, one call at a time
+    (oracle/ref_tab_driver.cpp).  Its table readers exit the process on a
+    missing file and its lookups on a distance below a 2-body table, so
+    callers keep to inputs inside the tables."""
+
+    def __init__(self, param_file):
+        self.L = ref_tab_lib()
+        d, f = os.path.split(os.path.abspath(param_file))
+        self.h = self.L.reft_open(d.encode(), f.encode())
+        if not self.h:
+            raise RuntimeError("reference could not open %s" % param_file)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.reft_close(self.h)
+            self.h = None
+
+    def flags(self):
+        return self.L.reft_flags(self.h)
+
+    def value_2b(self, pair, r, for_energy):
+        return self.L.reft_get_tab_2b(self.h, pair, r, int(for_energy))
+
+    def interp_3b(self, trip, rij, rik, rjk):
+        out = np.zeros(4)
+        self.L.reft_interp_3b(self.h, trip, rij, rik, rjk, out)
+        return out
+
+    def compute_2b(self, dx, dr, t):
+        f, s, e = np.zeros(6), np.zeros(6), C.c_double(0.0)
+        self.L.reft_compute_2b(self.h, dx, np.ascontiguousarray(dr, np.float64), np.ascontiguousarray(t, np.int32),
+                               f, s, C.byref(e))
+        return f, s, e.value
+
+    def compute_3b(self, dx, dr, t):
+        f, s, e = np.zeros(9), np.zeros(6), C.c_double(0.0)
+        self.L.reft_compute_3b(self.h, np.ascontiguousarray(dx, np.float64), np.ascontiguousarray(dr, np.float64),
+                               np.ascontiguousarray(t, np.int32), f, s, C.byref(e))
+        return f, s, e.value
 
 
 def read_xyz(path):
