@@ -2,12 +2,12 @@
 // checks and exact fixed-point accumulation.
 //
 // Fixed point: a contribution v is split as v = (hi + lo * 2^-40) * 2^-20
-// with integer hi and lo in [0, 2^40), both taken exactly from the double
-// (v * 2^20 and its fractional part are exact operations; only bits below
-// 2^-60 are dropped).  Accumulators hold the integer sums of hi and of lo, so
-// every total is the exact sum of its contributions, independent of the order
-// in which threads add them; converting a total back to double rounds once.
-// |v| must stay below 2^42; larger values set CHX_BAD_RANGE.
+// with integers hi and |lo| <= 2^40, both taken exactly from the double
+// (only bits below 2^-60 are rounded away).  Accumulators hold the integer
+// sums of hi and of lo (lo as a signed 64-bit sum), so every total is the
+// exact sum of its contributions, independent of the order in which threads
+// add them; converting a total back to double rounds once.  |v| must stay
+// below 2^42; larger values set CHX_BAD_RANGE.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -29,13 +29,25 @@ __device__ __forceinline__ bool block_stop(const int *status, int mask)
 
 __device__ __forceinline__ void fx_split(double v, long long &hi, long long &lo, bool &bad)
 {
+    const double s = v * 0x1p20;  // exact
+    if (fabs(v) < 0x1p31) {
+        // |s| < 2^51: adding 1.5 * 2^52 rounds s to an integer held in the
+        // low bits of the sum, and the remainder r = s - round(s), |r| <=
+        // 1/2, is exact; lo = round(r * 2^40) the same way (one fused
+        // operation, r * 2^40 being exact)
+        const double M = 0x1.8p52;
+        const double t = s + M;
+        hi = __double_as_longlong(t) - __double_as_longlong(M);
+        const double r = s - (t - M);
+        lo = __double_as_longlong(fma(r, 0x1p40, M)) - __double_as_longlong(M);
+        return;
+    }
     if (!(fabs(v) < CHX_FX_LIMIT)) {
         bad = true;
         hi = 0;
         lo = 0;
         return;
     }
-    const double s = v * 0x1p20;
     const double f = floor(s);
     hi = (long long)f;
     lo = (long long)((s - f) * 0x1p40);
@@ -162,6 +174,86 @@ __device__ __forceinline__ void fx_owner_runs(unsigned long long *Facc, unsigned
     }
 }
 
+// Sum of x over the wavefront's 64 lanes (each lane's x, or 0), returned in
+// every lane: inclusive row sums by row shifts, then the four row totals.
+__device__ __forceinline__ long long fx_wave_sum(long long x)
+{
+    x += row_shr_i64<1>(x);
+    x += row_shr_i64<2>(x);
+    x += row_shr_i64<4>(x);
+    x += row_shr_i64<8>(x);
+    return readlane_i64(x, 15) + readlane_i64(x, 31) + readlane_i64(x, 47) + readlane_i64(x, 63);
+}
+
+// Adds the energy and virial terms q of each lane's cluster into the
+// accumulator of its configuration, Cacc[(7*key + k)*2 + {0,1}] (key < 0:
+// nothing).  A wavefront whose clusters all belong to one configuration (the
+// common case: configurations hold thousands of clusters) sums each term
+// over its lanes and adds it once; otherwise the lanes' keys form runs
+// (clusters are stored by owner, owners by configuration) and each run adds
+// its sums.  Must be reached by all lanes of the wavefront.
+__device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, const double (&q)[7], bool &bad)
+{
+    const unsigned long long act = __ballot(key >= 0);
+    if (act == 0) return;
+    const int k0 = __builtin_amdgcn_readlane(key, __ffsll((long long)act) - 1);
+    if (__ballot(key >= 0 && key != k0) == 0) {
+        const int l = threadIdx.x & 63;
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            long long x[2] = {0, 0};
+            if (key >= 0) fx_split(q[k], x[0], x[1], bad);
+            const long long h = fx_wave_sum(x[0]), lo = fx_wave_sum(x[1]);
+            if (l == 0) {
+                unsigned long long *a = Cacc + 2L * (7L * k0 + k);
+                if (h) atomicAdd(a, (unsigned long long)h);
+                if (lo) atomicAdd(a + 1, (unsigned long long)lo);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        return;
+    }
+    const int seg = fx_run_first<64>(key);
+    const int l = threadIdx.x & 63;
+    const int knext = __shfl_down(key, 1, 64);
+    const bool last = key >= 0 && (l == 63 || knext != key);
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        long long x[2] = {0, 0};
+        if (key >= 0) fx_split(q[k], x[0], x[1], bad);
+        fx_run_scan<64, 2>(x, seg);
+        if (last) {
+            unsigned long long *a = Cacc + 2L * (7L * key + k);
+            if (x[0]) atomicAdd(a, (unsigned long long)x[0]);
+            if (x[1]) atomicAdd(a + 1, (unsigned long long)x[1]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// The owner's force alone: as fx_owner_runs, three values.
+template <int W>
+__device__ __forceinline__ void fx_owner_force_runs(unsigned long long *Facc, int key, const double (&f)[3], bool &bad)
+{
+    if (__ballot(key >= 0) == 0) return;
+    const int seg = fx_run_first<W>(key);
+    const int l = threadIdx.x & (W - 1);
+    const int knext = __shfl_down(key, 1, W);
+    const bool last = key >= 0 && (l == W - 1 || knext != key);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        long long x[2] = {0, 0};
+        if (key >= 0) fx_split(f[k], x[0], x[1], bad);
+        fx_run_scan<W, 2>(x, seg);
+        if (last) {
+            unsigned long long *a = Facc + 2L * (3L * key + k);
+            if (x[0]) atomicAdd(a, (unsigned long long)x[0]);
+            if (x[1]) atomicAdd(a + 1, (unsigned long long)x[1]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // Owner terms of tiles in which the four lanes l, l + 16, l + 32, l + 48
 // (sub = lane / 16) hold one cluster and the same key: sub 0 adds the owner's
 // force f into Facc[key], subs 1, 2, 3 add the energy and virial terms q
@@ -275,27 +367,29 @@ struct fx_table {
     }
 };
 
-// Exact running total of normalised accumulators: lo parts are reduced below
-// 2^40 before summing, so up to 2^23 accumulators add without overflow.
+// Exact running total of normalised accumulators: a signed lo sum is split
+// into its multiple of 2^40 (moved into hi, arithmetic shift) and a remainder
+// in [0, 2^40) before summing, so up to 2^23 accumulators add without
+// overflow.
 struct fx_sum {
     long long hi = 0;
-    unsigned long long lo = 0;
+    long long lo = 0;
     __device__ __forceinline__ void add(const unsigned long long *a)
     {
-        hi += (long long)a[0] + (long long)(a[1] >> 40);
-        lo += a[1] & 0xffffffffffull;
+        hi += (long long)a[0] + ((long long)a[1] >> 40);
+        lo += (long long)(a[1] & 0xffffffffffull);
     }
     __device__ __forceinline__ void add_value(double v, bool &bad)
     {
         long long h, l;
         fx_split(v, h, l, bad);
         hi += h;
-        lo += (unsigned long long)l;
+        lo += l;
     }
     __device__ __forceinline__ double value() const
     {
-        const long long h = hi + (long long)(lo >> 40);
-        const unsigned long long l = lo & 0xffffffffffull;
+        const long long h = hi + (lo >> 40);
+        const long long l = lo & 0xffffffffffll;
         return (double)h * 0x1p-20 + (double)l * 0x1p-60;
     }
 };

@@ -920,7 +920,7 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->gb4.ensure(G4 + 1);
     e->ge4.ensure(G4 + 1);
     e->Facc.ensure(6 * N);
-    e->Oacc.ensure(14 * N);
+    e->Oacc.ensure(14 * N + 14L * ncfg);  // per owner, then per configuration (Cacc)
     e->part.ensure(14L * e->nitems);
     e->done.ensure(ncfg + 1);
     // input blob
@@ -1001,7 +1001,7 @@ void size_work(chx_engine *e)
     cl[n++] = chx_clear_range{e->cell_cur.p, (long)e->ncell + 1};
     cl[n++] = chx_clear_range{e->cnt34.p, (long)(e->G3 + e->G4) * N + 1};
     cl[n++] = chx_clear_range{(int *)e->Facc.p, 12 * N};
-    cl[n++] = chx_clear_range{(int *)e->Oacc.p, 28 * N};
+    cl[n++] = chx_clear_range{(int *)e->Oacc.p, 28 * N + 28L * e->ncfg};
     cl[n++] = chx_clear_range{e->done.p, (long)e->ncfg + 1};
     e->nclear = n;
     memcpy(e->in_h.p + ((char *)e->clear - e->in_d.p), cl, sizeof(cl));
@@ -1096,6 +1096,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     la.es = e->es.p;
     la.Facc = e->Facc.p;
     la.Oacc = e->Oacc.p;
+    la.Cacc = e->Oacc.p + 14L * N;
     la.cfg_stat = e->cfg_stat;
     la.status = e->status;
     if (G3) {
@@ -1122,7 +1123,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     e->record(4);
     hipLaunchKernelGGL(k_fin, dim3(e->nitems + nblk(Nin, 256)), dim3(256), 0, st, ncfg, e->nitems,
                        e->cfg_item_off, (int)Nin, e->cfg_in_off, e->cfgs, e->as.p, e->mdev, e->Facc.p, e->Oacc.p,
-                       e->nbr_cnt.p, e->part.p, e->done.p, Fout, Ecfg, Vcfg, e->status);
+                       e->Oacc.p + 14L * N, e->nbr_cnt.p, e->part.p, e->done.p, Fout, Ecfg, Vcfg, e->status);
     HIPCHK(hipMemcpyAsync(e->out_h.p, e->out_d.p, e->out_bytes, hipMemcpyDeviceToHost, st));
     e->record(5);
 }
@@ -1392,6 +1393,7 @@ void plan_tune(chx_plan *p, chx_out *o)
         la.es = e->es.p;
         la.Facc = e->Facc.p;  // accumulators are overwritten by the next real run
         la.Oacc = e->Oacc.p;
+        la.Cacc = e->Oacc.p + 14L * e->N;
         la.cfg_stat = e->cfg_stat;
         la.status = e->status;
         la.cap = body == 3 ? e->l3_cap : e->l4_cap;

@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
                                                 const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
                                                 const chx_atom_s *__restrict__ as, int nb_cap,
                                                 const chx_entry_s *__restrict__ es, unsigned long long *Facc,
-                                                unsigned long long *Oacc, int *cfg_stat, int *status)
+                                                unsigned long long *Cacc, int *cfg_stat, int *status)
 {
     using PQ = pairs_of<NB>;
     constexpr int P = PQ::P;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
     const chx_group &grp = groups[g];
     wave_on = wave_on && grp.type >= 0 && grp.D == D;
     const long c = t0 + lane;
-    int owner = -1;
+    int owner = -1, ckey = -1;
     double q7[7] = {0, 0, 0, 0, 0, 0, 0};
     double Fr[NB][3];   // force on each runtime member
     int key[NB];        // replicated atom of each runtime member, -1 none
@@ -341,13 +341,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
                         Fr[m][0] = F[a][0]; Fr[m][1] = F[a][1]; Fr[m][2] = F[a][2];
                     }
             owner = rec.x;
+            ckey = cfg;
             note_flags(cfg_stat, cfg, flags);
         }
     }
 #pragma unroll
     for (int m = 1; m < NB; m++) tab.add(key[m], Fr[m], bad);
-    // runtime member 0 is the owner
-    fx_owner_runs<64>(Facc, Oacc, owner, Fr[0], q7, bad);
+    // runtime member 0 is the owner; energy and virial go to the configuration
+    fx_owner_force_runs<64>(Facc, owner, Fr[0], bad);
+    fx_cfg_sums(Cacc, ckey, q7, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -1082,6 +1084,7 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
                                               const chx_model_dev *__restrict__ models,
                                               const unsigned long long *__restrict__ Facc,
                                               const unsigned long long *__restrict__ Oacc,
+                                              const unsigned long long *__restrict__ Cacc,
                                               const int *__restrict__ nbr_cnt, long long *part, int *done,
                                               double *Fout, double *Ecfg, double *Vcfg, int *status)
 {
@@ -1133,12 +1136,16 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
         acc[0].add_value(M.eoff[as[r].type], bad);
         ni += nbr_cnt[r];
     }
+    // the configuration's own sums (clusters added per configuration), once
+    if (threadIdx.x == 0 && item == cfg_item_off[c])
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc[k].add(Cacc + 2L * (7L * c + k));
     __syncthreads();
     if (ni) atomicAdd(&nimg, ni);
 #pragma unroll
     for (int k = 0; k < 7; k++) {
-        sh_hi[k][threadIdx.x] = acc[k].hi + (long long)(acc[k].lo >> 40);
-        sh_lo[k][threadIdx.x] = acc[k].lo & 0xffffffffffull;
+        sh_hi[k][threadIdx.x] = acc[k].hi + (acc[k].lo >> 40);
+        sh_lo[k][threadIdx.x] = (unsigned long long)(acc[k].lo & 0xffffffffffll);
     }
     __syncthreads();
     for (int w = 128; w >= 1; w >>= 1) {
@@ -1160,7 +1167,7 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
         if (nit == 1) {
             fx_sum t;
             t.hi = h;
-            t.lo = (unsigned long long)l;
+            t.lo = l;
             if (k == 0) Ecfg[c] = t.value(); else Vcfg[6L * c + k - 1] = t.value();
         } else {
             __atomic_store_n(&part[14L * item + 2 * k], h, __ATOMIC_RELAXED);
@@ -1259,11 +1266,11 @@ void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t 
     if (D == d) {                                                                                              \
         if (f32)                                                                                               \
             hipLaunchKernelGGL((k_evalv<NB_, d, float>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups, \
-                               word, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc, la.cfg_stat,  \
+                               word, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Cacc, la.cfg_stat,  \
                                la.status);                                                                     \
         else                                                                                                   \
             hipLaunchKernelGGL((k_evalv<NB_, d, double>), grid, block, 0, st, la.G, la.gbase, la.gend,          \
-                               la.groups, word, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,  \
+                               la.groups, word, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Cacc,  \
                                la.cfg_stat, la.status);                                                        \
         return;                                                                                                \
     }

@@ -396,7 +396,7 @@ __global__ __launch_bounds__(256) void k_lmp_global(chx_lmp_args A, unsigned lon
         long long hi = 0, lo = 0;
         for (long a = (long)blockIdx.x * 256 + threadIdx.x; a < A.nall; a += stride) {
             const unsigned long long *g = A.Gacc + 2 * (7 * a + k);
-            hi += (long long)g[0] + (long long)(g[1] >> 40);
+            hi += (long long)g[0] + ((long long)g[1] >> 40);
             lo += (long long)(g[1] & 0xffffffffffull);
         }
         sh[threadIdx.x] = hi;
@@ -425,8 +425,8 @@ __global__ __launch_bounds__(256) void k_lmp_global(chx_lmp_args A, unsigned lon
         const unsigned long long h = __atomic_load_n(gsum + 2 * threadIdx.x, __ATOMIC_RELAXED);
         const unsigned long long l = __atomic_load_n(gsum + 2 * threadIdx.x + 1, __ATOMIC_RELAXED);
         fx_sum t;
-        t.hi = (long long)h + (long long)(l >> 40);
-        t.lo = l & 0xffffffffffull;
+        t.hi = (long long)h + ((long long)l >> 40);
+        t.lo = (long long)(l & 0xffffffffffull);
         glob[threadIdx.x] = t.value();
     }
 }
