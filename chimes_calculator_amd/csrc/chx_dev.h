@@ -174,6 +174,23 @@ __device__ __forceinline__ void fx_owner_runs(unsigned long long *Facc, unsigned
     }
 }
 
+// Inclusive sum of a double over the lanes of each row of 16 (row shifts
+// with zero fill, a fixed order): lane 15 of a row holds the row's total.
+__device__ __forceinline__ double row_sum16(double x)
+{
+#define CHX_ROW_ADD(D)                                                                      \
+    {                                                                                       \
+        const long long b = row_shr_i64<D>(__double_as_longlong(x));                        \
+        x += __longlong_as_double(b);                                                       \
+    }
+    CHX_ROW_ADD(1)
+    CHX_ROW_ADD(2)
+    CHX_ROW_ADD(4)
+    CHX_ROW_ADD(8)
+#undef CHX_ROW_ADD
+    return x;
+}
+
 // Sum of x over the wavefront's 64 lanes (each lane's x, or 0), returned in
 // every lane: inclusive row sums by row shifts, then the four row totals.
 __device__ __forceinline__ long long fx_wave_sum(long long x)

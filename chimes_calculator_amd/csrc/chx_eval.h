@@ -32,9 +32,10 @@ __global__ void k_fin(int ncfg, int nitems, const int *cfg_item_off, int natoms_
 
 bool supported_D3(int D);
 bool supported_D4(int D);
+// Cacc: the per-configuration energy and virial accumulators
 void launch_eval2(int natoms, int nb_cap, hipStream_t st, const int *nbr_cnt, const chx_cfg *cfgs,
                   const chx_atom_s *as, const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
-                  unsigned long long *Oacc, int *cfg_stat, int *status);
+                  unsigned long long *Cacc, int *cfg_stat, int *status);
 // Vector-unit kernel for nb = 3 or 4 bodies.
 void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t st);
 // nb-body evaluation at precision prec (chx_precision) with the given kernel

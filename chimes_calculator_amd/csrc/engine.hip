@@ -1086,8 +1086,8 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     }
     e->record(1);
     // evaluation with exact accumulation
-    launch_eval2((int)N, e->nb_cap, st, e->nbr_cnt.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p, e->Oacc.p,
-                 e->cfg_stat, e->status);
+    launch_eval2((int)N, e->nb_cap, st, e->nbr_cnt.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p,
+                 e->Oacc.p + 14L * N, e->cfg_stat, e->status);
     e->record(2);
     chx_list_args la{};
     la.cfgs = e->cfgs;

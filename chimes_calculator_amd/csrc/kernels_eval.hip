@@ -72,17 +72,17 @@ __device__ __forceinline__ int find_group(const int *__restrict__ gbase, int G, 
 // virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).  The owner's
 // force, energy and virial are summed in double in a fixed order (lane l of
 // the sixteen takes images l, l + 16, ... of the slab, which is sorted by
-// image key, and a fixed butterfly adds the sixteen lanes), so the sum
-// depends only on the owner's own images, never on the batch around it, and
-// is then added exactly.  Image forces are merged in the workgroup's LDS
-// table.
+// image key, and row shifts add the sixteen lanes), so the sum depends only
+// on the owner's own images, never on the batch around it; the force is then
+// added exactly per owner, the energy and virial exactly per configuration.
+// Image forces are merged in the workgroup's LDS table.
 // ---------------------------------------------------------------------------
 template <int OWN>
 __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) void k_eval2(int natoms, int nb_cap, const int *__restrict__ nbr_cnt,
                                                const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
                                                const chx_entry_s *__restrict__ es,
                                                const chx_model_dev *__restrict__ models, unsigned long long *Facc,
-                                               unsigned long long *Oacc, int *cfg_stat, int *status)
+                                               unsigned long long *Cacc, int *cfg_stat, int *status)
 {
     __shared__ fx_table<OWN == 4 ? 256 : 512> tab;  // image side of the workgroup's owners
     // status read overlapped with the table clear; one barrier for both
@@ -95,6 +95,7 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
     const int i = (int)(blockIdx.x * OWN + (threadIdx.x >> 4));
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // owner force, energy, virial
     bool bad = false;
+    int ckey = -1;  // configuration of the owner, on its sixteenth lane
     if (i < natoms) {
         const int n = nbr_cnt[i];
         const chx_atom_s ai = as[i];
@@ -124,17 +125,20 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
             acc[9] -= fs * (dr[2] * dr[2]);
         }
         note_flags(cfg_stat, ai.cfg, flags);
-#pragma unroll
-        for (int w = 8; w >= 1; w >>= 1)
-#pragma unroll
-            for (int k = 0; k < 10; k++) acc[k] += __shfl_xor(acc[k], w, 64);
-        if (sub == 0) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) fx_atomic(Facc + 2L * (3L * i + k), acc[k], bad);
-#pragma unroll
-            for (int k = 0; k < 7; k++) fx_atomic(Oacc + 2L * (7L * i + k), acc[3 + k], bad);
-        }
+        if (sub == 15) ckey = ai.cfg;
     }
+    // the sixteen lanes of each owner summed in double, in a fixed order
+    // (inclusive row sums by row shifts: lane 15 of the row holds the total)
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = row_sum16(acc[k]);
+    if (sub == 15 && i < natoms)
+#pragma unroll
+        for (int k = 0; k < 3; k++) fx_atomic(Facc + 2L * (3L * i + k), acc[k], bad);
+    // energy and virial per configuration, exact over the wavefront's owners
+    double q7[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) q7[k] = acc[3 + k];
+    fx_cfg_sums(Cacc, ckey, q7, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -1243,17 +1247,17 @@ bool supported_D4(int D) { return D >= 2 && D <= 6; }
 
 void launch_eval2(int natoms, int nb_cap, hipStream_t st, const int *nbr_cnt, const chx_cfg *cfgs,
                   const chx_atom_s *as, const chx_entry_s *es, const chx_model_dev *models, unsigned long long *Facc,
-                  unsigned long long *Oacc, int *cfg_stat, int *status)
+                  unsigned long long *Cacc, int *cfg_stat, int *status)
 {
     if (natoms <= 0) return;
     // results do not depend on the shape: per-owner sums run over that owner's
     // sixteen lanes and image forces are exact sums
     if (natoms < 16384)
         hipLaunchKernelGGL(k_eval2<4>, dim3((unsigned)((natoms + 3) / 4)), dim3(64), 0, st, natoms, nb_cap, nbr_cnt,
-                           cfgs, as, es, models, Facc, Oacc, cfg_stat, status);
+                           cfgs, as, es, models, Facc, Cacc, cfg_stat, status);
     else
         hipLaunchKernelGGL(k_eval2<16>, dim3((unsigned)((natoms + 15) / 16)), dim3(256), 0, st, natoms, nb_cap,
-                           nbr_cnt, cfgs, as, es, models, Facc, Oacc, cfg_stat, status);
+                           nbr_cnt, cfgs, as, es, models, Facc, Cacc, cfg_stat, status);
 }
 
 void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t st)
