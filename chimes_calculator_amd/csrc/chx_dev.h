@@ -248,20 +248,43 @@ __device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, c
     }
 }
 
-// The owner's force alone: as fx_owner_runs, three values.
-template <int W>
+// Inclusive prefix sum of x over the wavefront's 64 lanes, unsegmented
+// (integer sums, modulo 2^64): row sums by row shifts, then the totals of
+// the rows before.
+__device__ __forceinline__ long long fx_wave_prefix(long long x)
+{
+    x += row_shr_i64<1>(x);
+    x += row_shr_i64<2>(x);
+    x += row_shr_i64<4>(x);
+    x += row_shr_i64<8>(x);
+    const long long R0 = readlane_i64(x, 15), R1 = readlane_i64(x, 31), R2 = readlane_i64(x, 47);
+    const int r = (threadIdx.x & 63) >> 4;
+    return x + (r == 0 ? 0ll : (r == 1 ? R0 : (r == 2 ? R0 + R1 : R0 + R1 + R2)));
+}
+
+// The owner's force: each run of equal keys over the wavefront (clusters are
+// stored by owner) adds its exact sum once, from its last lane.  A run's sum
+// is the difference of the unsegmented prefix sums at its last lane and
+// before its first, which is exact in modular integer arithmetic and needs
+// no per-lane conditions.  Must be reached by all lanes of the wavefront.
 __device__ __forceinline__ void fx_owner_force_runs(unsigned long long *Facc, int key, const double (&f)[3], bool &bad)
 {
     if (__ballot(key >= 0) == 0) return;
-    const int seg = fx_run_first<W>(key);
-    const int l = threadIdx.x & (W - 1);
-    const int knext = __shfl_down(key, 1, W);
-    const bool last = key >= 0 && (l == W - 1 || knext != key);
+    const int seg = fx_run_first<64>(key);
+    const int l = threadIdx.x & 63;
+    const int knext = __shfl_down(key, 1, 64);
+    const bool last = key >= 0 && (l == 63 || knext != key);
+    const int before = seg > 0 ? seg - 1 : 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         long long x[2] = {0, 0};
         if (key >= 0) fx_split(f[k], x[0], x[1], bad);
-        fx_run_scan<W, 2>(x, seg);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const long long P = fx_wave_prefix(x[h]);
+            const long long Pb = __shfl(P, before, 64);
+            x[h] = seg > 0 ? P - Pb : P;
+        }
         if (last) {
             unsigned long long *a = Facc + 2L * (3L * key + k);
             if (x[0]) atomicAdd(a, (unsigned long long)x[0]);

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
 #pragma unroll
     for (int m = 1; m < NB; m++) tab.add(key[m], Fr[m], bad);
     // runtime member 0 is the owner; energy and virial go to the configuration
-    fx_owner_force_runs<64>(Facc, owner, Fr[0], bad);
+    fx_owner_force_runs(Facc, owner, Fr[0], bad);
     fx_cfg_sums(Cacc, ckey, q7, bad);
     __syncthreads();
     tab.flush(Facc);
