@@ -51,18 +51,21 @@ struct chx_enum_args {
     const chx_entry_s *es;
     const chx_entry_n *en;
     const chx_enum_model *em;
-    int *cnt3, *cnt4;              // MODE 0 output: per (group, owner) counts [G][N]
-    const int *off3, *off4;        // MODE 1: exclusive scans of the counts
-    const int *gb3, *gb4;          // MODE 1: aligned group bases
-    int2 *list3, *list4;           // MODE 1 output
+    int *cnt3, *cnt4;              // per (group, owner) counts [G][N]
+    int2 *rec;                     // per owner slab of rcap cluster records: (members, group | slot)
+    int *nrec;                     // [natoms] records of each owner
+    int rcap;
+    const int *off3, *off4;        // k_place: exclusive scans of the counts
+    const int *gb3, *gb4;          // k_place: aligned group bases
+    int2 *list3, *list4;           // k_place output
     int *big;                      // [natoms]: owners queued for k_enum_big
     int big_on;                    // k_enum_big runs in this pipeline (else a queued owner asks for a retry)
     int *status;
 };
-template <int MODE, int NT>
+template <int NT>
 __global__ void k_enum(chx_enum_args x);
-template <int MODE>
 __global__ void k_enum_big(chx_enum_args x);
+__global__ void k_place(chx_enum_args x);
 __global__ void k_scan_block(const int *in, int *out, long n, int *bsum);
 __global__ void k_scan_add(int *out, long n, const int *bscan);
 __global__ void k_scan_one(const int *in, int *out, long n, chx_bases_args gb);

@@ -26,7 +26,8 @@ enum {
     CHX_ST_C3 = 5,             // triplets listed
     CHX_ST_C4 = 6,             // quadruplets listed
     CHX_ST_NBIG = 7,           // owners queued for the large enumeration kernel
-    CHX_ST_WORDS = 8
+    CHX_ST_MAXREC = 8,         // largest cluster record count of an owner
+    CHX_ST_WORDS = 9
 };
 enum {
     CHX_BAD_NB = 1,            // an owner has more than CHX_MAX_NB images
@@ -36,7 +37,8 @@ enum {
     CHX_RETRY_L4 = 16,         // quadruplet list buffer too small
     CHX_BAD_RANGE = 32,        // a contribution outside the fixed-point range
     CHX_RETRY_BIG = 64,        // an owner needs the crowded-owner enumeration, which was not launched
-    CHX_RETRY_ANY = CHX_RETRY_ENT | CHX_RETRY_L3 | CHX_RETRY_L4 | CHX_RETRY_BIG,
+    CHX_RETRY_REC = 128,       // an owner's cluster records exceed the record slab
+    CHX_RETRY_ANY = CHX_RETRY_ENT | CHX_RETRY_L3 | CHX_RETRY_L4 | CHX_RETRY_BIG | CHX_RETRY_REC,
     CHX_STOP = CHX_BAD_NB | CHX_BAD_N34 | CHX_RETRY_ANY
 };
 

@@ -372,6 +372,7 @@ struct chx_engine {
     // buffer capacities (entries, list slots); grow on demand
     long l3_cap = 0, l4_cap = 0;
     int nb_cap = 0;  // neighbour images per owner slab
+    int rcap = 0;    // cluster records per owner slab
     long cap_key[4] = {-1, -1, -1, -1};  // batch shape the capacities were measured on
     bool big_on = false;  // launch the crowded-owner enumeration (set once an owner needed it)
     // one host-to-device and one device-to-host copy per run
@@ -402,6 +403,8 @@ struct chx_engine {
     DevBuf<int> cnt34, off34;   // per (group, owner) counts and their scan: 3-body groups, then 4-body
     DevBuf<int> gb3, ge3, gb4, ge4;
     DevBuf<int2> list3, list4;
+    DevBuf<int2> rec;           // per owner slab of rcap cluster records (enumeration -> placement)
+    DevBuf<int> nrec;
     DevBuf<unsigned long long> Facc, Oacc;
     DevBuf<long long> part;
     DevBuf<int> done;  // per configuration: reduction items finished
@@ -443,7 +446,8 @@ struct chx_engine {
     void each_buf(F f)
     {
         f(as); f(an); f(cell_cnt); f(cell_start); f(cell_cur); f(cell_atoms); f(nbr_cnt); f(nbr_off); f(big);
-        f(es); f(en); f(cnt34); f(off34); f(gb3); f(ge3); f(gb4); f(ge4); f(list3); f(list4); f(Facc); f(Oacc);
+        f(es); f(en); f(cnt34); f(off34); f(gb3); f(ge3); f(gb4); f(ge4); f(list3); f(list4); f(rec); f(nrec);
+        f(Facc); f(Oacc);
         f(part); f(done); f(in_d); f(out_d); f(cl_in); f(cl_out); f(cl_types); f(cl_flags);
         for (auto &b : scan_tmp) f(b);
     }
@@ -505,14 +509,14 @@ void chx_engine_trim(chx_engine *e)
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->st);
     e->each_buf([](auto &b) { b.release(); });
-    e->nb_cap = 0;
+    e->nb_cap = e->rcap = 0;
     e->l3_cap = e->l4_cap = 0;
     for (int k = 0; k < 4; k++) e->cap_key[k] = -1;
 }
 
 void chx_engine_reset_capacities(chx_engine *e)
 {
-    e->nb_cap = 0;
+    e->nb_cap = e->rcap = 0;
     e->l3_cap = e->l4_cap = 0;
     for (int k = 0; k < 4; k++) e->cap_key[k] = -1;
 }
@@ -776,20 +780,24 @@ void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const ch
 void guess_capacities(chx_engine *e, chx_model *const *models)
 {
     const double pi = 3.141592653589793;
-    double nbmax = 0, l3 = 0, l4 = 0;
+    double nbmax = 0, l3 = 0, l4 = 0, recmax = 0;
     for (const chx_cfg &C : e->cfg_h) {
         const chx_model *M = models[C.model];
         const double rho = C.natoms / C.vol;
         const double nb = 0.5 * rho * 4.0 / 3.0 * pi * C.search * C.search * C.search;
         nbmax = std::max(nbmax, 1.4 * nb + 12);
+        double r = 0;  // clusters of an owner, on average
         if (M->d.order[1] > 0) {
             const double n3 = 0.5 * rho * 4.0 / 3.0 * pi * C.rc3 * C.rc3 * C.rc3;
             l3 += C.natoms * (0.4 * n3 * n3 + 4);
+            r += 0.4 * n3 * n3 + 4;
         }
         if (M->d.order[2] > 0) {
             const double n4 = 0.5 * rho * 4.0 / 3.0 * pi * C.rc4 * C.rc4 * C.rc4;
             l4 += C.natoms * (0.08 * n4 * n4 * n4 + 4);
+            r += 0.08 * n4 * n4 * n4 + 4;
         }
+        recmax = std::max(recmax, 2.0 * r + 32);
     }
     // A batch of the shape of the last one (same atoms, configurations and
     // groups: a plan, or repeated evaluations of one system) keeps the
@@ -802,6 +810,7 @@ void guess_capacities(chx_engine *e, chx_model *const *models)
     if (!same) {
         for (int k = 0; k < 4; k++) e->cap_key[k] = key[k];
         e->nb_cap = std::min(CHX_MAX_NB, ((int)nbmax + 7) / 8 * 8);
+        e->rcap = ((int)std::min(recmax, 1e6) + 31) / 32 * 32;
         e->l3_cap = (long)l3 + (long)CHX_LIST_ALIGN * (e->G3 + 1);
         e->l4_cap = (long)l4 + (long)CHX_LIST_ALIGN * (e->G4 + 1);
     }
@@ -820,6 +829,8 @@ void tighten_capacities(chx_engine *e, double margin)
     const long n3 = ((long)(st[CHX_ST_N3] * margin) + 1024 + 255) / 256 * 256;
     const long n4 = ((long)(st[CHX_ST_N4] * margin) + 1024 + 255) / 256 * 256;
     e->nb_cap = std::min(e->nb_cap, std::max(nb, 8));
+    const int rc = ((int)(st[CHX_ST_MAXREC] * margin) + 16 + 31) / 32 * 32;
+    if (e->G3 + e->G4) e->rcap = std::min(e->rcap, std::max(rc, 32));
     if (e->G3) e->l3_cap = std::min(e->l3_cap, n3);
     if (e->G4) e->l4_cap = std::min(e->l4_cap, n4);
 }
@@ -890,12 +901,15 @@ void prepare(chx_engine *e, chx_model *const *models, int nmodels, const chx_bat
     }
     e->G3 = (int)e->g3h.size();
     e->G4 = (int)e->g4h.size();
+    if (e->G3 >= 2048 || e->G4 >= 2048) throw EngineError("too many (model, cluster type) groups in one batch");
     if (b->precision < CHX_FP64 || b->precision > CHX_BF16X3) throw EngineError("unknown precision");
     e->prec = b->precision;
     guess_capacities(e, models);
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+constexpr int CHX_NCLEAR = 10;  // ranges of the clear list
 
 // Sizes every buffer, lays out the input and output blobs and fills the host
 // input blob.  host_pos: positions/types are host arrays copied in the blob;
@@ -933,7 +947,7 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     const size_t o_md = off; off = align256(off + nmodels * sizeof(chx_model_dev));
     const size_t o_g3 = off; off = align256(off + (G3 + 1) * sizeof(chx_group));
     const size_t o_g4 = off; off = align256(off + (G4 + 1) * sizeof(chx_group));
-    const size_t o_cl = off; off = align256(off + 8 * sizeof(chx_clear_range));
+    const size_t o_cl = off; off = align256(off + CHX_NCLEAR * sizeof(chx_clear_range));
     const bool host_pos = !b->pos_dev;
     const size_t o_pos = off; if (host_pos) off = align256(off + 3 * Nin * sizeof(double));
     const size_t o_typ = off; if (host_pos) off = align256(off + Nin * sizeof(int));
@@ -993,13 +1007,19 @@ void size_work(chx_engine *e)
     e->list3.ensure(e->l3_cap);
     e->list4.ensure(e->l4_cap);
     const long N = e->N;
-    chx_clear_range cl[8];
+    if (e->G3 + e->G4) {
+        if (N * e->rcap >= (1L << 40)) throw EngineError("batch too large for the cluster record slabs");
+        e->rec.ensure(N * e->rcap);
+        e->nrec.ensure(N);
+    }
+    chx_clear_range cl[CHX_NCLEAR];
     int n = 0;
     cl[n++] = chx_clear_range{e->status, CHX_ST_WORDS};
     cl[n++] = chx_clear_range{e->cfg_stat, 4L * e->ncfg};
     cl[n++] = chx_clear_range{e->cell_cnt.p, (long)e->ncell + 1};
     cl[n++] = chx_clear_range{e->cell_cur.p, (long)e->ncell + 1};
     cl[n++] = chx_clear_range{e->cnt34.p, (long)(e->G3 + e->G4) * N + 1};
+    if (e->G3 + e->G4) cl[n++] = chx_clear_range{e->nrec.p, N};
     cl[n++] = chx_clear_range{(int *)e->Facc.p, 12 * N};
     cl[n++] = chx_clear_range{(int *)e->Oacc.p, 28 * N + 28L * e->ncfg};
     cl[n++] = chx_clear_range{e->done.p, (long)e->ncfg + 1};
@@ -1054,17 +1074,20 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         ea.gb4 = e->gb4.p;
         ea.list3 = e->list3.p;
         ea.list4 = e->list4.p;
+        ea.rec = e->rec.p;
+        ea.nrec = e->nrec.p;
+        ea.rcap = e->rcap;
         ea.big = e->big.p;
         ea.big_on = e->big_on;
         ea.status = e->status;
-        // owners with many images are queued by k_enum<0> for the large-LDS
+        // owners with many images are queued by k_enum for the large-LDS
         // kernel, which is launched once a batch has needed it (until then a
         // queued owner asks for a retry)
         const dim3 gbig(1280);
         // small batches: four wavefronts share an owner's enumeration (latency)
-        if (wide) hipLaunchKernelGGL((k_enum<0, 256>), dim3((unsigned)N), dim3(256), 0, st, ea);
-        else hipLaunchKernelGGL((k_enum<0, 64>), dim3((unsigned)N), dim3(64), 0, st, ea);
-        if (e->big_on) hipLaunchKernelGGL((k_enum_big<0>), gbig, dim3(64), 0, st, ea);
+        if (wide) hipLaunchKernelGGL((k_enum<256>), dim3((unsigned)N), dim3(256), 0, st, ea);
+        else hipLaunchKernelGGL((k_enum<64>), dim3((unsigned)N), dim3(64), 0, st, ea);
+        if (e->big_on) hipLaunchKernelGGL(k_enum_big, gbig, dim3(64), 0, st, ea);
         // one scan over both bodies' counts: segments and positions use
         // differences of the scan only
         chx_bases_args gb{};
@@ -1080,9 +1103,8 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         gb.end4 = e->ge4.p;
         gb.status = e->status;
         e->scan(e->cnt34.p, e->off34.p, (long)(G3 + G4) * N, 0, gb);
-        if (wide) hipLaunchKernelGGL((k_enum<1, 256>), dim3((unsigned)N), dim3(256), 0, st, ea);
-        else hipLaunchKernelGGL((k_enum<1, 64>), dim3((unsigned)N), dim3(64), 0, st, ea);
-        if (e->big_on) hipLaunchKernelGGL((k_enum_big<1>), gbig, dim3(64), 0, st, ea);
+        // every record to its list position
+        hipLaunchKernelGGL(k_place, dim3(nblk(N, 4)), dim3(256), 0, st, ea);
     }
     e->record(1);
     // evaluation with exact accumulation
@@ -1152,6 +1174,7 @@ bool check_status(chx_engine *e)
     if (fl & CHX_RETRY_BIG) e->big_on = true;
     if (fl & CHX_RETRY_ENT)
         e->nb_cap = std::max(e->nb_cap, std::min(CHX_MAX_NB, (st_h[CHX_ST_MAXNB] + 8 + 7) / 8 * 8));
+    if (fl & CHX_RETRY_REC) e->rcap = std::max(e->rcap, ((int)(st_h[CHX_ST_MAXREC] * 1.2) + 32 + 31) / 32 * 32);
     if (fl & CHX_RETRY_L3) e->l3_cap = std::max<long>(e->l3_cap, (long)(st_h[CHX_ST_N3] * 1.2) + 4096);
     if (fl & CHX_RETRY_L4) e->l4_cap = std::max<long>(e->l4_cap, (long)(st_h[CHX_ST_N4] * 1.2) + 4096);
     e->l3_cap = (e->l3_cap + 255) / 256 * 256;
@@ -1247,7 +1270,7 @@ struct chx_plan {
     double *dev_out[3] = {nullptr, nullptr, nullptr};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    long sig[7] = {0, 0, 0, 0, 0, 0, 0};  // N, Nin, ncell, nb_cap, l3_cap, l4_cap, nitems of the captured graph
+    long sig[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // N, Nin, ncell, nb_cap, l3_cap, l4_cap, nitems, rcap of the captured graph
     double tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
     std::string err;
     ~chx_plan()
@@ -1305,8 +1328,8 @@ void plan_capture(chx_plan *p, chx_out *o)
     e->capturing = false;
     HIPCHK(hipStreamEndCapture(e->st, &p->graph));
     HIPCHK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
-    const long sig[7] = {e->N, e->Nin, e->ncell, (long)e->nb_cap, e->l3_cap, e->l4_cap, e->nitems};
-    for (int k = 0; k < 7; k++) p->sig[k] = sig[k];
+    const long sig[8] = {e->N, e->Nin, e->ncell, (long)e->nb_cap, e->l3_cap, e->l4_cap, e->nitems, (long)e->rcap};
+    for (int k = 0; k < 8; k++) p->sig[k] = sig[k];
 }
 
 // Results of the last eager run (raw energy, virial and forces), read from
@@ -1518,9 +1541,9 @@ int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double 
         plan_batch(p, pos, types, cells, b);
         prepare(e, p->models.data(), (int)p->models.size(), &b);
         layout(e, &b, !o->out_dev);
-        const long sig[7] = {e->N, e->Nin, e->ncell, (long)e->nb_cap, e->l3_cap, e->l4_cap, e->nitems};
+        const long sig[8] = {e->N, e->Nin, e->ncell, (long)e->nb_cap, e->l3_cap, e->l4_cap, e->nitems, (long)e->rcap};
         bool same = p->exec != nullptr;
-        for (int k = 0; k < 7; k++) same = same && sig[k] == p->sig[k];
+        for (int k = 0; k < 8; k++) same = same && sig[k] == p->sig[k];
         if (same) {
             size_work(e);
             HIPCHK(hipGraphLaunch(p->exec, e->st));

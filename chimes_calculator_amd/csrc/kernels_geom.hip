@@ -508,11 +508,13 @@ __global__ void k_group_bases(chx_bases_args a)
 }
 
 // ---------------------------------------------------------------------------
-// Cluster enumeration over the sorted images, one 64-lane workgroup per owner.
-// MODE 0 counts clusters per (model, cluster type) group; MODE 1 writes the
-// records (owner, packed slot codes) into the owner's part of each group
-// segment (in any order: every later sum is exact, and exported lists are
-// sorted).  Predicates (serial_chimes_interface.cpp:749-857, with the max 4B
+// Cluster enumeration over the sorted images, one 64-lane workgroup per owner:
+// clusters are counted per (model, cluster type) group and each is recorded
+// in the owner's record slab as (packed member slot codes, group | its slot
+// among the owner's clusters of that group).  After the scan of the counts
+// k_place moves every record to the owner's part of its group segment (in
+// any order: every later sum is exact, and exported lists are sorted), so
+// the enumeration runs once.  Predicates (serial_chimes_interface.cpp:749-857, with the max 4B
 // cutoff <= the max 3B cutoff): j, k in N2(i), j != k, parent(j) <=
 // parent(k), all three distances < rc3; quadruplets add l in N2(i), l not in
 // {j, k}, parent(j), parent(k) <= parent(l), all six distances < rc4.
@@ -521,7 +523,7 @@ __global__ void k_group_bases(chx_bases_args a)
 //
 // The LDS working set is sized for C3 / C4 candidates.  k_enum uses the small
 // size (occupancy); an owner with more than CHX_ENUM_SMALL images is queued
-// in MODE 0 and handled, in both modes, by k_enum_big with the full size.
+// and handled by k_enum_big with the full size.
 // ---------------------------------------------------------------------------
 template <int C3, int C4>
 struct EnumShared {
@@ -534,13 +536,22 @@ struct EnumShared {
     int gst4[C4];
     unsigned A4[C4][EW4];
     int cnt[CHX_MAX_CLUSTER_TYPES + 1];
-    int wb[CHX_MAX_CLUSTER_TYPES + 1];   // MODE 1: list index of the owner's first record per group
     int map[256];                         // type tuple -> group, for nt <= 4
     int wc3[4], wc4[4];                   // per-wavefront candidate counts of one compaction pass
-    int n3, n4;
+    int n3, n4, nrec;
 };
 
-template <int MODE, int C3, int C4, int NT>
+// One cluster of group gg (global index) at its slot: a record in the
+// owner's slab; y = 4-body flag (sign bit) | gg << 20 | slot.
+__device__ __forceinline__ void enum_record(const chx_enum_args &x, int i, int &nrec_sh, int code, bool four, long gg,
+                                            int slot)
+{
+    const int r = atomicAdd(&nrec_sh, 1);
+    if (r < x.rcap)
+        x.rec[(long)i * x.rcap + r] = make_int2(code, (four ? (int)0x80000000 : 0) | ((int)gg << 20) | slot);
+}
+
+template <int C3, int C4, int NT>
 __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumShared<C3, C4> &sh)
 {
     static_assert(NT == 64 || NT == 256, "one or four wavefronts per owner");
@@ -557,7 +568,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     const bool any3 = M.has3 && rc3 > 0.0, any4 = M.has4 && rc4 > 0.0;
     const double rcS = rc3 > rc4 ? rc3 : rc4;  // many-body candidate radius
     if (!any3 && !any4) return;
-    if (lane == 0) { sh.n3 = 0; sh.n4 = 0; }
+    if (lane == 0) { sh.n3 = 0; sh.n4 = 0; sh.nrec = 0; }
     __syncthreads();
     // compact the images within rc3 / rc4, in sorted order (positions: the
     // running count, the counts of the lower wavefronts, the lower lanes)
@@ -631,23 +642,17 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     __syncthreads();
     const int nt = M.nt;
     const int ti = ai.type;
-    // group of a type tuple from LDS when the map is small, and the owner's
-    // write position per group
+    // group of a type tuple from LDS when the map is small
     const bool lmap = nt <= 4;
-    auto stage = [&](const int *map, int nmap, const int *gb, const int *offv, long gbase, int ngl) {
+    auto stage = [&](const int *map, int nmap) {
         if (lmap)
             for (int t = lane; t < nmap; t += NT) sh.map[t] = map[t];
-        if (MODE == 1)
-            for (int g = lane; g < ngl; g += NT) {
-                const long gg = gbase + g;
-                sh.wb[g] = gb[gg] + (offv[gg * N + i] - offv[gg * N]);
-            }
         __syncthreads();
     };
     if (any3) {
         const int ngl = M.ntrip + 1;
         const long gbase = M.g3;
-        stage(M.tripmap, nt * nt * nt, x.gb3, x.off3, gbase, ngl);
+        stage(M.tripmap, nt * nt * nt);
         // every unordered pair u < v of S3 once, spread over the lanes: (j, k)
         // = (u, v) always, and (v, u) too when both are images of one atom
         // (parent(j) <= parent(k) in both orders); the distance is symmetric
@@ -665,12 +670,12 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
                 int g = lmap ? sh.map[t] : M.tripmap[t];
                 if (g < 0) g = ngl - 1;
                 const int slot = atomicAdd(&sh.cnt[g], 1);
-                if (MODE == 1) x.list3[sh.wb[g] + slot] = make_int2(i, (sh.idx3[a] + 1) | ((sh.idx3[b] + 1) << 10));
+                enum_record(x, i, sh.nrec, (sh.idx3[a] + 1) | ((sh.idx3[b] + 1) << 10), false, gbase + g, slot);
             }
         }
         __syncthreads();
         for (int g = lane; g < ngl; g += NT) {
-            if (MODE == 0) x.cnt3[(gbase + g) * N + i] = sh.cnt[g];
+            x.cnt3[(gbase + g) * N + i] = sh.cnt[g];
             sh.cnt[g] = 0;
         }
         __syncthreads();
@@ -679,7 +684,7 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         const int ngl = M.nquad + 1;
         const long gbase = M.g4;
         const int W4 = (n4 + 31) >> 5;
-        stage(M.quadmap, nt * nt * nt * nt, x.gb4, x.off4, gbase, ngl);
+        stage(M.quadmap, nt * nt * nt * nt);
         // every adjacent unordered pair u < v of S4 once, spread over the
         // lanes, as (j, k) = (u, v) and, for two images of one atom, also
         // (v, u); then l over the common neighbours with parent(k) <= parent(l)
@@ -711,57 +716,80 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
                         int g = lmap ? sh.map[t] : M.quadmap[t];
                         if (g < 0) g = ngl - 1;
                         const int slot = atomicAdd(&sh.cnt[g], 1);
-                        if (MODE == 1)
-                            x.list4[sh.wb[g] + slot] =
-                                make_int2(i, (j + 1) | ((k + 1) << 10) | ((sh.idx3[aw] + 1) << 20));
+                        enum_record(x, i, sh.nrec, (j + 1) | ((k + 1) << 10) | ((sh.idx3[aw] + 1) << 20), true,
+                                    gbase + g, slot);
                     }
                 }
             }
         }
         __syncthreads();
         for (int g = lane; g < ngl; g += NT) {
-            if (MODE == 0) x.cnt4[(gbase + g) * N + i] = sh.cnt[g];
+            x.cnt4[(gbase + g) * N + i] = sh.cnt[g];
             sh.cnt[g] = 0;
         }
         __syncthreads();
     }
+    // the owner's record count; the largest one (a word that only grows, so a
+    // count not above it needs no atomic) sizes the slabs of a retry
+    if (lane == 0) {
+        const int nr = sh.nrec;
+        x.nrec[i] = nr;
+        if (nr > __atomic_load_n(&x.status[CHX_ST_MAXREC], __ATOMIC_RELAXED)) atomicMax(&x.status[CHX_ST_MAXREC], nr);
+        if (nr > x.rcap) atomicOr(&x.status[CHX_ST_FLAGS], CHX_RETRY_REC);
+    }
 }
 
-template <int MODE, int NT>
+template <int NT>
 __global__ __launch_bounds__(NT) void k_enum(chx_enum_args x)
 {
     __shared__ EnumShared<CHX_ENUM_SMALL, CHX_ENUM_SMALL> sh;
     if (block_stop(x.status, CHX_STOP)) return;
     const int i = blockIdx.x;
     if (x.nbr_cnt[i] > CHX_ENUM_SMALL) {
-        if (MODE == 0 && threadIdx.x == 0) {
+        if (threadIdx.x == 0) {
             x.big[atomicAdd(&x.status[CHX_ST_NBIG], 1)] = i;
             if (!x.big_on) atomicOr(&x.status[CHX_ST_FLAGS], CHX_RETRY_BIG);
         }
         return;
     }
-    enum_owner<MODE, CHX_ENUM_SMALL, CHX_ENUM_SMALL, NT>(x, i, sh);
+    enum_owner<CHX_ENUM_SMALL, CHX_ENUM_SMALL, NT>(x, i, sh);
 }
 
-// Owners queued by k_enum<0>, any number of them, over a fixed grid.
-template <int MODE>
+// Owners queued by k_enum, any number of them, over a fixed grid.
 __global__ __launch_bounds__(64) void k_enum_big(chx_enum_args x)
 {
     __shared__ EnumShared<CHX_MAX_N3, CHX_MAX_N4> sh;
     if (block_stop(x.status, CHX_STOP)) return;
     const int nbig = x.status[CHX_ST_NBIG];
     for (int t = blockIdx.x; t < nbig; t += gridDim.x) {
-        enum_owner<MODE, CHX_MAX_N3, CHX_MAX_N4, 64>(x, x.big[t], sh);
+        enum_owner<CHX_MAX_N3, CHX_MAX_N4, 64>(x, x.big[t], sh);
         __syncthreads();
     }
 }
 
-template __global__ void k_enum<0, 64>(chx_enum_args);
-template __global__ void k_enum<1, 64>(chx_enum_args);
-template __global__ void k_enum<0, 256>(chx_enum_args);
-template __global__ void k_enum<1, 256>(chx_enum_args);
-template __global__ void k_enum_big<0>(chx_enum_args);
-template __global__ void k_enum_big<1>(chx_enum_args);
+// Records to list positions, one wavefront per owner: a record of group gg at
+// its slot goes to the group's base + the owner's offset within the group
+// (the scan of the counts) + the slot.
+__global__ __launch_bounds__(256) void k_place(chx_enum_args x)
+{
+    if (block_stop(x.status, CHX_STOP)) return;
+    const int i = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= x.natoms) return;
+    const int lane = threadIdx.x & 63;
+    const long N = x.natoms;
+    const int n = x.nrec[i];
+    const int2 *rec = x.rec + (long)i * x.rcap;
+    for (int r = lane; r < n; r += 64) {
+        const int2 e = rec[r];
+        const long gg = (e.y >> 20) & 0x7ff;
+        const int slot = e.y & 0xfffff;
+        if (e.y < 0) x.list4[x.gb4[gg] + (x.off4[gg * N + i] - x.off4[gg * N]) + slot] = make_int2(i, e.x);
+        else x.list3[x.gb3[gg] + (x.off3[gg * N + i] - x.off3[gg * N]) + slot] = make_int2(i, e.x);
+    }
+}
+
+template __global__ void k_enum<64>(chx_enum_args);
+template __global__ void k_enum<256>(chx_enum_args);
 
 // ---------------------------------------------------------------------------
 // Exclusive scan of int32 (out[n] = total), 2048 elements per block.

@@ -42,6 +42,7 @@ struct Request {
     double nrep_atoms = 0;   // atoms after replication
     double nb_cap = 0;       // neighbour-slab entries per owner
     double l3 = 0, l4 = 0;   // triplet / quadruplet list slots
+    double rcap = 0;         // cluster records per owner
     double cells = 0;        // cell-grid cells
     // results: written by the worker before `done` is set under the lock
     double energy = 0.0;
@@ -80,6 +81,11 @@ void size_request(const chx::DeviceModel &m, Request &r)
     r.nb_cap = std::min(512.0, ceil((1.4 * nb + 12) / 8) * 8);
     r.l3 = rc3 > 0 ? n * (0.4 * n3 * n3 + 4) : 0.0;
     r.l4 = rc4 > 0 ? n * (0.08 * n4 * n4 * n4 + 4) : 0.0;
+    // record slabs: the engine starts at twice the mean clusters per owner
+    // and grows to the largest owner's count; three times the mean bounds
+    // the owners of uniform configurations
+    const double rmean = (rc3 > 0 ? 0.4 * n3 * n3 + 4 : 0.0) + (rc4 > 0 ? 0.08 * n4 * n4 * n4 + 4 : 0.0);
+    r.rcap = rmean > 0 ? 3.0 * rmean + 64 : 0.0;
     // half-search-radius cells over the replicated cell
     const double w = 0.5 * search;
     r.cells = std::max(1.0, vol * ncopy / (w * w * w)) + 64.0;
@@ -92,11 +98,12 @@ void size_request(const chx::DeviceModel &m, Request &r)
 // past the density estimate (an owner with more images than the estimate).
 size_t batch_bytes(const std::vector<Request *> &b, int groups)
 {
-    double N = 0, Nin = 0, nb_cap = 8, l3 = 0, l4 = 0, cells = 0;
+    double N = 0, Nin = 0, nb_cap = 8, l3 = 0, l4 = 0, cells = 0, rcap = 0;
     for (const Request *r : b) {
         N += r->nrep_atoms;
         Nin += r->natoms;
         nb_cap = std::max(nb_cap, r->nb_cap);
+        rcap = std::max(rcap, r->rcap);
         l3 += r->l3;
         l4 += r->l4;
         cells += r->cells;
@@ -109,6 +116,7 @@ size_t batch_bytes(const std::vector<Request *> &b, int groups)
                             + 8.0 * groups   // per-(group, owner) cluster counts and their scan
                             + 160.0          // force and owner accumulators
                             + 68.0 * nb_cap  // neighbour slabs
+                            + 8.0 * rcap + 4 // cluster record slabs and counts
                             + 8.0;           // scan partials
     const double bytes = N * per_atom + 8.0 * (l3 + l4) + 16.0 * cells + 40.0 * Nin  // blobs: positions, types, forces
                          + 1024.0 * ncfg + 65536.0;
