@@ -203,13 +203,20 @@ __device__ __forceinline__ long long fx_wave_sum(long long x)
 }
 
 // Adds the energy and virial terms q of each lane's cluster into the
-// accumulator of its configuration, Cacc[(7*key + k)*2 + {0,1}] (key < 0:
+// accumulator of its configuration, copy `slot` (wave-uniform) of the
+// CHX_CACC_SLOTS: Cacc[(key * SLOTS + slot) * STRIDE + 2k + {0,1}] (key < 0:
 // nothing).  A wavefront whose clusters all belong to one configuration (the
 // common case: configurations hold thousands of clusters) sums each term
 // over its lanes and adds it once; otherwise the lanes' keys form runs
 // (clusters are stored by owner, owners by configuration) and each run adds
 // its sums.  Must be reached by all lanes of the wavefront.
-__device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, const double (&q)[7], bool &bad)
+__device__ __forceinline__ unsigned long long *fx_cfg_line(unsigned long long *Cacc, int key, int slot)
+{
+    return Cacc + ((long)key * CHX_CACC_SLOTS + slot) * CHX_CACC_STRIDE;
+}
+
+__device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, int slot, const double (&q)[7],
+                                            bool &bad)
 {
     const unsigned long long act = __ballot(key >= 0);
     if (act == 0) return;
@@ -222,7 +229,7 @@ __device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, c
             if (key >= 0) fx_split(q[k], x[0], x[1], bad);
             const long long h = fx_wave_sum(x[0]), lo = fx_wave_sum(x[1]);
             if (l == 0) {
-                unsigned long long *a = Cacc + 2L * (7L * k0 + k);
+                unsigned long long *a = fx_cfg_line(Cacc, k0, slot) + 2 * k;
                 if (h) atomicAdd(a, (unsigned long long)h);
                 if (lo) atomicAdd(a + 1, (unsigned long long)lo);
             }
@@ -240,7 +247,7 @@ __device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, c
         if (key >= 0) fx_split(q[k], x[0], x[1], bad);
         fx_run_scan<64, 2>(x, seg);
         if (last) {
-            unsigned long long *a = Cacc + 2L * (7L * key + k);
+            unsigned long long *a = fx_cfg_line(Cacc, key, slot) + 2 * k;
             if (x[0]) atomicAdd(a, (unsigned long long)x[0]);
             if (x[1]) atomicAdd(a + 1, (unsigned long long)x[1]);
         }

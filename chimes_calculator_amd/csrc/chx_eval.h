@@ -19,7 +19,7 @@ struct chx_list_args {
     int nb_cap;  // neighbour images per owner slab
     const chx_entry_s *es;
     unsigned long long *Facc, *Oacc;
-    unsigned long long *Cacc;      // [ncfg * 7 * 2] energy and virial per configuration
+    unsigned long long *Cacc;      // [ncfg][CHX_CACC_SLOTS][CHX_CACC_STRIDE] energy and virial per configuration
     int *cfg_stat, *status;
 };
 

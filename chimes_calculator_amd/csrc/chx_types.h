@@ -15,6 +15,12 @@
 #define CHX_MFMA_MAX_D 4       // 4-body dense dimension handled by the matrix-core kernel
 #define CHX_MFMA3_MAX_D 13     // 3-body dense dimension handled by the matrix-core kernel
 #define CHX_STATIC_MAX_CLUSTERS 131072  // clusters per call of the allocation-free explicit-cluster entry
+// Per-configuration energy and virial accumulators: CHX_CACC_SLOTS copies per
+// configuration, one 128-byte line each (7 (hi, lo) pairs), picked by
+// workgroup, so the wavefronts of one large configuration spread their
+// atomics over many lines instead of queueing on one.
+#define CHX_CACC_SLOTS 32
+#define CHX_CACC_STRIDE 16
 
 // Status words of one evaluation (device int array, read back at the end).
 enum {

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
     double q7[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) q7[k] = acc[3 + k];
-    fx_cfg_sums(Cacc, ckey, q7, bad);
+    fx_cfg_sums(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
     for (int m = 1; m < NB; m++) tab.add(key[m], Fr[m], bad);
     // runtime member 0 is the owner; energy and virial go to the configuration
     fx_owner_force_runs(Facc, owner, Fr[0], bad);
-    fx_cfg_sums(Cacc, ckey, q7, bad);
+    fx_cfg_sums(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -1140,10 +1140,13 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
         acc[0].add_value(M.eoff[as[r].type], bad);
         ni += nbr_cnt[r];
     }
-    // the configuration's own sums (clusters added per configuration), once
-    if (threadIdx.x == 0 && item == cfg_item_off[c])
+    // the configuration's own sums (clusters added per configuration, one
+    // copy per thread of the configuration's first item)
+    if ((int)threadIdx.x < CHX_CACC_SLOTS && item == cfg_item_off[c]) {
+        const unsigned long long *a = Cacc + ((long)c * CHX_CACC_SLOTS + threadIdx.x) * CHX_CACC_STRIDE;
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc[k].add(Cacc + 2L * (7L * c + k));
+        for (int k = 0; k < 7; k++) acc[k].add(a + 2 * k);
+    }
     __syncthreads();
     if (ni) atomicAdd(&nimg, ni);
 #pragma unroll

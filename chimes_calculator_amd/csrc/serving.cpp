@@ -119,7 +119,7 @@ size_t batch_bytes(const std::vector<Request *> &b, int groups)
                             + 8.0 * rcap + 4 // cluster record slabs and counts
                             + 8.0;           // scan partials
     const double bytes = N * per_atom + 8.0 * (l3 + l4) + 16.0 * cells + 40.0 * Nin  // blobs: positions, types, forces
-                         + 1024.0 * ncfg + 65536.0;
+                         + (1024.0 + 4096.0) * ncfg + 65536.0;  // + per-configuration accumulator lines
     return (size_t)(1.5 * bytes);
 }
 
