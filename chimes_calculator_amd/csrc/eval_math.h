@@ -139,6 +139,10 @@ __device__ __forceinline__ void eval_pair(double r, const chx_slot &s, const dou
 template <typename R>
 using const_ptr = const __attribute__((address_space(4))) R *;
 
+// Index 0 of every pair is T_0 = 1 with dT_0 = 0 (cheby sets them exactly,
+// damping included), so its products are plain copies and its derivative
+// products vanish; the unrolled levels skip them (the same values: x * 1 and
+// fma(x, 0, y) are exact).
 template <int D, typename R, typename A>
 __device__ __forceinline__ void contract3(const R *__restrict__ C, const R (&T)[3][D], const R (&dT)[3][D],
                                           A &S, A (&G)[3])
@@ -155,15 +159,20 @@ __device__ __forceinline__ void contract3(const R *__restrict__ C, const R (&T)[
 #pragma unroll
         for (int a1 = 0; a1 < D; a1++) {
             const_ptr<R> c = (const_ptr<R>)C + (a0 * D + a1) * D;
-            R z = 0, z2 = 0;
+            R z = c[0], z2 = 0;
 #pragma unroll
-            for (int a2 = 0; a2 < D; a2++) {
+            for (int a2 = 1; a2 < D; a2++) {
                 z = fma(c[a2], T[2][a2], z);
                 z2 = fma(c[a2], dT[2][a2], z2);
             }
-            s1 = fma((A)T[1][a1], (A)z, s1);
-            g11 = fma((A)dT[1][a1], (A)z, g11);
-            g12 = fma((A)T[1][a1], (A)z2, g12);
+            if (a1 == 0) {
+                s1 = (A)z;
+                g12 = (A)z2;
+            } else {
+                s1 = fma((A)T[1][a1], (A)z, s1);
+                g11 = fma((A)dT[1][a1], (A)z, g11);
+                g12 = fma((A)T[1][a1], (A)z2, g12);
+            }
         }
         const A t0 = (A)t0v[0], dt0 = (A)d0v[0];
         s0 = fma(t0, s1, s0);
@@ -207,25 +216,42 @@ __device__ __forceinline__ void contract4(const R *__restrict__ C, const R (&T)[
 #pragma unroll
                     for (int a4 = 0; a4 < D; a4++) {
                         const_ptr<R> c = (const_ptr<R>)C + ((((a0 * D + a1) * D + a2) * D + a3) * D + a4) * D;
-                        R z = 0, z5 = 0;
+                        R z = c[0], z5 = 0;
 #pragma unroll
-                        for (int a5 = 0; a5 < D; a5++) {
+                        for (int a5 = 1; a5 < D; a5++) {
                             z = fma(c[a5], T[5][a5], z);
                             z5 = fma(c[a5], dT[5][a5], z5);
                         }
-                        s4 = fma(T[4][a4], z, s4);
-                        h4[0] = fma(dT[4][a4], z, h4[0]);
-                        h4[1] = fma(T[4][a4], z5, h4[1]);
+                        if (a4 == 0) {
+                            s4 = z;
+                            h4[1] = z5;
+                        } else {
+                            s4 = fma(T[4][a4], z, s4);
+                            h4[0] = fma(dT[4][a4], z, h4[0]);
+                            h4[1] = fma(T[4][a4], z5, h4[1]);
+                        }
                     }
-                    s3 = fma(T[3][a3], s4, s3);
-                    h3[0] = fma(dT[3][a3], s4, h3[0]);
-                    h3[1] = fma(T[3][a3], h4[0], h3[1]);
-                    h3[2] = fma(T[3][a3], h4[1], h3[2]);
+                    if (a3 == 0) {
+                        s3 = s4;
+                        h3[1] = h4[0];
+                        h3[2] = h4[1];
+                    } else {
+                        s3 = fma(T[3][a3], s4, s3);
+                        h3[0] = fma(dT[3][a3], s4, h3[0]);
+                        h3[1] = fma(T[3][a3], h4[0], h3[1]);
+                        h3[2] = fma(T[3][a3], h4[1], h3[2]);
+                    }
                 }
-                s2 = fma(T[2][a2], s3, s2);
-                h2[0] = fma(dT[2][a2], s3, h2[0]);
+                if (a2 == 0) {
+                    s2 = s3;
 #pragma unroll
-                for (int q = 0; q < 3; q++) h2[q + 1] = fma(T[2][a2], h3[q], h2[q + 1]);
+                    for (int q = 0; q < 3; q++) h2[q + 1] = h3[q];
+                } else {
+                    s2 = fma(T[2][a2], s3, s2);
+                    h2[0] = fma(dT[2][a2], s3, h2[0]);
+#pragma unroll
+                    for (int q = 0; q < 3; q++) h2[q + 1] = fma(T[2][a2], h3[q], h2[q + 1]);
+                }
             }
             const A t1 = (A)t1v[0], dt1 = (A)d1v[0];
             s1 = fma(t1, (A)s2, s1);
