@@ -209,12 +209,15 @@ __device__ __forceinline__ long long fx_wave_sum(long long x)
 // common case: configurations hold thousands of clusters) sums each term
 // over its lanes and adds it once; otherwise the lanes' keys form runs
 // (clusters are stored by owner, owners by configuration) and each run adds
-// its sums.  Must be reached by all lanes of the wavefront.
+// its sums.  ROWS: only lane 15 of each row of 16 carries a key (per-row
+// totals), so the wavefront's sum is the four row lanes' values.  Must be
+// reached by all lanes of the wavefront.
 __device__ __forceinline__ unsigned long long *fx_cfg_line(unsigned long long *Cacc, int key, int slot)
 {
     return Cacc + ((long)key * CHX_CACC_SLOTS + slot) * CHX_CACC_STRIDE;
 }
 
+template <bool ROWS = false>
 __device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, int slot, const double (&q)[7],
                                             bool &bad)
 {
@@ -227,7 +230,14 @@ __device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, i
         for (int k = 0; k < 7; k++) {
             long long x[2] = {0, 0};
             if (key >= 0) fx_split(q[k], x[0], x[1], bad);
-            const long long h = fx_wave_sum(x[0]), lo = fx_wave_sum(x[1]);
+            long long h, lo;
+            if constexpr (ROWS) {
+                h = readlane_i64(x[0], 15) + readlane_i64(x[0], 31) + readlane_i64(x[0], 47) + readlane_i64(x[0], 63);
+                lo = readlane_i64(x[1], 15) + readlane_i64(x[1], 31) + readlane_i64(x[1], 47) + readlane_i64(x[1], 63);
+            } else {
+                h = fx_wave_sum(x[0]);
+                lo = fx_wave_sum(x[1]);
+            }
             if (l == 0) {
                 unsigned long long *a = fx_cfg_line(Cacc, k0, slot) + 2 * k;
                 if (h) atomicAdd(a, (unsigned long long)h);

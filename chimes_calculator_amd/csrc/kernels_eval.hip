@@ -85,12 +85,38 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
                                                unsigned long long *Cacc, int *cfg_stat, int *status)
 {
     __shared__ fx_table<OWN == 4 ? 256 : 512> tab;  // image side of the workgroup's owners
+    // the pair tables (slots, coefficients, pair map) of the model of the
+    // workgroup's first owner, staged in LDS when they are small; lanes whose
+    // owner uses another model read them from memory
+    constexpr int SP = 12, SC = 320;
+    __shared__ chx_slot sl_s[SP];
+    __shared__ double c2_s[SC];
+    __shared__ int pm_s[SP * 2];
     // status read overlapped with the table clear; one barrier for both
-    __shared__ int flags_s;
-    if (threadIdx.x == 0) flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
+    __shared__ int flags_s, m_s;
+    if (threadIdx.x == 0) {
+        flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
+        const int i0 = (int)(blockIdx.x * OWN);
+        int m = -1;
+        if (i0 < natoms) {
+            m = cfgs[as[i0].cfg].model;
+            const chx_model_dev &M0 = models[m];
+            if (M0.npairs > SP || M0.npairs * (M0.order[0] + 1) > SC || M0.nt * M0.nt > 2 * SP) m = -1;
+        }
+        m_s = m;
+    }
     tab.clear();
     __syncthreads();
     if (flags_s & CHX_STOP) return;
+    const int ms = m_s;
+    if (ms >= 0) {
+        const chx_model_dev &M0 = models[ms];
+        const int nc = M0.npairs * (M0.order[0] + 1);
+        for (int t = threadIdx.x; t < nc; t += 16 * OWN) c2_s[t] = M0.c2[t];
+        for (int t = threadIdx.x; t < M0.npairs; t += 16 * OWN) sl_s[t] = M0.slots2[t];
+        for (int t = threadIdx.x; t < M0.nt * M0.nt; t += 16 * OWN) pm_s[t] = M0.pairmap[t];
+    }
+    __syncthreads();
     const int sub = threadIdx.x & 15;
     const int i = (int)(blockIdx.x * OWN + (threadIdx.x >> 4));
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // owner force, energy, virial
@@ -102,28 +128,33 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
         const chx_cfg &C = cfgs[ai.cfg];
         const chx_model_dev &M = models[C.model];
         int flags = 0;
-        for (int j = sub; j < n; j += 16) {
-            const chx_entry_s g = es[(long)i * nb_cap + j];
-            double dr[3];
-            const double r = pair_vec(C.h, ai.u, g.v, dr);
-            const int pt = M.pairmap[ai.type * M.nt + g.type];
-            double E, fs;
-            eval_pair(r, M.slots2[pt], M.c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
-                      M.pen_dist, M.pen_scale, E, fs, flags);
-            const double F0[3] = {fs * dr[0], fs * dr[1], fs * dr[2]};
-            const double F1[3] = {-F0[0], -F0[1], -F0[2]};
-            tab.add_or_global(g.parent, F1, Facc, bad);
-            acc[0] += F0[0];
-            acc[1] += F0[1];
-            acc[2] += F0[2];
-            acc[3] += E;
-            acc[4] -= fs * (dr[0] * dr[0]);
-            acc[5] -= fs * (dr[0] * dr[1]);
-            acc[6] -= fs * (dr[0] * dr[2]);
-            acc[7] -= fs * (dr[1] * dr[1]);
-            acc[8] -= fs * (dr[1] * dr[2]);
-            acc[9] -= fs * (dr[2] * dr[2]);
-        }
+        // the owner's images, with the pair tables at pm / sl / c2
+        auto images = [&](const int *pm, const chx_slot *sl, const double *c2) {
+            for (int j = sub; j < n; j += 16) {
+                const chx_entry_s g = es[(long)i * nb_cap + j];
+                double dr[3];
+                const double r = pair_vec(C.h, ai.u, g.v, dr);
+                const int pt = pm[ai.type * M.nt + g.type];
+                double E, fs;
+                eval_pair(r, sl[pt], c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
+                          M.pen_dist, M.pen_scale, E, fs, flags);
+                const double F0[3] = {fs * dr[0], fs * dr[1], fs * dr[2]};
+                const double F1[3] = {-F0[0], -F0[1], -F0[2]};
+                tab.add_or_global(g.parent, F1, Facc, bad);
+                acc[0] += F0[0];
+                acc[1] += F0[1];
+                acc[2] += F0[2];
+                acc[3] += E;
+                acc[4] -= fs * (dr[0] * dr[0]);
+                acc[5] -= fs * (dr[0] * dr[1]);
+                acc[6] -= fs * (dr[0] * dr[2]);
+                acc[7] -= fs * (dr[1] * dr[1]);
+                acc[8] -= fs * (dr[1] * dr[2]);
+                acc[9] -= fs * (dr[2] * dr[2]);
+            }
+        };
+        if (C.model == ms) images(pm_s, sl_s, c2_s);
+        else images(M.pairmap, M.slots2, M.c2);
         note_flags(cfg_stat, ai.cfg, flags);
         if (sub == 15) ckey = ai.cfg;
     }
@@ -138,7 +169,7 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
     double q7[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) q7[k] = acc[3 + k];
-    fx_cfg_sums(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
+    fx_cfg_sums<true>(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
