@@ -8,6 +8,7 @@
 #define CHX_MAX_N4 256         // images per owner within the 4-body cutoff
 #define CHX_FIN_CHUNK 2048     // owners per block of the energy/virial reduction
 #define CHX_NBR_CELLS 128      // searched cells per owner for the flattened candidate loop
+#define CHX_NBR_CANDS 512      // candidates per owner whose cell is tabulated in LDS (binary search above)
 #define CHX_ENUM_WIDE_MAX 16384 // batches up to this many atoms enumerate with four wavefronts per owner
 #define CHX_ENUM_SMALL 128     // owners with at most this many images use the small-LDS enumeration
 #define CHX_MAX_CLUSTER_TYPES 64
@@ -78,7 +79,7 @@ struct chx_cfg {
     int atom_in_off;           // first input atom (global index)
     int cell_off;              // first cell (global index)
     int model;                 // parameter set index
-    int pad[2];
+    double near2;              // squared bound of the cheap neighbour test (see nbr_near)
 };
 
 // Replicated atom, evaluation frame: fractional coordinates u = ih * x.

@@ -720,7 +720,23 @@ void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const ch
         C.search = rc2;
         if (rc3 > C.search) C.search = rc3;
         if (rc4 > C.search) C.search = rc4;
-        C.rc3 = M->d.order[1] > 0 ? rc3 : 0.0;
+        {
+            // the cheap neighbour test skips the round trip of an image
+            // through Cartesian coordinates: delta bounds what that and the
+            // rounding of both tests move a distance by (fractional
+            // coordinates within nl + 2 of the origin, B and Bi the row-sum
+            // norms of the neighbour frame and its inverse)
+            double B = 0.0, Bi = 0.0;
+            for (int r = 0; r < 3; r++) {
+                B = std::max(B, fabs(C.hn[3 * r]) + fabs(C.hn[3 * r + 1]) + fabs(C.hn[3 * r + 2]));
+                Bi = std::max(Bi, fabs(C.ihn[3 * r]) + fabs(C.ihn[3 * r + 1]) + fabs(C.ihn[3 * r + 2]));
+            }
+            const double eps = 0x1p-52;
+            const double delta = B * (nl + 2) * eps * (32.0 * B * Bi + 10.0);
+            const double lim = C.search * (1.0 + 1e-12) + 2.0 * delta;
+            C.near2 = lim * lim * (1.0 + 1e-12);
+        }
+        C.rc3 =M->d.order[1] > 0 ? rc3 : 0.0;
         C.rc4 = M->d.order[2] > 0 ? rc4 : 0.0;
         if (C.rc4 > C.rc3 && C.rc3 > 0.0)
             throw EngineError("4-body cutoff above the 3-body cutoff is not supported");

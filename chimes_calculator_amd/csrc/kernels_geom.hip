@@ -130,12 +130,186 @@ __device__ __forceinline__ bool nbr_image(const chx_cfg &C, const double *ui, co
     return frac_dist(C.hn, ui, vn) < C.search;
 }
 
+// The cheap first test of the same membership: the image's fractional offset
+// taken as ua + sh - ui, without the round trip through Cartesian
+// coordinates, and no square root.  C.near2 is the square of the search
+// radius plus a bound on what the round trip and the rounding of both tests
+// can move a distance by, so every image nbr_image accepts passes here;
+// those that pass are confirmed by nbr_image.
+__device__ __forceinline__ bool nbr_near(const chx_cfg &C, const double *ui, const double *ua, const int *sh)
+{
+    const double dx = (ua[0] + sh[0]) - ui[0], dy = (ua[1] + sh[1]) - ui[1], dz = (ua[2] + sh[2]) - ui[2];
+    double r[3];
+    mv(C.hn, dx, dy, dz, r);
+    return r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < C.near2;
+}
+
+// floor(u / m) and the remainder in [0, m), for m >= 1 and |u| < 2^20: a
+// single-precision estimate (relative error below 2^-22, so off by at most
+// one) and one correction, in place of two integer division sequences.
+__device__ __forceinline__ int floor_div(int u, int m, int &r)
+{
+    int q = (int)floorf((float)u * __builtin_amdgcn_rcpf((float)m));
+    r = u - q * m;
+    if (r < 0) {
+        r += m;
+        q--;
+    } else if (r >= m) {
+        r -= m;
+        q++;
+    }
+    return q;
+}
+
+// Cell coordinates of a cell of configuration C.
+__device__ __forceinline__ void cell_coords(const chx_cfg &C, int cell, int (&ci)[3])
+{
+    const int y = floor_div(cell - C.cell_off, C.m[0], ci[0]);
+    ci[2] = floor_div(y, C.m[1], ci[1]);
+}
+
+// Searched cell at offsets (dx, dy, dz) from the owner's cell ci: the wrapped
+// cell, the image shift, and whether the shift lies within the ghost layers.
+__device__ __forceinline__ bool searched_cell(const chx_cfg &C, const int (&ci)[3], int dx, int dy, int dz,
+                                              int &cell, int (&sh)[3])
+{
+    const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
+    int wc[3];
+    bool ok = true;
+    for (int d = 0; d < 3; d++) {
+        sh[d] = floor_div(u[d], C.m[d], wc[d]);
+        if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
+    }
+    cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
+    return ok;
+}
+
+__device__ __forceinline__ int pack_shift(const int (&sh)[3])
+{
+    return (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
+}
+
+__device__ __forceinline__ void unpack_shift(int spk, int (&sh)[3])
+{
+    sh[0] = (spk & 255) - 128;
+    sh[1] = ((spk >> 8) & 255) - 128;
+    sh[2] = ((spk >> 16) & 255) - 128;
+}
+
+// The searched cells of one owner as one index range of candidates, laid out
+// in LDS by one wavefront: per cell the inclusive prefix of the atom counts
+// (cpre), the cell_atoms index of its first atom minus its first candidate
+// index (cadj), and the packed image shift (cshf); per candidate below
+// CHX_NBR_CANDS the cell it lies in (ccell).  Cells whose shift lies outside
+// the ghost layers hold no candidates.  Returns the candidate count.
+__device__ __forceinline__ int nbr_cells(const chx_cfg &C, const int (&ci)[3], int nc, int lane,
+                                         const int *__restrict__ cell_start, int *cpre, int *cadj, int *cshf,
+                                         unsigned char *ccell)
+{
+    const int w0 = 2 * C.reach[0] + 1, w1 = 2 * C.reach[1] + 1;
+    int carry = 0;
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+        const int c = c0 + lane;
+        int cnt = 0, cs = 0, spk = 0;
+        if (c < nc) {
+            int r0, r1, cell, sh[3];
+            const int c2 = floor_div(floor_div(c, w0, r0), w1, r1);
+            if (searched_cell(C, ci, r0 - C.reach[0], r1 - C.reach[1], c2 - C.reach[2], cell, sh)) {
+                cs = cell_start[cell];
+                cnt = cell_start[cell + 1] - cs;
+                spk = pack_shift(sh);
+            }
+        }
+        int v = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(v, d, 64);
+            if (lane >= d) v += t;
+        }
+        if (c < nc) {
+            const int first = carry + v - cnt;
+            cpre[c] = carry + v;
+            cadj[c] = cs - first;
+            cshf[c] = spk;
+            for (int k = first; k < first + cnt && k < CHX_NBR_CANDS; k++) ccell[k] = (unsigned char)c;
+        }
+        carry += __shfl(v, 63, 64);
+    }
+    return carry;
+}
+
+// Candidate t of the layout above: its atom and packed shift.
+__device__ __forceinline__ void nbr_cand(int t, int T, int nc, const int *cpre, const int *cadj, const int *cshf,
+                                         const unsigned char *ccell, const int *__restrict__ cell_atoms, int &a,
+                                         int &spk)
+{
+    int c;
+    if (T <= CHX_NBR_CANDS) {
+        c = ccell[t];
+    } else {
+        int lo = 0, hi = nc - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (cpre[mid] > t) hi = mid; else lo = mid + 1;
+        }
+        c = lo;
+    }
+    a = cell_atoms[cadj[c] + t];
+    spk = cshf[c];
+}
+
+// Key of a listed image: parent in the high bits, packed shift in the low 24.
+// Images that fail the exact test are marked with the largest key.
+constexpr long long NBR_DROPPED = 0x7fffffffffffffffll;
+
+__device__ __forceinline__ bool nbr_confirm(const chx_cfg &C, const chx_atom_n &ni,
+                                            const chx_atom_n *__restrict__ an, long long k)
+{
+    int sh[3];
+    unpack_shift((int)(k & 0xffffff), sh);
+    const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
+    double vn[3];
+    return nbr_image(C, ni.u, an[(int)(k >> 24)].u, sh, zero, vn);
+}
+
+// Slab entry of image key k (both frames).
+__device__ __forceinline__ void nbr_write(const chx_cfg &C, const chx_atom_n &ni, const chx_atom_s *__restrict__ as,
+                                          const chx_atom_n *__restrict__ an, long long k, chx_entry_s *es,
+                                          chx_entry_n *en)
+{
+    const int a = (int)(k >> 24);
+    const int spack = (int)(k & 0xffffff);
+    int sh[3];
+    unpack_shift(spack, sh);
+    const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
+    chx_entry_n f;
+    nbr_image(C, ni.u, an[a].u, sh, zero, f.v);
+    f.shift = spack;
+    f.pad = 0;
+    const chx_atom_s sa = as[a];
+    chx_entry_s e;
+    if (zero) {
+        e.v[0] = sa.u[0]; e.v[1] = sa.u[1]; e.v[2] = sa.u[2];
+    } else {
+        double t[3] = {sa.u[0], sa.u[1], sa.u[2]};
+        t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
+        double xg[3];
+        mv(C.h, t[0], t[1], t[2], xg);
+        mv(C.ih, xg[0], xg[1], xg[2], e.v);
+    }
+    e.parent = a;
+    e.type = sa.type;
+    *es = e;
+    *en = f;
+}
+
 // ---------------------------------------------------------------------------
 // Neighbour lists: for real atom i every image g = (a, n) with i <= a,
 // g != i and |r_g - r_i| < search in the neighbour frame
 // (serial_chimes_interface.cpp:682-731).  One wavefront per owner, four
 // owners per workgroup; lanes stride over the atoms of each searched cell.
-// The (parent, shift) keys are collected in LDS, ranked, and each owner's
+// Candidates that pass the cheap test are collected as (parent, shift) keys
+// in LDS, confirmed by the reference's arithmetic, ranked, and each owner's
 // images written sorted by key into its slab, so every list and every later
 // traversal is in one fixed order.
 // ---------------------------------------------------------------------------
@@ -159,158 +333,88 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
                                               int nb_cap, int *status)
 {
     __shared__ long long keys[4][CHX_MAX_NB];
-    // searched cells of each wave's owner: inclusive prefix of their atom
-    // counts, first cell_atoms slot, packed image shift
-    __shared__ int cpre[4][CHX_NBR_CELLS], cbeg[4][CHX_NBR_CELLS], cshf[4][CHX_NBR_CELLS];
+    __shared__ int cpre[4][CHX_NBR_CELLS], cadj[4][CHX_NBR_CELLS], cshf[4][CHX_NBR_CELLS];
+    __shared__ unsigned char ccell[4][CHX_NBR_CANDS];
     if (block_stop(status, CHX_STOP)) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + wv;
     const bool own = i < natoms;
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    int n = 0;
+    int n1 = 0;  // candidates that passed the cheap test
     chx_atom_n ni{};
     const chx_cfg &C = cfgs[own ? as[i].cfg : 0];
-    int ci[3] = {0, 0, 0}, nc = 0;
-    // searched cell c = (dx, dy, dz) offsets from the owner's cell, x fastest:
-    // the wrapped cell, the image shift, and whether the shift is inside the
-    // ghost layers
-    auto searched_cell = [&](int dx, int dy, int dz, int &cell, int (&sh)[3]) {
-        const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
-        int wc[3];
-        bool ok = true;
-        for (int d = 0; d < 3; d++) {
-            int q = u[d] % C.m[d];
-            if (q < 0) q += C.m[d];
-            wc[d] = q;
-            sh[d] = (u[d] - q) / C.m[d];
-            if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
-        }
-        cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
-        return ok;
-    };
-    // one candidate per lane: image (a, sh) of the owner's list?
-    auto take = [&](bool valid, int a, const int (&sh)[3], int spack) {
+    int ci[3] = {0, 0, 0}, nc = 0, T = 0;
+    // one candidate per lane: cheap test, keys of those that pass appended
+    auto take = [&](bool valid, int a, int spk) {
         bool in = false;
         if (valid) {
+            int sh[3];
+            unpack_shift(spk, sh);
             const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
-            if (!(a < i || (a == i && zero))) {
-                double vn[3];
-                in = nbr_image(C, ni.u, an[a].u, sh, zero, vn);
-            }
+            if (!(a < i || (a == i && zero))) in = nbr_near(C, ni.u, an[a].u, sh);
         }
         const unsigned long long b = __ballot(in);
         if (in) {
-            const int idx = n + __popcll(b & below);
-            if (idx < CHX_MAX_NB) keys[wv][idx] = ((long long)a << 24) | spack;
+            const int idx = n1 + __popcll(b & below);
+            if (idx < CHX_MAX_NB) keys[wv][idx] = ((long long)a << 24) | spk;
         }
-        n += __popcll(b);
+        n1 += __popcll(b);
     };
     if (own) {
         ni = an[i];
-        const int lc = ni.cell - C.cell_off;
-        ci[0] = lc % C.m[0];
-        ci[1] = (lc / C.m[0]) % C.m[1];
-        ci[2] = lc / (C.m[0] * C.m[1]);
-        const int w0 = 2 * C.reach[0] + 1, w1 = 2 * C.reach[1] + 1;
-        nc = w0 * w1 * (2 * C.reach[2] + 1);
-        if (nc <= CHX_NBR_CELLS) {
-            int carry = 0;
-            for (int c0 = 0; c0 < nc; c0 += 64) {
-                const int c = c0 + lane;
-                int cnt = 0, cs = 0, spk = 0;
-                if (c < nc) {
-                    int cell, sh[3];
-                    if (searched_cell(c % w0 - C.reach[0], (c / w0) % w1 - C.reach[1], c / (w0 * w1) - C.reach[2],
-                                      cell, sh)) {
-                        cs = cell_start[cell];
-                        cnt = cell_start[cell + 1] - cs;
-                        spk = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
-                    }
-                }
-                int v = cnt;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const int t = __shfl_up(v, d, 64);
-                    if (lane >= d) v += t;
-                }
-                if (c < nc) {
-                    cpre[wv][c] = carry + v;
-                    cbeg[wv][c] = cs;
-                    cshf[wv][c] = spk;
-                }
-                carry += __shfl(v, 63, 64);
-            }
-        }
+        cell_coords(C, ni.cell, ci);
+        nc = (2 * C.reach[0] + 1) * (2 * C.reach[1] + 1) * (2 * C.reach[2] + 1);
+        if (nc <= CHX_NBR_CELLS) T = nbr_cells(C, ci, nc, lane, cell_start, cpre[wv], cadj[wv], cshf[wv], ccell[wv]);
     }
     __syncthreads();
     if (own) {
         if (nc <= CHX_NBR_CELLS) {
-            // the candidates of all searched cells as one index range over the lanes
-            const int T = cpre[wv][nc - 1];
             for (int t0 = 0; t0 < T; t0 += 64) {
                 const int t = t0 + lane;
-                int a = 0, spk = 0, sh[3] = {0, 0, 0};
-                if (t < T) {
-                    int lo = 0, hi = nc - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (cpre[wv][mid] > t) hi = mid; else lo = mid + 1;
-                    }
-                    const int before = lo ? cpre[wv][lo - 1] : 0;
-                    a = cell_atoms[cbeg[wv][lo] + (t - before)];
-                    spk = cshf[wv][lo];
-                    sh[0] = (spk & 255) - 128;
-                    sh[1] = ((spk >> 8) & 255) - 128;
-                    sh[2] = ((spk >> 16) & 255) - 128;
-                }
-                take(t < T, a, sh, spk);
+                int a = 0, spk = 0;
+                if (t < T) nbr_cand(t, T, nc, cpre[wv], cadj[wv], cshf[wv], ccell[wv], cell_atoms, a, spk);
+                take(t < T, a, spk);
             }
         } else {
             for (int dz = -C.reach[2]; dz <= C.reach[2]; dz++)
                 for (int dy = -C.reach[1]; dy <= C.reach[1]; dy++)
                     for (int dx = -C.reach[0]; dx <= C.reach[0]; dx++) {
                         int cell, sh[3];
-                        if (!searched_cell(dx, dy, dz, cell, sh)) continue;
+                        if (!searched_cell(C, ci, dx, dy, dz, cell, sh)) continue;
                         const int cs = cell_start[cell], ce = cell_start[cell + 1];
-                        const int spack = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
+                        const int spk = pack_shift(sh);
                         for (int p0 = cs; p0 < ce; p0 += 64) {
                             const int p = p0 + lane;
-                            take(p < ce, p < ce ? cell_atoms[p] : 0, sh, spack);
+                            take(p < ce, p < ce ? cell_atoms[p] : 0, spk);
                         }
                     }
         }
-        if (lane == 0) nbr_slab_note(i, n, nb_cap, nbr_cnt, nbr_off, status);
     }
+    __syncthreads();
+    // the reference's test on the keys that passed; failures are marked
+    int n = n1;
+    if (own && n1 <= CHX_MAX_NB) {
+        n = 0;
+        for (int j0 = 0; j0 < n1; j0 += 64) {
+            const int j = j0 + lane;
+            bool in = false;
+            if (j < n1) {
+                in = nbr_confirm(C, ni, an, keys[wv][j]);
+                if (!in) keys[wv][j] = NBR_DROPPED;
+            }
+            n += __popcll(__ballot(in));
+        }
+    }
+    if (own && lane == 0) nbr_slab_note(i, n, nb_cap, nbr_cnt, nbr_off, status);
     __syncthreads();
     if (!own || n > CHX_MAX_NB || n > nb_cap) return;
     const long off = (long)i * nb_cap;
-    for (int j = lane; j < n; j += 64) {
+    for (int j = lane; j < n1; j += 64) {
         const long long k = keys[wv][j];
+        if (k == NBR_DROPPED) continue;
         int rk = 0;
-        for (int q = 0; q < n; q++) rk += keys[wv][q] < k;
-        const int a = (int)(k >> 24);
-        const int spack = (int)(k & 0xffffff);
-        const int sh[3] = {(spack & 255) - 128, ((spack >> 8) & 255) - 128, ((spack >> 16) & 255) - 128};
-        const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
-        chx_entry_n f;
-        nbr_image(C, ni.u, an[a].u, sh, zero, f.v);
-        f.shift = spack;
-        f.pad = 0;
-        const chx_atom_s sa = as[a];
-        chx_entry_s e;
-        if (zero) {
-            e.v[0] = sa.u[0]; e.v[1] = sa.u[1]; e.v[2] = sa.u[2];
-        } else {
-            double t[3] = {sa.u[0], sa.u[1], sa.u[2]};
-            t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
-            double xg[3];
-            mv(C.h, t[0], t[1], t[2], xg);
-            mv(C.ih, xg[0], xg[1], xg[2], e.v);
-        }
-        e.parent = a;
-        e.type = sa.type;
-        es[off + rk] = e;
-        en[off + rk] = f;
+        for (int q = 0; q < n1; q++) rk += keys[wv][q] < k;
+        nbr_write(C, ni, as, an, k, es + off + rk, en + off + rk);
     }
 }
 
@@ -325,8 +429,9 @@ __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__r
                                                    chx_entry_s *es, chx_entry_n *en, int nb_cap, int *status)
 {
     __shared__ long long keys[CHX_MAX_NB];
-    __shared__ int cpre[CHX_NBR_CELLS], cbeg[CHX_NBR_CELLS], cshf[CHX_NBR_CELLS];
-    __shared__ int ncount;
+    __shared__ int cpre[CHX_NBR_CELLS], cadj[CHX_NBR_CELLS], cshf[CHX_NBR_CELLS];
+    __shared__ unsigned char ccell[CHX_NBR_CANDS];
+    __shared__ int ncount, nexact, ncand;
     if (block_stop(status, CHX_STOP)) return;
     const int i = blockIdx.x;
     if (i >= natoms) return;
@@ -334,33 +439,18 @@ __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__r
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
     const chx_atom_n ni = an[i];
     const chx_cfg &C = cfgs[as[i].cfg];
-    const int lc = ni.cell - C.cell_off;
-    const int ci[3] = {lc % C.m[0], (lc / C.m[0]) % C.m[1], lc / (C.m[0] * C.m[1])};
-    const int w0 = 2 * C.reach[0] + 1, w1 = 2 * C.reach[1] + 1;
-    const int nc = w0 * w1 * (2 * C.reach[2] + 1);
-    auto searched_cell = [&](int dx, int dy, int dz, int &cell, int (&sh)[3]) {
-        const int u[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz};
-        int wc[3];
-        bool ok = true;
-        for (int d = 0; d < 3; d++) {
-            int q = u[d] % C.m[d];
-            if (q < 0) q += C.m[d];
-            wc[d] = q;
-            sh[d] = (u[d] - q) / C.m[d];
-            if (sh[d] > C.n_layers || sh[d] < -C.n_layers) ok = false;
-        }
-        cell = C.cell_off + wc[0] + C.m[0] * (wc[1] + C.m[1] * wc[2]);
-        return ok;
-    };
-    // candidates of this wavefront's lanes: appended to the owner's key list
-    auto take = [&](bool valid, int a, const int (&sh)[3], int spack) {
+    int ci[3];
+    cell_coords(C, ni.cell, ci);
+    const int nc = (2 * C.reach[0] + 1) * (2 * C.reach[1] + 1) * (2 * C.reach[2] + 1);
+    // candidates of this wavefront's lanes: cheap test, keys of those that
+    // pass appended to the owner's key list
+    auto take = [&](bool valid, int a, int spk) {
         bool in = false;
         if (valid) {
+            int sh[3];
+            unpack_shift(spk, sh);
             const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
-            if (!(a < i || (a == i && zero))) {
-                double vn[3];
-                in = nbr_image(C, ni.u, an[a].u, sh, zero, vn);
-            }
+            if (!(a < i || (a == i && zero))) in = nbr_near(C, ni.u, an[a].u, sh);
         }
         const unsigned long long b = __ballot(in);
         if (b == 0) return;
@@ -369,105 +459,66 @@ __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__r
         base = __shfl(base, 0, 64);
         if (in) {
             const int idx = base + __popcll(b & below);
-            if (idx < CHX_MAX_NB) keys[idx] = ((long long)a << 24) | spack;
+            if (idx < CHX_MAX_NB) keys[idx] = ((long long)a << 24) | spk;
         }
     };
-    if (tid == 0) ncount = 0;
+    if (tid == 0) {
+        ncount = 0;
+        nexact = 0;
+    }
     if (nc <= CHX_NBR_CELLS && wv == 0) {
-        int carry = 0;
-        for (int c0 = 0; c0 < nc; c0 += 64) {
-            const int c = c0 + lane;
-            int cnt = 0, cs = 0, spk = 0;
-            if (c < nc) {
-                int cell, sh[3];
-                if (searched_cell(c % w0 - C.reach[0], (c / w0) % w1 - C.reach[1], c / (w0 * w1) - C.reach[2], cell,
-                                  sh)) {
-                    cs = cell_start[cell];
-                    cnt = cell_start[cell + 1] - cs;
-                    spk = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
-                }
-            }
-            int v = cnt;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int t = __shfl_up(v, d, 64);
-                if (lane >= d) v += t;
-            }
-            if (c < nc) {
-                cpre[c] = carry + v;
-                cbeg[c] = cs;
-                cshf[c] = spk;
-            }
-            carry += __shfl(v, 63, 64);
-        }
+        const int T = nbr_cells(C, ci, nc, lane, cell_start, cpre, cadj, cshf, ccell);
+        if (lane == 0) ncand = T;
     }
     __syncthreads();
     if (nc <= CHX_NBR_CELLS) {
-        const int T = cpre[nc - 1];
+        const int T = ncand;
         for (int t0 = wv * 64; t0 < T; t0 += 256) {
             const int t = t0 + lane;
-            int a = 0, spk = 0, sh[3] = {0, 0, 0};
-            if (t < T) {
-                int lo = 0, hi = nc - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (cpre[mid] > t) hi = mid; else lo = mid + 1;
-                }
-                const int before = lo ? cpre[lo - 1] : 0;
-                a = cell_atoms[cbeg[lo] + (t - before)];
-                spk = cshf[lo];
-                sh[0] = (spk & 255) - 128;
-                sh[1] = ((spk >> 8) & 255) - 128;
-                sh[2] = ((spk >> 16) & 255) - 128;
-            }
-            take(t < T, a, sh, spk);
+            int a = 0, spk = 0;
+            if (t < T) nbr_cand(t, T, nc, cpre, cadj, cshf, ccell, cell_atoms, a, spk);
+            take(t < T, a, spk);
         }
     } else if (wv == 0) {
         for (int dz = -C.reach[2]; dz <= C.reach[2]; dz++)
             for (int dy = -C.reach[1]; dy <= C.reach[1]; dy++)
                 for (int dx = -C.reach[0]; dx <= C.reach[0]; dx++) {
                     int cell, sh[3];
-                    if (!searched_cell(dx, dy, dz, cell, sh)) continue;
+                    if (!searched_cell(C, ci, dx, dy, dz, cell, sh)) continue;
                     const int cs = cell_start[cell], ce = cell_start[cell + 1];
-                    const int spack = (sh[0] + 128) | ((sh[1] + 128) << 8) | ((sh[2] + 128) << 16);
+                    const int spk = pack_shift(sh);
                     for (int p0 = cs; p0 < ce; p0 += 64) {
                         const int p = p0 + lane;
-                        take(p < ce, p < ce ? cell_atoms[p] : 0, sh, spack);
+                        take(p < ce, p < ce ? cell_atoms[p] : 0, spk);
                     }
                 }
     }
     __syncthreads();
-    const int n = ncount;
+    // the reference's test on the keys that passed; failures are marked
+    const int n1 = ncount;
+    if (n1 <= CHX_MAX_NB) {
+        for (int j0 = wv * 64; j0 < n1; j0 += 256) {
+            const int j = j0 + lane;
+            bool in = false;
+            if (j < n1) {
+                in = nbr_confirm(C, ni, an, keys[j]);
+                if (!in) keys[j] = NBR_DROPPED;
+            }
+            const unsigned long long b = __ballot(in);
+            if (lane == 0 && b) atomicAdd(&nexact, __popcll(b));
+        }
+    }
+    __syncthreads();
+    const int n = n1 <= CHX_MAX_NB ? nexact : n1;
     if (tid == 0) nbr_slab_note(i, n, nb_cap, nbr_cnt, nbr_off, status);
     if (n > CHX_MAX_NB || n > nb_cap) return;
     const long off = (long)i * nb_cap;
-    for (int j = tid; j < n; j += 256) {
+    for (int j = tid; j < n1; j += 256) {
         const long long k = keys[j];
+        if (k == NBR_DROPPED) continue;
         int rk = 0;
-        for (int q = 0; q < n; q++) rk += keys[q] < k;
-        const int a = (int)(k >> 24);
-        const int spack = (int)(k & 0xffffff);
-        const int sh[3] = {(spack & 255) - 128, ((spack >> 8) & 255) - 128, ((spack >> 16) & 255) - 128};
-        const bool zero = sh[0] == 0 && sh[1] == 0 && sh[2] == 0;
-        chx_entry_n f;
-        nbr_image(C, ni.u, an[a].u, sh, zero, f.v);
-        f.shift = spack;
-        f.pad = 0;
-        const chx_atom_s sa = as[a];
-        chx_entry_s e;
-        if (zero) {
-            e.v[0] = sa.u[0]; e.v[1] = sa.u[1]; e.v[2] = sa.u[2];
-        } else {
-            double t[3] = {sa.u[0], sa.u[1], sa.u[2]};
-            t[0] += sh[0]; t[1] += sh[1]; t[2] += sh[2];
-            double xg[3];
-            mv(C.h, t[0], t[1], t[2], xg);
-            mv(C.ih, xg[0], xg[1], xg[2], e.v);
-        }
-        e.parent = a;
-        e.type = sa.type;
-        es[off + rk] = e;
-        en[off + rk] = f;
+        for (int q = 0; q < n1; q++) rk += keys[q] < k;
+        nbr_write(C, ni, as, an, k, es + off + rk, en + off + rk);
     }
 }
 
