@@ -290,12 +290,12 @@ def c1_form(args, cpu=False, precision=None):
             tp.append(time.perf_counter() - t1)
         hip.sync()
         dt = time.perf_counter() - t0
-        vv, tune_ms, pcounts = (C.c_int * 2)(), (C.c_double * 6)(), (C.c_longlong * 3)()
+        vv, tune_ms, pcounts = (C.c_int * 2)(), (C.c_double * (2 * len(cc.VARIANTS)))(), (C.c_longlong * 3)()
         L.chimes_gpu_plan_info.argtypes = [C.c_void_p] * 4
         L.chimes_gpu_plan_info(plan, vv, tune_ms, pcounts)
         L.chimes_gpu_plan_free.argtypes = [C.c_void_p]
         L.chimes_gpu_plan_free(plan)
-    variants = ("mfma64", "mfma32", "vector")
+    variants = cc.VARIANTS
 
     # error of the measured result against the reference's own FP64 values
     E = hip.download(d_E, (1,), np.float64)[0]
@@ -453,7 +453,7 @@ def c2_plan(cc, L, hip, model, cfgs, prec, steps, warmup):
         run()
     hip.sync()
     dt = time.perf_counter() - t0
-    vv, tune_ms, counts = (C.c_int * 2)(), (C.c_double * 6)(), (C.c_longlong * 3)()
+    vv, tune_ms, counts = (C.c_int * 2)(), (C.c_double * (2 * len(cc.VARIANTS)))(), (C.c_longlong * 3)()
     L.chimes_gpu_plan_info(plan, vv, tune_ms, counts)
     L.chimes_gpu_plan_free(plan)
     E = hip.download(d_E, (nconf,), np.float64)

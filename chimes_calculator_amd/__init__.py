@@ -394,7 +394,7 @@ def eval_batch(models, configs, model_of=None, small=False, precision=FP64):
     return E, [F[offs[k]:offs[k + 1]] for k in range(nconf)], S.reshape(nconf, 9)
 
 
-VARIANTS = ("mfma64", "mfma32", "vector")
+VARIANTS = ("mfma64", "mfma32", "vector", "mfmav")
 
 
 def set_variants(var3=-1, var4=-1, device=0):
@@ -457,9 +457,9 @@ class GpuPlan:
     def info(self):
         """(3-body variant, 4-body variant, tuning ms per (body, variant) - negative
         for a variant the tuner rejected - , (pairs, triplets, quads))."""
-        v, t, c = np.zeros(2, np.int32), np.zeros(6), np.zeros(3, np.int64)
+        v, t, c = np.zeros(2, np.int32), np.zeros(2 * len(VARIANTS)), np.zeros(3, np.int64)
         self.L.chimes_gpu_plan_info(self.h, v, t, c)
-        return VARIANTS[v[0]], VARIANTS[v[1]], t.reshape(2, 3), tuple(int(x) for x in c)
+        return VARIANTS[v[0]], VARIANTS[v[1]], t.reshape(2, len(VARIANTS)), tuple(int(x) for x in c)
 
     def close(self):
         if getattr(self, "h", None):

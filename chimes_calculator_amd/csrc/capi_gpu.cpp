@@ -290,7 +290,7 @@ int chimes_gpu_plan_run_device(chimes_gpu_plan *P, const double *cells, double *
     return 0;
 }
 
-void chimes_gpu_plan_info(const chimes_gpu_plan *P, int variants[2], double tune_ms[6], long long counts[3])
+void chimes_gpu_plan_info(const chimes_gpu_plan *P, int variants[2], double tune_ms[8], long long counts[3])
 {
     chx_plan_info(P->p, variants, tune_ms, counts);
 }

@@ -191,64 +191,81 @@ __device__ __forceinline__ double row_sum16(double x)
     return x;
 }
 
-// Sum of x over the wavefront's 64 lanes (each lane's x, or 0), returned in
-// every lane: inclusive row sums by row shifts, then the four row totals.
-__device__ __forceinline__ long long fx_wave_sum(long long x)
+// A lane's value at DPP control CTRL (every source lane inside the row).
+template <int CTRL>
+__device__ __forceinline__ long long dpp_i64(long long x)
 {
-    x += row_shr_i64<1>(x);
-    x += row_shr_i64<2>(x);
-    x += row_shr_i64<4>(x);
-    x += row_shr_i64<8>(x);
-    return readlane_i64(x, 15) + readlane_i64(x, 31) + readlane_i64(x, 47) + readlane_i64(x, 63);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned long long)x, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)((unsigned long long)x >> 32), CTRL, 0xf, 0xf, false);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Wavefront totals of 16 integers per lane, transposed: lane l returns the
+// sum over all 64 lanes of x[l & 15].  Within each row of 16 lanes, four
+// exchange steps pair every lane with one whose bit 3, 2, 1, 0 differs and
+// whose higher bits agree (row rotation by 8, half-row mirror, two quad
+// permutations); at each step a lane keeps the half of its slots selected
+// by that bit, adds the partner's copy of them and sends the other half, so
+// 8 + 4 + 2 + 1 exchanges replace sixteen separate row sums.  The four rows
+// are then added by two lane exchanges.  Must be reached by all lanes.
+__device__ __forceinline__ long long fx_wave_sums16(long long (&x)[16])
+{
+    const int i = threadIdx.x & 15;
+#define CHX_T_STEP(N, BIT, CTRL)                                           \
+    {                                                                      \
+        const bool up = (i >> BIT) & 1;                                    \
+        _Pragma("unroll") for (int j = 0; j < N; j++)                      \
+        {                                                                  \
+            const long long keep = up ? x[j + N] : x[j];                   \
+            const long long send = up ? x[j] : x[j + N];                   \
+            x[j] = keep + dpp_i64<CTRL>(send);                             \
+        }                                                                  \
+    }
+    CHX_T_STEP(8, 3, 0x128)  // row_ror:8 (lane i ^ 8)
+    CHX_T_STEP(4, 2, 0x141)  // row_half_mirror (lane i ^ 7)
+    CHX_T_STEP(2, 1, 0x4e)   // quad_perm [2, 3, 0, 1] (lane i ^ 2)
+    CHX_T_STEP(1, 0, 0xb1)   // quad_perm [1, 0, 3, 2] (lane i ^ 1)
+#undef CHX_T_STEP
+    long long t = x[0];
+    t += __shfl_xor(t, 16, 64);
+    t += __shfl_xor(t, 32, 64);
+    return t;
 }
 
 // Adds the energy and virial terms q of each lane's cluster into the
 // accumulator of its configuration, copy `slot` (wave-uniform) of the
 // CHX_CACC_SLOTS: Cacc[(key * SLOTS + slot) * STRIDE + 2k + {0,1}] (key < 0:
 // nothing).  A wavefront whose clusters all belong to one configuration (the
-// common case: configurations hold thousands of clusters) sums each term
-// over its lanes and adds it once; otherwise the lanes' keys form runs
+// common case: configurations hold thousands of clusters) sums the fourteen
+// integer parts over its lanes at once (fx_wave_sums16) and its first
+// fourteen lanes add one each; otherwise the lanes' keys form runs
 // (clusters are stored by owner, owners by configuration) and each run adds
-// its sums.  ROWS: only lane 15 of each row of 16 carries a key (per-row
-// totals), so the wavefront's sum is the four row lanes' values.  Must be
-// reached by all lanes of the wavefront.
+// its sums.  Must be reached by all lanes of the wavefront.
 __device__ __forceinline__ unsigned long long *fx_cfg_line(unsigned long long *Cacc, int key, int slot)
 {
     return Cacc + ((long)key * CHX_CACC_SLOTS + slot) * CHX_CACC_STRIDE;
 }
 
-template <bool ROWS = false>
 __device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, int slot, const double (&q)[7],
                                             bool &bad)
 {
     const unsigned long long act = __ballot(key >= 0);
     if (act == 0) return;
     const int k0 = __builtin_amdgcn_readlane(key, __ffsll((long long)act) - 1);
+    const int l = threadIdx.x & 63;
     if (__ballot(key >= 0 && key != k0) == 0) {
-        const int l = threadIdx.x & 63;
+        long long x[16];
 #pragma unroll
         for (int k = 0; k < 7; k++) {
-            long long x[2] = {0, 0};
-            if (key >= 0) fx_split(q[k], x[0], x[1], bad);
-            long long h, lo;
-            if constexpr (ROWS) {
-                h = readlane_i64(x[0], 15) + readlane_i64(x[0], 31) + readlane_i64(x[0], 47) + readlane_i64(x[0], 63);
-                lo = readlane_i64(x[1], 15) + readlane_i64(x[1], 31) + readlane_i64(x[1], 47) + readlane_i64(x[1], 63);
-            } else {
-                h = fx_wave_sum(x[0]);
-                lo = fx_wave_sum(x[1]);
-            }
-            if (l == 0) {
-                unsigned long long *a = fx_cfg_line(Cacc, k0, slot) + 2 * k;
-                if (h) atomicAdd(a, (unsigned long long)h);
-                if (lo) atomicAdd(a + 1, (unsigned long long)lo);
-            }
-            __builtin_amdgcn_sched_barrier(0);
+            x[2 * k] = x[2 * k + 1] = 0;
+            if (key >= 0) fx_split(q[k], x[2 * k], x[2 * k + 1], bad);
         }
+        x[14] = x[15] = 0;
+        const long long t = fx_wave_sums16(x);
+        if (l < 14 && t) atomicAdd(fx_cfg_line(Cacc, k0, slot) + l, (unsigned long long)t);
         return;
     }
     const int seg = fx_run_first<64>(key);
-    const int l = threadIdx.x & 63;
     const int knext = __shfl_down(key, 1, 64);
     const bool last = key >= 0 && (l == 63 || knext != key);
 #pragma unroll

@@ -118,8 +118,8 @@ chx_plan *chx_plan_create(int device, chx_model *const *models, int nmodels, con
 int chx_plan_run(chx_plan *p, const double *pos, const int *types, const double *cells, chx_out *o);
 void chx_plan_destroy(chx_plan *p);
 const char *chx_plan_error(chx_plan *p);
-// variants[2]: chosen variant of the 3- and 4-body kernels; tune_ms[6]: the
-// timed kernel milliseconds per (body, variant), negated for a variant whose
+// variants[2]: chosen variant of the 3- and 4-body kernels; tune_ms[8]: the
+// timed kernel milliseconds per (body, variant) as [2][CHX_NVAR], negated for a variant whose
 // results disagreed with the reference evaluation; counts[3]: as
 // chx_last_counts.
 void chx_plan_info(chx_plan *p, int *variants, double *tune_ms, long long *counts);

@@ -43,6 +43,9 @@ void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t 
 // (FP32 for the reduced precisions) stands in where the matrix-core kernels
 // do not take D.
 void launch_eval_body(int nb, int D, int prec, int variant, const chx_list_args &la, hipStream_t st);
+// Whether variant has a kernel of its own for nb bodies at precision prec
+// (the one-cluster-per-lane matrix-core variant: 3-body FP32-mixed only).
+bool variant_applies(int nb, int prec, int variant);
 void launch_clusters(int nb, int D, int n, hipStream_t st, const double *dist, const double *drv, const int *types,
                      const chx_model_dev &M, double *E, double *F, double *S, double *last, int *flags);
 

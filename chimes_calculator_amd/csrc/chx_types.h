@@ -15,6 +15,7 @@
 #define CHX_LIST_ALIGN 64      // 3-/4-body list segments of one cluster type start on this multiple
 #define CHX_MFMA_MAX_D 4       // 4-body dense dimension handled by the matrix-core kernel
 #define CHX_MFMA3_MAX_D 13     // 3-body dense dimension handled by the matrix-core kernel
+#define CHX_MFMA3V_MAX_D 10    // 3-body dense dimension of the one-cluster-per-lane matrix-core kernel (FP32)
 #define CHX_STATIC_MAX_CLUSTERS 131072  // clusters per call of the allocation-free explicit-cluster entry
 // Per-configuration energy and virial accumulators: CHX_CACC_SLOTS copies per
 // configuration, one 128-byte line each (7 (hi, lo) pairs), picked by
@@ -57,8 +58,10 @@ struct chx_clear_range {
 
 // 3-/4-body evaluation kernel variants (chosen per body order by the plan
 // autotuner): matrix cores with 4 or 2 wavefronts per workgroup (64 or 32
-// clusters per tile), or the vector units.
-enum { CHX_VAR_MFMA4 = 0, CHX_VAR_MFMA2 = 1, CHX_VAR_VECTOR = 2, CHX_NVAR = 3 };
+// clusters per tile), the vector units, or (3-body, FP32-mixed) one cluster
+// per lane with the contraction on the matrix cores; where a variant does
+// not apply the vector kernel stands in.
+enum { CHX_VAR_MFMA4 = 0, CHX_VAR_MFMA2 = 1, CHX_VAR_VECTOR = 2, CHX_VAR_MFMAV = 3, CHX_NVAR = 4 };
 #define CHX_SCAN1_MAX 32768    // scans up to this length run in one workgroup
 
 // Per-configuration frame data (one per configuration in a batch).

@@ -1291,7 +1291,7 @@ struct chx_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     long sig[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // N, Nin, ncell, nb_cap, l3_cap, l4_cap, nitems, rcap of the captured graph
-    double tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double tune_ms[2][CHX_NVAR] = {};  // per (body, variant); 0 where a variant was not timed
     std::string err;
     ~chx_plan()
     {
@@ -1449,6 +1449,7 @@ void plan_tune(chx_plan *p, chx_out *o)
         // the BF16 forms are defined by their matrix-core contraction: no vector variant
         const int nvar = e->prec >= CHX_BF16 ? CHX_VAR_VECTOR : CHX_NVAR;
         for (int var = 0; var < nvar; var++) {
+            if (!variant_applies(body, e->prec, var)) continue;
             float best_rep = 1e30f;
             for (int rep = 0; rep < 3; rep++) {
                 HIPCHK(hipEventRecord(a, e->st));
@@ -1484,7 +1485,8 @@ void plan_tune(chx_plan *p, chx_out *o)
         const int def = var;
         const int nvar = prec >= CHX_BF16 ? CHX_VAR_VECTOR : CHX_NVAR;
         std::vector<int> order;
-        for (int v = 0; v < nvar; v++) order.push_back(v);
+        for (int v = 0; v < nvar; v++)
+            if (variant_applies(body, prec, v)) order.push_back(v);
         std::stable_sort(order.begin(), order.end(),
                          [&](int x, int y) { return p->tune_ms[body - 3][x] < p->tune_ms[body - 3][y]; });
         int chosen = -1;
@@ -1608,7 +1610,7 @@ void chx_plan_info(chx_plan *p, int *variants, double *tune_ms, long long *count
     variants[0] = p->e->var3;
     variants[1] = p->e->var4;
     for (int b = 0; b < 2; b++)
-        for (int v = 0; v < 3; v++) tune_ms[3 * b + v] = p->tune_ms[b][v];
+        for (int v = 0; v < CHX_NVAR; v++) tune_ms[CHX_NVAR * b + v] = p->tune_ms[b][v];
     for (int k = 0; k < 3; k++) counts[k] = p->e->counts[k];
 }
 

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
     double q7[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) q7[k] = acc[3 + k];
-    fx_cfg_sums<true>(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
+    fx_cfg_sums(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
     __syncthreads();
     tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -1104,6 +1104,207 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
 }
 
 // ---------------------------------------------------------------------------
+// 3-body with one cluster per lane, as k_evalv, and the contraction over the
+// first pair's index on the matrix cores (FP32 forms, D <= CHX_MFMA3V_MAX_D).
+// Each lane stages its cluster's polynomials in LDS; a wavefront's 64
+// clusters are then contracted as four 16-cluster MFMA tiles:
+//   Z_p[(b, c)][n] = sum_a C[a][b][c] X_p[a][n],  X_0 = T_01, X_1 = dT_01,
+// rows (b, c) = b CP + c (c padded to CP, a multiple of 4), the coefficient
+// operands held in registers for the whole wavefront.  Output lane l holds
+// rows 16 rb + 4 (l / 16) + v, v = 0..3: one b and four consecutive c, so
+//   E   += T_02[b] sum_v Z_0 T_12[c],   G_02 += dT_02[b] sum_v Z_0 T_12[c],
+//   G_12 += T_02[b] sum_v Z_0 dT_12[c], G_01 += T_02[b] sum_v Z_1 T_12[c]
+// (the four-term sums in FP32, the rest in FP64), and the four lanes of a
+// cluster column are added by two exchanges.  Every cluster's sums follow
+// the same fixed order wherever it sits in a tile, so results keep the
+// batch-independence of the other kernels.  Scalars, forces and
+// accumulation are those of k_evalv.
+// ---------------------------------------------------------------------------
+template <int D>
+struct mfma3v_shape {
+    static constexpr int CP = (D + 3) / 4 * 4;       // padded c (and a) range
+    static constexpr int RB = (D * CP + 15) / 16;    // 16-row blocks over (b, c)
+    static constexpr int KS = CP / 4;                // k steps of 4 over a
+    static constexpr int STR = 6 * CP + 1;           // per-cluster LDS stride (odd: conflict-free columns)
+};
+
+template <int D>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eval3v(
+    int G, const int *__restrict__ gbase, const int *__restrict__ gend, const chx_group *__restrict__ groups,
+    int total_word, const int2 *__restrict__ list, const chx_cfg *__restrict__ cfgs,
+    const chx_atom_s *__restrict__ as, int nb_cap, const chx_entry_s *__restrict__ es, unsigned long long *Facc,
+    unsigned long long *Cacc, int *cfg_stat, int *status)
+{
+    using SH = mfma3v_shape<D>;
+    constexpr int NB = 3, P = 3, CP = SH::CP, STR = SH::STR;
+    // the staged polynomials (T_01, dT_01, T_02, dT_02, T_12, dT_12, each
+    // padded with zeros to CP) and, after the contraction, the force table
+    union shared_t {
+        float T[4][64 * STR];
+        fx_table<512> tab;
+    };
+    __shared__ shared_t sh;
+    __shared__ int flags_s;
+    if (threadIdx.x == 0) flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
+    __syncthreads();
+    if (flags_s & CHX_STOP) return;
+    const int lane = threadIdx.x & 63, col = lane & 15, r4 = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const long t0 = ((long)blockIdx.x * 4 + wave) * 64;
+    bool wave_on = t0 < status[total_word];
+    const int g = wave_on ? find_group(gbase, G, t0) : 0;
+    const chx_group &grp = groups[g];
+    wave_on = wave_on && grp.type >= 0 && grp.D == D;
+    const long c = t0 + lane;
+    float *Tw = sh.T[wave];
+    // ---- geometry and polynomials, one cluster per lane
+    bool act = false;
+    int2 rec = make_int2(-1, 0);
+    double p[NB][3], r[P], f[P], fd[P];
+    int par[NB], cfg = 0, pm[NB], flags = 0;
+    {
+        float T[P][D], dT[P][D];
+#pragma unroll
+        for (int q = 0; q < P; q++)
+#pragma unroll
+            for (int x = 0; x < D; x++) { T[q][x] = 0.f; dT[q][x] = 0.f; }
+        if (wave_on && c < gend[g]) {
+            rec = list[c];
+            act = cluster_geometry<NB>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, pm);
+            if (act)
+#pragma unroll
+                for (int q = 0; q < P; q++) {
+                    cheby<D, float>(r[q], grp.s[q], T[q], dT[q], flags);
+                    fcut(r[q], grp.s[q], grp.tersoff, grp.fcut_var, f[q], fd[q]);
+                }
+        }
+        float *Tl = Tw + lane * STR;
+#pragma unroll
+        for (int q = 0; q < P; q++)
+#pragma unroll
+            for (int x = 0; x < CP; x++) {
+                Tl[(2 * q) * CP + x] = x < D ? T[q][x] : 0.f;
+                Tl[(2 * q + 1) * CP + x] = x < D ? dT[q][x] : 0.f;
+            }
+    }
+    __syncthreads();
+    // ---- contraction on the matrix cores (whole wavefronts)
+    double S = 0.0, Gq[P] = {0.0, 0.0, 0.0};
+    if (wave_on) {
+        // coefficient operands: A[row][a] = C[a][b][c], row = 16 rb + col = b CP + c, a = 4 s + r4
+        const float *Cf = grp.Cf;
+        float A[SH::RB][SH::KS];
+#pragma unroll
+        for (int rb = 0; rb < SH::RB; rb++)
+#pragma unroll
+            for (int s = 0; s < SH::KS; s++) {
+                const int row = 16 * rb + col, a = 4 * s + r4;
+                const int b = row / CP, cc = row % CP;
+                A[rb][s] = (a < D && b < D && cc < D) ? Cf[(a * D + b) * D + cc] : 0.f;
+            }
+        // per sub-tile q: energy and the three gradients of the column's
+        // cluster, partial over this lane's rows
+        double part[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float *Tn = Tw + (16 * q + col) * STR;  // the cluster of this lane's column
+            float x0[SH::KS], x1[SH::KS];
+#pragma unroll
+            for (int s = 0; s < SH::KS; s++) {
+                x0[s] = Tn[4 * s + r4];
+                x1[s] = Tn[CP + 4 * s + r4];
+            }
+            double e = 0.0, g01 = 0.0, g02 = 0.0, g12 = 0.0;
+#pragma unroll
+            for (int rb = 0; rb < SH::RB; rb++) {
+                flt4 z0 = {0.f, 0.f, 0.f, 0.f}, z1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < SH::KS; s++) {
+                    z0 = __builtin_amdgcn_mfma_f32_16x16x4f32(A[rb][s], x0[s], z0, 0, 0, 0);
+                    z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(A[rb][s], x1[s], z1, 0, 0, 0);
+                }
+                // rows 16 rb + 4 r4 + v: b = row0 / CP, c = row0 % CP + v
+                const int row0 = 16 * rb + 4 * r4, b = row0 / CP, c0 = row0 % CP;
+                if (b < D) {
+                    float u0 = z0[0] * Tn[4 * CP + c0], u1 = z0[0] * Tn[5 * CP + c0], w = z1[0] * Tn[4 * CP + c0];
+#pragma unroll
+                    for (int v = 1; v < 4; v++) {
+                        const float t12 = Tn[4 * CP + c0 + v], d12 = Tn[5 * CP + c0 + v];
+                        u0 = fmaf(z0[v], t12, u0);
+                        u1 = fmaf(z0[v], d12, u1);
+                        w = fmaf(z1[v], t12, w);
+                    }
+                    const double t02 = Tn[2 * CP + b], d02 = Tn[3 * CP + b];
+                    e = fma((double)u0, t02, e);
+                    g02 = fma((double)u0, d02, g02);
+                    g12 = fma((double)u1, t02, g12);
+                    g01 = fma((double)w, t02, g01);
+                }
+            }
+            part[q][0] = e;
+            part[q][1] = g01;
+            part[q][2] = g02;
+            part[q][3] = g12;
+        }
+        // the four lanes of each column added, sub-tile q ending on the lane
+        // with r4 = q: exchange across lane halves (r4 ^ 2) keeping the
+        // sub-tiles of this lane's half, then across rows (r4 ^ 1)
+        const bool hi2 = (r4 & 2) != 0, hi1 = (r4 & 1) != 0;
+        double h[2][4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const double keep = hi2 ? part[2 + j][k] : part[j][k];
+                const double send = hi2 ? part[j][k] : part[2 + j][k];
+                h[j][k] = keep + __shfl_xor(send, 32, 64);
+            }
+            const double keep = hi1 ? h[1][k] : h[0][k];
+            const double send = hi1 ? h[0][k] : h[1][k];
+            const double t = keep + __shfl_xor(send, 16, 64);
+            if (k == 0) S = t;
+            else Gq[k - 1] = t;
+        }
+    }
+    __syncthreads();
+    sh.tab.clear();
+    __syncthreads();
+    // ---- scalars, forces and accumulation (as k_evalv)
+    int owner = -1, ckey = -1;
+    double q7[7] = {0, 0, 0, 0, 0, 0, 0};
+    double Fr[NB][3];
+    int key[NB];
+#pragma unroll
+    for (int m = 0; m < NB; m++) { key[m] = -1; Fr[m][0] = Fr[m][1] = Fr[m][2] = 0.0; }
+    bool bad = false;
+    if (act) {
+        double E, fs[P];
+        cluster_scalars<P, double>(r, f, fd, S, Gq, E, fs);
+        q7[0] = E;
+        double F[NB][3];
+        cluster_forces<NB>(cfgs[cfg].h, p, fs, F, q7);
+#pragma unroll
+        for (int a = 0; a < NB; a++)
+#pragma unroll
+            for (int m = 0; m < NB; m++)
+                if (pm[a] == m) {
+                    key[m] = par[a];
+                    Fr[m][0] = F[a][0]; Fr[m][1] = F[a][1]; Fr[m][2] = F[a][2];
+                }
+        owner = rec.x;
+        ckey = cfg;
+        note_flags(cfg_stat, cfg, flags);
+    }
+#pragma unroll
+    for (int m = 1; m < NB; m++) sh.tab.add(key[m], Fr[m], bad);
+    fx_owner_force_runs(Facc, owner, Fr[0], bad);
+    fx_cfg_sums(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
+    __syncthreads();
+    sh.tab.flush(Facc);
+    if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
+}
+
+// ---------------------------------------------------------------------------
 // Totals, one launch.  Blocks [0, nitems): energy and virial, one chunk of
 // CHX_FIN_CHUNK owners of one configuration each (item; cfg_item_off[c] is
 // the first item of c), exact partial sums normalised to part[item][7][2] =
@@ -1275,6 +1476,7 @@ __global__ __launch_bounds__(128) void k_clusters(int n, const double *__restric
 // Host-side launchers: dispatch on the dense dimension D and precision.
 // ---------------------------------------------------------------------------
 #define CHX_D4M_CASES(X) X(2) X(3) X(4)
+#define CHX_D3V_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 
 bool supported_D3(int D) { return D >= 2 && D <= 13; }
 bool supported_D4(int D) { return D >= 2 && D <= 6; }
@@ -1373,10 +1575,33 @@ bool launch_eval4m(int D, int prec, const chx_list_args &la, hipStream_t st)
     return false;
 }
 
+bool launch_eval3v(int D, const chx_list_args &la, hipStream_t st)
+{
+    if (D > CHX_MFMA3V_MAX_D) return false;
+    if (la.cap <= 0 || la.G <= 0) return true;
+    const dim3 grid((unsigned)((la.cap + 255) / 256)), block(256);
+#define X(d)                                                                                                   \
+    if (D == d) {                                                                                              \
+        hipLaunchKernelGGL((k_eval3v<d>), grid, block, 0, st, la.G, la.gbase, la.gend, la.groups, CHX_ST_N3,     \
+                           la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Cacc, la.cfg_stat, la.status); \
+        return true;                                                                                           \
+    }
+    CHX_D3V_CASES(X)
+#undef X
+    return false;
+}
+
+bool variant_applies(int nb, int prec, int variant)
+{
+    return variant != CHX_VAR_MFMAV || (nb == 3 && prec == CHX_MIXED);
+}
+
 void launch_eval_body(int nb, int D, int prec, int variant, const chx_list_args &la, hipStream_t st)
 {
     bool done = false;
-    if (variant == CHX_VAR_MFMA4 || (variant == CHX_VAR_VECTOR && prec >= CHX_BF16))
+    if (variant == CHX_VAR_MFMAV && nb == 3 && prec == CHX_MIXED)
+        done = launch_eval3v(D, la, st);
+    else if (variant == CHX_VAR_MFMA4 || (variant >= CHX_VAR_VECTOR && prec >= CHX_BF16))
         done = nb == 3 ? launch_eval3m<4>(D, prec, la, st) : launch_eval4m<4>(D, prec, la, st);
     else if (variant == CHX_VAR_MFMA2)
         done = nb == 3 ? launch_eval3m<2>(D, prec, la, st) : launch_eval4m<2>(D, prec, la, st);

@@ -20,13 +20,46 @@ __device__ __forceinline__ void mv(const double *m, double x, double y, double z
     o[2] = m[6] * x + m[7] * y + m[8] * z;
 }
 
-// Length of h * (b - a), the reference's get_dist on fractional coordinates.
-__device__ __forceinline__ double frac_dist(const double *h, const double *a, const double *b)
+// Squared length of h * (b - a); its square root is the reference's get_dist
+// on fractional coordinates.
+__device__ __forceinline__ double frac_dist2(const double *h, const double *a, const double *b)
 {
     const double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
     double r[3];
     mv(h, dx, dy, dz, r);
-    return sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    return r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+}
+
+// The reference's test get_dist < c from the squared length s: decided by s
+// wherever that is clear and by the square root only within a few ulps of
+// c * c.  The square root is correctly rounded, so both give the same answer
+// everywhere: a square below lo = c*c (1 - 2^-49) has a rounded root below c,
+// one at or above hi = c*c (1 + 2^-49) a rounded root above it.
+struct dist_cut {
+    double c, lo, hi;
+};
+
+__device__ __forceinline__ dist_cut make_cut(double c)
+{
+    const double c2 = c * c;
+    return {c, c2 * (1.0 - 0x1p-49), c2 * (1.0 + 0x1p-49)};
+}
+
+__device__ __forceinline__ bool within(double s, const dist_cut &k)
+{
+    if (s < k.lo) return true;
+    if (s >= k.hi) return false;
+    return sqrt(s) < k.c;
+}
+
+// Unordered pair p of n items as (u, v), u < v, v the larger index: v from a
+// single-precision root (within one of the answer) and one correction each way.
+__device__ __forceinline__ void pair_of(int p, int &u, int &v)
+{
+    v = (int)((1.0f + __builtin_amdgcn_sqrtf(1.0f + 8.0f * (float)p)) * 0.5f);
+    while (v * (v - 1) / 2 > p) v--;
+    while ((v + 1) * v / 2 <= p) v++;
+    u = p - v * (v - 1) / 2;
 }
 
 __device__ __forceinline__ int find_cfg(const int *off, int n, int r)
@@ -127,7 +160,7 @@ __device__ __forceinline__ bool nbr_image(const chx_cfg &C, const double *ui, co
         mv(C.hn, t[0], t[1], t[2], xg);
         mv(C.ihn, xg[0], xg[1], xg[2], vn);
     }
-    return frac_dist(C.hn, ui, vn) < C.search;
+    return within(frac_dist2(C.hn, ui, vn), make_cut(C.search));
 }
 
 // The cheap first test of the same membership: the image's fractional offset
@@ -619,19 +652,21 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     const bool any3 = M.has3 && rc3 > 0.0, any4 = M.has4 && rc4 > 0.0;
     const double rcS = rc3 > rc4 ? rc3 : rc4;  // many-body candidate radius
     if (!any3 && !any4) return;
+    const dist_cut cutS = make_cut(rcS), cut3 = make_cut(rc3), cut4 = make_cut(rc4);
     if (lane == 0) { sh.n3 = 0; sh.n4 = 0; sh.nrec = 0; }
     __syncthreads();
     // compact the images within rc3 / rc4, in sorted order (positions: the
     // running count, the counts of the lower wavefronts, the lower lanes)
     for (int base = 0; base < nb; base += NT) {
         const int j = base + lane;
-        double d = 1e300;
+        bool in3 = false, in4 = false;
         chx_entry_n e;
         if (j < nb) {
             e = x.en[off + j];
-            d = frac_dist(C.hn, ni.u, e.v);
+            const double d2 = frac_dist2(C.hn, ni.u, e.v);
+            in3 = within(d2, cutS);
+            in4 = within(d2, cut4);
         }
-        const bool in3 = j < nb && d < rcS, in4 = j < nb && d < rc4;
         const unsigned long long b3 = __ballot(in3), b4 = __ballot(in4);
         const unsigned long long below = wl ? (~0ull >> (64 - wl)) : 0ull;
         int n3b = sh.n3, n4b = sh.n4, t3 = __popcll(b3), t4 = __popcll(b4);
@@ -680,11 +715,9 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     if (any4) {
         const int npair = n4 * (n4 - 1) / 2;
         for (int p = lane; p < npair; p += NT) {
-            int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
-            while (v * (v - 1) / 2 > p) v--;
-            while ((v + 1) * v / 2 <= p) v++;
-            const int u = p - v * (v - 1) / 2;
-            if (frac_dist(C.hn, sh.vn3[sh.idx4[u]], sh.vn3[sh.idx4[v]]) < rc4) {
+            int u, v;
+            pair_of(p, u, v);
+            if (within(frac_dist2(C.hn, sh.vn3[sh.idx4[u]], sh.vn3[sh.idx4[v]]), cut4)) {
                 atomicOr(&sh.A4[u][v >> 5], 1u << (v & 31));
                 atomicOr(&sh.A4[v][u >> 5], 1u << (u & 31));
             }
@@ -709,11 +742,9 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         // (parent(j) <= parent(k) in both orders); the distance is symmetric
         const int npair = n3 * (n3 - 1) / 2;
         for (int p = lane; p < npair; p += NT) {
-            int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
-            while (v * (v - 1) / 2 > p) v--;
-            while ((v + 1) * v / 2 <= p) v++;
-            const int u = p - v * (v - 1) / 2;
-            if (!(frac_dist(C.hn, sh.vn3[u], sh.vn3[v]) < rc3)) continue;
+            int u, v;
+            pair_of(p, u, v);
+            if (!within(frac_dist2(C.hn, sh.vn3[u], sh.vn3[v]), cut3)) continue;
             const int nrec = sh.par3[u] == sh.par3[v] ? 2 : 1;
             for (int q = 0; q < nrec; q++) {
                 const int a = q ? v : u, b = q ? u : v;
@@ -741,10 +772,8 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         // (v, u); then l over the common neighbours with parent(k) <= parent(l)
         const int npair = n4 * (n4 - 1) / 2;
         for (int p = lane; p < npair; p += NT) {
-            int v = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
-            while (v * (v - 1) / 2 > p) v--;
-            while ((v + 1) * v / 2 <= p) v++;
-            const int u = p - v * (v - 1) / 2;
+            int u, v;
+            pair_of(p, u, v);
             if (!((sh.A4[u][v >> 5] >> (v & 31)) & 1u)) continue;
             const int nrec = sh.par3[sh.idx4[u]] == sh.par3[sh.idx4[v]] ? 2 : 1;
             for (int q = 0; q < nrec; q++) {

@@ -86,13 +86,16 @@ chimes_gpu_plan *chimes_gpu_plan_create_device(chimes_gpu_model *const *models, 
 int chimes_gpu_plan_run_device(chimes_gpu_plan *p, const double *cells, double *energy_dev, double *forces_dev,
                                double *virial_dev);
 /* variants[2]: kernel variant of the 3- and 4-body terms (0 matrix cores,
- * 64-cluster tiles; 1 matrix cores, 32-cluster tiles; 2 vector units);
- * tune_ms[6]: timed milliseconds per (body, variant), 0 if not tuned, negated
+ * 64-cluster tiles; 1 matrix cores, 32-cluster tiles; 2 vector units; 3 one
+ * cluster per lane with the contraction on the matrix cores, 3-body
+ * FP32-mixed only, D <= 10);
+ * tune_ms[8]: timed milliseconds per (body, variant), tune_ms[4 (body - 3) +
+ * variant], 0 if not tuned or not applicable, negated
  * for a variant the tuner rejected: the tuner keeps the fastest variant whose
  * results agree with a reference evaluation of the batch (the default
  * variants, in FP64 for the FP32-mixed form) within the precision's bars;
  * counts[3]: pair images, triplets, quadruplets of the last run. */
-void chimes_gpu_plan_info(const chimes_gpu_plan *p, int variants[2], double tune_ms[6], long long counts[3]);
+void chimes_gpu_plan_info(const chimes_gpu_plan *p, int variants[2], double tune_ms[8], long long counts[3]);
 void chimes_gpu_plan_free(chimes_gpu_plan *p);
 
 /* Kernel variants of the 3- and 4-body terms (numbered as in
@@ -103,8 +106,9 @@ void chimes_gpu_plan_free(chimes_gpu_plan *p);
  * 64-cluster tiles) for the 4-body term.  chimes_gpu_plan_set_variants sets
  * them for a plan, whose next run evaluates directly and captures again.  A
  * matrix-core variant for a dense dimension above the matrix-core kernels'
- * range (4-body D > 4) runs on the vector units, and a vector-unit request of
- * a BF16 form on the matrix cores.  Nonzero for an unknown variant. */
+ * range (4-body D > 4) runs on the vector units, and so does variant 3 where
+ * it does not apply (4-body, FP64, 3-body D > 10); a vector-unit request of a
+ * BF16 form runs on the matrix cores.  Nonzero for an unknown variant. */
 int chimes_gpu_set_variants(int device, int var3, int var4);
 int chimes_gpu_get_variants(int device, int variants[2]);
 int chimes_gpu_plan_set_variants(chimes_gpu_plan *p, int var3, int var4);

@@ -1,9 +1,11 @@
 """Every 3-/4-body kernel variant the autotuner can choose, pinned to the
 reference's goldens.
 
-The variants are the matrix-core kernels with 64- and 32-cluster tiles and
-the vector-unit kernel (the 4-body matrix-core kernels cover D <= 4; above
-that the request runs on the vector units).  Each is forced in turn through
+The variants are the matrix-core kernels with 64- and 32-cluster tiles, the
+vector-unit kernel, and the one-cluster-per-lane kernel with the 3-body
+contraction on the matrix cores (FP32-mixed, D <= 10; the 4-body matrix-core
+kernels cover D <= 4; outside those ranges the request runs on the vector
+units).  Each is forced in turn through
 chimes_gpu_set_variants and evaluated in FP64 and FP32-mixed on:
   * the dense-dimension scan (tests/golden/dscan_cases.json): two-type C/O
     models covering every instantiated dense dimension, 3-body D = 2..13
@@ -28,7 +30,7 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
 FATOL = 1e-9
-VARIANTS = (0, 1, 2)
+VARIANTS = (0, 1, 2, 3)
 
 
 @pytest.fixture(scope="module")
@@ -104,6 +106,6 @@ def test_workload_goldens(cc, model_dir, synth_cases, precision):
 
 def test_set_variants_rejects_unknown(cc):
     with pytest.raises(cc.ChimesError):
-        cc.set_variants(3, 0)
+        cc.set_variants(4, 0)
     with pytest.raises(cc.ChimesError):
         cc.set_variants(0, -2)
