@@ -148,39 +148,81 @@ __device__ __forceinline__ void contract3(const R *__restrict__ C, const R (&T)[
                                           A &S, A (&G)[3])
 {
     A s0 = 0, g00 = 0, g01 = 0, g02 = 0;
-    // T[0] and dT[0] are consumed from the front of rotating copies: the a0
-    // loop is not unrolled, and indexing by a0 would place them in scratch
-    R t0v[D], d0v[D];
+    if constexpr (D <= 7) {
+        // unrolled: every T[0] factor is a register of its own, the second
+        // pair's factors are converted once, and the a0 = 0 level is a copy
+        A t1[D], d1[D];
 #pragma unroll
-    for (int u = 0; u < D; u++) { t0v[u] = T[0][u]; d0v[u] = dT[0][u]; }
-#pragma unroll 1
-    for (int a0 = 0; a0 < D; a0++) {
-        A s1 = 0, g11 = 0, g12 = 0;
+        for (int u = 0; u < D; u++) { t1[u] = (A)T[1][u]; d1[u] = (A)dT[1][u]; }
+        auto level = [&](int a0, A t0, A dt0, bool first) {
+            A s1 = 0, g11 = 0, g12 = 0;
 #pragma unroll
-        for (int a1 = 0; a1 < D; a1++) {
-            const_ptr<R> c = (const_ptr<R>)C + (a0 * D + a1) * D;
-            R z = c[0], z2 = 0;
+            for (int a1 = 0; a1 < D; a1++) {
+                const_ptr<R> c = (const_ptr<R>)C + (a0 * D + a1) * D;
+                R z = c[0], z2 = 0;
 #pragma unroll
-            for (int a2 = 1; a2 < D; a2++) {
-                z = fma(c[a2], T[2][a2], z);
-                z2 = fma(c[a2], dT[2][a2], z2);
+                for (int a2 = 1; a2 < D; a2++) {
+                    z = fma(c[a2], T[2][a2], z);
+                    z2 = fma(c[a2], dT[2][a2], z2);
+                }
+                if (a1 == 0) {
+                    s1 = (A)z;
+                    g12 = (A)z2;
+                } else {
+                    s1 = fma(t1[a1], (A)z, s1);
+                    g11 = fma(d1[a1], (A)z, g11);
+                    g12 = fma(t1[a1], (A)z2, g12);
+                }
             }
-            if (a1 == 0) {
-                s1 = (A)z;
-                g12 = (A)z2;
+            if (first) {
+                s0 = s1;
+                g01 = g11;
+                g02 = g12;
             } else {
-                s1 = fma((A)T[1][a1], (A)z, s1);
-                g11 = fma((A)dT[1][a1], (A)z, g11);
-                g12 = fma((A)T[1][a1], (A)z2, g12);
+                s0 = fma(t0, s1, s0);
+                g00 = fma(dt0, s1, g00);
+                g01 = fma(t0, g11, g01);
+                g02 = fma(t0, g12, g02);
             }
-        }
-        const A t0 = (A)t0v[0], dt0 = (A)d0v[0];
-        s0 = fma(t0, s1, s0);
-        g00 = fma(dt0, s1, g00);
-        g01 = fma(t0, g11, g01);
-        g02 = fma(t0, g12, g02);
+        };
 #pragma unroll
-        for (int u = 0; u + 1 < D; u++) { t0v[u] = t0v[u + 1]; d0v[u] = d0v[u + 1]; }
+        for (int a0 = 0; a0 < D; a0++) level(a0, (A)T[0][a0], (A)dT[0][a0], a0 == 0);
+    } else {
+        // larger D: a loop over a0 (code size and registers), T[0] and dT[0]
+        // consumed from the front of rotating copies, since indexing them by
+        // a0 would place them in scratch
+        R t0v[D], d0v[D];
+#pragma unroll
+        for (int u = 0; u < D; u++) { t0v[u] = T[0][u]; d0v[u] = dT[0][u]; }
+#pragma unroll 1
+        for (int a0 = 0; a0 < D; a0++) {
+            A s1 = 0, g11 = 0, g12 = 0;
+#pragma unroll
+            for (int a1 = 0; a1 < D; a1++) {
+                const_ptr<R> c = (const_ptr<R>)C + (a0 * D + a1) * D;
+                R z = c[0], z2 = 0;
+#pragma unroll
+                for (int a2 = 1; a2 < D; a2++) {
+                    z = fma(c[a2], T[2][a2], z);
+                    z2 = fma(c[a2], dT[2][a2], z2);
+                }
+                if (a1 == 0) {
+                    s1 = (A)z;
+                    g12 = (A)z2;
+                } else {
+                    s1 = fma((A)T[1][a1], (A)z, s1);
+                    g11 = fma((A)dT[1][a1], (A)z, g11);
+                    g12 = fma((A)T[1][a1], (A)z2, g12);
+                }
+            }
+            const A t0 = (A)t0v[0], dt0 = (A)d0v[0];
+            s0 = fma(t0, s1, s0);
+            g00 = fma(dt0, s1, g00);
+            g01 = fma(t0, g11, g01);
+            g02 = fma(t0, g12, g02);
+#pragma unroll
+            for (int u = 0; u + 1 < D; u++) { t0v[u] = t0v[u + 1]; d0v[u] = d0v[u + 1]; }
+        }
     }
     S = s0;
     G[0] = g00; G[1] = g01; G[2] = g02;
@@ -194,7 +236,87 @@ __device__ __forceinline__ void contract4(const R *__restrict__ C, const R (&T)[
                                           A &S, A (&G)[6])
 {
     A s0 = 0, h0[6] = {0, 0, 0, 0, 0, 0};
-    // rotating copies of T[0], dT[0], T[1], dT[1], as in contract3
+    // the (a2, a3, a4, a5) sums of one (a0, a1): s2 and the gradients of
+    // pairs 2..5 (h2)
+    auto inner = [&](int a0, int a1, R &s2, R (&h2)[4]) {
+        s2 = 0;
+        h2[0] = h2[1] = h2[2] = h2[3] = 0;
+#pragma unroll
+        for (int a2 = 0; a2 < D; a2++) {
+            R s3 = 0, h3[3] = {0, 0, 0};
+#pragma unroll
+            for (int a3 = 0; a3 < D; a3++) {
+                R s4 = 0, h4[2] = {0, 0};
+#pragma unroll
+                for (int a4 = 0; a4 < D; a4++) {
+                    const_ptr<R> c = (const_ptr<R>)C + ((((a0 * D + a1) * D + a2) * D + a3) * D + a4) * D;
+                    R z = c[0], z5 = 0;
+#pragma unroll
+                    for (int a5 = 1; a5 < D; a5++) {
+                        z = fma(c[a5], T[5][a5], z);
+                        z5 = fma(c[a5], dT[5][a5], z5);
+                    }
+                    if (a4 == 0) {
+                        s4 = z;
+                        h4[1] = z5;
+                    } else {
+                        s4 = fma(T[4][a4], z, s4);
+                        h4[0] = fma(dT[4][a4], z, h4[0]);
+                        h4[1] = fma(T[4][a4], z5, h4[1]);
+                    }
+                }
+                if (a3 == 0) {
+                    s3 = s4;
+                    h3[1] = h4[0];
+                    h3[2] = h4[1];
+                } else {
+                    s3 = fma(T[3][a3], s4, s3);
+                    h3[0] = fma(dT[3][a3], s4, h3[0]);
+                    h3[1] = fma(T[3][a3], h4[0], h3[1]);
+                    h3[2] = fma(T[3][a3], h4[1], h3[2]);
+                }
+            }
+            if (a2 == 0) {
+                s2 = s3;
+#pragma unroll
+                for (int q = 0; q < 3; q++) h2[q + 1] = h3[q];
+            } else {
+                s2 = fma(T[2][a2], s3, s2);
+                h2[0] = fma(dT[2][a2], s3, h2[0]);
+#pragma unroll
+                for (int q = 0; q < 3; q++) h2[q + 1] = fma(T[2][a2], h3[q], h2[q + 1]);
+            }
+        }
+    };
+    // s1 / h1 += the a1 term (first: T[1][0] = 1, dT[1][0] = 0)
+    auto add1 = [&](A t1, A dt1, bool first, R s2, const R (&h2)[4], A &s1, A (&h1)[5]) {
+        if (first) {
+            s1 = (A)s2;
+#pragma unroll
+            for (int q = 0; q < 4; q++) h1[q + 1] = (A)h2[q];
+        } else {
+            s1 = fma(t1, (A)s2, s1);
+            h1[0] = fma(dt1, (A)s2, h1[0]);
+#pragma unroll
+            for (int q = 0; q < 4; q++) h1[q + 1] = fma(t1, (A)h2[q], h1[q + 1]);
+        }
+    };
+    // s0 / h0 += the a0 term (first: T[0][0] = 1, dT[0][0] = 0)
+    auto add0 = [&](A t0, A dt0, bool first, A s1, const A (&h1)[5]) {
+        if (first) {
+            s0 = s1;
+#pragma unroll
+            for (int q = 0; q < 5; q++) h0[q + 1] = h1[q];
+        } else {
+            s0 = fma(t0, s1, s0);
+            h0[0] = fma(dt0, s1, h0[0]);
+#pragma unroll
+            for (int q = 0; q < 5; q++) h0[q + 1] = fma(t0, h1[q], h0[q + 1]);
+        }
+    };
+    // loops over a0 and a1 (code size and registers), the T[0] and T[1]
+    // factors consumed from the front of rotating copies (indexing them by a
+    // loop index would place them in scratch)
     R t0v[D], d0v[D];
 #pragma unroll
     for (int u = 0; u < D; u++) { t0v[u] = T[0][u]; d0v[u] = dT[0][u]; }
@@ -206,66 +328,13 @@ __device__ __forceinline__ void contract4(const R *__restrict__ C, const R (&T)[
         for (int u = 0; u < D; u++) { t1v[u] = T[1][u]; d1v[u] = dT[1][u]; }
 #pragma unroll 1
         for (int a1 = 0; a1 < D; a1++) {
-            R s2 = 0, h2[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int a2 = 0; a2 < D; a2++) {
-                R s3 = 0, h3[3] = {0, 0, 0};
-#pragma unroll
-                for (int a3 = 0; a3 < D; a3++) {
-                    R s4 = 0, h4[2] = {0, 0};
-#pragma unroll
-                    for (int a4 = 0; a4 < D; a4++) {
-                        const_ptr<R> c = (const_ptr<R>)C + ((((a0 * D + a1) * D + a2) * D + a3) * D + a4) * D;
-                        R z = c[0], z5 = 0;
-#pragma unroll
-                        for (int a5 = 1; a5 < D; a5++) {
-                            z = fma(c[a5], T[5][a5], z);
-                            z5 = fma(c[a5], dT[5][a5], z5);
-                        }
-                        if (a4 == 0) {
-                            s4 = z;
-                            h4[1] = z5;
-                        } else {
-                            s4 = fma(T[4][a4], z, s4);
-                            h4[0] = fma(dT[4][a4], z, h4[0]);
-                            h4[1] = fma(T[4][a4], z5, h4[1]);
-                        }
-                    }
-                    if (a3 == 0) {
-                        s3 = s4;
-                        h3[1] = h4[0];
-                        h3[2] = h4[1];
-                    } else {
-                        s3 = fma(T[3][a3], s4, s3);
-                        h3[0] = fma(dT[3][a3], s4, h3[0]);
-                        h3[1] = fma(T[3][a3], h4[0], h3[1]);
-                        h3[2] = fma(T[3][a3], h4[1], h3[2]);
-                    }
-                }
-                if (a2 == 0) {
-                    s2 = s3;
-#pragma unroll
-                    for (int q = 0; q < 3; q++) h2[q + 1] = h3[q];
-                } else {
-                    s2 = fma(T[2][a2], s3, s2);
-                    h2[0] = fma(dT[2][a2], s3, h2[0]);
-#pragma unroll
-                    for (int q = 0; q < 3; q++) h2[q + 1] = fma(T[2][a2], h3[q], h2[q + 1]);
-                }
-            }
-            const A t1 = (A)t1v[0], dt1 = (A)d1v[0];
-            s1 = fma(t1, (A)s2, s1);
-            h1[0] = fma(dt1, (A)s2, h1[0]);
-#pragma unroll
-            for (int q = 0; q < 4; q++) h1[q + 1] = fma(t1, (A)h2[q], h1[q + 1]);
+            R s2, h2[4];
+            inner(a0, a1, s2, h2);
+            add1((A)t1v[0], (A)d1v[0], false, s2, h2, s1, h1);
 #pragma unroll
             for (int u = 0; u + 1 < D; u++) { t1v[u] = t1v[u + 1]; d1v[u] = d1v[u + 1]; }
         }
-        const A t0 = (A)t0v[0], dt0 = (A)d0v[0];
-        s0 = fma(t0, s1, s0);
-        h0[0] = fma(dt0, s1, h0[0]);
-#pragma unroll
-        for (int q = 0; q < 5; q++) h0[q + 1] = fma(t0, h1[q], h0[q + 1]);
+        add0((A)t0v[0], (A)d0v[0], false, s1, h1);
 #pragma unroll
         for (int u = 0; u + 1 < D; u++) { t0v[u] = t0v[u + 1]; d0v[u] = d0v[u + 1]; }
     }
