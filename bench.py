@@ -314,6 +314,12 @@ def c1_form(args, cpu=False, precision=None):
     t4_s = np.mean(t4) * 1e-3
     achieved = flops4 / t4_s
     peak = PEAK_FP64 if prec == cc.FP64 else PEAK_FP32
+    # counters of the chosen 4-body kernel from the committed summaries
+    v4 = vv[1]
+    k4 = ("k_eval4m<%d,%s,%d,0>" % (order4, "double" if prec == cc.FP64 else "float", 4 if v4 == 0 else 2)
+          if v4 in (0, 1) and order4 <= 4 else "k_evalv<4,%d,%s>" % (order4, "double" if prec == cc.FP64 else "float"))
+    prof4 = "k_eval4m_water1536_%s.md" % ("fp64_traffic" if prec == cc.FP64 else "mixed_counters")
+    figs4 = profile_figures(prof4, k4)
     out = {
         "metric": METRIC,
         "value": n * args.steps / dt,
@@ -340,11 +346,24 @@ def c1_form(args, cpu=False, precision=None):
         "error_vs_reference_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": measured_traffic("k_eval4m") if prec == cc.FP64 else None,
+                     "traffic": measured_traffic(k4) if prec == cc.FP64 else None,
                      "traffic_source": "profiles/k_eval4m_water1536_fp64_traffic.md (bytes per launch)",
-                     "kernel": "k_eval4 (4-body evaluation, %d cluster types)" % len(t4terms),
-                     "flops_per_step": flops4, "kernel_ms_per_step": t4_s * 1e3},
+                     "kernel": "%s (4-body evaluation, %d cluster types)" % (k4, len(t4terms)),
+                     "flops_per_step": flops4, "kernel_ms_per_step": t4_s * 1e3,
+                     "counters_source": "profiles/" + prof4 if figs4 else None, "counters": figs4 or None},
     }
+    if v4 in (0, 1) and order4 <= 4:
+        # executed matrix-core work of the 4-body kernel: per 16-cluster tile
+        # two products of RB row blocks x D^2 k-steps of 16 x 16 x 4 MFMAs
+        # (2048 FLOP each), RB = the 4 D^2 rows padded to 16, over 16
+        rb = -(-4 * order4 * order4 // 16)
+        ex = counts[2] * 2 * rb * order4 * order4 * 2048 / 16
+        out["roofline"]["executed_mfma_tflops"] = ex / t4_s / 1e12
+        out["roofline"]["executed_mfma_frac"] = ex / t4_s / peak
+        if figs4:
+            out["roofline"]["mfma_util_pct"] = figs4.get("MfmaUtil_pct")
+            out["roofline"]["valu_per_mfma"] = figs4["valu_per_wave"] / figs4["mfma_per_wave"] \
+                if figs4.get("mfma_per_wave") else None
     if cpu:
         out["cpu_baseline"] = cpu_baseline(pf, cfgs[0])
     return out
@@ -564,9 +583,12 @@ def bench_c2(args):
     body = 3 if kt["3body"] >= kt["4body"] else 4
     var = variants[body - 3]
     f32 = head_prec == cc.MIXED
-    if var == 2 or (body == 4 and d4 > 4):
+    if var == 2 or (body == 4 and d4 > 4) or (var == 3 and (body == 4 or not f32 or d3 > 10)):
         kname = "k_evalv<%d, %d, %s>" % (body, d3 if body == 3 else d4, "float" if f32 else "double")
         unit = "vector ALU"
+    elif var == 3:
+        kname = "k_eval3v<%d>" % d3
+        unit = "matrix cores (one cluster per lane)"
     else:
         kname = "%s<%d, %s, %d, 0>" % ("k_eval3m" if body == 3 else "k_eval4m", d3 if body == 3 else d4,
                                        "float" if f32 else "double", 4 if var == 0 else 2)

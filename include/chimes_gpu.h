@@ -22,6 +22,13 @@ typedef struct chimes_gpu_model chimes_gpu_model;
  * FP32); BF16X3 (each BF16 operand split into a high part and a residual,
  * three products per term).  The 2-body term is always FP64. */
 enum { CHIMES_GPU_FP64 = 0, CHIMES_GPU_MIXED = 1, CHIMES_GPU_BF16 = 2, CHIMES_GPU_BF16X3 = 3 };
+/* 3-/4-body kernel variants (chimes_gpu_set_variants, chimes_gpu_plan_info) */
+enum {
+    CHIMES_GPU_VAR_MFMA4 = 0,  /* matrix cores, 64-cluster tiles */
+    CHIMES_GPU_VAR_MFMA2 = 1,  /* matrix cores, 32-cluster tiles */
+    CHIMES_GPU_VAR_VECTOR = 2, /* vector units, one cluster per lane */
+    CHIMES_GPU_VAR_MFMAV = 3   /* one cluster per lane, 3-body contraction on the matrix cores (FP32-mixed) */
+};
 
 /* Parses a ChIMES parameter file and uploads its tables to GPU `device`. */
 chimes_gpu_model *chimes_gpu_model_load(const char *param_file, int device);
