@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
 {
     using PQ = pairs_of<NB>;
     constexpr int P = PQ::P;
-    __shared__ fx_table<(NB == 3 ? 512 : 1024)> tab;  // >= 256 clusters x (NB - 1) non-owner members
+    __shared__ fx_table<1024> tab;  // >= 256 clusters x NB members
     // status read overlapped with the table clear; one barrier for both
     __shared__ int flags_s;
     if (threadIdx.x == 0) flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
     const chx_group &grp = groups[g];
     wave_on = wave_on && grp.type >= 0 && grp.D == D;
     const long c = t0 + lane;
-    int owner = -1, ckey = -1;
+    int ckey = -1;
     double q7[7] = {0, 0, 0, 0, 0, 0, 0};
     double Fr[NB][3];   // force on each runtime member
     int key[NB];        // replicated atom of each runtime member, -1 none
@@ -375,15 +375,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
                         key[m] = par[a];
                         Fr[m][0] = F[a][0]; Fr[m][1] = F[a][1]; Fr[m][2] = F[a][2];
                     }
-            owner = rec.x;
             ckey = cfg;
             note_flags(cfg_stat, cfg, flags);
         }
     }
+    // every member's force through the LDS table (the owner's too: its
+    // lanes' LDS atomics cost fewer instructions than six wavefront scans);
+    // energy and virial go to the configuration
 #pragma unroll
-    for (int m = 1; m < NB; m++) tab.add(key[m], Fr[m], bad);
-    // runtime member 0 is the owner; energy and virial go to the configuration
-    fx_owner_force_runs(Facc, owner, Fr[0], bad);
+    for (int m = 0; m < NB; m++) tab.add(key[m], Fr[m], bad);
     fx_cfg_sums(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
     __syncthreads();
     tab.flush(Facc);
@@ -1141,7 +1141,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // padded with zeros to CP) and, after the contraction, the force table
     union shared_t {
         float T[4][64 * STR];
-        fx_table<512> tab;
+        fx_table<1024> tab;
     };
     __shared__ shared_t sh;
     __shared__ int flags_s;
@@ -1270,7 +1270,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     sh.tab.clear();
     __syncthreads();
     // ---- scalars, forces and accumulation (as k_evalv)
-    int owner = -1, ckey = -1;
+    int ckey = -1;
     double q7[7] = {0, 0, 0, 0, 0, 0, 0};
     double Fr[NB][3];
     int key[NB];
@@ -1291,13 +1291,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     key[m] = par[a];
                     Fr[m][0] = F[a][0]; Fr[m][1] = F[a][1]; Fr[m][2] = F[a][2];
                 }
-        owner = rec.x;
         ckey = cfg;
         note_flags(cfg_stat, cfg, flags);
     }
 #pragma unroll
-    for (int m = 1; m < NB; m++) sh.tab.add(key[m], Fr[m], bad);
-    fx_owner_force_runs(Facc, owner, Fr[0], bad);
+    for (int m = 0; m < NB; m++) sh.tab.add(key[m], Fr[m], bad);
     fx_cfg_sums(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
     __syncthreads();
     sh.tab.flush(Facc);
