@@ -147,33 +147,6 @@ __device__ __forceinline__ void fx_run_scan(long long (&x)[K], int seg)
     }
 }
 
-// Adds the owner's force f into Facc[key] (acc[(3*key + k)*2 + {0,1}]) and
-// the energy and virial terms q into Oacc[key] (acc[(7*key + k)*2 + {0,1}]),
-// one atomic pair per nonzero component and run.  key < 0 contributes nothing.
-template <int W>
-__device__ __forceinline__ void fx_owner_runs(unsigned long long *Facc, unsigned long long *Oacc, int key,
-                                              const double (&f)[3], const double (&q)[7], bool &bad)
-{
-    if (__ballot(key >= 0) == 0) return;
-    const int seg = fx_run_first<W>(key);
-    const int l = threadIdx.x & (W - 1);
-    const int knext = __shfl_down(key, 1, W);
-    const bool last = key >= 0 && (l == W - 1 || knext != key);
-    // one value at a time (bounded register use): split, scan, add
-#pragma unroll
-    for (int k = 0; k < 10; k++) {
-        long long x[2] = {0, 0};
-        if (key >= 0) fx_split(k < 3 ? f[k] : q[k - 3], x[0], x[1], bad);
-        fx_run_scan<W, 2>(x, seg);
-        if (last) {
-            unsigned long long *a = k < 3 ? Facc + 2L * (3L * key + k) : Oacc + 2L * (7L * key + (k - 3));
-            if (x[0]) atomicAdd(a, (unsigned long long)x[0]);
-            if (x[1]) atomicAdd(a + 1, (unsigned long long)x[1]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
-}
-
 // Inclusive sum of a double over the lanes of each row of 16 (row shifts
 // with zero fill, a fixed order): lane 15 of a row holds the row's total.
 __device__ __forceinline__ double row_sum16(double x)
@@ -279,91 +252,6 @@ __device__ __forceinline__ void fx_cfg_sums(unsigned long long *Cacc, int key, i
             if (x[1]) atomicAdd(a + 1, (unsigned long long)x[1]);
         }
         __builtin_amdgcn_sched_barrier(0);
-    }
-}
-
-// Inclusive prefix sum of x over the wavefront's 64 lanes, unsegmented
-// (integer sums, modulo 2^64): row sums by row shifts, then the totals of
-// the rows before.
-__device__ __forceinline__ long long fx_wave_prefix(long long x)
-{
-    x += row_shr_i64<1>(x);
-    x += row_shr_i64<2>(x);
-    x += row_shr_i64<4>(x);
-    x += row_shr_i64<8>(x);
-    const long long R0 = readlane_i64(x, 15), R1 = readlane_i64(x, 31), R2 = readlane_i64(x, 47);
-    const int r = (threadIdx.x & 63) >> 4;
-    return x + (r == 0 ? 0ll : (r == 1 ? R0 : (r == 2 ? R0 + R1 : R0 + R1 + R2)));
-}
-
-// The owner's force: each run of equal keys over the wavefront (clusters are
-// stored by owner) adds its exact sum once, from its last lane.  A run's sum
-// is the difference of the unsegmented prefix sums at its last lane and
-// before its first, which is exact in modular integer arithmetic and needs
-// no per-lane conditions.  Must be reached by all lanes of the wavefront.
-__device__ __forceinline__ void fx_owner_force_runs(unsigned long long *Facc, int key, const double (&f)[3], bool &bad)
-{
-    if (__ballot(key >= 0) == 0) return;
-    const int seg = fx_run_first<64>(key);
-    const int l = threadIdx.x & 63;
-    const int knext = __shfl_down(key, 1, 64);
-    const bool last = key >= 0 && (l == 63 || knext != key);
-    const int before = seg > 0 ? seg - 1 : 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        long long x[2] = {0, 0};
-        if (key >= 0) fx_split(f[k], x[0], x[1], bad);
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const long long P = fx_wave_prefix(x[h]);
-            const long long Pb = __shfl(P, before, 64);
-            x[h] = seg > 0 ? P - Pb : P;
-        }
-        if (last) {
-            unsigned long long *a = Facc + 2L * (3L * key + k);
-            if (x[0]) atomicAdd(a, (unsigned long long)x[0]);
-            if (x[1]) atomicAdd(a + 1, (unsigned long long)x[1]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
-}
-
-// Owner terms of tiles in which the four lanes l, l + 16, l + 32, l + 48
-// (sub = lane / 16) hold one cluster and the same key: sub 0 adds the owner's
-// force f into Facc[key], subs 1, 2, 3 add the energy and virial terms q
-// (components 0-2, 3-4 and 5-6) into Oacc[key]; each sub sums over runs of
-// equal keys in its 16 lanes, so a lane scans three values, not ten.
-__device__ __forceinline__ void fx_owner_runs4(unsigned long long *Facc, unsigned long long *Oacc, int key,
-                                               const double (&f)[3], const double (&q)[7], bool &bad)
-{
-    if (__ballot(key >= 0) == 0) return;
-    const int sub = (threadIdx.x >> 4) & 3;
-    double v[3];
-    v[0] = sub == 0 ? f[0] : (sub == 1 ? q[0] : (sub == 2 ? q[3] : q[5]));
-    v[1] = sub == 0 ? f[1] : (sub == 1 ? q[1] : (sub == 2 ? q[4] : q[6]));
-    v[2] = sub == 0 ? f[2] : (sub == 1 ? q[2] : 0.0);
-    long long hi[3], lo[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        hi[k] = 0;
-        lo[k] = 0;
-        if (key >= 0) fx_split(v[k], hi[k], lo[k], bad);
-    }
-    const int seg = fx_run_first<16>(key);
-    fx_run_scan<16, 3>(hi, seg);
-    fx_run_scan<16, 3>(lo, seg);
-    const int l = threadIdx.x & 15;
-    const int knext = __shfl_down(key, 1, 16);
-    if (key >= 0 && (l == 15 || knext != key)) {
-        unsigned long long *a = sub == 0 ? Facc + 6L * key
-                                         : Oacc + 14L * key + (sub == 1 ? 0 : (sub == 2 ? 6 : 10));
-        const int n = sub < 2 ? 3 : 2;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            if (k >= n) break;
-            if (hi[k]) atomicAdd(a + 2 * k, (unsigned long long)hi[k]);
-            if (lo[k]) atomicAdd(a + 2 * k + 1, (unsigned long long)lo[k]);
-        }
     }
 }
 

@@ -18,14 +18,14 @@ struct chx_list_args {
     const chx_atom_s *as;
     int nb_cap;  // neighbour images per owner slab
     const chx_entry_s *es;
-    unsigned long long *Facc, *Oacc;
+    unsigned long long *Facc;
     unsigned long long *Cacc;      // [ncfg][CHX_CACC_SLOTS][CHX_CACC_STRIDE] energy and virial per configuration
     int *cfg_stat, *status;
 };
 
 __global__ void k_fin(int ncfg, int nitems, const int *cfg_item_off, int natoms_in, const int *cfg_in_off,
                       const chx_cfg *cfgs, const chx_atom_s *as, const chx_model_dev *models,
-                      const unsigned long long *Facc, const unsigned long long *Oacc,
+                      const unsigned long long *Facc,
                       const unsigned long long *Cacc, const int *nbr_cnt,
                       long long *part, int *done,
                       double *Fout, double *Ecfg, double *Vcfg, int *status);

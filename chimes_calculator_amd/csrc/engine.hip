@@ -405,7 +405,8 @@ struct chx_engine {
     DevBuf<int2> list3, list4;
     DevBuf<int2> rec;           // per owner slab of rcap cluster records (enumeration -> placement)
     DevBuf<int> nrec;
-    DevBuf<unsigned long long> Facc, Oacc;
+    DevBuf<unsigned long long> Facc;
+    DevBuf<unsigned long long> Cacc;  // [ncfg][CHX_CACC_SLOTS][CHX_CACC_STRIDE] energy and virial per configuration
     DevBuf<long long> part;
     DevBuf<int> done;  // per configuration: reduction items finished
     std::vector<DevBuf<int>> scan_tmp;
@@ -447,7 +448,7 @@ struct chx_engine {
     {
         f(as); f(an); f(cell_cnt); f(cell_start); f(cell_cur); f(cell_atoms); f(nbr_cnt); f(nbr_off); f(big);
         f(es); f(en); f(cnt34); f(off34); f(gb3); f(ge3); f(gb4); f(ge4); f(list3); f(list4); f(rec); f(nrec);
-        f(Facc); f(Oacc);
+        f(Facc); f(Cacc);
         f(part); f(done); f(in_d); f(out_d); f(cl_in); f(cl_out); f(cl_types); f(cl_flags);
         for (auto &b : scan_tmp) f(b);
     }
@@ -927,10 +928,6 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int CHX_NCLEAR = 10;  // ranges of the clear list
 
-// Offset (u64 words) of the per-configuration accumulators behind the
-// per-owner ones, on a 128-byte boundary of the buffer.
-long cacc_off(long N) { return (14 * N + CHX_CACC_STRIDE - 1) / CHX_CACC_STRIDE * CHX_CACC_STRIDE; }
-
 // Sizes every buffer, lays out the input and output blobs and fills the host
 // input blob.  host_pos: positions/types are host arrays copied in the blob;
 // host_out: energy/virial/forces come back in the output blob.
@@ -954,7 +951,7 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->gb4.ensure(G4 + 1);
     e->ge4.ensure(G4 + 1);
     e->Facc.ensure(6 * N);
-    e->Oacc.ensure(cacc_off(N) + (long)CHX_CACC_SLOTS * CHX_CACC_STRIDE * ncfg);  // per owner, then Cacc
+    e->Cacc.ensure((long)CHX_CACC_SLOTS * CHX_CACC_STRIDE * ncfg);
     e->part.ensure(14L * e->nitems);
     e->done.ensure(ncfg + 1);
     // input blob
@@ -1041,7 +1038,7 @@ void size_work(chx_engine *e)
     cl[n++] = chx_clear_range{e->cnt34.p, (long)(e->G3 + e->G4) * N + 1};
     if (e->G3 + e->G4) cl[n++] = chx_clear_range{e->nrec.p, N};
     cl[n++] = chx_clear_range{(int *)e->Facc.p, 12 * N};
-    cl[n++] = chx_clear_range{(int *)e->Oacc.p, 2 * cacc_off(N) + 2L * CHX_CACC_SLOTS * CHX_CACC_STRIDE * e->ncfg};
+    cl[n++] = chx_clear_range{(int *)e->Cacc.p, 2L * CHX_CACC_SLOTS * CHX_CACC_STRIDE * e->ncfg};
     cl[n++] = chx_clear_range{e->done.p, (long)e->ncfg + 1};
     e->nclear = n;
     memcpy(e->in_h.p + ((char *)e->clear - e->in_d.p), cl, sizeof(cl));
@@ -1129,7 +1126,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     e->record(1);
     // evaluation with exact accumulation
     launch_eval2((int)N, e->nb_cap, st, e->nbr_cnt.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p,
-                 e->Oacc.p + cacc_off(N), e->cfg_stat, e->status);
+                 e->Cacc.p, e->cfg_stat, e->status);
     e->record(2);
     chx_list_args la{};
     la.cfgs = e->cfgs;
@@ -1137,8 +1134,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     la.nb_cap = e->nb_cap;
     la.es = e->es.p;
     la.Facc = e->Facc.p;
-    la.Oacc = e->Oacc.p;
-    la.Cacc = e->Oacc.p + cacc_off(N);
+    la.Cacc = e->Cacc.p;
     la.cfg_stat = e->cfg_stat;
     la.status = e->status;
     if (G3) {
@@ -1164,8 +1160,8 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     }
     e->record(4);
     hipLaunchKernelGGL(k_fin, dim3(e->nitems + nblk(Nin, 256)), dim3(256), 0, st, ncfg, e->nitems,
-                       e->cfg_item_off, (int)Nin, e->cfg_in_off, e->cfgs, e->as.p, e->mdev, e->Facc.p, e->Oacc.p,
-                       e->Oacc.p + cacc_off(N), e->nbr_cnt.p, e->part.p, e->done.p, Fout, Ecfg, Vcfg, e->status);
+                       e->cfg_item_off, (int)Nin, e->cfg_in_off, e->cfgs, e->as.p, e->mdev, e->Facc.p,
+                       e->Cacc.p, e->nbr_cnt.p, e->part.p, e->done.p, Fout, Ecfg, Vcfg, e->status);
     HIPCHK(hipMemcpyAsync(e->out_h.p, e->out_d.p, e->out_bytes, hipMemcpyDeviceToHost, st));
     e->record(5);
 }
@@ -1435,8 +1431,7 @@ void plan_tune(chx_plan *p, chx_out *o)
         la.nb_cap = e->nb_cap;
         la.es = e->es.p;
         la.Facc = e->Facc.p;  // accumulators are overwritten by the next real run
-        la.Oacc = e->Oacc.p;
-        la.Cacc = e->Oacc.p + cacc_off(e->N);
+        la.Cacc = e->Cacc.p;
         la.cfg_stat = e->cfg_stat;
         la.status = e->status;
         la.cap = body == 3 ? e->l3_cap : e->l4_cap;

@@ -545,7 +545,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
                                                     const chx_atom_s *__restrict__ as,
                                                     int nb_cap,
                                                     const chx_entry_s *__restrict__ es, unsigned long long *Facc,
-                                                    unsigned long long *Oacc, int *cfg_stat, int *status)
+                                                    unsigned long long *Cacc, int *cfg_stat, int *status)
 {
     using SH = mfma_shape<D>;
     using MA = mfma_acc<R>;
@@ -807,7 +807,6 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
         double q7[7] = {0, 0, 0, 0, 0, 0, 0};
         double F[3] = {0.0, 0.0, 0.0};
         bool bad = false;
-        int owner = -1;
         if (act) {
             double r[6], f[6], fd[6], E, fs[6], Gd[6];
 #pragma unroll
@@ -847,18 +846,16 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
                 q7[0] = E;
                 note_flags(cfg_stat, cfg, flags);
             }
-            owner = rec.x;
         }
-        // energy and virial of each cluster summed over its four lanes; the
-        // owner's force (runtime member 0, lane sub 0) and these go to the
-        // owner's accumulators per run of equal owners, the other members'
-        // forces through the LDS table
+        // energy and virial of each cluster summed over its four lanes and
+        // added to its configuration from lane sub 0; the members' forces
+        // (lane sub: runtime member sub) through the LDS table
 #pragma unroll
         for (int w = 16; w <= 32; w <<= 1)
 #pragma unroll
             for (int k = 0; k < 7; k++) q7[k] += __shfl_xor(q7[k], w, 64);
-        st.tab.add((act && sub != 0) ? key_s : -1, F, bad);
-        fx_owner_runs4(Facc, Oacc, owner, F, q7, bad);
+        st.tab.add(act ? key_s : -1, F, bad);
+        fx_cfg_sums(Cacc, (act && sub == 0) ? cfg : -1, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
         __syncthreads();
         st.tab.flush(Facc);
         if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -889,7 +886,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
                                                     const chx_atom_s *__restrict__ as,
                                                     int nb_cap,
                                                     const chx_entry_s *__restrict__ es, unsigned long long *Facc,
-                                                    unsigned long long *Oacc, int *cfg_stat, int *status)
+                                                    unsigned long long *Cacc, int *cfg_stat, int *status)
 {
     using SH = mfma3_shape<D>;
     using MA = mfma_acc<R>;
@@ -1090,14 +1087,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval3m(int G, const int *__rest
             note_flags(cfg_stat, cfg, flags);
         }
     }
-    // as in the 4-body kernel: per-cluster energy and virial over the lanes,
-    // owner terms per run, other members through the LDS table
+    // as in the 4-body kernel: per-cluster energy and virial over the lanes
+    // to the configuration, the members' forces through the LDS table
 #pragma unroll
     for (int w = 16; w <= 32; w <<= 1)
 #pragma unroll
         for (int k = 0; k < 7; k++) q7[k] += __shfl_xor(q7[k], w, 64);
-    st.tab.add((act && sub != 0 && sub < 3) ? key_s : -1, F, bad);
-    fx_owner_runs4(Facc, Oacc, act ? rec.x : -1, F, q7, bad);
+    st.tab.add((act && sub < 3) ? key_s : -1, F, bad);
+    fx_cfg_sums(Cacc, (act && sub == 0) ? cfg : -1, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
     __syncthreads();
     st.tab.flush(Facc);
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
@@ -1317,7 +1314,6 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
                                               const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
                                               const chx_model_dev *__restrict__ models,
                                               const unsigned long long *__restrict__ Facc,
-                                              const unsigned long long *__restrict__ Oacc,
                                               const unsigned long long *__restrict__ Cacc,
                                               const int *__restrict__ nbr_cnt, long long *part, int *done,
                                               double *Fout, double *Ecfg, double *Vcfg, int *status)
@@ -1365,8 +1361,6 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
     int ni = 0;  // neighbour images of these owners (reported count)
     if (threadIdx.x == 0) nimg = 0;
     for (long r = r0 + threadIdx.x; r < r1; r += 256) {
-#pragma unroll
-        for (int k = 0; k < 7; k++) acc[k].add(Oacc + 2L * (7L * r + k));
         acc[0].add_value(M.eoff[as[r].type], bad);
         ni += nbr_cnt[r];
     }
@@ -1530,19 +1524,19 @@ void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t 
     {                                                                                                          \
         if (prec == CHX_FP64)                                                                                  \
             hipLaunchKernelGGL((KER<d, double, WPB, 0>), grid, block, 0, st, la.G, la.gbase, la.gend,           \
-                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,        \
+                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Cacc,        \
                                la.cfg_stat, la.status);                                                        \
         else if (prec == CHX_MIXED)                                                                            \
             hipLaunchKernelGGL((KER<d, float, WPB, 0>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
-                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,        \
+                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Cacc,        \
                                la.cfg_stat, la.status);                                                        \
         else if (prec == CHX_BF16)                                                                             \
             hipLaunchKernelGGL((KER<d, float, WPB, 1>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
-                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,        \
+                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Cacc,        \
                                la.cfg_stat, la.status);                                                        \
         else                                                                                                   \
             hipLaunchKernelGGL((KER<d, float, WPB, 3>), grid, block, 0, st, la.G, la.gbase, la.gend,            \
-                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Oacc,        \
+                               la.groups, la.list, la.cfgs, la.as, la.nb_cap, la.es, la.Facc, la.Cacc,        \
                                la.cfg_stat, la.status);                                                        \
         return true;                                                                                           \
     }

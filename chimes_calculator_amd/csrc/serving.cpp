@@ -114,7 +114,7 @@ size_t batch_bytes(const std::vector<Request *> &b, int groups)
     const double per_atom = 64.0             // atom records (two frames)
                             + 16.0           // cell index, neighbour counts, offsets, queue
                             + 8.0 * groups   // per-(group, owner) cluster counts and their scan
-                            + 160.0          // force and owner accumulators
+                            + 48.0           // force accumulators
                             + 68.0 * nb_cap  // neighbour slabs
                             + 8.0 * rcap + 4 // cluster record slabs and counts
                             + 8.0;           // scan partials
