@@ -341,8 +341,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
     const long c = t0 + lane;
     int ckey = -1;
     double q7[7] = {0, 0, 0, 0, 0, 0, 0};
-    double Fr[NB][3];   // force on each runtime member
-    int key[NB];        // replicated atom of each runtime member, -1 none
+    double Fr[NB][3];   // force on each member (canonical order)
+    int key[NB];        // replicated atom of each member, -1 none
 #pragma unroll
     for (int m = 0; m < NB; m++) { key[m] = -1; Fr[m][0] = Fr[m][1] = Fr[m][2] = 0.0; }
     bool bad = false;
@@ -368,13 +368,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
             double F[NB][3];
             cluster_forces<NB>(cfgs[cfg].h, p, fs, F, q7);
 #pragma unroll
-            for (int a = 0; a < NB; a++)
-#pragma unroll
-                for (int m = 0; m < NB; m++)
-                    if (pm[a] == m) {
-                        key[m] = par[a];
-                        Fr[m][0] = F[a][0]; Fr[m][1] = F[a][1]; Fr[m][2] = F[a][2];
-                    }
+            for (int a = 0; a < NB; a++) {
+                key[a] = par[a];
+                Fr[a][0] = F[a][0]; Fr[a][1] = F[a][1]; Fr[a][2] = F[a][2];
+            }
             ckey = cfg;
             note_flags(cfg_stat, cfg, flags);
         }
@@ -1281,13 +1278,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         double F[NB][3];
         cluster_forces<NB>(cfgs[cfg].h, p, fs, F, q7);
 #pragma unroll
-        for (int a = 0; a < NB; a++)
-#pragma unroll
-            for (int m = 0; m < NB; m++)
-                if (pm[a] == m) {
-                    key[m] = par[a];
-                    Fr[m][0] = F[a][0]; Fr[m][1] = F[a][1]; Fr[m][2] = F[a][2];
-                }
+        for (int a = 0; a < NB; a++) {
+            key[a] = par[a];
+            Fr[a][0] = F[a][0]; Fr[a][1] = F[a][1]; Fr[a][2] = F[a][2];
+        }
         ckey = cfg;
         note_flags(cfg_stat, cfg, flags);
     }
