@@ -26,7 +26,7 @@ struct chx_lmp_args {
     unsigned long long *Facc;  // [nall * 3 * 2] fixed-point forces
     unsigned long long *Eacc;  // [nall * 2] per-atom energy, or null
     unsigned long long *Vacc;  // [nall * 6 * 2] per-atom virial (LAMMPS order), or null
-    unsigned long long *Gacc;  // [nall * 7 * 2] global energy + virial per owner atom, or null
+    unsigned long long *Gacc;  // [nall * 7 * 2] partial sums of the global energy + virial (only the total over all atoms is read), or null
     int *status;               // [CHX_LMP_ST_WORDS]
     int eflag_global, vflag_global;
 };

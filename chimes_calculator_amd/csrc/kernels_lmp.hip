@@ -296,6 +296,10 @@ template <int NB, int D, bool TAB>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_lmp_clusters(long n, const int *__restrict__ list, chx_lmp_args A,
                                                       chx_model_dev M, chx_tabs_dev T)
 {
+    // the members' forces of the workgroup's clusters, merged per atom
+    __shared__ fx_table<512> tab;  // >= 128 clusters x NB members
+    tab.clear();
+    __syncthreads();
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = c < n;
     constexpr int P = NB * (NB - 1) / 2;
@@ -327,34 +331,54 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
     bool bad = false;
     double v[6];
     lmp_order(So, v);
-    // members 1.. : exact atomics per cluster
-    if (evaluated) {
+    // forces of every member, the owner's included, through the LDS table
 #pragma unroll
-        for (int a = 1; a < NB; a++) {
-            for (int k = 0; k < 3; k++) fx_atomic(A.Facc + 2L * (3L * mem[a] + k), fo[a][k], bad);
-            if (A.Eacc) fx_atomic(A.Eacc + 2L * mem[a], Ec / NB, bad);
+    for (int a = 0; a < NB; a++) tab.add(evaluated ? mem[a] : -1, fo[a], bad);
+    // per-atom tallies (ev_tally_mb's equal shares): members 1.. by exact
+    // atomics, the owner by runs of equal owners over the wavefront
+    if (A.Eacc || A.Vacc) {
+        if (evaluated)
+#pragma unroll
+            for (int a = 1; a < NB; a++) {
+                if (A.Eacc) fx_atomic(A.Eacc + 2L * mem[a], Ec / NB, bad);
+                if (A.Vacc)
+                    for (int k = 0; k < 6; k++) fx_atomic(A.Vacc + 2L * (6L * mem[a] + k), v[k] / NB, bad);
+            }
+        const int key = evaluated ? mem[0] : -1;
+        if (__ballot(key >= 0)) {
+            const int seg = fx_run_first<64>(key);
+            const int l = threadIdx.x & 63;
+            const int knext = __shfl_down(key, 1, 64);
+            const bool last = key >= 0 && (l == 63 || knext != key);
+            const long o = key >= 0 ? key : 0;
+            if (A.Eacc) lmp_run_add(key, seg, last, Ec / NB, A.Eacc + 2L * o, bad);
             if (A.Vacc)
-                for (int k = 0; k < 6; k++) fx_atomic(A.Vacc + 2L * (6L * mem[a] + k), v[k] / NB, bad);
+                for (int k = 0; k < 6; k++) lmp_run_add(key, seg, last, v[k] / NB, A.Vacc + 2L * (6L * o + k), bad);
         }
     }
-    // member 0, the owner: runs of equal owners over the wavefront
-    const int key = evaluated ? mem[0] : -1;
-    if (__ballot(key >= 0)) {
-        const int seg = fx_run_first<64>(key);
-        const int l = threadIdx.x & 63;
-        const int knext = __shfl_down(key, 1, 64);
-        const bool last = key >= 0 && (l == 63 || knext != key);
-        const long o = key >= 0 ? key : 0;
-        for (int k = 0; k < 3; k++) lmp_run_add(key, seg, last, fo[0][k], A.Facc + 2L * (3L * o + k), bad);
-        if (A.Eacc) lmp_run_add(key, seg, last, Ec / NB, A.Eacc + 2L * o, bad);
-        if (A.Vacc)
-            for (int k = 0; k < 6; k++) lmp_run_add(key, seg, last, v[k] / NB, A.Vacc + 2L * (6L * o + k), bad);
-        if (A.Gacc) {
-            if (A.eflag_global) lmp_run_add(key, seg, last, Ec, A.Gacc + 2L * (7L * o), bad);
-            if (A.vflag_global)
-                for (int k = 0; k < 6; k++) lmp_run_add(key, seg, last, v[k], A.Gacc + 2L * (7L * o + 1 + k), bad);
+    // global energy and virial: only their sum over all atoms' words is
+    // read, so the wavefront's totals (fourteen integer parts summed at
+    // once) are added to the words of its first evaluated cluster's owner
+    if (A.Gacc) {
+        const unsigned long long on = __ballot(evaluated);
+        if (on) {
+            const int k0 = __builtin_amdgcn_readlane(mem[0], __ffsll((long long)on) - 1);
+            long long x[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) x[k] = 0;
+            if (evaluated) {
+                if (A.eflag_global) fx_split(Ec, x[0], x[1], bad);
+                if (A.vflag_global)
+#pragma unroll
+                    for (int k = 0; k < 6; k++) fx_split(v[k], x[2 + 2 * k], x[3 + 2 * k], bad);
+            }
+            const long long t = fx_wave_sums16(x);
+            const int l = threadIdx.x & 63;
+            if (l < 14 && t) atomicAdd(A.Gacc + 14L * k0 + l, (unsigned long long)t);
         }
     }
+    __syncthreads();
+    tab.flush(A.Facc);
     if (bad) atomicOr(&A.status[CHX_LMP_ST_FLAGS], CHX_LMP_BAD_RANGE);
 }
 
