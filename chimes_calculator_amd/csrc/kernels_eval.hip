@@ -252,7 +252,7 @@ __device__ __forceinline__ bool cluster_geometry(const chx_group &grp, const int
                                                  const chx_atom_s *__restrict__ as, int nb_cap,
                                                  const chx_entry_s *__restrict__ es, double (&p)[NB][3],
                                                  int (&par)[NB], double (&r)[pairs_of<NB>::P], int &cfg,
-                                                 int (&pm)[NB])
+                                                 int (&pm)[NB], double (&dr)[pairs_of<NB>::P][3])
 {
     using PQ = pairs_of<NB>;
     int t[NB];
@@ -265,18 +265,30 @@ __device__ __forceinline__ bool cluster_geometry(const chx_group &grp, const int
     bool in = true;
 #pragma unroll
     for (int q = 0; q < PQ::P; q++) {
-        double dr[3];
-        r[q] = pair_vec(C.h, p[PQ::a[q]], p[PQ::b[q]], dr);
+        r[q] = pair_vec(C.h, p[PQ::a[q]], p[PQ::b[q]], dr[q]);
         in = in && (r[q] < grp.s[q].rout);
     }
     return in;
+}
+
+// The same without the pair vectors.
+template <int NB>
+__device__ __forceinline__ bool cluster_geometry(const chx_group &grp, const int2 rec,
+                                                 const chx_cfg *__restrict__ cfgs,
+                                                 const chx_atom_s *__restrict__ as, int nb_cap,
+                                                 const chx_entry_s *__restrict__ es, double (&p)[NB][3],
+                                                 int (&par)[NB], double (&r)[pairs_of<NB>::P], int &cfg,
+                                                 int (&pm)[NB])
+{
+    double dr[pairs_of<NB>::P][3];
+    return cluster_geometry<NB>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, pm, dr);
 }
 
 // Forces of one cluster from its pair force scalars: F_a += fs dr, F_b -= fs
 // dr for pair (a, b) of canonical members; virial -= fs dr dr into the q7
 // owner accumulator slots 1..6.
 template <int NB>
-__device__ __forceinline__ void cluster_forces(const double *h, const double (&p)[NB][3],
+__device__ __forceinline__ void cluster_forces(const double (&drq)[pairs_of<NB>::P][3],
                                                const double (&fs)[pairs_of<NB>::P], double (&F)[NB][3],
                                                double (&q7)[7])
 {
@@ -285,8 +297,7 @@ __device__ __forceinline__ void cluster_forces(const double *h, const double (&p
     for (int m = 0; m < NB; m++) F[m][0] = F[m][1] = F[m][2] = 0.0;
 #pragma unroll
     for (int q = 0; q < PQ::P; q++) {
-        double dr[3];
-        pair_vec(h, p[PQ::a[q]], p[PQ::b[q]], dr);
+        const double (&dr)[3] = drq[q];
         const double f = fs[q];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -348,9 +359,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
     bool bad = false;
     if (wave_on && c < gend[g]) {
         const int2 rec = list[c];
-        double p[NB][3], r[P];
+        double p[NB][3], r[P], dr[P][3];
         int par[NB], cfg, pm[NB];
-        if (cluster_geometry<NB>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, pm)) {
+        if (cluster_geometry<NB>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, pm, dr)) {
             R T[P][D], dT[P][D];
             double f[P], fd[P];
             int flags = 0;
@@ -366,7 +377,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
             cluster_scalars<P, double>(r, f, fd, S, Gq, E, fs);
             q7[0] = E;
             double F[NB][3];
-            cluster_forces<NB>(cfgs[cfg].h, p, fs, F, q7);
+            cluster_forces<NB>(dr, fs, F, q7);
 #pragma unroll
             for (int a = 0; a < NB; a++) {
                 key[a] = par[a];
@@ -1154,7 +1165,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // ---- geometry and polynomials, one cluster per lane
     bool act = false;
     int2 rec = make_int2(-1, 0);
-    double p[NB][3], r[P], f[P], fd[P];
+    double p[NB][3], r[P], f[P], fd[P], dr[P][3];
     int par[NB], cfg = 0, pm[NB], flags = 0;
     {
         float T[P][D], dT[P][D];
@@ -1164,7 +1175,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             for (int x = 0; x < D; x++) { T[q][x] = 0.f; dT[q][x] = 0.f; }
         if (wave_on && c < gend[g]) {
             rec = list[c];
-            act = cluster_geometry<NB>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, pm);
+            act = cluster_geometry<NB>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, pm, dr);
             if (act)
 #pragma unroll
                 for (int q = 0; q < P; q++) {
@@ -1276,7 +1287,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         cluster_scalars<P, double>(r, f, fd, S, Gq, E, fs);
         q7[0] = E;
         double F[NB][3];
-        cluster_forces<NB>(cfgs[cfg].h, p, fs, F, q7);
+        cluster_forces<NB>(dr, fs, F, q7);
 #pragma unroll
         for (int a = 0; a < NB; a++) {
             key[a] = par[a];
