@@ -147,19 +147,22 @@ __device__ __forceinline__ void fx_run_scan(long long (&x)[K], int seg)
     }
 }
 
-// Inclusive sum of a double over the lanes of each row of 16 (row shifts
-// with zero fill, a fixed order): lane 15 of a row holds the row's total.
-__device__ __forceinline__ double row_sum16(double x)
+// Inclusive sum of a double over each group of W = 8 or 16 lanes (row
+// shifts, a fixed order): the group's last lane holds its total.
+template <int W>
+__device__ __forceinline__ double group_sum(double x)
 {
+    static_assert(W == 8 || W == 16, "groups of 8 or 16 lanes");
+    const int l = threadIdx.x & (W - 1);
 #define CHX_ROW_ADD(D)                                                                      \
     {                                                                                       \
         const long long b = row_shr_i64<D>(__double_as_longlong(x));                        \
-        x += __longlong_as_double(b);                                                       \
+        if (W == 16 || l >= D) x += __longlong_as_double(b);                                \
     }
     CHX_ROW_ADD(1)
     CHX_ROW_ADD(2)
     CHX_ROW_ADD(4)
-    CHX_ROW_ADD(8)
+    if constexpr (W == 16) CHX_ROW_ADD(8)
 #undef CHX_ROW_ADD
     return x;
 }

@@ -11,6 +11,7 @@
 #define CHX_NBR_CANDS 512      // candidates per owner whose cell is tabulated in LDS (binary search above)
 #define CHX_ENUM_WIDE_MAX 16384 // batches up to this many atoms enumerate with four wavefronts per owner
 #define CHX_ENUM_SMALL 128     // owners with at most this many images use the small-LDS enumeration
+#define CHX_EVAL2_LANES 8      // lanes per owner of the 2-body kernel (fixes its summation order: one value for every batch)
 #define CHX_MAX_CLUSTER_TYPES 64
 #define CHX_LIST_ALIGN 64      // 3-/4-body list segments of one cluster type start on this multiple
 #define CHX_MFMA_MAX_D 4       // 4-body dense dimension handled by the matrix-core kernel

@@ -66,25 +66,25 @@ __device__ __forceinline__ int find_group(const int *__restrict__ gbase, int G, 
 }
 
 // ---------------------------------------------------------------------------
-// 2-body: sixteen lanes per owner atom over its image slab (four owners per
-// wavefront; OWN = 4 or 16 owners per workgroup, the small shape for batches
-// too small to fill the device with the large one).  F_owner += f dr, F_image -= f dr,
-// virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).  The owner's
+// 2-body: LPO lanes per owner atom over its image slab (64 / LPO owners per
+// wavefront; OWN owners per workgroup, a small workgroup for batches too
+// small to fill the device with the large one).  F_owner += f dr, F_image -=
+// f dr, virial -= f dr dr (compute_2B, chimesFF.cpp:1657-1775).  The owner's
 // force, energy and virial are summed in double in a fixed order (lane l of
-// the sixteen takes images l, l + 16, ... of the slab, which is sorted by
-// image key, and row shifts add the sixteen lanes), so the sum depends only
-// on the owner's own images, never on the batch around it; the force is then
-// added exactly per owner, the energy and virial exactly per configuration.
-// Image forces are merged in the workgroup's LDS table.
+// the LPO takes images l, l + LPO, ... of the slab, which is sorted by image
+// key, and row shifts add the lanes), so the sum depends only on the owner's
+// own images, never on the batch around it; the force is then added exactly
+// per owner, the energy and virial exactly per configuration.  Image forces
+// are merged in the workgroup's LDS table.
 // ---------------------------------------------------------------------------
-template <int OWN>
-__global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) void k_eval2(int natoms, int nb_cap, const int *__restrict__ nbr_cnt,
+template <int OWN, int LPO>
+__global__ __launch_bounds__(LPO * OWN) __attribute__((amdgpu_waves_per_eu(3))) void k_eval2(int natoms, int nb_cap, const int *__restrict__ nbr_cnt,
                                                const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
                                                const chx_entry_s *__restrict__ es,
                                                const chx_model_dev *__restrict__ models, unsigned long long *Facc,
                                                unsigned long long *Cacc, int *cfg_stat, int *status)
 {
-    __shared__ fx_table<OWN == 4 ? 256 : 512> tab;  // image side of the workgroup's owners
+    __shared__ fx_table<LPO * OWN <= 64 ? 256 : 512> tab;  // image side of the workgroup's owners
     // the pair tables (slots, coefficients, pair map) of the model of the
     // workgroup's first owner, staged in LDS when they are small; lanes whose
     // owner uses another model read them from memory
@@ -112,16 +112,16 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
     if (ms >= 0) {
         const chx_model_dev &M0 = models[ms];
         const int nc = M0.npairs * (M0.order[0] + 1);
-        for (int t = threadIdx.x; t < nc; t += 16 * OWN) c2_s[t] = M0.c2[t];
-        for (int t = threadIdx.x; t < M0.npairs; t += 16 * OWN) sl_s[t] = M0.slots2[t];
-        for (int t = threadIdx.x; t < M0.nt * M0.nt; t += 16 * OWN) pm_s[t] = M0.pairmap[t];
+        for (int t = threadIdx.x; t < nc; t += LPO * OWN) c2_s[t] = M0.c2[t];
+        for (int t = threadIdx.x; t < M0.npairs; t += LPO * OWN) sl_s[t] = M0.slots2[t];
+        for (int t = threadIdx.x; t < M0.nt * M0.nt; t += LPO * OWN) pm_s[t] = M0.pairmap[t];
     }
     __syncthreads();
-    const int sub = threadIdx.x & 15;
-    const int i = (int)(blockIdx.x * OWN + (threadIdx.x >> 4));
+    const int sub = threadIdx.x & (LPO - 1);
+    const int i = (int)(blockIdx.x * OWN + threadIdx.x / LPO);
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // owner force, energy, virial
     bool bad = false;
-    int ckey = -1;  // configuration of the owner, on its sixteenth lane
+    int ckey = -1;  // configuration of the owner, on its last lane
     if (i < natoms) {
         const int n = nbr_cnt[i];
         const chx_atom_s ai = as[i];
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
         int flags = 0;
         // the owner's images, with the pair tables at pm / sl / c2
         auto images = [&](const int *pm, const chx_slot *sl, const double *c2) {
-            for (int j = sub; j < n; j += 16) {
+            for (int j = sub; j < n; j += LPO) {
                 const chx_entry_s g = es[(long)i * nb_cap + j];
                 double dr[3];
                 const double r = pair_vec(C.h, ai.u, g.v, dr);
@@ -156,13 +156,13 @@ __global__ __launch_bounds__(16 * OWN) __attribute__((amdgpu_waves_per_eu(3))) v
         if (C.model == ms) images(pm_s, sl_s, c2_s);
         else images(M.pairmap, M.slots2, M.c2);
         note_flags(cfg_stat, ai.cfg, flags);
-        if (sub == 15) ckey = ai.cfg;
+        if (sub == LPO - 1) ckey = ai.cfg;
     }
-    // the sixteen lanes of each owner summed in double, in a fixed order
-    // (inclusive row sums by row shifts: lane 15 of the row holds the total)
+    // the LPO lanes of each owner summed in double, in a fixed order
+    // (inclusive sums by row shifts: the owner's last lane holds the total)
 #pragma unroll
-    for (int k = 0; k < 10; k++) acc[k] = row_sum16(acc[k]);
-    if (sub == 15 && i < natoms)
+    for (int k = 0; k < 10; k++) acc[k] = group_sum<LPO>(acc[k]);
+    if (sub == LPO - 1 && i < natoms)
 #pragma unroll
         for (int k = 0; k < 3; k++) fx_atomic(Facc + 2L * (3L * i + k), acc[k], bad);
     // energy and virial per configuration, exact over the wavefront's owners
@@ -1485,12 +1485,13 @@ void launch_eval2(int natoms, int nb_cap, hipStream_t st, const int *nbr_cnt, co
     if (natoms <= 0) return;
     // results do not depend on the shape: per-owner sums run over that owner's
     // sixteen lanes and image forces are exact sums
+    constexpr int LPO = CHX_EVAL2_LANES;
     if (natoms < 16384)
-        hipLaunchKernelGGL(k_eval2<4>, dim3((unsigned)((natoms + 3) / 4)), dim3(64), 0, st, natoms, nb_cap, nbr_cnt,
-                           cfgs, as, es, models, Facc, Cacc, cfg_stat, status);
+        hipLaunchKernelGGL((k_eval2<64 / LPO, LPO>), dim3((unsigned)((natoms + 64 / LPO - 1) / (64 / LPO))), dim3(64), 0,
+                           st, natoms, nb_cap, nbr_cnt, cfgs, as, es, models, Facc, Cacc, cfg_stat, status);
     else
-        hipLaunchKernelGGL(k_eval2<16>, dim3((unsigned)((natoms + 15) / 16)), dim3(256), 0, st, natoms, nb_cap,
-                           nbr_cnt, cfgs, as, es, models, Facc, Cacc, cfg_stat, status);
+        hipLaunchKernelGGL((k_eval2<256 / LPO, LPO>), dim3((unsigned)((natoms + 256 / LPO - 1) / (256 / LPO))), dim3(256),
+                           0, st, natoms, nb_cap, nbr_cnt, cfgs, as, es, models, Facc, Cacc, cfg_stat, status);
 }
 
 void launch_evalv(int nb, int D, int prec, const chx_list_args &la, hipStream_t st)
