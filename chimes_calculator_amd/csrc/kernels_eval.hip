@@ -130,14 +130,15 @@ __global__ __launch_bounds__(LPO * OWN) __attribute__((amdgpu_waves_per_eu(3))) 
         int flags = 0;
         // the owner's images, with the pair tables at pm / sl / c2
         auto images = [&](const int *pm, const chx_slot *sl, const double *c2) {
-            for (int j = sub; j < n; j += LPO) {
-                const chx_entry_s g = es[(long)i * nb_cap + j];
-                double dr[3];
+            // one image: energy, force scalar and pair vector
+            auto pair = [&](const chx_entry_s &g, double &E, double &fs, double (&dr)[3]) {
                 const double r = pair_vec(C.h, ai.u, g.v, dr);
                 const int pt = pm[ai.type * M.nt + g.type];
-                double E, fs;
                 eval_pair(r, sl[pt], c2 + (long)pt * (M.order[0] + 1), M.order[0], M.tersoff, M.fcut_var,
                           M.pen_dist, M.pen_scale, E, fs, flags);
+            };
+            // its contributions, added in slab order
+            auto add = [&](const chx_entry_s &g, double E, double fs, const double (&dr)[3]) {
                 const double F0[3] = {fs * dr[0], fs * dr[1], fs * dr[2]};
                 const double F1[3] = {-F0[0], -F0[1], -F0[2]};
                 tab.add_or_global(g.parent, F1, Facc, bad);
@@ -151,6 +152,19 @@ __global__ __launch_bounds__(LPO * OWN) __attribute__((amdgpu_waves_per_eu(3))) 
                 acc[7] -= fs * (dr[1] * dr[1]);
                 acc[8] -= fs * (dr[1] * dr[2]);
                 acc[9] -= fs * (dr[2] * dr[2]);
+            };
+            // two images per pass, evaluated side by side (independent
+            // latency chains) and added in slab order, j then j + LPO, so
+            // the sums are those of one image per pass
+            const chx_entry_s *slab = es + (long)i * nb_cap;
+            for (int j = sub; j < n; j += 2 * LPO) {
+                const bool two = j + LPO < n;
+                const chx_entry_s g0 = slab[j], g1 = slab[two ? j + LPO : j];
+                double E0, fs0, dr0[3], E1, fs1, dr1[3];
+                pair(g0, E0, fs0, dr0);
+                pair(g1, E1, fs1, dr1);
+                add(g0, E0, fs0, dr0);
+                if (two) add(g1, E1, fs1, dr1);
             }
         };
         if (C.model == ms) images(pm_s, sl_s, c2_s);
