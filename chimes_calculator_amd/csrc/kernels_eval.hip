@@ -613,35 +613,66 @@ __global__ __launch_bounds__(64 * WPB, 2) void k_eval4m(int G, const int *__rest
         } else {
             cm.load(sizeof(R) == 8 ? (const R *)grp.Cm : (const R *)grp.Cmf);
         }
-        // ---- stage: geometry (all four lanes of a cluster), polynomials of pairs sub and sub + 4
+        // ---- stage: geometry (lane sub loads runtime member sub and computes
+        // pairs sub and sub + 4), polynomials of those pairs
         const long c = t0 + wv * 16 + j;
         bool act = c < gend[g];
         int2 rec = make_int2(-1, 0);
         int cfg = 0, flags = 0;
         int a_s = 0, key_s = -1;  // canonical index and replicated atom of runtime member sub
         {
-            double p[4][3], r[6];
-            int par[4], perm[4];
+            double xr[3] = {0.0, 0.0, 0.0};
+            int ty = 0;
             if (act) {
                 rec = list[c];
-                act = cluster_geometry<4>(grp, rec, cfgs, as, nb_cap, es, p, par, r, cfg, perm);
+                const chx_atom_s ai = as[rec.x];
+                cfg = ai.cfg;
+                if (sub == 0) {
+                    xr[0] = ai.u[0]; xr[1] = ai.u[1]; xr[2] = ai.u[2];
+                    ty = ai.type;
+                    key_s = rec.x;
+                } else {
+                    const int slot = (rec.y >> (10 * (sub - 1))) & 1023;
+                    const chx_entry_s e = es[(long)rec.x * nb_cap - 1 + slot];
+                    xr[0] = e.v[0]; xr[1] = e.v[1]; xr[2] = e.v[2];
+                    ty = e.type;
+                    key_s = e.parent;
+                }
             }
             if constexpr (!BF) cm.store([&](int x) -> R & { return Cl[LQ::at(x / SH::MP, x % SH::MP)]; });
-            double pm[3] = {p[0][0], p[0][1], p[0][2]};
+            // the cluster's type tuple from its four lanes, and the canonical
+            // slot of this lane's member (the first matching permutation)
+            int tup = 0;
 #pragma unroll
-            for (int m = 1; m < 4; m++)
-                if (sub == m) { pm[0] = p[m][0]; pm[1] = p[m][1]; pm[2] = p[m][2]; }
-            Pm[wv][j][sub][0] = pm[0];
-            Pm[wv][j][sub][1] = pm[1];
-            Pm[wv][j][sub][2] = pm[2];
+            for (int m = 0; m < 4; m++) tup = tup * grp.nt + __shfl(ty, j + 16 * m, 64);
             if (act)
 #pragma unroll
                 for (int a = 0; a < 4; a++)
-                    if (perm[a] == sub) { a_s = a; key_s = par[a]; }
-            for (int q = sub; q < 6; q += 4) {
-                double rq = r[0];
+                    if (grp.perm[tup * 4 + a] == sub) a_s = a;
+            if (act) {
+                Pm[wv][j][a_s][0] = xr[0];
+                Pm[wv][j][a_s][1] = xr[1];
+                Pm[wv][j][a_s][2] = xr[2];
+            }
+            __syncthreads();
+            // pairs sub and sub + 4 in canonical order; the cluster is in
+            // when all six of its pairs are inside their cutoffs
+            const double *h = cfgs[cfg].h;
+            double rp[2] = {0.0, 0.0};
+            bool in = true;
 #pragma unroll
-                for (int x = 1; x < 6; x++) rq = q == x ? r[x] : rq;
+            for (int u = 0; u < 2; u++) {
+                const int q = sub + 4 * u;
+                if (q < 6 && act) {
+                    double dr[3];
+                    rp[u] = pair_vec(h, Pm[wv][j][pairs_of<4>::a[q]], Pm[wv][j][pairs_of<4>::b[q]], dr);
+                    in = in && rp[u] < grp.s[q].rout;
+                }
+            }
+            const unsigned long long okm = __ballot(in) >> j;
+            act = act && (okm & 0x0001000100010001ull) == 0x0001000100010001ull;
+            for (int q = sub; q < 6; q += 4) {
+                const double rq = q == sub ? rp[0] : rp[1];
                 R Tq[D], dTq[D];
                 double fq = 0.0, fdq = 0.0;
                 if (act) {
