@@ -84,6 +84,7 @@ struct chx_cfg {
     int cell_off;              // first cell (global index)
     int model;                 // parameter set index
     double near2;              // squared bound of the cheap neighbour test (see nbr_near)
+    double lam_min;            // lower bound of the smallest eigenvalue of hn^T hn (cell pruning)
 };
 
 // Replicated atom, evaluation frame: fractional coordinates u = ih * x.

@@ -736,6 +736,35 @@ void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const ch
             const double delta = B * (nl + 2) * eps * (32.0 * B * Bi + 10.0);
             const double lim = C.search * (1.0 + 1e-12) + 2.0 * delta;
             C.near2 = lim * lim * (1.0 + 1e-12);
+            // smallest eigenvalue of the metric hn^T hn (closed form for a
+            // symmetric 3 x 3), lowered by margins: |hn du|^2 >= lam_min |du|^2
+            // lets the neighbour search skip cells beyond the search radius
+            double A[3][3];
+            for (int r = 0; r < 3; r++)
+                for (int k = 0; k < 3; k++)
+                    A[r][k] = C.hn[r] * C.hn[k] + C.hn[3 + r] * C.hn[3 + k] + C.hn[6 + r] * C.hn[6 + k];
+            const double p1 = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+            const double q = (A[0][0] + A[1][1] + A[2][2]) / 3.0;
+            double emin, emax;
+            if (p1 == 0.0) {
+                emin = std::min(A[0][0], std::min(A[1][1], A[2][2]));
+                emax = std::max(A[0][0], std::max(A[1][1], A[2][2]));
+            } else {
+                const double p2 = (A[0][0] - q) * (A[0][0] - q) + (A[1][1] - q) * (A[1][1] - q) +
+                                  (A[2][2] - q) * (A[2][2] - q) + 2.0 * p1;
+                const double p = sqrt(p2 / 6.0);
+                double Bm[3][3];
+                for (int r = 0; r < 3; r++)
+                    for (int k = 0; k < 3; k++) Bm[r][k] = (A[r][k] - (r == k ? q : 0.0)) / p;
+                const double det = Bm[0][0] * (Bm[1][1] * Bm[2][2] - Bm[1][2] * Bm[2][1]) -
+                                   Bm[0][1] * (Bm[1][0] * Bm[2][2] - Bm[1][2] * Bm[2][0]) +
+                                   Bm[0][2] * (Bm[1][0] * Bm[2][1] - Bm[1][1] * Bm[2][0]);
+                const double rr = std::max(-1.0, std::min(1.0, det / 2.0));
+                const double phi = acos(rr) / 3.0;
+                emax = q + 2.0 * p * cos(phi);
+                emin = q + 2.0 * p * cos(phi + 2.0 * M_PI / 3.0);
+            }
+            C.lam_min = std::max(0.0, emin - 1e-9 * emax) * (1.0 - 1e-6);
         }
         C.rc3 =M->d.order[1] > 0 ? rc3 : 0.0;
         C.rc4 = M->d.order[2] > 0 ? rc4 : 0.0;

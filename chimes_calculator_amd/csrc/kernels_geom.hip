@@ -235,11 +235,12 @@ __device__ __forceinline__ void unpack_shift(int spk, int (&sh)[3])
 // index (cadj), and the packed image shift (cshf); per candidate below
 // CHX_NBR_CANDS the cell it lies in (ccell).  Cells whose shift lies outside
 // the ghost layers hold no candidates.  Returns the candidate count.
-__device__ __forceinline__ int nbr_cells(const chx_cfg &C, const int (&ci)[3], int nc, int lane,
+__device__ __forceinline__ int nbr_cells(const chx_cfg &C, const int (&ci)[3], const double *ui, int nc, int lane,
                                          const int *__restrict__ cell_start, int *cpre, int *cadj, int *cshf,
                                          unsigned char *ccell)
 {
     const int w0 = 2 * C.reach[0] + 1, w1 = 2 * C.reach[1] + 1;
+    const double im[3] = {1.0 / C.m[0], 1.0 / C.m[1], 1.0 / C.m[2]};
     int carry = 0;
     for (int c0 = 0; c0 < nc; c0 += 64) {
         const int c = c0 + lane;
@@ -247,7 +248,20 @@ __device__ __forceinline__ int nbr_cells(const chx_cfg &C, const int (&ci)[3], i
         if (c < nc) {
             int r0, r1, cell, sh[3];
             const int c2 = floor_div(floor_div(c, w0, r0), w1, r1);
-            if (searched_cell(C, ci, r0 - C.reach[0], r1 - C.reach[1], c2 - C.reach[2], cell, sh)) {
+            const int d[3] = {r0 - C.reach[0], r1 - C.reach[1], c2 - C.reach[2]};
+            // the cell's fractional box (unwrapped: its image) against the
+            // owner: |h (u - u_i)|^2 >= lam_min |u - u_i|^2, so a cell whose
+            // nearest point lies beyond the cheap test's radius holds no
+            // image that passes it (1e-7 absorbs the rounding of the binning)
+            double gap2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int cu = ci[k] + d[k];
+                const double lo = cu * im[k], hi = (cu + 1) * im[k];
+                const double g = fmax(0.0, fmax(lo - ui[k], ui[k] - hi) - 1e-7);
+                gap2 += g * g;
+            }
+            if (!(gap2 * C.lam_min > C.near2) && searched_cell(C, ci, d[0], d[1], d[2], cell, sh)) {
                 cs = cell_start[cell];
                 cnt = cell_start[cell + 1] - cs;
                 spk = pack_shift(sh);
@@ -397,7 +411,7 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
         ni = an[i];
         cell_coords(C, ni.cell, ci);
         nc = (2 * C.reach[0] + 1) * (2 * C.reach[1] + 1) * (2 * C.reach[2] + 1);
-        if (nc <= CHX_NBR_CELLS) T = nbr_cells(C, ci, nc, lane, cell_start, cpre[wv], cadj[wv], cshf[wv], ccell[wv]);
+        if (nc <= CHX_NBR_CELLS) T = nbr_cells(C, ci, ni.u, nc, lane, cell_start, cpre[wv], cadj[wv], cshf[wv], ccell[wv]);
     }
     __syncthreads();
     if (own) {
@@ -500,7 +514,7 @@ __global__ __launch_bounds__(256) void k_nbr_wide(int natoms, const chx_cfg *__r
         nexact = 0;
     }
     if (nc <= CHX_NBR_CELLS && wv == 0) {
-        const int T = nbr_cells(C, ci, nc, lane, cell_start, cpre, cadj, cshf, ccell);
+        const int T = nbr_cells(C, ci, ni.u, nc, lane, cell_start, cpre, cadj, cshf, ccell);
         if (lane == 0) ncand = T;
     }
     __syncthreads();
