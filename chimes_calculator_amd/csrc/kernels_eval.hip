@@ -347,7 +347,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
 {
     using PQ = pairs_of<NB>;
     constexpr int P = PQ::P;
-    __shared__ fx_table<1024> tab;  // >= 256 clusters x NB members
+    // the workgroup's member forces merged per atom; the table holds the
+    // distinct atoms of a typical tile (a few hundred) with room to spare,
+    // and anything past it goes straight to memory, so the workgroup stays
+    // small enough for five per CU
+    __shared__ fx_table<512> tab;
     // status read overlapped with the table clear; one barrier for both
     __shared__ int flags_s;
     if (threadIdx.x == 0) flags_s = __atomic_load_n(&status[CHX_ST_FLAGS], __ATOMIC_RELAXED);
@@ -405,7 +409,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 4 && 
     // lanes' LDS atomics cost fewer instructions than six wavefront scans);
     // energy and virial go to the configuration
 #pragma unroll
-    for (int m = 0; m < NB; m++) tab.add(key[m], Fr[m], bad);
+    for (int m = 0; m < NB; m++) tab.add_or_global(key[m], Fr[m], Facc, bad);
     fx_cfg_sums(Cacc, ckey, blockIdx.x & (CHX_CACC_SLOTS - 1), q7, bad);
     __syncthreads();
     tab.flush(Facc);
