@@ -369,6 +369,9 @@ struct chx_engine {
     // matrix-core 4-body kernel (the fastest on the C1-C3 workloads)
     int var3 = CHX_VAR_VECTOR, var4 = CHX_VAR_MFMA4;
     bool capturing = false;  // inside a graph capture: no event records
+    // second stream of a captured plan, and its fork / join events
+    hipStream_t st2 = nullptr;
+    hipEvent_t fork_ev[2] = {nullptr, nullptr}, join_ev = nullptr;
     // buffer capacities (entries, list slots); grow on demand
     long l3_cap = 0, l4_cap = 0;
     int nb_cap = 0;  // neighbour images per owner slab
@@ -466,7 +469,14 @@ chx_engine *chx_engine_create(int device)
     chx_engine *e = new chx_engine;
     e->device = device;
     if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) { delete e; return nullptr; }
+    if (hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(e->st);
+        delete e;
+        return nullptr;
+    }
     for (int k = 0; k < 6; k++) (void)hipEventCreate(&e->ev[k]);
+    for (hipEvent_t *ev : {&e->fork_ev[0], &e->fork_ev[1], &e->join_ev})
+        (void)hipEventCreateWithFlags(ev, hipEventDisableTiming);
     return e;
 }
 
@@ -475,7 +485,10 @@ void chx_engine_destroy(chx_engine *e)
     if (!e) return;
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->st);
+    (void)hipStreamSynchronize(e->st2);
     for (int k = 0; k < 6; k++) (void)hipEventDestroy(e->ev[k]);
+    for (hipEvent_t ev : {e->fork_ev[0], e->fork_ev[1], e->join_ev}) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(e->st2);
     (void)hipStreamDestroy(e->st);
     delete e;
 }
@@ -1081,6 +1094,20 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     const long N = e->N, Nin = e->Nin, ncell = e->ncell;
     const int ncfg = e->ncfg, G3 = e->G3, G4 = e->G4;
     const int prec = e->prec;
+    // A captured plan evaluates the 2-body terms beside the enumeration and
+    // the 4-body terms beside the 3-body ones, on a second stream (every sum
+    // is exact, so the order of the atomic adds cannot change a bit).  Eager
+    // runs keep one stream, in the order the kernel-group timing assumes.
+    const bool fork = e->capturing;
+    hipStream_t s2 = fork ? e->st2 : st;
+    auto fork_at = [&](int k) {
+        HIPCHK(hipEventRecord(e->fork_ev[k], st));
+        HIPCHK(hipStreamWaitEvent(e->st2, e->fork_ev[k], 0));
+    };
+    auto eval2 = [&] {
+        launch_eval2((int)N, e->nb_cap, s2, e->nbr_cnt.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p,
+                     e->Cacc.p, e->cfg_stat, e->status);
+    };
     e->record(0);
     HIPCHK(hipMemcpyAsync(e->in_d.p, e->in_h.p, e->in_bytes, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, e->clear, e->nclear);
@@ -1100,6 +1127,10 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         hipLaunchKernelGGL(k_nbr, dim3(nblk(N, 4)), dim3(256), 0, st, (int)N, e->cfgs, e->as.p, e->an.p,
                            e->cell_start.p, e->cell_atoms.p, e->nbr_cnt.p, e->nbr_off.p, e->es.p, e->en.p,
                            e->nb_cap, e->status);
+    if (fork) {
+        fork_at(0);
+        eval2();
+    }
     // 3-/4-body lists in type-aligned segments
     if (G3 + G4 > 0) {
         chx_enum_args ea{};
@@ -1154,9 +1185,9 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     }
     e->record(1);
     // evaluation with exact accumulation
-    launch_eval2((int)N, e->nb_cap, st, e->nbr_cnt.p, e->cfgs, e->as.p, e->es.p, e->mdev, e->Facc.p,
-                 e->Cacc.p, e->cfg_stat, e->status);
+    if (!fork) eval2();
     e->record(2);
+    if (fork) fork_at(1);
     chx_list_args la{};
     la.cfgs = e->cfgs;
     la.as = e->as.p;
@@ -1185,7 +1216,11 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         l4.gend = e->ge4.p;
         l4.groups = e->grp4;
         l4.list = e->list4.p;
-        for (int D : e->D4s) launch_eval_body(4, D, prec, e->var4, l4, st);
+        for (int D : e->D4s) launch_eval_body(4, D, prec, e->var4, l4, s2);
+    }
+    if (fork) {
+        HIPCHK(hipEventRecord(e->join_ev, e->st2));
+        HIPCHK(hipStreamWaitEvent(st, e->join_ev, 0));
     }
     e->record(4);
     hipLaunchKernelGGL(k_fin, dim3(e->nitems + nblk(Nin, 256)), dim3(256), 0, st, ncfg, e->nitems,
