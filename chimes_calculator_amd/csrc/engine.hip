@@ -20,11 +20,15 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "chx_engine.h"
@@ -47,6 +51,68 @@ namespace {
 struct EngineError {
     std::string msg;
     explicit EngineError(std::string m) : msg(std::move(m)) {}
+};
+
+// Persistent host workers: run(f) calls f(k) once for each k in [0, size()),
+// k = 0 on the calling thread, and returns when all have returned (f must
+// not throw).
+class WorkerPool {
+public:
+    explicit WorkerPool(int n) : n_(n)
+    {
+        for (int k = 1; k < n; k++) th_.emplace_back([this, k] { loop(k); });
+    }
+    ~WorkerPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : th_) t.join();
+    }
+    int size() const { return n_; }
+    void run(const std::function<void(int)> &f)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            left_ = n_ - 1;
+            gen_++;
+        }
+        cv_.notify_all();
+        f(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return left_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void loop(int k)
+    {
+        long seen = 0;
+        for (;;) {
+            const std::function<void(int)> *f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                f = job_;
+            }
+            (*f)(k);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--left_ == 0) done_.notify_one();
+        }
+    }
+    int n_;
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *job_ = nullptr;
+    int left_ = 0;
+    long gen_ = 0;
+    bool stop_ = false;
 };
 
 template <typename T>
@@ -353,6 +419,7 @@ struct chx_engine {
     std::vector<chx_cfg> cfg_h;
     std::vector<int> cfg_atom_off_h, cfg_in_off_h;
     std::vector<int> cfg_item_off_h;   // energy/virial reduction items (owner chunks) per configuration
+    std::vector<long> cfg_cells_h;     // cells of each configuration's grid
     int nitems = 0;
     int N = 0, Nin = 0, ncfg = 0, ncell = 0;
     int G3 = 0, G4 = 0;
@@ -369,6 +436,8 @@ struct chx_engine {
     // matrix-core 4-body kernel (the fastest on the C1-C3 workloads)
     int var3 = CHX_VAR_VECTOR, var4 = CHX_VAR_MFMA4;
     bool capturing = false;  // inside a graph capture: no event records
+    // host workers of the frame set-up of large batches (created on first use)
+    std::unique_ptr<WorkerPool> pool;
     // second stream of a captured plan, and its fork / join events
     hipStream_t st2 = nullptr;
     hipEvent_t fork_ev[2] = {nullptr, nullptr}, join_ev = nullptr;
@@ -666,169 +735,208 @@ void chx_model_destroy(chx_model *m) { delete m; }
 namespace {
 
 // Host frame set-up of every configuration (serial_chimes_interface.cpp:150-534).
+// Frame of configuration c: everything but its offsets in the batch; returns
+// its cell count.  It depends only on the configuration's cell, atom count
+// and model.
+long frame_of(chx_model *const *models, int nmodels, const chx_batch *b, int c, chx_cfg &C)
+{
+    const int mi = b->model[c];
+    if (mi < 0 || mi >= nmodels) throw EngineError("model index out of range");
+    const chx_model *M = models[mi];
+    const double rc2 = M->rc[0], rc3 = M->rc[1], rc4 = M->rc[2];
+    const double *ca = b->cells + 9 * c, *cb = ca + 3, *cc = ca + 6;
+    const int nin = b->natoms[c];
+    if (nin <= 0) throw EngineError("configuration without atoms");
+    double la = mag3(ca[0], ca[1], ca[2]), lb = mag3(cb[0], cb[1], cb[2]), lc = mag3(cc[0], cc[1], cc[2]);
+    double mn = la;
+    if (lb < mn) mn = lb;
+    if (lc < mn) mn = lc;
+    C.n_rep = b->small ? (int)(ceil(rc2 / mn) - 1) : 0;
+    if (C.n_rep < 0) C.n_rep = 0;
+    set_hmat(ca, cb, cc, C.h0, C.ih0, 0);
+    set_hmat(ca, cb, cc, C.h, C.ih, C.n_rep);
+    double L[3];
+    lattice_lengths(C.h, L);
+    double A[3], B[3], Cv[3];
+    col(C.h, 0, A); col(C.h, 1, B); col(C.h, 2, Cv);
+    const double al = angle3(B, Cv), be = angle3(Cv, A), ga = angle3(A, B);
+    double vv = 1;
+    vv += 2 * cos(al) * cos(be) * cos(ga);
+    vv -= cos(al) * cos(al);
+    vv -= cos(be) * cos(be);
+    vv -= cos(ga) * cos(ga);
+    C.vol = L[0] * L[1] * L[2] * sqrt(vv);
+    // ghost layers (serial_chimes_interface.cpp:477-534) in both frames
+    double lm = L[0];
+    if (L[1] < lm) lm = L[1];
+    if (L[2] < lm) lm = L[2];
+    const int nl_s = (int)ceil(rc2 * 2.0 / lm + 1);
+    reorient(C.h, C.hn, C.ihn);
+    double Ln[3];
+    lattice_lengths(C.hn, Ln);
+    double lmn = Ln[0];
+    if (Ln[1] < lmn) lmn = Ln[1];
+    if (Ln[2] < lmn) lmn = Ln[2];
+    C.n_layers = (int)ceil(rc2 * 2.0 / lmn + 1);
+    if (C.n_layers != nl_s) throw EngineError("ghost layer counts of the two frames differ");
+    const int nl = C.n_layers;
+    for (int f = 0; f < 2; f++) {
+        const double *LL = f ? Ln : L;
+        const double ex = LL[0] * (2 * nl + 1), ey = LL[1] * (2 * nl + 1), ez = LL[2] * (2 * nl + 1);
+        if (rc2 > 0.5 * ex || rc2 > 0.5 * ey || rc2 > 0.5 * ez ||
+            (M->d.order[1] > 0 && (rc3 > 0.5 * ex || rc3 > 0.5 * ey || rc3 > 0.5 * ez)) ||
+            (M->d.order[2] > 0 && (rc4 > 0.5 * ex || rc4 > 0.5 * ey || rc4 > 0.5 * ez)))
+            throw EngineError("Maximum cutoff is greater than half at least one box length.");
+    }
+    {
+        const double mc[3] = {rc2, rc3, rc4};
+        for (int k = 0; k < 3; k++)
+            if (mc[k] > 2 * L[0] * (2 * nl + 1) || mc[k] > 2 * L[1] * (2 * nl + 1) ||
+                mc[k] > 2 * L[2] * (2 * nl + 1))
+                throw EngineError("Layered system is smaller than 2x the model maximum outer cutoff.");
+    }
+    C.search = rc2;
+    if (rc3 > C.search) C.search = rc3;
+    if (rc4 > C.search) C.search = rc4;
+    {
+        // the cheap neighbour test skips the round trip of an image
+        // through Cartesian coordinates: delta bounds what that and the
+        // rounding of both tests move a distance by (fractional
+        // coordinates within nl + 2 of the origin, B and Bi the row-sum
+        // norms of the neighbour frame and its inverse)
+        double B = 0.0, Bi = 0.0;
+        for (int r = 0; r < 3; r++) {
+            B = std::max(B, fabs(C.hn[3 * r]) + fabs(C.hn[3 * r + 1]) + fabs(C.hn[3 * r + 2]));
+            Bi = std::max(Bi, fabs(C.ihn[3 * r]) + fabs(C.ihn[3 * r + 1]) + fabs(C.ihn[3 * r + 2]));
+        }
+        const double eps = 0x1p-52;
+        const double delta = B * (nl + 2) * eps * (32.0 * B * Bi + 10.0);
+        const double lim = C.search * (1.0 + 1e-12) + 2.0 * delta;
+        C.near2 = lim * lim * (1.0 + 1e-12);
+        // smallest eigenvalue of the metric hn^T hn (closed form for a
+        // symmetric 3 x 3), lowered by margins: |hn du|^2 >= lam_min |du|^2
+        // lets the neighbour search skip cells beyond the search radius
+        double A[3][3];
+        for (int r = 0; r < 3; r++)
+            for (int k = 0; k < 3; k++)
+                A[r][k] = C.hn[r] * C.hn[k] + C.hn[3 + r] * C.hn[3 + k] + C.hn[6 + r] * C.hn[6 + k];
+        const double p1 = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const double q = (A[0][0] + A[1][1] + A[2][2]) / 3.0;
+        double emin, emax;
+        if (p1 == 0.0) {
+            emin = std::min(A[0][0], std::min(A[1][1], A[2][2]));
+            emax = std::max(A[0][0], std::max(A[1][1], A[2][2]));
+        } else {
+            const double p2 = (A[0][0] - q) * (A[0][0] - q) + (A[1][1] - q) * (A[1][1] - q) +
+                              (A[2][2] - q) * (A[2][2] - q) + 2.0 * p1;
+            const double p = sqrt(p2 / 6.0);
+            double Bm[3][3];
+            for (int r = 0; r < 3; r++)
+                for (int k = 0; k < 3; k++) Bm[r][k] = (A[r][k] - (r == k ? q : 0.0)) / p;
+            const double det = Bm[0][0] * (Bm[1][1] * Bm[2][2] - Bm[1][2] * Bm[2][1]) -
+                               Bm[0][1] * (Bm[1][0] * Bm[2][2] - Bm[1][2] * Bm[2][0]) +
+                               Bm[0][2] * (Bm[1][0] * Bm[2][1] - Bm[1][1] * Bm[2][0]);
+            const double rr = std::max(-1.0, std::min(1.0, det / 2.0));
+            const double phi = acos(rr) / 3.0;
+            emax = q + 2.0 * p * cos(phi);
+            emin = q + 2.0 * p * cos(phi + 2.0 * M_PI / 3.0);
+        }
+        C.lam_min = std::max(0.0, emin - 1e-9 * emax) * (1.0 - 1e-6);
+    }
+    C.rc3 =M->d.order[1] > 0 ? rc3 : 0.0;
+    C.rc4 = M->d.order[2] > 0 ? rc4 : 0.0;
+    if (C.rc4 > C.rc3 && C.rc3 > 0.0)
+        throw EngineError("4-body cutoff above the 3-body cutoff is not supported");
+    // cell grid over the cell's fractional coordinates
+    double a[3], bb[3], cv[3];
+    col(C.hn, 0, a); col(C.hn, 1, bb); col(C.hn, 2, cv);
+    auto cross = [](const double *x, const double *y, double *z) {
+        z[0] = x[1] * y[2] - x[2] * y[1];
+        z[1] = x[2] * y[0] - x[0] * y[2];
+        z[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    double bc[3], ca2[3], ab[3];
+    cross(bb, cv, bc); cross(cv, a, ca2); cross(a, bb, ab);
+    const double V = fabs(dot3(a, bc));
+    const double w[3] = {V / mag3(bc[0], bc[1], bc[2]), V / mag3(ca2[0], ca2[1], ca2[2]),
+                         V / mag3(ab[0], ab[1], ab[2])};
+    long cells = 1;
+    for (int k = 0; k < 3; k++) {
+        // cells at least half the search radius wide: 5 x 5 x 5 searched
+        // cells hold ~2.4x fewer candidates than 3 x 3 x 3 of full width
+        int m = (int)floor(2.0 * w[k] / (C.search * (1.0 + 1e-6)));
+        if (m < 1) m = 1;
+        if (m > 256) m = 256;
+        const double delta = C.search / w[k] * (1.0 + 1e-8) + 1e-12;
+        int reach = (int)ceil(delta * m);
+        if (reach < 1) reach = 1;
+        C.m[k] = m;
+        C.reach[k] = reach;
+        cells *= m;
+    }
+    C.natoms_in = nin;
+    C.natoms = nin * (C.n_rep + 1) * (C.n_rep + 1) * (C.n_rep + 1);
+    C.model = mi;
+    return cells;
+}
+
+// Frames of a batch, then the configurations' offsets.  Large batches share
+// the frames out over the engine's workers in fixed chunks; a failing
+// configuration reports the error of the first one in batch order either way.
 void setup_frames(chx_engine *e, chx_model *const *models, int nmodels, const chx_batch *b)
 {
     const int ncfg = b->nconf;
-    e->cfg_h.assign(ncfg, chx_cfg{});
-    e->cfg_atom_off_h.assign(ncfg + 1, 0);
-    e->cfg_in_off_h.assign(ncfg + 1, 0);
-    e->cfg_item_off_h.assign(ncfg + 1, 0);
+    e->cfg_h.resize(ncfg);
+    e->cfg_atom_off_h.resize(ncfg + 1);
+    e->cfg_in_off_h.resize(ncfg + 1);
+    e->cfg_item_off_h.resize(ncfg + 1);
+    std::vector<long> &ncells = e->cfg_cells_h;
+    ncells.resize(ncfg);
+    // up to eight workers, for batches of 512 configurations or more
+    const int hw = std::min(8, (int)std::thread::hardware_concurrency());
+    const int nw = ncfg >= 512 && hw > 1 ? hw : 1;
+    std::vector<int> bad(nw, ncfg);
+    std::vector<std::string> why(nw);
+    auto chunk = [&](int k) {
+        const int c0 = (int)((long)ncfg * k / nw), c1 = (int)((long)ncfg * (k + 1) / nw);
+        for (int c = c0; c < c1; c++) {
+            try {
+                e->cfg_h[c] = chx_cfg{};
+                ncells[c] = frame_of(models, nmodels, b, c, e->cfg_h[c]);
+            } catch (const EngineError &x) {
+                bad[k] = c;
+                why[k] = x.msg;
+                return;
+            }
+        }
+    };
+    if (nw > 1) {
+        if (!e->pool) e->pool.reset(new WorkerPool(nw));
+        e->pool->run(chunk);
+    } else {
+        chunk(0);
+    }
+    for (int k = 0; k < nw; k++)
+        if (bad[k] < ncfg) throw EngineError(why[k]);
     long N = 0, Nin = 0, ncell = 0, items = 0;
     for (int c = 0; c < ncfg; c++) {
         chx_cfg &C = e->cfg_h[c];
-        const int mi = b->model[c];
-        if (mi < 0 || mi >= nmodels) throw EngineError("model index out of range");
-        const chx_model *M = models[mi];
-        const double rc2 = M->rc[0], rc3 = M->rc[1], rc4 = M->rc[2];
-        const double *ca = b->cells + 9 * c, *cb = ca + 3, *cc = ca + 6;
-        const int nin = b->natoms[c];
-        if (nin <= 0) throw EngineError("configuration without atoms");
-        double la = mag3(ca[0], ca[1], ca[2]), lb = mag3(cb[0], cb[1], cb[2]), lc = mag3(cc[0], cc[1], cc[2]);
-        double mn = la;
-        if (lb < mn) mn = lb;
-        if (lc < mn) mn = lc;
-        C.n_rep = b->small ? (int)(ceil(rc2 / mn) - 1) : 0;
-        if (C.n_rep < 0) C.n_rep = 0;
-        set_hmat(ca, cb, cc, C.h0, C.ih0, 0);
-        set_hmat(ca, cb, cc, C.h, C.ih, C.n_rep);
-        double L[3];
-        lattice_lengths(C.h, L);
-        double A[3], B[3], Cv[3];
-        col(C.h, 0, A); col(C.h, 1, B); col(C.h, 2, Cv);
-        const double al = angle3(B, Cv), be = angle3(Cv, A), ga = angle3(A, B);
-        double vv = 1;
-        vv += 2 * cos(al) * cos(be) * cos(ga);
-        vv -= cos(al) * cos(al);
-        vv -= cos(be) * cos(be);
-        vv -= cos(ga) * cos(ga);
-        C.vol = L[0] * L[1] * L[2] * sqrt(vv);
-        // ghost layers (serial_chimes_interface.cpp:477-534) in both frames
-        double lm = L[0];
-        if (L[1] < lm) lm = L[1];
-        if (L[2] < lm) lm = L[2];
-        const int nl_s = (int)ceil(rc2 * 2.0 / lm + 1);
-        reorient(C.h, C.hn, C.ihn);
-        double Ln[3];
-        lattice_lengths(C.hn, Ln);
-        double lmn = Ln[0];
-        if (Ln[1] < lmn) lmn = Ln[1];
-        if (Ln[2] < lmn) lmn = Ln[2];
-        C.n_layers = (int)ceil(rc2 * 2.0 / lmn + 1);
-        if (C.n_layers != nl_s) throw EngineError("ghost layer counts of the two frames differ");
-        const int nl = C.n_layers;
-        for (int f = 0; f < 2; f++) {
-            const double *LL = f ? Ln : L;
-            const double ex = LL[0] * (2 * nl + 1), ey = LL[1] * (2 * nl + 1), ez = LL[2] * (2 * nl + 1);
-            if (rc2 > 0.5 * ex || rc2 > 0.5 * ey || rc2 > 0.5 * ez ||
-                (M->d.order[1] > 0 && (rc3 > 0.5 * ex || rc3 > 0.5 * ey || rc3 > 0.5 * ez)) ||
-                (M->d.order[2] > 0 && (rc4 > 0.5 * ex || rc4 > 0.5 * ey || rc4 > 0.5 * ez)))
-                throw EngineError("Maximum cutoff is greater than half at least one box length.");
-        }
-        {
-            const double mc[3] = {rc2, rc3, rc4};
-            for (int k = 0; k < 3; k++)
-                if (mc[k] > 2 * L[0] * (2 * nl + 1) || mc[k] > 2 * L[1] * (2 * nl + 1) ||
-                    mc[k] > 2 * L[2] * (2 * nl + 1))
-                    throw EngineError("Layered system is smaller than 2x the model maximum outer cutoff.");
-        }
-        C.search = rc2;
-        if (rc3 > C.search) C.search = rc3;
-        if (rc4 > C.search) C.search = rc4;
-        {
-            // the cheap neighbour test skips the round trip of an image
-            // through Cartesian coordinates: delta bounds what that and the
-            // rounding of both tests move a distance by (fractional
-            // coordinates within nl + 2 of the origin, B and Bi the row-sum
-            // norms of the neighbour frame and its inverse)
-            double B = 0.0, Bi = 0.0;
-            for (int r = 0; r < 3; r++) {
-                B = std::max(B, fabs(C.hn[3 * r]) + fabs(C.hn[3 * r + 1]) + fabs(C.hn[3 * r + 2]));
-                Bi = std::max(Bi, fabs(C.ihn[3 * r]) + fabs(C.ihn[3 * r + 1]) + fabs(C.ihn[3 * r + 2]));
-            }
-            const double eps = 0x1p-52;
-            const double delta = B * (nl + 2) * eps * (32.0 * B * Bi + 10.0);
-            const double lim = C.search * (1.0 + 1e-12) + 2.0 * delta;
-            C.near2 = lim * lim * (1.0 + 1e-12);
-            // smallest eigenvalue of the metric hn^T hn (closed form for a
-            // symmetric 3 x 3), lowered by margins: |hn du|^2 >= lam_min |du|^2
-            // lets the neighbour search skip cells beyond the search radius
-            double A[3][3];
-            for (int r = 0; r < 3; r++)
-                for (int k = 0; k < 3; k++)
-                    A[r][k] = C.hn[r] * C.hn[k] + C.hn[3 + r] * C.hn[3 + k] + C.hn[6 + r] * C.hn[6 + k];
-            const double p1 = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
-            const double q = (A[0][0] + A[1][1] + A[2][2]) / 3.0;
-            double emin, emax;
-            if (p1 == 0.0) {
-                emin = std::min(A[0][0], std::min(A[1][1], A[2][2]));
-                emax = std::max(A[0][0], std::max(A[1][1], A[2][2]));
-            } else {
-                const double p2 = (A[0][0] - q) * (A[0][0] - q) + (A[1][1] - q) * (A[1][1] - q) +
-                                  (A[2][2] - q) * (A[2][2] - q) + 2.0 * p1;
-                const double p = sqrt(p2 / 6.0);
-                double Bm[3][3];
-                for (int r = 0; r < 3; r++)
-                    for (int k = 0; k < 3; k++) Bm[r][k] = (A[r][k] - (r == k ? q : 0.0)) / p;
-                const double det = Bm[0][0] * (Bm[1][1] * Bm[2][2] - Bm[1][2] * Bm[2][1]) -
-                                   Bm[0][1] * (Bm[1][0] * Bm[2][2] - Bm[1][2] * Bm[2][0]) +
-                                   Bm[0][2] * (Bm[1][0] * Bm[2][1] - Bm[1][1] * Bm[2][0]);
-                const double rr = std::max(-1.0, std::min(1.0, det / 2.0));
-                const double phi = acos(rr) / 3.0;
-                emax = q + 2.0 * p * cos(phi);
-                emin = q + 2.0 * p * cos(phi + 2.0 * M_PI / 3.0);
-            }
-            C.lam_min = std::max(0.0, emin - 1e-9 * emax) * (1.0 - 1e-6);
-        }
-        C.rc3 =M->d.order[1] > 0 ? rc3 : 0.0;
-        C.rc4 = M->d.order[2] > 0 ? rc4 : 0.0;
-        if (C.rc4 > C.rc3 && C.rc3 > 0.0)
-            throw EngineError("4-body cutoff above the 3-body cutoff is not supported");
-        // cell grid over the cell's fractional coordinates
-        double a[3], bb[3], cv[3];
-        col(C.hn, 0, a); col(C.hn, 1, bb); col(C.hn, 2, cv);
-        auto cross = [](const double *x, const double *y, double *z) {
-            z[0] = x[1] * y[2] - x[2] * y[1];
-            z[1] = x[2] * y[0] - x[0] * y[2];
-            z[2] = x[0] * y[1] - x[1] * y[0];
-        };
-        double bc[3], ca2[3], ab[3];
-        cross(bb, cv, bc); cross(cv, a, ca2); cross(a, bb, ab);
-        const double V = fabs(dot3(a, bc));
-        const double w[3] = {V / mag3(bc[0], bc[1], bc[2]), V / mag3(ca2[0], ca2[1], ca2[2]),
-                             V / mag3(ab[0], ab[1], ab[2])};
-        long cells = 1;
-        for (int k = 0; k < 3; k++) {
-            // cells at least half the search radius wide: 5 x 5 x 5 searched
-            // cells hold ~2.4x fewer candidates than 3 x 3 x 3 of full width
-            int m = (int)floor(2.0 * w[k] / (C.search * (1.0 + 1e-6)));
-            if (m < 1) m = 1;
-            if (m > 256) m = 256;
-            const double delta = C.search / w[k] * (1.0 + 1e-8) + 1e-12;
-            int reach = (int)ceil(delta * m);
-            if (reach < 1) reach = 1;
-            C.m[k] = m;
-            C.reach[k] = reach;
-            cells *= m;
-        }
-        C.natoms_in = nin;
-        C.natoms = nin * (C.n_rep + 1) * (C.n_rep + 1) * (C.n_rep + 1);
         C.atom_off = (int)N;
         C.atom_in_off = (int)Nin;
         C.cell_off = (int)ncell;
-        C.model = mi;
         e->cfg_atom_off_h[c] = (int)N;
         e->cfg_in_off_h[c] = (int)Nin;
         e->cfg_item_off_h[c] = (int)items;
         items += std::max<long>(1, (C.natoms + CHX_FIN_CHUNK - 1) / CHX_FIN_CHUNK);
         N += C.natoms;
-        Nin += nin;
-        ncell += cells;
+        Nin += C.natoms_in;
+        ncell += ncells[c];
+        if (N >= (1L << 28)) throw EngineError("batch too large");
     }
     e->cfg_atom_off_h[ncfg] = (int)N;
     e->cfg_in_off_h[ncfg] = (int)Nin;
     e->cfg_item_off_h[ncfg] = (int)items;
     e->nitems = (int)items;
-    if (N >= (1L << 28)) throw EngineError("batch too large");
     e->N = (int)N;
     e->Nin = (int)Nin;
     e->ncell = (int)ncell;
