@@ -619,9 +619,10 @@ __global__ void k_group_bases(chx_bases_args a)
 // Candidates are the images within rc3 (S3) of the owner; quadruplets use
 // adjacency bit masks over the images within rc4 (S4).
 //
-// The LDS working set is sized for C3 / C4 candidates.  k_enum uses the small
-// size (occupancy); an owner with more than CHX_ENUM_SMALL images is queued
-// and handled by k_enum_big with the full size.
+// The LDS working set is sized for C3 / C4 candidates.  k_enum uses a small
+// size (occupancy); an owner with more than CHX_ENUM_SMALL images, or more
+// candidates than its working set holds, is queued and handled by k_enum_big
+// with the full size.
 // ---------------------------------------------------------------------------
 template <int C3, int C4>
 struct EnumShared {
@@ -647,6 +648,14 @@ __device__ __forceinline__ void enum_record(const chx_enum_args &x, int i, int &
     const int r = atomicAdd(&nrec_sh, 1);
     if (r < x.rcap)
         x.rec[(long)i * x.rcap + r] = make_int2(code, (four ? (int)0x80000000 : 0) | ((int)gg << 20) | slot);
+}
+
+// Owner i to the large-LDS enumeration's queue (nothing of it is written
+// yet); a retry when that kernel was not launched.
+__device__ __forceinline__ void enum_queue_big(const chx_enum_args &x, int i)
+{
+    x.big[atomicAdd(&x.status[CHX_ST_NBIG], 1)] = i;
+    if (!x.big_on) atomicOr(&x.status[CHX_ST_FLAGS], CHX_RETRY_BIG);
 }
 
 template <int C3, int C4, int NT>
@@ -714,7 +723,10 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
     }
     const int n3 = sh.n3, n4 = sh.n4;
     if (n3 > C3 || n4 > C4) {
-        if (lane == 0) atomicOr(&x.status[CHX_ST_FLAGS], CHX_BAD_N34);
+        if (lane == 0) {
+            if constexpr (C3 < CHX_MAX_N3 || C4 < CHX_MAX_N4) enum_queue_big(x, i);
+            else atomicOr(&x.status[CHX_ST_FLAGS], CHX_BAD_N34);
+        }
         return;
     }
     for (int u = lane; u < n4; u += NT) {
@@ -836,17 +848,18 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
 template <int NT>
 __global__ __launch_bounds__(NT) void k_enum(chx_enum_args x)
 {
-    __shared__ EnumShared<CHX_ENUM_SMALL, CHX_ENUM_SMALL> sh;
+    // one wavefront per owner: room for CHX_ENUM_SMALL / 2 images within
+    // each many-body cutoff (several times the usual count), so more owners
+    // share a CU; an owner with more is queued once they are counted
+    constexpr int CS = NT == 64 ? CHX_ENUM_SMALL / 2 : CHX_ENUM_SMALL;
+    __shared__ EnumShared<CS, CS> sh;
     if (block_stop(x.status, CHX_STOP)) return;
     const int i = blockIdx.x;
     if (x.nbr_cnt[i] > CHX_ENUM_SMALL) {
-        if (threadIdx.x == 0) {
-            x.big[atomicAdd(&x.status[CHX_ST_NBIG], 1)] = i;
-            if (!x.big_on) atomicOr(&x.status[CHX_ST_FLAGS], CHX_RETRY_BIG);
-        }
+        if (threadIdx.x == 0) enum_queue_big(x, i);
         return;
     }
-    enum_owner<CHX_ENUM_SMALL, CHX_ENUM_SMALL, NT>(x, i, sh);
+    enum_owner<CS, CS, NT>(x, i, sh);
 }
 
 // Owners queued by k_enum, any number of them, over a fixed grid.
