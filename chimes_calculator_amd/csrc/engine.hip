@@ -1,11 +1,11 @@
 // HIP engine: device-resident parameter tables, per-configuration frame
 // set-up and the evaluation pipeline behind the internal C ABI (chx_engine.h).
 //
-// Pipeline of one batch (all on one stream):
+// Pipeline of one batch (one stream; a captured plan runs the 2-body and the
+// 4-body evaluation on a second one):
 //   k_prep        replicate, wrap, evaluation and neighbour frames, cell index
 //   cells         count, scan, fill
-//   k_nbr         image neighbour lists (count, scan, fill sorted by
-//                 (parent, shift) per owner)
+//   k_nbr         image neighbour lists sorted by (parent, shift) per owner
 //   k_enum        3-/4-body lists in segments of one (model, cluster type),
 //                 aligned so evaluation tiles share coefficients
 //   k_eval2/3/4   cluster energies, forces and virials, accumulated exactly

@@ -383,7 +383,9 @@ __global__ __launch_bounds__(256) void k_nbr(int natoms, const chx_cfg *__restri
     __shared__ int cpre[4][CHX_NBR_CELLS], cadj[4][CHX_NBR_CELLS], cshf[4][CHX_NBR_CELLS];
     __shared__ unsigned char ccell[4][CHX_NBR_CANDS];
     if (block_stop(status, CHX_STOP)) return;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wavefront index as a wave-uniform value: the owner's atom and frame
+    // are then read with scalar loads into scalar registers
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + wv;
     const bool own = i < natoms;
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -880,7 +882,7 @@ __global__ __launch_bounds__(64) void k_enum_big(chx_enum_args x)
 __global__ __launch_bounds__(256) void k_place(chx_enum_args x)
 {
     if (block_stop(x.status, CHX_STOP)) return;
-    const int i = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int i = (int)(blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
     if (i >= x.natoms) return;
     const int lane = threadIdx.x & 63;
     const long N = x.natoms;

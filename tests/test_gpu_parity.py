@@ -155,6 +155,26 @@ def test_dense_owners(cc, model_dir, synth_cases):
         check_lists(m, rec, conf=k)
 
 
+def test_dense_owners_large_batch(cc, model_dir, synth_cases):
+    """The same cube and slab, 129 of each in one batch (16,512 atoms): above
+    16,384 atoms the neighbour search and the enumeration run one owner per
+    wavefront, and owners with up to ~300 images go through them (and the
+    enumeration's queue for crowded owners).  Every copy must give the
+    reference's values and lists."""
+    from chimes_calculator_amd import workloads as W
+    m = cc.GpuModel(os.path.join(model_dir, "carbon_wide.txt"))
+    tags = ("cube", "slab")
+    pair = [W.dense_carbon((1.0, 1.0, 1.0)), W.dense_carbon((4.0, 0.5, 0.5))]
+    cfgs = pair * 129
+    assert sum(len(c[0]) for c in cfgs) > 16384
+    E, F, S = cc.eval_batch(m, cfgs)
+    for k in (0, 1, 128, 255, 256, 257):
+        rec = synth_cases["DENSE|" + tags[k % 2]]
+        check_efs("dense %s [%d]" % (tags[k % 2], k), E[k], F[k], S[k], rec)
+    for k in (0, 257):
+        check_lists(m, synth_cases["DENSE|" + tags[k % 2]], conf=k)
+
+
 @pytest.mark.parametrize("reps", [(4, 4, 4), (8, 8, 4)])
 def test_tiled_extensivity(cc, model_dir, synth_cases, reps):
     """C0 tiled 4 x 4 x 4 (4096 atoms, more than one energy-reduction chunk)
