@@ -660,18 +660,31 @@ __device__ __forceinline__ void enum_queue_big(const chx_enum_args &x, int i)
     if (!x.big_on) atomicOr(&x.status[CHX_ST_FLAGS], CHX_RETRY_BIG);
 }
 
+// Data that earlier kernels wrote and a launch only reads, at a
+// workgroup-uniform index, read through the constant address space: scalar
+// loads into scalar registers.
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T *p, long k)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T *)p)[k];
+#else
+    return p[k];  // the host pass never runs device code
+#endif
+}
+
 template <int C3, int C4, int NT>
 __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumShared<C3, C4> &sh)
 {
     static_assert(NT == 64 || NT == 256, "one or four wavefronts per owner");
     constexpr int EW4 = EnumShared<C3, C4>::EW4;
     const int lane = threadIdx.x, wl = lane & 63, wv = lane >> 6;
-    const int off = x.nbr_off[i];
-    const int nb = x.nbr_cnt[i];
-    const chx_atom_s ai = x.as[i];
-    const chx_atom_n ni = x.an[i];
-    const chx_cfg &C = x.cfgs[ai.cfg];
-    const chx_enum_model &M = x.em[C.model];
+    const int off = uniform_load(x.nbr_off, i);
+    const int nb = uniform_load(x.nbr_cnt, i);
+    const chx_atom_s ai = uniform_load(x.as, i);
+    const chx_atom_n ni = uniform_load(x.an, i);
+    const chx_cfg C = uniform_load(x.cfgs, ai.cfg);
+    const chx_enum_model M = uniform_load(x.em, C.model);
     const long N = x.natoms;
     const double rc3 = C.rc3, rc4 = C.rc4;
     const bool any3 = M.has3 && rc3 > 0.0, any4 = M.has4 && rc4 > 0.0;
