@@ -57,6 +57,7 @@ struct chx_enum_args {
     int rcap;
     const int *off3, *off4;        // k_place: exclusive scans of the counts
     const int *gb3, *gb4;          // k_place: aligned group bases
+    int G3, G4;                    // k_place: triplet and quadruplet groups
     int2 *list3, *list4;           // k_place output
     int *big;                      // [natoms]: owners queued for k_enum_big
     int big_on;                    // k_enum_big runs in this pipeline (else a queued owner asks for a retry)

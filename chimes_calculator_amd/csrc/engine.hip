@@ -1257,6 +1257,8 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
         ea.off4 = e->off34.p + (long)G3 * N;
         ea.gb3 = e->gb3.p;
         ea.gb4 = e->gb4.p;
+        ea.G3 = G3;
+        ea.G4 = G4;
         ea.list3 = e->list3.p;
         ea.list4 = e->list4.p;
         ea.rec = e->rec.p;

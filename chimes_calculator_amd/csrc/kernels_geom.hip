@@ -891,22 +891,39 @@ __global__ __launch_bounds__(64) void k_enum_big(chx_enum_args x)
 
 // Records to list positions, one wavefront per owner: a record of group gg at
 // its slot goes to the group's base + the owner's offset within the group
-// (the scan of the counts) + the slot.
+// (the scan of the counts) + the slot.  With few groups the owner's position
+// in each group is computed once, into LDS.
+constexpr int PLACE_GROUPS = 32;
+
 __global__ __launch_bounds__(256) void k_place(chx_enum_args x)
 {
+    __shared__ int pos[4][2][PLACE_GROUPS];
     if (block_stop(x.status, CHX_STOP)) return;
-    const int i = (int)(blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
-    if (i >= x.natoms) return;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = (int)(blockIdx.x * 4 + wv);
     const int lane = threadIdx.x & 63;
     const long N = x.natoms;
-    const int n = x.nrec[i];
+    const bool own = i < x.natoms;
+    const bool table = x.G3 <= PLACE_GROUPS && x.G4 <= PLACE_GROUPS;
+    if (own && table) {
+        if (lane < x.G3) pos[wv][0][lane] = x.gb3[lane] + (x.off3[lane * N + i] - x.off3[lane * N]);
+        else if (lane >= 32 && lane - 32 < x.G4)
+            pos[wv][1][lane - 32] = x.gb4[lane - 32] + (x.off4[(lane - 32) * N + i] - x.off4[(lane - 32) * N]);
+    }
+    __syncthreads();
+    if (!own) return;
+    const int n = uniform_load(x.nrec, i);
     const int2 *rec = x.rec + (long)i * x.rcap;
     for (int r = lane; r < n; r += 64) {
         const int2 e = rec[r];
-        const long gg = (e.y >> 20) & 0x7ff;
+        const int gg = (e.y >> 20) & 0x7ff;
         const int slot = e.y & 0xfffff;
-        if (e.y < 0) x.list4[x.gb4[gg] + (x.off4[gg * N + i] - x.off4[gg * N]) + slot] = make_int2(i, e.x);
-        else x.list3[x.gb3[gg] + (x.off3[gg * N + i] - x.off3[gg * N]) + slot] = make_int2(i, e.x);
+        const bool four = e.y < 0;
+        int p;
+        if (table) p = pos[wv][four][gg];
+        else if (four) p = x.gb4[gg] + (x.off4[gg * N + i] - x.off4[gg * N]);
+        else p = x.gb3[gg] + (x.off3[gg * N + i] - x.off3[gg * N]);
+        (four ? x.list4 : x.list3)[p + slot] = make_int2(i, e.x);
     }
 }
 
