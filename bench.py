@@ -1005,6 +1005,7 @@ def bench_c4(args):
            "config": {"workload": W.WORKLOADS["C4"], "requests": len(reqs), "rate_per_s": 200.0,
                       "atoms": sum(len(r[2][0]) for r in reqs)}}
     forms = {}
+    sizes = np.array([len(r[2][0]) for r in reqs])
     for prec, tag in ((cc.FP64, "fp64"), (cc.MIXED, "mixed")):
         for scale in (1.0, 10.0, 1e9):
             srv = cc.GpuServer(models, precision=prec, nslots=2, max_batch_atoms=65536, hbm_budget_gb=32.0)
@@ -1015,7 +1016,11 @@ def bench_c4(args):
             forms[key] = {
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                 "mean_ms": float(lat.mean()), "atom_evals_per_s": thr, "batches": st["batches"] - 8,
-                "max_batch": st["max_batch"]}
+                "max_batch": st["max_batch"],
+                # what the tail is made of: the slowest requests and their sizes
+                "slowest": [{"ms": round(float(lat[k]), 3), "atoms": int(sizes[k]), "request": int(k)}
+                            for k in np.argsort(lat)[::-1][:4]],
+                "p50_ms_2048_atoms_up": float(np.percentile(lat[sizes >= 2048], 50)) if (sizes >= 2048).any() else None}
     out["forms"] = forms
     best = forms["mixed_burst"]
     out["value"] = best["atom_evals_per_s"]

@@ -131,8 +131,13 @@ struct DevBuf {
         if (n <= cap && p) return;
         if (p) (void)hipFree(p);
         p = nullptr;
+        // 25% slack, and at least double a buffer that grows: a stream of
+        // growing batches (a server's first minutes) reallocates a few times,
+        // not once per new largest batch (each free and allocation waits for
+        // the device)
         size_t c = std::max<size_t>(n, 16);
         c += c / 4;
+        c = std::max(c, 2 * cap);
         HIPCHK(hipMalloc((void **)&p, c * sizeof(T)));
         cap = c;
     }
@@ -402,7 +407,7 @@ struct PinnedBuf {
         if (n <= cap && p) return;
         if (p) (void)hipHostFree(p);
         p = nullptr;
-        const size_t c = std::max<size_t>(n + n / 4, 4096);
+        const size_t c = std::max(std::max<size_t>(n + n / 4, 4096), 2 * cap);  // as DevBuf::ensure
         HIPCHK(hipHostMalloc((void **)&p, c, hipHostMallocDefault));
         cap = c;
     }
@@ -444,6 +449,7 @@ struct chx_engine {
     // buffer capacities (entries, list slots); grow on demand
     long l3_cap = 0, l4_cap = 0;
     int nb_cap = 0;  // neighbour images per owner slab
+    int nb_cap_run = 0;  // the slab stride of the last run (capacities may change after it)
     int rcap = 0;    // cluster records per owner slab
     long cap_key[4] = {-1, -1, -1, -1};  // batch shape the capacities were measured on
     bool big_on = false;  // launch the crowded-owner enumeration (set once an owner needed it)
@@ -951,8 +957,10 @@ void guess_capacities(chx_engine *e, chx_model *const *models)
     for (const chx_cfg &C : e->cfg_h) {
         const chx_model *M = models[C.model];
         const double rho = C.natoms / C.vol;
+        // the mean half-list count; the first owners of a configuration see
+        // nearly their whole neighbourhood (twice that), plus fluctuation
         const double nb = 0.5 * rho * 4.0 / 3.0 * pi * C.search * C.search * C.search;
-        nbmax = std::max(nbmax, 1.4 * nb + 12);
+        nbmax = std::max(nbmax, 2.0 * nb + 3.0 * sqrt(2.0 * nb) + 12);
         double r = 0;  // clusters of an owner, on average
         if (M->d.order[1] > 0) {
             const double n3 = 0.5 * rho * 4.0 / 3.0 * pi * C.rc3 * C.rc3 * C.rc3;
@@ -1202,6 +1210,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     const long N = e->N, Nin = e->Nin, ncell = e->ncell;
     const int ncfg = e->ncfg, G3 = e->G3, G4 = e->G4;
     const int prec = e->prec;
+    e->nb_cap_run = e->nb_cap;
     // A captured plan evaluates the 2-body terms beside the enumeration and
     // the 4-body terms beside the 3-body ones, on a second stream (every sum
     // is exact, so the order of the atomic adds cannot change a bit).  Eager
@@ -1797,7 +1806,7 @@ int chx_export_lists(chx_engine *e, int cfg, long *n2, int **pairs, long *n3, in
         // the configuration's owners' slabs, and their counts
         std::vector<int> cnt(C.natoms);
         HIPCHK(hipMemcpy(cnt.data(), e->nbr_cnt.p + C.atom_off, C.natoms * sizeof(int), hipMemcpyDeviceToHost));
-        const long cap = e->nb_cap, e0 = (long)C.atom_off * cap, nslab = (long)C.natoms * cap;
+        const long cap = e->nb_cap_run, e0 = (long)C.atom_off * cap, nslab = (long)C.natoms * cap;
         std::vector<long> off(C.natoms + 1, 0);
         for (int i = 0; i < C.natoms; i++) off[i + 1] = off[i] + cnt[i];
         const long ne = off[C.natoms];

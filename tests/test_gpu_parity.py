@@ -175,6 +175,19 @@ def test_dense_owners_large_batch(cc, model_dir, synth_cases):
         check_lists(m, synth_cases["DENSE|" + tags[k % 2]], conf=k)
 
 
+def test_lists_after_capacity_change(cc, model_dir, synth_cases):
+    """A denser 64-atom box first, then the reference's dense cube (the same
+    batch shape, so the slab capacity carries over and is tightened after the
+    second run): the cube's exported lists must still be the reference's."""
+    from chimes_calculator_amd import workloads as W
+    m = cc.GpuModel(os.path.join(model_dir, "carbon_wide.txt"))
+    cc.eval_batch(m, [W.dense_carbon((1.0, 1.0, 1.0), density=5.0, dmin=1.0)])
+    rec = synth_cases["DENSE|cube"]
+    E, F, S = cc.eval_batch(m, [W.dense_carbon((1.0, 1.0, 1.0))])
+    check_efs("dense cube after a denser box", E[0], F[0], S[0], rec)
+    check_lists(m, rec, conf=0)
+
+
 @pytest.mark.parametrize("reps", [(4, 4, 4), (8, 8, 4)])
 def test_tiled_extensivity(cc, model_dir, synth_cases, reps):
     """C0 tiled 4 x 4 x 4 (4096 atoms, more than one energy-reduction chunk)
