@@ -634,6 +634,7 @@ struct EnumShared {
     int par3[C3];
     int typ3[C3];
     int idx4[C4];      // S4 member -> S3 index
+    int s4of[C3];      // S3 member -> S4 index, -1 outside S4
     int gst4[C4];
     unsigned A4[C4][EW4];
     int cnt[CHX_MAX_CLUSTER_TYPES + 1];
@@ -744,16 +745,21 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         }
         return;
     }
+    for (int u = lane; u < n3; u += NT) sh.s4of[u] = -1;
+    __syncthreads();
     for (int u = lane; u < n4; u += NT) {
         int g = u;
         const int p = sh.par3[sh.idx4[u]];
         while (g > 0 && sh.par3[sh.idx4[g - 1]] == p) g--;
         sh.gst4[u] = g;
+        sh.s4of[sh.idx4[u]] = u;
         for (int w = 0; w < EW4; w++) sh.A4[u][w] = 0u;
     }
     for (int g = lane; g <= CHX_MAX_CLUSTER_TYPES; g += NT) sh.cnt[g] = 0;
     __syncthreads();
-    if (any4) {
+    // 4-body adjacency: in the 3-body pair pass below when there is one
+    // (S4 lies within S3, and the distance is the same), else here
+    if (any4 && !any3) {
         const int npair = n4 * (n4 - 1) / 2;
         for (int p = lane; p < npair; p += NT) {
             int u, v;
@@ -785,7 +791,15 @@ __device__ __forceinline__ void enum_owner(const chx_enum_args &x, int i, EnumSh
         for (int p = lane; p < npair; p += NT) {
             int u, v;
             pair_of(p, u, v);
-            if (!within(frac_dist2(C.hn, sh.vn3[u], sh.vn3[v]), cut3)) continue;
+            const double d2 = frac_dist2(C.hn, sh.vn3[u], sh.vn3[v]);
+            if (any4) {
+                const int u4 = sh.s4of[u], v4 = sh.s4of[v];
+                if (u4 >= 0 && v4 >= 0 && within(d2, cut4)) {
+                    atomicOr(&sh.A4[u4][v4 >> 5], 1u << (v4 & 31));
+                    atomicOr(&sh.A4[v4][u4 >> 5], 1u << (u4 & 31));
+                }
+            }
+            if (!within(d2, cut3)) continue;
             const int nrec = sh.par3[u] == sh.par3[v] ? 2 : 1;
             for (int q = 0; q < nrec; q++) {
                 const int a = q ? v : u, b = q ? u : v;
