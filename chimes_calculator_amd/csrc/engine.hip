@@ -499,7 +499,8 @@ struct chx_engine {
     {
         if (scan_tmp.size() < 32) scan_tmp.resize(32);  // fixed: references below stay valid
         if (level >= 15) throw EngineError("scan too deep");
-        if (level == 0 && n <= CHX_SCAN1_MAX) {
+        // one workgroup for short arrays, the block sums of long ones included
+        if (n <= CHX_SCAN1_MAX) {
             hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, st, in, out, n, gb);
             return;
         }
