@@ -1142,7 +1142,16 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->clear = (chx_clear_range *)(db + o_cl);
     e->pos_in = host_pos ? (double *)(db + o_pos) : const_cast<double *>(b->pos);
     e->types_in = host_pos ? (int *)(db + o_typ) : const_cast<int *>(b->types);
-    memcpy(hb + o_cfg, e->cfg_h.data(), ncfg * sizeof(chx_cfg));
+    // the frames of a large batch are copied by the frame workers in chunks
+    if (e->pool && ncfg >= 512) {
+        const int nw = e->pool->size();
+        e->pool->run([&](int k) {
+            const long c0 = (long)ncfg * k / nw, c1 = (long)ncfg * (k + 1) / nw;
+            memcpy(hb + o_cfg + c0 * sizeof(chx_cfg), e->cfg_h.data() + c0, (c1 - c0) * sizeof(chx_cfg));
+        });
+    } else {
+        memcpy(hb + o_cfg, e->cfg_h.data(), ncfg * sizeof(chx_cfg));
+    }
     memcpy(hb + o_cao, e->cfg_atom_off_h.data(), (ncfg + 1) * sizeof(int));
     memcpy(hb + o_cio, e->cfg_in_off_h.data(), (ncfg + 1) * sizeof(int));
     memcpy(hb + o_cfi, e->cfg_item_off_h.data(), (ncfg + 1) * sizeof(int));
