@@ -1374,8 +1374,7 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
                                               const int *__restrict__ nbr_cnt, long long *part, int *done,
                                               double *Fout, double *Ecfg, double *Vcfg, int *status)
 {
-    __shared__ long long sh_hi[7][256];
-    __shared__ unsigned long long sh_lo[7][256];
+    __shared__ long long wsum[4][16];  // per wavefront: (hi, lo) of the seven sums
     __shared__ int last, nimg;
     if ((int)blockIdx.x >= nitems) {
         const int a = ((int)blockIdx.x - nitems) * 256 + threadIdx.x;
@@ -1429,28 +1428,34 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
     }
     __syncthreads();
     if (ni) atomicAdd(&nimg, ni);
+    // the seven exact sums over the workgroup: each wavefront's by the
+    // transposed exchanges (lane l holds slot l of its wavefront), then the
+    // four wavefronts' through LDS
+    {
+        long long x[16];
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
-        sh_hi[k][threadIdx.x] = acc[k].hi + (acc[k].lo >> 40);
-        sh_lo[k][threadIdx.x] = (unsigned long long)(acc[k].lo & 0xffffffffffll);
+        for (int k = 0; k < 7; k++) {
+            x[2 * k] = acc[k].hi + (acc[k].lo >> 40);
+            x[2 * k + 1] = acc[k].lo & 0xffffffffffll;
+        }
+        x[14] = x[15] = 0;
+        const long long t = fx_wave_sums16(x);
+        if ((threadIdx.x & 63) < 16) wsum[threadIdx.x >> 6][threadIdx.x & 15] = t;
     }
     __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
-        if ((int)threadIdx.x < w)
-#pragma unroll
-            for (int k = 0; k < 7; k++) {
-                sh_hi[k][threadIdx.x] += sh_hi[k][threadIdx.x + w];
-                sh_lo[k][threadIdx.x] += sh_lo[k][threadIdx.x + w];
-            }
-        __syncthreads();
-    }
     if (bad) atomicOr(&status[CHX_ST_FLAGS], CHX_BAD_RANGE);
     if (threadIdx.x == 0 && nimg) atomicAdd(&status[CHX_ST_NENT], nimg);
     const int nit = cfg_item_off[c + 1] - cfg_item_off[c];
     if (threadIdx.x < 7) {
         const int k = threadIdx.x;
-        const long long h = sh_hi[k][0] + (long long)(sh_lo[k][0] >> 40);
-        const long long l = (long long)(sh_lo[k][0] & 0xffffffffffull);
+        long long hs = 0, ls = 0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            hs += wsum[w][2 * k];
+            ls += wsum[w][2 * k + 1];
+        }
+        const long long h = hs + (ls >> 40);
+        const long long l = ls & 0xffffffffffll;
         if (nit == 1) {
             fx_sum t;
             t.hi = h;
