@@ -23,7 +23,8 @@ struct chx_list_args {
     int *cfg_stat, *status;
 };
 
-__global__ void k_fin(int ncfg, int nitems, const int *cfg_item_off, int natoms_in, const int *cfg_in_off,
+__global__ void k_fin(int ncfg, int nitems, const int *cfg_item_off, const int *item_cfg, int natoms_in,
+                      const int *in_cfg,
                       const chx_cfg *cfgs, const chx_atom_s *as, const chx_model_dev *models,
                       const unsigned long long *Facc,
                       const unsigned long long *Cacc, const int *nbr_cnt,

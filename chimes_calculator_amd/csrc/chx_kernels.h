@@ -22,7 +22,8 @@ struct chx_red_model {
 __global__ void k_fill_i32(int *p, long n, int v);
 __global__ void k_fill_f64(double *p, long n);
 __global__ void k_prep(int natoms, int ncfg, const int *cfg_atom_off, const chx_cfg *cfgs, const double *pos,
-                       const int *types, chx_atom_s *as, chx_atom_n *an, int *cell_count);
+                       const int *types, chx_atom_s *as, chx_atom_n *an, int *cell_count,
+                       const int *cfg_item_off, int *in_cfg, int *item_cfg);
 __global__ void k_cell_fill(int natoms, const chx_atom_n *an, const int *cell_start, int *cursor, int *cell_atoms);
 __global__ void k_nbr(int natoms, const chx_cfg *cfgs, const chx_atom_s *as, const chx_atom_n *an,
                       const int *cell_start, const int *cell_atoms, int *nbr_cnt, int *nbr_off, chx_entry_s *es,

@@ -476,6 +476,7 @@ struct chx_engine {
     DevBuf<chx_atom_n> an;
     DevBuf<int> cell_cnt, cell_start, cell_cur, cell_atoms;
     DevBuf<int> nbr_cnt, nbr_off, big;
+    DevBuf<int> in_cfg, item_cfg;  // configuration of each input atom and of each totals item
     DevBuf<chx_entry_s> es;
     DevBuf<chx_entry_n> en;
     DevBuf<int> cnt34, off34;   // per (group, owner) counts and their scan: 3-body groups, then 4-body
@@ -526,6 +527,7 @@ struct chx_engine {
     void each_buf(F f)
     {
         f(as); f(an); f(cell_cnt); f(cell_start); f(cell_cur); f(cell_atoms); f(nbr_cnt); f(nbr_off); f(big);
+        f(in_cfg); f(item_cfg);
         f(es); f(en); f(cnt34); f(off34); f(gb3); f(ge3); f(gb4); f(ge4); f(list3); f(list4); f(rec); f(nrec);
         f(Facc); f(Cacc);
         f(part); f(done); f(in_d); f(out_d); f(cl_in); f(cl_out); f(cl_types); f(cl_flags);
@@ -1102,6 +1104,8 @@ void layout(chx_engine *e, const chx_batch *b, bool host_out)
     e->cell_atoms.ensure(N);
     e->nbr_cnt.ensure(N + 1);
     e->big.ensure(N + 1);
+    e->in_cfg.ensure(Nin + 1);
+    e->item_cfg.ensure(e->nitems + 1);
     e->nbr_off.ensure(N + 1);
     e->cnt34.ensure((long)(G3 + G4) * N + 1);
     e->off34.ensure((long)(G3 + G4) * N + 2);
@@ -1239,7 +1243,8 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     HIPCHK(hipMemcpyAsync(e->in_d.p, e->in_h.p, e->in_bytes, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, e->clear, e->nclear);
     hipLaunchKernelGGL(k_prep, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, ncfg, e->cfg_atom_off, e->cfgs,
-                       e->pos_in, e->types_in, e->as.p, e->an.p, e->cell_cnt.p);
+                       e->pos_in, e->types_in, e->as.p, e->an.p, e->cell_cnt.p, e->cfg_item_off, e->in_cfg.p,
+                       e->item_cfg.p);
     e->scan(e->cell_cnt.p, e->cell_start.p, ncell);
     hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256)), dim3(256), 0, st, (int)N, e->an.p, e->cell_start.p,
                        e->cell_cur.p, e->cell_atoms.p);
@@ -1353,7 +1358,7 @@ void run_pipeline(chx_engine *e, double *Fout, double *Ecfg, double *Vcfg)
     }
     e->record(4);
     hipLaunchKernelGGL(k_fin, dim3(e->nitems + nblk(Nin, 256)), dim3(256), 0, st, ncfg, e->nitems,
-                       e->cfg_item_off, (int)Nin, e->cfg_in_off, e->cfgs, e->as.p, e->mdev, e->Facc.p,
+                       e->cfg_item_off, e->item_cfg.p, (int)Nin, e->in_cfg.p, e->cfgs, e->as.p, e->mdev, e->Facc.p,
                        e->Cacc.p, e->nbr_cnt.p, e->part.p, e->done.p, Fout, Ecfg, Vcfg, e->status);
     HIPCHK(hipMemcpyAsync(e->out_h.p, e->out_d.p, e->out_bytes, hipMemcpyDeviceToHost, st));
     e->record(5);

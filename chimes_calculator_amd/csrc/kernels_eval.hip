@@ -1366,7 +1366,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // accumulators.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__restrict__ cfg_item_off,
-                                              int natoms_in, const int *__restrict__ cfg_in_off,
+                                              const int *__restrict__ item_cfg, int natoms_in,
+                                              const int *__restrict__ in_cfg,
                                               const chx_cfg *__restrict__ cfgs, const chx_atom_s *__restrict__ as,
                                               const chx_model_dev *__restrict__ models,
                                               const unsigned long long *__restrict__ Facc,
@@ -1379,12 +1380,7 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
     if ((int)blockIdx.x >= nitems) {
         const int a = ((int)blockIdx.x - nitems) * 256 + threadIdx.x;
         if (a >= natoms_in) return;
-        int lo = 0, hi = ncfg - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (cfg_in_off[mid] <= a) lo = mid; else hi = mid - 1;
-        }
-        const chx_cfg &C = cfgs[lo];
+        const chx_cfg &C = cfgs[in_cfg[a]];
         const int la = a - C.atom_in_off;
         const int ncopy = (C.n_rep + 1) * (C.n_rep + 1) * (C.n_rep + 1);
         fx_sum f[3];
@@ -1398,14 +1394,7 @@ __global__ __launch_bounds__(256) void k_fin(int ncfg, int nitems, const int *__
         return;
     }
     const int item = blockIdx.x;
-    int c = 0;
-    {
-        int hi = ncfg - 1;
-        while (c < hi) {
-            const int mid = (c + hi + 1) >> 1;
-            if (cfg_item_off[mid] <= item) c = mid; else hi = mid - 1;
-        }
-    }
+    const int c = item_cfg[item];
     const chx_cfg &C = cfgs[c];
     const chx_model_dev &M = models[C.model];
     const long r0 = C.atom_off + (long)(item - cfg_item_off[c]) * CHX_FIN_CHUNK;

@@ -89,7 +89,8 @@ __global__ void k_fill_f64(double *p, long n)
 // ---------------------------------------------------------------------------
 __global__ void k_prep(int natoms, int ncfg, const int *__restrict__ cfg_atom_off, const chx_cfg *__restrict__ cfgs,
                        const double *__restrict__ pos, const int *__restrict__ types, chx_atom_s *as,
-                       chx_atom_n *an, int *cell_count)
+                       chx_atom_n *an, int *cell_count, const int *__restrict__ cfg_item_off, int *in_cfg,
+                       int *item_cfg)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= natoms) return;
@@ -99,6 +100,10 @@ __global__ void k_prep(int natoms, int ncfg, const int *__restrict__ cfg_atom_of
     const int rho = l / C.natoms_in;
     const int a = l - rho * C.natoms_in;
     const int ain = C.atom_in_off + a;
+    // the configuration of each input atom and of each totals item (one
+    // writer each: the first replica, the first owner of the item's chunk)
+    if (rho == 0) in_cfg[ain] = c;
+    if (l % CHX_FIN_CHUNK == 0) item_cfg[cfg_item_off[c] + l / CHX_FIN_CHUNK] = c;
     double x[3] = {pos[3 * (long)ain], pos[3 * (long)ain + 1], pos[3 * (long)ain + 2]};
     if (rho > 0) {  // replica (i, j, k) of serial_chimes_interface.cpp:243-283
         const int w = C.n_rep + 1;
